@@ -1,0 +1,445 @@
+// C ABI of the engine (declared in include/tfpwa_amd.h): device buffers, launches and the
+// kernel dispatcher. No C++ exception crosses the boundary; errors are status codes plus a
+// per-thread message.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "tfpwa_amd.h"
+#include "tpa_kernels.h"
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) {
+  g_err = msg;
+  return code;
+}
+
+#define HIPCHK(x)                                                                 \
+  do {                                                                            \
+    hipError_t e_ = (x);                                                          \
+    if (e_ != hipSuccess)                                                         \
+      return fail(TPA_ERR_HIP, std::string(#x) + ": " + hipGetErrorString(e_));   \
+  } while (0)
+
+std::mutex g_timing_mu;
+bool g_timing = false;
+double g_timing_ms = 0.0;
+int64_t g_timing_n = 0;
+
+constexpr int kEvalGridMax = 1024;
+
+// small_d_weight (tf_pwa/dfun.py:118-143) for 2J = 0..TPA_MAX_J2, tables concatenated.
+std::vector<double> wigner_weights() {
+  std::vector<double> out;
+  auto f = [](int x) {
+    double r = 1.0;
+    for (int i = 2; i <= (x >> 1); ++i) r *= i;
+    return r;
+  };
+  for (int j = 0; j <= TPA_MAX_J2; ++j) {
+    std::vector<double> w((j + 1) * (j + 1) * (j + 1), 0.0);
+    for (int m = -j; m <= j; m += 2)
+      for (int n = -j; n <= j; n += 2)
+        for (int k = std::max(0, n - m); k <= std::min(j - m, j + n); k += 2) {
+          int l = (2 * k + (m - n)) / 2;
+          double sign = (((k + m - n) / 2) % 2 == 0) ? 1.0 : -1.0;
+          double tmp = sign * std::sqrt(1.0 * f(j + m) * f(j - m) * f(j + n) * f(j - n));
+          tmp /= f(j - m - k) * f(j + n - k) * f(k + m - n) * f(k);
+          w[(l * (j + 1) + (m + j) / 2) * (j + 1) + (n + j) / 2] = tmp;
+        }
+    out.insert(out.end(), w.begin(), w.end());
+  }
+  return out;
+}
+
+bool valid_hel(int j2, int h) { return h >= -j2 && h <= j2 && ((h + j2) % 2 == 0); }
+
+std::string check_model(const TpaModel& M, int n_cg) {
+  if (M.n_final != 3) return "only three-body final states are supported";
+  if (M.n_topo < 1 || M.n_topo > TPA_MAX_TOPO) return "n_topo out of range";
+  if (M.n_chain < 1 || M.n_chain > TPA_MAX_CHAIN) return "n_chain out of range";
+  if (M.P < 1 || M.P > TPA_MAX_P) return "P out of range";
+  if (M.top_j2 < 0 || M.top_j2 > TPA_MAX_J2) return "top spin out of range";
+  if (M.n_top < 1 || M.n_top > TPA_MAX_HEL) return "top helicity count out of range";
+  for (int i = 0; i < M.n_top; ++i)
+    if (!valid_hel(M.top_j2, M.hel_top[i])) return "bad top helicity";
+  int lam = M.n_top;
+  for (int f = 0; f < 3; ++f) {
+    if (M.final_j2[f] < 0 || M.final_j2[f] > 2) return "final spin above 1 unsupported";
+    if (M.n_fh[f] < 1 || M.n_fh[f] > 3) return "final helicity count out of range";
+    for (int i = 0; i < M.n_fh[f]; ++i)
+      if (!valid_hel(M.final_j2[f], M.hel_f[f][i])) return "bad final helicity";
+    if (M.ref_topo[f] < 0 || M.ref_topo[f] >= M.n_topo) return "bad reference topology";
+    lam *= M.n_fh[f];
+  }
+  if (lam != M.lam) return "lam does not match the helicity counts";
+  if (M.lam_pad < M.lam || M.lam_pad > TPA_MAX_LAM || (M.lam_pad & (M.lam_pad - 1)))
+    return "lam_pad must be a power of two >= lam";
+  for (int t = 0; t < M.n_topo; ++t) {
+    const TpaTopo& T = M.topo[t];
+    int s[3] = {T.x, T.y, T.z};
+    for (int i = 0; i < 3; ++i)
+      if (s[i] < 0 || s[i] > 2) return "bad topology final index";
+    if (T.x == T.y || T.x == T.z || T.y == T.z) return "topology finals must differ";
+  }
+  int next_k = 0;
+  for (int c = 0; c < M.n_chain; ++c) {
+    const TpaChain& C = M.chain[c];
+    if (C.topo < 0 || C.topo >= M.n_topo) return "chain topology out of range";
+    if (C.jr2 < 0 || C.jr2 > TPA_MAX_J2 || C.n_r < 1 || C.n_r > C.jr2 + 1)
+      return "bad resonance spin";
+    for (int i = 0; i < C.n_r; ++i)
+      if (!valid_hel(C.jr2, C.hel_r[i])) return "bad resonance helicity";
+    if (C.n_ls1 < 1 || C.n_ls1 > TPA_MAX_LS || C.n_ls2 < 1 || C.n_ls2 > TPA_MAX_LS)
+      return "LS count out of range";
+    for (int s2 = 0; s2 < C.n_ls1; ++s2)
+      if (C.l1[s2] < 0 || C.l1[s2] > 5) return "l out of range";
+    for (int s2 = 0; s2 < C.n_ls2; ++s2)
+      if (C.l2[s2] < 0 || C.l2[s2] > 5) return "l out of range";
+    if (C.k0 != next_k) return "chains must tile the partial-wave index range";
+    next_k += C.n_ls1 * C.n_ls2;
+    if (C.shape < 0 || C.shape > 2 || C.bw_l < 0 || C.bw_l > 5) return "bad lineshape";
+    const TpaTopo& T = M.topo[C.topo];
+    int nX = M.n_fh[T.x];
+    int n1 = C.n_ls1 * C.n_r * nX;
+    int fy = C.r_y_first ? T.y : T.z, fz = C.r_y_first ? T.z : T.y;
+    int n2 = C.n_ls2 * M.n_fh[fy] * M.n_fh[fz];
+    if (C.cg1_off < 0 || C.cg1_off + n1 > n_cg || C.cg2_off < 0 || C.cg2_off + n2 > n_cg)
+      return "CG table offsets out of range";
+  }
+  if (next_k != M.P) return "sum of chain partial waves differs from P";
+  if (n_cg < 0 || n_cg > TPA_MAX_CG || M.n_cg != n_cg) return "bad CG table size";
+  return "";
+}
+
+}  // namespace
+
+struct tpa_model_s {
+  int device = 0;
+  TpaModel h{};
+  TpaModel* d = nullptr;
+  double* d_cg = nullptr;
+  double* d_dw = nullptr;
+  hipStream_t stream = nullptr;
+};
+
+struct tpa_events_s {
+  tpa_model_s* m = nullptr;
+  int64_t n = 0;
+  int nf = 0;
+  double* d_ang = nullptr;
+  double2* d_h = nullptr;
+  double* d_w = nullptr;
+  double* d_part = nullptr;
+  double* d_out = nullptr;
+  double2* d_coef = nullptr;
+  double* d_f = nullptr;
+  hipEvent_t t0 = nullptr, t1 = nullptr;
+};
+
+// --------------------------------------------------------------------------- dispatcher
+// Partial waves are split over KG lane groups so that each lane holds at most 8 of them.
+template <int LG, int KG>
+static void launch_eval(const EvalArgs& a, int grid, hipStream_t s) {
+  hipLaunchKernelGGL((k_eval_fold<LG, KG, 8>), dim3(grid), dim3(256), 0, s, a);
+}
+
+static int eval_kgroups(int lg, int P) {
+  int kg = 1;
+  while (kg * 8 < P) kg <<= 1;
+  if (lg * kg > 64) return -1;
+  return kg;
+}
+
+template <int LG>
+static int dispatch_kg(int kg, const EvalArgs& a, int grid, hipStream_t s) {
+  switch (kg) {
+    case 1: launch_eval<LG, 1>(a, grid, s); return TPA_OK;
+    case 2: launch_eval<LG, 2>(a, grid, s); return TPA_OK;
+    case 4: launch_eval<LG, 4>(a, grid, s); return TPA_OK;
+    case 8: if (LG * 8 <= 64) { launch_eval<LG, (LG * 8 <= 64 ? 8 : 1)>(a, grid, s); return TPA_OK; }
+            return TPA_ERR_UNSUPPORTED;
+    case 16: if (LG * 16 <= 64) { launch_eval<LG, (LG * 16 <= 64 ? 16 : 1)>(a, grid, s); return TPA_OK; }
+             return TPA_ERR_UNSUPPORTED;
+    default: return TPA_ERR_UNSUPPORTED;
+  }
+}
+
+static int dispatch_eval(int lg, int kg, const EvalArgs& a, int grid, hipStream_t s) {
+  switch (lg) {
+    case 1: return dispatch_kg<1>(kg, a, grid, s);
+    case 2: return dispatch_kg<2>(kg, a, grid, s);
+    case 4: return dispatch_kg<4>(kg, a, grid, s);
+    case 8: return dispatch_kg<8>(kg, a, grid, s);
+    case 16: return dispatch_kg<16>(kg, a, grid, s);
+    default: return TPA_ERR_UNSUPPORTED;
+  }
+}
+
+static int eval_grid(int64_t n, int lanes_per_event) {
+  const int64_t epb = 4 * (64 / lanes_per_event);
+  int64_t g = (n + epb - 1) / epb;
+  if (g < 1) g = 1;
+  return (int)std::min<int64_t>(g, kEvalGridMax);
+}
+
+extern "C" {
+
+int tpa_abi_version(void) { return 1; }
+
+const char* tpa_last_error(void) { return g_err.c_str(); }
+
+int tpa_device_count(int* n) {
+  if (!n) return fail(TPA_ERR_ARG, "null pointer");
+  HIPCHK(hipGetDeviceCount(n));
+  return TPA_OK;
+}
+
+int tpa_model_create(const void* desc, int64_t desc_bytes, const double* cg, int n_cg,
+                     tpa_model_t* out) {
+  if (!desc || !out || (n_cg > 0 && !cg)) return fail(TPA_ERR_ARG, "null pointer");
+  if (desc_bytes != (int64_t)sizeof(TpaModel))
+    return fail(TPA_ERR_ARG, "model description size mismatch: got " +
+                                 std::to_string(desc_bytes) + " expected " +
+                                 std::to_string(sizeof(TpaModel)));
+  tpa_model_s* m = new tpa_model_s();
+  std::memcpy(&m->h, desc, sizeof(TpaModel));
+  std::string why = check_model(m->h, n_cg);
+  if (!why.empty()) {
+    delete m;
+    return fail(TPA_ERR_ARG, "invalid model: " + why);
+  }
+  hipError_t e;
+  std::vector<double> dw = wigner_weights();
+  if ((e = hipGetDevice(&m->device)) != hipSuccess ||
+      (e = hipStreamCreateWithFlags(&m->stream, hipStreamNonBlocking)) != hipSuccess ||
+      (e = hipMalloc(&m->d, sizeof(TpaModel))) != hipSuccess ||
+      (e = hipMalloc(&m->d_cg, sizeof(double) * std::max(1, n_cg))) != hipSuccess ||
+      (e = hipMalloc(&m->d_dw, sizeof(double) * dw.size())) != hipSuccess ||
+      (e = hipMemcpy(m->d, &m->h, sizeof(TpaModel), hipMemcpyHostToDevice)) != hipSuccess ||
+      (n_cg > 0 &&
+       (e = hipMemcpy(m->d_cg, cg, sizeof(double) * n_cg, hipMemcpyHostToDevice)) != hipSuccess) ||
+      (e = hipMemcpy(m->d_dw, dw.data(), sizeof(double) * dw.size(), hipMemcpyHostToDevice)) !=
+          hipSuccess) {
+    tpa_model_destroy(m);
+    return fail(TPA_ERR_HIP, std::string("model upload: ") + hipGetErrorString(e));
+  }
+  *out = m;
+  return TPA_OK;
+}
+
+int tpa_model_destroy(tpa_model_t m) {
+  if (!m) return TPA_OK;
+  if (m->d) (void)hipFree(m->d);
+  if (m->d_cg) (void)hipFree(m->d_cg);
+  if (m->d_dw) (void)hipFree(m->d_dw);
+  if (m->stream) (void)hipStreamDestroy(m->stream);
+  delete m;
+  return TPA_OK;
+}
+
+int tpa_model_set_shape(tpa_model_t m, const double* shape) {
+  if (!m || !shape) return fail(TPA_ERR_ARG, "null pointer");
+  for (int c = 0; c < m->h.n_chain; ++c) {
+    m->h.chain[c].m0 = shape[2 * c];
+    m->h.chain[c].g0 = shape[2 * c + 1];
+  }
+  HIPCHK(hipMemcpy(m->d, &m->h, sizeof(TpaModel), hipMemcpyHostToDevice));
+  return TPA_OK;
+}
+
+int tpa_events_destroy(tpa_events_t ev) {
+  if (!ev) return TPA_OK;
+  if (ev->d_ang) (void)hipFree(ev->d_ang);
+  if (ev->d_h) (void)hipFree(ev->d_h);
+  if (ev->d_w) (void)hipFree(ev->d_w);
+  if (ev->d_part) (void)hipFree(ev->d_part);
+  if (ev->d_out) (void)hipFree(ev->d_out);
+  if (ev->d_coef) (void)hipFree(ev->d_coef);
+  if (ev->d_f) (void)hipFree(ev->d_f);
+  if (ev->t0) (void)hipEventDestroy(ev->t0);
+  if (ev->t1) (void)hipEventDestroy(ev->t1);
+  delete ev;
+  return TPA_OK;
+}
+
+int tpa_events_create(tpa_model_t m, const double* p4, int64_t n, const double* weight,
+                      tpa_events_t* out) {
+  if (!m || !p4 || !out) return fail(TPA_ERR_ARG, "null pointer");
+  if (n < 1) return fail(TPA_ERR_ARG, "event count must be positive");
+  tpa_events_s* ev = new tpa_events_s();
+  ev->m = m;
+  ev->n = n;
+  ev->nf = TPA_NF_TOPO * m->h.n_topo + 4;
+  const int P = m->h.P;
+  double* d_p4 = nullptr;
+  hipError_t e;
+  auto bail = [&](hipError_t er, const char* what) {
+    if (d_p4) (void)hipFree(d_p4);
+    tpa_events_destroy(ev);
+    return fail(TPA_ERR_HIP, std::string(what) + ": " + hipGetErrorString(er));
+  };
+  if ((e = hipMalloc(&d_p4, sizeof(double) * 12 * n)) != hipSuccess) return bail(e, "alloc p4");
+  if ((e = hipMalloc(&ev->d_ang, sizeof(double) * ev->nf * n)) != hipSuccess)
+    return bail(e, "alloc angles");
+  if ((e = hipMalloc(&ev->d_w, sizeof(double) * n)) != hipSuccess) return bail(e, "alloc w");
+  if ((e = hipMalloc(&ev->d_part, sizeof(double) * kEvalGridMax * (2 + 2 * TPA_MAX_P))) !=
+      hipSuccess)
+    return bail(e, "alloc partials");
+  if ((e = hipMalloc(&ev->d_out, sizeof(double) * (2 + 2 * TPA_MAX_P))) != hipSuccess)
+    return bail(e, "alloc out");
+  if ((e = hipMalloc(&ev->d_coef, sizeof(double2) * std::max(P, 1))) != hipSuccess)
+    return bail(e, "alloc coef");
+  if ((e = hipEventCreate(&ev->t0)) != hipSuccess || (e = hipEventCreate(&ev->t1)) != hipSuccess)
+    return bail(e, "event create");
+  if ((e = hipMemcpyAsync(d_p4, p4, sizeof(double) * 12 * n, hipMemcpyHostToDevice, m->stream)) !=
+      hipSuccess)
+    return bail(e, "upload p4");
+  if (weight) {
+    if ((e = hipMemcpyAsync(ev->d_w, weight, sizeof(double) * n, hipMemcpyHostToDevice,
+                            m->stream)) != hipSuccess)
+      return bail(e, "upload weights");
+  } else {
+    std::vector<double> ones(n, 1.0);
+    if ((e = hipMemcpyAsync(ev->d_w, ones.data(), sizeof(double) * n, hipMemcpyHostToDevice,
+                            m->stream)) != hipSuccess)
+      return bail(e, "upload weights");
+    if ((e = hipStreamSynchronize(m->stream)) != hipSuccess) return bail(e, "sync");
+  }
+  const int block = 256;
+  const int64_t grid = (n + block - 1) / block;
+  hipLaunchKernelGGL(k_cal_angle3, dim3((unsigned)grid), dim3(block), 0, m->stream, m->d, d_p4,
+                     n, ev->d_ang);
+  if ((e = hipGetLastError()) != hipSuccess) return bail(e, "launch cal_angle");
+  if ((e = hipStreamSynchronize(m->stream)) != hipSuccess) return bail(e, "cal_angle");
+  (void)hipFree(d_p4);
+  d_p4 = nullptr;
+  *out = ev;
+  return TPA_OK;
+}
+
+int64_t tpa_events_size(tpa_events_t ev) { return ev ? ev->n : -1; }
+
+int tpa_events_set_weight(tpa_events_t ev, const double* weight) {
+  if (!ev || !weight) return fail(TPA_ERR_ARG, "null pointer");
+  HIPCHK(hipMemcpyAsync(ev->d_w, weight, sizeof(double) * ev->n, hipMemcpyHostToDevice,
+                        ev->m->stream));
+  HIPCHK(hipStreamSynchronize(ev->m->stream));
+  return TPA_OK;
+}
+
+int tpa_events_get_angles(tpa_events_t ev, double* out, int64_t out_len) {
+  if (!ev || !out) return fail(TPA_ERR_ARG, "null pointer");
+  if (out_len < (int64_t)ev->nf * ev->n) return fail(TPA_ERR_ARG, "output buffer too small");
+  HIPCHK(hipMemcpyAsync(out, ev->d_ang, sizeof(double) * ev->nf * ev->n, hipMemcpyDeviceToHost,
+                        ev->m->stream));
+  HIPCHK(hipStreamSynchronize(ev->m->stream));
+  return TPA_OK;
+}
+
+int tpa_events_build(tpa_events_t ev, int fold) {
+  if (!ev) return fail(TPA_ERR_ARG, "null pointer");
+  const TpaModel& M = ev->m->h;
+  if (!ev->d_h) HIPCHK(hipMalloc(&ev->d_h, sizeof(double2) * (size_t)M.P * ev->n * M.lam_pad));
+  const int block = 128;
+  const int64_t grid = (ev->n + block - 1) / block;
+  hipLaunchKernelGGL(k_build_pw, dim3((unsigned)grid), dim3(block), 0, ev->m->stream, ev->m->d,
+                     ev->d_ang, ev->n, ev->m->d_cg, ev->m->d_dw, fold, ev->d_h);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(ev->m->stream));
+  return TPA_OK;
+}
+
+int tpa_events_get_pw(tpa_events_t ev, double* out, int64_t out_len) {
+  if (!ev || !out) return fail(TPA_ERR_ARG, "null pointer");
+  if (!ev->d_h) return fail(TPA_ERR_ARG, "partial-wave cache not built");
+  const TpaModel& M = ev->m->h;
+  const int64_t len = 2 * (int64_t)M.P * ev->n * M.lam_pad;
+  if (out_len < len) return fail(TPA_ERR_ARG, "output buffer too small");
+  HIPCHK(hipMemcpyAsync(out, ev->d_h, sizeof(double) * len, hipMemcpyDeviceToHost,
+                        ev->m->stream));
+  HIPCHK(hipStreamSynchronize(ev->m->stream));
+  return TPA_OK;
+}
+
+int tpa_eval(tpa_events_t ev, const double* coef, int mode, int flags, double* out,
+             double* f_out) {
+  if (!ev || !coef || !out) return fail(TPA_ERR_ARG, "null pointer");
+  if (!ev->d_h) return fail(TPA_ERR_ARG, "partial-wave cache not built");
+  if (mode != TPA_MODE_DATA && mode != TPA_MODE_MC) return fail(TPA_ERR_ARG, "bad mode");
+  const TpaModel& M = ev->m->h;
+  hipStream_t s = ev->m->stream;
+  const bool want_f = (flags & TPA_FLAG_FOUT) != 0;
+  if (want_f && !f_out) return fail(TPA_ERR_ARG, "f_out required");
+  if (want_f && !ev->d_f) HIPCHK(hipMalloc(&ev->d_f, sizeof(double) * ev->n));
+  HIPCHK(hipMemcpyAsync(ev->d_coef, coef, sizeof(double2) * M.P, hipMemcpyHostToDevice, s));
+  EvalArgs a;
+  a.h = ev->d_h;
+  a.w = ev->d_w;
+  a.coef = ev->d_coef;
+  a.partials = ev->d_part;
+  a.f_out = want_f ? ev->d_f : nullptr;
+  a.N = ev->n;
+  a.P = M.P;
+  a.mode = mode;
+  a.grad = (flags & TPA_FLAG_GRAD) ? 1 : 0;
+  a.nacc = 2 + 2 * M.P;
+  const int kg = eval_kgroups(M.lam_pad, M.P);
+  if (kg < 0) return fail(TPA_ERR_UNSUPPORTED, "too many partial waves for this helicity count");
+  const int grid = eval_grid(ev->n, M.lam_pad * kg);
+  bool timing;
+  {
+    std::lock_guard<std::mutex> lk(g_timing_mu);
+    timing = g_timing;
+  }
+  if (timing) HIPCHK(hipEventRecord(ev->t0, s));
+  int rc = dispatch_eval(M.lam_pad, kg, a, grid, s);
+  if (rc != TPA_OK) return fail(rc, "no evaluation kernel for this helicity/partial-wave count");
+  HIPCHK(hipGetLastError());
+  if (timing) HIPCHK(hipEventRecord(ev->t1, s));
+  hipLaunchKernelGGL(k_reduce, dim3(1), dim3(256), 0, s, ev->d_part, grid, a.nacc, ev->d_out);
+  HIPCHK(hipGetLastError());
+  const int nout = a.grad ? a.nacc : 2;
+  HIPCHK(hipMemcpyAsync(out, ev->d_out, sizeof(double) * nout, hipMemcpyDeviceToHost, s));
+  if (want_f)
+    HIPCHK(hipMemcpyAsync(f_out, ev->d_f, sizeof(double) * ev->n, hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+  if (timing) {
+    float ms = 0.0f;
+    HIPCHK(hipEventElapsedTime(&ms, ev->t0, ev->t1));
+    std::lock_guard<std::mutex> lk(g_timing_mu);
+    g_timing_ms += ms;
+    g_timing_n += 1;
+  }
+  return TPA_OK;
+}
+
+int tpa_timing_enable(int on) {
+  std::lock_guard<std::mutex> lk(g_timing_mu);
+  g_timing = on != 0;
+  return TPA_OK;
+}
+
+int tpa_timing_read(double* ms, int64_t* launches) {
+  std::lock_guard<std::mutex> lk(g_timing_mu);
+  if (ms) *ms = g_timing_ms;
+  if (launches) *launches = g_timing_n;
+  return TPA_OK;
+}
+
+int tpa_timing_reset(void) {
+  std::lock_guard<std::mutex> lk(g_timing_mu);
+  g_timing_ms = 0.0;
+  g_timing_n = 0;
+  return TPA_OK;
+}
+
+}  // extern "C"

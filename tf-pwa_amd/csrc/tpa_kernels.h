@@ -1,0 +1,465 @@
+// HIP kernels of the unbinned-likelihood engine (gfx950, FP64 path).
+//
+//   k_cal_angle3   p4 (SoA) -> per-topology helicity + aligned angles and masses
+//                  (tf_pwa/cal_angle.py:262-490, angle.py:98-392)
+//   k_build_pw     angles -> partial-wave amplitudes h[k][event][lambda] that carry every
+//                  parameter-independent factor: Wigner D of both vertices, CG x barrier
+//                  factor per LS coupling, the contraction over the resonance helicity and
+//                  the aligned D of spin>0 finals (amp/core.py:931-1164, 1392-1445,
+//                  dfun.py:118-266); optionally the Breit-Wigner is folded in.
+//   k_eval_fold    per evaluation: A_lambda = sum_k c_k h_k,lambda, f = sum |A|^2, the
+//                  weighted log-sum (clip_log) or linear MC sum with a compensated
+//                  accumulator, and the adjoint sums G_k = sum_e u_e sum_lambda conj(A) h_k
+//                  (model/model.py:41-94, 283-292, 407-449).
+//   k_reduce       deterministic fixed-order reduction of the per-block partial sums.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdint.h>
+
+#include "tpa_math.h"
+
+// ----------------------------------------------------------------------------- cal_angle
+__global__ __launch_bounds__(256) void k_cal_angle3(const TpaModel* __restrict__ M,
+                                                    const double* __restrict__ p4, int64_t N,
+                                                    double* __restrict__ ang) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= N) return;
+  const int nt = M->n_topo;
+  const int64_t gbase = (int64_t)TPA_NF_TOPO * nt;
+  v4 p[TPA_MAX_FINAL];
+  for (int f = 0; f < 3; ++f) {
+    p[f].t = p4[(int64_t)(f * 4 + 0) * N + e];
+    p[f].x = p4[(int64_t)(f * 4 + 1) * N + e];
+    p[f].y = p4[(int64_t)(f * 4 + 2) * N + e];
+    p[f].z = p4[(int64_t)(f * 4 + 3) * N + e];
+  }
+  v4 ptop = lv_add(lv_add(p[0], p[1]), p[2]);
+  ang[(gbase + 0) * N + e] = lv_M(ptop);
+  for (int f = 0; f < 3; ++f) ang[(gbase + 1 + f) * N + e] = lv_M(p[f]);
+
+  v3 base_z = v_make(0.0, 0.0, 1.0);
+  if (M->random_z) {
+    v3 p3 = lv_vect(ptop);
+    if (!(v_norm(p3) < 1.0e-5)) base_z = p3;
+  }
+  const v3 base_x = v_make(1.0, 0.0, 0.0);
+
+  su2 rm[TPA_MAX_TOPO][TPA_MAX_FINAL];
+  su2 bm[TPA_MAX_TOPO][TPA_MAX_FINAL];
+  for (int t = 0; t < nt; ++t) {
+    const TpaTopo T = M->topo[t];
+    double* out = ang + (int64_t)(TPA_NF_TOPO * t) * N + e;
+    v4 pR = lv_add(p[T.y < T.z ? T.y : T.z], p[T.y < T.z ? T.z : T.y]);
+    out[TPA_F_MR * N] = lv_M(pR);
+    // cal_chain_boost: everything into the top rest frame, daughters of R into R's frame
+    v4 pRt = lv_rest(ptop, pR);
+    v4 pXt = lv_rest(ptop, p[T.x]);
+    v4 pYt = lv_rest(ptop, p[T.y]);
+    v4 pZt = lv_rest(ptop, p[T.z]);
+    // top decay in the representative order (alpha folded to [-pi,pi) then [-2pi,0))
+    double bias = -M_PI;
+    v3 xR = v_make(1.0, 0.0, 0.0);
+    su2 rR = su2_rot_y(0.0), bR = su2_rot_y(0.0);
+    for (int pos = 0; pos < 2; ++pos) {
+      const bool isR = (pos == 0) == (T.r_first != 0);
+      const v4 pj = isR ? pRt : pXt;
+      double a, b;
+      v3 x;
+      angle_zx_z_getx(base_z, base_x, lv_vect(pj), &a, &b, &x);
+      a = fmod_floor(a - bias, 2.0 * M_PI) + bias;
+      bias -= M_PI;
+      su2 bmat = su2_boost_z(lv_omega(pj));
+      su2 rmat = su2_mul(su2_rot_y(b), su2_rot_z(a));
+      if (isR) {
+        out[(TPA_F_AR + 0) * N] = a;
+        out[(TPA_F_AR + 1) * N] = b;
+        xR = x;
+        rR = rmat;
+        bR = bmat;
+      } else {
+        out[(TPA_F_AX + 0) * N] = a;
+        out[(TPA_F_AX + 1) * N] = b;
+        rm[t][T.x] = rmat;
+        bm[t][T.x] = bmat;
+      }
+    }
+    // resonance decay, daughters y then z
+    bias = -M_PI;
+    const v3 zR = lv_vect(pRt);
+    for (int pos = 0; pos < 2; ++pos) {
+      const int f = pos == 0 ? T.y : T.z;
+      const v4 pjr = lv_rest(pRt, pos == 0 ? pYt : pZt);
+      double a, b;
+      v3 x;
+      angle_zx_z_getx(zR, xR, lv_vect(pjr), &a, &b, &x);
+      a = fmod_floor(a - bias, 2.0 * M_PI) + bias;
+      bias -= M_PI;
+      const int fld = pos == 0 ? TPA_F_AY : TPA_F_AZ;
+      out[(fld + 0) * N] = a;
+      out[(fld + 1) * N] = b;
+      bm[t][f] = su2_boost_z(lv_omega(pjr));
+      rm[t][f] = su2_mul(su2_mul(su2_mul(su2_rot_y(b), su2_rot_z(a)), bR), rR);
+    }
+  }
+  // aligned angles (cal_angle.py:444-474, rule 1 reference frames, r_boost + final_rest)
+  for (int t = 0; t < nt; ++t) {
+    double* out = ang + (int64_t)(TPA_NF_TOPO * t) * N + e;
+    for (int f = 0; f < 3; ++f) {
+      double al = 0.0, be = 0.0, ga = 0.0;
+      if (M->topo[t].aligned[f]) {
+        const int r0 = M->ref_topo[f];
+        su2 R = su2_mul(rm[r0][f], su2_inv(rm[t][f]));
+        R = su2_mul(su2_mul(bm[r0][f], R), su2_inv(bm[t][f]));
+        su2_euler(R, &al, &be, &ga);
+      }
+      out[(TPA_F_AL + 3 * f + 0) * N] = al;
+      out[(TPA_F_AL + 3 * f + 1) * N] = be;
+      out[(TPA_F_AL + 3 * f + 2) * N] = ga;
+    }
+  }
+}
+
+// ------------------------------------------------------------------ partial-wave build
+__global__ __launch_bounds__(128) void k_build_pw(const TpaModel* __restrict__ M,
+                                                  const double* __restrict__ ang, int64_t N,
+                                                  const double* __restrict__ cg,
+                                                  const double* __restrict__ dw, int fold,
+                                                  double2* __restrict__ h) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= N) return;
+  const int nt = M->n_topo;
+  const int lam_pad = M->lam_pad;
+  const int64_t gbase = (int64_t)TPA_NF_TOPO * nt;
+  const double mtop = ang[(gbase + 0) * N + e];
+  double mf[3];
+  for (int f = 0; f < 3; ++f) mf[f] = ang[(gbase + 1 + f) * N + e];
+  const int ja = M->top_j2;
+  const int nA = M->n_top;
+  int nfh[3];
+  for (int f = 0; f < 3; ++f) nfh[f] = M->n_fh[f];
+
+  for (int c = 0; c < M->n_chain; ++c) {
+    const TpaChain& C = M->chain[c];
+    const TpaTopo T = M->topo[C.topo];
+    const double* A = ang + (int64_t)(TPA_NF_TOPO * C.topo) * N + e;
+    const double mR = A[TPA_F_MR * N];
+    const int fx = T.x;
+    const int fy = C.r_y_first ? T.y : T.z;  // this chain's first daughter of R
+    const int fz = C.r_y_first ? T.z : T.y;
+    // vertex 1: top -> (R, X) in this chain's order; D uses the angle of outs[0]
+    double a1, b1;
+    if (C.top_r_first) {
+      a1 = A[(TPA_F_AR + 0) * N];
+      b1 = A[(TPA_F_AR + 1) * N];
+    } else {
+      a1 = A[(TPA_F_AX + 0) * N];
+      b1 = A[(TPA_F_AX + 1) * N];
+    }
+    const double q1sq = C.top_r_first ? rel_p2(mtop, mR, mf[fx]) : rel_p2(mtop, mf[fx], mR);
+    // vertex 2: R -> (fy, fz)
+    const int fld2 = (fy == T.y) ? TPA_F_AY : TPA_F_AZ;
+    const double a2 = A[(fld2 + 0) * N];
+    const double b2 = A[(fld2 + 1) * N];
+    const double q2sq = rel_p2(mR, mf[fy], mf[fz]);
+    const double dd = C.d * C.d;
+    double bf1[TPA_MAX_LS], bf2[TPA_MAX_LS];
+    for (int s = 0; s < C.n_ls1; ++s) {
+      const int l = C.l1[s];
+      const double ql = pow(q1sq, 0.5 * l);
+      bf1[s] = C.act1[s] ? ql / sqrt(bprime_poly(l, q1sq * dd)) : ql;
+    }
+    for (int s = 0; s < C.n_ls2; ++s) {
+      const int l = C.l2[s];
+      const double ql = pow(q2sq, 0.5 * l);
+      bf2[s] = C.act2[s] ? ql / sqrt(bprime_poly(l, q2sq * dd)) : ql;
+    }
+    // D^{J_top*}_{lambda_A, delta}(a1, b1, 0) for all delta in [-J, J]
+    double sp[TPA_MAX_J2 + 1], cp[TPA_MAX_J2 + 1];
+    cd D1[TPA_MAX_HEL][TPA_MAX_J2 + 1];
+    half_angle_powers(b1, ja, sp, cp);
+    const double* wA = dw + dweight_offset(ja);
+    for (int iA = 0; iA < nA; ++iA) {
+      const int m = M->hel_top[iA];
+      const cd ph = c_expi((0.5 * m) * a1);
+      for (int dl = -ja; dl <= ja; dl += 2)
+        D1[iA][(dl + ja) / 2] = c_scale(ph, wigner_d(wA, ja, m, dl, sp, cp));
+    }
+    const int jr = C.jr2;
+    cd D2[TPA_MAX_HEL][TPA_MAX_J2 + 1];
+    half_angle_powers(b2, jr, sp, cp);
+    const double* wR = dw + dweight_offset(jr);
+    for (int iR = 0; iR < C.n_r; ++iR) {
+      const int m = C.hel_r[iR];
+      const cd ph = c_expi((0.5 * m) * a2);
+      for (int dl = -jr; dl <= jr; dl += 2)
+        D2[iR][(dl + jr) / 2] = c_scale(ph, wigner_d(wR, jr, m, dl, sp, cp));
+    }
+    // aligned D^{J_f*}_{lambda, lambda'}(alpha, beta, gamma) of spin>0 finals
+    cd Da[3][3][3];
+    int has_al[3];
+    for (int f = 0; f < 3; ++f) {
+      const int jf = M->final_j2[f];
+      has_al[f] = (T.aligned[f] && jf != 0) ? 1 : 0;
+      if (!has_al[f]) continue;
+      const double al = A[(TPA_F_AL + 3 * f + 0) * N];
+      const double be = A[(TPA_F_AL + 3 * f + 1) * N];
+      const double ga = A[(TPA_F_AL + 3 * f + 2) * N];
+      half_angle_powers(be, jf, sp, cp);
+      const double* wf = dw + dweight_offset(jf);
+      for (int i = 0; i < nfh[f]; ++i)
+        for (int i2 = 0; i2 < nfh[f]; ++i2) {
+          const int m1 = M->hel_f[f][i], m2 = M->hel_f[f][i2];
+          const cd eae = c_mul(c_expi((0.5 * m1) * al), c_expi((0.5 * m2) * ga));
+          Da[f][i][i2] = c_scale(eae, wigner_d(wf, jf, m1, m2, sp, cp));
+        }
+    }
+    cd bw = c_make(1.0, 0.0);
+    if (fold && C.shape) {
+      const double q = rel_p(mR, mf[fy], mf[fz]);
+      const double q0 = rel_p(C.m0, C.m_y, C.m_z);
+      bw = lineshape(C.shape, mR, C.m0, C.g0, q, q0, C.bw_l, C.d);
+    }
+    // helicity bookkeeping of vertex 1 (b1, c1) and vertex 2 (b2, c2)
+    const int nX = nfh[fx];
+    const int nb1 = C.top_r_first ? C.n_r : nX;
+    const int nc1 = C.top_r_first ? nX : C.n_r;
+    const int nb2 = nfh[fy], nc2 = nfh[fz];
+    for (int s1 = 0; s1 < C.n_ls1; ++s1) {
+      for (int s2 = 0; s2 < C.n_ls2; ++s2) {
+        const int64_t k = C.k0 + s1 * C.n_ls2 + s2;
+        double2* hk = h + ((int64_t)k * N + e) * lam_pad;
+        int lf = 0;
+        for (int iA = 0; iA < nA; ++iA)
+          for (int o0 = 0; o0 < nfh[0]; ++o0)
+            for (int o1 = 0; o1 < nfh[1]; ++o1)
+              for (int o2 = 0; o2 < nfh[2]; ++o2, ++lf) {
+                const int oo[3] = {o0, o1, o2};
+                cd sum = c_make(0.0, 0.0);
+                int lo[3], hi[3];
+                for (int f = 0; f < 3; ++f) {
+                  lo[f] = has_al[f] ? 0 : oo[f];
+                  hi[f] = has_al[f] ? nfh[f] : oo[f] + 1;
+                }
+                int j[3];
+                for (j[0] = lo[0]; j[0] < hi[0]; ++j[0])
+                  for (j[1] = lo[1]; j[1] < hi[1]; ++j[1])
+                    for (j[2] = lo[2]; j[2] < hi[2]; ++j[2]) {
+                      cd dal = c_make(1.0, 0.0);
+                      for (int f = 0; f < 3; ++f)
+                        if (has_al[f]) dal = c_mul(dal, Da[f][j[f]][oo[f]]);
+                      const int jX = j[fx];
+                      const int lX = M->hel_f[fx][jX];
+                      const int ib2 = j[fy], ic2 = j[fz];
+                      const int d2 = M->hel_f[fy][ib2] - M->hel_f[fz][ic2];
+                      if (d2 > jr || d2 < -jr) continue;
+                      const double cg2v = cg[C.cg2_off + (s2 * nb2 + ib2) * nc2 + ic2];
+                      cd acc = c_make(0.0, 0.0);
+                      for (int iR = 0; iR < C.n_r; ++iR) {
+                        const int lR = C.hel_r[iR];
+                        const int d1 = C.top_r_first ? lR - lX : lX - lR;
+                        if (d1 > ja || d1 < -ja) continue;
+                        const int ib1 = C.top_r_first ? iR : jX;
+                        const int ic1 = C.top_r_first ? jX : iR;
+                        const double cg1v = cg[C.cg1_off + (s1 * nb1 + ib1) * nc1 + ic1];
+                        const cd v1 = c_scale(D1[iA][(d1 + ja) / 2], cg1v * bf1[s1]);
+                        const cd v2 = c_scale(D2[iR][(d2 + jr) / 2], cg2v * bf2[s2]);
+                        acc = c_add(acc, c_mul(v1, v2));
+                      }
+                      sum = c_add(sum, c_mul(acc, dal));
+                    }
+                sum = c_mul(sum, bw);
+                hk[lf] = make_double2(sum.r, sum.i);
+              }
+        for (; lf < lam_pad; ++lf) hk[lf] = make_double2(0.0, 0.0);
+      }
+    }
+  }
+}
+
+// ------------------------------------------------------------------------- evaluation
+struct EvalArgs {
+  const double2* __restrict__ h;     // [P][N][LG]
+  const double* __restrict__ w;      // [N]
+  const double2* __restrict__ coef;  // [P]
+  double* __restrict__ partials;     // [grid][nacc]
+  double* __restrict__ f_out;        // [N] or null
+  int64_t N;
+  int P;
+  int mode;  // 0: data, sum w * clip_log(f); 1: MC, sum w * f
+  int grad;
+  int nacc;  // 2 + 2P
+};
+
+TPA_DEV void neumaier_add(double& s, double& c, double y) {
+  const double t = s + y;
+  if (fabs(s) >= fabs(y))
+    c += (s - t) + y;
+  else
+    c += (y - t) + s;
+  s = t;
+}
+TPA_DEV void neumaier_merge(double& s, double& c, double s2, double c2) {
+  const double t = s + s2;
+  double err;
+  if (fabs(s) >= fabs(s2))
+    err = (s - t) + s2;
+  else
+    err = (s2 - t) + s;
+  c = (c + c2) + err;
+  s = t;
+}
+
+#define TPA_NACC_MAX (2 + 2 * TPA_MAX_P)
+
+// Lane layout: lane = ev * (LG * KG) + kg * LG + lam. The LG lanes of a k-group hold the
+// helicity components of one event, and the KG groups split the partial waves
+// k = kg + KG * j (j < PML). A_lambda is completed by a reduction over the k-groups and f by a
+// reduction over lambda, so each lane keeps only PML amplitudes and PML gradient sums.
+template <int LG, int KG, int PML>
+__global__ __launch_bounds__(256) void k_eval_fold(EvalArgs a) {
+  constexpr int LPE = LG * KG;  // lanes per event
+  constexpr int EPW = 64 / LPE;
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  const int lam = lane & (LG - 1);
+  const int kg = (lane / LG) & (KG - 1);
+  const int el = lane / LPE;
+  const double eps = 1.0e-6;
+  const double logeps = log(1.0e-6);
+  double vs = 0.0, vc = 0.0;
+  double gr[PML], gi[PML], cr[PML], ci[PML];
+#pragma unroll
+  for (int j = 0; j < PML; ++j) {
+    const int k = kg + KG * j;
+    gr[j] = gi[j] = 0.0;
+    const double2 c = k < a.P ? a.coef[k] : make_double2(0.0, 0.0);
+    cr[j] = c.x;
+    ci[j] = c.y;
+  }
+  const int64_t stride = (int64_t)gridDim.x * 4 * EPW;
+  for (int64_t base = ((int64_t)blockIdx.x * 4 + wave) * EPW; base < a.N; base += stride) {
+    const int64_t e = base + el;
+    const bool valid = e < a.N;
+    double2 hv[PML];
+    double ar = 0.0, ai = 0.0;
+#pragma unroll
+    for (int j = 0; j < PML; ++j) {
+      const int k = kg + KG * j;
+      hv[j] = (valid && k < a.P) ? a.h[((int64_t)k * a.N + e) * LG + lam] : make_double2(0.0, 0.0);
+    }
+#pragma unroll
+    for (int j = 0; j < PML; ++j) {
+      ar += cr[j] * hv[j].x - ci[j] * hv[j].y;
+      ai += cr[j] * hv[j].y + ci[j] * hv[j].x;
+    }
+#pragma unroll
+    for (int m = LG; m < LPE; m <<= 1) {
+      ar += __shfl_xor(ar, m, 64);
+      ai += __shfl_xor(ai, m, 64);
+    }
+    double f = ar * ar + ai * ai;
+#pragma unroll
+    for (int m = 1; m < LG; m <<= 1) f += __shfl_xor(f, m, 64);
+    double y = 0.0, u = 0.0;
+    if (valid) {
+      const double w = a.w[e];
+      if (a.mode == 0) {
+        const double dom = (w == 0.0) ? 1.0 : w;
+        const double x = (f * w) / dom;
+        double cl, dcl;
+        if (x > eps) {
+          cl = log(x);
+          dcl = 1.0 / x;
+        } else {
+          const double dx = x - eps;
+          const double t = dx / eps;
+          cl = (logeps + t) - (t * t) / 2.0;
+          dcl = 1.0 / eps - t / eps;
+        }
+        y = w * cl;
+        u = (w * dcl) * (w / dom);
+      } else {
+        y = w * f;
+        u = w;
+      }
+      if (a.f_out != nullptr && lam == 0 && kg == 0) a.f_out[e] = f;
+    }
+    if (lam == 0 && kg == 0) neumaier_add(vs, vc, y);
+    if (a.grad) {
+      const double tr = u * ar, ti = -(u * ai);
+#pragma unroll
+      for (int j = 0; j < PML; ++j) {
+        gr[j] += tr * hv[j].x - ti * hv[j].y;
+        gi[j] += tr * hv[j].y + ti * hv[j].x;
+      }
+    }
+  }
+  // value: full wave reduction of the compensated pair
+#pragma unroll
+  for (int m = 32; m >= 1; m >>= 1) {
+    const double s2 = __shfl_xor(vs, m, 64);
+    const double c2 = __shfl_xor(vc, m, 64);
+    neumaier_merge(vs, vc, s2, c2);
+  }
+  __shared__ double sh[4][TPA_NACC_MAX];
+  if (a.grad) {
+    // gradient slots: reduce over lambda and events, keep the k-group bits
+#pragma unroll
+    for (int j = 0; j < PML; ++j) {
+      double xr = gr[j], xi = gi[j];
+#pragma unroll
+      for (int m = 1; m < LG; m <<= 1) {
+        xr += __shfl_xor(xr, m, 64);
+        xi += __shfl_xor(xi, m, 64);
+      }
+#pragma unroll
+      for (int m = LPE; m < 64; m <<= 1) {
+        xr += __shfl_xor(xr, m, 64);
+        xi += __shfl_xor(xi, m, 64);
+      }
+      const int k = kg + KG * j;
+      if (el == 0 && lam == 0 && k < a.P) {
+        sh[wave][2 + 2 * k] = xr;
+        sh[wave][3 + 2 * k] = xi;
+      }
+    }
+  }
+  if (lane == 0) {
+    sh[wave][0] = vs;
+    sh[wave][1] = vc;
+  }
+  __syncthreads();
+  double* out = a.partials + (int64_t)blockIdx.x * a.nacc;
+  if (threadIdx.x == 0) {
+    double s = sh[0][0], c = sh[0][1];
+    for (int wv = 1; wv < 4; ++wv) neumaier_merge(s, c, sh[wv][0], sh[wv][1]);
+    out[0] = s;
+    out[1] = c;
+  }
+  if (a.grad) {
+    for (int j = 2 + threadIdx.x; j < a.nacc; j += blockDim.x)
+      out[j] = ((sh[0][j] + sh[1][j]) + sh[2][j]) + sh[3][j];
+  }
+}
+
+// Fixed-order reduction over blocks: slot 0/1 are the (sum, compensation) of the value,
+// the remaining slots are summed with a compensated accumulator each.
+__global__ __launch_bounds__(256) void k_reduce(const double* __restrict__ partials, int nblocks,
+                                                int nacc, double* __restrict__ out) {
+  for (int j = threadIdx.x; j < nacc; j += blockDim.x) {
+    if (j == 1) {
+      out[1] = 0.0;
+      continue;
+    }
+    double s = 0.0, c = 0.0;
+    for (int b = 0; b < nblocks; ++b) neumaier_add(s, c, partials[(int64_t)b * nacc + j]);
+    if (j == 0) {
+      double cc = 0.0, ccc = 0.0;
+      for (int b = 0; b < nblocks; ++b) neumaier_add(cc, ccc, partials[(int64_t)b * nacc + 1]);
+      c += cc + ccc;
+    }
+    out[j] = s + c;
+  }
+}
+

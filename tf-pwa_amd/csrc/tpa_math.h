@@ -1,0 +1,231 @@
+// Device math for the likelihood kernels: complex numbers, Lorentz boosts, SU(2) frames,
+// Wigner d-functions, barrier factors and Breit-Wigner lineshapes, all in FP64.
+//
+// Each function states the tf-pwa function whose arithmetic it reproduces; the operation
+// order follows the reference so results agree to rounding.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <math.h>
+
+#include "tpa_model.h"
+
+#define TPA_DEV __device__ __forceinline__
+
+struct cd {
+  double r, i;
+};
+
+TPA_DEV cd c_make(double r, double i) { return cd{r, i}; }
+TPA_DEV cd c_add(cd a, cd b) { return cd{a.r + b.r, a.i + b.i}; }
+TPA_DEV cd c_sub(cd a, cd b) { return cd{a.r - b.r, a.i - b.i}; }
+TPA_DEV cd c_mul(cd a, cd b) { return cd{a.r * b.r - a.i * b.i, a.r * b.i + a.i * b.r}; }
+TPA_DEV cd c_scale(cd a, double s) { return cd{a.r * s, a.i * s}; }
+TPA_DEV cd c_conj(cd a) { return cd{a.r, -a.i}; }
+TPA_DEV cd c_neg(cd a) { return cd{-a.r, -a.i}; }
+TPA_DEV double c_abs2(cd a) { return a.r * a.r + a.i * a.i; }
+// 1/a (used for the SU(2) diagonal elements; |a| is O(1) there)
+TPA_DEV cd c_inv(cd a) {
+  double d = a.r * a.r + a.i * a.i;
+  return cd{a.r / d, -a.i / d};
+}
+TPA_DEV cd c_expi(double x) {
+  double s, c;
+  sincos(x, &s, &c);
+  return cd{c, s};
+}
+
+// ---------------------------------------------------------------- 3-vectors (angle.py:13-95)
+struct v3 {
+  double x, y, z;
+};
+TPA_DEV v3 v_make(double x, double y, double z) { return v3{x, y, z}; }
+TPA_DEV double v_dot(v3 a, v3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+TPA_DEV double v_norm(v3 a) { return sqrt(a.x * a.x + a.y * a.y + a.z * a.z); }
+TPA_DEV v3 v_scale(v3 a, double s) { return v3{a.x * s, a.y * s, a.z * s}; }
+TPA_DEV v3 v_div(v3 a, double s) { return v3{a.x / s, a.y / s, a.z / s}; }
+TPA_DEV v3 v_unit(v3 a) { return v_div(a, v_norm(a)); }
+TPA_DEV v3 v_cross(v3 a, v3 b) {
+  return v3{a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+// Vector3.cross_unit (angle.py:57-67): fall back to a x (1 + b) for parallel inputs.
+TPA_DEV v3 v_cross_unit(v3 a, v3 b) {
+  v3 c = v_cross(a, b);
+  if (v_norm(c) < 1.0e-14) c = v_cross(a, v3{1.0 + b.x, 1.0 + b.y, 1.0 + b.z});
+  return v_unit(c);
+}
+TPA_DEV double v_angle_from(v3 v, v3 x, v3 y) { return atan2(v_dot(v, y), v_dot(v, x)); }
+
+// --------------------------------------------------------- 4-vectors (angle.py:98-242)
+struct v4 {
+  double t, x, y, z;
+};
+TPA_DEV v3 lv_vect(v4 p) { return v3{p.x, p.y, p.z}; }
+TPA_DEV v4 lv_add(v4 a, v4 b) { return v4{a.t + b.t, a.x + b.x, a.y + b.y, a.z + b.z}; }
+TPA_DEV double lv_M(v4 p) {
+  double m2 = p.t * p.t - p.x * p.x - p.y * p.y - p.z * p.z;
+  return sqrt(fabs(m2));
+}
+// LorentzVector.boost (angle.py:148-163)
+TPA_DEV v4 lv_boost(v4 p, v3 pb) {
+  double beta2 = v_dot(pb, pb);
+  double gamma = 1.0 / sqrt(1.0 - beta2);
+  double bp = v_dot(pb, lv_vect(p));
+  double gamma2 = beta2 > 1.0e-14 ? (gamma - 1.0) / beta2 : 0.0;
+  double a = gamma2 * bp;
+  double b = gamma * p.t;
+  v4 r;
+  r.x = (p.x + a * pb.x) + b * pb.x;
+  r.y = (p.y + a * pb.y) + b * pb.y;
+  r.z = (p.z + a * pb.z) + b * pb.z;
+  r.t = gamma * (p.t + bp);
+  return r;
+}
+// LorentzVector.rest_vector (angle.py:140-146): `other` in the rest frame of `frame`.
+TPA_DEV v4 lv_rest(v4 frame, v4 other) {
+  v3 pb = v3{-(frame.x / frame.t), -(frame.y / frame.t), -(frame.z / frame.t)};
+  return lv_boost(other, pb);
+}
+// LorentzVector.omega (angle.py:198-213)
+TPA_DEV double lv_omega(v4 p) {
+  v3 pb = v3{p.x / p.t, p.y / p.t, p.z / p.t};
+  double beta2 = v_dot(pb, pb);
+  beta2 = beta2 < 1.0 ? beta2 : 0.0;
+  double gamma = 1.0 / sqrt(1.0 - beta2);
+  return acosh(gamma);
+}
+
+// EulerAngle.angle_zx_z_getx (angle.py:285-308)
+TPA_DEV void angle_zx_z_getx(v3 z1, v3 x1, v3 z2, double* alpha, double* beta, v3* x2) {
+  v3 u_z1 = v_unit(z1);
+  v3 u_z2 = v_unit(z2);
+  v3 u_y1 = v_cross_unit(z1, x1);
+  v3 u_x1 = v_cross_unit(u_y1, z1);
+  v3 u_yr = v_cross_unit(z1, z2);
+  v3 u_xr = v_cross_unit(u_yr, z1);
+  *alpha = v_angle_from(u_xr, u_x1, u_y1);
+  *beta = v_angle_from(u_z2, u_z1, u_xr);
+  *x2 = v_cross_unit(u_yr, u_z2);
+}
+
+// Python float modulo (floor mod) as used by the alpha folding of cal_angle.py:291-303
+TPA_DEV double fmod_floor(double a, double b) {
+  double r = fmod(a, b);
+  if (r != 0.0 && ((r < 0.0) != (b < 0.0))) r += b;
+  return r;
+}
+
+// ------------------------------------------------------------ SU(2) (angle.py:329-392)
+struct su2 {
+  cd a, b, c, d;  // [[a, b], [c, d]]
+};
+TPA_DEV su2 su2_mul(su2 x, su2 y) {
+  su2 r;
+  r.a = c_add(c_mul(x.a, y.a), c_mul(x.b, y.c));
+  r.b = c_add(c_mul(x.a, y.b), c_mul(x.b, y.d));
+  r.c = c_add(c_mul(x.c, y.a), c_mul(x.d, y.c));
+  r.d = c_add(c_mul(x.c, y.b), c_mul(x.d, y.d));
+  return r;
+}
+TPA_DEV su2 su2_inv(su2 x) { return su2{x.d, c_neg(x.b), c_neg(x.c), x.a}; }
+TPA_DEV su2 su2_boost_z(double omega) {
+  cd a = cd{exp(omega / 2.0), 0.0};
+  return su2{c_inv(a), cd{0, 0}, cd{0, 0}, a};
+}
+TPA_DEV su2 su2_rot_z(double alpha) {
+  cd a = c_expi(alpha / 2.0);
+  return su2{c_inv(a), cd{0, 0}, cd{0, 0}, a};
+}
+TPA_DEV su2 su2_rot_y(double beta) {
+  double s = sin(beta / 2.0), c = cos(beta / 2.0);
+  return su2{cd{c, 0}, cd{-s, 0}, cd{s, 0}, cd{c, 0}};
+}
+// SU2M.get_euler_angle (angle.py:374-392)
+TPA_DEV void su2_euler(su2 x, double* alpha, double* beta, double* gamma) {
+  double cb = c_add(c_mul(x.a, x.d), c_mul(x.b, x.c)).r;
+  cb = fmin(fmax(cb, -1.0), 1.0);
+  *beta = acos(cb);
+  double apg = atan2(x.d.i, x.d.r);
+  double amg = -atan2(x.c.i, x.c.r);
+  *alpha = apg + amg;
+  *gamma = apg - amg;
+}
+
+// ------------------------------------------------------ Wigner d (dfun.py:118-218)
+// d^{j/2}_{m/2, n/2}(beta) = sum_l w[l][m][n] sin^l(beta/2) cos^{j-l}(beta/2) with the weight
+// table of small_d_weight; `w` points at the (j+1)^3 table of this j.
+TPA_DEV double wigner_d(const double* __restrict__ w, int j, int m2, int n2, const double* sp,
+                        const double* cp) {
+  int im = (m2 + j) / 2, in = (n2 + j) / 2;
+  double acc = 0.0;
+  for (int l = 0; l <= j; ++l) acc += w[(l * (j + 1) + im) * (j + 1) + in] * (sp[l] * cp[j - l]);
+  return acc;
+}
+TPA_DEV void half_angle_powers(double beta, int j, double* sp, double* cp) {
+  double s = sin(0.5 * beta), c = cos(0.5 * beta);
+  sp[0] = 1.0;
+  cp[0] = 1.0;
+  for (int l = 1; l <= j; ++l) {
+    sp[l] = sp[l - 1] * s;
+    cp[l] = cp[l - 1] * c;
+  }
+}
+TPA_DEV int dweight_offset(int j) {
+  int off = 0;
+  for (int k = 0; k < j; ++k) off += (k + 1) * (k + 1) * (k + 1);
+  return off;
+}
+
+// ------------------------------------- barrier factors and lineshapes (breit_wigner.py)
+// Bprime_polynomial (breit_wigner.py:332-355): |theta_l(i w)|^2 coefficients, highest first.
+TPA_DEV double bprime_poly(int l, double z) {
+  switch (l) {
+    case 0: return 1.0;
+    case 1: return z + 1.0;
+    case 2: return (z + 3.0) * z + 9.0;
+    case 3: return ((z + 6.0) * z + 45.0) * z + 225.0;
+    case 4: return (((z + 10.0) * z + 135.0) * z + 1575.0) * z + 11025.0;
+    default: return ((((z + 15.0) * z + 315.0) * z + 6300.0) * z + 99225.0) * z + 893025.0;
+  }
+}
+TPA_DEV double bprime_poly_deriv(int l, double z) {
+  switch (l) {
+    case 0: return 0.0;
+    case 1: return 1.0;
+    case 2: return 2.0 * z + 3.0;
+    case 3: return (3.0 * z + 12.0) * z + 45.0;
+    case 4: return ((4.0 * z + 30.0) * z + 270.0) * z + 1575.0;
+    default: return (((5.0 * z + 60.0) * z + 945.0) * z + 12600.0) * z + 99225.0;
+  }
+}
+// get_relative_p2 (amp/core.py:324-334)
+TPA_DEV double rel_p2(double m0, double m1, double m2) {
+  double s = m1 + m2, d = m1 - m2;
+  double p = (m0 - s) * (m0 + s) * (m0 - d) * (m0 + d);
+  double t = 2.0 * m0;
+  return p / (t * t);
+}
+// get_relative_p (amp/core.py:311-321), m_eff = max(m0, m1 + m2)
+TPA_DEV double rel_p(double m0, double m1, double m2) {
+  double s = m1 + m2, d = m1 - m2;
+  double me = m0 > s ? m0 : s;
+  double p = (me - s) * (me + s) * (me - d) * (me + d);
+  return sqrt(p) / (2.0 * me);
+}
+
+// BWR (breit_wigner.py:74-91) with Gamma (231-245) and Bprime (290-306); BW (56-71).
+TPA_DEV cd lineshape(int shape, double m, double m0, double g0, double q, double q0, int L,
+                     double d) {
+  if (shape == 0) return cd{1.0, 0.0};
+  double gamma = g0;
+  if (shape == 2) {
+    double qq0 = q0 > 1.0e-15 ? pow(q / q0, (double)(2 * L + 1)) : 1.0;
+    double mm0 = m0 / m;
+    double qd0 = q0 * d, qd = q * d;
+    double bp = sqrt(bprime_poly(L, qd0 * qd0)) / sqrt(bprime_poly(L, qd * qd));
+    gamma = g0 * qq0 * mm0 * (bp * bp);
+  }
+  double x = m0 * m0 - m * m;
+  double y = m0 * gamma;
+  double s = x * x + y * y;
+  return cd{x / s, y / s};
+}

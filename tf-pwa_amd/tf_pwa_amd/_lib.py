@@ -1,0 +1,148 @@
+"""ctypes binding of libtfpwa_amd.so (the C ABI declared in include/tfpwa_amd.h).
+
+The library is built in-tree by ``tf-pwa_amd/build.py`` into ``tf-pwa_amd/lib``. There is no
+fallback: if the library is missing or fails to load, every engine call raises.
+"""
+
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(os.path.dirname(_HERE), "lib", "libtfpwa_amd.so")
+
+MAX_FINAL = 3
+MAX_TOPO = 3
+MAX_CHAIN = 32
+MAX_LS = 8
+MAX_HEL = 8
+MAX_J2 = 7
+MAX_P = 128
+MAX_LAM = 64
+NF_TOPO = 18
+
+MODE_DATA = 0
+MODE_MC = 1
+FLAG_GRAD = 1
+FLAG_FOUT = 2
+
+_i32 = ctypes.c_int32
+
+
+class TpaTopo(ctypes.Structure):
+    _fields_ = [
+        ("x", _i32), ("y", _i32), ("z", _i32),
+        ("r_first", _i32),
+        ("aligned", _i32 * MAX_FINAL),
+    ]
+
+
+class TpaChain(ctypes.Structure):
+    _fields_ = [
+        ("topo", _i32),
+        ("top_r_first", _i32),
+        ("r_y_first", _i32),
+        ("jr2", _i32),
+        ("n_r", _i32),
+        ("hel_r", _i32 * MAX_HEL),
+        ("n_ls1", _i32), ("n_ls2", _i32),
+        ("l1", _i32 * MAX_LS), ("l2", _i32 * MAX_LS),
+        ("act1", _i32 * MAX_LS), ("act2", _i32 * MAX_LS),
+        ("k0", _i32),
+        ("shape", _i32),
+        ("bw_l", _i32),
+        ("cg1_off", _i32), ("cg2_off", _i32),
+        ("m0", ctypes.c_double), ("g0", ctypes.c_double),
+        ("m_y", ctypes.c_double), ("m_z", ctypes.c_double),
+        ("d", ctypes.c_double),
+    ]
+
+
+class TpaModel(ctypes.Structure):
+    _fields_ = [
+        ("n_final", _i32), ("n_topo", _i32), ("n_chain", _i32), ("P", _i32), ("lam", _i32),
+        ("lam_pad", _i32),
+        ("top_j2", _i32), ("n_top", _i32),
+        ("hel_top", _i32 * MAX_HEL),
+        ("final_j2", _i32 * MAX_FINAL),
+        ("n_fh", _i32 * MAX_FINAL),
+        ("hel_f", (_i32 * MAX_HEL) * MAX_FINAL),
+        ("ref_topo", _i32 * MAX_FINAL),
+        ("random_z", _i32),
+        ("n_cg", _i32),
+        ("topo", TpaTopo * MAX_TOPO),
+        ("chain", TpaChain * MAX_CHAIN),
+    ]
+
+
+class EngineError(RuntimeError):
+    pass
+
+
+_lib = None
+
+
+def lib():
+    """Load the engine library once; raise if it is not built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise EngineError(
+            "libtfpwa_amd.so is not built ({}); run `python tf-pwa_amd/build.py` or "
+            "`__graft_entry__.build()`".format(LIB_PATH))
+    L = ctypes.CDLL(LIB_PATH)
+    vp, dp, i64 = ctypes.c_void_p, ctypes.POINTER(ctypes.c_double), ctypes.c_int64
+    sig = {
+        "tpa_abi_version": ([], ctypes.c_int),
+        "tpa_last_error": ([], ctypes.c_char_p),
+        "tpa_device_count": ([ctypes.POINTER(ctypes.c_int)], ctypes.c_int),
+        "tpa_model_create": ([vp, i64, dp, ctypes.c_int, ctypes.POINTER(vp)], ctypes.c_int),
+        "tpa_model_destroy": ([vp], ctypes.c_int),
+        "tpa_model_set_shape": ([vp, dp], ctypes.c_int),
+        "tpa_events_create": ([vp, dp, i64, dp, ctypes.POINTER(vp)], ctypes.c_int),
+        "tpa_events_destroy": ([vp], ctypes.c_int),
+        "tpa_events_size": ([vp], i64),
+        "tpa_events_set_weight": ([vp, dp], ctypes.c_int),
+        "tpa_events_get_angles": ([vp, dp, i64], ctypes.c_int),
+        "tpa_events_build": ([vp, ctypes.c_int], ctypes.c_int),
+        "tpa_events_get_pw": ([vp, dp, i64], ctypes.c_int),
+        "tpa_eval": ([vp, dp, ctypes.c_int, ctypes.c_int, dp, dp], ctypes.c_int),
+        "tpa_timing_enable": ([ctypes.c_int], ctypes.c_int),
+        "tpa_timing_read": ([dp, ctypes.POINTER(i64)], ctypes.c_int),
+        "tpa_timing_reset": ([], ctypes.c_int),
+    }
+    for name, (args, res) in sig.items():
+        fn = getattr(L, name)
+        fn.argtypes = args
+        fn.restype = res
+    if L.tpa_abi_version() != 1:
+        raise EngineError("unexpected ABI version {}".format(L.tpa_abi_version()))
+    _lib = L
+    return L
+
+
+def check(rc):
+    if rc != 0:
+        msg = lib().tpa_last_error().decode(errors="replace")
+        raise EngineError("engine call failed ({}): {}".format(rc, msg))
+
+
+def dptr(a):
+    """Pointer to a C-contiguous float64 numpy array (or None)."""
+    if a is None:
+        return None
+    assert a.dtype == np.float64 and a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def exported_symbols():
+    """Names declared in include/tfpwa_amd.h that the loaded library must export."""
+    return [
+        "tpa_abi_version", "tpa_last_error", "tpa_device_count", "tpa_model_create",
+        "tpa_model_destroy", "tpa_model_set_shape", "tpa_events_create", "tpa_events_destroy",
+        "tpa_events_size", "tpa_events_set_weight", "tpa_events_get_angles",
+        "tpa_events_build", "tpa_events_get_pw", "tpa_eval", "tpa_timing_enable",
+        "tpa_timing_read", "tpa_timing_reset",
+    ]
