@@ -80,9 +80,10 @@ int tpa_events_get_angles(tpa_events_t ev, double* out, int64_t out_len);
  * (tf_pwa/amp/core.py:931-1164, 1392-1445). */
 int tpa_events_build(tpa_events_t ev, int fold);
 
-/* Update the lineshape parameters (m0, g0) of every chain in the device description.
- * shape: host [2*n_chain]. Takes effect at the next tpa_events_build(fold=1). */
-int tpa_model_set_shape(tpa_model_t model, const double* shape);
+/* Replace the model description (same chains and CG table; new lineshape parameters or
+ * barrier clamp flags). Event sets of the model must be rebuilt (tpa_events_build) before
+ * their next evaluation. */
+int tpa_model_update(tpa_model_t model, const void* desc, int64_t desc_bytes);
 
 /* Copy the partial-wave cache to the host: out [P][n][lam_pad] complex (interleaved). */
 int tpa_events_get_pw(tpa_events_t ev, double* out, int64_t out_len);

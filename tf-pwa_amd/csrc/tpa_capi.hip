@@ -246,12 +246,17 @@ int tpa_model_destroy(tpa_model_t m) {
   return TPA_OK;
 }
 
-int tpa_model_set_shape(tpa_model_t m, const double* shape) {
-  if (!m || !shape) return fail(TPA_ERR_ARG, "null pointer");
-  for (int c = 0; c < m->h.n_chain; ++c) {
-    m->h.chain[c].m0 = shape[2 * c];
-    m->h.chain[c].g0 = shape[2 * c + 1];
-  }
+int tpa_model_update(tpa_model_t m, const void* desc, int64_t desc_bytes) {
+  if (!m || !desc) return fail(TPA_ERR_ARG, "null pointer");
+  if (desc_bytes != (int64_t)sizeof(TpaModel))
+    return fail(TPA_ERR_ARG, "model description size mismatch");
+  TpaModel next;
+  std::memcpy(&next, desc, sizeof(TpaModel));
+  if (next.n_cg != m->h.n_cg) return fail(TPA_ERR_ARG, "CG table size cannot change");
+  std::string why = check_model(next, next.n_cg);
+  if (!why.empty()) return fail(TPA_ERR_ARG, "invalid model: " + why);
+  HIPCHK(hipStreamSynchronize(m->stream));
+  m->h = next;
   HIPCHK(hipMemcpy(m->d, &m->h, sizeof(TpaModel), hipMemcpyHostToDevice));
   return TPA_OK;
 }

@@ -100,7 +100,7 @@ def lib():
         "tpa_device_count": ([ctypes.POINTER(ctypes.c_int)], ctypes.c_int),
         "tpa_model_create": ([vp, i64, dp, ctypes.c_int, ctypes.POINTER(vp)], ctypes.c_int),
         "tpa_model_destroy": ([vp], ctypes.c_int),
-        "tpa_model_set_shape": ([vp, dp], ctypes.c_int),
+        "tpa_model_update": ([vp, vp, i64], ctypes.c_int),
         "tpa_events_create": ([vp, dp, i64, dp, ctypes.POINTER(vp)], ctypes.c_int),
         "tpa_events_destroy": ([vp], ctypes.c_int),
         "tpa_events_size": ([vp], i64),
@@ -141,7 +141,7 @@ def exported_symbols():
     """Names declared in include/tfpwa_amd.h that the loaded library must export."""
     return [
         "tpa_abi_version", "tpa_last_error", "tpa_device_count", "tpa_model_create",
-        "tpa_model_destroy", "tpa_model_set_shape", "tpa_events_create", "tpa_events_destroy",
+        "tpa_model_destroy", "tpa_model_update", "tpa_events_create", "tpa_events_destroy",
         "tpa_events_size", "tpa_events_set_weight", "tpa_events_get_angles",
         "tpa_events_build", "tpa_events_get_pw", "tpa_eval", "tpa_timing_enable",
         "tpa_timing_read", "tpa_timing_reset",

@@ -1,0 +1,341 @@
+"""Amplitude model: the config-built decay group, its parameters, and its lowering to the
+engine's model description.
+
+The amplitude of tf-pwa (tf_pwa/amp/core.py:1392-1445, 1696-1728, 1919-1931) is
+
+    A_lambda(x) = sum_chains total_c * BW_c(m) * sum_{ls1, ls2} g1_ls1 g2_ls2 *
+                  [q^l B'_l CG D]_vertex1 [q^l B'_l CG D]_vertex2 (contracted over the
+                  resonance helicity, rotated by the aligned D of spin>0 finals).
+
+The engine stores, per event, the parameter-free partial-wave amplitudes h_k,lambda with
+k = (chain, ls1, ls2) and evaluates A_lambda = sum_k c_k h_k,lambda with host coefficients
+c_k = total_c g1_ls1 g2_ls2 sqrt(P_l1(z0_1)) sqrt(P_l2(z0_2)). The barrier factor
+q^l sqrt(P_l(z0)/P_l(z)) of Bprime_q2 (tf_pwa/breit_wigner.py:268-277) splits into the
+per-event q^l / sqrt(P_l(z)) and the parameter constant sqrt(P_l(z0)); when P_l(z0) <= 0 the
+reference clamps the ratio to 1 and the per-event factor becomes q^l (flag ``act`` = 0).
+"""
+
+import ctypes
+
+import numpy as np
+
+from . import _lib
+from .particle import ls_couplings  # noqa: F401  (re-exported for callers)
+
+
+def rel_p2(m0, m1, m2):
+    """get_relative_p2 (tf_pwa/amp/core.py:324-334)."""
+    s, d = m1 + m2, m1 - m2
+    return (m0 - s) * (m0 + s) * (m0 - d) * (m0 + d) / (2 * m0) ** 2
+
+
+def rel_p(m0, m1, m2):
+    """get_relative_p (tf_pwa/amp/core.py:311-321) with m_eff = max(m0, m1+m2)."""
+    s, d = m1 + m2, m1 - m2
+    me = m0 if m0 > s else s
+    return np.sqrt((me - s) * (me + s) * (me - d) * (me + d)) / (2 * me)
+
+
+_BPRIME = {0: [1.0], 1: [1.0, 1.0], 2: [1.0, 3.0, 9.0], 3: [1.0, 6.0, 45.0, 225.0],
+           4: [1.0, 10.0, 135.0, 1575.0, 11025.0],
+           5: [1.0, 15.0, 315.0, 6300.0, 99225.0, 893025.0]}
+
+
+def bprime_poly(l, z):
+    """|theta_l(i w)|^2 of the reverse Bessel polynomial (tf_pwa/breit_wigner.py:332-355)."""
+    r = 0.0
+    for c in _BPRIME[l]:
+        r = r * z + c
+    return r
+
+
+def bprime_poly_deriv(l, z):
+    cof = _BPRIME[l]
+    n = len(cof) - 1
+    r = 0.0
+    for i, c in enumerate(cof[:-1]):
+        r = r * z + c * (n - i)
+    return r
+
+
+class PartialWave:
+    __slots__ = ("chain", "ls1", "ls2")
+
+    def __init__(self, chain, ls1, ls2):
+        self.chain, self.ls1, self.ls2 = chain, ls1, ls2
+
+
+class AmplitudeModel:
+    """A decay group plus its VarsManager, lowered for the engine (three-body finals)."""
+
+    def __init__(self, decay_group, vm, random_z=True):
+        self.decay_group = decay_group
+        self.vm = vm
+        self.random_z = bool(random_z)
+        self._dev = None
+        self._lower()
+
+    # ---------------------------------------------------------------- lowering
+    def _lower(self):
+        g = self.decay_group
+        if len(g.outs) != 3:
+            raise NotImplementedError("the engine supports three-body final states")
+        self.finals = list(g.outs)
+        fidx = {p.name: i for i, p in enumerate(self.finals)}
+        reps = g.topologies()
+        self.reps = reps
+        topo = []
+        for rep in reps:
+            d1 = rep.decay_of(g.top)
+            R = [o for o in d1.outs if o in rep.inner]
+            if len(R) != 1 or len(rep) != 2:
+                raise NotImplementedError("only A -> R X, R -> Y Z chains are supported")
+            R = R[0]
+            X = d1.outs[1] if d1.outs[0] is R else d1.outs[0]
+            d2 = rep.decay_of(R)
+            topo.append(dict(x=fidx[X.name], y=fidx[d2.outs[0].name], z=fidx[d2.outs[1].name],
+                             r_first=int(d1.outs[0] is R)))
+        ref_topo = []
+        for f in range(3):
+            cand = [t for t, T in enumerate(topo) if T["x"] == f]
+            ref_topo.append(cand[0] if cand else 0)
+        self.topo = topo
+        self.ref_topo = ref_topo
+        chains = []
+        cg = []
+        k0 = 0
+        waves = []
+        for ci, c in enumerate(g.chains):
+            key = c.topology_key()
+            t = [i for i, r in enumerate(reps) if r.topology_key() == key][0]
+            d1 = c.decay_of(g.top)
+            R = [o for o in d1.outs if o in c.inner][0]
+            d2 = c.decay_of(R)
+            if R.width is None:
+                shape = 0
+            elif R.model in ("BW",):
+                shape = 1
+            else:
+                shape = 2
+            cg1 = d1.cg_matrix()
+            cg2 = d2.cg_matrix()
+            ch = dict(topo=t, top_r_first=int(d1.outs[0] is R),
+                      r_y_first=int(fidx[d2.outs[0].name] == topo[t]["y"]),
+                      R=R, d1=d1, d2=d2, k0=k0, shape=shape,
+                      bw_l=min(R.decays[0].l_list) if R.decays else 0,
+                      cg1_off=len(cg), cg2_off=len(cg) + cg1.size)
+            cg.extend(cg1.ravel().tolist())
+            cg.extend(cg2.ravel().tolist())
+            for s1 in range(len(d1.ls)):
+                for s2 in range(len(d2.ls)):
+                    waves.append(PartialWave(ci, s1, s2))
+            k0 += len(d1.ls) * len(d2.ls)
+            chains.append(ch)
+        self.chains = chains
+        self.waves = waves
+        self.P = k0
+        self.cg = np.array(cg, dtype=np.float64)
+        lam = len(g.top.spins)
+        for p in self.finals:
+            lam *= len(p.spins)
+        self.lam = lam
+        lp = 1
+        while lp < lam:
+            lp <<= 1
+        self.lam_pad = lp
+
+    def _names(self):
+        """Variable names of every chain: total, the g_ls of both vertices, mass and width."""
+        out = []
+        for c, ch in zip(self.decay_group.chains, self.chains):
+            out.append(dict(total=c.params_head() + "_total_0",
+                            g1=[ch["d1"].params_head() + "_g_ls_{}".format(i)
+                                for i in range(len(ch["d1"].ls))],
+                            g2=[ch["d2"].params_head() + "_g_ls_{}".format(i)
+                                for i in range(len(ch["d2"].ls))],
+                            mass=ch["R"].name + "_mass", width=ch["R"].name + "_width"))
+        return out
+
+    def shape_params(self):
+        """(m0, g0) of every chain's resonance from the parameter store."""
+        vm = self.vm
+        out = []
+        for ch in self.chains:
+            R = ch["R"]
+            m0 = vm.variables.get(R.name + "_mass", R.mass)
+            g0 = vm.variables.get(R.name + "_width", R.width)
+            out.append((float(m0), float(g0) if g0 is not None else 0.0))
+        return out
+
+    def barrier_constants(self):
+        """Per chain: sqrt(P_l(z0)) (or 1 when clamped) and the act flags of both vertices."""
+        top = self.decay_group.top
+        out = []
+        for ch, (m0, g0) in zip(self.chains, self.shape_params()):
+            d1, d2 = ch["d1"], ch["d2"]
+
+            def pm(p):
+                return m0 if p is ch["R"] else p.mass
+
+            d = d1.d
+            q01 = rel_p2(top.mass, pm(d1.outs[0]), pm(d1.outs[1]))
+            q02 = rel_p2(m0, d2.outs[0].mass, d2.outs[1].mass)
+            c1, a1, c2, a2 = [], [], [], []
+            for l in d1.l_list:
+                p0 = bprime_poly(l, q01 * d * d)
+                a1.append(int(p0 > 0))
+                c1.append(np.sqrt(p0) if p0 > 0 else 1.0)
+            for l in d2.l_list:
+                p0 = bprime_poly(l, q02 * d * d)
+                a2.append(int(p0 > 0))
+                c2.append(np.sqrt(p0) if p0 > 0 else 1.0)
+            out.append(dict(c1=np.array(c1), c2=np.array(c2), act1=a1, act2=a2,
+                            q01=q01, q02=q02))
+        return out
+
+    def describe(self, fold=True):
+        """The TpaModel structure for the current parameter values."""
+        M = _lib.TpaModel()
+        g = self.decay_group
+        M.n_final = 3
+        M.n_topo = len(self.topo)
+        M.n_chain = len(self.chains)
+        M.P = self.P
+        M.lam = self.lam
+        M.lam_pad = self.lam_pad
+        M.top_j2 = int(round(2 * g.top.J))
+        M.n_top = len(g.top.spins)
+        for i, s in enumerate(g.top.spins):
+            M.hel_top[i] = int(round(2 * s))
+        for f, p in enumerate(self.finals):
+            M.final_j2[f] = int(round(2 * p.J))
+            M.n_fh[f] = len(p.spins)
+            for i, s in enumerate(p.spins):
+                M.hel_f[f][i] = int(round(2 * s))
+            M.ref_topo[f] = self.ref_topo[f]
+        M.random_z = int(self.random_z)
+        M.n_cg = len(self.cg)
+        for t, T in enumerate(self.topo):
+            M.topo[t].x, M.topo[t].y, M.topo[t].z = T["x"], T["y"], T["z"]
+            M.topo[t].r_first = T["r_first"]
+            for f in range(3):
+                M.topo[t].aligned[f] = int(self.ref_topo[f] != t)
+        bc = self.barrier_constants()
+        for c, (ch, (m0, g0), b) in enumerate(zip(self.chains, self.shape_params(), bc)):
+            C = M.chain[c]
+            R = ch["R"]
+            C.topo = ch["topo"]
+            C.top_r_first = ch["top_r_first"]
+            C.r_y_first = ch["r_y_first"]
+            C.jr2 = int(round(2 * R.J))
+            C.n_r = len(R.spins)
+            for i, s in enumerate(R.spins):
+                C.hel_r[i] = int(round(2 * s))
+            C.n_ls1, C.n_ls2 = len(ch["d1"].ls), len(ch["d2"].ls)
+            for i, l in enumerate(ch["d1"].l_list):
+                C.l1[i] = l
+                C.act1[i] = b["act1"][i]
+            for i, l in enumerate(ch["d2"].l_list):
+                C.l2[i] = l
+                C.act2[i] = b["act2"][i]
+            C.k0 = ch["k0"]
+            C.shape = ch["shape"]
+            C.bw_l = ch["bw_l"]
+            C.cg1_off, C.cg2_off = ch["cg1_off"], ch["cg2_off"]
+            C.m0, C.g0 = m0, g0
+            C.m_y, C.m_z = ch["d2"].outs[0].mass, ch["d2"].outs[1].mass
+            C.d = ch["d1"].d
+        return M
+
+    def build_state(self, fold=True):
+        """Everything the partial-wave cache depends on besides the events."""
+        bc = self.barrier_constants()
+        key = tuple((tuple(b["act1"]), tuple(b["act2"])) for b in bc)
+        if fold:
+            key = key + tuple(self.shape_params())
+        return key
+
+    def device(self):
+        """The engine model handle (one per amplitude model); its description is refreshed
+        when the lineshape parameters or barrier clamp flags change."""
+        state = self.build_state()
+        L = _lib.lib()
+        if self._dev is None:
+            M = self.describe()
+            h = ctypes.c_void_p()
+            _lib.check(L.tpa_model_create(ctypes.byref(M), ctypes.sizeof(M),
+                                          _lib.dptr(self.cg), len(self.cg), ctypes.byref(h)))
+            self._dev = h
+            self._dev_state = state
+        elif self._dev_state != state:
+            M = self.describe()
+            _lib.check(L.tpa_model_update(self._dev, ctypes.byref(M), ctypes.sizeof(M)))
+            self._dev_state = state
+        return self._dev
+
+    def __del__(self):
+        h = getattr(self, "_dev", None)
+        if h is not None and _lib._lib is not None:
+            _lib._lib.tpa_model_destroy(h)
+            self._dev = None
+
+    # ---------------------------------------------------------------- coefficients
+    def coefficients(self):
+        """c_k for the current parameters (complex array of length P)."""
+        vm = self.vm
+        names = self._names()
+        bc = self.barrier_constants()
+        c = np.zeros(self.P, dtype=np.complex128)
+        for w_i, w in enumerate(self.waves):
+            nm, b = names[w.chain], bc[w.chain]
+            c[w_i] = (vm.complex_value(nm["total"]) * vm.complex_value(nm["g1"][w.ls1])
+                      * vm.complex_value(nm["g2"][w.ls2]) * b["c1"][w.ls1] * b["c2"][w.ls2])
+        return c
+
+    def jacobian(self):
+        """J[i, k] = d c_k / d trainable_vars[i] (complex) for the coupling parameters.
+        Raises if a lineshape parameter is trainable (needs the shape-gradient path)."""
+        vm = self.vm
+        index = {n: i for i, n in enumerate(vm.trainable_vars)}
+        names = self._names()
+        for nm in names:
+            for key in ("mass", "width"):
+                if nm[key] in index:
+                    raise NotImplementedError(
+                        "gradients with respect to floating masses/widths use the shape path")
+        bc = self.barrier_constants()
+        J = np.zeros((len(vm.trainable_vars), self.P), dtype=np.complex128)
+        for w_i, w in enumerate(self.waves):
+            nm, b = names[w.chain], bc[w.chain]
+            facs = [nm["total"], nm["g1"][w.ls1], nm["g2"][w.ls2]]
+            vals = [vm.complex_value(f) for f in facs]
+            const = b["c1"][w.ls1] * b["c2"][w.ls2]
+            for j, f in enumerate(facs):
+                rest = const
+                for jj, v in enumerate(vals):
+                    if jj != j:
+                        rest = rest * v
+                dr, di = vm.complex_derivs(f)
+                if f + "r" in index:
+                    J[index[f + "r"], w_i] += dr * rest
+                if f + "i" in index:
+                    J[index[f + "i"], w_i] += di * rest
+        return J
+
+    # ---------------------------------------------------------------- parameters
+    def set_params(self, var):
+        if isinstance(var, dict):
+            self.vm.set_all(var)
+        else:
+            self.vm.set_all(var)
+
+    def get_params(self, trainable_only=False):
+        return self.vm.get_all_dic(trainable_only)
+
+    @property
+    def trainable_variables(self):
+        return list(self.vm.trainable_vars)
+
+    def __call__(self, events):
+        """Per-event f = sum_lambda |A_lambda|^2 (AbsPDF.__call__, tf_pwa/amp/amp.py:101-110)."""
+        return events.amp2(self)

@@ -1,0 +1,107 @@
+"""Device-resident event sets.
+
+An EventSet owns one data or MC sample on the GPU: the four-momenta are uploaded once, the
+helicity and aligned angles are computed on the device (the role of
+tf_pwa/cal_angle.py:762-865 for FCN's data and MC, tf_pwa/model/model.py:1126-1164), and the
+partial-wave cache is (re)built on the device whenever the amplitude model's parameter-free
+state changes. Likelihood passes are single kernel launches over the whole set.
+"""
+
+import ctypes
+
+import numpy as np
+
+from . import _lib
+
+
+def p4_array(p4, finals, order=None):
+    """Stack four-momenta into the engine layout [f*4 + c][n] with finals in ``finals``
+    order. ``p4``: dict name -> (n, 4), or an (n, n_final, 4) / (n*n_final, 4) array in
+    ``order`` (the dat_order of the config, like tf_pwa/data.py:300-364)."""
+    names = [p.name for p in finals]
+    if isinstance(p4, dict):
+        arrs = [np.asarray(p4[n], dtype=np.float64) for n in names]
+    else:
+        a = np.asarray(p4, dtype=np.float64)
+        order = order or names
+        a = a.reshape(-1, len(order), 4)
+        arrs = [a[:, order.index(n), :] for n in names]
+    n = arrs[0].shape[0]
+    out = np.empty((len(names) * 4, n), dtype=np.float64)
+    for f, a in enumerate(arrs):
+        out[4 * f:4 * f + 4] = a.T
+    return out
+
+
+class EventSet:
+    """One sample on the device. ``amp`` is the AmplitudeModel whose description the
+    kernels use; ``weight`` defaults to 1."""
+
+    def __init__(self, amp, p4, weight=None, order=None):
+        self.amp = amp
+        arr = p4_array(p4, amp.finals, order)
+        self.n = arr.shape[1]
+        self.weight = np.ones(self.n) if weight is None else np.ascontiguousarray(
+            weight, dtype=np.float64).copy()
+        if self.weight.shape != (self.n,):
+            raise ValueError("weight must have one entry per event")
+        L = _lib.lib()
+        h = ctypes.c_void_p()
+        _lib.check(L.tpa_events_create(amp.device(), _lib.dptr(np.ascontiguousarray(arr)),
+                                       self.n, _lib.dptr(self.weight), ctypes.byref(h)))
+        self._h = h
+        self._dev_weight = self.weight.copy()
+        self._state = None
+
+    def __len__(self):
+        return self.n
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and _lib._lib is not None:
+            _lib._lib.tpa_events_destroy(h)
+            self._h = None
+
+    def set_weight(self, w):
+        w = np.ascontiguousarray(w, dtype=np.float64)
+        if w.shape != (self.n,):
+            raise ValueError("weight must have one entry per event")
+        if not np.array_equal(w, self._dev_weight):
+            _lib.check(_lib.lib().tpa_events_set_weight(self._h, _lib.dptr(w)))
+            self._dev_weight = w.copy()
+
+    def angles(self):
+        nf = _lib.NF_TOPO * len(self.amp.topo) + 4
+        out = np.empty((nf, self.n), dtype=np.float64)
+        _lib.check(_lib.lib().tpa_events_get_angles(self._h, _lib.dptr(out), out.size))
+        return out
+
+    def ensure_built(self):
+        self.amp.device()
+        if self._state == self.amp._dev_state:
+            return
+        _lib.check(_lib.lib().tpa_events_build(self._h, 1))
+        self._state = self.amp._dev_state
+
+    def partial_waves(self):
+        self.ensure_built()
+        out = np.empty(2 * self.amp.P * self.n * self.amp.lam_pad, dtype=np.float64)
+        _lib.check(_lib.lib().tpa_events_get_pw(self._h, _lib.dptr(out), out.size))
+        return out.view(np.complex128).reshape(self.amp.P, self.n, self.amp.lam_pad)
+
+    def eval(self, coef, mode, grad=False, fout=False):
+        """One pass: (value, G or None, f or None)."""
+        self.ensure_built()
+        P = self.amp.P
+        c = np.ascontiguousarray(np.asarray(coef, dtype=np.complex128)).view(np.float64)
+        out = np.zeros(2 + 2 * P, dtype=np.float64)
+        f = np.empty(self.n, dtype=np.float64) if fout else None
+        flags = (_lib.FLAG_GRAD if grad else 0) | (_lib.FLAG_FOUT if fout else 0)
+        _lib.check(_lib.lib().tpa_eval(self._h, _lib.dptr(c), mode, flags, _lib.dptr(out),
+                                       _lib.dptr(f)))
+        G = out[2:].view(np.complex128).copy() if grad else None
+        return out[0], G, f
+
+    def amp2(self, amp=None):
+        _, _, f = self.eval(self.amp.coefficients(), _lib.MODE_MC, fout=True)
+        return f

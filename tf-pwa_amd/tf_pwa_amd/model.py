@@ -1,0 +1,285 @@
+"""Likelihood surface of tf_pwa.model (Model, FCN) on top of the engine.
+
+NLL = -sum_i w_i clip_log(f_i) + (sum_i w_i) log(sum_j w~_j f_j) and its gradient
+(tf_pwa/model/model.py:407-449): data weights carry the alpha = sum w / sum w^2 rescale
+(get_weight_data, 603-641) and MC weights are normalised to unit sum (FCN.__init__,
+1126-1164). Each sum is one engine pass per event set; the gradient with respect to the
+trainable variables is assembled on the host from the engine's adjoint sums G_k through
+the coupling Jacobian: d sum / d theta = 2 Re sum_k (d c_k / d theta) G_k.
+"""
+
+import time
+
+import numpy as np
+
+from . import _lib
+
+_MODELS = {}
+
+
+def register_nll_model(name):
+    """Registry of NLL models selected by ``data: model:`` (tf_pwa/model/model.py:38)."""
+
+    def reg(cls):
+        _MODELS[name] = cls
+        return cls
+
+    return reg
+
+
+def get_nll_model(name):
+    return _MODELS[name]
+
+
+class WeightedData:
+    """A list of event sets with one weight array over their concatenation (the merged
+    data + background of get_weight_data)."""
+
+    def __init__(self, sets, weight):
+        self.sets = list(sets)
+        self.weight = np.asarray(weight, dtype=np.float64)
+        n = sum(s.n for s in self.sets)
+        if self.weight.shape != (n,):
+            raise ValueError("weight length {} != events {}".format(self.weight.shape, n))
+
+    def split(self):
+        out, i = [], 0
+        for s in self.sets:
+            out.append(self.weight[i:i + s.n])
+            i += s.n
+        return out
+
+
+def _as_sets(x):
+    if x is None:
+        return []
+    if isinstance(x, WeightedData):
+        return x.sets
+    if isinstance(x, (list, tuple)):
+        return list(x)
+    return [x]
+
+
+@register_nll_model("default")
+class Model:
+    """Model.nll / nll_grad / nll_grad_batch / nll_grad_hessian (model.py:576-790)."""
+
+    def __init__(self, amp, w_bkg=1.0, resolution_size=1, extended=False, **kwargs):
+        if resolution_size != 1:
+            raise NotImplementedError("resolution folding is not supported")
+        if extended:
+            raise NotImplementedError("extended likelihood is not supported")
+        self.Amp = amp
+        self.vm = amp.vm
+        self.w_bkg = w_bkg
+        self.resolution_size = 1
+        self.n_eval = 0
+
+    def get_weight_data(self, data, weight=None, bg=None, alpha=True):
+        """Concatenate data (+ background with weight -w_bkg) and apply the alpha rescale
+        (model.py:603-641). Returns (WeightedData, weight)."""
+        sets = _as_sets(data)
+        if weight is None:
+            weight = np.concatenate([s.weight for s in sets])
+        elif np.isscalar(weight):
+            weight = np.full(sum(s.n for s in sets), float(weight))
+        weight = np.asarray(weight, dtype=np.float64)
+        if bg is not None:
+            bsets = _as_sets(bg)
+            bw = []
+            for s in bsets:
+                if np.all(s.weight == 1.0):
+                    bw.append(np.full(s.n, -self.w_bkg))
+                else:
+                    bw.append(s.weight)
+            sets = sets + bsets
+            weight = np.concatenate([weight] + bw)
+        if alpha:
+            weight = weight * (weight.sum() / (weight * weight).sum())
+        return WeightedData(sets, weight), weight
+
+    @staticmethod
+    def _mc_weight(mcdata, mc_weight=None):
+        sets = _as_sets(mcdata)
+        if mc_weight is None or np.isscalar(mc_weight):
+            w = np.concatenate([s.weight for s in sets])
+        else:
+            w = np.asarray(mc_weight, dtype=np.float64)
+        return sets, w / w.sum()
+
+    def _pass(self, sets, weights, coef, mode, grad):
+        value, G = 0.0, np.zeros(self.Amp.P, dtype=np.complex128) if grad else None
+        for s, w in zip(sets, weights):
+            s.set_weight(w)
+            v, g, _ = s.eval(coef, mode, grad=grad)
+            value += v
+            if grad:
+                G += g
+        return value, G
+
+    def _nll_grad(self, wdata, mc_sets, mc_w, grad=True):
+        amp = self.Amp
+        coef = amp.coefficients()
+        sw = wdata.weight.sum()
+        ln, Gd = self._pass(wdata.sets, wdata.split(), coef, _lib.MODE_DATA, grad)
+        mw, i = [], 0
+        for s in mc_sets:
+            mw.append(mc_w[i:i + s.n])
+            i += s.n
+        int_mc, Gm = self._pass(mc_sets, mw, coef, _lib.MODE_MC, grad)
+        self.n_eval += 1
+        nll = -ln + sw * np.log(int_mc)
+        if not grad:
+            return nll, None
+        J = amp.jacobian()
+        g = -2.0 * np.real(J @ Gd) + sw * 2.0 * np.real(J @ Gm) / int_mc
+        return nll, g
+
+    def nll(self, data, mcdata, weight=1.0, batch=None, bg=None, mc_weight=1.0):
+        wdata, _ = self.get_weight_data(data, None if np.isscalar(weight) and weight == 1.0
+                                        else weight, bg=bg)
+        mc_sets, mc_w = self._mc_weight(mcdata, None if np.isscalar(mc_weight) else mc_weight)
+        return self._nll_grad(wdata, mc_sets, mc_w, grad=False)[0]
+
+    def nll_grad(self, data, mcdata, weight=1.0, batch=65000, bg=None, mc_weight=1.0):
+        wdata, _ = self.get_weight_data(data, None if np.isscalar(weight) and weight == 1.0
+                                        else weight, bg=bg)
+        mc_sets, mc_w = self._mc_weight(mcdata, None if np.isscalar(mc_weight) else mc_weight)
+        return self._nll_grad(wdata, mc_sets, mc_w)
+
+    def nll_grad_batch(self, data, mcdata, weight, mc_weight):
+        """``data``/``mcdata``: event sets (or lists); ``weight``/``mc_weight``: the already
+        rescaled / normalised weights over their concatenation (model.py:739-759)."""
+        wdata = WeightedData(_as_sets(data), np.concatenate([np.atleast_1d(w) for w in weight])
+                             if isinstance(weight, (list, tuple)) else weight)
+        mc_sets = _as_sets(mcdata)
+        mc_w = (np.concatenate([np.atleast_1d(w) for w in mc_weight])
+                if isinstance(mc_weight, (list, tuple)) else np.asarray(mc_weight))
+        return self._nll_grad(wdata, mc_sets, mc_w)
+
+    def nll_grad_hessian(self, data, mcdata, weight=1.0, batch=24000, bg=None, mc_weight=1.0,
+                         eps=1e-5):
+        """NLL, gradient and Hessian; the Hessian is the central difference of the analytic
+        gradient with a relative step ``eps`` per trainable parameter."""
+        nll, g = self.nll_grad(data, mcdata, weight=weight, bg=bg, mc_weight=mc_weight)
+        x0 = np.array(self.vm.get_all_val())
+        n = len(x0)
+        H = np.zeros((n, n))
+        for i in range(n):
+            h = eps * max(1.0, abs(x0[i]))
+            for sgn in (1, -1):
+                x = x0.copy()
+                x[i] += sgn * h
+                self.vm.set_all(x)
+                _, gi = self.nll_grad(data, mcdata, weight=weight, bg=bg, mc_weight=mc_weight)
+                H[:, i] += sgn * gi / (2 * h)
+        self.vm.set_all(x0)
+        H = 0.5 * (H + H.T)
+        return nll, g, H
+
+    def set_params(self, var):
+        self.Amp.set_params(var)
+
+    def get_params(self, trainable_only=False):
+        return self.Amp.get_params(trainable_only)
+
+
+class FCN:
+    """The callable minimised by the fit drivers (tf_pwa/model/model.py:1115-1285):
+    ``fcn(x)``, ``fcn.nll_grad(x)`` -> (float, ndarray in vm.trainable_vars order),
+    ``fcn.nll_grad_hessian(x)``, ``fcn.grad_hessp(x, p)``."""
+
+    def __init__(self, model, data, mcdata, bg=None, batch=65000, inmc=None, gauss_constr=None):
+        if inmc is not None:
+            raise NotImplementedError("inmc is not supported")
+        self.model = model
+        self.vm = model.vm
+        self.n_call = 0
+        self.cached_nll = None
+        self.batch = batch
+        self.wdata, self.weight = model.get_weight_data(data, bg=bg)
+        self.data = self.wdata
+        self.alpha = self.weight.sum() / (self.weight * self.weight).sum()
+        self.mc_sets, self.mc_weight = model._mc_weight(mcdata)
+        self.mcdata = self.mc_sets
+        self.gauss_constr = dict(gauss_constr or {})
+        self.last_time = 0.0
+
+    def _set(self, x):
+        if x is None or (isinstance(x, dict) and not x):
+            return
+        self.vm.set_all(x)
+
+    def _constr(self):
+        """Gaussian constraints sum ((x - mu)/sigma)^2 / 2 (model.py:915-987)."""
+        v, g = 0.0, np.zeros(len(self.vm.trainable_vars))
+        for name, (mu, sigma) in self.gauss_constr.items():
+            x = self.vm.get(name)
+            v += 0.5 * ((x - mu) / sigma) ** 2
+            if name in self.vm.trainable_vars:
+                g[self.vm.trainable_vars.index(name)] += (x - mu) / sigma ** 2
+        return v, g
+
+    def get_params(self, trainable_only=False):
+        return self.vm.get_all_dic(trainable_only)
+
+    def get_nll(self, x=None):
+        self._set(x)
+        nll, _ = self.model._nll_grad(self.wdata, self.mc_sets, self.mc_weight, grad=False)
+        self.n_call += 1
+        return nll
+
+    def __call__(self, x=None):
+        self.cached_nll = self.get_nll(x) + self._constr()[0]
+        return self.cached_nll
+
+    def get_nll_grad(self, x=None):
+        self._set(x)
+        nll, g = self.model._nll_grad(self.wdata, self.mc_sets, self.mc_weight)
+        self.n_call += 1
+        return nll, g
+
+    def nll_grad(self, x=None):
+        t = time.perf_counter()
+        nll, g = self.get_nll_grad(x)
+        cv, cg = self._constr()
+        self.cached_nll = nll + cv
+        self.last_time = time.perf_counter() - t
+        return float(self.cached_nll), g + cg
+
+    def grad(self, x=None):
+        return self.nll_grad(x)[1]
+
+    def nll_grad_hessian(self, x=None, batch=None, eps=1e-5):
+        self._set(x)
+        nll, g = self.get_nll_grad()
+        x0 = np.array(self.vm.get_all_val())
+        n = len(x0)
+        H = np.zeros((n, n))
+        for i in range(n):
+            h = eps * max(1.0, abs(x0[i]))
+            for sgn in (1, -1):
+                xx = x0.copy()
+                xx[i] += sgn * h
+                _, gi = self.get_nll_grad(xx)
+                H[:, i] += sgn * gi / (2 * h)
+        self.vm.set_all(x0)
+        H = 0.5 * (H + H.T)
+        cv, cg = self._constr()
+        for name, (mu, sigma) in self.gauss_constr.items():
+            if name in self.vm.trainable_vars:
+                i = self.vm.trainable_vars.index(name)
+                H[i, i] += 1.0 / sigma ** 2
+        return nll + cv, g + cg, H
+
+    def grad_hessp(self, x, p, batch=None, eps=1e-6):
+        """Gradient and Hessian-vector product H p (central difference of the gradient
+        along p)."""
+        x = np.asarray(x, dtype=np.float64)
+        p = np.asarray(p, dtype=np.float64)
+        _, g = self.nll_grad(x)
+        h = eps * max(1.0, np.linalg.norm(x)) / max(np.linalg.norm(p), 1e-300)
+        _, gp = self.get_nll_grad(x + h * p)
+        _, gm = self.get_nll_grad(x - h * p)
+        self.vm.set_all(x)
+        return g, (gp - gm) / (2 * h)

@@ -1,0 +1,179 @@
+"""Parameter store with tf-pwa's names, ordering and complex/bound conventions.
+
+Follows tf_pwa/variable.py: every parameter is a named real; complex couplings are a pair
+``<name>r``/``<name>i`` in polar form (r, phi) by default (variable.py:171-197, 1715-1765);
+``trainable_vars`` is the order of the gradient vector (638-646); bounds transform the fit
+variable x to the physical y with dy/dx applied to the gradient (797-821, 1029-1135).
+"""
+
+import numpy as np
+
+
+class Bound:
+    """Bound transform y(x) of tf_pwa/variable.py:1029-1135 ("asin" style)."""
+
+    def __init__(self, a=None, b=None):
+        self.lower = a
+        self.upper = b
+
+    def get_x2y(self, x):
+        a, b = self.lower, self.upper
+        if a is not None and b is not None:
+            return (b - a) * (np.sin(x) + 1) / 2 + a
+        if a is not None:
+            return a + np.sqrt(x * x + 1) - 1
+        if b is not None:
+            return b + 1 - np.sqrt(x * x + 1)
+        return x
+
+    def get_y2x(self, y):
+        a, b = self.lower, self.upper
+        if a is not None and b is not None:
+            return np.arcsin(np.clip(2 * (y - a) / (b - a) - 1, -1, 1))
+        if a is not None:
+            return np.sqrt((y - a + 1) ** 2 - 1)
+        if b is not None:
+            return np.sqrt((b - y + 1) ** 2 - 1)
+        return y
+
+    def get_dydx(self, x):
+        a, b = self.lower, self.upper
+        if a is not None and b is not None:
+            return (b - a) * np.cos(x) / 2
+        if a is not None:
+            return x / np.sqrt(x * x + 1)
+        if b is not None:
+            return -x / np.sqrt(x * x + 1)
+        return 1.0
+
+
+class VarsManager:
+    """Named real parameters, the trainable subset and complex pairs."""
+
+    def __init__(self, polar=True):
+        self.polar = polar
+        self.variables = {}
+        self.trainable_vars = []
+        self.complex_vars = {}
+        self.bnd_dic = {}
+        self.init_val = {}
+
+    # -------------------------------------------------------------- creation
+    def add_real_var(self, name, value=None, range_=None, trainable=True, rng=None):
+        if name in self.trainable_vars:
+            self.trainable_vars.remove(name)
+        if value is None:
+            rng = rng or np.random
+            lo, hi = (0.0, 1.0) if range_ is None else range_
+            value = rng.uniform(lo, hi)
+        self.variables[name] = float(value)
+        self.init_val[name] = float(value)
+        if trainable:
+            self.trainable_vars.append(name)
+
+    def add_complex_var(self, name, polar=None, trainable=True, fix_vals=(1.0, 0.0), rng=None):
+        polar = self.polar if polar is None else polar
+        if trainable:
+            if polar:
+                self.add_real_var(name + "r", range_=(0.0, 2.0), rng=rng)
+                self.add_real_var(name + "i", range_=(-np.pi, np.pi), rng=rng)
+            else:
+                self.add_real_var(name + "r", range_=(-1.0, 1.0), rng=rng)
+                self.add_real_var(name + "i", range_=(-1.0, 1.0), rng=rng)
+        else:
+            self.add_real_var(name + "r", value=fix_vals[0], trainable=False)
+            self.add_real_var(name + "i", value=fix_vals[1], trainable=False)
+        self.complex_vars[name] = polar
+
+    def set_fix(self, name, value=None, unfix=False):
+        """variable.py:415-440: fixing removes from trainable_vars, freeing appends."""
+        if value is not None:
+            self.variables[name] = float(value)
+        if unfix:
+            if name not in self.trainable_vars:
+                self.trainable_vars.append(name)
+        elif name in self.trainable_vars:
+            self.trainable_vars.remove(name)
+
+    def set_bound(self, bound_dic):
+        for name, (a, b) in bound_dic.items():
+            self.bnd_dic[name] = Bound(a, b)
+
+    # -------------------------------------------------------------- access
+    def get(self, name):
+        return self.variables[name]
+
+    def set(self, name, value):
+        self.variables[name] = float(value)
+
+    def get_all_val(self):
+        return [self.variables[n] for n in self.trainable_vars]
+
+    def get_all_dic(self, trainable_only=False):
+        if trainable_only:
+            return {n: self.variables[n] for n in self.trainable_vars}
+        return dict(self.variables)
+
+    def set_all(self, vals):
+        """dict: by name; sequence: in trainable_vars order (variable.py:684-697)."""
+        if isinstance(vals, dict):
+            for k, v in vals.items():
+                if k in self.variables:
+                    self.variables[k] = float(v)
+        else:
+            vals = list(vals)
+            if len(vals) != len(self.trainable_vars):
+                raise ValueError("expected {} values, got {}".format(
+                    len(self.trainable_vars), len(vals)))
+            for n, v in zip(self.trainable_vars, vals):
+                self.variables[n] = float(v)
+
+    def complex_value(self, name):
+        """Variable.__call__ (variable.py:1715-1765)."""
+        r, i = self.variables[name + "r"], self.variables[name + "i"]
+        if self.complex_vars.get(name, self.polar):
+            return complex(r * np.cos(i), r * np.sin(i))
+        return complex(r, i)
+
+    def complex_derivs(self, name):
+        """d value / d (name+'r'), d value / d (name+'i')."""
+        r, i = self.variables[name + "r"], self.variables[name + "i"]
+        if self.complex_vars.get(name, self.polar):
+            e = complex(np.cos(i), np.sin(i))
+            return e, 1j * r * e
+        return 1.0 + 0j, 1j
+
+    # -------------------------------------------------------------- bounds
+    def trans_fcn_grad(self, fcn_grad):
+        """F(x) = F(y(x)), dF/dx = dF/dy dy/dx (variable.py:797-821)."""
+
+        def fcn_t(xvals):
+            xvals = np.asarray(xvals, dtype=np.float64)
+            yvals = xvals.copy()
+            dydx = np.ones_like(xvals)
+            for i, name in enumerate(self.trainable_vars):
+                if name in self.bnd_dic:
+                    yvals[i] = self.bnd_dic[name].get_x2y(xvals[i])
+                    dydx[i] = self.bnd_dic[name].get_dydx(xvals[i])
+            f, g = fcn_grad(yvals)
+            return f, np.asarray(g) * dydx
+
+        return fcn_t
+
+    def get_x_from_y(self):
+        x = []
+        for name in self.trainable_vars:
+            y = self.variables[name]
+            x.append(self.bnd_dic[name].get_y2x(y) if name in self.bnd_dic else y)
+        return np.array(x)
+
+    def std_polar_all(self):
+        """r >= 0 and phi in [-pi, pi) for every polar pair (variable.py:716-741)."""
+        for name, polar in self.complex_vars.items():
+            if not polar:
+                continue
+            r, p = self.variables[name + "r"], self.variables[name + "i"]
+            if r < 0:
+                r, p = -r, p + np.pi
+            self.variables[name + "r"] = r
+            self.variables[name + "i"] = (p + np.pi) % (2 * np.pi) - np.pi
