@@ -116,6 +116,9 @@ def main():
     ap.add_argument("--n-data", type=int, default=1_000_000)
     ap.add_argument("--n-mc", type=int, default=10_000_000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--calib-copy", action="store_true",
+                    help="also run a 1 GiB device-to-device copy (the counter calibration "
+                         "kernel for FETCH_SIZE / WRITE_SIZE)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU only (the likelihood is sharded by events across "
@@ -179,6 +182,15 @@ def main():
                      "kernel_ms_avg": ms.value / max(1, nl.value),
                      "bytes_per_event": bytes_per_event},
     }
+    if args.calib_copy:
+        import torch
+
+        src = torch.ones(1 << 27, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        res["calib_copy_bytes"] = src.numel() * 8
     if not args.no_cpu_baseline:
         log("cpu baseline ...")
         res["cpu_baseline"] = cpu_baseline(conf, data_p4, args.n_data, args.n_mc)

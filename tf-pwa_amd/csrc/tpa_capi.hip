@@ -5,6 +5,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <mutex>
 #include <string>
@@ -147,9 +148,24 @@ struct tpa_events_s {
 
 // --------------------------------------------------------------------------- dispatcher
 // Partial waves are split over KG lane groups so that each lane holds at most 8 of them.
+// Events per wave step: U event groups are loaded before any is processed, so more bytes
+// are in flight per wave (TFPWA_EVAL_UNROLL = 1, 2 or 4 overrides the default of 2).
+static int eval_unroll() {
+  static int u = [] {
+    const char* v = getenv("TFPWA_EVAL_UNROLL");
+    int x = v ? atoi(v) : 2;
+    return (x == 1 || x == 2 || x == 4) ? x : 2;
+  }();
+  return u;
+}
+
 template <int LG, int KG>
 static void launch_eval(const EvalArgs& a, int grid, hipStream_t s) {
-  hipLaunchKernelGGL((k_eval_fold<LG, KG, 8>), dim3(grid), dim3(256), 0, s, a);
+  switch (eval_unroll()) {
+    case 1: hipLaunchKernelGGL((k_eval_fold<LG, KG, 8, 1>), dim3(grid), dim3(256), 0, s, a); break;
+    case 4: hipLaunchKernelGGL((k_eval_fold<LG, KG, 8, 4>), dim3(grid), dim3(256), 0, s, a); break;
+    default: hipLaunchKernelGGL((k_eval_fold<LG, KG, 8, 2>), dim3(grid), dim3(256), 0, s, a); break;
+  }
 }
 
 static int eval_kgroups(int lg, int P) {
@@ -410,9 +426,9 @@ int tpa_eval(tpa_events_t ev, const double* coef, int mode, int flags, double* o
   if (rc != TPA_OK) return fail(rc, "no evaluation kernel for this helicity/partial-wave count");
   HIPCHK(hipGetLastError());
   if (timing) HIPCHK(hipEventRecord(ev->t1, s));
-  hipLaunchKernelGGL(k_reduce, dim3(1), dim3(256), 0, s, ev->d_part, grid, a.nacc, ev->d_out);
-  HIPCHK(hipGetLastError());
   const int nout = a.grad ? a.nacc : 2;
+  hipLaunchKernelGGL(k_reduce, dim3(nout), dim3(256), 0, s, ev->d_part, grid, a.nacc, ev->d_out);
+  HIPCHK(hipGetLastError());
   HIPCHK(hipMemcpyAsync(out, ev->d_out, sizeof(double) * nout, hipMemcpyDeviceToHost, s));
   if (want_f)
     HIPCHK(hipMemcpyAsync(f_out, ev->d_f, sizeof(double) * ev->n, hipMemcpyDeviceToHost, s));

@@ -316,7 +316,7 @@ TPA_DEV void neumaier_merge(double& s, double& c, double s2, double c2) {
 // helicity components of one event, and the KG groups split the partial waves
 // k = kg + KG * j (j < PML). A_lambda is completed by a reduction over the k-groups and f by a
 // reduction over lambda, so each lane keeps only PML amplitudes and PML gradient sums.
-template <int LG, int KG, int PML>
+template <int LG, int KG, int PML, int U>
 __global__ __launch_bounds__(256) void k_eval_fold(EvalArgs a) {
   constexpr int LPE = LG * KG;  // lanes per event
   constexpr int EPW = 64 / LPE;
@@ -337,61 +337,73 @@ __global__ __launch_bounds__(256) void k_eval_fold(EvalArgs a) {
     cr[j] = c.x;
     ci[j] = c.y;
   }
-  const int64_t stride = (int64_t)gridDim.x * 4 * EPW;
-  for (int64_t base = ((int64_t)blockIdx.x * 4 + wave) * EPW; base < a.N; base += stride) {
-    const int64_t e = base + el;
-    const bool valid = e < a.N;
-    double2 hv[PML];
-    double ar = 0.0, ai = 0.0;
+  // each wave takes U * EPW consecutive events per step and issues all their loads first
+  const int64_t stride = (int64_t)gridDim.x * 4 * EPW * U;
+  for (int64_t base = ((int64_t)blockIdx.x * 4 + wave) * EPW * U; base < a.N; base += stride) {
+    double2 hv[U][PML];
+    int64_t ev[U];
+    bool ok[U];
 #pragma unroll
-    for (int j = 0; j < PML; ++j) {
-      const int k = kg + KG * j;
-      hv[j] = (valid && k < a.P) ? a.h[((int64_t)k * a.N + e) * LG + lam] : make_double2(0.0, 0.0);
-    }
-#pragma unroll
-    for (int j = 0; j < PML; ++j) {
-      ar += cr[j] * hv[j].x - ci[j] * hv[j].y;
-      ai += cr[j] * hv[j].y + ci[j] * hv[j].x;
-    }
-#pragma unroll
-    for (int m = LG; m < LPE; m <<= 1) {
-      ar += __shfl_xor(ar, m, 64);
-      ai += __shfl_xor(ai, m, 64);
-    }
-    double f = ar * ar + ai * ai;
-#pragma unroll
-    for (int m = 1; m < LG; m <<= 1) f += __shfl_xor(f, m, 64);
-    double y = 0.0, u = 0.0;
-    if (valid) {
-      const double w = a.w[e];
-      if (a.mode == 0) {
-        const double dom = (w == 0.0) ? 1.0 : w;
-        const double x = (f * w) / dom;
-        double cl, dcl;
-        if (x > eps) {
-          cl = log(x);
-          dcl = 1.0 / x;
-        } else {
-          const double dx = x - eps;
-          const double t = dx / eps;
-          cl = (logeps + t) - (t * t) / 2.0;
-          dcl = 1.0 / eps - t / eps;
-        }
-        y = w * cl;
-        u = (w * dcl) * (w / dom);
-      } else {
-        y = w * f;
-        u = w;
-      }
-      if (a.f_out != nullptr && lam == 0 && kg == 0) a.f_out[e] = f;
-    }
-    if (lam == 0 && kg == 0) neumaier_add(vs, vc, y);
-    if (a.grad) {
-      const double tr = u * ar, ti = -(u * ai);
+    for (int uu = 0; uu < U; ++uu) {
+      ev[uu] = base + uu * EPW + el;
+      ok[uu] = ev[uu] < a.N;
 #pragma unroll
       for (int j = 0; j < PML; ++j) {
-        gr[j] += tr * hv[j].x - ti * hv[j].y;
-        gi[j] += tr * hv[j].y + ti * hv[j].x;
+        const int k = kg + KG * j;
+        hv[uu][j] = (ok[uu] && k < a.P) ? a.h[((int64_t)k * a.N + ev[uu]) * LG + lam]
+                                        : make_double2(0.0, 0.0);
+      }
+    }
+#pragma unroll
+    for (int uu = 0; uu < U; ++uu) {
+      const int64_t e = ev[uu];
+      const bool valid = ok[uu];
+      double ar = 0.0, ai = 0.0;
+#pragma unroll
+      for (int j = 0; j < PML; ++j) {
+        ar += cr[j] * hv[uu][j].x - ci[j] * hv[uu][j].y;
+        ai += cr[j] * hv[uu][j].y + ci[j] * hv[uu][j].x;
+      }
+#pragma unroll
+      for (int m = LG; m < LPE; m <<= 1) {
+        ar += __shfl_xor(ar, m, 64);
+        ai += __shfl_xor(ai, m, 64);
+      }
+      double f = ar * ar + ai * ai;
+#pragma unroll
+      for (int m = 1; m < LG; m <<= 1) f += __shfl_xor(f, m, 64);
+      double y = 0.0, u = 0.0;
+      if (valid) {
+        const double w = a.w[e];
+        if (a.mode == 0) {
+          const double dom = (w == 0.0) ? 1.0 : w;
+          const double x = (f * w) / dom;
+          double cl, dcl;
+          if (x > eps) {
+            cl = log(x);
+            dcl = 1.0 / x;
+          } else {
+            const double dx = x - eps;
+            const double t = dx / eps;
+            cl = (logeps + t) - (t * t) / 2.0;
+            dcl = 1.0 / eps - t / eps;
+          }
+          y = w * cl;
+          u = (w * dcl) * (w / dom);
+        } else {
+          y = w * f;
+          u = w;
+        }
+        if (a.f_out != nullptr && lam == 0 && kg == 0) a.f_out[e] = f;
+      }
+      if (lam == 0 && kg == 0) neumaier_add(vs, vc, y);
+      if (a.grad) {
+        const double tr = u * ar, ti = -(u * ai);
+#pragma unroll
+        for (int j = 0; j < PML; ++j) {
+          gr[j] += tr * hv[uu][j].x - ti * hv[uu][j].y;
+          gi[j] += tr * hv[uu][j].y + ti * hv[uu][j].x;
+        }
       }
     }
   }
@@ -443,23 +455,32 @@ __global__ __launch_bounds__(256) void k_eval_fold(EvalArgs a) {
   }
 }
 
-// Fixed-order reduction over blocks: slot 0/1 are the (sum, compensation) of the value,
-// the remaining slots are summed with a compensated accumulator each.
+// Fixed-order reduction over the per-block partial sums: one block per output slot. Slot 0/1
+// are the (sum, compensation) pair of the value and fold into out[0]; every other slot is a
+// compensated sum. Each thread reduces a fixed strided subset, then a fixed tree in LDS
+// combines the threads, so the result does not depend on scheduling.
 __global__ __launch_bounds__(256) void k_reduce(const double* __restrict__ partials, int nblocks,
                                                 int nacc, double* __restrict__ out) {
-  for (int j = threadIdx.x; j < nacc; j += blockDim.x) {
-    if (j == 1) {
-      out[1] = 0.0;
-      continue;
-    }
-    double s = 0.0, c = 0.0;
-    for (int b = 0; b < nblocks; ++b) neumaier_add(s, c, partials[(int64_t)b * nacc + j]);
-    if (j == 0) {
-      double cc = 0.0, ccc = 0.0;
-      for (int b = 0; b < nblocks; ++b) neumaier_add(cc, ccc, partials[(int64_t)b * nacc + 1]);
-      c += cc + ccc;
-    }
-    out[j] = s + c;
+  const int j = blockIdx.x;
+  const int t = threadIdx.x;
+  __shared__ double ss[256], cc[256];
+  double s = 0.0, c = 0.0;
+  if (j != 1) {
+    for (int b = t; b < nblocks; b += 256) neumaier_add(s, c, partials[(int64_t)b * nacc + j]);
+    if (j == 0)
+      for (int b = t; b < nblocks; b += 256) neumaier_add(s, c, partials[(int64_t)b * nacc + 1]);
   }
+  ss[t] = s;
+  cc[t] = c;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (t < w) {
+      double s1 = ss[t], c1 = cc[t];
+      neumaier_merge(s1, c1, ss[t + w], cc[t + w]);
+      ss[t] = s1;
+      cc[t] = c1;
+    }
+    __syncthreads();
+  }
+  if (t == 0) out[j] = (j == 1) ? 0.0 : ss[0] + cc[0];
 }
-

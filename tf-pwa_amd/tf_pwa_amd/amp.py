@@ -280,46 +280,90 @@ class AmplitudeModel:
             self._dev = None
 
     # ---------------------------------------------------------------- coefficients
+    def _factor_index(self):
+        """Unique complex factors (chain totals and g_ls of both vertices) and, per partial
+        wave, the indices of its three factors."""
+        if getattr(self, "_fidx", None) is None:
+            names = self._names()
+            cnames, pos = [], {}
+
+            def ix(n):
+                if n not in pos:
+                    pos[n] = len(cnames)
+                    cnames.append(n)
+                return pos[n]
+
+            it, i1, i2, ch, s1, s2 = [], [], [], [], [], []
+            for w in self.waves:
+                nm = names[w.chain]
+                it.append(ix(nm["total"]))
+                i1.append(ix(nm["g1"][w.ls1]))
+                i2.append(ix(nm["g2"][w.ls2]))
+                ch.append(w.chain)
+                s1.append(w.ls1)
+                s2.append(w.ls2)
+            self._fidx = dict(cnames=cnames, it=np.array(it), i1=np.array(i1),
+                              i2=np.array(i2), ch=np.array(ch), s1=np.array(s1),
+                              s2=np.array(s2))
+        return self._fidx
+
+    def _factor_values(self):
+        """Complex value of every unique factor and its derivatives d/dr, d/di."""
+        fx = self._factor_index()
+        vm = self.vm
+        r = np.array([vm.variables[n + "r"] for n in fx["cnames"]])
+        i = np.array([vm.variables[n + "i"] for n in fx["cnames"]])
+        polar = np.array([vm.complex_vars.get(n, vm.polar) for n in fx["cnames"]])
+        e = np.exp(1j * i)
+        val = np.where(polar, r * e, r + 1j * i)
+        d_r = np.where(polar, e, 1.0 + 0j)
+        d_i = np.where(polar, 1j * r * e, 1j)
+        return val, d_r, d_i
+
+    def _barrier_per_wave(self):
+        key = tuple(self.shape_params())
+        if getattr(self, "_bpw_key", None) != key:
+            fx = self._factor_index()
+            bc = self.barrier_constants()
+            self._bpw = np.array([bc[c]["c1"][a] * bc[c]["c2"][b]
+                                  for c, a, b in zip(fx["ch"], fx["s1"], fx["s2"])])
+            self._bpw_key = key
+        return self._bpw
+
     def coefficients(self):
         """c_k for the current parameters (complex array of length P)."""
-        vm = self.vm
-        names = self._names()
-        bc = self.barrier_constants()
-        c = np.zeros(self.P, dtype=np.complex128)
-        for w_i, w in enumerate(self.waves):
-            nm, b = names[w.chain], bc[w.chain]
-            c[w_i] = (vm.complex_value(nm["total"]) * vm.complex_value(nm["g1"][w.ls1])
-                      * vm.complex_value(nm["g2"][w.ls2]) * b["c1"][w.ls1] * b["c2"][w.ls2])
-        return c
+        fx = self._factor_index()
+        val, _, _ = self._factor_values()
+        return val[fx["it"]] * val[fx["i1"]] * val[fx["i2"]] * self._barrier_per_wave()
 
     def jacobian(self):
         """J[i, k] = d c_k / d trainable_vars[i] (complex) for the coupling parameters.
         Raises if a lineshape parameter is trainable (needs the shape-gradient path)."""
         vm = self.vm
         index = {n: i for i, n in enumerate(vm.trainable_vars)}
-        names = self._names()
-        for nm in names:
+        for nm in self._names():
             for key in ("mass", "width"):
                 if nm[key] in index:
                     raise NotImplementedError(
                         "gradients with respect to floating masses/widths use the shape path")
-        bc = self.barrier_constants()
+        fx = self._factor_index()
+        val, d_r, d_i = self._factor_values()
+        C = self._barrier_per_wave()
+        slots = (fx["it"], fx["i1"], fx["i2"])
         J = np.zeros((len(vm.trainable_vars), self.P), dtype=np.complex128)
-        for w_i, w in enumerate(self.waves):
-            nm, b = names[w.chain], bc[w.chain]
-            facs = [nm["total"], nm["g1"][w.ls1], nm["g2"][w.ls2]]
-            vals = [vm.complex_value(f) for f in facs]
-            const = b["c1"][w.ls1] * b["c2"][w.ls2]
-            for j, f in enumerate(facs):
-                rest = const
-                for jj, v in enumerate(vals):
-                    if jj != j:
-                        rest = rest * v
-                dr, di = vm.complex_derivs(f)
-                if f + "r" in index:
-                    J[index[f + "r"], w_i] += dr * rest
-                if f + "i" in index:
-                    J[index[f + "i"], w_i] += di * rest
+        ks = np.arange(self.P)
+        row_r = np.array([index.get(n + "r", -1) for n in fx["cnames"]])
+        row_i = np.array([index.get(n + "i", -1) for n in fx["cnames"]])
+        for a in range(3):
+            rest = C.astype(np.complex128)
+            for b in range(3):
+                if b != a:
+                    rest = rest * val[slots[b]]
+            f = slots[a]
+            for rows, d in ((row_r, d_r), (row_i, d_i)):
+                rr = rows[f]
+                m = rr >= 0
+                np.add.at(J, (rr[m], ks[m]), d[f][m] * rest[m])
         return J
 
     # ---------------------------------------------------------------- parameters

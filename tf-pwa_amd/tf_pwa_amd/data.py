@@ -63,12 +63,16 @@ class EventSet:
             self._h = None
 
     def set_weight(self, w):
+        if w is getattr(self, "_src_weight", None):
+            return
+        src = w
         w = np.ascontiguousarray(w, dtype=np.float64)
         if w.shape != (self.n,):
             raise ValueError("weight must have one entry per event")
         if not np.array_equal(w, self._dev_weight):
             _lib.check(_lib.lib().tpa_events_set_weight(self._h, _lib.dptr(w)))
             self._dev_weight = w.copy()
+        self._src_weight = src
 
     def angles(self):
         nf = _lib.NF_TOPO * len(self.amp.topo) + 4

@@ -117,15 +117,23 @@ class Model:
                 G += g
         return value, G
 
-    def _nll_grad(self, wdata, mc_sets, mc_w, grad=True):
+    @staticmethod
+    def _split(sets, w):
+        out, i = [], 0
+        for s in sets:
+            out.append(w[i:i + s.n])
+            i += s.n
+        return out
+
+    def _nll_grad(self, wdata, mc_sets, mc_w, grad=True, split=None):
+        """``split``: optional precomputed (data weights per set, MC weights per set, sum w)
+        so repeated calls do not touch the weight arrays."""
         amp = self.Amp
         coef = amp.coefficients()
-        sw = wdata.weight.sum()
-        ln, Gd = self._pass(wdata.sets, wdata.split(), coef, _lib.MODE_DATA, grad)
-        mw, i = [], 0
-        for s in mc_sets:
-            mw.append(mc_w[i:i + s.n])
-            i += s.n
+        if split is None:
+            split = (wdata.split(), self._split(mc_sets, mc_w), wdata.weight.sum())
+        dw, mw, sw = split
+        ln, Gd = self._pass(wdata.sets, dw, coef, _lib.MODE_DATA, grad)
         int_mc, Gm = self._pass(mc_sets, mw, coef, _lib.MODE_MC, grad)
         self.n_eval += 1
         nll = -ln + sw * np.log(int_mc)
@@ -204,6 +212,8 @@ class FCN:
         self.mcdata = self.mc_sets
         self.gauss_constr = dict(gauss_constr or {})
         self.last_time = 0.0
+        self._split = (self.wdata.split(), Model._split(self.mc_sets, self.mc_weight),
+                       float(self.weight.sum()))
 
     def _set(self, x):
         if x is None or (isinstance(x, dict) and not x):
@@ -225,7 +235,8 @@ class FCN:
 
     def get_nll(self, x=None):
         self._set(x)
-        nll, _ = self.model._nll_grad(self.wdata, self.mc_sets, self.mc_weight, grad=False)
+        nll, _ = self.model._nll_grad(self.wdata, self.mc_sets, self.mc_weight, grad=False,
+                                      split=self._split)
         self.n_call += 1
         return nll
 
@@ -235,7 +246,8 @@ class FCN:
 
     def get_nll_grad(self, x=None):
         self._set(x)
-        nll, g = self.model._nll_grad(self.wdata, self.mc_sets, self.mc_weight)
+        nll, g = self.model._nll_grad(self.wdata, self.mc_sets, self.mc_weight,
+                                      split=self._split)
         self.n_call += 1
         return nll, g
 
