@@ -11,6 +11,7 @@ on one GPU.
 """
 
 import argparse
+import ctypes
 import json
 import os
 import sys
@@ -128,69 +129,93 @@ def main():
     from tf_pwa_amd import FCN, _lib
 
     conf, data, mc, data_p4, _ = make_workload(args.n_data, args.n_mc)
-    fcn = FCN(conf.get_model(), data, mc)
-    x0 = np.array(conf.vm.get_all_val())
-    for _ in range(args.warmup):
-        fcn.nll_grad(x0)
     L = _lib.lib()
-    L.tpa_timing_reset()
-    L.tpa_timing_enable(1)
-    try:
-        import torch
-
-        sync = torch.cuda.synchronize if torch.cuda.is_available() else (lambda: None)
-    except Exception:  # torch is only the stream-sync helper here
-        sync = lambda: None
-    sync()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        nll, g = fcn.nll_grad(x0)
-    sync()
-    elapsed = time.perf_counter() - t0
-    L.tpa_timing_enable(0)
-    import ctypes
-
-    ms = ctypes.c_double(0)
-    nl = ctypes.c_int64(0)
-    L.tpa_timing_read(ctypes.byref(ms), ctypes.byref(nl))
+    x0 = np.array(conf.vm.get_all_val())
     amp = conf.get_amplitude()
+
+    def sync():
+        _lib.check(L.tpa_device_synchronize())
+
+    def time_path(fcn):
+        for _ in range(args.warmup):
+            fcn.nll_grad(x0)
+        L.tpa_timing_reset()
+        L.tpa_timing_enable(1)
+        sync()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            nll, g = fcn.nll_grad(x0)
+        sync()
+        elapsed = time.perf_counter() - t0
+        L.tpa_timing_enable(0)
+        ms = ctypes.c_double(0)
+        nl = ctypes.c_int64(0)
+        L.tpa_timing_read(ctypes.byref(ms), ctypes.byref(nl))
+        return dict(evals=args.steps / elapsed, ms_per_step=1e3 * elapsed / args.steps,
+                    kernel_ms=ms.value, launches=int(nl.value), nll=nll, g=g)
+
     bytes_per_event = amp.P * amp.lam_pad * 16 + 8
-    launches_per_step = 2
-    bytes_per_step = (args.n_data + args.n_mc) * bytes_per_event
-    kernel_s = ms.value / 1e3
-    achieved = bytes_per_step * args.steps / kernel_s / 1e9 if kernel_s > 0 else 0.0
+
+    def roofline(r, n_events_per_step):
+        kernel_s = r["kernel_ms"] / 1e3
+        bytes_total = n_events_per_step * bytes_per_event * args.steps
+        achieved = bytes_total / kernel_s / 1e9 if kernel_s > 0 else 0.0
+        return {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": "k_eval_fold",
+                "launches": r["launches"], "kernel_ms_avg": r["kernel_ms"] / max(1, r["launches"]),
+                "bytes_per_event": bytes_per_event,
+                "bytes_per_launch": n_events_per_step * bytes_per_event / max(
+                    1, r["launches"] // args.steps)}
+
+    log("fp64 path ...")
+    r64 = time_path(FCN(conf.get_model(), data, mc))
+    log("fp64: {:.1f} evals/s".format(r64["evals"]))
+    fmix = FCN(conf.get_model(), data, mc, precision="mixed")
+    t = time.perf_counter()
+    mc.gram()
+    gram_wall = time.perf_counter() - t
+    log("gram: kernel {:.3f} ms, wall {:.3f} s".format(mc.gram_ms, gram_wall))
+    rmix = time_path(fmix)
+    dnll = abs(rmix["nll"] - r64["nll"])
+    grel = float(np.linalg.norm(rmix["g"] - r64["g"]) / np.linalg.norm(r64["g"]))
+    log("mixed: {:.1f} evals/s, |dNLL| {:.2e}, grad rel {:.2e}".format(rmix["evals"], dnll, grel))
+    ok = dnll < 1e-3
+    head = rmix if ok else r64
+    gram_flops = 8.0 * args.n_mc * amp.lam * amp.P ** 2
     res = {
         "metric": METRIC,
-        "value": args.steps / elapsed,
+        "value": head["evals"],
         "unit": "evals/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": 1e3 * elapsed / args.steps,
+        "ms_per_step": head["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f64 data log-sum + f32 MFMA MC Gram" if ok else "f64",
         "data": "synthetic",
-        "config": {"workload": "Lc->pKpi 6 resonances, 22 partial waves, FP64 path",
+        "config": {"workload": "Lc->pKpi 6 resonances, 22 partial waves, 1M data + 10M MC",
                    "n_data": args.n_data, "n_mc": args.n_mc, "n_params": len(x0),
-                   "path": "fp64"},
-        "nll": nll,
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "k_eval_fold", "launches": int(nl.value),
-                     "kernel_ms_avg": ms.value / max(1, nl.value),
-                     "bytes_per_event": bytes_per_event},
+                   "path": "mixed" if ok else "fp64"},
+        "roofline": roofline(head, args.n_data if ok else args.n_data + args.n_mc),
+        "paths": {
+            "fp64": {"evals_per_s": r64["evals"], "ms_per_step": r64["ms_per_step"],
+                     "nll": r64["nll"], "roofline": roofline(r64, args.n_data + args.n_mc)},
+            "mixed": {"evals_per_s": rmix["evals"], "ms_per_step": rmix["ms_per_step"],
+                      "nll": rmix["nll"], "abs_dnll_vs_fp64": dnll, "grad_rel_err_vs_fp64": grel,
+                      "speedup_vs_fp64": rmix["evals"] / r64["evals"]},
+            "gram_build": {"kernel": "k_gram_f32", "ms": mc.gram_ms, "wall_s": gram_wall,
+                           "bound": "mfma", "flops": gram_flops,
+                           "achieved_TFLOPs": gram_flops / (mc.gram_ms * 1e-3) / 1e12,
+                           "peak_TFLOPs": 157.3},
+        },
     }
     if args.calib_copy:
-        import torch
-
-        src = torch.ones(1 << 27, dtype=torch.float64, device="cuda")
-        dst = torch.empty_like(src)
-        for _ in range(3):
-            dst.copy_(src)
-        torch.cuda.synchronize()
-        res["calib_copy_bytes"] = src.numel() * 8
+        cms = ctypes.c_double(0)
+        _lib.check(L.tpa_calib_copy(1 << 30, 3, ctypes.byref(cms)))
+        res["calib_copy"] = {"bytes": 1 << 30, "ms": cms.value,
+                             "GBps_read+write": 2 * (1 << 30) / cms.value / 1e6}
     if not args.no_cpu_baseline:
         log("cpu baseline ...")
         res["cpu_baseline"] = cpu_baseline(conf, data_p4, args.n_data, args.n_mc)

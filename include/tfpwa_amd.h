@@ -106,6 +106,22 @@ int tpa_timing_enable(int on);
 int tpa_timing_read(double* ms, int64_t* launches);
 int tpa_timing_reset(void);
 
+/* Partial-wave Gram matrix of an event set on FP32 MFMA with FP64 block refolds:
+ * I_kl = sum_e w_e sum_lambda conj(h_k) h_l (out: host [P][P] complex, interleaved). With it
+ * the weighted MC sum at any coefficients is c^dagger I c, and its adjoint G = conj(I c).
+ * Replaces build_int_matrix of tf-pwa's cached-integral model
+ * (tf_pwa/experimental/opt_int.py:30-52, tf_pwa/model/opt_int.py:128-296). P <= 32.
+ * ms (optional): the MFMA kernel's duration. */
+int tpa_gram(tpa_events_t ev, double* out, double* ms);
+
+/* Wait for all device work of the process (the timing barrier of benchmarks). */
+int tpa_device_synchronize(void);
+
+/* Measurement helper: copy `bytes` between two fresh device buffers `reps` times with a
+ * plain 16-byte streaming kernel (k_copy) and return the average milliseconds per copy.
+ * It is the calibration kernel for the FETCH_SIZE / WRITE_SIZE counters. */
+int tpa_calib_copy(int64_t bytes, int reps, double* ms);
+
 #ifdef __cplusplus
 }
 #endif

@@ -79,3 +79,38 @@ def test_nll_grad_matches_oracle(which, request, gpu_available):
     assert np.linalg.norm(g - ref_g) <= 1e-7 * np.linalg.norm(ref_g), (g, ref_g)
     # value-only path agrees with the gradient path
     assert abs(fcn(None) - nll) < 1e-9 * max(1.0, abs(nll))
+
+
+def test_gram_matches_partial_waves(case2, gpu_available):
+    """FP32 MFMA Gram against the FP64 Gram of the same partial waves (read back)."""
+    conf = case2["conf"]
+    mc = conf.data_from_p4(case2["mc"])
+    w = np.random.default_rng(1).uniform(0.5, 1.5, mc.n)
+    mc.set_weight(w)
+    h = mc.partial_waves()  # [P][n][lam_pad]
+    ref = np.einsum("e,kel,jel->kj", w, np.conj(h), h)
+    got = mc.gram()
+    scale = np.abs(ref).max()
+    assert np.abs(got - ref).max() < 1e-6 * scale
+    np.testing.assert_allclose(got, np.conj(got.T), atol=1e-12 * scale)
+    # c^H I c equals the engine's direct weighted MC sum
+    c = conf.get_amplitude().coefficients()
+    from tf_pwa_amd import _lib
+
+    v, G, _ = mc.eval(c, _lib.MODE_MC, grad=True)
+    assert abs(np.real(np.vdot(c, got @ c)) - v) < 1e-6 * abs(v)
+    np.testing.assert_allclose(np.conj(got @ c), G, rtol=0, atol=1e-6 * np.abs(G).max())
+
+
+def test_mixed_path_close_to_fp64(case2, gpu_available):
+    from tf_pwa_amd import FCN
+
+    conf = case2["conf"]
+    data = conf.data_from_p4(case2["data"], weight=case2["wd"])
+    mc = conf.data_from_p4(case2["mc"])
+    f64 = FCN(conf.get_model(), data, mc)
+    mix = FCN(conf.get_model(), data, mc, precision="mixed")
+    n1, g1 = f64.nll_grad()
+    n2, g2 = mix.nll_grad()
+    assert abs(n1 - n2) < 1e-3
+    assert np.linalg.norm(g1 - g2) < 1e-4 * np.linalg.norm(g1)

@@ -143,6 +143,8 @@ struct tpa_events_s {
   double* d_out = nullptr;
   double2* d_coef = nullptr;
   double* d_f = nullptr;
+  double* d_gpart = nullptr;
+  double* d_gout = nullptr;
   hipEvent_t t0 = nullptr, t1 = nullptr;
 };
 
@@ -286,6 +288,8 @@ int tpa_events_destroy(tpa_events_t ev) {
   if (ev->d_out) (void)hipFree(ev->d_out);
   if (ev->d_coef) (void)hipFree(ev->d_coef);
   if (ev->d_f) (void)hipFree(ev->d_f);
+  if (ev->d_gpart) (void)hipFree(ev->d_gpart);
+  if (ev->d_gout) (void)hipFree(ev->d_gout);
   if (ev->t0) (void)hipEventDestroy(ev->t0);
   if (ev->t1) (void)hipEventDestroy(ev->t1);
   delete ev;
@@ -440,6 +444,83 @@ int tpa_eval(tpa_events_t ev, const double* coef, int mode, int flags, double* o
     g_timing_ms += ms;
     g_timing_n += 1;
   }
+  return TPA_OK;
+}
+
+int tpa_gram(tpa_events_t ev, double* out, double* ms) {
+  if (!ev || !out) return fail(TPA_ERR_ARG, "null pointer");
+  if (!ev->d_h) return fail(TPA_ERR_ARG, "partial-wave cache not built");
+  const TpaModel& M = ev->m->h;
+  if (M.P > 32) return fail(TPA_ERR_UNSUPPORTED, "the MFMA Gram supports up to 32 partial waves");
+  hipStream_t s = ev->m->stream;
+  const int64_t rows = ev->n * (int64_t)M.lam_pad;
+  const int grid = (int)std::min<int64_t>((rows + TPA_GRAM_ROWS - 1) / TPA_GRAM_ROWS, 1024);
+  if (!ev->d_gpart) HIPCHK(hipMalloc(&ev->d_gpart, sizeof(double) * 1024 * TPA_GRAM_SLOTS));
+  if (!ev->d_gout) HIPCHK(hipMalloc(&ev->d_gout, sizeof(double) * TPA_GRAM_SLOTS));
+  HIPCHK(hipEventRecord(ev->t0, s));
+  hipLaunchKernelGGL(k_gram_f32, dim3(grid), dim3(256), 0, s, ev->d_h, ev->d_w, ev->n, M.lam_pad,
+                     M.lam, M.P, ev->d_gpart);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipEventRecord(ev->t1, s));
+  hipLaunchKernelGGL(k_reduce_slots, dim3(TPA_GRAM_SLOTS), dim3(256), 0, s, ev->d_gpart, grid,
+                     TPA_GRAM_SLOTS, ev->d_gout);
+  HIPCHK(hipGetLastError());
+  std::vector<double> raw(TPA_GRAM_SLOTS);
+  HIPCHK(hipMemcpyAsync(raw.data(), ev->d_gout, sizeof(double) * TPA_GRAM_SLOTS,
+                        hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+  if (ms) {
+    float f = 0.0f;
+    HIPCHK(hipEventElapsedTime(&f, ev->t0, ev->t1));
+    *ms = f;
+  }
+  // decode the 32x32 MFMA accumulator layout: element (i, j) of tile q sits in lane
+  // j + 32 * ((i / 4) % 2), accumulator register (i % 4) + 4 * (i / 8)
+  auto G = [&](int q, int i, int j) {
+    const int lane = j + 32 * ((i / 4) % 2);
+    const int v = (i % 4) + 4 * (i / 8);
+    return raw[(q * 16 + v) * 64 + lane];
+  };
+  const int P = M.P;
+  for (int k = 0; k < P; ++k)
+    for (int l = 0; l < P; ++l) {
+      out[2 * (k * P + l)] = G(0, k, l) + G(2, k, l);
+      out[2 * (k * P + l) + 1] = G(1, k, l) - G(1, l, k);
+    }
+  return TPA_OK;
+}
+
+int tpa_device_synchronize(void) {
+  HIPCHK(hipDeviceSynchronize());
+  return TPA_OK;
+}
+
+int tpa_calib_copy(int64_t bytes, int reps, double* ms) {
+  if (bytes < 16 || reps < 1) return fail(TPA_ERR_ARG, "bad copy size");
+  const int64_t n = bytes / 16;
+  double2 *a = nullptr, *b = nullptr;
+  hipEvent_t t0, t1;
+  HIPCHK(hipMalloc(&a, n * 16));
+  if (hipMalloc(&b, n * 16) != hipSuccess) {
+    (void)hipFree(a);
+    return fail(TPA_ERR_HIP, "copy buffer allocation failed");
+  }
+  HIPCHK(hipMemset(a, 0, n * 16));
+  HIPCHK(hipEventCreate(&t0));
+  HIPCHK(hipEventCreate(&t1));
+  HIPCHK(hipEventRecord(t0, 0));
+  for (int r = 0; r < reps; ++r)
+    hipLaunchKernelGGL(k_copy, dim3(4096), dim3(256), 0, 0, a, b, n);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipEventRecord(t1, 0));
+  HIPCHK(hipEventSynchronize(t1));
+  float f = 0.0f;
+  HIPCHK(hipEventElapsedTime(&f, t0, t1));
+  if (ms) *ms = f / reps;
+  (void)hipEventDestroy(t0);
+  (void)hipEventDestroy(t1);
+  (void)hipFree(a);
+  (void)hipFree(b);
   return TPA_OK;
 }
 

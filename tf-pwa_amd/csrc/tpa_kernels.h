@@ -484,3 +484,121 @@ __global__ __launch_bounds__(256) void k_reduce(const double* __restrict__ parti
   }
   if (t == 0) out[j] = (j == 1) ? 0.0 : ss[0] + cc[0];
 }
+
+// Plain streaming copy (16-byte loads and stores), the calibration kernel for HBM byte
+// counters: it moves exactly n * 16 bytes in and n * 16 bytes out.
+__global__ __launch_bounds__(256) void k_copy(const double2* __restrict__ src,
+                                              double2* __restrict__ dst, int64_t n) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
+    dst[i] = src[i];
+}
+
+// ------------------------------------------------------------- MFMA Gram (MC normalisation)
+// G = sum_rows w_row Z_row Z_row^T with Z_row = [Re h_k, Im h_k] over rows = (event, lambda),
+// from which I_kl = sum w conj(h_k) h_l = (Gxx + Gyy)_kl + i (Gxy_kl - Gxy_lk)
+// (the cached_int matrix of tf_pwa/experimental/opt_int.py:30-52). Rows are staged through
+// LDS in blocks of 256 (each k a coalesced 4 KB read), converted to FP32, and each wave
+// accumulates 64 rows with v_mfma_f32_32x32x2f32 into three 32x32 tiles (xx, xy, yy) in FP32,
+// then folds them into FP64 accumulators after every block. Requires P <= 32.
+typedef float tpa_floatx16 __attribute__((ext_vector_type(16)));
+
+#define TPA_GRAM_ROWS 256
+#define TPA_GRAM_LDS_STRIDE (TPA_GRAM_ROWS + 1)
+#define TPA_GRAM_SLOTS (3 * 16 * 64)
+
+__global__ __launch_bounds__(256) void k_gram_f32(const double2* __restrict__ h,
+                                                  const double* __restrict__ w, int64_t N,
+                                                  int lam_pad, int lam, int P,
+                                                  double* __restrict__ partials) {
+  __shared__ float Xs[32 * TPA_GRAM_LDS_STRIDE];
+  __shared__ float Ys[32 * TPA_GRAM_LDS_STRIDE];
+  __shared__ float Ws[TPA_GRAM_ROWS];
+  const int t = threadIdx.x;
+  const int lane = t & 63;
+  const int wave = t >> 6;
+  const int64_t R = N * lam_pad;
+  double acc[3][16];
+#pragma unroll
+  for (int q = 0; q < 3; ++q)
+#pragma unroll
+    for (int v = 0; v < 16; ++v) acc[q][v] = 0.0;
+  for (int64_t rb = (int64_t)blockIdx.x * TPA_GRAM_ROWS; rb < R;
+       rb += (int64_t)gridDim.x * TPA_GRAM_ROWS) {
+    const int64_t row = rb + t;
+    float wr = 0.0f;
+    if (row < R) {
+      const int64_t e = row / lam_pad;
+      const int l = (int)(row - e * lam_pad);
+      wr = l < lam ? (float)w[e] : 0.0f;
+    }
+    Ws[t] = wr;
+    for (int k = 0; k < 32; ++k) {
+      double2 v = make_double2(0.0, 0.0);
+      if (k < P && row < R) v = h[(int64_t)k * R + row];
+      Xs[k * TPA_GRAM_LDS_STRIDE + t] = (float)v.x;
+      Ys[k * TPA_GRAM_LDS_STRIDE + t] = (float)v.y;
+    }
+    __syncthreads();
+    tpa_floatx16 cxx, cxy, cyy;
+#pragma unroll
+    for (int v = 0; v < 16; ++v) cxx[v] = cxy[v] = cyy[v] = 0.0f;
+    const int i = lane & 31;
+    for (int s = 0; s < 32; ++s) {
+      const int rl = wave * 64 + 2 * s + (lane >> 5);
+      const float x = Xs[i * TPA_GRAM_LDS_STRIDE + rl];
+      const float y = Ys[i * TPA_GRAM_LDS_STRIDE + rl];
+      const float ww = Ws[rl];
+      cxx = __builtin_amdgcn_mfma_f32_32x32x2f32(x, ww * x, cxx, 0, 0, 0);
+      cxy = __builtin_amdgcn_mfma_f32_32x32x2f32(x, ww * y, cxy, 0, 0, 0);
+      cyy = __builtin_amdgcn_mfma_f32_32x32x2f32(y, ww * y, cyy, 0, 0, 0);
+    }
+#pragma unroll
+    for (int v = 0; v < 16; ++v) {
+      acc[0][v] += (double)cxx[v];
+      acc[1][v] += (double)cxy[v];
+      acc[2][v] += (double)cyy[v];
+    }
+    __syncthreads();
+  }
+  // block sum of the four waves in a fixed order through LDS (reusing the staging area)
+  double* red = reinterpret_cast<double*>(Xs);  // 3*16*64 doubles = 24 KB < sizeof(Xs)
+  for (int wv = 0; wv < 4; ++wv) {
+    if (wave == wv) {
+#pragma unroll
+      for (int q = 0; q < 3; ++q)
+#pragma unroll
+        for (int v = 0; v < 16; ++v) {
+          const int slot = (q * 16 + v) * 64 + lane;
+          red[slot] = (wv == 0) ? acc[q][v] : red[slot] + acc[q][v];
+        }
+    }
+    __syncthreads();
+  }
+  double* out = partials + (int64_t)blockIdx.x * TPA_GRAM_SLOTS;
+  for (int j = t; j < TPA_GRAM_SLOTS; j += blockDim.x) out[j] = red[j];
+}
+
+// Compensated fixed-order sum of every slot over the blocks (one block per slot).
+__global__ __launch_bounds__(256) void k_reduce_slots(const double* __restrict__ partials,
+                                                      int nblocks, int nslots,
+                                                      double* __restrict__ out) {
+  const int j = blockIdx.x;
+  const int t = threadIdx.x;
+  __shared__ double ss[256], cc[256];
+  double s = 0.0, c = 0.0;
+  for (int b = t; b < nblocks; b += 256) neumaier_add(s, c, partials[(int64_t)b * nslots + j]);
+  ss[t] = s;
+  cc[t] = c;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (t < w) {
+      double s1 = ss[t], c1 = cc[t];
+      neumaier_merge(s1, c1, ss[t + w], cc[t + w]);
+      ss[t] = s1;
+      cc[t] = c1;
+    }
+    __syncthreads();
+  }
+  if (t == 0) out[j] = ss[0] + cc[0];
+}

@@ -112,6 +112,9 @@ def lib():
         "tpa_timing_enable": ([ctypes.c_int], ctypes.c_int),
         "tpa_timing_read": ([dp, ctypes.POINTER(i64)], ctypes.c_int),
         "tpa_timing_reset": ([], ctypes.c_int),
+        "tpa_device_synchronize": ([], ctypes.c_int),
+        "tpa_gram": ([vp, dp, dp], ctypes.c_int),
+        "tpa_calib_copy": ([i64, ctypes.c_int, dp], ctypes.c_int),
     }
     for name, (args, res) in sig.items():
         fn = getattr(L, name)
@@ -144,5 +147,6 @@ def exported_symbols():
         "tpa_model_destroy", "tpa_model_update", "tpa_events_create", "tpa_events_destroy",
         "tpa_events_size", "tpa_events_set_weight", "tpa_events_get_angles",
         "tpa_events_build", "tpa_events_get_pw", "tpa_eval", "tpa_timing_enable",
-        "tpa_timing_read", "tpa_timing_reset",
+        "tpa_timing_read", "tpa_timing_reset", "tpa_device_synchronize", "tpa_calib_copy",
+        "tpa_gram",
     ]

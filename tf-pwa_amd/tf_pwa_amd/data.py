@@ -51,6 +51,7 @@ class EventSet:
                                        self.n, _lib.dptr(self.weight), ctypes.byref(h)))
         self._h = h
         self._dev_weight = self.weight.copy()
+        self._weight_version = 0
         self._state = None
 
     def __len__(self):
@@ -72,6 +73,7 @@ class EventSet:
         if not np.array_equal(w, self._dev_weight):
             _lib.check(_lib.lib().tpa_events_set_weight(self._h, _lib.dptr(w)))
             self._dev_weight = w.copy()
+            self._weight_version += 1
         self._src_weight = src
 
     def angles(self):
@@ -105,6 +107,21 @@ class EventSet:
                                        _lib.dptr(f)))
         G = out[2:].view(np.complex128).copy() if grad else None
         return out[0], G, f
+
+    def gram(self):
+        """Partial-wave Gram matrix I_kl = sum_e w_e sum_lambda conj(h_k) h_l (FP32 MFMA with
+        FP64 refolds), cached until the partial waves or the weights change."""
+        self.ensure_built()
+        key = (self._state, self._weight_version)
+        if getattr(self, "_gram_key", None) != key:
+            P = self.amp.P
+            out = np.zeros(2 * P * P, dtype=np.float64)
+            ms = ctypes.c_double(0)
+            _lib.check(_lib.lib().tpa_gram(self._h, _lib.dptr(out), ctypes.byref(ms)))
+            self._gram = out.view(np.complex128).reshape(P, P).copy()
+            self._gram_key = key
+            self.gram_ms = ms.value
+        return self._gram
 
     def amp2(self, amp=None):
         _, _, f = self.eval(self.amp.coefficients(), _lib.MODE_MC, fout=True)

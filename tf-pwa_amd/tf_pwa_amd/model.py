@@ -64,7 +64,14 @@ def _as_sets(x):
 class Model:
     """Model.nll / nll_grad / nll_grad_batch / nll_grad_hessian (model.py:576-790)."""
 
-    def __init__(self, amp, w_bkg=1.0, resolution_size=1, extended=False, **kwargs):
+    def __init__(self, amp, w_bkg=1.0, resolution_size=1, extended=False, precision="fp64",
+                 **kwargs):
+        """precision: "fp64" evaluates every MC event per call; "mixed" replaces the MC pass
+        by the partial-wave Gram matrix built once on MFMA (valid while the partial waves,
+        i.e. lineshape parameters and MC weights, are unchanged; rebuilt otherwise)."""
+        if precision not in ("fp64", "mixed"):
+            raise ValueError("precision must be 'fp64' or 'mixed'")
+        self.precision = precision
         if resolution_size != 1:
             raise NotImplementedError("resolution folding is not supported")
         if extended:
@@ -111,7 +118,13 @@ class Model:
         value, G = 0.0, np.zeros(self.Amp.P, dtype=np.complex128) if grad else None
         for s, w in zip(sets, weights):
             s.set_weight(w)
-            v, g, _ = s.eval(coef, mode, grad=grad)
+            if mode == _lib.MODE_MC and self.precision == "mixed":
+                I = s.gram()
+                Ic = I @ coef
+                v = float(np.real(np.vdot(coef, Ic)))
+                g = np.conj(Ic)
+            else:
+                v, g, _ = s.eval(coef, mode, grad=grad)
             value += v
             if grad:
                 G += g
@@ -197,7 +210,10 @@ class FCN:
     ``fcn(x)``, ``fcn.nll_grad(x)`` -> (float, ndarray in vm.trainable_vars order),
     ``fcn.nll_grad_hessian(x)``, ``fcn.grad_hessp(x, p)``."""
 
-    def __init__(self, model, data, mcdata, bg=None, batch=65000, inmc=None, gauss_constr=None):
+    def __init__(self, model, data, mcdata, bg=None, batch=65000, inmc=None, gauss_constr=None,
+                 precision=None):
+        if precision is not None:
+            model.precision = precision
         if inmc is not None:
             raise NotImplementedError("inmc is not supported")
         self.model = model
