@@ -484,7 +484,9 @@ int tpa_gram(tpa_events_t ev, double* out, double* ms) {
   const int P = M.P;
   for (int k = 0; k < P; ++k)
     for (int l = 0; l < P; ++l) {
-      out[2 * (k * P + l)] = G(0, k, l) + G(2, k, l);
+      // the two accumulator positions of a symmetric entry round differently in FP32;
+      // their mean makes I exactly Hermitian
+      out[2 * (k * P + l)] = 0.5 * ((G(0, k, l) + G(2, k, l)) + (G(0, l, k) + G(2, l, k)));
       out[2 * (k * P + l) + 1] = G(1, k, l) - G(1, l, k);
     }
   return TPA_OK;
