@@ -111,8 +111,9 @@ int tpa_timing_reset(void);
  * the weighted MC sum at any coefficients is c^dagger I c, and its adjoint G = conj(I c).
  * Replaces build_int_matrix of tf-pwa's cached-integral model
  * (tf_pwa/experimental/opt_int.py:30-52, tf_pwa/model/opt_int.py:128-296). P <= 32.
+ * w_mean: the mean event weight (the kernel works with w / w_mean in FP32).
  * ms (optional): the MFMA kernel's duration. */
-int tpa_gram(tpa_events_t ev, double* out, double* ms);
+int tpa_gram(tpa_events_t ev, double w_mean, double* out, double* ms);
 
 /* Wait for all device work of the process (the timing barrier of benchmarks). */
 int tpa_device_synchronize(void);

@@ -142,3 +142,15 @@ def test_phase_space_generator_invariants():
     m23sq = (ps[1][:, 0] + ps[2][:, 0]) ** 2 - ((ps[1][:, 1:] + ps[2][:, 1:]) ** 2).sum(1)
     assert m23sq.min() > (0.494 + 0.140) ** 2 - 1e-12
     assert m23sq.max() < (2.286 - 0.938) ** 2 + 1e-12
+
+
+def test_grad_from_adjoint_matches_jacobian():
+    from tf_pwa_amd import ConfigLoader
+
+    c = ConfigLoader(config_path("lc_pkpi_6res.yml"), seed=9)
+    amp = c.get_amplitude()
+    rng = np.random.default_rng(2)
+    G = rng.normal(size=amp.P) + 1j * rng.normal(size=amp.P)
+    J = amp.jacobian()
+    np.testing.assert_allclose(amp.grad_from_adjoint(G), 2 * np.real(J @ G), rtol=1e-12,
+                               atol=1e-12)

@@ -447,8 +447,10 @@ int tpa_eval(tpa_events_t ev, const double* coef, int mode, int flags, double* o
   return TPA_OK;
 }
 
-int tpa_gram(tpa_events_t ev, double* out, double* ms) {
+int tpa_gram(tpa_events_t ev, double w_mean, double* out, double* ms) {
   if (!ev || !out) return fail(TPA_ERR_ARG, "null pointer");
+  if (!(w_mean > 0.0) && !(w_mean < 0.0)) return fail(TPA_ERR_ARG, "w_mean must be nonzero");
+  const double wscale = 1.0 / w_mean;
   if (!ev->d_h) return fail(TPA_ERR_ARG, "partial-wave cache not built");
   const TpaModel& M = ev->m->h;
   if (M.P > 32) return fail(TPA_ERR_UNSUPPORTED, "the MFMA Gram supports up to 32 partial waves");
@@ -458,8 +460,8 @@ int tpa_gram(tpa_events_t ev, double* out, double* ms) {
   if (!ev->d_gpart) HIPCHK(hipMalloc(&ev->d_gpart, sizeof(double) * 1024 * TPA_GRAM_SLOTS));
   if (!ev->d_gout) HIPCHK(hipMalloc(&ev->d_gout, sizeof(double) * TPA_GRAM_SLOTS));
   HIPCHK(hipEventRecord(ev->t0, s));
-  hipLaunchKernelGGL(k_gram_f32, dim3(grid), dim3(256), 0, s, ev->d_h, ev->d_w, ev->n, M.lam_pad,
-                     M.lam, M.P, ev->d_gpart);
+  hipLaunchKernelGGL(k_gram_f32, dim3(grid), dim3(256), 0, s, ev->d_h, ev->d_w, wscale, ev->n,
+                     M.lam_pad, M.lam, M.P, ev->d_gpart);
   HIPCHK(hipGetLastError());
   HIPCHK(hipEventRecord(ev->t1, s));
   hipLaunchKernelGGL(k_reduce_slots, dim3(TPA_GRAM_SLOTS), dim3(256), 0, s, ev->d_gpart, grid,
@@ -486,8 +488,9 @@ int tpa_gram(tpa_events_t ev, double* out, double* ms) {
     for (int l = 0; l < P; ++l) {
       // the two accumulator positions of a symmetric entry round differently in FP32;
       // their mean makes I exactly Hermitian
-      out[2 * (k * P + l)] = 0.5 * ((G(0, k, l) + G(2, k, l)) + (G(0, l, k) + G(2, l, k)));
-      out[2 * (k * P + l) + 1] = G(1, k, l) - G(1, l, k);
+      out[2 * (k * P + l)] =
+          w_mean * (0.5 * ((G(0, k, l) + G(2, k, l)) + (G(0, l, k) + G(2, l, k))));
+      out[2 * (k * P + l) + 1] = w_mean * (G(1, k, l) - G(1, l, k));
     }
   return TPA_OK;
 }

@@ -508,8 +508,8 @@ typedef float tpa_floatx16 __attribute__((ext_vector_type(16)));
 #define TPA_GRAM_SLOTS (3 * 16 * 64)
 
 __global__ __launch_bounds__(256) void k_gram_f32(const double2* __restrict__ h,
-                                                  const double* __restrict__ w, int64_t N,
-                                                  int lam_pad, int lam, int P,
+                                                  const double* __restrict__ w, double wscale,
+                                                  int64_t N, int lam_pad, int lam, int P,
                                                   double* __restrict__ partials) {
   __shared__ float Xs[32 * TPA_GRAM_LDS_STRIDE];
   __shared__ float Ys[32 * TPA_GRAM_LDS_STRIDE];
@@ -530,14 +530,19 @@ __global__ __launch_bounds__(256) void k_gram_f32(const double2* __restrict__ h,
     if (row < R) {
       const int64_t e = row / lam_pad;
       const int l = (int)(row - e * lam_pad);
-      wr = l < lam ? (float)w[e] : 0.0f;
+      // weights enter relative to their mean (wscale = 1/mean), so a uniform MC weight is
+      // exactly 1.0f and its FP32 rounding cannot scale the whole matrix
+      wr = l < lam ? (float)(w[e] * wscale) : 0.0f;
     }
     Ws[t] = wr;
+    double2 hv[32];
+#pragma unroll
+    for (int k = 0; k < 32; ++k)
+      hv[k] = (k < P && row < R) ? h[(int64_t)k * R + row] : make_double2(0.0, 0.0);
+#pragma unroll
     for (int k = 0; k < 32; ++k) {
-      double2 v = make_double2(0.0, 0.0);
-      if (k < P && row < R) v = h[(int64_t)k * R + row];
-      Xs[k * TPA_GRAM_LDS_STRIDE + t] = (float)v.x;
-      Ys[k * TPA_GRAM_LDS_STRIDE + t] = (float)v.y;
+      Xs[k * TPA_GRAM_LDS_STRIDE + t] = (float)hv[k].x;
+      Ys[k * TPA_GRAM_LDS_STRIDE + t] = (float)hv[k].y;
     }
     __syncthreads();
     tpa_floatx16 cxx, cxy, cyy;

@@ -113,7 +113,7 @@ def lib():
         "tpa_timing_read": ([dp, ctypes.POINTER(i64)], ctypes.c_int),
         "tpa_timing_reset": ([], ctypes.c_int),
         "tpa_device_synchronize": ([], ctypes.c_int),
-        "tpa_gram": ([vp, dp, dp], ctypes.c_int),
+        "tpa_gram": ([vp, ctypes.c_double, dp, dp], ctypes.c_int),
         "tpa_calib_copy": ([i64, ctypes.c_int, dp], ctypes.c_int),
     }
     for name, (args, res) in sig.items():

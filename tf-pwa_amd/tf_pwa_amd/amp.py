@@ -248,7 +248,15 @@ class AmplitudeModel:
         return M
 
     def build_state(self, fold=True):
-        """Everything the partial-wave cache depends on besides the events."""
+        """Everything the partial-wave cache depends on besides the events (cached by the
+        parameter store's version)."""
+        if getattr(self, "_bs_version", None) == (self.vm.version, fold):
+            return self._bs
+        self._bs = self._build_state(fold)
+        self._bs_version = (self.vm.version, fold)
+        return self._bs
+
+    def _build_state(self, fold):
         bc = self.barrier_constants()
         key = tuple((tuple(b["act1"]), tuple(b["act2"])) for b in bc)
         if fold:
@@ -365,6 +373,47 @@ class AmplitudeModel:
                 m = rr >= 0
                 np.add.at(J, (rr[m], ks[m]), d[f][m] * rest[m])
         return J
+
+    def _grad_rows(self):
+        vm = self.vm
+        key = tuple(vm.trainable_vars)
+        if getattr(self, "_rows_key", None) != key:
+            index = {n: i for i, n in enumerate(vm.trainable_vars)}
+            for nm in self._names():
+                for k in ("mass", "width"):
+                    if nm[k] in index:
+                        raise NotImplementedError(
+                            "gradients with respect to floating masses/widths use the shape path")
+            fx = self._factor_index()
+            self._rows = (np.array([index.get(n + "r", -1) for n in fx["cnames"]]),
+                          np.array([index.get(n + "i", -1) for n in fx["cnames"]]))
+            self._rows_key = key
+        return self._rows
+
+    def grad_from_adjoint(self, G):
+        """2 Re sum_k (d c_k / d theta) G_k for every trainable theta, without forming the
+        Jacobian (the chain rule through the three complex factors of each c_k)."""
+        row_r, row_i = self._grad_rows()
+        fx = self._factor_index()
+        val, d_r, d_i = self._factor_values()
+        C = self._barrier_per_wave()
+        slots = (fx["it"], fx["i1"], fx["i2"])
+        n = len(self.vm.trainable_vars)
+        out = np.zeros(n)
+        CG = C * np.asarray(G)
+        for a in range(3):
+            rest = CG
+            for b in range(3):
+                if b != a:
+                    rest = rest * val[slots[b]]
+            f = slots[a]
+            for rows, d in ((row_r, d_r), (row_i, d_i)):
+                rr = rows[f]
+                m = rr >= 0
+                if m.any():
+                    out += np.bincount(rr[m], weights=2.0 * np.real(d[f][m] * rest[m]),
+                                       minlength=n)
+        return out
 
     # ---------------------------------------------------------------- parameters
     def set_params(self, var):

@@ -117,7 +117,8 @@ class EventSet:
             P = self.amp.P
             out = np.zeros(2 * P * P, dtype=np.float64)
             ms = ctypes.c_double(0)
-            _lib.check(_lib.lib().tpa_gram(self._h, _lib.dptr(out), ctypes.byref(ms)))
+            _lib.check(_lib.lib().tpa_gram(self._h, float(np.mean(self._dev_weight)),
+                                           _lib.dptr(out), ctypes.byref(ms)))
             self._gram = out.view(np.complex128).reshape(P, P).copy()
             self._gram_key = key
             self.gram_ms = ms.value

@@ -152,8 +152,7 @@ class Model:
         nll = -ln + sw * np.log(int_mc)
         if not grad:
             return nll, None
-        J = amp.jacobian()
-        g = -2.0 * np.real(J @ Gd) + sw * 2.0 * np.real(J @ Gm) / int_mc
+        g = amp.grad_from_adjoint(-Gd + (sw / int_mc) * Gm)
         return nll, g
 
     def nll(self, data, mcdata, weight=1.0, batch=None, bg=None, mc_weight=1.0):

@@ -57,6 +57,9 @@ class VarsManager:
         self.complex_vars = {}
         self.bnd_dic = {}
         self.init_val = {}
+        # bumped on every change of a value or of the trainable set, so derived quantities
+        # (coefficients, lineshape state) can be cached by version
+        self.version = 0
 
     # -------------------------------------------------------------- creation
     def add_real_var(self, name, value=None, range_=None, trainable=True, rng=None):
@@ -70,6 +73,7 @@ class VarsManager:
         self.init_val[name] = float(value)
         if trainable:
             self.trainable_vars.append(name)
+        self.version += 1
 
     def add_complex_var(self, name, polar=None, trainable=True, fix_vals=(1.0, 0.0), rng=None):
         polar = self.polar if polar is None else polar
@@ -87,6 +91,7 @@ class VarsManager:
 
     def set_fix(self, name, value=None, unfix=False):
         """variable.py:415-440: fixing removes from trainable_vars, freeing appends."""
+        self.version += 1
         if value is not None:
             self.variables[name] = float(value)
         if unfix:
@@ -105,6 +110,7 @@ class VarsManager:
 
     def set(self, name, value):
         self.variables[name] = float(value)
+        self.version += 1
 
     def get_all_val(self):
         return [self.variables[n] for n in self.trainable_vars]
@@ -116,6 +122,7 @@ class VarsManager:
 
     def set_all(self, vals):
         """dict: by name; sequence: in trainable_vars order (variable.py:684-697)."""
+        self.version += 1
         if isinstance(vals, dict):
             for k, v in vals.items():
                 if k in self.variables:
@@ -169,6 +176,7 @@ class VarsManager:
 
     def std_polar_all(self):
         """r >= 0 and phi in [-pi, pi) for every polar pair (variable.py:716-741)."""
+        self.version += 1
         for name, polar in self.complex_vars.items():
             if not polar:
                 continue
