@@ -26,6 +26,18 @@ sys.path.insert(0, ROOT)
 CONFIG = os.path.join(ROOT, "tf-pwa_amd", "tf_pwa_amd", "configs", "lc_pkpi_6res.yml")
 METRIC = "NLL+grad evals/s @1M data+10M MC, |dNLL|<1e-3 vs FP64; toy-fit p50/p99 ms"
 HBM_PEAK_GBS = 8000.0
+TRAFFIC = os.path.join(ROOT, "profiles", "k_eval_fold_traffic.json")
+
+
+def measured_traffic():
+    """HBM bytes per k_eval_fold launch from the committed FETCH_SIZE / WRITE_SIZE profile
+    of this workload (counter values already scaled by the copy-kernel calibration)."""
+    try:
+        with open(TRAFFIC) as f:
+            t = json.load(f)
+        return {k: t[k]["read_bytes"] + t[k]["write_bytes"] for k in ("data_1M", "mc_10M")}
+    except (OSError, KeyError, ValueError):
+        return None
 
 
 def log(*a):
@@ -156,12 +168,18 @@ def main():
 
     bytes_per_event = amp.P * amp.lam_pad * 16 + 8
 
+    traffic = measured_traffic() if (args.n_data, args.n_mc) == (1_000_000, 10_000_000) else None
+
     def roofline(r, n_events_per_step):
         kernel_s = r["kernel_ms"] / 1e3
         bytes_total = n_events_per_step * bytes_per_event * args.steps
         achieved = bytes_total / kernel_s / 1e9 if kernel_s > 0 else 0.0
+        tr = None
+        if traffic is not None:
+            tr = traffic["data_1M"] if n_events_per_step == args.n_data else 0.5 * (
+                traffic["data_1M"] + traffic["mc_10M"])
         return {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": "k_eval_fold",
+                "frac": achieved / HBM_PEAK_GBS, "traffic": tr, "kernel": "k_eval_fold",
                 "launches": r["launches"], "kernel_ms_avg": r["kernel_ms"] / max(1, r["launches"]),
                 "bytes_per_event": bytes_per_event,
                 "bytes_per_launch": n_events_per_step * bytes_per_event / max(

@@ -500,7 +500,7 @@ __global__ __launch_bounds__(256) void k_copy(const double2* __restrict__ src,
 // (the cached_int matrix of tf_pwa/experimental/opt_int.py:30-52). Rows are staged through
 // LDS in blocks of 256 (each k a coalesced 4 KB read), converted to FP32, and each wave
 // accumulates 64 rows with v_mfma_f32_32x32x2f32 into three 32x32 tiles (xx, xy, yy) in FP32,
-// then folds them into FP64 accumulators after every block. Requires P <= 32.
+// folding them into FP64 accumulators every 16 rows. Requires P <= 32.
 typedef float tpa_floatx16 __attribute__((ext_vector_type(16)));
 
 #define TPA_GRAM_ROWS 256
@@ -545,24 +545,29 @@ __global__ __launch_bounds__(256) void k_gram_f32(const double2* __restrict__ h,
       Ys[k * TPA_GRAM_LDS_STRIDE + t] = (float)hv[k].y;
     }
     __syncthreads();
-    tpa_floatx16 cxx, cxy, cyy;
-#pragma unroll
-    for (int v = 0; v < 16; ++v) cxx[v] = cxy[v] = cyy[v] = 0.0f;
     const int i = lane & 31;
-    for (int s = 0; s < 32; ++s) {
-      const int rl = wave * 64 + 2 * s + (lane >> 5);
-      const float x = Xs[i * TPA_GRAM_LDS_STRIDE + rl];
-      const float y = Ys[i * TPA_GRAM_LDS_STRIDE + rl];
-      const float ww = Ws[rl];
-      cxx = __builtin_amdgcn_mfma_f32_32x32x2f32(x, ww * x, cxx, 0, 0, 0);
-      cxy = __builtin_amdgcn_mfma_f32_32x32x2f32(x, ww * y, cxy, 0, 0, 0);
-      cyy = __builtin_amdgcn_mfma_f32_32x32x2f32(y, ww * y, cyy, 0, 0, 0);
-    }
+    // FP32 accumulation runs over 16 rows (8 MFMA steps) before each FP64 refold, which keeps
+    // the FP32 rounding of the partial sums at the level of single products
+    for (int s0 = 0; s0 < 32; s0 += 8) {
+      tpa_floatx16 cxx, cxy, cyy;
 #pragma unroll
-    for (int v = 0; v < 16; ++v) {
-      acc[0][v] += (double)cxx[v];
-      acc[1][v] += (double)cxy[v];
-      acc[2][v] += (double)cyy[v];
+      for (int v = 0; v < 16; ++v) cxx[v] = cxy[v] = cyy[v] = 0.0f;
+#pragma unroll
+      for (int s = s0; s < s0 + 8; ++s) {
+        const int rl = wave * 64 + 2 * s + (lane >> 5);
+        const float x = Xs[i * TPA_GRAM_LDS_STRIDE + rl];
+        const float y = Ys[i * TPA_GRAM_LDS_STRIDE + rl];
+        const float ww = Ws[rl];
+        cxx = __builtin_amdgcn_mfma_f32_32x32x2f32(x, ww * x, cxx, 0, 0, 0);
+        cxy = __builtin_amdgcn_mfma_f32_32x32x2f32(x, ww * y, cxy, 0, 0, 0);
+        cyy = __builtin_amdgcn_mfma_f32_32x32x2f32(y, ww * y, cyy, 0, 0, 0);
+      }
+#pragma unroll
+      for (int v = 0; v < 16; ++v) {
+        acc[0][v] += (double)cxx[v];
+        acc[1][v] += (double)cxy[v];
+        acc[2][v] += (double)cyy[v];
+      }
     }
     __syncthreads();
   }
