@@ -154,3 +154,39 @@ def test_grad_from_adjoint_matches_jacobian():
     J = amp.jacobian()
     np.testing.assert_allclose(amp.grad_from_adjoint(G), 2 * np.real(J @ G), rtol=1e-12,
                                atol=1e-12)
+
+
+def test_cached_fg_patches_nan_gradient():
+    from tf_pwa_amd.fit import CachedFG
+
+    def fg(x):
+        g = 2 * x
+        g[1] = np.nan
+        return float(np.sum(x * x)), g
+
+    c = CachedFG(fg)
+    f, g = c(np.array([1.0, 2.0]))
+    assert abs(g[1] - 4.0) < 1e-5 and g[0] == 2.0
+    assert c(np.array([1.0, 2.0])) is c.fg  # cached
+
+
+def test_fit_scipy_on_quadratic_with_bounds():
+    from tf_pwa_amd.fit import fit_scipy
+    from tf_pwa_amd.variable import VarsManager
+
+    vm = VarsManager()
+    vm.add_real_var("a", 0.3)
+    vm.add_real_var("b", 0.8)
+    vm.set_bound({"b": (0.0, 1.0)})
+
+    class F:
+        def __init__(self):
+            self.vm = vm
+
+        def nll_grad(self, y):
+            a, b = y
+            return (a - 0.7) ** 2 + (b - 0.25) ** 2, np.array([2 * (a - 0.7), 2 * (b - 0.25)])
+
+    r = fit_scipy(F())
+    assert abs(r.params["a"] - 0.7) < 1e-5 and abs(r.params["b"] - 0.25) < 1e-5
+    assert r.min_nll < 1e-9
