@@ -36,6 +36,11 @@ typedef struct tpa_events_s* tpa_events_t;
 /* Flags (tpa_eval). */
 #define TPA_FLAG_GRAD 1 /* also return the adjoint sums G_k */
 #define TPA_FLAG_FOUT 2 /* also return the per-event f_e */
+/* Lineshapes evaluated per event at the model's current (m0, g0) instead of being folded
+ * into the cache (cache built with fold = 0); with TPA_FLAG_GRAD the output then also holds
+ * Dm_k = sum_e u_e dB/dm0 H_k at out[2+2P+2k..] and Dg_k (dB/dg0) at out[2+4P+2k..],
+ * H_k = sum_lambda conj(A_lambda) h_k,lambda, so out needs 2+6P doubles. */
+#define TPA_FLAG_SHAPE 4
 
 /* Version of this ABI. */
 int tpa_abi_version(void);
@@ -92,7 +97,8 @@ int tpa_events_get_pw(tpa_events_t ev, double* out, int64_t out_len);
  * (coef: host [2P] interleaved re/im): out[0] = the weighted sum of the mode; with
  * TPA_FLAG_GRAD out[2+2k], out[3+2k] = Re/Im of G_k = sum_e u_e sum_lambda conj(A) h_k
  * where u_e is the derivative weight of the mode, so that d(sum)/d theta =
- * 2 Re sum_k (d c_k / d theta) G_k. out needs 2+2P doubles. With TPA_FLAG_FOUT, f_out
+ * 2 Re sum_k (d c_k / d theta) G_k. out needs 2+2P doubles (2+6P with TPA_FLAG_SHAPE).
+ * The cache must be built with fold = 1 (fold = 0 for TPA_FLAG_SHAPE). With TPA_FLAG_FOUT, f_out
  * (host [n]) receives f_e = sum_lambda |A_lambda|^2.
  * Replaces sum_gradient / _batch_sum (tf_pwa/model/model.py:41-94) as called from
  * BaseModel.nll_grad_batch (407-449) and BaseModel.nll (322-339). */

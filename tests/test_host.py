@@ -190,3 +190,29 @@ def test_fit_scipy_on_quadratic_with_bounds():
     r = fit_scipy(F())
     assert abs(r.params["a"] - 0.7) < 1e-5 and abs(r.params["b"] - 0.25) < 1e-5
     assert r.min_nll < 1e-9
+
+
+def test_barrier_constant_mass_derivatives():
+    import yaml
+
+    from tf_pwa_amd import ConfigLoader
+
+    cfg = yaml.safe_load(open(config_path("lc_pkpi_6res.yml")))
+    for r in ("Kst", "K0st", "L1520", "D1232", "L1405", "L1820"):
+        cfg["particle"][r]["float"] = "mg"
+    c = ConfigLoader(cfg, seed=1)
+    amp = c.get_amplitude()
+    assert amp.shape_mode
+    bc = amp.barrier_constants()
+    for ci, ch in enumerate(amp.chains):
+        name = ch["R"].name + "_mass"
+        m0 = c.vm.get(name)
+        h = 1e-6
+        c.vm.set(name, m0 + h)
+        bp = amp.barrier_constants()[ci]
+        c.vm.set(name, m0 - h)
+        bm = amp.barrier_constants()[ci]
+        c.vm.set(name, m0)
+        for key, dkey in (("c1", "dlog1"), ("c2", "dlog2")):
+            fd = (np.log(bp[key]) - np.log(bm[key])) / (2 * h)
+            np.testing.assert_allclose(bc[ci][dkey], fd, atol=1e-6, rtol=1e-6)

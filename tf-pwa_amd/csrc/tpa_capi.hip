@@ -129,6 +129,7 @@ struct tpa_model_s {
   TpaModel* d = nullptr;
   double* d_cg = nullptr;
   double* d_dw = nullptr;
+  int* d_kchain = nullptr;
   hipStream_t stream = nullptr;
 };
 
@@ -145,6 +146,8 @@ struct tpa_events_s {
   double* d_f = nullptr;
   double* d_gpart = nullptr;
   double* d_gout = nullptr;
+  double2* d_bw = nullptr;
+  int built_fold = -1;
   hipEvent_t t0 = nullptr, t1 = nullptr;
 };
 
@@ -202,6 +205,36 @@ static int dispatch_eval(int lg, int kg, const EvalArgs& a, int grid, hipStream_
   }
 }
 
+template <int LG, int KG>
+static void launch_shape(const EvalShapeArgs& a, int grid, hipStream_t s) {
+  hipLaunchKernelGGL((k_eval_shape<LG, KG, 8>), dim3(grid), dim3(256), 0, s, a);
+}
+
+template <int LG>
+static int dispatch_shape_kg(int kg, const EvalShapeArgs& a, int grid, hipStream_t s) {
+  switch (kg) {
+    case 1: launch_shape<LG, 1>(a, grid, s); return TPA_OK;
+    case 2: launch_shape<LG, 2>(a, grid, s); return TPA_OK;
+    case 4: launch_shape<LG, 4>(a, grid, s); return TPA_OK;
+    case 8: if (LG * 8 <= 64) { launch_shape<LG, (LG * 8 <= 64 ? 8 : 1)>(a, grid, s); return TPA_OK; }
+            return TPA_ERR_UNSUPPORTED;
+    case 16: if (LG * 16 <= 64) { launch_shape<LG, (LG * 16 <= 64 ? 16 : 1)>(a, grid, s); return TPA_OK; }
+             return TPA_ERR_UNSUPPORTED;
+    default: return TPA_ERR_UNSUPPORTED;
+  }
+}
+
+static int dispatch_shape(int lg, int kg, const EvalShapeArgs& a, int grid, hipStream_t s) {
+  switch (lg) {
+    case 1: return dispatch_shape_kg<1>(kg, a, grid, s);
+    case 2: return dispatch_shape_kg<2>(kg, a, grid, s);
+    case 4: return dispatch_shape_kg<4>(kg, a, grid, s);
+    case 8: return dispatch_shape_kg<8>(kg, a, grid, s);
+    case 16: return dispatch_shape_kg<16>(kg, a, grid, s);
+    default: return TPA_ERR_UNSUPPORTED;
+  }
+}
+
 static int eval_grid(int64_t n, int lanes_per_event) {
   const int64_t epb = 4 * (64 / lanes_per_event);
   int64_t g = (n + epb - 1) / epb;
@@ -250,6 +283,16 @@ int tpa_model_create(const void* desc, int64_t desc_bytes, const double* cg, int
     tpa_model_destroy(m);
     return fail(TPA_ERR_HIP, std::string("model upload: ") + hipGetErrorString(e));
   }
+  std::vector<int> kchain(m->h.P);
+  for (int c = 0; c < m->h.n_chain; ++c)
+    for (int k = 0; k < m->h.chain[c].n_ls1 * m->h.chain[c].n_ls2; ++k)
+      kchain[m->h.chain[c].k0 + k] = c;
+  if ((e = hipMalloc(&m->d_kchain, sizeof(int) * m->h.P)) != hipSuccess ||
+      (e = hipMemcpy(m->d_kchain, kchain.data(), sizeof(int) * m->h.P, hipMemcpyHostToDevice)) !=
+          hipSuccess) {
+    tpa_model_destroy(m);
+    return fail(TPA_ERR_HIP, std::string("model upload: ") + hipGetErrorString(e));
+  }
   *out = m;
   return TPA_OK;
 }
@@ -259,6 +302,7 @@ int tpa_model_destroy(tpa_model_t m) {
   if (m->d) (void)hipFree(m->d);
   if (m->d_cg) (void)hipFree(m->d_cg);
   if (m->d_dw) (void)hipFree(m->d_dw);
+  if (m->d_kchain) (void)hipFree(m->d_kchain);
   if (m->stream) (void)hipStreamDestroy(m->stream);
   delete m;
   return TPA_OK;
@@ -290,6 +334,7 @@ int tpa_events_destroy(tpa_events_t ev) {
   if (ev->d_f) (void)hipFree(ev->d_f);
   if (ev->d_gpart) (void)hipFree(ev->d_gpart);
   if (ev->d_gout) (void)hipFree(ev->d_gout);
+  if (ev->d_bw) (void)hipFree(ev->d_bw);
   if (ev->t0) (void)hipEventDestroy(ev->t0);
   if (ev->t1) (void)hipEventDestroy(ev->t1);
   delete ev;
@@ -316,10 +361,10 @@ int tpa_events_create(tpa_model_t m, const double* p4, int64_t n, const double* 
   if ((e = hipMalloc(&ev->d_ang, sizeof(double) * ev->nf * n)) != hipSuccess)
     return bail(e, "alloc angles");
   if ((e = hipMalloc(&ev->d_w, sizeof(double) * n)) != hipSuccess) return bail(e, "alloc w");
-  if ((e = hipMalloc(&ev->d_part, sizeof(double) * kEvalGridMax * (2 + 2 * TPA_MAX_P))) !=
+  if ((e = hipMalloc(&ev->d_part, sizeof(double) * kEvalGridMax * TPA_NACC_SHAPE_MAX)) !=
       hipSuccess)
     return bail(e, "alloc partials");
-  if ((e = hipMalloc(&ev->d_out, sizeof(double) * (2 + 2 * TPA_MAX_P))) != hipSuccess)
+  if ((e = hipMalloc(&ev->d_out, sizeof(double) * TPA_NACC_SHAPE_MAX)) != hipSuccess)
     return bail(e, "alloc out");
   if ((e = hipMalloc(&ev->d_coef, sizeof(double2) * std::max(P, 1))) != hipSuccess)
     return bail(e, "alloc coef");
@@ -380,6 +425,7 @@ int tpa_events_build(tpa_events_t ev, int fold) {
                      ev->d_ang, ev->n, ev->m->d_cg, ev->m->d_dw, fold, ev->d_h);
   HIPCHK(hipGetLastError());
   HIPCHK(hipStreamSynchronize(ev->m->stream));
+  ev->built_fold = fold ? 1 : 0;
   return TPA_OK;
 }
 
@@ -416,7 +462,12 @@ int tpa_eval(tpa_events_t ev, const double* coef, int mode, int flags, double* o
   a.P = M.P;
   a.mode = mode;
   a.grad = (flags & TPA_FLAG_GRAD) ? 1 : 0;
-  a.nacc = 2 + 2 * M.P;
+  const bool shape = (flags & TPA_FLAG_SHAPE) != 0;
+  if (shape && ev->built_fold != 0)
+    return fail(TPA_ERR_ARG, "shape evaluation needs a cache built without lineshapes (fold=0)");
+  if (!shape && ev->built_fold != 1)
+    return fail(TPA_ERR_ARG, "evaluation without TPA_FLAG_SHAPE needs a folded cache (fold=1)");
+  a.nacc = shape ? 2 + 6 * M.P : 2 + 2 * M.P;
   const int kg = eval_kgroups(M.lam_pad, M.P);
   if (kg < 0) return fail(TPA_ERR_UNSUPPORTED, "too many partial waves for this helicity count");
   const int grid = eval_grid(ev->n, M.lam_pad * kg);
@@ -425,8 +476,26 @@ int tpa_eval(tpa_events_t ev, const double* coef, int mode, int flags, double* o
     std::lock_guard<std::mutex> lk(g_timing_mu);
     timing = g_timing;
   }
+  if (shape) {
+    if (!ev->d_bw)
+      HIPCHK(hipMalloc(&ev->d_bw, sizeof(double2) * 3 * (size_t)M.n_chain * ev->n));
+    const int64_t lgrid = (ev->n + 255) / 256;
+    hipLaunchKernelGGL(k_lineshape, dim3((unsigned)lgrid), dim3(256), 0, s, ev->m->d, ev->d_ang,
+                       ev->n, ev->d_bw);
+    HIPCHK(hipGetLastError());
+  }
   if (timing) HIPCHK(hipEventRecord(ev->t0, s));
-  int rc = dispatch_eval(M.lam_pad, kg, a, grid, s);
+  int rc;
+  if (shape) {
+    EvalShapeArgs sa;
+    sa.a = a;
+    sa.bw = ev->d_bw;
+    sa.kchain = ev->m->d_kchain;
+    sa.n_chain = M.n_chain;
+    rc = dispatch_shape(M.lam_pad, kg, sa, grid, s);
+  } else {
+    rc = dispatch_eval(M.lam_pad, kg, a, grid, s);
+  }
   if (rc != TPA_OK) return fail(rc, "no evaluation kernel for this helicity/partial-wave count");
   HIPCHK(hipGetLastError());
   if (timing) HIPCHK(hipEventRecord(ev->t1, s));

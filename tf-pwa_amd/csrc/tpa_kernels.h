@@ -612,3 +612,187 @@ __global__ __launch_bounds__(256) void k_reduce_slots(const double* __restrict__
   }
   if (t == 0) out[j] = ss[0] + cc[0];
 }
+
+// ------------------------------------------------- lineshapes for floating masses / widths
+// Per chain and event: B, dB/dm0, dB/dg0 at the chain's current (m0, g0), stored as
+// bw[(q * n_chain + c) * N + e] for q = 0, 1, 2.
+__global__ __launch_bounds__(256) void k_lineshape(const TpaModel* __restrict__ M,
+                                                   const double* __restrict__ ang, int64_t N,
+                                                   double2* __restrict__ bw) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= N) return;
+  const int nt = M->n_topo;
+  const int nc = M->n_chain;
+  const int64_t gbase = (int64_t)TPA_NF_TOPO * nt;
+  double mf[3];
+  for (int f = 0; f < 3; ++f) mf[f] = ang[(gbase + 1 + f) * N + e];
+  for (int c = 0; c < nc; ++c) {
+    const TpaChain& C = M->chain[c];
+    const TpaTopo T = M->topo[C.topo];
+    const double mR = ang[(int64_t)(TPA_NF_TOPO * C.topo + TPA_F_MR) * N + e];
+    const int fy = C.r_y_first ? T.y : T.z;
+    const int fz = C.r_y_first ? T.z : T.y;
+    const double q = rel_p(mR, mf[fy], mf[fz]);
+    cd B, dBm, dBg;
+    lineshape_grad(C.shape, mR, C.m0, C.g0, q, C.m_y, C.m_z, C.bw_l, C.d, &B, &dBm, &dBg);
+    bw[(int64_t)(0 * nc + c) * N + e] = make_double2(B.r, B.i);
+    bw[(int64_t)(1 * nc + c) * N + e] = make_double2(dBm.r, dBm.i);
+    bw[(int64_t)(2 * nc + c) * N + e] = make_double2(dBg.r, dBg.i);
+  }
+}
+
+// Evaluation with per-event lineshapes (partial-wave cache built without them):
+// A = sum_k c_k B_c(k) h_k. Besides G_k = sum u B H_k it returns Dm_k = sum u dB/dm0 H_k and
+// Dg_k = sum u dB/dg0 H_k with H_k = sum_lambda conj(A_lambda) h_k,lambda, from which the host
+// forms the mass and width gradients.
+struct EvalShapeArgs {
+  EvalArgs a;
+  const double2* __restrict__ bw;  // [3][n_chain][N]
+  const int* __restrict__ kchain;  // [P] chain of each partial wave
+  int n_chain;
+};
+
+#define TPA_NACC_SHAPE_MAX (2 + 6 * TPA_MAX_P)
+
+template <int LG, int KG, int PML>
+__global__ __launch_bounds__(256) void k_eval_shape(EvalShapeArgs s) {
+  const EvalArgs& a = s.a;
+  constexpr int LPE = LG * KG;
+  constexpr int EPW = 64 / LPE;
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  const int lam = lane & (LG - 1);
+  const int kg = (lane / LG) & (KG - 1);
+  const int el = lane / LPE;
+  const double eps = 1.0e-6;
+  const double logeps = log(1.0e-6);
+  double vs = 0.0, vc = 0.0;
+  double gr[PML], gi[PML], mr[PML], mi[PML], wr[PML], wi[PML], cr[PML], ci[PML];
+  int kc[PML];
+#pragma unroll
+  for (int j = 0; j < PML; ++j) {
+    const int k = kg + KG * j;
+    gr[j] = gi[j] = mr[j] = mi[j] = wr[j] = wi[j] = 0.0;
+    const double2 c = k < a.P ? a.coef[k] : make_double2(0.0, 0.0);
+    cr[j] = c.x;
+    ci[j] = c.y;
+    kc[j] = k < a.P ? s.kchain[k] : 0;
+  }
+  const int64_t stride = (int64_t)gridDim.x * 4 * EPW;
+  for (int64_t base = ((int64_t)blockIdx.x * 4 + wave) * EPW; base < a.N; base += stride) {
+    const int64_t e = base + el;
+    const bool valid = e < a.N;
+    double2 hv[PML], bv[PML];
+#pragma unroll
+    for (int j = 0; j < PML; ++j) {
+      const int k = kg + KG * j;
+      const bool ld = valid && k < a.P;
+      hv[j] = ld ? a.h[((int64_t)k * a.N + e) * LG + lam] : make_double2(0.0, 0.0);
+      bv[j] = ld ? s.bw[(int64_t)kc[j] * a.N + e] : make_double2(0.0, 0.0);
+    }
+    double ar = 0.0, ai = 0.0;
+#pragma unroll
+    for (int j = 0; j < PML; ++j) {
+      const double cbr = cr[j] * bv[j].x - ci[j] * bv[j].y;
+      const double cbi = cr[j] * bv[j].y + ci[j] * bv[j].x;
+      ar += cbr * hv[j].x - cbi * hv[j].y;
+      ai += cbr * hv[j].y + cbi * hv[j].x;
+    }
+#pragma unroll
+    for (int m = LG; m < LPE; m <<= 1) {
+      ar += __shfl_xor(ar, m, 64);
+      ai += __shfl_xor(ai, m, 64);
+    }
+    double f = ar * ar + ai * ai;
+#pragma unroll
+    for (int m = 1; m < LG; m <<= 1) f += __shfl_xor(f, m, 64);
+    double y = 0.0, u = 0.0;
+    if (valid) {
+      const double w = a.w[e];
+      if (a.mode == 0) {
+        const double dom = (w == 0.0) ? 1.0 : w;
+        const double x = (f * w) / dom;
+        double cl, dcl;
+        if (x > eps) {
+          cl = log(x);
+          dcl = 1.0 / x;
+        } else {
+          const double dx = x - eps;
+          const double t = dx / eps;
+          cl = (logeps + t) - (t * t) / 2.0;
+          dcl = 1.0 / eps - t / eps;
+        }
+        y = w * cl;
+        u = (w * dcl) * (w / dom);
+      } else {
+        y = w * f;
+        u = w;
+      }
+      if (a.f_out != nullptr && lam == 0 && kg == 0) a.f_out[e] = f;
+    }
+    if (lam == 0 && kg == 0) neumaier_add(vs, vc, y);
+    if (a.grad) {
+#pragma unroll
+      for (int j = 0; j < PML; ++j) {
+        const int k = kg + KG * j;
+        if (!(valid && k < a.P)) continue;
+        // u * conj(A) h_k
+        const double hr = u * (ar * hv[j].x + ai * hv[j].y);
+        const double hi = u * (ar * hv[j].y - ai * hv[j].x);
+        gr[j] += bv[j].x * hr - bv[j].y * hi;
+        gi[j] += bv[j].x * hi + bv[j].y * hr;
+        const double2 dm = s.bw[(int64_t)(s.n_chain + kc[j]) * a.N + e];
+        const double2 dg = s.bw[(int64_t)(2 * s.n_chain + kc[j]) * a.N + e];
+        mr[j] += dm.x * hr - dm.y * hi;
+        mi[j] += dm.x * hi + dm.y * hr;
+        wr[j] += dg.x * hr - dg.y * hi;
+        wi[j] += dg.x * hi + dg.y * hr;
+      }
+    }
+  }
+#pragma unroll
+  for (int m = 32; m >= 1; m >>= 1) {
+    const double s2 = __shfl_xor(vs, m, 64);
+    const double c2 = __shfl_xor(vc, m, 64);
+    neumaier_merge(vs, vc, s2, c2);
+  }
+  __shared__ double sh[4][TPA_NACC_SHAPE_MAX];
+  if (a.grad) {
+#pragma unroll
+    for (int j = 0; j < PML; ++j) {
+      double x[6] = {gr[j], gi[j], mr[j], mi[j], wr[j], wi[j]};
+#pragma unroll
+      for (int q = 0; q < 6; ++q) {
+#pragma unroll
+        for (int m = 1; m < LG; m <<= 1) x[q] += __shfl_xor(x[q], m, 64);
+#pragma unroll
+        for (int m = LPE; m < 64; m <<= 1) x[q] += __shfl_xor(x[q], m, 64);
+      }
+      const int k = kg + KG * j;
+      if (el == 0 && lam == 0 && k < a.P) {
+        sh[wave][2 + 2 * k] = x[0];
+        sh[wave][3 + 2 * k] = x[1];
+        sh[wave][2 + 2 * a.P + 2 * k] = x[2];
+        sh[wave][3 + 2 * a.P + 2 * k] = x[3];
+        sh[wave][2 + 4 * a.P + 2 * k] = x[4];
+        sh[wave][3 + 4 * a.P + 2 * k] = x[5];
+      }
+    }
+  }
+  if (lane == 0) {
+    sh[wave][0] = vs;
+    sh[wave][1] = vc;
+  }
+  __syncthreads();
+  double* out = a.partials + (int64_t)blockIdx.x * a.nacc;
+  if (threadIdx.x == 0) {
+    double s0 = sh[0][0], c0 = sh[0][1];
+    for (int wv = 1; wv < 4; ++wv) neumaier_merge(s0, c0, sh[wv][0], sh[wv][1]);
+    out[0] = s0;
+    out[1] = c0;
+  }
+  if (a.grad) {
+    for (int j = 2 + threadIdx.x; j < a.nacc; j += blockDim.x)
+      out[j] = ((sh[0][j] + sh[1][j]) + sh[2][j]) + sh[3][j];
+  }
+}

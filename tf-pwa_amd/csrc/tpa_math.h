@@ -229,3 +229,54 @@ TPA_DEV cd lineshape(int shape, double m, double m0, double g0, double q, double
   double s = x * x + y * y;
   return cd{x / s, y / s};
 }
+
+// Lineshape with its derivatives in the pole mass m0 and width g0 (the reverse-mode
+// gradient of BWR / BW, tf_pwa/breit_wigner.py:56-91, 231-245, 290-306, with q0 from
+// get_relative_p's m_eff = max(m0, m1+m2) clamp, whose clamped branch has zero derivative).
+TPA_DEV void lineshape_grad(int shape, double m, double m0, double g0, double q, double my,
+                            double mz, int L, double d, cd* B, cd* dBm, cd* dBg) {
+  if (shape == 0) {
+    *B = cd{1.0, 0.0};
+    *dBm = *dBg = cd{0.0, 0.0};
+    return;
+  }
+  double gamma = g0, dgm = 0.0, dgg = 1.0;
+  if (shape == 2) {
+    const double s = my + mz, df = my - mz;
+    const double me = m0 > s ? m0 : s;
+    const double p = (me - s) * (me + s) * (me - df) * (me + df);
+    const double q0 = sqrt(p) / (2.0 * me);
+    double dq0 = 0.0;
+    if (m0 > s && p > 0.0) {
+      const double dp = 2.0 * me * ((me * me - df * df) + (me * me - s * s));
+      dq0 = dp / (4.0 * me * sqrt(p)) - q0 / me;
+    }
+    double qq0 = 1.0, dqq0 = 0.0;
+    if (q0 > 1.0e-15) {
+      qq0 = pow(q / q0, (double)(2 * L + 1));
+      dqq0 = -(2.0 * L + 1.0) * qq0 / q0 * dq0;
+    }
+    const double qd0 = q0 * d, qd = q * d;
+    const double p0 = bprime_poly(L, qd0 * qd0), pz = bprime_poly(L, qd * qd);
+    const double bp = sqrt(p0) / sqrt(pz);
+    const double bp2 = bp * bp;
+    const double dbp2 = bprime_poly_deriv(L, qd0 * qd0) * 2.0 * q0 * d * d * dq0 / pz;
+    const double mm0 = m0 / m;
+    gamma = g0 * qq0 * mm0 * bp2;
+    dgm = g0 * (dqq0 * mm0 * bp2 + qq0 * (1.0 / m) * bp2 + qq0 * mm0 * dbp2);
+    dgg = qq0 * mm0 * bp2;
+  } else {
+    dgm = 0.0;
+    dgg = 1.0;
+  }
+  const double x = m0 * m0 - m * m;
+  const double y = m0 * gamma;
+  const double ss = x * x + y * y;
+  *B = cd{x / ss, y / ss};
+  // B = 1 / D with D = x - i y; dB = -B^2 dD
+  const cd B2 = c_mul(*B, *B);
+  const cd dDm = cd{2.0 * m0, -(gamma + m0 * dgm)};
+  const cd dDg = cd{0.0, -(m0 * dgg)};
+  *dBm = c_neg(c_mul(B2, dDm));
+  *dBg = c_neg(c_mul(B2, dDg));
+}

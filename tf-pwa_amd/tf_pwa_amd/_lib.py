@@ -26,6 +26,7 @@ MODE_DATA = 0
 MODE_MC = 1
 FLAG_GRAD = 1
 FLAG_FOUT = 2
+FLAG_SHAPE = 4
 
 _i32 = ctypes.c_int32
 

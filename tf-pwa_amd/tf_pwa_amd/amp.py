@@ -58,6 +58,22 @@ def bprime_poly_deriv(l, z):
     return r
 
 
+def _rel_p2_d_parent(M, m1, m2):
+    """d rel_p2 / d M."""
+    s, d = m1 + m2, m1 - m2
+    F = (M * M - s * s) * (M * M - d * d)
+    dF = 2 * M * (M * M - d * d) + 2 * M * (M * M - s * s)
+    return dF / (4 * M * M) - F / (2 * M ** 3)
+
+
+def _rel_p2_d_daughter(M, m1, m2, which):
+    """d rel_p2 / d m1 (which = 0) or d m2 (which = 1)."""
+    s, d = m1 + m2, m1 - m2
+    sd = 1.0 if which == 0 else -1.0
+    dF = -2 * s * (M * M - d * d) - 2 * sd * d * (M * M - s * s)
+    return dF / (4 * M * M)
+
+
 class PartialWave:
     __slots__ = ("chain", "ls1", "ls2")
 
@@ -180,17 +196,25 @@ class AmplitudeModel:
             d = d1.d
             q01 = rel_p2(top.mass, pm(d1.outs[0]), pm(d1.outs[1]))
             q02 = rel_p2(m0, d2.outs[0].mass, d2.outs[1].mass)
-            c1, a1, c2, a2 = [], [], [], []
+            # d q0^2 / d m0: R is a daughter of vertex 1 and the parent of vertex 2
+            r_pos = 0 if d1.outs[0] is ch["R"] else 1
+            dq01 = _rel_p2_d_daughter(top.mass, pm(d1.outs[0]), pm(d1.outs[1]), r_pos)
+            dq02 = _rel_p2_d_parent(m0, d2.outs[0].mass, d2.outs[1].mass)
+            c1, a1, c2, a2, g1, g2 = [], [], [], [], [], []
             for l in d1.l_list:
                 p0 = bprime_poly(l, q01 * d * d)
                 a1.append(int(p0 > 0))
                 c1.append(np.sqrt(p0) if p0 > 0 else 1.0)
+                g1.append(bprime_poly_deriv(l, q01 * d * d) * d * d * dq01 / (2 * p0)
+                          if p0 > 0 else 0.0)
             for l in d2.l_list:
                 p0 = bprime_poly(l, q02 * d * d)
                 a2.append(int(p0 > 0))
                 c2.append(np.sqrt(p0) if p0 > 0 else 1.0)
+                g2.append(bprime_poly_deriv(l, q02 * d * d) * d * d * dq02 / (2 * p0)
+                          if p0 > 0 else 0.0)
             out.append(dict(c1=np.array(c1), c2=np.array(c2), act1=a1, act2=a2,
-                            q01=q01, q02=q02))
+                            q01=q01, q02=q02, dlog1=np.array(g1), dlog2=np.array(g2)))
         return out
 
     def describe(self, fold=True):
@@ -247,26 +271,32 @@ class AmplitudeModel:
             C.d = ch["d1"].d
         return M
 
-    def build_state(self, fold=True):
-        """Everything the partial-wave cache depends on besides the events (cached by the
-        parameter store's version)."""
-        if getattr(self, "_bs_version", None) == (self.vm.version, fold):
-            return self._bs
-        self._bs = self._build_state(fold)
-        self._bs_version = (self.vm.version, fold)
-        return self._bs
+    @property
+    def shape_mode(self):
+        """True when a mass or width is trainable: the cache is then built without the
+        lineshapes, which are evaluated per event with their derivatives."""
+        tv = set(self.vm.trainable_vars)
+        return any(nm["mass"] in tv or nm["width"] in tv for nm in self._names())
 
-    def _build_state(self, fold):
+    def build_state(self):
+        """(cache state, description state), cached by the parameter store's version.
+        The cache depends on the barrier clamp flags and, when lineshapes are folded, on the
+        lineshape parameters; the device description always carries the latter."""
+        if getattr(self, "_bs_version", None) == self.vm.version:
+            return self._bs
         bc = self.barrier_constants()
-        key = tuple((tuple(b["act1"]), tuple(b["act2"])) for b in bc)
-        if fold:
-            key = key + tuple(self.shape_params())
-        return key
+        flags = tuple((tuple(b["act1"]), tuple(b["act2"])) for b in bc)
+        shapes = tuple(self.shape_params())
+        fold = not self.shape_mode
+        cache = (fold, flags, shapes if fold else None)
+        self._bs = (cache, (flags, shapes))
+        self._bs_version = self.vm.version
+        return self._bs
 
     def device(self):
         """The engine model handle (one per amplitude model); its description is refreshed
         when the lineshape parameters or barrier clamp flags change."""
-        state = self.build_state()
+        cache, desc = self.build_state()
         L = _lib.lib()
         if self._dev is None:
             M = self.describe()
@@ -274,11 +304,12 @@ class AmplitudeModel:
             _lib.check(L.tpa_model_create(ctypes.byref(M), ctypes.sizeof(M),
                                           _lib.dptr(self.cg), len(self.cg), ctypes.byref(h)))
             self._dev = h
-            self._dev_state = state
-        elif self._dev_state != state:
+            self._desc_state = desc
+        elif self._desc_state != desc:
             M = self.describe()
             _lib.check(L.tpa_model_update(self._dev, ctypes.byref(M), ctypes.sizeof(M)))
-            self._dev_state = state
+            self._desc_state = desc
+        self._dev_state = cache
         return self._dev
 
     def __del__(self):
@@ -379,11 +410,6 @@ class AmplitudeModel:
         key = tuple(vm.trainable_vars)
         if getattr(self, "_rows_key", None) != key:
             index = {n: i for i, n in enumerate(vm.trainable_vars)}
-            for nm in self._names():
-                for k in ("mass", "width"):
-                    if nm[k] in index:
-                        raise NotImplementedError(
-                            "gradients with respect to floating masses/widths use the shape path")
             fx = self._factor_index()
             self._rows = (np.array([index.get(n + "r", -1) for n in fx["cnames"]]),
                           np.array([index.get(n + "i", -1) for n in fx["cnames"]]))
@@ -413,6 +439,28 @@ class AmplitudeModel:
                 if m.any():
                     out += np.bincount(rr[m], weights=2.0 * np.real(d[f][m] * rest[m]),
                                        minlength=n)
+        return out
+
+    def shape_grad(self, G, Dm, Dg):
+        """Gradient of the lineshape parameters: for the resonance of chain c,
+        d/dm0 = 2 Re sum_{k in c} [(d c_k/d m0) G_k + c_k Dm_k] (the barrier constants of both
+        vertices depend on m0) and d/dg0 = 2 Re sum_{k in c} c_k Dg_k."""
+        vm = self.vm
+        index = {n: i for i, n in enumerate(vm.trainable_vars)}
+        out = np.zeros(len(vm.trainable_vars))
+        c = self.coefficients()
+        bc = self.barrier_constants()
+        fx = self._factor_index()
+        names = self._names()
+        dlog = np.array([bc[ch]["dlog1"][a] + bc[ch]["dlog2"][b]
+                         for ch, a, b in zip(fx["ch"], fx["s1"], fx["s2"])])
+        for ci, nm in enumerate(names):
+            ks = fx["ch"] == ci
+            if nm["mass"] in index:
+                out[index[nm["mass"]]] += 2.0 * np.real(
+                    np.sum(c[ks] * dlog[ks] * G[ks]) + np.sum(c[ks] * Dm[ks]))
+            if nm["width"] in index:
+                out[index[nm["width"]]] += 2.0 * np.real(np.sum(c[ks] * Dg[ks]))
         return out
 
     # ---------------------------------------------------------------- parameters

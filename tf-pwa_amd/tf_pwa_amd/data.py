@@ -84,10 +84,11 @@ class EventSet:
 
     def ensure_built(self):
         self.amp.device()
-        if self._state == self.amp._dev_state:
+        state = self.amp._dev_state
+        if self._state == state:
             return
-        _lib.check(_lib.lib().tpa_events_build(self._h, 1))
-        self._state = self.amp._dev_state
+        _lib.check(_lib.lib().tpa_events_build(self._h, 1 if state[0] else 0))
+        self._state = state
 
     def partial_waves(self):
         self.ensure_built()
@@ -96,16 +97,22 @@ class EventSet:
         return out.view(np.complex128).reshape(self.amp.P, self.n, self.amp.lam_pad)
 
     def eval(self, coef, mode, grad=False, fout=False):
-        """One pass: (value, G or None, f or None)."""
+        """One pass: (value, G or None, f or None). When the model floats lineshape
+        parameters, G is the tuple (G, Dm, Dg) of coupling and lineshape adjoints."""
         self.ensure_built()
         P = self.amp.P
+        shape = not self._state[0]
         c = np.ascontiguousarray(np.asarray(coef, dtype=np.complex128)).view(np.float64)
-        out = np.zeros(2 + 2 * P, dtype=np.float64)
+        out = np.zeros(2 + (6 if shape else 2) * P, dtype=np.float64)
         f = np.empty(self.n, dtype=np.float64) if fout else None
-        flags = (_lib.FLAG_GRAD if grad else 0) | (_lib.FLAG_FOUT if fout else 0)
+        flags = ((_lib.FLAG_GRAD if grad else 0) | (_lib.FLAG_FOUT if fout else 0)
+                 | (_lib.FLAG_SHAPE if shape else 0))
         _lib.check(_lib.lib().tpa_eval(self._h, _lib.dptr(c), mode, flags, _lib.dptr(out),
                                        _lib.dptr(f)))
-        G = out[2:].view(np.complex128).copy() if grad else None
+        G = None
+        if grad:
+            z = out[2:].view(np.complex128)
+            G = (z[:P].copy(), z[P:2 * P].copy(), z[2 * P:].copy()) if shape else z.copy()
         return out[0], G, f
 
     def gram(self):

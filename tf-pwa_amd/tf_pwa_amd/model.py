@@ -115,19 +115,27 @@ class Model:
         return sets, w / w.sum()
 
     def _pass(self, sets, weights, coef, mode, grad):
-        value, G = 0.0, np.zeros(self.Amp.P, dtype=np.complex128) if grad else None
+        """Sum of one mode over event sets: value and adjoints (G, or (G, Dm, Dg) when the
+        lineshapes float)."""
+        shape = self.Amp.shape_mode
+        P = self.Amp.P
+        value = 0.0
+        G = [np.zeros(P, dtype=np.complex128) for _ in range(3 if shape else 1)] if grad else None
         for s, w in zip(sets, weights):
             s.set_weight(w)
-            if mode == _lib.MODE_MC and self.precision == "mixed":
+            if mode == _lib.MODE_MC and self.precision == "mixed" and not shape:
                 I = s.gram()
                 Ic = I @ coef
                 v = float(np.real(np.vdot(coef, Ic)))
-                g = np.conj(Ic)
+                g = (np.conj(Ic),)
             else:
                 v, g, _ = s.eval(coef, mode, grad=grad)
+                if grad and not shape:
+                    g = (g,)
             value += v
             if grad:
-                G += g
+                for acc, x in zip(G, g):
+                    acc += x
         return value, G
 
     @staticmethod
@@ -152,7 +160,10 @@ class Model:
         nll = -ln + sw * np.log(int_mc)
         if not grad:
             return nll, None
-        g = amp.grad_from_adjoint(-Gd + (sw / int_mc) * Gm)
+        comb = [-a + (sw / int_mc) * b for a, b in zip(Gd, Gm)]
+        g = amp.grad_from_adjoint(comb[0])
+        if len(comb) == 3:
+            g = g + amp.shape_grad(*comb)
         return nll, g
 
     def nll(self, data, mcdata, weight=1.0, batch=None, bg=None, mc_weight=1.0):
