@@ -23,6 +23,8 @@ extern "C" {
 
 typedef struct tpa_model_s* tpa_model_t;
 typedef struct tpa_events_s* tpa_events_t;
+typedef struct tpa_pool_s* tpa_pool_t;
+typedef struct tpa_server_s* tpa_server_t;
 
 #define TPA_OK 0
 #define TPA_ERR_ARG -1
@@ -128,6 +130,102 @@ int tpa_device_synchronize(void);
  * plain 16-byte streaming kernel (k_copy) and return the average milliseconds per copy.
  * It is the calibration kernel for the FETCH_SIZE / WRITE_SIZE counters. */
 int tpa_calib_copy(int64_t bytes, int reps, double* ms);
+
+/* ---- Many fits on one GPU (the serving path).
+ * A pool is one device event store (angles, fold-1 partial-wave cache, weights) that holds the
+ * data samples of many concurrent fits. Each fit owns a contiguous range. One call of
+ * tpa_pool_eval evaluates the data sum and its adjoint for every pending request of every fit in
+ * a single batched launch (a hipGraph per grid size class, replayed), replacing one
+ * BaseModel.nll_grad_batch call per fit (tf_pwa/model/model.py:407-449, driven by
+ * fit_scipy's minimize loop, tf_pwa/fit.py:189-300) when many fits run at once.
+ * The MC normalisation of these fits is shared and comes from tpa_gram on an event set. */
+
+/* capacity: events the pool can hold; max_items: the most requests one tpa_pool_eval serves. */
+int tpa_pool_create(tpa_model_t model, int64_t capacity, int max_items, tpa_pool_t* out);
+int tpa_pool_destroy(tpa_pool_t pool);
+
+/* Upload n events (p4 host [n_final*4][n], weight host [n]) into a free range, compute their
+ * angles and partial-wave cache there, and return the range's first event in *offset.
+ * Fails with "pool full" when no free range is long enough (the caller's admission control). */
+int tpa_pool_add(tpa_pool_t pool, const double* p4, int64_t n, const double* weight,
+                 int64_t* offset);
+
+/* Return the range that starts at offset to the free list (merged with free neighbours). */
+int tpa_pool_release(tpa_pool_t pool, int64_t offset);
+
+/* Free events left in the pool. */
+int64_t tpa_pool_free_events(tpa_pool_t pool);
+
+/* Batched data sums: request i covers events [ranges[2i], ranges[2i] + ranges[2i+1]) of one
+ * allocation, with coefficients coefs[i*2P .. i*2P+2P) (interleaved re/im). out[i*(2+2P) + ...]
+ * has tpa_eval's layout for TPA_MODE_DATA with TPA_FLAG_GRAD: out[0] = sum w clip_log(f),
+ * out[2+2k], out[3+2k] = Re/Im G_k. Each request's result does not depend on the other
+ * requests of the batch (fixed per-request chunking and reduction order). mode must be
+ * TPA_MODE_DATA. ms (optional): the batched graph's duration. */
+int tpa_pool_eval(tpa_pool_t pool, int n_items, const int64_t* ranges, const double* coefs,
+                  int mode, double* out, double* ms);
+
+/* ---- The serving runtime: many BFGS fits at once over one pool.
+ * The coupling map says how the partial-wave coefficients follow from a fit's trainable
+ * parameters y (tf-pwa's trainable_vars order): c_k = wave_const[k] * v[f0] * v[f1] * v[f2]
+ * with f = wave_factor[3k..3k+2] (the chain total and the g_ls of both vertices,
+ * tf_pwa/amp/core.py:1000-1080) and v[f] = r e^{i phi} (factor_polar) or r + i phi, where r and
+ * phi are y[factor_var[2f]] and y[factor_var[2f+1]], or factor_fixed[2f], [2f+1] when the index
+ * is -1. bound_kind[j] (0 none, 1 [a, b], 2 lower a, 3 upper b, with a, b = bound_ab[2j],
+ * [2j+1]) is the bound transform of y[j] (tf_pwa/variable.py:1029-1135); BFGS runs in the
+ * unbounded variables. */
+typedef struct {
+  int32_t P;
+  int32_t n_factor;
+  int32_t n_var;
+  int32_t reserved;
+  const int32_t* wave_factor;   /* [P][3] */
+  const double* wave_const;     /* [P] */
+  const int32_t* factor_polar;  /* [n_factor] */
+  const int32_t* factor_var;    /* [n_factor][2] */
+  const double* factor_fixed;   /* [n_factor][2] */
+  const int32_t* bound_kind;    /* [n_var] */
+  const double* bound_ab;       /* [n_var][2] */
+} TpaCouplingMap;
+
+/* A server evaluates NLL = -sum_e w_e clip_log f_e + sw log(c^dagger I c) (Model.nll,
+ * tf_pwa/model/model.py:407-449) for every fit: the data sums of all pending requests go to the
+ * GPU in one tpa_pool_eval batch, the MC normalisation is the shared Gram matrix `gram`
+ * (host [P][P] complex, from tpa_gram on MC with unit-sum weights). Each fit minimises with
+ * BFGS on a host thread (strong-Wolfe line search, stop at max |g| <= gtol; maxiter <= 0 means
+ * 200 * n_var), like fit_scipy (tf_pwa/fit.py:189-300). The batcher launches when every fit is
+ * waiting, when max_batch requests are pending, or window_us after the first one arrived. */
+int tpa_server_create(tpa_pool_t pool, const TpaCouplingMap* map, const double* gram,
+                      double gtol, int maxiter, int max_batch, double window_us,
+                      tpa_server_t* out);
+
+/* Stops the batcher, cancels running fits (status 3) and joins every thread. */
+int tpa_server_destroy(tpa_server_t server);
+
+/* One NLL and gradient (in y) evaluation of the data range [offset, offset+n) with data weight
+ * sum sw, through the batcher (FCN.nll_grad, tf_pwa/model/model.py:1216-1240). */
+int tpa_server_nll_grad(tpa_server_t server, int64_t offset, int64_t n, double sw,
+                        const double* y, double* nll, double* grad);
+
+/* Start a fit on the data range [offset, offset+n) from parameters y0; returns its id. */
+int tpa_server_submit(tpa_server_t server, int64_t offset, int64_t n, double sw,
+                      const double* y0, int* fit_id);
+
+/* Ids of fits that ended since the last poll (at most max_ids). */
+int tpa_server_poll(tpa_server_t server, int* ids, int max_ids, int* n_done);
+
+/* Result of an ended fit: y (n_var), NLL at y, status (0 converged, 1 iteration limit, 2 line
+ * search failed, 3 cancelled, 4 engine error), evaluations and iterations. */
+int tpa_server_result(tpa_server_t server, int fit_id, double* y, double* nll, int* status,
+                      int64_t* nfev, int64_t* niter);
+
+/* Per-request latency (submit of the request to its result) of every request so far, in
+ * microseconds; *n receives the total count, at most max_n values are copied. */
+int tpa_server_latencies(tpa_server_t server, double* us, int64_t max_n, int64_t* n);
+
+/* Requests served, batches launched and summed batched-graph milliseconds. */
+int tpa_server_counters(tpa_server_t server, int64_t* requests, int64_t* batches,
+                        double* gpu_ms);
 
 #ifdef __cplusplus
 }

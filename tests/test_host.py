@@ -216,3 +216,29 @@ def test_barrier_constant_mass_derivatives():
         for key, dkey in (("c1", "dlog1"), ("c2", "dlog2")):
             fd = (np.log(bp[key]) - np.log(bm[key])) / (2 * h)
             np.testing.assert_allclose(bc[ci][dkey], fd, atol=1e-6, rtol=1e-6)
+
+
+@pytest.mark.parametrize("cfg", ["lc_pkpi_2res.yml", "lc_pkpi_6res.yml"])
+def test_serving_coupling_map_reproduces_coefficients(cfg):
+    """The native server rebuilds c_k from the coupling map; the map must give the model's
+    coefficients at any parameters, polar and cartesian factors alike."""
+    import yaml
+
+    from tf_pwa_amd import ConfigLoader
+    from tf_pwa_amd.serving import coupling_map, map_coefficients
+
+    for polar in (True, False):
+        conf = yaml.safe_load(open(config_path(cfg)))
+        conf["polar"] = polar
+        c = ConfigLoader(conf, seed=5)
+        amp = c.get_amplitude()
+        _, keep = coupling_map(amp)
+        rng = np.random.default_rng(3)
+        for _ in range(3):
+            y = np.array(c.vm.get_all_val()) + rng.normal(scale=0.3, size=len(c.vm.trainable_vars))
+            c.vm.set_all(y)
+            np.testing.assert_allclose(map_coefficients(keep, y), amp.coefficients(), rtol=1e-14,
+                                       atol=1e-14)
+        assert keep["factor_var"].max() < len(c.vm.trainable_vars)
+        # the fixed chain total of fix_chain_idx appears as a fixed factor
+        assert (keep["factor_var"] < 0).any()

@@ -7,11 +7,13 @@
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
+#include <map>
 #include <mutex>
 #include <string>
 #include <vector>
 
 #include "tfpwa_amd.h"
+#include "tpa_internal.h"
 #include "tpa_kernels.h"
 
 namespace {
@@ -242,6 +244,70 @@ static int eval_grid(int64_t n, int lanes_per_event) {
   return (int)std::min<int64_t>(g, kEvalGridMax);
 }
 
+struct tpa_pool_s {
+  std::mutex mu;  // one call at a time: the range maps, staging buffers and graphs are shared
+  tpa_model_s* m = nullptr;
+  int64_t cap = 0;
+  double* d_ang = nullptr;
+  double2* d_h = nullptr;
+  double* d_w = nullptr;
+  std::map<int64_t, int64_t> free_ranges;  // offset -> length
+  std::map<int64_t, int64_t> used_ranges;
+  int max_items = 0, max_chunks = 0;
+  int* d_chunk_item = nullptr;
+  int64_t* d_chunk_start = nullptr;
+  int64_t* d_chunk_end = nullptr;
+  int* d_counts = nullptr;
+  int* d_item_chunk = nullptr;
+  double2* d_coef = nullptr;
+  double* d_part = nullptr;
+  double* d_out = nullptr;
+  int* h_chunk_item = nullptr;
+  int64_t* h_chunk_start = nullptr;
+  int64_t* h_chunk_end = nullptr;
+  int* h_counts = nullptr;
+  int* h_item_chunk = nullptr;
+  double2* h_coef = nullptr;
+  double* h_out = nullptr;
+  std::map<int, hipGraphExec_t> graphs;  // grid bucket -> instantiated graph
+  hipEvent_t t0 = nullptr, t1 = nullptr;
+  hipStream_t stream = nullptr;  // the pool's own stream (graph capture never meets other work)
+};
+
+static void release_range(tpa_pool_s* p, int64_t offset);
+
+int tpa_set_error(int code, const std::string& msg) { return fail(code, msg); }
+
+template <int LG, int KG>
+static void launch_multi(const MultiArgs& a, int grid, hipStream_t s) {
+  hipLaunchKernelGGL((k_eval_multi<LG, KG, 8>), dim3(grid), dim3(256), 0, s, a);
+}
+
+template <int LG>
+static int dispatch_multi_kg(int kg, const MultiArgs& a, int grid, hipStream_t s) {
+  switch (kg) {
+    case 1: launch_multi<LG, 1>(a, grid, s); return TPA_OK;
+    case 2: launch_multi<LG, 2>(a, grid, s); return TPA_OK;
+    case 4: launch_multi<LG, 4>(a, grid, s); return TPA_OK;
+    case 8: if (LG * 8 <= 64) { launch_multi<LG, (LG * 8 <= 64 ? 8 : 1)>(a, grid, s); return TPA_OK; }
+            return TPA_ERR_UNSUPPORTED;
+    case 16: if (LG * 16 <= 64) { launch_multi<LG, (LG * 16 <= 64 ? 16 : 1)>(a, grid, s); return TPA_OK; }
+             return TPA_ERR_UNSUPPORTED;
+    default: return TPA_ERR_UNSUPPORTED;
+  }
+}
+
+static int dispatch_multi(int lg, int kg, const MultiArgs& a, int grid, hipStream_t s) {
+  switch (lg) {
+    case 1: return dispatch_multi_kg<1>(kg, a, grid, s);
+    case 2: return dispatch_multi_kg<2>(kg, a, grid, s);
+    case 4: return dispatch_multi_kg<4>(kg, a, grid, s);
+    case 8: return dispatch_multi_kg<8>(kg, a, grid, s);
+    case 16: return dispatch_multi_kg<16>(kg, a, grid, s);
+    default: return TPA_ERR_UNSUPPORTED;
+  }
+}
+
 extern "C" {
 
 int tpa_abi_version(void) { return 1; }
@@ -387,7 +453,7 @@ int tpa_events_create(tpa_model_t m, const double* p4, int64_t n, const double* 
   const int block = 256;
   const int64_t grid = (n + block - 1) / block;
   hipLaunchKernelGGL(k_cal_angle3, dim3((unsigned)grid), dim3(block), 0, m->stream, m->d, d_p4,
-                     n, ev->d_ang);
+                     n, (int64_t)0, n, ev->d_ang);
   if ((e = hipGetLastError()) != hipSuccess) return bail(e, "launch cal_angle");
   if ((e = hipStreamSynchronize(m->stream)) != hipSuccess) return bail(e, "cal_angle");
   (void)hipFree(d_p4);
@@ -422,7 +488,8 @@ int tpa_events_build(tpa_events_t ev, int fold) {
   const int block = 128;
   const int64_t grid = (ev->n + block - 1) / block;
   hipLaunchKernelGGL(k_build_pw, dim3((unsigned)grid), dim3(block), 0, ev->m->stream, ev->m->d,
-                     ev->d_ang, ev->n, ev->m->d_cg, ev->m->d_dw, fold, ev->d_h);
+                     ev->d_ang, ev->n, (int64_t)0, ev->n, ev->m->d_cg, ev->m->d_dw, fold,
+                     ev->d_h);
   HIPCHK(hipGetLastError());
   HIPCHK(hipStreamSynchronize(ev->m->stream));
   ev->built_fold = fold ? 1 : 0;
@@ -561,6 +628,232 @@ int tpa_gram(tpa_events_t ev, double w_mean, double* out, double* ms) {
           w_mean * (0.5 * ((G(0, k, l) + G(2, k, l)) + (G(0, l, k) + G(2, l, k))));
       out[2 * (k * P + l) + 1] = w_mean * (G(1, k, l) - G(1, l, k));
     }
+  return TPA_OK;
+}
+
+// ------------------------------------------------------------------------ serving pool
+int tpa_pool_destroy(tpa_pool_t p) {
+  if (!p) return TPA_OK;
+  for (auto& g : p->graphs) (void)hipGraphExecDestroy(g.second);
+  void* dev[] = {p->d_ang, p->d_h, p->d_w, p->d_chunk_item, p->d_chunk_start, p->d_chunk_end,
+                 p->d_counts, p->d_item_chunk, p->d_coef, p->d_part, p->d_out};
+  for (void* x : dev)
+    if (x) (void)hipFree(x);
+  void* host[] = {p->h_chunk_item, p->h_chunk_start, p->h_chunk_end, p->h_counts,
+                  p->h_item_chunk, p->h_coef, p->h_out};
+  for (void* x : host)
+    if (x) (void)hipHostFree(x);
+  if (p->t0) (void)hipEventDestroy(p->t0);
+  if (p->t1) (void)hipEventDestroy(p->t1);
+  if (p->stream) (void)hipStreamDestroy(p->stream);
+  delete p;
+  return TPA_OK;
+}
+
+int tpa_pool_create(tpa_model_t m, int64_t capacity, int max_items, tpa_pool_t* out) {
+  if (!m || !out) return fail(TPA_ERR_ARG, "null pointer");
+  if (capacity < 1 || max_items < 1) return fail(TPA_ERR_ARG, "bad pool size");
+  const TpaModel& M = m->h;
+  tpa_pool_s* p = new tpa_pool_s();
+  p->m = m;
+  p->cap = capacity;
+  p->max_items = max_items;
+  p->max_chunks = (int)std::min<int64_t>((capacity + TPA_CHUNK - 1) / TPA_CHUNK + max_items,
+                                         1 << 22);
+  const int nf = TPA_NF_TOPO * M.n_topo + 4;
+  const int nacc = 2 + 2 * M.P;
+  hipError_t e = hipSuccess;
+  auto D = [&](void** ptr, size_t bytes) {
+    if (e == hipSuccess) e = hipMalloc(ptr, bytes);
+  };
+  auto H = [&](void** ptr, size_t bytes) {
+    if (e == hipSuccess) e = hipHostMalloc(ptr, bytes, 0);
+  };
+  D((void**)&p->d_ang, sizeof(double) * nf * capacity);
+  D((void**)&p->d_h, sizeof(double2) * (size_t)M.P * capacity * M.lam_pad);
+  D((void**)&p->d_w, sizeof(double) * capacity);
+  D((void**)&p->d_chunk_item, sizeof(int) * p->max_chunks);
+  D((void**)&p->d_chunk_start, sizeof(int64_t) * p->max_chunks);
+  D((void**)&p->d_chunk_end, sizeof(int64_t) * p->max_chunks);
+  D((void**)&p->d_counts, sizeof(int) * 2);
+  D((void**)&p->d_item_chunk, sizeof(int) * (max_items + 1));
+  D((void**)&p->d_coef, sizeof(double2) * (size_t)max_items * M.P);
+  D((void**)&p->d_part, sizeof(double) * (size_t)p->max_chunks * nacc);
+  D((void**)&p->d_out, sizeof(double) * (size_t)max_items * nacc);
+  H((void**)&p->h_chunk_item, sizeof(int) * p->max_chunks);
+  H((void**)&p->h_chunk_start, sizeof(int64_t) * p->max_chunks);
+  H((void**)&p->h_chunk_end, sizeof(int64_t) * p->max_chunks);
+  H((void**)&p->h_counts, sizeof(int) * 2);
+  H((void**)&p->h_item_chunk, sizeof(int) * (max_items + 1));
+  H((void**)&p->h_coef, sizeof(double2) * (size_t)max_items * M.P);
+  H((void**)&p->h_out, sizeof(double) * (size_t)max_items * nacc);
+  if (e == hipSuccess) e = hipEventCreate(&p->t0);
+  if (e == hipSuccess) e = hipEventCreate(&p->t1);
+  if (e == hipSuccess) e = hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking);
+  if (e != hipSuccess) {
+    tpa_pool_destroy(p);
+    return fail(TPA_ERR_HIP, std::string("pool allocation: ") + hipGetErrorString(e));
+  }
+  p->free_ranges[0] = capacity;
+  *out = p;
+  return TPA_OK;
+}
+
+int64_t tpa_pool_free_events(tpa_pool_t p) {
+  if (!p) return -1;
+  std::lock_guard<std::mutex> lk(p->mu);
+  int64_t n = 0;
+  for (auto& r : p->free_ranges) n += r.second;
+  return n;
+}
+
+int tpa_pool_add(tpa_pool_t p, const double* p4, int64_t n, const double* weight,
+                 int64_t* offset) {
+  if (!p || !p4 || !weight || !offset) return fail(TPA_ERR_ARG, "null pointer");
+  if (n < 1) return fail(TPA_ERR_ARG, "event count must be positive");
+  std::lock_guard<std::mutex> lk(p->mu);
+  auto it = p->free_ranges.begin();
+  for (; it != p->free_ranges.end(); ++it)
+    if (it->second >= n) break;
+  if (it == p->free_ranges.end()) return fail(TPA_ERR_ARG, "pool full");
+  const int64_t off = it->first, len = it->second;
+  tpa_model_s* m = p->m;
+  double* d_p4 = nullptr;
+  HIPCHK(hipMalloc(&d_p4, sizeof(double) * 12 * n));
+  p->free_ranges.erase(it);
+  if (len > n) p->free_ranges[off + n] = len - n;
+  p->used_ranges[off] = n;
+  hipError_t e = hipMemcpyAsync(d_p4, p4, sizeof(double) * 12 * n, hipMemcpyHostToDevice,
+                                p->stream);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(p->d_w + off, weight, sizeof(double) * n, hipMemcpyHostToDevice,
+                       p->stream);
+  if (e == hipSuccess) {
+    hipLaunchKernelGGL(k_cal_angle3, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, p->stream,
+                       m->d, d_p4, n, off, p->cap, p->d_ang);
+    hipLaunchKernelGGL(k_build_pw, dim3((unsigned)((n + 127) / 128)), dim3(128), 0, p->stream,
+                       m->d, p->d_ang, n, off, p->cap, m->d_cg, m->d_dw, 1, p->d_h);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipStreamSynchronize(p->stream);
+  (void)hipFree(d_p4);
+  if (e != hipSuccess) {
+    release_range(p, off);
+    return fail(TPA_ERR_HIP, std::string("pool add: ") + hipGetErrorString(e));
+  }
+  *offset = off;
+  return TPA_OK;
+}
+
+int tpa_pool_release(tpa_pool_t p, int64_t offset) {
+  if (!p) return fail(TPA_ERR_ARG, "null pointer");
+  std::lock_guard<std::mutex> lk(p->mu);
+  if (!p->used_ranges.count(offset)) return fail(TPA_ERR_ARG, "no allocation at this offset");
+  release_range(p, offset);
+  return TPA_OK;
+}
+
+static void release_range(tpa_pool_s* p, int64_t offset) {
+  auto it = p->used_ranges.find(offset);
+  int64_t off = offset, len = it->second;
+  p->used_ranges.erase(it);
+  auto nx = p->free_ranges.lower_bound(off);
+  if (nx != p->free_ranges.end() && nx->first == off + len) {
+    len += nx->second;
+    p->free_ranges.erase(nx);
+  }
+  auto pv = p->free_ranges.lower_bound(off);
+  if (pv != p->free_ranges.begin()) {
+    --pv;
+    if (pv->first + pv->second == off) {
+      off = pv->first;
+      len += pv->second;
+      p->free_ranges.erase(pv);
+    }
+  }
+  p->free_ranges[off] = len;
+}
+
+int tpa_pool_eval(tpa_pool_t p, int n_items, const int64_t* ranges, const double* coefs,
+                  int mode, double* out, double* ms) {
+  if (!p || !ranges || !coefs || !out) return fail(TPA_ERR_ARG, "null pointer");
+  if (n_items < 1 || n_items > p->max_items) return fail(TPA_ERR_ARG, "bad request count");
+  if (mode != TPA_MODE_DATA) return fail(TPA_ERR_UNSUPPORTED, "batched evaluation serves data sums");
+  std::lock_guard<std::mutex> lk(p->mu);
+  const TpaModel& M = p->m->h;
+  const int nacc = 2 + 2 * M.P;
+  int nch = 0;
+  for (int i = 0; i < n_items; ++i) {
+    const int64_t off = ranges[2 * i], n = ranges[2 * i + 1];
+    auto it = p->used_ranges.upper_bound(off);
+    if (n < 1 || it == p->used_ranges.begin()) return fail(TPA_ERR_ARG, "range outside the pool");
+    --it;
+    if (off < it->first || off + n > it->first + it->second)
+      return fail(TPA_ERR_ARG, "range outside an allocation");
+    p->h_item_chunk[i] = nch;
+    for (int64_t st = off; st < off + n; st += TPA_CHUNK) {
+      if (nch >= p->max_chunks) return fail(TPA_ERR_ARG, "too many chunks in one batch");
+      p->h_chunk_item[nch] = i;
+      p->h_chunk_start[nch] = st;
+      p->h_chunk_end[nch] = std::min<int64_t>(st + TPA_CHUNK, off + n);
+      ++nch;
+    }
+  }
+  p->h_item_chunk[n_items] = nch;
+  p->h_counts[0] = nch;
+  p->h_counts[1] = n_items;
+  std::memcpy(p->h_coef, coefs, sizeof(double2) * (size_t)n_items * M.P);
+  hipStream_t s = p->stream;
+  HIPCHK(hipMemcpyAsync(p->d_chunk_item, p->h_chunk_item, sizeof(int) * nch, hipMemcpyHostToDevice, s));
+  HIPCHK(hipMemcpyAsync(p->d_chunk_start, p->h_chunk_start, sizeof(int64_t) * nch, hipMemcpyHostToDevice, s));
+  HIPCHK(hipMemcpyAsync(p->d_chunk_end, p->h_chunk_end, sizeof(int64_t) * nch, hipMemcpyHostToDevice, s));
+  HIPCHK(hipMemcpyAsync(p->d_item_chunk, p->h_item_chunk, sizeof(int) * (n_items + 1), hipMemcpyHostToDevice, s));
+  HIPCHK(hipMemcpyAsync(p->d_counts, p->h_counts, sizeof(int) * 2, hipMemcpyHostToDevice, s));
+  HIPCHK(hipMemcpyAsync(p->d_coef, p->h_coef, sizeof(double2) * (size_t)n_items * M.P, hipMemcpyHostToDevice, s));
+  int bucket = 64;
+  while (bucket < nch) bucket <<= 1;
+  auto g = p->graphs.find(bucket);
+  if (g == p->graphs.end()) {
+    MultiArgs a;
+    a.h = p->d_h;
+    a.w = p->d_w;
+    a.coef = p->d_coef;
+    a.chunk_item = p->d_chunk_item;
+    a.chunk_start = p->d_chunk_start;
+    a.chunk_end = p->d_chunk_end;
+    a.counts = p->d_counts;
+    a.partials = p->d_part;
+    a.S = p->cap;
+    a.P = M.P;
+    a.mode = mode;
+    a.nacc = nacc;
+    const int kg = eval_kgroups(M.lam_pad, M.P);
+    if (kg < 0) return fail(TPA_ERR_UNSUPPORTED, "too many partial waves for this helicity count");
+    hipGraph_t graph;
+    HIPCHK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    int rc = dispatch_multi(M.lam_pad, kg, a, bucket, s);
+    hipLaunchKernelGGL(k_reduce_items, dim3(p->max_items), dim3(256), 0, s, p->d_part,
+                       p->d_item_chunk, p->d_counts, nacc, p->d_out);
+    hipError_t ce = hipStreamEndCapture(s, &graph);
+    if (rc != TPA_OK) return fail(rc, "no batched kernel for this helicity count");
+    if (ce != hipSuccess) return fail(TPA_ERR_HIP, std::string("capture: ") + hipGetErrorString(ce));
+    hipGraphExec_t exec;
+    hipError_t ie = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (ie != hipSuccess) return fail(TPA_ERR_HIP, std::string("instantiate: ") + hipGetErrorString(ie));
+    g = p->graphs.emplace(bucket, exec).first;
+  }
+  HIPCHK(hipEventRecord(p->t0, s));
+  HIPCHK(hipGraphLaunch(g->second, s));
+  HIPCHK(hipEventRecord(p->t1, s));
+  HIPCHK(hipMemcpyAsync(p->h_out, p->d_out, sizeof(double) * (size_t)n_items * nacc, hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+  if (ms) {
+    float f = 0.0f;
+    HIPCHK(hipEventElapsedTime(&f, p->t0, p->t1));
+    *ms = f;
+  }
+  std::memcpy(out, p->h_out, sizeof(double) * (size_t)n_items * nacc);
   return TPA_OK;
 }
 

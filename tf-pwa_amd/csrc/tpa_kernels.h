@@ -21,22 +21,25 @@
 
 // ----------------------------------------------------------------------------- cal_angle
 __global__ __launch_bounds__(256) void k_cal_angle3(const TpaModel* __restrict__ M,
-                                                    const double* __restrict__ p4, int64_t N,
+                                                    const double* __restrict__ p4, int64_t n,
+                                                    int64_t off, int64_t S,
                                                     double* __restrict__ ang) {
+  // events e < n of the input p4 [f*4+c][n] go to column off + e of the cache (stride S)
   const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= N) return;
+  if (e >= n) return;
+  const int64_t col = off + e;
   const int nt = M->n_topo;
   const int64_t gbase = (int64_t)TPA_NF_TOPO * nt;
   v4 p[TPA_MAX_FINAL];
   for (int f = 0; f < 3; ++f) {
-    p[f].t = p4[(int64_t)(f * 4 + 0) * N + e];
-    p[f].x = p4[(int64_t)(f * 4 + 1) * N + e];
-    p[f].y = p4[(int64_t)(f * 4 + 2) * N + e];
-    p[f].z = p4[(int64_t)(f * 4 + 3) * N + e];
+    p[f].t = p4[(int64_t)(f * 4 + 0) * n + e];
+    p[f].x = p4[(int64_t)(f * 4 + 1) * n + e];
+    p[f].y = p4[(int64_t)(f * 4 + 2) * n + e];
+    p[f].z = p4[(int64_t)(f * 4 + 3) * n + e];
   }
   v4 ptop = lv_add(lv_add(p[0], p[1]), p[2]);
-  ang[(gbase + 0) * N + e] = lv_M(ptop);
-  for (int f = 0; f < 3; ++f) ang[(gbase + 1 + f) * N + e] = lv_M(p[f]);
+  ang[(gbase + 0) * S + col] = lv_M(ptop);
+  for (int f = 0; f < 3; ++f) ang[(gbase + 1 + f) * S + col] = lv_M(p[f]);
 
   v3 base_z = v_make(0.0, 0.0, 1.0);
   if (M->random_z) {
@@ -49,9 +52,9 @@ __global__ __launch_bounds__(256) void k_cal_angle3(const TpaModel* __restrict__
   su2 bm[TPA_MAX_TOPO][TPA_MAX_FINAL];
   for (int t = 0; t < nt; ++t) {
     const TpaTopo T = M->topo[t];
-    double* out = ang + (int64_t)(TPA_NF_TOPO * t) * N + e;
+    double* out = ang + (int64_t)(TPA_NF_TOPO * t) * S + col;
     v4 pR = lv_add(p[T.y < T.z ? T.y : T.z], p[T.y < T.z ? T.z : T.y]);
-    out[TPA_F_MR * N] = lv_M(pR);
+    out[TPA_F_MR * S] = lv_M(pR);
     // cal_chain_boost: everything into the top rest frame, daughters of R into R's frame
     v4 pRt = lv_rest(ptop, pR);
     v4 pXt = lv_rest(ptop, p[T.x]);
@@ -72,14 +75,14 @@ __global__ __launch_bounds__(256) void k_cal_angle3(const TpaModel* __restrict__
       su2 bmat = su2_boost_z(lv_omega(pj));
       su2 rmat = su2_mul(su2_rot_y(b), su2_rot_z(a));
       if (isR) {
-        out[(TPA_F_AR + 0) * N] = a;
-        out[(TPA_F_AR + 1) * N] = b;
+        out[(TPA_F_AR + 0) * S] = a;
+        out[(TPA_F_AR + 1) * S] = b;
         xR = x;
         rR = rmat;
         bR = bmat;
       } else {
-        out[(TPA_F_AX + 0) * N] = a;
-        out[(TPA_F_AX + 1) * N] = b;
+        out[(TPA_F_AX + 0) * S] = a;
+        out[(TPA_F_AX + 1) * S] = b;
         rm[t][T.x] = rmat;
         bm[t][T.x] = bmat;
       }
@@ -96,15 +99,15 @@ __global__ __launch_bounds__(256) void k_cal_angle3(const TpaModel* __restrict__
       a = fmod_floor(a - bias, 2.0 * M_PI) + bias;
       bias -= M_PI;
       const int fld = pos == 0 ? TPA_F_AY : TPA_F_AZ;
-      out[(fld + 0) * N] = a;
-      out[(fld + 1) * N] = b;
+      out[(fld + 0) * S] = a;
+      out[(fld + 1) * S] = b;
       bm[t][f] = su2_boost_z(lv_omega(pjr));
       rm[t][f] = su2_mul(su2_mul(su2_mul(su2_rot_y(b), su2_rot_z(a)), bR), rR);
     }
   }
   // aligned angles (cal_angle.py:444-474, rule 1 reference frames, r_boost + final_rest)
   for (int t = 0; t < nt; ++t) {
-    double* out = ang + (int64_t)(TPA_NF_TOPO * t) * N + e;
+    double* out = ang + (int64_t)(TPA_NF_TOPO * t) * S + col;
     for (int f = 0; f < 3; ++f) {
       double al = 0.0, be = 0.0, ga = 0.0;
       if (M->topo[t].aligned[f]) {
@@ -113,27 +116,30 @@ __global__ __launch_bounds__(256) void k_cal_angle3(const TpaModel* __restrict__
         R = su2_mul(su2_mul(bm[r0][f], R), su2_inv(bm[t][f]));
         su2_euler(R, &al, &be, &ga);
       }
-      out[(TPA_F_AL + 3 * f + 0) * N] = al;
-      out[(TPA_F_AL + 3 * f + 1) * N] = be;
-      out[(TPA_F_AL + 3 * f + 2) * N] = ga;
+      out[(TPA_F_AL + 3 * f + 0) * S] = al;
+      out[(TPA_F_AL + 3 * f + 1) * S] = be;
+      out[(TPA_F_AL + 3 * f + 2) * S] = ga;
     }
   }
 }
 
 // ------------------------------------------------------------------ partial-wave build
 __global__ __launch_bounds__(128) void k_build_pw(const TpaModel* __restrict__ M,
-                                                  const double* __restrict__ ang, int64_t N,
+                                                  const double* __restrict__ ang, int64_t n,
+                                                  int64_t off, int64_t S,
                                                   const double* __restrict__ cg,
                                                   const double* __restrict__ dw, int fold,
                                                   double2* __restrict__ h) {
+  // columns off .. off+n-1 of the angle cache and of h (both with stride S)
   const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= N) return;
+  if (e >= n) return;
+  const int64_t col = off + e;
   const int nt = M->n_topo;
   const int lam_pad = M->lam_pad;
   const int64_t gbase = (int64_t)TPA_NF_TOPO * nt;
-  const double mtop = ang[(gbase + 0) * N + e];
+  const double mtop = ang[(gbase + 0) * S + col];
   double mf[3];
-  for (int f = 0; f < 3; ++f) mf[f] = ang[(gbase + 1 + f) * N + e];
+  for (int f = 0; f < 3; ++f) mf[f] = ang[(gbase + 1 + f) * S + col];
   const int ja = M->top_j2;
   const int nA = M->n_top;
   int nfh[3];
@@ -142,25 +148,25 @@ __global__ __launch_bounds__(128) void k_build_pw(const TpaModel* __restrict__ M
   for (int c = 0; c < M->n_chain; ++c) {
     const TpaChain& C = M->chain[c];
     const TpaTopo T = M->topo[C.topo];
-    const double* A = ang + (int64_t)(TPA_NF_TOPO * C.topo) * N + e;
-    const double mR = A[TPA_F_MR * N];
+    const double* A = ang + (int64_t)(TPA_NF_TOPO * C.topo) * S + col;
+    const double mR = A[TPA_F_MR * S];
     const int fx = T.x;
     const int fy = C.r_y_first ? T.y : T.z;  // this chain's first daughter of R
     const int fz = C.r_y_first ? T.z : T.y;
     // vertex 1: top -> (R, X) in this chain's order; D uses the angle of outs[0]
     double a1, b1;
     if (C.top_r_first) {
-      a1 = A[(TPA_F_AR + 0) * N];
-      b1 = A[(TPA_F_AR + 1) * N];
+      a1 = A[(TPA_F_AR + 0) * S];
+      b1 = A[(TPA_F_AR + 1) * S];
     } else {
-      a1 = A[(TPA_F_AX + 0) * N];
-      b1 = A[(TPA_F_AX + 1) * N];
+      a1 = A[(TPA_F_AX + 0) * S];
+      b1 = A[(TPA_F_AX + 1) * S];
     }
     const double q1sq = C.top_r_first ? rel_p2(mtop, mR, mf[fx]) : rel_p2(mtop, mf[fx], mR);
     // vertex 2: R -> (fy, fz)
     const int fld2 = (fy == T.y) ? TPA_F_AY : TPA_F_AZ;
-    const double a2 = A[(fld2 + 0) * N];
-    const double b2 = A[(fld2 + 1) * N];
+    const double a2 = A[(fld2 + 0) * S];
+    const double b2 = A[(fld2 + 1) * S];
     const double q2sq = rel_p2(mR, mf[fy], mf[fz]);
     const double dd = C.d * C.d;
     double bf1[TPA_MAX_LS], bf2[TPA_MAX_LS];
@@ -202,9 +208,9 @@ __global__ __launch_bounds__(128) void k_build_pw(const TpaModel* __restrict__ M
       const int jf = M->final_j2[f];
       has_al[f] = (T.aligned[f] && jf != 0) ? 1 : 0;
       if (!has_al[f]) continue;
-      const double al = A[(TPA_F_AL + 3 * f + 0) * N];
-      const double be = A[(TPA_F_AL + 3 * f + 1) * N];
-      const double ga = A[(TPA_F_AL + 3 * f + 2) * N];
+      const double al = A[(TPA_F_AL + 3 * f + 0) * S];
+      const double be = A[(TPA_F_AL + 3 * f + 1) * S];
+      const double ga = A[(TPA_F_AL + 3 * f + 2) * S];
       half_angle_powers(be, jf, sp, cp);
       const double* wf = dw + dweight_offset(jf);
       for (int i = 0; i < nfh[f]; ++i)
@@ -228,7 +234,7 @@ __global__ __launch_bounds__(128) void k_build_pw(const TpaModel* __restrict__ M
     for (int s1 = 0; s1 < C.n_ls1; ++s1) {
       for (int s2 = 0; s2 < C.n_ls2; ++s2) {
         const int64_t k = C.k0 + s1 * C.n_ls2 + s2;
-        double2* hk = h + ((int64_t)k * N + e) * lam_pad;
+        double2* hk = h + ((int64_t)k * S + col) * lam_pad;
         int lf = 0;
         for (int iA = 0; iA < nA; ++iA)
           for (int o0 = 0; o0 < nfh[0]; ++o0)
@@ -794,5 +800,171 @@ __global__ __launch_bounds__(256) void k_eval_shape(EvalShapeArgs s) {
   if (a.grad) {
     for (int j = 2 + threadIdx.x; j < a.nacc; j += blockDim.x)
       out[j] = ((sh[0][j] + sh[1][j]) + sh[2][j]) + sh[3][j];
+  }
+}
+
+// ------------------------------------------------------------ batched multi-fit evaluation
+// Serving: many fits keep their data in one device pool (stride S). A batch is a list of
+// chunks (<= TPA_CHUNK events of one request each); block b evaluates chunk b with that
+// request's coefficients and writes its partial sums, and k_reduce_items sums each request's
+// chunks in order. Counts live in device memory so one captured graph serves any batch up to
+// its grid size.
+#define TPA_CHUNK 2048
+
+struct MultiArgs {
+  const double2* __restrict__ h;     // pool cache [P][S][LG]
+  const double* __restrict__ w;      // pool weights [S]
+  const double2* __restrict__ coef;  // [n_items][P]
+  const int* __restrict__ chunk_item;
+  const int64_t* __restrict__ chunk_start;
+  const int64_t* __restrict__ chunk_end;
+  const int* __restrict__ counts;    // [0] = chunks, [1] = items
+  double* __restrict__ partials;     // [chunk][nacc]
+  int64_t S;
+  int P;
+  int mode;
+  int nacc;
+};
+
+template <int LG, int KG, int PML>
+__global__ __launch_bounds__(256) void k_eval_multi(MultiArgs a) {
+  const int chunk = blockIdx.x;
+  if (chunk >= a.counts[0]) return;  // uniform per block, before any barrier
+  constexpr int LPE = LG * KG;
+  constexpr int EPW = 64 / LPE;
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  const int lam = lane & (LG - 1);
+  const int kg = (lane / LG) & (KG - 1);
+  const int el = lane / LPE;
+  const int item = a.chunk_item[chunk];
+  const int64_t end = a.chunk_end[chunk];
+  const double2* __restrict__ coef = a.coef + (int64_t)item * a.P;
+  const double eps = 1.0e-6;
+  const double logeps = log(1.0e-6);
+  double vs = 0.0, vc = 0.0;
+  double gr[PML], gi[PML], cr[PML], ci[PML];
+#pragma unroll
+  for (int j = 0; j < PML; ++j) {
+    const int k = kg + KG * j;
+    gr[j] = gi[j] = 0.0;
+    const double2 c = k < a.P ? coef[k] : make_double2(0.0, 0.0);
+    cr[j] = c.x;
+    ci[j] = c.y;
+  }
+  for (int64_t base = a.chunk_start[chunk] + wave * EPW; base < end; base += 4 * EPW) {
+    const int64_t e = base + el;
+    const bool valid = e < end;
+    double2 hv[PML];
+#pragma unroll
+    for (int j = 0; j < PML; ++j) {
+      const int k = kg + KG * j;
+      hv[j] = (valid && k < a.P) ? a.h[((int64_t)k * a.S + e) * LG + lam] : make_double2(0.0, 0.0);
+    }
+    double ar = 0.0, ai = 0.0;
+#pragma unroll
+    for (int j = 0; j < PML; ++j) {
+      ar += cr[j] * hv[j].x - ci[j] * hv[j].y;
+      ai += cr[j] * hv[j].y + ci[j] * hv[j].x;
+    }
+#pragma unroll
+    for (int m = LG; m < LPE; m <<= 1) {
+      ar += __shfl_xor(ar, m, 64);
+      ai += __shfl_xor(ai, m, 64);
+    }
+    double f = ar * ar + ai * ai;
+#pragma unroll
+    for (int m = 1; m < LG; m <<= 1) f += __shfl_xor(f, m, 64);
+    double y = 0.0, u = 0.0;
+    if (valid) {
+      const double w = a.w[e];
+      if (a.mode == 0) {
+        const double dom = (w == 0.0) ? 1.0 : w;
+        const double x = (f * w) / dom;
+        double cl, dcl;
+        if (x > eps) {
+          cl = log(x);
+          dcl = 1.0 / x;
+        } else {
+          const double dx = x - eps;
+          const double t = dx / eps;
+          cl = (logeps + t) - (t * t) / 2.0;
+          dcl = 1.0 / eps - t / eps;
+        }
+        y = w * cl;
+        u = (w * dcl) * (w / dom);
+      } else {
+        y = w * f;
+        u = w;
+      }
+    }
+    if (lam == 0 && kg == 0) neumaier_add(vs, vc, y);
+    const double tr = u * ar, ti = -(u * ai);
+#pragma unroll
+    for (int j = 0; j < PML; ++j) {
+      gr[j] += tr * hv[j].x - ti * hv[j].y;
+      gi[j] += tr * hv[j].y + ti * hv[j].x;
+    }
+  }
+#pragma unroll
+  for (int m = 32; m >= 1; m >>= 1) {
+    const double s2 = __shfl_xor(vs, m, 64);
+    const double c2 = __shfl_xor(vc, m, 64);
+    neumaier_merge(vs, vc, s2, c2);
+  }
+  __shared__ double sh[4][TPA_NACC_MAX];
+#pragma unroll
+  for (int j = 0; j < PML; ++j) {
+    double xr = gr[j], xi = gi[j];
+#pragma unroll
+    for (int m = 1; m < LG; m <<= 1) {
+      xr += __shfl_xor(xr, m, 64);
+      xi += __shfl_xor(xi, m, 64);
+    }
+#pragma unroll
+    for (int m = LPE; m < 64; m <<= 1) {
+      xr += __shfl_xor(xr, m, 64);
+      xi += __shfl_xor(xi, m, 64);
+    }
+    const int k = kg + KG * j;
+    if (el == 0 && lam == 0 && k < a.P) {
+      sh[wave][2 + 2 * k] = xr;
+      sh[wave][3 + 2 * k] = xi;
+    }
+  }
+  if (lane == 0) {
+    sh[wave][0] = vs;
+    sh[wave][1] = vc;
+  }
+  __syncthreads();
+  double* out = a.partials + (int64_t)chunk * a.nacc;
+  if (threadIdx.x == 0) {
+    double s0 = sh[0][0], c0 = sh[0][1];
+    for (int wv = 1; wv < 4; ++wv) neumaier_merge(s0, c0, sh[wv][0], sh[wv][1]);
+    out[0] = s0;
+    out[1] = c0;
+  }
+  for (int j = 2 + threadIdx.x; j < a.nacc; j += blockDim.x)
+    out[j] = ((sh[0][j] + sh[1][j]) + sh[2][j]) + sh[3][j];
+}
+
+// Per request: the compensated, fixed-order sum of its chunks (item_chunk[i] .. item_chunk[i+1]).
+__global__ __launch_bounds__(256) void k_reduce_items(const double* __restrict__ partials,
+                                                      const int* __restrict__ item_chunk,
+                                                      const int* __restrict__ counts, int nacc,
+                                                      double* __restrict__ out) {
+  const int item = blockIdx.x;
+  if (item >= counts[1]) return;
+  const int c0 = item_chunk[item], c1 = item_chunk[item + 1];
+  for (int j = threadIdx.x; j < nacc; j += blockDim.x) {
+    if (j == 1) {
+      out[(int64_t)item * nacc + 1] = 0.0;
+      continue;
+    }
+    double s = 0.0, c = 0.0;
+    for (int b = c0; b < c1; ++b) neumaier_add(s, c, partials[(int64_t)b * nacc + j]);
+    if (j == 0)
+      for (int b = c0; b < c1; ++b) neumaier_add(s, c, partials[(int64_t)b * nacc + 1]);
+    out[(int64_t)item * nacc + j] = s + c;
   }
 }

@@ -1,0 +1,625 @@
+// Many concurrent fits on one GPU (host runtime of the serving path).
+//
+// Every fit runs its own BFGS minimisation (the role of fit_scipy's scipy.optimize.minimize
+// loop, tf_pwa/fit.py:189-300) on a host thread. An NLL + gradient evaluation of a fit
+// (FCN.nll_grad, tf_pwa/model/model.py:1216-1240) computes the partial-wave coefficients
+// c_k = const_k * v_total * v_g1 * v_g2 from the fit's parameters, hands them to one batcher
+// thread and waits. The batcher gathers the pending requests of all fits and evaluates them in
+// ONE tpa_pool_eval launch (data sum and its adjoint per request); the MC normalisation is the
+// shared partial-wave Gram matrix I, so int_mc = c^dagger I c and its adjoint conj(I c) cost
+// O(P^2) on the host. The gradient in the fit's parameters follows from the adjoints by the
+// chain rule through the three complex factors of each c_k.
+//
+// A fit's result does not depend on which other fits share its batches: tpa_pool_eval's
+// per-request sums have a fixed order, and each fit's BFGS is deterministic.
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <complex>
+#include <condition_variable>
+#include <cstdint>
+#include <cstring>
+#include <deque>
+#include <mutex>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "tfpwa_amd.h"
+#include "tpa_internal.h"
+
+namespace {
+
+using cplx = std::complex<double>;
+using Clock = std::chrono::steady_clock;
+
+struct Cancelled {};
+struct EngineFailure {};
+
+struct Fit {
+  int id = -1;
+  int64_t off = 0, n = 0;
+  double sw = 0.0;
+  std::vector<double> x0;
+  // result (y space: physical values of the trainable parameters)
+  std::vector<double> y;
+  double nll = NAN;
+  int status = -1;
+  int64_t nfev = 0, niter = 0;
+  // the request slot shared with the batcher (guarded by the server mutex)
+  std::vector<double> coef, out;
+  bool ready = false, failed = false;
+  Clock::time_point t_submit;
+  std::condition_variable cv;
+  std::thread th;
+};
+
+}  // namespace
+
+struct tpa_server_s {
+  tpa_pool_t pool = nullptr;
+  int P = 0, nf = 0, nv = 0;
+  std::vector<int> wave_factor;  // [P][3]
+  std::vector<double> wave_const;
+  std::vector<int> factor_polar, factor_var;  // [nf], [nf][2]
+  std::vector<double> factor_fixed;           // [nf][2]
+  std::vector<int> bound_kind;                // [nv]
+  std::vector<double> bound_ab;               // [nv][2]
+  std::vector<cplx> gram;                     // [P][P]
+  double gtol = 1e-5;
+  int maxiter = 0;
+  int max_batch = 1;
+  double window_us = 200.0;
+
+  std::mutex mu;
+  std::condition_variable cv;  // batcher wake-up: new request, a client leaving, stop
+  std::vector<Fit*> fits;      // by id
+  std::deque<Fit*> pending;
+  int clients = 0;             // fit threads alive + direct evaluations in progress
+  std::vector<int> finished;   // ids not yet polled
+  bool stop = false;
+  std::string batch_error;
+  std::thread batcher;
+  std::vector<double> lat_us;
+  int64_t n_requests = 0, n_batches = 0;
+  double gpu_ms = 0.0;
+};
+
+namespace {
+
+// ------------------------------------------------------------------ parameters -> coefficients
+void factor_values(const tpa_server_s* S, const double* y, cplx* v, cplx* dr, cplx* di) {
+  for (int f = 0; f < S->nf; ++f) {
+    const int vr = S->factor_var[2 * f], vi = S->factor_var[2 * f + 1];
+    const double r = vr >= 0 ? y[vr] : S->factor_fixed[2 * f];
+    const double i = vi >= 0 ? y[vi] : S->factor_fixed[2 * f + 1];
+    if (S->factor_polar[f]) {
+      const cplx e(std::cos(i), std::sin(i));
+      v[f] = r * e;
+      dr[f] = e;
+      di[f] = cplx(0.0, 1.0) * r * e;
+    } else {
+      v[f] = cplx(r, i);
+      dr[f] = 1.0;
+      di[f] = cplx(0.0, 1.0);
+    }
+  }
+}
+
+// Bound transform y(x) and dy/dx (the "asin" bounds of tf_pwa/variable.py:1029-1135).
+void to_y(const tpa_server_s* S, const double* x, double* y, double* dydx) {
+  for (int j = 0; j < S->nv; ++j) {
+    const double a = S->bound_ab[2 * j], b = S->bound_ab[2 * j + 1], t = x[j];
+    switch (S->bound_kind[j]) {
+      case 1:
+        y[j] = (b - a) * (std::sin(t) + 1.0) / 2.0 + a;
+        dydx[j] = (b - a) * std::cos(t) / 2.0;
+        break;
+      case 2:
+        y[j] = a + std::sqrt(t * t + 1.0) - 1.0;
+        dydx[j] = t / std::sqrt(t * t + 1.0);
+        break;
+      case 3:
+        y[j] = b + 1.0 - std::sqrt(t * t + 1.0);
+        dydx[j] = -t / std::sqrt(t * t + 1.0);
+        break;
+      default:
+        y[j] = t;
+        dydx[j] = 1.0;
+    }
+  }
+}
+
+double to_x(const tpa_server_s* S, int j, double y) {
+  const double a = S->bound_ab[2 * j], b = S->bound_ab[2 * j + 1];
+  switch (S->bound_kind[j]) {
+    case 1: return std::asin(std::min(1.0, std::max(-1.0, 2.0 * (y - a) / (b - a) - 1.0)));
+    case 2: return std::sqrt((y - a + 1.0) * (y - a + 1.0) - 1.0);
+    case 3: return std::sqrt((b - y + 1.0) * (b - y + 1.0) - 1.0);
+    default: return y;
+  }
+}
+
+// One request through the batcher: coefficients in, data sum and adjoint out.
+void request(tpa_server_s* S, Fit* F, const std::vector<cplx>& c) {
+  std::unique_lock<std::mutex> lk(S->mu);
+  if (S->stop) throw Cancelled();
+  F->coef.resize(2 * S->P);
+  for (int k = 0; k < S->P; ++k) {
+    F->coef[2 * k] = c[k].real();
+    F->coef[2 * k + 1] = c[k].imag();
+  }
+  F->ready = false;
+  F->failed = false;
+  F->t_submit = Clock::now();
+  S->pending.push_back(F);
+  S->cv.notify_all();
+  F->cv.wait(lk, [&] { return F->ready || S->stop; });
+  if (!F->ready) throw Cancelled();
+  if (F->failed) throw EngineFailure();
+}
+
+// NLL and its gradient in y: NLL = -sum w clip_log f + sw log(c^dagger I c) (the Model.nll of
+// tf_pwa/model/model.py:407-449 with the MC sum replaced by its Gram form).
+double nll_grad_y(tpa_server_s* S, Fit* F, const double* y, double* gy) {
+  const int P = S->P, nf = S->nf;
+  std::vector<cplx> v(nf), dr(nf), di(nf), c(P);
+  factor_values(S, y, v.data(), dr.data(), di.data());
+  for (int k = 0; k < P; ++k) {
+    const int* w = &S->wave_factor[3 * k];
+    c[k] = S->wave_const[k] * v[w[0]] * v[w[1]] * v[w[2]];
+  }
+  request(S, F, c);
+  const double sd = F->out[0];
+  std::vector<cplx> Ic(P);
+  double smc = 0.0;
+  for (int k = 0; k < P; ++k) {
+    cplx acc = 0.0;
+    const cplx* row = &S->gram[(size_t)k * P];
+    for (int l = 0; l < P; ++l) acc += row[l] * c[l];
+    Ic[k] = acc;
+    smc += (std::conj(c[k]) * acc).real();
+  }
+  const double nll = -sd + F->sw * std::log(smc);
+  if (gy) {
+    std::fill(gy, gy + S->nv, 0.0);
+    const double r = F->sw / smc;
+    for (int k = 0; k < P; ++k) {
+      const cplx gd(F->out[2 + 2 * k], F->out[3 + 2 * k]);
+      const cplx comb = S->wave_const[k] * (-gd + r * std::conj(Ic[k]));
+      const int* w = &S->wave_factor[3 * k];
+      for (int a = 0; a < 3; ++a) {
+        cplx rest = comb;
+        for (int b = 0; b < 3; ++b)
+          if (b != a) rest *= v[w[b]];
+        const int f = w[a];
+        const int vr = S->factor_var[2 * f], vi = S->factor_var[2 * f + 1];
+        if (vr >= 0) gy[vr] += 2.0 * (dr[f] * rest).real();
+        if (vi >= 0) gy[vi] += 2.0 * (di[f] * rest).real();
+      }
+    }
+  }
+  F->nfev += 1;
+  return nll;
+}
+
+// ------------------------------------------------------------------ BFGS
+struct Objective {
+  tpa_server_s* S;
+  Fit* F;
+  std::vector<double> y, dydx, gy;
+  double operator()(const std::vector<double>& x, std::vector<double>& gx) {
+    const int n = S->nv;
+    y.resize(n);
+    dydx.resize(n);
+    gy.resize(n);
+    to_y(S, x.data(), y.data(), dydx.data());
+    const double f = nll_grad_y(S, F, y.data(), gy.data());
+    gx.resize(n);
+    for (int j = 0; j < n; ++j) gx[j] = gy[j] * dydx[j];
+    return f;
+  }
+};
+
+double dot(const std::vector<double>& a, const std::vector<double>& b) {
+  double s = 0.0;
+  for (size_t i = 0; i < a.size(); ++i) s += a[i] * b[i];
+  return s;
+}
+
+double norm_inf(const std::vector<double>& a) {
+  double m = 0.0;
+  for (double v : a) m = std::max(m, std::fabs(v));
+  return m;
+}
+
+struct Trial {
+  double a, f, d;
+  std::vector<double> g;
+};
+
+// Strong-Wolfe line search (Nocedal & Wright, Numerical Optimization, Algorithms 3.5 and 3.6,
+// cubic interpolation of eq. 3.59 with bisection as safeguard). Returns false on failure.
+bool line_search(Objective& obj, const std::vector<double>& x, const std::vector<double>& p,
+                 double f0, double d0, double a1, Trial& out) {
+  const double c1 = 1e-4, c2 = 0.9, amax = 1e10;
+  const int n = (int)x.size();
+  std::vector<double> xt(n);
+  auto eval = [&](double a, Trial& t) {
+    for (int i = 0; i < n; ++i) xt[i] = x[i] + a * p[i];
+    t.a = a;
+    t.f = obj(xt, t.g);
+    t.d = dot(t.g, p);
+  };
+  auto armijo_fails = [&](const Trial& t) {
+    return !std::isfinite(t.f) || t.f > f0 + c1 * t.a * d0;
+  };
+  auto zoom = [&](Trial lo, Trial hi) -> bool {
+    for (int it = 0; it < 30; ++it) {
+      double a = 0.5 * (lo.a + hi.a);
+      if (std::isfinite(hi.f) && std::isfinite(hi.d)) {
+        const double d1 = lo.d + hi.d - 3.0 * (lo.f - hi.f) / (lo.a - hi.a);
+        const double disc = d1 * d1 - lo.d * hi.d;
+        if (disc >= 0.0) {
+          const double d2 = std::copysign(std::sqrt(disc), hi.a - lo.a);
+          const double den = hi.d - lo.d + 2.0 * d2;
+          if (den != 0.0) {
+            const double ac = hi.a - (hi.a - lo.a) * (hi.d + d2 - d1) / den;
+            const double lo_b = std::min(lo.a, hi.a), hi_b = std::max(lo.a, hi.a);
+            const double margin = 0.1 * (hi_b - lo_b);
+            if (std::isfinite(ac) && ac > lo_b + margin && ac < hi_b - margin) a = ac;
+          }
+        }
+      }
+      if (std::fabs(hi.a - lo.a) <= 1e-16 * std::max(1.0, std::fabs(lo.a))) return false;
+      Trial t;
+      eval(a, t);
+      if (armijo_fails(t) || t.f >= lo.f) {
+        hi = std::move(t);
+      } else {
+        if (std::fabs(t.d) <= -c2 * d0) {
+          out = std::move(t);
+          return true;
+        }
+        if (t.d * (hi.a - lo.a) >= 0.0) hi = lo;
+        lo = std::move(t);
+      }
+    }
+    // no strong-Wolfe point; accept a point of sufficient decrease if we have one
+    if (lo.a > 0.0 && lo.f < f0) {
+      out = std::move(lo);
+      return true;
+    }
+    return false;
+  };
+  Trial prev;
+  prev.a = 0.0;
+  prev.f = f0;
+  prev.d = d0;
+  double a = a1;
+  for (int i = 0; i < 40; ++i) {
+    Trial t;
+    eval(a, t);
+    if (armijo_fails(t) || (i > 0 && t.f >= prev.f)) return zoom(std::move(prev), std::move(t));
+    if (std::fabs(t.d) <= -c2 * d0) {
+      out = std::move(t);
+      return true;
+    }
+    if (t.d >= 0.0) return zoom(std::move(t), std::move(prev));
+    prev = std::move(t);
+    a = std::min(2.0 * a, amax);
+  }
+  return false;
+}
+
+// Status codes: 0 converged (|g|_inf <= gtol), 1 iteration limit, 2 line search failed
+// (precision loss), 3 cancelled, 4 engine error.
+void run_bfgs(tpa_server_s* S, Fit* F) {
+  const int n = S->nv;
+  Objective obj{S, F, {}, {}, {}};
+  std::vector<double> x(n), g, p(n), s(n), yv(n), Hy(n);
+  for (int j = 0; j < n; ++j) x[j] = to_x(S, j, F->x0[j]);
+  std::vector<double> H((size_t)n * n, 0.0);
+  for (int j = 0; j < n; ++j) H[(size_t)j * n + j] = 1.0;
+  double f = obj(x, g);
+  double f_old_old = f + std::sqrt(dot(g, g)) / 2.0;
+  int status = 1;
+  int64_t k = 0;
+  const int maxiter = S->maxiter > 0 ? S->maxiter : 200 * n;
+  if (norm_inf(g) <= S->gtol) status = 0;
+  while (status == 1 && k < maxiter) {
+    for (int i = 0; i < n; ++i) {
+      double acc = 0.0;
+      for (int j = 0; j < n; ++j) acc -= H[(size_t)i * n + j] * g[j];
+      p[i] = acc;
+    }
+    double d0 = dot(g, p);
+    if (!(d0 < 0.0)) {  // not a descent direction: restart from steepest descent
+      std::fill(H.begin(), H.end(), 0.0);
+      for (int j = 0; j < n; ++j) H[(size_t)j * n + j] = 1.0;
+      for (int j = 0; j < n; ++j) p[j] = -g[j];
+      d0 = -dot(g, g);
+    }
+    // first trial step from the previous decrease (Nocedal & Wright eq. 3.60)
+    double a1 = 1.0;
+    const double guess = 1.01 * 2.0 * (f - f_old_old) / d0;
+    if (std::isfinite(guess) && guess > 0.0) a1 = std::min(1.0, guess);
+    Trial t;
+    if (!line_search(obj, x, p, f, d0, a1, t)) {
+      status = 2;
+      break;
+    }
+    for (int j = 0; j < n; ++j) {
+      s[j] = t.a * p[j];
+      x[j] += s[j];
+      yv[j] = t.g[j] - g[j];
+    }
+    f_old_old = f;
+    f = t.f;
+    g = std::move(t.g);
+    ++k;
+    if (norm_inf(g) <= S->gtol) {
+      status = 0;
+      break;
+    }
+    const double ys = dot(yv, s);
+    if (!(ys > 1e-12 * std::sqrt(dot(yv, yv) * dot(s, s)))) continue;  // keep H (curvature)
+    if (k == 1) {  // scale the initial inverse Hessian (Nocedal & Wright eq. 6.20)
+      const double sc = ys / dot(yv, yv);
+      for (double& h : H) h *= sc;
+    }
+    const double rho = 1.0 / ys;
+    for (int i = 0; i < n; ++i) {
+      double acc = 0.0;
+      for (int j = 0; j < n; ++j) acc += H[(size_t)i * n + j] * yv[j];
+      Hy[i] = acc;
+    }
+    const double yHy = dot(yv, Hy);
+    const double cs = rho * rho * yHy + rho;
+    for (int i = 0; i < n; ++i)
+      for (int j = 0; j < n; ++j)
+        H[(size_t)i * n + j] += -rho * (s[i] * Hy[j] + Hy[i] * s[j]) + cs * s[i] * s[j];
+  }
+  F->niter = k;
+  F->status = status;
+  F->nll = f;
+  F->y.resize(n);
+  std::vector<double> dydx(n);
+  to_y(S, x.data(), F->y.data(), dydx.data());
+}
+
+void fit_main(tpa_server_s* S, Fit* F) {
+  try {
+    run_bfgs(S, F);
+  } catch (const Cancelled&) {
+    F->status = 3;
+  } catch (const EngineFailure&) {
+    F->status = 4;
+  }
+  std::lock_guard<std::mutex> lk(S->mu);
+  S->clients -= 1;
+  S->finished.push_back(F->id);
+  S->cv.notify_all();
+}
+
+// ------------------------------------------------------------------ batcher
+void batcher_main(tpa_server_s* S) {
+  const int P = S->P;
+  const int nacc = 2 + 2 * P;
+  std::vector<Fit*> batch;
+  std::vector<int64_t> ranges;
+  std::vector<double> coefs, out;
+  std::unique_lock<std::mutex> lk(S->mu);
+  for (;;) {
+    S->cv.wait(lk, [&] { return S->stop || !S->pending.empty(); });
+    if (S->stop) break;
+    // gather: until every client is waiting, the batch is full, or the window has passed
+    const auto deadline = Clock::now() + std::chrono::duration<double, std::micro>(S->window_us);
+    while (!S->stop && (int)S->pending.size() < S->clients &&
+           (int)S->pending.size() < S->max_batch) {
+      if (S->cv.wait_until(lk, deadline) == std::cv_status::timeout) break;
+    }
+    if (S->stop) break;
+    batch.clear();
+    while (!S->pending.empty() && (int)batch.size() < S->max_batch) {
+      batch.push_back(S->pending.front());
+      S->pending.pop_front();
+    }
+    const int m = (int)batch.size();
+    ranges.resize(2 * m);
+    coefs.resize((size_t)2 * P * m);
+    for (int i = 0; i < m; ++i) {
+      ranges[2 * i] = batch[i]->off;
+      ranges[2 * i + 1] = batch[i]->n;
+      std::memcpy(&coefs[(size_t)2 * P * i], batch[i]->coef.data(), sizeof(double) * 2 * P);
+    }
+    lk.unlock();
+    out.resize((size_t)nacc * m);
+    double ms = 0.0;
+    const int rc = tpa_pool_eval(S->pool, m, ranges.data(), coefs.data(), TPA_MODE_DATA,
+                                 out.data(), &ms);
+    const std::string err = rc == TPA_OK ? std::string() : std::string(tpa_last_error());
+    lk.lock();
+    const auto now = Clock::now();
+    S->n_batches += 1;
+    S->n_requests += m;
+    S->gpu_ms += ms;
+    if (rc != TPA_OK) S->batch_error = err;
+    for (int i = 0; i < m; ++i) {
+      Fit* F = batch[i];
+      F->out.assign(out.begin() + (size_t)nacc * i, out.begin() + (size_t)nacc * (i + 1));
+      F->failed = rc != TPA_OK;
+      F->ready = true;
+      S->lat_us.push_back(std::chrono::duration<double, std::micro>(now - F->t_submit).count());
+      F->cv.notify_one();
+    }
+  }
+  // wake every waiting client so it sees the stop
+  for (Fit* F : S->fits) F->cv.notify_all();
+  for (Fit* F : S->pending) F->cv.notify_all();
+}
+
+}  // namespace
+
+extern "C" {
+
+int tpa_server_create(tpa_pool_t pool, const TpaCouplingMap* map, const double* gram,
+                      double gtol, int maxiter, int max_batch, double window_us,
+                      tpa_server_t* out) {
+  if (!pool || !map || !gram || !out) return tpa_set_error(TPA_ERR_ARG, "null pointer");
+  if (map->P < 1 || map->n_factor < 1 || map->n_var < 1 || max_batch < 1 || !(gtol > 0.0) ||
+      window_us < 0.0)
+    return tpa_set_error(TPA_ERR_ARG, "bad server options");
+  if (!map->wave_factor || !map->wave_const || !map->factor_polar || !map->factor_var ||
+      !map->factor_fixed || !map->bound_kind || !map->bound_ab)
+    return tpa_set_error(TPA_ERR_ARG, "incomplete coupling map");
+  const int P = map->P, nf = map->n_factor, nv = map->n_var;
+  for (int i = 0; i < 3 * P; ++i)
+    if (map->wave_factor[i] < 0 || map->wave_factor[i] >= nf)
+      return tpa_set_error(TPA_ERR_ARG, "wave factor index out of range");
+  for (int i = 0; i < 2 * nf; ++i)
+    if (map->factor_var[i] < -1 || map->factor_var[i] >= nv)
+      return tpa_set_error(TPA_ERR_ARG, "factor variable index out of range");
+  for (int j = 0; j < nv; ++j)
+    if (map->bound_kind[j] < 0 || map->bound_kind[j] > 3)
+      return tpa_set_error(TPA_ERR_ARG, "bad bound kind");
+  tpa_server_s* S = new tpa_server_s();
+  S->pool = pool;
+  S->P = P;
+  S->nf = nf;
+  S->nv = nv;
+  S->wave_factor.assign(map->wave_factor, map->wave_factor + 3 * P);
+  S->wave_const.assign(map->wave_const, map->wave_const + P);
+  S->factor_polar.assign(map->factor_polar, map->factor_polar + nf);
+  S->factor_var.assign(map->factor_var, map->factor_var + 2 * nf);
+  S->factor_fixed.assign(map->factor_fixed, map->factor_fixed + 2 * nf);
+  S->bound_kind.assign(map->bound_kind, map->bound_kind + nv);
+  S->bound_ab.assign(map->bound_ab, map->bound_ab + 2 * nv);
+  S->gram.resize((size_t)P * P);
+  for (size_t i = 0; i < (size_t)P * P; ++i) S->gram[i] = cplx(gram[2 * i], gram[2 * i + 1]);
+  S->gtol = gtol;
+  S->maxiter = maxiter;
+  S->max_batch = max_batch;
+  S->window_us = window_us;
+  S->batcher = std::thread(batcher_main, S);
+  *out = S;
+  return TPA_OK;
+}
+
+int tpa_server_destroy(tpa_server_t S) {
+  if (!S) return TPA_OK;
+  {
+    std::lock_guard<std::mutex> lk(S->mu);
+    S->stop = true;
+    S->cv.notify_all();
+    for (Fit* F : S->fits) F->cv.notify_all();
+  }
+  if (S->batcher.joinable()) S->batcher.join();
+  for (Fit* F : S->fits)
+    if (F->th.joinable()) F->th.join();
+  for (Fit* F : S->fits) delete F;
+  delete S;
+  return TPA_OK;
+}
+
+int tpa_server_nll_grad(tpa_server_t S, int64_t offset, int64_t n, double sw, const double* y,
+                        double* nll, double* grad) {
+  if (!S || !y || !nll) return tpa_set_error(TPA_ERR_ARG, "null pointer");
+  Fit F;
+  F.off = offset;
+  F.n = n;
+  F.sw = sw;
+  {
+    std::lock_guard<std::mutex> lk(S->mu);
+    if (S->stop) return tpa_set_error(TPA_ERR_ARG, "server stopped");
+    S->clients += 1;
+  }
+  int rc = TPA_OK;
+  try {
+    *nll = nll_grad_y(S, &F, y, grad);
+  } catch (const Cancelled&) {
+    rc = tpa_set_error(TPA_ERR_ARG, "server stopped");
+  } catch (const EngineFailure&) {
+    std::lock_guard<std::mutex> lk(S->mu);
+    rc = tpa_set_error(TPA_ERR_HIP, "batched evaluation failed: " + S->batch_error);
+  }
+  std::lock_guard<std::mutex> lk(S->mu);
+  S->clients -= 1;
+  S->cv.notify_all();
+  return rc;
+}
+
+int tpa_server_submit(tpa_server_t S, int64_t offset, int64_t n, double sw, const double* y0,
+                      int* fit_id) {
+  if (!S || !y0 || !fit_id) return tpa_set_error(TPA_ERR_ARG, "null pointer");
+  if (n < 1) return tpa_set_error(TPA_ERR_ARG, "event count must be positive");
+  std::lock_guard<std::mutex> lk(S->mu);
+  if (S->stop) return tpa_set_error(TPA_ERR_ARG, "server stopped");
+  Fit* F = new Fit();
+  F->id = (int)S->fits.size();
+  F->off = offset;
+  F->n = n;
+  F->sw = sw;
+  F->x0.assign(y0, y0 + S->nv);
+  S->fits.push_back(F);
+  S->clients += 1;
+  F->th = std::thread(fit_main, S, F);
+  *fit_id = F->id;
+  return TPA_OK;
+}
+
+int tpa_server_poll(tpa_server_t S, int* ids, int max_ids, int* n_done) {
+  if (!S || !n_done || (max_ids > 0 && !ids)) return tpa_set_error(TPA_ERR_ARG, "null pointer");
+  std::vector<Fit*> done;
+  {
+    std::lock_guard<std::mutex> lk(S->mu);
+    const int m = std::min<int>(max_ids, (int)S->finished.size());
+    for (int i = 0; i < m; ++i) {
+      ids[i] = S->finished[i];
+      done.push_back(S->fits[S->finished[i]]);
+    }
+    S->finished.erase(S->finished.begin(), S->finished.begin() + m);
+    *n_done = m;
+  }
+  for (Fit* F : done)
+    if (F->th.joinable()) F->th.join();
+  return TPA_OK;
+}
+
+int tpa_server_result(tpa_server_t S, int fit_id, double* y, double* nll, int* status,
+                      int64_t* nfev, int64_t* niter) {
+  if (!S) return tpa_set_error(TPA_ERR_ARG, "null pointer");
+  std::lock_guard<std::mutex> lk(S->mu);
+  if (fit_id < 0 || fit_id >= (int)S->fits.size()) return tpa_set_error(TPA_ERR_ARG, "bad fit id");
+  Fit* F = S->fits[fit_id];
+  if (F->status < 0) return tpa_set_error(TPA_ERR_ARG, "fit still running");
+  if (y && !F->y.empty()) std::memcpy(y, F->y.data(), sizeof(double) * S->nv);
+  if (nll) *nll = F->nll;
+  if (status) *status = F->status;
+  if (nfev) *nfev = F->nfev;
+  if (niter) *niter = F->niter;
+  return TPA_OK;
+}
+
+int tpa_server_latencies(tpa_server_t S, double* us, int64_t max_n, int64_t* n) {
+  if (!S || !n) return tpa_set_error(TPA_ERR_ARG, "null pointer");
+  std::lock_guard<std::mutex> lk(S->mu);
+  const int64_t m = std::min<int64_t>(max_n, (int64_t)S->lat_us.size());
+  if (m > 0) {
+    if (!us) return tpa_set_error(TPA_ERR_ARG, "null pointer");
+    std::memcpy(us, S->lat_us.data(), sizeof(double) * m);
+  }
+  *n = (int64_t)S->lat_us.size();
+  return TPA_OK;
+}
+
+int tpa_server_counters(tpa_server_t S, int64_t* requests, int64_t* batches, double* gpu_ms) {
+  if (!S) return tpa_set_error(TPA_ERR_ARG, "null pointer");
+  std::lock_guard<std::mutex> lk(S->mu);
+  if (requests) *requests = S->n_requests;
+  if (batches) *batches = S->n_batches;
+  if (gpu_ms) *gpu_ms = S->gpu_ms;
+  return TPA_OK;
+}
+
+}  // extern "C"

@@ -9,7 +9,8 @@ from .amp import AmplitudeModel
 from .config_loader import ConfigLoader
 from .data import EventSet
 from .model import FCN, Model, get_nll_model, register_nll_model
+from .serving import EventPool, FitServer, PoolFull
 from .variable import VarsManager
 
-__all__ = ["AmplitudeModel", "ConfigLoader", "EventSet", "FCN", "Model", "VarsManager",
-           "get_nll_model", "register_nll_model"]
+__all__ = ["AmplitudeModel", "ConfigLoader", "EventPool", "EventSet", "FCN", "FitServer",
+           "Model", "PoolFull", "VarsManager", "get_nll_model", "register_nll_model"]

@@ -77,6 +77,19 @@ class TpaModel(ctypes.Structure):
     ]
 
 
+class TpaCouplingMap(ctypes.Structure):
+    _fields_ = [
+        ("P", _i32), ("n_factor", _i32), ("n_var", _i32), ("reserved", _i32),
+        ("wave_factor", ctypes.POINTER(_i32)),
+        ("wave_const", ctypes.POINTER(ctypes.c_double)),
+        ("factor_polar", ctypes.POINTER(_i32)),
+        ("factor_var", ctypes.POINTER(_i32)),
+        ("factor_fixed", ctypes.POINTER(ctypes.c_double)),
+        ("bound_kind", ctypes.POINTER(_i32)),
+        ("bound_ab", ctypes.POINTER(ctypes.c_double)),
+    ]
+
+
 class EngineError(RuntimeError):
     pass
 
@@ -116,6 +129,27 @@ def lib():
         "tpa_device_synchronize": ([], ctypes.c_int),
         "tpa_gram": ([vp, ctypes.c_double, dp, dp], ctypes.c_int),
         "tpa_calib_copy": ([i64, ctypes.c_int, dp], ctypes.c_int),
+        "tpa_pool_create": ([vp, i64, ctypes.c_int, ctypes.POINTER(vp)], ctypes.c_int),
+        "tpa_pool_destroy": ([vp], ctypes.c_int),
+        "tpa_pool_add": ([vp, dp, i64, dp, ctypes.POINTER(i64)], ctypes.c_int),
+        "tpa_pool_release": ([vp, i64], ctypes.c_int),
+        "tpa_pool_free_events": ([vp], i64),
+        "tpa_pool_eval": ([vp, ctypes.c_int, ctypes.POINTER(i64), dp, ctypes.c_int, dp, dp],
+                          ctypes.c_int),
+        "tpa_server_create": ([vp, ctypes.POINTER(TpaCouplingMap), dp, ctypes.c_double,
+                               ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.POINTER(vp)],
+                              ctypes.c_int),
+        "tpa_server_destroy": ([vp], ctypes.c_int),
+        "tpa_server_nll_grad": ([vp, i64, i64, ctypes.c_double, dp, dp, dp], ctypes.c_int),
+        "tpa_server_submit": ([vp, i64, i64, ctypes.c_double, dp, ctypes.POINTER(ctypes.c_int)],
+                              ctypes.c_int),
+        "tpa_server_poll": ([vp, ctypes.POINTER(ctypes.c_int), ctypes.c_int,
+                             ctypes.POINTER(ctypes.c_int)], ctypes.c_int),
+        "tpa_server_result": ([vp, ctypes.c_int, dp, dp, ctypes.POINTER(ctypes.c_int),
+                               ctypes.POINTER(i64), ctypes.POINTER(i64)], ctypes.c_int),
+        "tpa_server_latencies": ([vp, dp, i64, ctypes.POINTER(i64)], ctypes.c_int),
+        "tpa_server_counters": ([vp, ctypes.POINTER(i64), ctypes.POINTER(i64), dp],
+                                ctypes.c_int),
     }
     for name, (args, res) in sig.items():
         fn = getattr(L, name)
@@ -149,5 +183,8 @@ def exported_symbols():
         "tpa_events_size", "tpa_events_set_weight", "tpa_events_get_angles",
         "tpa_events_build", "tpa_events_get_pw", "tpa_eval", "tpa_timing_enable",
         "tpa_timing_read", "tpa_timing_reset", "tpa_device_synchronize", "tpa_calib_copy",
-        "tpa_gram",
+        "tpa_gram", "tpa_pool_create", "tpa_pool_destroy", "tpa_pool_add", "tpa_pool_release",
+        "tpa_pool_free_events", "tpa_pool_eval", "tpa_server_create", "tpa_server_destroy",
+        "tpa_server_nll_grad", "tpa_server_submit", "tpa_server_poll", "tpa_server_result",
+        "tpa_server_latencies", "tpa_server_counters",
     ]
