@@ -5,8 +5,8 @@ Bars: a batched request equals the single-set pass (tpa_eval) on the same events
 relative (only the summation order differs); a request's result is bit-identical whether it
 is evaluated alone or among other requests, in any order; a fit run alone and the same fit run
 among others end bit-identically; the server's NLL and gradient equal FCN's mixed-precision
-ones to 1e-9 relative; a served fit reaches the NLL minimum that scipy's BFGS on FCN reaches
-from the same start to 1e-4.
+ones to 1e-9 relative; a served fit ends at least as low as scipy's BFGS on FCN from the same
+start (to 1e-4), at a point from which scipy finds nothing lower.
 """
 
 import numpy as np
@@ -191,11 +191,16 @@ def test_served_fit_reaches_scipy_minimum(case2res, gpu_available):
     conf.vm.set_all(y0)
     fcn = FCN(conf.get_model(), conf.data_from_p4(p4), mc, precision="mixed")
     ref = fit_scipy(fcn)
-    conf.vm.set_all(x_true)
     assert res.message in ("converged", "line search failed (precision loss)")
-    assert abs(res.min_nll - ref.min_nll) < 1e-4, (res.min_nll, ref.min_nll)
-    # the reported minimum is the NLL at the reported parameters
+    # at least as deep as scipy's BFGS (which may stop early on precision loss), and a
+    # minimum: scipy restarted from the served result finds nothing lower
+    assert res.min_nll <= ref.min_nll + 1e-4, (res.min_nll, ref.min_nll)
     y = np.array([res.params[n] for n in conf.vm.trainable_vars])
+    conf.vm.set_all(y)
+    again = fit_scipy(fcn)
+    assert again.min_nll >= res.min_nll - 1e-4, (again.min_nll, res.min_nll)
+    conf.vm.set_all(x_true)
+    # the reported minimum is the NLL at the reported parameters
     n_at, _ = srv.nll_grad(slot, y)
     assert abs(n_at - res.min_nll) < 1e-6
     srv.close()
@@ -205,7 +210,7 @@ def test_serve_loop_admission_and_results(case2res, gpu_available):
     """Poisson arrivals into a pool too small for all samples at once: admission waits for
     fits to end, and every fit ends exactly as it does alone."""
     conf = case2res["conf"]
-    sizes = [2500, 3000, 1800, 2200, 2600, 1500]
+    sizes = [2500, 3000, 1800, 2200, 1500, 1000]
     starts = _starts(conf, len(sizes), 4)
     jobs, a = [], 0
     for n, y0 in zip(sizes, starts):

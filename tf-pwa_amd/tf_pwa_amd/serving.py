@@ -35,6 +35,8 @@ class PoolFull(RuntimeError):
 
 def _alpha_weights(n, weight):
     """Data weights with the alpha = sum w / sum w^2 rescale (tf_pwa/model/model.py:603-641)."""
+    if n < 1:
+        raise ValueError("a data sample needs at least one event")
     w = np.ones(n) if weight is None else np.ascontiguousarray(weight, dtype=np.float64).copy()
     if w.shape != (n,):
         raise ValueError("weight must have one entry per event")
