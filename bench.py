@@ -121,6 +121,79 @@ def cpu_baseline(conf, data_p4, n_data_full, n_mc_full, seconds=15.0):
                           nd, nm, dt, reps, n_data_full, n_mc_full)}
 
 
+def serving_leg(conf, mc, data_p4, x_true, args, seed=4242):
+    """Toy-fit serving: ``args.serve_fits`` independent BFGS fits of the six-resonance model
+    arriving as a Poisson stream, sharing the 10M-event MC (its Gram matrix) and an event pool
+    sized by the HBM budget. Per-fit data sizes are lognormal (median 30k, sigma 0.8) clipped to
+    [5k, 200k]; each fit's sample is a seeded resample of the accept-reject data set, and each
+    fit starts from the generating couplings plus N(0, 0.1) noise."""
+    from tf_pwa_amd import FitServer
+
+    rng = np.random.default_rng(seed)
+    m = args.serve_fits
+    sizes = np.clip(np.round(np.exp(np.log(30000) + 0.8 * rng.normal(size=m))), 5000,
+                    200000).astype(np.int64)
+    arrivals = np.cumsum(rng.exponential(1.0 / args.serve_rate, m))
+    names = list(data_p4.keys())
+    n_pool = len(data_p4[names[0]])
+
+    def sample(i, n):
+        def draw():
+            idx = np.random.default_rng(seed + 1 + i).integers(0, n_pool, n)
+            return {k: v[idx] for k, v in data_p4.items()}
+        return draw
+
+    jobs = [dict(p4=sample(i, int(n)), y0=x_true + np.random.default_rng(seed + 100000 + i)
+                 .normal(scale=0.1, size=x_true.size)) for i, n in enumerate(sizes)]
+    t = time.perf_counter()
+    srv = FitServer(conf.get_amplitude(), mc, hbm_budget=args.serve_budget_gb * 1e9,
+                    max_batch=m)
+    setup_s = time.perf_counter() - t
+    log("serving: pool {} events ({:.1f} GB) in {:.1f}s; {} fits, {} events".format(
+        srv.pool.capacity, srv.pool.capacity * srv.pool.bytes_per_event(srv.amp) / 1e9, setup_s,
+        m, int(sizes.sum())))
+    t0 = time.perf_counter()
+    results, times = srv.serve(jobs, arrivals, timeout_s=600)
+    wall = time.perf_counter() - t0
+    lat = srv.latencies_us() / 1e3
+    c = srv.counters()
+    status = {}
+    for r in results:
+        st = status.setdefault(r.message, {"fits": 0, "nfev_mean": 0.0, "niter_mean": 0.0})
+        st["fits"] += 1
+        st["nfev_mean"] += r.nfev
+        st["niter_mean"] += r.niter
+    for st in status.values():
+        st["nfev_mean"] /= st["fits"]
+        st["niter_mean"] /= st["fits"]
+    span = max(t["done"] for t in times) - min(t["arrival"] for t in times)
+    out = {
+        "workload": "{} toy fits of the Lc->pKpi 6-resonance model (34 free couplings), data "
+                    "sizes lognormal median 30k clipped [5k, 200k] (total {}), shared 10M MC "
+                    "Gram, Poisson arrivals at {}/s, HBM budget {} GB".format(
+                        m, int(sizes.sum()), args.serve_rate, args.serve_budget_gb),
+        "evals_per_s": c["requests"] / span,
+        "p50_ms": float(np.percentile(lat, 50)),
+        "p99_ms": float(np.percentile(lat, 99)),
+        "requests": c["requests"],
+        "batches": c["batches"],
+        "mean_batch": c["requests"] / max(1, c["batches"]),
+        "gpu_busy_frac": c["gpu_ms"] / 1e3 / span,
+        "span_s": span,
+        "wall_s": wall,
+        "fits_per_s": m / span,
+        "fit_s_p50": float(np.median([t["done"] - t["admitted"] for t in times])),
+        "admission_wait_s_max": float(max(t["admitted"] - t["arrival"] for t in times)),
+        "peak_pool_GB": srv.peak_events * srv.pool.bytes_per_event(srv.amp) / 1e9,
+        "status": status,
+        "setup_s": setup_s,
+    }
+    log("serving: {:.0f} evals/s, p50 {:.2f} ms, p99 {:.2f} ms, mean batch {:.1f}".format(
+        out["evals_per_s"], out["p50_ms"], out["p99_ms"], out["mean_batch"]))
+    srv.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -129,6 +202,11 @@ def main():
     ap.add_argument("--n-data", type=int, default=1_000_000)
     ap.add_argument("--n-mc", type=int, default=10_000_000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--serve-fits", type=int, default=512,
+                    help="toy fits of the serving leg (0 skips it)")
+    ap.add_argument("--serve-rate", type=float, default=256.0,
+                    help="Poisson arrival rate of the serving leg's fits per second")
+    ap.add_argument("--serve-budget-gb", type=float, default=200.0)
     ap.add_argument("--calib-copy", action="store_true",
                     help="also run a 1 GiB device-to-device copy (the counter calibration "
                          "kernel for FETCH_SIZE / WRITE_SIZE)")
@@ -229,6 +307,9 @@ def main():
                            "peak_TFLOPs": 157.3},
         },
     }
+    if args.serve_fits > 0:
+        del fmix, data
+        res["serving"] = serving_leg(conf, mc, data_p4, x0, args)
     if args.calib_copy:
         cms = ctypes.c_double(0)
         _lib.check(L.tpa_calib_copy(1 << 30, 3, ctypes.byref(cms)))

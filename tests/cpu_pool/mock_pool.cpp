@@ -1,0 +1,116 @@
+// CPU stand-in for the device event pool, for host tests of the serving runtime
+// (tpa_serve.cpp) without a GPU: the same tpa_pool_eval contract (per request: the weighted
+// clip_log data sum and its adjoint G_k) computed in plain loops from a host partial-wave cache
+// h [P][capacity][lam_pad] that the test supplies.
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "tfpwa_amd.h"
+
+struct tpa_pool_s {
+  int P = 0, lg = 0;
+  int64_t cap = 0, used = 0;
+  std::vector<double> h;  // interleaved complex
+  std::vector<double> w;
+  int64_t evals = 0;
+};
+
+static thread_local std::string g_err;
+
+int tpa_set_error(int code, const std::string& msg) {
+  g_err = msg;
+  return code;
+}
+
+extern "C" {
+
+const char* tpa_last_error(void) { return g_err.c_str(); }
+
+int mock_pool_create(int P, int lam_pad, int64_t capacity, tpa_pool_t* out) {
+  tpa_pool_s* p = new tpa_pool_s();
+  p->P = P;
+  p->lg = lam_pad;
+  p->cap = capacity;
+  p->h.assign((size_t)2 * P * capacity * lam_pad, 0.0);
+  p->w.assign(capacity, 0.0);
+  *out = p;
+  return 0;
+}
+
+int mock_pool_destroy(tpa_pool_t p) {
+  delete p;
+  return 0;
+}
+
+// h: [P][n][lam_pad] complex (interleaved) of n events, appended after the last sample.
+int mock_pool_add(tpa_pool_t p, const double* h, int64_t n, const double* w, int64_t* offset) {
+  if (p->used + n > p->cap) return tpa_set_error(-1, "pool full");
+  const int64_t off = p->used;
+  for (int k = 0; k < p->P; ++k)
+    std::memcpy(&p->h[2 * (((size_t)k * p->cap + off) * p->lg)], &h[2 * ((size_t)k * n * p->lg)],
+                sizeof(double) * 2 * n * p->lg);
+  std::memcpy(&p->w[off], w, sizeof(double) * n);
+  p->used += n;
+  *offset = off;
+  return 0;
+}
+
+int64_t mock_pool_evals(tpa_pool_t p) { return p->evals; }
+
+int tpa_pool_eval(tpa_pool_t p, int n_items, const int64_t* ranges, const double* coefs, int mode,
+                  double* out, double* ms) {
+  if (mode != TPA_MODE_DATA) return tpa_set_error(-3, "data sums only");
+  const int P = p->P, lg = p->lg, nacc = 2 + 2 * P;
+  const double eps = 1e-6, logeps = std::log(1e-6);
+  for (int i = 0; i < n_items; ++i) {
+    const int64_t off = ranges[2 * i], n = ranges[2 * i + 1];
+    if (off < 0 || n < 1 || off + n > p->used) return tpa_set_error(-1, "range outside the pool");
+    const double* c = coefs + (size_t)2 * P * i;
+    double* o = out + (size_t)nacc * i;
+    std::memset(o, 0, sizeof(double) * nacc);
+    std::vector<double> ar(lg), ai(lg);
+    for (int64_t e = off; e < off + n; ++e) {
+      double f = 0.0;
+      for (int l = 0; l < lg; ++l) {
+        double sr = 0.0, si = 0.0;
+        for (int k = 0; k < P; ++k) {
+          const double* hv = &p->h[2 * (((size_t)k * p->cap + e) * lg + l)];
+          sr += c[2 * k] * hv[0] - c[2 * k + 1] * hv[1];
+          si += c[2 * k] * hv[1] + c[2 * k + 1] * hv[0];
+        }
+        ar[l] = sr;
+        ai[l] = si;
+        f += sr * sr + si * si;
+      }
+      const double w = p->w[e];
+      const double dom = (w == 0.0) ? 1.0 : w;
+      const double x = (f * w) / dom;
+      double cl, dcl;
+      if (x > eps) {
+        cl = std::log(x);
+        dcl = 1.0 / x;
+      } else {
+        const double t = (x - eps) / eps;
+        cl = (logeps + t) - (t * t) / 2.0;
+        dcl = 1.0 / eps - t / eps;
+      }
+      o[0] += w * cl;
+      const double u = (w * dcl) * (w / dom);
+      for (int k = 0; k < P; ++k)
+        for (int l = 0; l < lg; ++l) {
+          const double* hv = &p->h[2 * (((size_t)k * p->cap + e) * lg + l)];
+          // u conj(A) h
+          o[2 + 2 * k] += u * (ar[l] * hv[0] + ai[l] * hv[1]);
+          o[3 + 2 * k] += u * (ar[l] * hv[1] - ai[l] * hv[0]);
+        }
+    }
+    p->evals += 1;
+  }
+  if (ms) *ms = 0.0;
+  return 0;
+}
+
+}  // extern "C"

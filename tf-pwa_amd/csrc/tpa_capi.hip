@@ -272,6 +272,12 @@ struct tpa_pool_s {
   std::map<int, hipGraphExec_t> graphs;  // grid bucket -> instantiated graph
   hipEvent_t t0 = nullptr, t1 = nullptr;
   hipStream_t stream = nullptr;  // the pool's own stream (graph capture never meets other work)
+  // adds: serialised among themselves, overlapping the batched evaluations on their own stream
+  std::mutex add_mu;
+  hipStream_t add_stream = nullptr;
+  double* d_stage = nullptr;  // p4 [12][n] then weight [n]
+  double* h_stage = nullptr;  // pinned mirror
+  int64_t stage_n = 0;
 };
 
 static void release_range(tpa_pool_s* p, int64_t offset);
@@ -646,6 +652,9 @@ int tpa_pool_destroy(tpa_pool_t p) {
   if (p->t0) (void)hipEventDestroy(p->t0);
   if (p->t1) (void)hipEventDestroy(p->t1);
   if (p->stream) (void)hipStreamDestroy(p->stream);
+  if (p->add_stream) (void)hipStreamDestroy(p->add_stream);
+  if (p->d_stage) (void)hipFree(p->d_stage);
+  if (p->h_stage) (void)hipHostFree(p->h_stage);
   delete p;
   return TPA_OK;
 }
@@ -690,6 +699,7 @@ int tpa_pool_create(tpa_model_t m, int64_t capacity, int max_items, tpa_pool_t* 
   if (e == hipSuccess) e = hipEventCreate(&p->t0);
   if (e == hipSuccess) e = hipEventCreate(&p->t1);
   if (e == hipSuccess) e = hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipStreamCreateWithFlags(&p->add_stream, hipStreamNonBlocking);
   if (e != hipSuccess) {
     tpa_pool_destroy(p);
     return fail(TPA_ERR_HIP, std::string("pool allocation: ") + hipGetErrorString(e));
@@ -711,36 +721,55 @@ int tpa_pool_add(tpa_pool_t p, const double* p4, int64_t n, const double* weight
                  int64_t* offset) {
   if (!p || !p4 || !weight || !offset) return fail(TPA_ERR_ARG, "null pointer");
   if (n < 1) return fail(TPA_ERR_ARG, "event count must be positive");
-  std::lock_guard<std::mutex> lk(p->mu);
-  auto it = p->free_ranges.begin();
-  for (; it != p->free_ranges.end(); ++it)
-    if (it->second >= n) break;
-  if (it == p->free_ranges.end()) return fail(TPA_ERR_ARG, "pool full");
-  const int64_t off = it->first, len = it->second;
-  tpa_model_s* m = p->m;
-  double* d_p4 = nullptr;
-  HIPCHK(hipMalloc(&d_p4, sizeof(double) * 12 * n));
-  p->free_ranges.erase(it);
-  if (len > n) p->free_ranges[off + n] = len - n;
-  p->used_ranges[off] = n;
-  hipError_t e = hipMemcpyAsync(d_p4, p4, sizeof(double) * 12 * n, hipMemcpyHostToDevice,
-                                p->stream);
-  if (e == hipSuccess)
-    e = hipMemcpyAsync(p->d_w + off, weight, sizeof(double) * n, hipMemcpyHostToDevice,
-                       p->stream);
-  if (e == hipSuccess) {
-    hipLaunchKernelGGL(k_cal_angle3, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, p->stream,
-                       m->d, d_p4, n, off, p->cap, p->d_ang);
-    hipLaunchKernelGGL(k_build_pw, dim3((unsigned)((n + 127) / 128)), dim3(128), 0, p->stream,
-                       m->d, p->d_ang, n, off, p->cap, m->d_cg, m->d_dw, 1, p->d_h);
-    e = hipGetLastError();
+  std::lock_guard<std::mutex> add_lk(p->add_mu);
+  int64_t off = 0;
+  {
+    std::lock_guard<std::mutex> lk(p->mu);
+    auto it = p->free_ranges.begin();
+    for (; it != p->free_ranges.end(); ++it)
+      if (it->second >= n) break;
+    if (it == p->free_ranges.end()) return fail(TPA_ERR_ARG, "pool full");
+    off = it->first;
+    const int64_t len = it->second;
+    p->free_ranges.erase(it);
+    if (len > n) p->free_ranges[off + n] = len - n;
+    p->used_ranges[off] = n;
   }
-  if (e == hipSuccess) e = hipStreamSynchronize(p->stream);
-  (void)hipFree(d_p4);
-  if (e != hipSuccess) {
+  auto undo = [&](hipError_t e, const char* what) {
+    std::lock_guard<std::mutex> lk(p->mu);
     release_range(p, off);
-    return fail(TPA_ERR_HIP, std::string("pool add: ") + hipGetErrorString(e));
+    return fail(TPA_ERR_HIP, std::string("pool add: ") + what + ": " + hipGetErrorString(e));
+  };
+  hipError_t e = hipSuccess;
+  if (p->stage_n < n) {  // grow the staging buffers (rare: sizes only increase)
+    const int64_t want = std::max<int64_t>(n, p->stage_n + p->stage_n / 2);
+    if (p->d_stage) (void)hipFree(p->d_stage);
+    if (p->h_stage) (void)hipHostFree(p->h_stage);
+    p->d_stage = nullptr;
+    p->h_stage = nullptr;
+    p->stage_n = 0;
+    if ((e = hipMalloc(&p->d_stage, sizeof(double) * 13 * want)) != hipSuccess)
+      return undo(e, "staging");
+    if ((e = hipHostMalloc(&p->h_stage, sizeof(double) * 13 * want, 0)) != hipSuccess)
+      return undo(e, "pinned staging");
+    p->stage_n = want;
   }
+  std::memcpy(p->h_stage, p4, sizeof(double) * 12 * n);
+  std::memcpy(p->h_stage + 12 * n, weight, sizeof(double) * n);
+  tpa_model_s* m = p->m;
+  hipStream_t s = p->add_stream;
+  if ((e = hipMemcpyAsync(p->d_stage, p->h_stage, sizeof(double) * 13 * n, hipMemcpyHostToDevice,
+                          s)) != hipSuccess)
+    return undo(e, "upload");
+  if ((e = hipMemcpyAsync(p->d_w + off, p->d_stage + 12 * n, sizeof(double) * n,
+                          hipMemcpyDeviceToDevice, s)) != hipSuccess)
+    return undo(e, "weights");
+  hipLaunchKernelGGL(k_cal_angle3, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, m->d,
+                     p->d_stage, n, off, p->cap, p->d_ang);
+  hipLaunchKernelGGL(k_build_pw, dim3((unsigned)((n + 127) / 128)), dim3(128), 0, s, m->d,
+                     p->d_ang, n, off, p->cap, m->d_cg, m->d_dw, 1, p->d_h);
+  if ((e = hipGetLastError()) != hipSuccess) return undo(e, "launch");
+  if ((e = hipStreamSynchronize(s)) != hipSuccess) return undo(e, "build");
   *offset = off;
   return TPA_OK;
 }

@@ -254,8 +254,11 @@ bool line_search(Objective& obj, const std::vector<double>& x, const std::vector
   auto armijo_fails = [&](const Trial& t) {
     return !std::isfinite(t.f) || t.f > f0 + c1 * t.a * d0;
   };
+  // below this change of f the comparisons are rounding noise (|f| ~ 1e4..1e6 for NLLs)
+  const double noise = 1e-14 * (1.0 + std::fabs(f0));
   auto zoom = [&](Trial lo, Trial hi) -> bool {
     for (int it = 0; it < 30; ++it) {
+      if (std::fabs(hi.a - lo.a) * std::fabs(d0) <= noise) break;
       double a = 0.5 * (lo.a + hi.a);
       if (std::isfinite(hi.f) && std::isfinite(hi.d)) {
         const double d1 = lo.d + hi.d - 3.0 * (lo.f - hi.f) / (lo.a - hi.a);
@@ -271,7 +274,7 @@ bool line_search(Objective& obj, const std::vector<double>& x, const std::vector
           }
         }
       }
-      if (std::fabs(hi.a - lo.a) <= 1e-16 * std::max(1.0, std::fabs(lo.a))) return false;
+      if (std::fabs(hi.a - lo.a) <= 1e-16 * std::max(1.0, std::fabs(lo.a))) break;
       Trial t;
       eval(a, t);
       if (armijo_fails(t) || t.f >= lo.f) {
@@ -364,10 +367,6 @@ void run_bfgs(tpa_server_s* S, Fit* F) {
     }
     const double ys = dot(yv, s);
     if (!(ys > 1e-12 * std::sqrt(dot(yv, yv) * dot(s, s)))) continue;  // keep H (curvature)
-    if (k == 1) {  // scale the initial inverse Hessian (Nocedal & Wright eq. 6.20)
-      const double sc = ys / dot(yv, yv);
-      for (double& h : H) h *= sc;
-    }
     const double rho = 1.0 / ys;
     for (int i = 0; i < n; ++i) {
       double acc = 0.0;

@@ -273,8 +273,10 @@ class FitServer:
             if r < 0:
                 r, p = -r, p + np.pi
             params[name + "r"], params[name + "i"] = r, (p + np.pi) % (2 * np.pi) - np.pi
-        return FitResult(params, None, nll, ndf=self.n_var, success=st == 0,
-                         message=STATUS.get(st, str(st)), nfev=nfev, ngrad=nfev)
+        r = FitResult(params, None, nll, ndf=self.n_var, success=st == 0,
+                      message=STATUS.get(st, str(st)), nfev=nfev, ngrad=nfev)
+        r.niter = nit
+        return r
 
     # -------------------------------------------------------------- statistics
     def latencies_us(self):
@@ -305,14 +307,16 @@ class FitServer:
 
     # -------------------------------------------------------------- the serving loop
     def serve(self, jobs, arrivals, poll_s=2e-4, timeout_s=None):
-        """Run ``jobs`` (each a dict with "p4", "y0" and optional "weight", "order") arriving at
-        ``arrivals`` seconds from now. Admission is first come, first served: the head of the
+        """Run ``jobs`` (each a dict with "p4", "y0" and optional "weight", "order"; "p4" may be a
+        callable that draws the sample when the fit is admitted) arriving at ``arrivals``
+        seconds from now. Admission is first come, first served: the head of the
         queue waits until the pool has a free range for its sample. Returns (results in job
         order, per-fit dict of times: arrival, admitted, done)."""
         t0 = time.perf_counter()
         order = np.argsort(np.asarray(arrivals), kind="stable")
         queue = list(order)
         waiting = []  # arrived, not admitted
+        staged = {}  # p4 of waiting jobs whose sample was already drawn
         running = {}  # fit id -> job index
         results = [None] * len(jobs)
         times = [dict(arrival=float(arrivals[j])) for j in range(len(jobs))]
@@ -324,13 +328,16 @@ class FitServer:
                 waiting.append(queue.pop(0))
             while waiting:
                 j = waiting[0]
+                if j not in staged:
+                    p4 = jobs[j]["p4"]
+                    staged[j] = p4() if callable(p4) else p4
                 try:
-                    slot = self.add_data(jobs[j]["p4"], jobs[j].get("weight"),
-                                         jobs[j].get("order"))
+                    slot = self.add_data(staged[j], jobs[j].get("weight"), jobs[j].get("order"))
                 except PoolFull:
                     if not running:
                         raise
                     break
+                del staged[j]
                 waiting.pop(0)
                 fid = self.submit(slot, jobs[j]["y0"])
                 running[fid] = j
