@@ -123,6 +123,21 @@ int tpa_timing_reset(void);
  * ms (optional): the MFMA kernel's duration. */
 int tpa_gram(tpa_events_t ev, double w_mean, double* out, double* ms);
 
+/* Second moments for the exact Hessian (fold = 1 cache, P <= 32). out (host, 2 + 2P + Q(Q+1)
+ * doubles, Q = 2P): out[0] the sum of the mode, out[2+2k], out[3+2k] Re/Im G_k as in tpa_eval,
+ * then the packed upper triangles (row-major, i <= j) of
+ *   O1 = sum_e v_e z_e z_e^T with z_e = (Re q_e, Im q_e), q_ek = sum_lambda conj(A) h_k,
+ *   O2 = sum_e u_e sum_lambda y y^T with y = (Re h_.lambda, Im h_.lambda),
+ * u_e, v_e the first and second derivative weights of the mode (MC: u = w, v = 0, so O2 is the
+ * FP64 Gram matrix). The second derivatives of the sum in (c, conj c) are
+ *   d2/dc_k dc_l = sum v q_k q_l,  d2/dc_k dconj(c_l) = sum v q_k conj(q_l) + sum u h_k conj(h_l).
+ * Replaces the nested tf.GradientTape Hessians (sum_hessian, tf_pwa/model/model.py:97-150) of
+ * BaseModel.nll_grad_hessian (513-560) for the coupling parameters. */
+int tpa_hess(tpa_events_t ev, const double* coef, int mode, double* out);
+
+/* The same for the data range [offset, offset+n) of a pool allocation (TPA_MODE_DATA). */
+int tpa_pool_hess(tpa_pool_t pool, int64_t offset, int64_t n, const double* coef, double* out);
+
 /* Wait for all device work of the process (the timing barrier of benchmarks). */
 int tpa_device_synchronize(void);
 

@@ -918,7 +918,7 @@ def nll_grad_hessian(model, ang_data, ang_mc, w, wm, x=None):
     for i in range(len(g)):
         (h,) = torch.autograd.grad(g[i], x, retain_graph=True)
         rows.append(h)
-    return float(nll), g.detach().numpy(), torch.stack(rows).numpy()
+    return float(nll.detach()), g.detach().numpy(), torch.stack(rows).numpy()
 
 
 # --------------------------------------------------------------------------------------

@@ -81,6 +81,31 @@ def test_nll_grad_matches_oracle(which, request, gpu_available):
     assert abs(fcn(None) - nll) < 1e-9 * max(1.0, abs(nll))
 
 
+@pytest.mark.parametrize("which", ["case1", "case2"])
+def test_exact_hessian_matches_oracle(which, request, gpu_available):
+    """Second-moment passes + coupling chain rule against the oracle's autograd Hessian
+    (FP64 path), and the mixed path's Hessian against the FP64 one."""
+    from tf_pwa_amd import FCN
+
+    case = request.getfixturevalue(which)
+    conf = case["conf"]
+    data = conf.data_from_p4(case["data"], weight=case["wd"])
+    mc = conf.data_from_p4(case["mc"])
+    fcn = FCN(conf.get_model(), data, mc)
+    nll, g, H = fcn.nll_grad_hessian()
+    ref_nll, ref_g, ref_H = oracle_nll_grad(case, hessian=True)
+    assert abs(nll - ref_nll) < 1e-6
+    assert np.linalg.norm(g - ref_g) <= 1e-7 * np.linalg.norm(ref_g)
+    scale = np.abs(ref_H).max()
+    assert np.abs(H - ref_H).max() <= 1e-7 * scale, np.abs(H - ref_H).max() / scale
+    # the engine's own gradient path agrees with the second-moment pass
+    n2, g2 = fcn.nll_grad()
+    assert abs(n2 - nll) < 1e-9 * abs(nll) and np.abs(g2 - g).max() < 1e-9 * np.abs(g).max()
+    mix = FCN(conf.get_model(), data, mc, precision="mixed")
+    _, _, Hm = mix.nll_grad_hessian()
+    assert np.abs(Hm - H).max() <= 1e-4 * scale
+
+
 def test_gram_matches_partial_waves(case2, gpu_available):
     """FP32 MFMA Gram against the FP64 Gram of the same partial waves (read back)."""
     conf = case2["conf"]

@@ -242,3 +242,78 @@ def test_serving_coupling_map_reproduces_coefficients(cfg):
         assert keep["factor_var"].max() < len(c.vm.trainable_vars)
         # the fixed chain total of fix_chain_idx appears as a fixed factor
         assert (keep["factor_var"] < 0).any()
+
+
+def test_hessian_chain_rule_on_a_known_function():
+    """hess_from_adjoint / grad_from_adjoint against central differences of
+    F(theta) = 2 Re(a^T c) + 2 Re(c^T B c) + c^H C c, whose Wirtinger derivatives are
+    F1 = a + 2 B c + C^T conj(c), Fcc = 2 B, Fcb = C^T."""
+    import yaml
+
+    from tf_pwa_amd import ConfigLoader
+
+    for polar in (True, False):
+        conf = yaml.safe_load(open(config_path("lc_pkpi_6res.yml")))
+        conf["polar"] = polar
+        c = ConfigLoader(conf, seed=8)
+        amp = c.get_amplitude()
+        P = amp.P
+        rng = np.random.default_rng(4)
+        a = rng.normal(size=P) + 1j * rng.normal(size=P)
+        B = rng.normal(size=(P, P)) + 1j * rng.normal(size=(P, P))
+        B = 0.5 * (B + B.T)
+        C = rng.normal(size=(P, P)) + 1j * rng.normal(size=(P, P))
+        C = 0.5 * (C + C.conj().T)
+
+        def F(x):
+            c.vm.set_all(x)
+            cc = amp.coefficients()
+            return float(2 * np.real(a @ cc) + 2 * np.real(cc @ B @ cc)
+                         + np.real(cc.conj() @ C @ cc))
+
+        x0 = np.array(c.vm.get_all_val())
+        c.vm.set_all(x0)
+        cc = amp.coefficients()
+        F1 = a + 2 * B @ cc + C.T @ cc.conj()
+        g = amp.grad_from_adjoint(F1)
+        H = amp.hess_from_adjoint(F1, 2 * B, C.T)
+        n = len(x0)
+        h = 1e-4
+        for i in range(n):
+            e = np.zeros(n)
+            e[i] = h
+            gd = (F(x0 + e) - F(x0 - e)) / (2 * h)
+            assert abs(gd - g[i]) < 1e-6 * max(1, abs(g).max())
+        for i in range(0, n, 3):
+            for j in range(0, n, 2):
+                ei, ej = np.zeros(n), np.zeros(n)
+                ei[i], ej[j] = h, h
+                fd = (F(x0 + ei + ej) - F(x0 + ei - ej) - F(x0 - ei + ej) + F(x0 - ei - ej)) / (
+                    4 * h * h)
+                assert abs(fd - H[i, j]) < 1e-5 * max(1, np.abs(H).max()), (i, j, fd, H[i, j])
+        c.vm.set_all(x0)
+
+
+def test_second_moment_decode_layout():
+    """decode_second_moments against explicit sums over synthetic q and h vectors packed the
+    way the kernel packs them (upper triangles of O1 = sum v z z^T, O2 = sum u y y^T)."""
+    from tf_pwa_amd.data import decode_second_moments
+
+    P, n, L = 5, 40, 3
+    rng = np.random.default_rng(0)
+    h = rng.normal(size=(n, L, P)) + 1j * rng.normal(size=(n, L, P))
+    q = rng.normal(size=(n, P)) + 1j * rng.normal(size=(n, P))
+    u, v = rng.normal(size=n), rng.normal(size=n)
+    z = np.concatenate([q.real, q.imag], 1)
+    y = np.concatenate([h.real, h.imag], 2).reshape(n * L, 2 * P)
+    O1 = np.einsum("e,ei,ej->ij", v, z, z)
+    O2 = np.einsum("e,ei,ej->ij", np.repeat(u, L), y, y)
+    iu = np.triu_indices(2 * P)
+    G = (u[:, None] * q).sum(0)
+    out = np.concatenate([[1.5, 0.0], G.view(np.float64), O1[iu], O2[iu]])
+    val, G2, Dcc, Dcb = decode_second_moments(out, P)
+    assert val == 1.5
+    np.testing.assert_allclose(G2, G, rtol=1e-13)
+    np.testing.assert_allclose(Dcc, np.einsum("e,ek,el->kl", v, q, q), rtol=1e-12, atol=1e-12)
+    ref = np.einsum("e,ek,el->kl", v, q, q.conj()) + np.einsum("e,elk,elm->km", u, h, h.conj())
+    np.testing.assert_allclose(Dcb, ref, rtol=1e-12, atol=1e-12)

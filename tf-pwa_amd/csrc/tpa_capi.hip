@@ -149,6 +149,9 @@ struct tpa_events_s {
   double* d_gpart = nullptr;
   double* d_gout = nullptr;
   double2* d_bw = nullptr;
+  double* d_hpart = nullptr;  // second-moment pass: partials and result
+  double* d_hout = nullptr;
+  double2* d_hcoef = nullptr;
   int built_fold = -1;
   hipEvent_t t0 = nullptr, t1 = nullptr;
 };
@@ -278,9 +281,62 @@ struct tpa_pool_s {
   double* d_stage = nullptr;  // p4 [12][n] then weight [n]
   double* h_stage = nullptr;  // pinned mirror
   int64_t stage_n = 0;
+  double* d_hpart = nullptr;  // second-moment pass (tpa_pool_hess)
+  double* d_hout = nullptr;
+  double2* d_hcoef = nullptr;
 };
 
 static void release_range(tpa_pool_s* p, int64_t offset);
+
+// ---- second-moment pass (k_hess) shared by event sets and the pool
+constexpr int kHessGridMax = 512;
+
+static int hess_nacc(int P) { return 2 + 2 * P + 2 * P * (2 * P + 1); }
+
+template <int LG>
+static void launch_hess(const HessArgs& a, int grid, hipStream_t s) {
+  hipLaunchKernelGGL(k_hess<LG>, dim3(grid), dim3(256), 0, s, a);
+}
+
+static int run_hess(const TpaModel& M, const double2* h, const double* w, int64_t off, int64_t n,
+                    int64_t S, const double* coef, int mode, hipStream_t s, double** d_part,
+                    double** d_out, double2** d_coef, double* out) {
+  if (M.P > TPA_HESS_PMAX) return fail(TPA_ERR_UNSUPPORTED, "second moments need P <= 32");
+  const int nacc = hess_nacc(M.P);
+  const int cap_nacc = hess_nacc(TPA_HESS_PMAX);
+  if (!*d_part) HIPCHK(hipMalloc(d_part, sizeof(double) * (size_t)kHessGridMax * cap_nacc));
+  if (!*d_out) HIPCHK(hipMalloc(d_out, sizeof(double) * cap_nacc));
+  if (!*d_coef) HIPCHK(hipMalloc(d_coef, sizeof(double2) * TPA_HESS_PMAX));
+  HIPCHK(hipMemcpyAsync(*d_coef, coef, sizeof(double2) * M.P, hipMemcpyHostToDevice, s));
+  HessArgs a;
+  a.h = h;
+  a.w = w;
+  a.coef = *d_coef;
+  a.partials = *d_part;
+  a.off = off;
+  a.n = n;
+  a.S = S;
+  a.P = M.P;
+  a.mode = mode;
+  a.nacc = nacc;
+  const int E = TPA_HESS_PAIRS / M.lam_pad;
+  const int64_t tiles = (n + E - 1) / E;
+  const int grid = (int)std::max<int64_t>(1, std::min<int64_t>(tiles, kHessGridMax));
+  switch (M.lam_pad) {
+    case 1: launch_hess<1>(a, grid, s); break;
+    case 2: launch_hess<2>(a, grid, s); break;
+    case 4: launch_hess<4>(a, grid, s); break;
+    case 8: launch_hess<8>(a, grid, s); break;
+    case 16: launch_hess<16>(a, grid, s); break;
+    default: return fail(TPA_ERR_UNSUPPORTED, "no second-moment kernel for this helicity count");
+  }
+  HIPCHK(hipGetLastError());
+  hipLaunchKernelGGL(k_reduce, dim3(nacc), dim3(256), 0, s, *d_part, grid, nacc, *d_out);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(out, *d_out, sizeof(double) * nacc, hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+  return TPA_OK;
+}
 
 int tpa_set_error(int code, const std::string& msg) { return fail(code, msg); }
 
@@ -407,6 +463,9 @@ int tpa_events_destroy(tpa_events_t ev) {
   if (ev->d_gpart) (void)hipFree(ev->d_gpart);
   if (ev->d_gout) (void)hipFree(ev->d_gout);
   if (ev->d_bw) (void)hipFree(ev->d_bw);
+  if (ev->d_hpart) (void)hipFree(ev->d_hpart);
+  if (ev->d_hout) (void)hipFree(ev->d_hout);
+  if (ev->d_hcoef) (void)hipFree(ev->d_hcoef);
   if (ev->t0) (void)hipEventDestroy(ev->t0);
   if (ev->t1) (void)hipEventDestroy(ev->t1);
   delete ev;
@@ -655,6 +714,9 @@ int tpa_pool_destroy(tpa_pool_t p) {
   if (p->add_stream) (void)hipStreamDestroy(p->add_stream);
   if (p->d_stage) (void)hipFree(p->d_stage);
   if (p->h_stage) (void)hipHostFree(p->h_stage);
+  if (p->d_hpart) (void)hipFree(p->d_hpart);
+  if (p->d_hout) (void)hipFree(p->d_hout);
+  if (p->d_hcoef) (void)hipFree(p->d_hcoef);
   delete p;
   return TPA_OK;
 }
@@ -884,6 +946,26 @@ int tpa_pool_eval(tpa_pool_t p, int n_items, const int64_t* ranges, const double
   }
   std::memcpy(out, p->h_out, sizeof(double) * (size_t)n_items * nacc);
   return TPA_OK;
+}
+
+int tpa_hess(tpa_events_t ev, const double* coef, int mode, double* out) {
+  if (!ev || !coef || !out) return fail(TPA_ERR_ARG, "null pointer");
+  if (mode != TPA_MODE_DATA && mode != TPA_MODE_MC) return fail(TPA_ERR_ARG, "bad mode");
+  if (ev->built_fold != 1) return fail(TPA_ERR_ARG, "second moments need a fold = 1 cache");
+  return run_hess(ev->m->h, ev->d_h, ev->d_w, 0, ev->n, ev->n, coef, mode, ev->m->stream,
+                  &ev->d_hpart, &ev->d_hout, &ev->d_hcoef, out);
+}
+
+int tpa_pool_hess(tpa_pool_t p, int64_t offset, int64_t n, const double* coef, double* out) {
+  if (!p || !coef || !out) return fail(TPA_ERR_ARG, "null pointer");
+  std::lock_guard<std::mutex> lk(p->mu);
+  auto it = p->used_ranges.upper_bound(offset);
+  if (n < 1 || it == p->used_ranges.begin()) return fail(TPA_ERR_ARG, "range outside the pool");
+  --it;
+  if (offset < it->first || offset + n > it->first + it->second)
+    return fail(TPA_ERR_ARG, "range outside an allocation");
+  return run_hess(p->m->h, p->d_h, p->d_w, offset, n, p->cap, coef, TPA_MODE_DATA, p->stream,
+                  &p->d_hpart, &p->d_hout, &p->d_hcoef, out);
 }
 
 int tpa_device_synchronize(void) {

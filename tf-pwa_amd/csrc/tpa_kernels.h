@@ -968,3 +968,198 @@ __global__ __launch_bounds__(256) void k_reduce_items(const double* __restrict__
     out[(int64_t)item * nacc + j] = s + c;
   }
 }
+
+// ------------------------------------------------------------------ second derivatives
+// One pass over events [off, off+n) of a fold-1 cache with coefficients c that accumulates,
+// besides the sum and its adjoint G_k, the two real second-moment matrices from which the
+// exact Hessian of the sum in (c, conj c) follows (Q = 2P):
+//   O1 = sum_e v_e z_e z_e^T,            z_e = (Re q_e, Im q_e), q_ek = sum_l conj(A_el) h_ekl
+//   O2 = sum_e u_e sum_l y_el y_el^T,    y_el = (Re h_e.l, Im h_e.l)
+// with u_e = w dL/df and v_e = w d2L/df2 (data: L = clip_log of f w / dom; MC: L = f, v = 0,
+// and O2 is the FP64 Gram matrix). Both are stored as packed upper triangles (row-major, i <= j)
+// after the value and G (interleaved re / im, the layout of k_eval_fold's partials).
+// A block stages TPA_HESS_PAIRS (event, helicity) pairs of the cache in LDS per tile and each
+// thread owns one or two 4x4 register blocks of O1 / O2.
+#define TPA_HESS_PAIRS 128
+#define TPA_HESS_PMAX 32
+
+struct HessArgs {
+  const double2* __restrict__ h;     // [P][S][LG]
+  const double* __restrict__ w;      // [S]
+  const double2* __restrict__ coef;  // [P]
+  double* __restrict__ partials;     // [grid][nacc]
+  int64_t off, n, S;
+  int P;
+  int mode;
+  int nacc;  // 2 + 2P + Q(Q+1)
+};
+
+TPA_DEV int tri_index(int i, int j, int Q) { return i * Q - (i * (i - 1)) / 2 + (j - i); }
+
+template <int LG>
+__global__ __launch_bounds__(256) void k_hess(HessArgs a) {
+  constexpr int E = TPA_HESS_PAIRS / LG;  // events per tile
+  const int P = a.P, Q = 2 * P;
+  const int nb = (Q + 3) / 4;             // 4-wide row/column blocks
+  const int NB = nb * (nb + 1) / 2;        // upper-triangle blocks per matrix
+  __shared__ double2 hs[TPA_HESS_PMAX][TPA_HESS_PAIRS];
+  __shared__ double2 As[TPA_HESS_PAIRS];
+  __shared__ double zs[E][2 * TPA_HESS_PMAX + 4];
+  __shared__ double us[E], vs[E];
+  __shared__ double red_s[256], red_c[256];
+  const int t = threadIdx.x;
+  const double eps = 1.0e-6;
+  const double logeps = log(1.0e-6);
+  // the (up to two) register blocks of this thread: ids < NB are O1 blocks, then O2
+  int blk[2], bi[2], bj[2];
+  for (int s = 0; s < 2; ++s) {
+    blk[s] = t + 256 * s;
+    bi[s] = bj[s] = 0;
+    if (blk[s] >= 2 * NB) continue;  // no block for this slot
+    int id = blk[s] < NB ? blk[s] : blk[s] - NB;  // id < NB = nb (nb + 1) / 2
+    int r = 0;
+    while (id >= nb - r) {
+      id -= nb - r;
+      ++r;
+    }
+    bi[s] = r;
+    bj[s] = r + id;
+  }
+  double acc[2][4][4];
+  for (int s = 0; s < 2; ++s)
+    for (int i = 0; i < 4; ++i)
+      for (int j = 0; j < 4; ++j) acc[s][i][j] = 0.0;
+  double gacc = 0.0;  // thread t < Q: component t of (Re G, Im G)
+  double vsum = 0.0, vcomp = 0.0;
+  const int64_t tiles = (a.n + E - 1) / E;
+  for (int64_t tile = blockIdx.x; tile < tiles; tile += gridDim.x) {
+    const int64_t e0 = a.off + tile * E;
+    const int64_t e_end = a.off + a.n;
+    // stage the tile's cache: for each wave, E * LG contiguous pairs
+    for (int idx = t; idx < P * TPA_HESS_PAIRS; idx += 256) {
+      const int k = idx / TPA_HESS_PAIRS, p = idx % TPA_HESS_PAIRS;
+      const int64_t e = e0 + p / LG;
+      hs[k][p] = (e < e_end) ? a.h[((int64_t)k * a.S + e) * LG + (p % LG)] : make_double2(0.0, 0.0);
+    }
+    __syncthreads();
+    if (t < TPA_HESS_PAIRS) {
+      double ar = 0.0, ai = 0.0;
+      for (int k = 0; k < P; ++k) {
+        const double2 c = a.coef[k], hv = hs[k][t];
+        ar += c.x * hv.x - c.y * hv.y;
+        ai += c.x * hv.y + c.y * hv.x;
+      }
+      As[t] = make_double2(ar, ai);
+    }
+    __syncthreads();
+    if (t < E) {
+      const int64_t e = e0 + t;
+      double u = 0.0, v = 0.0;
+      if (e < e_end) {
+        double f = 0.0;
+        for (int l = 0; l < LG; ++l) f += As[t * LG + l].x * As[t * LG + l].x + As[t * LG + l].y * As[t * LG + l].y;
+        const double w = a.w[e];
+        double y;
+        if (a.mode == 0) {
+          const double dom = (w == 0.0) ? 1.0 : w;
+          const double r = w / dom;
+          const double x = f * r;
+          double cl, d1, d2;
+          if (x > eps) {
+            cl = log(x);
+            d1 = 1.0 / x;
+            d2 = -1.0 / (x * x);
+          } else {
+            const double tt = (x - eps) / eps;
+            cl = (logeps + tt) - (tt * tt) / 2.0;
+            d1 = 1.0 / eps - tt / eps;
+            d2 = -1.0 / (eps * eps);
+          }
+          y = w * cl;
+          u = (w * d1) * r;
+          v = (w * d2) * (r * r);
+        } else {
+          y = w * f;
+          u = w;
+        }
+        neumaier_add(vsum, vcomp, y);
+      }
+      us[t] = u;
+      vs[t] = v;
+    }
+    for (int idx = t; idx < E * P; idx += 256) {
+      const int e = idx / P, k = idx % P;
+      double qr = 0.0, qi = 0.0;
+      for (int l = 0; l < LG; ++l) {
+        const double2 A = As[e * LG + l], hv = hs[k][e * LG + l];
+        qr += A.x * hv.x + A.y * hv.y;  // conj(A) h
+        qi += A.x * hv.y - A.y * hv.x;
+      }
+      zs[e][k] = qr;
+      zs[e][P + k] = qi;
+    }
+    __syncthreads();
+    if (t < Q)
+      for (int e = 0; e < E; ++e) gacc += us[e] * zs[e][t];
+    for (int s = 0; s < 2; ++s) {
+      if (blk[s] >= 2 * NB) continue;
+      const int r0 = 4 * bi[s], c0 = 4 * bj[s];
+      if (blk[s] < NB) {
+        for (int e = 0; e < E; ++e) {
+          double x[4], y[4];
+          for (int i = 0; i < 4; ++i) {
+            x[i] = (r0 + i < Q) ? vs[e] * zs[e][r0 + i] : 0.0;
+            y[i] = (c0 + i < Q) ? zs[e][c0 + i] : 0.0;
+          }
+          for (int i = 0; i < 4; ++i)
+            for (int j = 0; j < 4; ++j) acc[s][i][j] += x[i] * y[j];
+        }
+      } else {
+        for (int p = 0; p < TPA_HESS_PAIRS; ++p) {
+          const double u = us[p / LG];
+          double x[4], y[4];
+          for (int i = 0; i < 4; ++i) {
+            const int ri = r0 + i, ci = c0 + i;
+            const double hr = ri < P ? hs[ri][p].x : (ri < Q ? hs[ri - P][p].y : 0.0);
+            const double hc = ci < P ? hs[ci][p].x : (ci < Q ? hs[ci - P][p].y : 0.0);
+            x[i] = u * hr;
+            y[i] = hc;
+          }
+          for (int i = 0; i < 4; ++i)
+            for (int j = 0; j < 4; ++j) acc[s][i][j] += x[i] * y[j];
+        }
+      }
+    }
+    __syncthreads();
+  }
+  // write this block's partials: [0] value, [1] compensation, [2..2+Q) G, O1, O2 triangles
+  double* out = a.partials + (int64_t)blockIdx.x * a.nacc;
+  red_s[t] = vsum;
+  red_c[t] = vcomp;
+  __syncthreads();
+  for (int m = 128; m > 0; m >>= 1) {
+    if (t < m) {
+      double s1 = red_s[t], c1 = red_c[t];
+      neumaier_merge(s1, c1, red_s[t + m], red_c[t + m]);
+      red_s[t] = s1;
+      red_c[t] = c1;
+    }
+    __syncthreads();
+  }
+  if (t == 0) {
+    out[0] = red_s[0];
+    out[1] = red_c[0];
+  }
+  if (t < Q) out[t < P ? 2 + 2 * t : 3 + 2 * (t - P)] = gacc;  // interleaved re / im
+  const int T = Q * (Q + 1) / 2;
+  for (int s = 0; s < 2; ++s) {
+    if (blk[s] >= 2 * NB) continue;
+    double* o = out + 2 + Q + (blk[s] < NB ? 0 : T);
+    const int r0 = 4 * bi[s], c0 = 4 * bj[s];
+    for (int i = 0; i < 4; ++i)
+      for (int j = 0; j < 4; ++j) {
+        const int ri = r0 + i, cj = c0 + j;
+        if (ri < Q && cj < Q && ri <= cj) o[tri_index(ri, cj, Q)] = acc[s][i][j];
+      }
+  }
+}

@@ -441,6 +441,51 @@ class AmplitudeModel:
                                        minlength=n)
         return out
 
+    def hess_from_adjoint(self, F1, Fcc, Fcb):
+        """Hessian in the trainable couplings of a real function F of the coefficients, from
+        its Wirtinger derivatives F1_k = dF/dc_k, Fcc_kl = d2F/dc_k dc_l and
+        Fcb_kl = d2F/dc_k dconj(c_l):
+        H = 2 Re[J Fcc J^T + J Fcb J^H + sum_k F1_k d2c_k/dtheta dtheta] (J = jacobian())."""
+        J = self.jacobian()
+        H = 2.0 * np.real(J @ Fcc @ J.T + J @ Fcb @ J.conj().T)
+        row_r, row_i = self._grad_rows()
+        fx = self._factor_index()
+        val, d_r, d_i = self._factor_values()
+        polar = np.array([self.vm.complex_vars.get(n, self.vm.polar) for n in fx["cnames"]])
+        C = self._barrier_per_wave() * np.asarray(F1)
+        slots = (fx["it"], fx["i1"], fx["i2"])
+        n = H.shape[0]
+        M = np.zeros((n, n), dtype=np.complex128)
+        i_unit = 1j
+        for a in range(3):
+            f = slots[a]
+            rest = C.copy().astype(np.complex128)
+            for b in range(3):
+                if b != a:
+                    rest = rest * val[slots[b]]
+            # second derivatives of one polar factor: d2/dr dphi = i e^{i phi},
+            # d2/dphi2 = -r e^{i phi} (cartesian factors are linear)
+            e = d_r[f]
+            rr, ri = row_r[f], row_i[f]
+            m = polar[f] & (rr >= 0) & (ri >= 0)
+            np.add.at(M, (rr[m], ri[m]), i_unit * e[m] * rest[m])
+            np.add.at(M, (ri[m], rr[m]), i_unit * e[m] * rest[m])
+            m = polar[f] & (ri >= 0)
+            np.add.at(M, (ri[m], ri[m]), -val[f][m] * rest[m])
+            # products of first derivatives of two different factors
+            for b in range(3):
+                if b == a:
+                    continue
+                g = slots[b]
+                third = [s_ for s_ in range(3) if s_ not in (a, b)][0]
+                rest2 = C * val[slots[third]]
+                for rows_a, da in ((row_r, d_r), (row_i, d_i)):
+                    for rows_b, db in ((row_r, d_r), (row_i, d_i)):
+                        ra, rb = rows_a[f], rows_b[g]
+                        m = (ra >= 0) & (rb >= 0)
+                        np.add.at(M, (ra[m], rb[m]), da[f][m] * db[g][m] * rest2[m])
+        return H + 2.0 * np.real(M)
+
     def shape_grad(self, G, Dm, Dg):
         """Gradient of the lineshape parameters: for the resonance of chain c,
         d/dm0 = 2 Re sum_{k in c} [(d c_k/d m0) G_k + c_k Dm_k] (the barrier constants of both

@@ -33,6 +33,29 @@ def p4_array(p4, finals, order=None):
     return out
 
 
+def _unpack_upper(v, Q):
+    M = np.zeros((Q, Q))
+    iu = np.triu_indices(Q)
+    M[iu] = v
+    M.T[iu] = v
+    return M
+
+
+def decode_second_moments(out, P):
+    """(value, G, Dcc, Dcb) from a tpa_hess / tpa_pool_hess result (layout in tfpwa_amd.h):
+    Dcc = sum v q q^T, Dcb = sum v q q^H + sum u h h^H from the real moments O1, O2."""
+    Q = 2 * P
+    T = Q * (Q + 1) // 2
+    G = out[2:2 + 2 * P].view(np.complex128).copy()
+    O1 = _unpack_upper(out[2 + 2 * P:2 + 2 * P + T], Q)
+    O2 = _unpack_upper(out[2 + 2 * P + T:2 + 2 * P + 2 * T], Q)
+    a, b = slice(0, P), slice(P, Q)
+    Dcc = (O1[a, a] - O1[b, b]) + 1j * (O1[a, b] + O1[b, a])
+    Dcb = ((O1[a, a] + O1[b, b]) + 1j * (O1[b, a] - O1[a, b])
+           + (O2[a, a] + O2[b, b]) + 1j * (O2[b, a] - O2[a, b]))
+    return out[0], G, Dcc, Dcb
+
+
 class EventSet:
     """One sample on the device. ``amp`` is the AmplitudeModel whose description the
     kernels use; ``weight`` defaults to 1."""
@@ -114,6 +137,19 @@ class EventSet:
             z = out[2:].view(np.complex128)
             G = (z[:P].copy(), z[P:2 * P].copy(), z[2 * P:].copy()) if shape else z.copy()
         return out[0], G, f
+
+    def hess(self, coef, mode):
+        """Second-moment pass: (value, G, Dcc, Dcb) with G_k = d sum / dc_k,
+        Dcc_kl = d2 sum / dc_k dc_l and Dcb_kl = d2 sum / dc_k dconj(c_l) (complex [P, P])."""
+        self.ensure_built()
+        if not self._state[0]:
+            raise NotImplementedError("second moments need fixed lineshapes (fold = 1 cache)")
+        P = self.amp.P
+        Q = 2 * P
+        c = np.ascontiguousarray(np.asarray(coef, dtype=np.complex128)).view(np.float64)
+        out = np.zeros(2 + 2 * P + Q * (Q + 1), dtype=np.float64)
+        _lib.check(_lib.lib().tpa_hess(self._h, _lib.dptr(c), mode, _lib.dptr(out)))
+        return decode_second_moments(out, P)
 
     def gram(self):
         """Partial-wave Gram matrix I_kl = sum_e w_e sum_lambda conj(h_k) h_l (FP32 MFMA with

@@ -166,6 +166,47 @@ class Model:
             g = g + amp.shape_grad(*comb)
         return nll, g
 
+    def _nll_grad_hess(self, wdata, mc_sets, mc_w, split=None):
+        """NLL, gradient and exact Hessian in the trainable couplings (fixed lineshapes):
+        NLL = -D + sw log S with D, S and their Wirtinger derivatives from the second-moment
+        passes (the MC through its Gram matrix in mixed precision), then the coupling chain
+        rule (AmplitudeModel.hess_from_adjoint). Follows BaseModel.nll_grad_hessian
+        (tf_pwa/model/model.py:513-560): h = -h_data + sw (g_mc g_mc^T int_h + h_mc int_g)."""
+        amp = self.Amp
+        coef = amp.coefficients()
+        if split is None:
+            split = (wdata.split(), self._split(mc_sets, mc_w), wdata.weight.sum())
+        dw, mw, sw = split
+        P = amp.P
+        D, Dk = 0.0, np.zeros(P, dtype=np.complex128)
+        Dcc = np.zeros((P, P), dtype=np.complex128)
+        Dcb = np.zeros((P, P), dtype=np.complex128)
+        for s, w in zip(wdata.sets, dw):
+            s.set_weight(w)
+            v, g, cc, cb = s.hess(coef, _lib.MODE_DATA)
+            D, Dk, Dcc, Dcb = D + v, Dk + g, Dcc + cc, Dcb + cb
+        S, Sk = 0.0, np.zeros(P, dtype=np.complex128)
+        Scb = np.zeros((P, P), dtype=np.complex128)
+        for s, w in zip(mc_sets, mw):
+            s.set_weight(w)
+            if self.precision == "mixed":
+                I = s.gram()
+                Ic = I @ coef
+                S += float(np.real(np.vdot(coef, Ic)))
+                Sk += np.conj(Ic)
+                Scb += I.T
+            else:
+                v, g, _, cb = s.hess(coef, _lib.MODE_MC)
+                S, Sk, Scb = S + v, Sk + g, Scb + cb
+        self.n_eval += 1
+        nll = -D + sw * np.log(S)
+        F1 = -Dk + (sw / S) * Sk
+        Fcc = -Dcc - (sw / S ** 2) * np.outer(Sk, Sk)
+        Fcb = -Dcb + sw * (Scb / S - np.outer(Sk, np.conj(Sk)) / S ** 2)
+        g = amp.grad_from_adjoint(F1)
+        H = amp.hess_from_adjoint(F1, Fcc, Fcb)
+        return nll, g, 0.5 * (H + H.T)
+
     def nll(self, data, mcdata, weight=1.0, batch=None, bg=None, mc_weight=1.0):
         wdata, _ = self.get_weight_data(data, None if np.isscalar(weight) and weight == 1.0
                                         else weight, bg=bg)
@@ -190,8 +231,15 @@ class Model:
 
     def nll_grad_hessian(self, data, mcdata, weight=1.0, batch=24000, bg=None, mc_weight=1.0,
                          eps=1e-5):
-        """NLL, gradient and Hessian; the Hessian is the central difference of the analytic
+        """NLL, gradient and Hessian (model.py:513-560). Exact for coupling parameters; with
+        floating masses or widths the Hessian is the central difference of the analytic
         gradient with a relative step ``eps`` per trainable parameter."""
+        if not self.Amp.shape_mode:
+            wdata, _ = self.get_weight_data(data, None if np.isscalar(weight) and weight == 1.0
+                                            else weight, bg=bg)
+            mc_sets, mc_w = self._mc_weight(mcdata,
+                                            None if np.isscalar(mc_weight) else mc_weight)
+            return self._nll_grad_hess(wdata, mc_sets, mc_w)
         nll, g = self.nll_grad(data, mcdata, weight=weight, bg=bg, mc_weight=mc_weight)
         x0 = np.array(self.vm.get_all_val())
         n = len(x0)
@@ -289,7 +337,19 @@ class FCN:
         return self.nll_grad(x)[1]
 
     def nll_grad_hessian(self, x=None, batch=None, eps=1e-5):
+        """Exact Hessian for couplings (second-moment passes); central differences of the
+        gradient when masses or widths float (FCN.nll_grad_hessian, model.py:1258-1265)."""
         self._set(x)
+        if not self.model.Amp.shape_mode:
+            nll, g, H = self.model._nll_grad_hess(self.wdata, self.mc_sets, self.mc_weight,
+                                                  split=self._split)
+            self.n_call += 1
+            cv, cg = self._constr()
+            for name, (mu, sigma) in self.gauss_constr.items():
+                if name in self.vm.trainable_vars:
+                    i = self.vm.trainable_vars.index(name)
+                    H[i, i] += 1.0 / sigma ** 2
+            return nll + cv, g + cg, H
         nll, g = self.get_nll_grad()
         x0 = np.array(self.vm.get_all_val())
         n = len(x0)
