@@ -209,9 +209,12 @@ typedef struct {
  * (host [P][P] complex, from tpa_gram on MC with unit-sum weights). Each fit minimises with
  * BFGS on a host thread (strong-Wolfe line search, stop at max |g| <= gtol; maxiter <= 0 means
  * 200 * n_var), like fit_scipy (tf_pwa/fit.py:189-300). The batcher launches when every fit is
- * waiting, when max_batch requests are pending, or window_us after the first one arrived. */
+ * waiting, when max_batch requests are pending, or window_us after the first one arrived.
+ * hess_start != 0: each fit first takes one exact Hessian (tpa_pool_hess) at its start point and
+ * starts BFGS from its inverse (shifted to positive definite) with a unit first step, instead
+ * of from the identity. */
 int tpa_server_create(tpa_pool_t pool, const TpaCouplingMap* map, const double* gram,
-                      double gtol, int maxiter, int max_batch, double window_us,
+                      double gtol, int maxiter, int max_batch, double window_us, int hess_start,
                       tpa_server_t* out);
 
 /* Stops the batcher, cancels running fits (status 3) and joins every thread. */
@@ -221,6 +224,11 @@ int tpa_server_destroy(tpa_server_t server);
  * sum sw, through the batcher (FCN.nll_grad, tf_pwa/model/model.py:1216-1240). */
 int tpa_server_nll_grad(tpa_server_t server, int64_t offset, int64_t n, double sw,
                         const double* y, double* nll, double* grad);
+
+/* NLL, gradient and exact Hessian (n_var x n_var, row-major) in y from one second-moment pass
+ * (tpa_pool_hess) and the Gram matrix (FCN.nll_grad_hessian, tf_pwa/model/model.py:1258-1265). */
+int tpa_server_nll_grad_hess(tpa_server_t server, int64_t offset, int64_t n, double sw,
+                             const double* y, double* nll, double* grad, double* hess);
 
 /* Start a fit on the data range [offset, offset+n) from parameters y0; returns its id. */
 int tpa_server_submit(tpa_server_t server, int64_t offset, int64_t n, double sw,

@@ -2,6 +2,7 @@
 // (tpa_serve.cpp) without a GPU: the same tpa_pool_eval contract (per request: the weighted
 // clip_log data sum and its adjoint G_k) computed in plain loops from a host partial-wave cache
 // h [P][capacity][lam_pad] that the test supplies.
+#include <algorithm>
 #include <cmath>
 #include <cstdint>
 #include <cstring>
@@ -110,6 +111,71 @@ int tpa_pool_eval(tpa_pool_t p, int n_items, const int64_t* ranges, const double
     p->evals += 1;
   }
   if (ms) *ms = 0.0;
+  return 0;
+}
+
+// Second moments of one range in the packed layout of tpa_pool_hess (data mode).
+int tpa_pool_hess(tpa_pool_t p, int64_t off, int64_t n, const double* c, double* out) {
+  const int P = p->P, lg = p->lg, Q = 2 * P;
+  const int T = Q * (Q + 1) / 2;
+  if (off < 0 || n < 1 || off + n > p->used) return tpa_set_error(-1, "range outside the pool");
+  const double eps = 1e-6, logeps = std::log(1e-6);
+  std::fill(out, out + 2 + 2 * P + 2 * T, 0.0);
+  double* O1 = out + 2 + 2 * P;
+  double* O2 = O1 + T;
+  std::vector<double> ar(lg), ai(lg), z(Q), y((size_t)lg * Q);
+  for (int64_t e = off; e < off + n; ++e) {
+    double f = 0.0;
+    for (int l = 0; l < lg; ++l) {
+      double sr = 0.0, si = 0.0;
+      for (int k = 0; k < P; ++k) {
+        const double* hv = &p->h[2 * (((size_t)k * p->cap + e) * lg + l)];
+        sr += c[2 * k] * hv[0] - c[2 * k + 1] * hv[1];
+        si += c[2 * k] * hv[1] + c[2 * k + 1] * hv[0];
+        y[(size_t)l * Q + k] = hv[0];
+        y[(size_t)l * Q + P + k] = hv[1];
+      }
+      ar[l] = sr;
+      ai[l] = si;
+      f += sr * sr + si * si;
+    }
+    const double w = p->w[e];
+    const double dom = (w == 0.0) ? 1.0 : w;
+    const double r = w / dom, x = f * r;
+    double cl, d1, d2;
+    if (x > eps) {
+      cl = std::log(x);
+      d1 = 1.0 / x;
+      d2 = -1.0 / (x * x);
+    } else {
+      const double t = (x - eps) / eps;
+      cl = (logeps + t) - (t * t) / 2.0;
+      d1 = 1.0 / eps - t / eps;
+      d2 = -1.0 / (eps * eps);
+    }
+    const double u = (w * d1) * r, v = (w * d2) * (r * r);
+    out[0] += w * cl;
+    for (int k = 0; k < P; ++k) {
+      double qr = 0.0, qi = 0.0;
+      for (int l = 0; l < lg; ++l) {
+        const double hr = y[(size_t)l * Q + k], hi = y[(size_t)l * Q + P + k];
+        qr += ar[l] * hr + ai[l] * hi;
+        qi += ar[l] * hi - ai[l] * hr;
+      }
+      z[k] = qr;
+      z[P + k] = qi;
+      out[2 + 2 * k] += u * qr;
+      out[3 + 2 * k] += u * qi;
+    }
+    int idx = 0;
+    for (int i = 0; i < Q; ++i)
+      for (int j = i; j < Q; ++j, ++idx) {
+        O1[idx] += v * z[i] * z[j];
+        double s2 = 0.0;
+        for (int l = 0; l < lg; ++l) s2 += y[(size_t)l * Q + i] * y[(size_t)l * Q + j];
+        O2[idx] += u * s2;
+      }
+  }
   return 0;
 }
 

@@ -35,9 +35,11 @@ def mocklib(tmp_path_factory):
         "mock_pool_destroy": [vp],
         "mock_pool_add": [vp, dp, i64, dp, ctypes.POINTER(i64)],
         "tpa_server_create": [vp, ctypes.POINTER(_lib.TpaCouplingMap), dp, ctypes.c_double,
-                              ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.POINTER(vp)],
+                              ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_int,
+                              ctypes.POINTER(vp)],
         "tpa_server_destroy": [vp],
         "tpa_server_nll_grad": [vp, i64, i64, ctypes.c_double, dp, dp, dp],
+        "tpa_server_nll_grad_hess": [vp, i64, i64, ctypes.c_double, dp, dp, dp, dp],
         "tpa_server_submit": [vp, i64, i64, ctypes.c_double, dp, ip],
         "tpa_server_poll": [vp, ip, ctypes.c_int, ip],
         "tpa_server_result": [vp, ctypes.c_int, dp, dp, ip, ctypes.POINTER(i64),
@@ -111,11 +113,12 @@ class Problem:
     def server(self, **kw):
         g = np.ascontiguousarray(self.gram).view(np.float64).ravel()
         s = ctypes.c_void_p()
-        opts = dict(gtol=1e-5, maxiter=0, max_batch=64, window_us=200.0)
+        opts = dict(gtol=1e-5, maxiter=0, max_batch=64, window_us=200.0, hess_start=0)
         opts.update(kw)
         assert self.L.tpa_server_create(self.pool, ctypes.byref(self.map), _dp(g), opts["gtol"],
                                         opts["maxiter"], opts["max_batch"], opts["window_us"],
-                                        ctypes.byref(s)) == 0, self.L.tpa_last_error()
+                                        opts["hess_start"], ctypes.byref(s)) == 0, \
+            self.L.tpa_last_error()
         return s
 
     def coefs(self, y):
@@ -185,11 +188,39 @@ def test_server_nll_and_gradient(mocklib):
     mocklib.tpa_server_destroy(s)
 
 
-def test_server_fits_converge_and_ignore_the_mix(mocklib):
+def test_server_exact_hessian(mocklib):
+    """The server's Hessian (second moments + chain rule in C++) against central differences of
+    its own gradient, and its NLL/gradient against the batched path."""
+    pb = Problem(mocklib, [1200])
+    s = pb.server()
+    slot = pb.slots[0]
+    rng = np.random.default_rng(5)
+    y = np.ascontiguousarray(pb.x_true + rng.normal(scale=0.2, size=pb.nv))
+    g = np.zeros(pb.nv)
+    H = np.zeros((pb.nv, pb.nv))
+    f = ctypes.c_double()
+    assert mocklib.tpa_server_nll_grad_hess(s, slot[0], slot[1], float(slot[1]), _dp(y),
+                                            ctypes.byref(f), _dp(g), _dp(H)) == 0
+    f2, g2 = _nll_grad(mocklib, s, slot, y)
+    assert abs(f.value - f2) <= 1e-12 * abs(f2)
+    np.testing.assert_allclose(g, g2, rtol=1e-10, atol=1e-10 * np.abs(g2).max())
+    h = 1e-6
+    for j in range(pb.nv):
+        e = np.zeros(pb.nv)
+        e[j] = h
+        col = (_nll_grad(mocklib, s, slot, y + e)[1] - _nll_grad(mocklib, s, slot, y - e)[1]) / (
+            2 * h)
+        np.testing.assert_allclose(H[:, j], col, rtol=0, atol=1e-5 * np.abs(H).max())
+    assert np.array_equal(H, H.T)
+    mocklib.tpa_server_destroy(s)
+
+
+@pytest.mark.parametrize("hess_start", [0, 1])
+def test_server_fits_converge_and_ignore_the_mix(mocklib, hess_start):
     pb = Problem(mocklib, [2000, 1200, 1700, 800])
     rng = np.random.default_rng(2)
     starts = [pb.x_true + rng.normal(scale=0.1, size=pb.nv) for _ in pb.slots]
-    s = pb.server()
+    s = pb.server(hess_start=hess_start)
     fid = ctypes.c_int()
     y0 = np.ascontiguousarray(starts[0])
     slot = pb.slots[0]
@@ -240,5 +271,5 @@ def test_server_rejects_bad_maps(mocklib):
     g = np.ascontiguousarray(pb.gram).view(np.float64).ravel()
     s = ctypes.c_void_p()
     assert mocklib.tpa_server_create(pb.pool, ctypes.byref(pb.map), _dp(g), 1e-5, 0, 8, 100.0,
-                                     ctypes.byref(s)) != 0
+                                     0, ctypes.byref(s)) != 0
     assert b"out of range" in mocklib.tpa_last_error()

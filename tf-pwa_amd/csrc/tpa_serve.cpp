@@ -70,6 +70,7 @@ struct tpa_server_s {
   int maxiter = 0;
   int max_batch = 1;
   double window_us = 200.0;
+  int hess_start = 0;  // BFGS starts from the exact inverse Hessian at the first point
 
   std::mutex mu;
   std::condition_variable cv;  // batcher wake-up: new request, a client leaving, stop
@@ -140,6 +141,64 @@ double to_x(const tpa_server_s* S, int j, double y) {
   }
 }
 
+double d2y_dx2(const tpa_server_s* S, int j, double t) {
+  const double a = S->bound_ab[2 * j], b = S->bound_ab[2 * j + 1];
+  switch (S->bound_kind[j]) {
+    case 1: return -(b - a) * std::sin(t) / 2.0;
+    case 2: return 1.0 / std::pow(t * t + 1.0, 1.5);
+    case 3: return -1.0 / std::pow(t * t + 1.0, 1.5);
+    default: return 0.0;
+  }
+}
+
+// Inverse of A + tau I with the smallest tau >= 0 (doubling from 1e-3 max|A_ii|) for which the
+// Cholesky factorisation succeeds (Nocedal & Wright, Algorithm 3.3). A is n x n, row-major.
+void shifted_cholesky_inverse(const std::vector<double>& A, int n, std::vector<double>& inv) {
+  double maxd = 0.0, mind = INFINITY;
+  for (int i = 0; i < n; ++i) {
+    maxd = std::max(maxd, std::fabs(A[(size_t)i * n + i]));
+    mind = std::min(mind, A[(size_t)i * n + i]);
+  }
+  const double beta = 1e-3 * std::max(maxd, 1e-300);
+  double tau = mind > 0.0 ? 0.0 : -mind + beta;
+  std::vector<double> L((size_t)n * n);
+  for (int it = 0; it < 200; ++it) {
+    bool ok = true;
+    std::fill(L.begin(), L.end(), 0.0);
+    for (int j = 0; j < n && ok; ++j) {
+      double d = A[(size_t)j * n + j] + tau;
+      for (int k = 0; k < j; ++k) d -= L[(size_t)j * n + k] * L[(size_t)j * n + k];
+      if (!(d > 0.0) || !std::isfinite(d)) {
+        ok = false;
+        break;
+      }
+      const double ljj = std::sqrt(d);
+      L[(size_t)j * n + j] = ljj;
+      for (int i = j + 1; i < n; ++i) {
+        double s = A[(size_t)i * n + j];
+        for (int k = 0; k < j; ++k) s -= L[(size_t)i * n + k] * L[(size_t)j * n + k];
+        L[(size_t)i * n + j] = s / ljj;
+      }
+    }
+    if (ok) break;
+    tau = std::max(2.0 * tau, beta);
+  }
+  inv.assign((size_t)n * n, 0.0);
+  std::vector<double> z(n);
+  for (int col = 0; col < n; ++col) {  // solve L L^T x = e_col
+    for (int i = 0; i < n; ++i) {
+      double s = (i == col) ? 1.0 : 0.0;
+      for (int k = 0; k < i; ++k) s -= L[(size_t)i * n + k] * z[k];
+      z[i] = s / L[(size_t)i * n + i];
+    }
+    for (int i = n - 1; i >= 0; --i) {
+      double s = z[i];
+      for (int k = i + 1; k < n; ++k) s -= L[(size_t)k * n + i] * inv[(size_t)k * n + col];
+      inv[(size_t)i * n + col] = s / L[(size_t)i * n + i];
+    }
+  }
+}
+
 // One request through the batcher: coefficients in, data sum and adjoint out.
 void request(tpa_server_s* S, Fit* F, const std::vector<cplx>& c) {
   std::unique_lock<std::mutex> lk(S->mu);
@@ -159,48 +218,184 @@ void request(tpa_server_s* S, Fit* F, const std::vector<cplx>& c) {
   if (F->failed) throw EngineFailure();
 }
 
-// NLL and its gradient in y: NLL = -sum w clip_log f + sw log(c^dagger I c) (the Model.nll of
-// tf_pwa/model/model.py:407-449 with the MC sum replaced by its Gram form).
-double nll_grad_y(tpa_server_s* S, Fit* F, const double* y, double* gy) {
-  const int P = S->P, nf = S->nf;
-  std::vector<cplx> v(nf), dr(nf), di(nf), c(P);
+// Gradient in y of a function whose Wirtinger derivative in the coefficients is F1
+// (dF = 2 Re sum_k F1_k dc_k): the chain rule through the three factors of each c_k.
+void grad_from_F1(const tpa_server_s* S, const cplx* v, const cplx* dr, const cplx* di,
+                  const cplx* F1, double* gy) {
+  std::fill(gy, gy + S->nv, 0.0);
+  for (int k = 0; k < S->P; ++k) {
+    const cplx comb = S->wave_const[k] * F1[k];
+    const int* w = &S->wave_factor[3 * k];
+    for (int a = 0; a < 3; ++a) {
+      cplx rest = comb;
+      for (int b = 0; b < 3; ++b)
+        if (b != a) rest *= v[w[b]];
+      const int f = w[a];
+      const int vr = S->factor_var[2 * f], vi = S->factor_var[2 * f + 1];
+      if (vr >= 0) gy[vr] += 2.0 * (dr[f] * rest).real();
+      if (vi >= 0) gy[vi] += 2.0 * (di[f] * rest).real();
+    }
+  }
+}
+
+void coefficients(const tpa_server_s* S, const double* y, std::vector<cplx>& v,
+                  std::vector<cplx>& dr, std::vector<cplx>& di, std::vector<cplx>& c) {
+  v.resize(S->nf);
+  dr.resize(S->nf);
+  di.resize(S->nf);
+  c.resize(S->P);
   factor_values(S, y, v.data(), dr.data(), di.data());
-  for (int k = 0; k < P; ++k) {
+  for (int k = 0; k < S->P; ++k) {
     const int* w = &S->wave_factor[3 * k];
     c[k] = S->wave_const[k] * v[w[0]] * v[w[1]] * v[w[2]];
   }
-  request(S, F, c);
-  const double sd = F->out[0];
-  std::vector<cplx> Ic(P);
+}
+
+// MC normalisation through the Gram matrix: S = c^dagger I c and dS/dc_k = conj((I c)_k).
+double mc_term(const tpa_server_s* S, const std::vector<cplx>& c, std::vector<cplx>& Sk) {
+  const int P = S->P;
+  Sk.resize(P);
   double smc = 0.0;
   for (int k = 0; k < P; ++k) {
     cplx acc = 0.0;
     const cplx* row = &S->gram[(size_t)k * P];
     for (int l = 0; l < P; ++l) acc += row[l] * c[l];
-    Ic[k] = acc;
+    Sk[k] = std::conj(acc);
     smc += (std::conj(c[k]) * acc).real();
   }
+  return smc;
+}
+
+// NLL and its gradient in y: NLL = -sum w clip_log f + sw log(c^dagger I c) (the Model.nll of
+// tf_pwa/model/model.py:407-449 with the MC sum replaced by its Gram form).
+double nll_grad_y(tpa_server_s* S, Fit* F, const double* y, double* gy) {
+  const int P = S->P;
+  std::vector<cplx> v, dr, di, c, Sk;
+  coefficients(S, y, v, dr, di, c);
+  request(S, F, c);
+  const double sd = F->out[0];
+  const double smc = mc_term(S, c, Sk);
   const double nll = -sd + F->sw * std::log(smc);
   if (gy) {
-    std::fill(gy, gy + S->nv, 0.0);
+    std::vector<cplx> F1(P);
     const double r = F->sw / smc;
-    for (int k = 0; k < P; ++k) {
-      const cplx gd(F->out[2 + 2 * k], F->out[3 + 2 * k]);
-      const cplx comb = S->wave_const[k] * (-gd + r * std::conj(Ic[k]));
-      const int* w = &S->wave_factor[3 * k];
-      for (int a = 0; a < 3; ++a) {
-        cplx rest = comb;
-        for (int b = 0; b < 3; ++b)
-          if (b != a) rest *= v[w[b]];
-        const int f = w[a];
-        const int vr = S->factor_var[2 * f], vi = S->factor_var[2 * f + 1];
-        if (vr >= 0) gy[vr] += 2.0 * (dr[f] * rest).real();
-        if (vi >= 0) gy[vi] += 2.0 * (di[f] * rest).real();
-      }
-    }
+    for (int k = 0; k < P; ++k) F1[k] = -cplx(F->out[2 + 2 * k], F->out[3 + 2 * k]) + r * Sk[k];
+    grad_from_F1(S, v.data(), dr.data(), di.data(), F1.data(), gy);
   }
   F->nfev += 1;
   return nll;
+}
+
+// NLL, gradient and exact Hessian in y from one second-moment pass over the fit's data
+// (tpa_pool_hess) and the Gram matrix: with F1, Fcc = d2/dc dc, Fcb = d2/dc dconj(c),
+// H = 2 Re[J Fcc J^T + J Fcb J^H + sum_k F1_k d2c_k/dy dy] (BaseModel.nll_grad_hessian,
+// tf_pwa/model/model.py:513-560, for the couplings).
+double nll_grad_hess_y(tpa_server_s* S, Fit* F, const double* y, double* gy, double* H) {
+  const int P = S->P, Q = 2 * P, nv = S->nv;
+  const int T = Q * (Q + 1) / 2;
+  std::vector<cplx> v, dr, di, c, Sk;
+  coefficients(S, y, v, dr, di, c);
+  std::vector<double> coef(2 * P), out(2 + 2 * P + 2 * T);
+  for (int k = 0; k < P; ++k) {
+    coef[2 * k] = c[k].real();
+    coef[2 * k + 1] = c[k].imag();
+  }
+  if (tpa_pool_hess(S->pool, F->off, F->n, coef.data(), out.data()) != TPA_OK) {
+    std::lock_guard<std::mutex> lk(S->mu);
+    S->batch_error = tpa_last_error();
+    throw EngineFailure();
+  }
+  const double* O1 = out.data() + 2 + 2 * P;
+  const double* O2 = O1 + T;
+  auto O = [Q](const double* m, int i, int j) {
+    if (i > j) std::swap(i, j);
+    return m[i * Q - (i * (i - 1)) / 2 + (j - i)];
+  };
+  const double smc = mc_term(S, c, Sk);
+  const double sw = F->sw;
+  std::vector<cplx> F1(P), Fcc((size_t)P * P), Fcb((size_t)P * P);
+  for (int k = 0; k < P; ++k) F1[k] = -cplx(out[2 + 2 * k], out[3 + 2 * k]) + (sw / smc) * Sk[k];
+  for (int k = 0; k < P; ++k)
+    for (int l = 0; l < P; ++l) {
+      const cplx dcc(O(O1, k, l) - O(O1, P + k, P + l), O(O1, k, P + l) + O(O1, P + k, l));
+      const cplx dcb(O(O1, k, l) + O(O1, P + k, P + l) + O(O2, k, l) + O(O2, P + k, P + l),
+                     O(O1, P + k, l) - O(O1, k, P + l) + O(O2, P + k, l) - O(O2, k, P + l));
+      const cplx scb = S->gram[(size_t)l * P + k];  // d2S/dc_k dconj(c_l) = I_lk
+      Fcc[(size_t)k * P + l] = -dcc - (sw / (smc * smc)) * Sk[k] * Sk[l];
+      Fcb[(size_t)k * P + l] = -dcb + sw * (scb / smc - Sk[k] * std::conj(Sk[l]) / (smc * smc));
+    }
+  grad_from_F1(S, v.data(), dr.data(), di.data(), F1.data(), gy);
+  // J[a][k] = d c_k / d y_a
+  std::vector<cplx> J((size_t)nv * P, 0.0);
+  for (int k = 0; k < P; ++k) {
+    const int* w = &S->wave_factor[3 * k];
+    for (int a = 0; a < 3; ++a) {
+      cplx rest = S->wave_const[k];
+      for (int b = 0; b < 3; ++b)
+        if (b != a) rest *= v[w[b]];
+      const int f = w[a];
+      const int vr = S->factor_var[2 * f], vi = S->factor_var[2 * f + 1];
+      if (vr >= 0) J[(size_t)vr * P + k] += dr[f] * rest;
+      if (vi >= 0) J[(size_t)vi * P + k] += di[f] * rest;
+    }
+  }
+  std::vector<cplx> X((size_t)nv * P), Y((size_t)nv * P);
+  for (int a = 0; a < nv; ++a)
+    for (int l = 0; l < P; ++l) {
+      cplx xa = 0.0, ya = 0.0;
+      for (int k = 0; k < P; ++k) {
+        xa += J[(size_t)a * P + k] * Fcc[(size_t)k * P + l];
+        ya += J[(size_t)a * P + k] * Fcb[(size_t)k * P + l];
+      }
+      X[(size_t)a * P + l] = xa;
+      Y[(size_t)a * P + l] = ya;
+    }
+  std::vector<cplx> M((size_t)nv * nv, 0.0);
+  for (int a = 0; a < nv; ++a)
+    for (int b = 0; b < nv; ++b) {
+      cplx acc = 0.0;
+      for (int l = 0; l < P; ++l)
+        acc += X[(size_t)a * P + l] * J[(size_t)b * P + l] +
+               Y[(size_t)a * P + l] * std::conj(J[(size_t)b * P + l]);
+      M[(size_t)a * nv + b] = acc;
+    }
+  // sum_k F1_k d2c_k/dy dy: one polar factor twice, or two different factors once each
+  for (int k = 0; k < P; ++k) {
+    const cplx CF = S->wave_const[k] * F1[k];
+    const int* w = &S->wave_factor[3 * k];
+    for (int a = 0; a < 3; ++a) {
+      const int f = w[a];
+      const int fr = S->factor_var[2 * f], fi = S->factor_var[2 * f + 1];
+      cplx rest = CF;
+      for (int b = 0; b < 3; ++b)
+        if (b != a) rest *= v[w[b]];
+      if (S->factor_polar[f]) {
+        const cplx e = dr[f];  // e^{i phi}
+        if (fr >= 0 && fi >= 0) {
+          M[(size_t)fr * nv + fi] += cplx(0.0, 1.0) * e * rest;
+          M[(size_t)fi * nv + fr] += cplx(0.0, 1.0) * e * rest;
+        }
+        if (fi >= 0) M[(size_t)fi * nv + fi] += -v[f] * rest;
+      }
+      for (int b = 0; b < 3; ++b) {
+        if (b == a) continue;
+        const int g = w[b];
+        const int third = 3 - a - b;
+        const cplx rest2 = CF * v[w[third]];
+        const int va[2] = {S->factor_var[2 * f], S->factor_var[2 * f + 1]};
+        const int vb[2] = {S->factor_var[2 * g], S->factor_var[2 * g + 1]};
+        const cplx da[2] = {dr[f], di[f]}, db[2] = {dr[g], di[g]};
+        for (int p = 0; p < 2; ++p)
+          for (int q = 0; q < 2; ++q)
+            if (va[p] >= 0 && vb[q] >= 0) M[(size_t)va[p] * nv + vb[q]] += da[p] * db[q] * rest2;
+      }
+    }
+  }
+  for (int a = 0; a < nv; ++a)
+    for (int b = 0; b < nv; ++b)
+      H[(size_t)a * nv + b] = (M[(size_t)a * nv + b] + M[(size_t)b * nv + a]).real();
+  F->nfev += 1;
+  return -out[0] + sw * std::log(smc);
 }
 
 // ------------------------------------------------------------------ BFGS
@@ -324,7 +519,24 @@ void run_bfgs(tpa_server_s* S, Fit* F) {
   for (int j = 0; j < n; ++j) x[j] = to_x(S, j, F->x0[j]);
   std::vector<double> H((size_t)n * n, 0.0);
   for (int j = 0; j < n; ++j) H[(size_t)j * n + j] = 1.0;
-  double f = obj(x, g);
+  double f;
+  bool newton_start = false;
+  if (S->hess_start) {
+    // inverse of the exact Hessian in x: dy/dx Hy dy/dx + diag(g_y d2y/dx2)
+    std::vector<double> y(n), dydx(n), gy(n), Hyy((size_t)n * n), Hx((size_t)n * n);
+    to_y(S, x.data(), y.data(), dydx.data());
+    f = nll_grad_hess_y(S, F, y.data(), gy.data(), Hyy.data());
+    g.resize(n);
+    for (int a = 0; a < n; ++a) {
+      g[a] = gy[a] * dydx[a];
+      for (int b = 0; b < n; ++b) Hx[(size_t)a * n + b] = dydx[a] * Hyy[(size_t)a * n + b] * dydx[b];
+      Hx[(size_t)a * n + a] += gy[a] * d2y_dx2(S, a, x[a]);
+    }
+    shifted_cholesky_inverse(Hx, n, H);
+    newton_start = true;
+  } else {
+    f = obj(x, g);
+  }
   double f_old_old = f + std::sqrt(dot(g, g)) / 2.0;
   int status = 1;
   int64_t k = 0;
@@ -346,7 +558,7 @@ void run_bfgs(tpa_server_s* S, Fit* F) {
     // first trial step from the previous decrease (Nocedal & Wright eq. 3.60)
     double a1 = 1.0;
     const double guess = 1.01 * 2.0 * (f - f_old_old) / d0;
-    if (std::isfinite(guess) && guess > 0.0) a1 = std::min(1.0, guess);
+    if (!(newton_start && k == 0) && std::isfinite(guess) && guess > 0.0) a1 = std::min(1.0, guess);
     Trial t;
     if (!line_search(obj, x, p, f, d0, a1, t)) {
       status = 2;
@@ -463,7 +675,7 @@ void batcher_main(tpa_server_s* S) {
 extern "C" {
 
 int tpa_server_create(tpa_pool_t pool, const TpaCouplingMap* map, const double* gram,
-                      double gtol, int maxiter, int max_batch, double window_us,
+                      double gtol, int maxiter, int max_batch, double window_us, int hess_start,
                       tpa_server_t* out) {
   if (!pool || !map || !gram || !out) return tpa_set_error(TPA_ERR_ARG, "null pointer");
   if (map->P < 1 || map->n_factor < 1 || map->n_var < 1 || max_batch < 1 || !(gtol > 0.0) ||
@@ -500,6 +712,7 @@ int tpa_server_create(tpa_pool_t pool, const TpaCouplingMap* map, const double* 
   S->maxiter = maxiter;
   S->max_batch = max_batch;
   S->window_us = window_us;
+  S->hess_start = hess_start ? 1 : 0;
   S->batcher = std::thread(batcher_main, S);
   *out = S;
   return TPA_OK;
@@ -546,6 +759,22 @@ int tpa_server_nll_grad(tpa_server_t S, int64_t offset, int64_t n, double sw, co
   S->clients -= 1;
   S->cv.notify_all();
   return rc;
+}
+
+int tpa_server_nll_grad_hess(tpa_server_t S, int64_t offset, int64_t n, double sw,
+                             const double* y, double* nll, double* grad, double* hess) {
+  if (!S || !y || !nll || !grad || !hess) return tpa_set_error(TPA_ERR_ARG, "null pointer");
+  Fit F;
+  F.off = offset;
+  F.n = n;
+  F.sw = sw;
+  try {
+    *nll = nll_grad_hess_y(S, &F, y, grad, hess);
+  } catch (const EngineFailure&) {
+    std::lock_guard<std::mutex> lk(S->mu);
+    return tpa_set_error(TPA_ERR_HIP, "second-moment pass failed: " + S->batch_error);
+  }
+  return TPA_OK;
 }
 
 int tpa_server_submit(tpa_server_t S, int64_t offset, int64_t n, double sw, const double* y0,

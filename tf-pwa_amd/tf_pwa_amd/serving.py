@@ -191,10 +191,12 @@ class FitServer:
     tf_pwa/model/model.py:1126-1164); its Gram matrix is the MC normalisation of every fit.
     ``hbm_budget``: device bytes for the MC set plus the pool; the pool gets what the MC set
     leaves (or ``capacity`` events when given). Fits whose data do not fit wait for others to end.
+    ``hess_start``: each fit's BFGS starts from the inverse exact Hessian at its start point
+    (one second-moment pass) rather than from the identity.
     """
 
     def __init__(self, amp, mc, hbm_budget=200e9, capacity=None, max_batch=512,
-                 window_us=200.0, gtol=1e-5, maxiter=0):
+                 window_us=200.0, gtol=1e-5, maxiter=0, hess_start=False):
         self.amp = amp
         self.vm = amp.vm
         self.mc = mc
@@ -211,7 +213,8 @@ class FitServer:
         h = ctypes.c_void_p()
         _lib.check(_lib.lib().tpa_server_create(
             self.pool._h, ctypes.byref(self._map), _lib.dptr(self.gram.view(np.float64).ravel()),
-            float(gtol), int(maxiter), int(max_batch), float(window_us), ctypes.byref(h)))
+            float(gtol), int(maxiter), int(max_batch), float(window_us), int(bool(hess_start)),
+            ctypes.byref(h)))
         self._h = h
         self.slots = {}
 
@@ -231,6 +234,17 @@ class FitServer:
                                                   _lib.dptr(y), ctypes.byref(nll),
                                                   _lib.dptr(g)))
         return nll.value, g
+
+    def nll_grad_hessian(self, slot, y):
+        """NLL, gradient and exact Hessian at ``y`` on ``slot`` (one second-moment pass)."""
+        y = np.ascontiguousarray(y, dtype=np.float64)
+        g = np.zeros(self.n_var)
+        H = np.zeros((self.n_var, self.n_var))
+        nll = ctypes.c_double()
+        _lib.check(_lib.lib().tpa_server_nll_grad_hess(self._h, slot.offset, slot.n, slot.sw,
+                                                       _lib.dptr(y), ctypes.byref(nll),
+                                                       _lib.dptr(g), _lib.dptr(H)))
+        return nll.value, g, H
 
     # -------------------------------------------------------------- fits
     def submit(self, slot, y0):
