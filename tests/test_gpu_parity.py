@@ -106,6 +106,31 @@ def test_exact_hessian_matches_oracle(which, request, gpu_available):
     assert np.abs(Hm - H).max() <= 1e-4 * scale
 
 
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_native_session_matches_python_assembly(case2, precision, gpu_available):
+    """FCN's one-call native session (tpa_fcn_*) against the Python host assembly of the same
+    engine passes (Model._nll_grad), at several parameter points and after a weight change."""
+    from tf_pwa_amd import FCN
+
+    conf = case2["conf"]
+    data = conf.data_from_p4(case2["data"], weight=case2["wd"])
+    mc = conf.data_from_p4(case2["mc"])
+    nat = FCN(conf.get_model(), data, mc, precision=precision)
+    ref = FCN(conf.get_model(), data, mc, precision=precision)
+    ref.native = False
+    x0 = np.array(conf.vm.get_all_val())
+    rng = np.random.default_rng(12)
+    for _ in range(3):
+        x = x0 + rng.normal(scale=0.2, size=x0.size)
+        n1, g1 = nat.nll_grad(x)
+        n2, g2 = ref.nll_grad(x)
+        assert abs(n1 - n2) <= 1e-12 * abs(n2)
+        assert np.abs(g1 - g2).max() <= 1e-10 * np.abs(g2).max()
+        assert abs(nat(x) - n1) <= 1e-12 * abs(n1)
+    assert nat._native is not None and ref._native is None
+    conf.vm.set_all(x0)
+
+
 def test_gram_matches_partial_waves(case2, gpu_available):
     """FP32 MFMA Gram against the FP64 Gram of the same partial waves (read back)."""
     conf = case2["conf"]

@@ -1,5 +1,5 @@
 """Build libtfpwa_amd.so in-tree for gfx950 (hipcc: the device code and C ABI in tpa_capi.hip,
-the host serving runtime in tpa_serve.cpp)."""
+the host serving runtime in tpa_serve.cpp, the single-fit session in tpa_fcn.cpp)."""
 
 import os
 import subprocess
@@ -7,7 +7,7 @@ import sys
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
-SRC = [os.path.join(HERE, "csrc", "tpa_capi.hip"), os.path.join(HERE, "csrc", "tpa_serve.cpp")]
+SRC = [os.path.join(HERE, "csrc", n) for n in ("tpa_capi.hip", "tpa_serve.cpp", "tpa_fcn.cpp")]
 OUT = os.path.join(HERE, "lib", "libtfpwa_amd.so")
 
 FLAGS = [

@@ -142,6 +142,11 @@ def lib():
                                ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_int,
                                ctypes.POINTER(vp)], ctypes.c_int),
         "tpa_server_destroy": ([vp], ctypes.c_int),
+        "tpa_fcn_create": ([ctypes.POINTER(TpaCouplingMap), ctypes.POINTER(vp), ctypes.c_int,
+                            ctypes.POINTER(vp), ctypes.c_int, dp, ctypes.c_double,
+                            ctypes.POINTER(vp)], ctypes.c_int),
+        "tpa_fcn_destroy": ([vp], ctypes.c_int),
+        "tpa_fcn_nll_grad": ([vp, dp, dp, dp], ctypes.c_int),
         "tpa_server_nll_grad": ([vp, i64, i64, ctypes.c_double, dp, dp, dp], ctypes.c_int),
         "tpa_server_nll_grad_hess": ([vp, i64, i64, ctypes.c_double, dp, dp, dp, dp],
                                      ctypes.c_int),
@@ -190,5 +195,6 @@ def exported_symbols():
         "tpa_gram", "tpa_hess", "tpa_pool_hess", "tpa_pool_create", "tpa_pool_destroy", "tpa_pool_add", "tpa_pool_release",
         "tpa_pool_free_events", "tpa_pool_eval", "tpa_server_create", "tpa_server_destroy",
         "tpa_server_nll_grad", "tpa_server_nll_grad_hess", "tpa_server_submit", "tpa_server_poll", "tpa_server_result",
-        "tpa_server_latencies", "tpa_server_counters",
+        "tpa_server_latencies", "tpa_server_counters", "tpa_fcn_create", "tpa_fcn_destroy",
+        "tpa_fcn_nll_grad",
     ]

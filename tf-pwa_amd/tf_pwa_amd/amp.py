@@ -275,14 +275,20 @@ class AmplitudeModel:
     def shape_mode(self):
         """True when a mass or width is trainable: the cache is then built without the
         lineshapes, which are evaluated per event with their derivatives."""
-        tv = set(self.vm.trainable_vars)
-        return any(nm["mass"] in tv or nm["width"] in tv for nm in self._names())
+        key = self.vm.struct_version
+        if getattr(self, "_sm_key", None) != key:
+            tv = set(self.vm.trainable_vars)
+            self._sm = any(nm["mass"] in tv or nm["width"] in tv for nm in self._names())
+            self._sm_key = key
+        return self._sm
 
     def build_state(self):
         """(cache state, description state), cached by the parameter store's version.
         The cache depends on the barrier clamp flags and, when lineshapes are folded, on the
         lineshape parameters; the device description always carries the latter."""
-        if getattr(self, "_bs_version", None) == self.vm.version:
+        # with fixed lineshapes only fixed values matter; trainable lineshapes follow every change
+        key = ("v", self.vm.version) if self.shape_mode else ("s", self.vm.struct_version)
+        if getattr(self, "_bs_version", None) == key:
             return self._bs
         bc = self.barrier_constants()
         flags = tuple((tuple(b["act1"]), tuple(b["act2"])) for b in bc)
@@ -290,7 +296,7 @@ class AmplitudeModel:
         fold = not self.shape_mode
         cache = (fold, flags, shapes if fold else None)
         self._bs = (cache, (flags, shapes))
-        self._bs_version = self.vm.version
+        self._bs_version = key
         return self._bs
 
     def device(self):

@@ -263,6 +263,66 @@ class Model:
         return self.Amp.get_params(trainable_only)
 
 
+class _NativeSession:
+    """One tpa_fcn session (fixed lineshapes): the coupling map of the current parameter store,
+    the event sets with their weights in place and, in mixed precision, the summed MC Gram
+    matrix. Valid while the store's fixed values, the trainable set and the weights are
+    unchanged (the caller keys it on those)."""
+
+    def __init__(self, model, data_sets, data_w, mc_sets, mc_w, sw):
+        import ctypes
+
+        from .serving import coupling_map
+
+        amp = model.Amp
+        for s, w in zip(data_sets, data_w):
+            s.set_weight(w)
+            s.ensure_built()
+        gram = None
+        if model.precision == "mixed":
+            I = np.zeros((amp.P, amp.P), dtype=np.complex128)
+            for s, w in zip(mc_sets, mc_w):
+                s.set_weight(w)
+                I = I + s.gram()
+            self._gram = np.ascontiguousarray(I).view(np.float64).ravel()
+            gram = _lib.dptr(self._gram)
+        else:
+            for s, w in zip(mc_sets, mc_w):
+                s.set_weight(w)
+                s.ensure_built()
+        self._map, self._keep = coupling_map(amp)
+        vp = ctypes.c_void_p
+        self._data = (vp * len(data_sets))(*[s._h.value for s in data_sets])
+        self._mc = (vp * max(1, len(mc_sets)))(*[s._h.value for s in mc_sets])
+        self._sets = list(data_sets) + list(mc_sets)  # keep the handles alive
+        h = vp()
+        _lib.check(_lib.lib().tpa_fcn_create(ctypes.byref(self._map), self._data, len(data_sets),
+                                             self._mc, len(mc_sets), gram, float(sw),
+                                             ctypes.byref(h)))
+        self._h = h
+        self.n = len(self._keep["bound_kind"])
+        self._g = np.zeros(self.n)
+        self._nll = ctypes.c_double()
+
+    def nll_grad(self, y, grad=True):
+        import ctypes
+
+        y = np.ascontiguousarray(y, dtype=np.float64)
+        g = np.zeros(self.n) if grad else None
+        _lib.check(_lib.lib().tpa_fcn_nll_grad(self._h, _lib.dptr(y), ctypes.byref(self._nll),
+                                               _lib.dptr(g)))
+        return self._nll.value, g
+
+    def close(self):
+        h = getattr(self, "_h", None)
+        if h is not None and _lib._lib is not None:
+            _lib._lib.tpa_fcn_destroy(h)
+        self._h = None
+
+    def __del__(self):
+        self.close()
+
+
 class FCN:
     """The callable minimised by the fit drivers (tf_pwa/model/model.py:1115-1285):
     ``fcn(x)``, ``fcn.nll_grad(x)`` -> (float, ndarray in vm.trainable_vars order),
@@ -288,6 +348,11 @@ class FCN:
         self.last_time = 0.0
         self._split = (self.wdata.split(), Model._split(self.mc_sets, self.mc_weight),
                        float(self.weight.sum()))
+        # fixed lineshapes: one native call per evaluation (tpa_fcn_*); False keeps the Python
+        # host assembly (Model._nll_grad), the reference implementation of the same sums
+        self.native = True
+        self._native = None
+        self._native_key = None
 
     def _set(self, x):
         if x is None or (isinstance(x, dict) and not x):
@@ -307,10 +372,40 @@ class FCN:
     def get_params(self, trainable_only=False):
         return self.vm.get_all_dic(trainable_only)
 
+    def _native_session(self):
+        """The native session for fixed lineshapes (None when masses or widths float), rebuilt
+        when the precision, fixed values, trainable set or any set's weights change."""
+        amp = self.model.Amp
+        if amp.shape_mode or not self.native:
+            return None
+        dw, mw, sw = self._split
+        sets = list(self.wdata.sets) + list(self.mc_sets)
+        for st, w in zip(sets, list(dw) + list(mw)):
+            st.set_weight(w)
+        key = (self.model.precision, self.vm.struct_version,
+               tuple(st._weight_version for st in sets))
+        if self._native_key != key:
+            if self._native is not None:
+                self._native.close()
+            self._native = None
+            self._native = _NativeSession(self.model, self.wdata.sets, dw, self.mc_sets, mw, sw)
+            self._native_key = key
+        return self._native
+
+    def _y(self, x):
+        if x is not None and not isinstance(x, dict) and len(x) == len(self.vm.trainable_vars):
+            return np.asarray(x, dtype=np.float64)
+        return np.array(self.vm.get_all_val())
+
     def get_nll(self, x=None):
         self._set(x)
-        nll, _ = self.model._nll_grad(self.wdata, self.mc_sets, self.mc_weight, grad=False,
-                                      split=self._split)
+        nat = self._native_session()
+        if nat is not None:
+            nll, _ = nat.nll_grad(self._y(x), grad=False)
+            self.model.n_eval += 1
+        else:
+            nll, _ = self.model._nll_grad(self.wdata, self.mc_sets, self.mc_weight, grad=False,
+                                          split=self._split)
         self.n_call += 1
         return nll
 
@@ -320,8 +415,13 @@ class FCN:
 
     def get_nll_grad(self, x=None):
         self._set(x)
-        nll, g = self.model._nll_grad(self.wdata, self.mc_sets, self.mc_weight,
-                                      split=self._split)
+        nat = self._native_session()
+        if nat is not None:
+            nll, g = nat.nll_grad(self._y(x))
+            self.model.n_eval += 1
+        else:
+            nll, g = self.model._nll_grad(self.wdata, self.mc_sets, self.mc_weight,
+                                          split=self._split)
         self.n_call += 1
         return nll, g
 

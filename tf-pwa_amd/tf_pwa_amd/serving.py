@@ -76,8 +76,15 @@ class EventPool:
         nf = _lib.NF_TOPO * len(amp.topo) + 4
         return 16 * amp.P * amp.lam_pad + 8 * nf + 8
 
-    def add(self, p4, weight=None, order=None):
-        arr = np.ascontiguousarray(p4_array(p4, self.amp.finals, order))
+    def add(self, p4, weight=None, order=None, engine_layout=False):
+        """Upload a sample into a free range. ``p4`` as for EventSet (dict or array in
+        ``order``), or with ``engine_layout`` the [n_final*4][n] array of p4_array."""
+        if engine_layout:
+            arr = np.ascontiguousarray(p4, dtype=np.float64)
+            if arr.ndim != 2 or arr.shape[0] != 4 * len(self.amp.finals):
+                raise ValueError("engine layout is [n_final*4][n]")
+        else:
+            arr = np.ascontiguousarray(p4_array(p4, self.amp.finals, order))
         n = arr.shape[1]
         w = _alpha_weights(n, weight)
         off = ctypes.c_int64()
@@ -219,8 +226,8 @@ class FitServer:
         self.slots = {}
 
     # -------------------------------------------------------------- data and single calls
-    def add_data(self, p4, weight=None, order=None):
-        return self.pool.add(p4, weight, order)
+    def add_data(self, p4, weight=None, order=None, engine_layout=False):
+        return self.pool.add(p4, weight, order, engine_layout)
 
     def release(self, slot):
         self.pool.release(slot)
@@ -321,9 +328,9 @@ class FitServer:
 
     # -------------------------------------------------------------- the serving loop
     def serve(self, jobs, arrivals, poll_s=2e-4, timeout_s=None):
-        """Run ``jobs`` (each a dict with "p4", "y0" and optional "weight", "order"; "p4" may be a
-        callable that draws the sample when the fit is admitted) arriving at ``arrivals``
-        seconds from now. Admission is first come, first served: the head of the
+        """Run ``jobs`` (each a dict with "p4", "y0" and optional "weight", "order",
+        "engine_layout"; "p4" may be a callable that draws the sample when the fit is admitted)
+        arriving at ``arrivals`` seconds from now. Admission is first come, first served: the head of the
         queue waits until the pool has a free range for its sample. Returns (results in job
         order, per-fit dict of times: arrival, admitted, done)."""
         t0 = time.perf_counter()
@@ -346,7 +353,8 @@ class FitServer:
                     p4 = jobs[j]["p4"]
                     staged[j] = p4() if callable(p4) else p4
                 try:
-                    slot = self.add_data(staged[j], jobs[j].get("weight"), jobs[j].get("order"))
+                    slot = self.add_data(staged[j], jobs[j].get("weight"), jobs[j].get("order"),
+                                         jobs[j].get("engine_layout", False))
                 except PoolFull:
                     if not running:
                         raise

@@ -60,6 +60,10 @@ class VarsManager:
         # bumped on every change of a value or of the trainable set, so derived quantities
         # (coefficients, lineshape state) can be cached by version
         self.version = 0
+        # bumped on every change except new values of the trainable variables in order
+        # (set_all with a sequence): what only depends on fixed values and on the trainable
+        # set is cached by this one
+        self.struct_version = 0
 
     # -------------------------------------------------------------- creation
     def add_real_var(self, name, value=None, range_=None, trainable=True, rng=None):
@@ -74,6 +78,7 @@ class VarsManager:
         if trainable:
             self.trainable_vars.append(name)
         self.version += 1
+        self.struct_version += 1
 
     def add_complex_var(self, name, polar=None, trainable=True, fix_vals=(1.0, 0.0), rng=None):
         polar = self.polar if polar is None else polar
@@ -92,6 +97,7 @@ class VarsManager:
     def set_fix(self, name, value=None, unfix=False):
         """variable.py:415-440: fixing removes from trainable_vars, freeing appends."""
         self.version += 1
+        self.struct_version += 1
         if value is not None:
             self.variables[name] = float(value)
         if unfix:
@@ -101,6 +107,7 @@ class VarsManager:
             self.trainable_vars.remove(name)
 
     def set_bound(self, bound_dic):
+        self.struct_version += 1
         for name, (a, b) in bound_dic.items():
             self.bnd_dic[name] = Bound(a, b)
 
@@ -111,6 +118,8 @@ class VarsManager:
     def set(self, name, value):
         self.variables[name] = float(value)
         self.version += 1
+        if name not in self.trainable_vars:
+            self.struct_version += 1
 
     def get_all_val(self):
         return [self.variables[n] for n in self.trainable_vars]
@@ -124,6 +133,7 @@ class VarsManager:
         """dict: by name; sequence: in trainable_vars order (variable.py:684-697)."""
         self.version += 1
         if isinstance(vals, dict):
+            self.struct_version += 1
             for k, v in vals.items():
                 if k in self.variables:
                     self.variables[k] = float(v)
@@ -177,6 +187,7 @@ class VarsManager:
     def std_polar_all(self):
         """r >= 0 and phi in [-pi, pi) for every polar pair (variable.py:716-741)."""
         self.version += 1
+        self.struct_version += 1
         for name, polar in self.complex_vars.items():
             if not polar:
                 continue
