@@ -131,6 +131,44 @@ def test_native_session_matches_python_assembly(case2, precision, gpu_available)
     conf.vm.set_all(x0)
 
 
+@pytest.mark.parametrize("name", ["lc_pkpi_2res", "lc_pkpi_6res"])
+def test_engine_matches_committed_golden_vectors(name, gpu_available):
+    """The FP64 engine on the committed golden inputs against the oracle's committed outputs:
+    per-event |A|^2 (1e-9 relative), NLL (1e-6 absolute), gradient (1e-7 relative) and the
+    exact Hessian (1e-7 of its largest entry)."""
+    import json
+    import os
+
+    from conftest import config_path
+    from tf_pwa_amd import FCN, ConfigLoader
+
+    here = os.path.dirname(os.path.abspath(__file__))
+    with open(os.path.join(here, "golden", "golden_{}.json".format(name))) as fh:
+        fx = json.load(fh)
+    inp, ref = fx["inputs"], fx["outputs"]
+    conf = ConfigLoader(config_path(inp["config"]), seed=0)
+    conf.set_params(inp["params"])
+    assert conf.vm.trainable_vars == ref["trainable"]
+    data = conf.data_from_p4({k: np.array(v) for k, v in inp["data_p4"].items()},
+                             weight=np.array(inp["data_weight"]))
+    mc = conf.data_from_p4({k: np.array(v) for k, v in inp["mc_p4"].items()})
+    amp = conf.get_amplitude()
+    np.testing.assert_allclose(amp(data), ref["f_data"], rtol=1e-9)
+    np.testing.assert_allclose(amp(mc), ref["f_mc"], rtol=1e-9)
+    fcn = FCN(conf.get_model(), data, mc)
+    nll, g = fcn.nll_grad()
+    assert abs(nll - ref["nll"]) < 1e-6
+    rg = np.array(ref["grad"])
+    assert np.linalg.norm(g - rg) <= 1e-7 * np.linalg.norm(rg)
+    _, _, H = fcn.nll_grad_hessian()
+    rH = np.array(ref["hessian"])
+    assert np.abs(H - rH).max() <= 1e-7 * np.abs(rH).max()
+    p = np.random.default_rng(0).normal(size=len(g))
+    g2, Hp = fcn.grad_hessp(np.array(conf.vm.get_all_val()), p)
+    assert np.linalg.norm(g2 - rg) <= 1e-7 * np.linalg.norm(rg)
+    assert np.abs(Hp - rH @ p).max() <= 1e-7 * np.abs(rH @ p).max()
+
+
 def test_gram_matches_partial_waves(case2, gpu_available):
     """FP32 MFMA Gram against the FP64 Gram of the same partial waves (read back)."""
     conf = case2["conf"]

@@ -211,3 +211,25 @@ def test_oracle_gradient_finite_difference(small_model):
         xm[i] -= h
         fd = (float(f(xp)) - float(f(xm))) / (2 * h)
         assert abs(fd - g[i]) < 1e-5 * max(1.0, abs(g[i])), (i, fd, g[i])
+
+
+@pytest.mark.parametrize("name", ["lc_pkpi_2res", "lc_pkpi_6res"])
+def test_oracle_reproduces_committed_golden_vectors(name):
+    """The committed golden vectors are the oracle's outputs on their inputs: re-running the
+    oracle must give the same f, NLL, gradient and Hessian (a regression pin of the oracle)."""
+    import json
+    import os
+
+    from golden.make_golden import compute
+
+    here = os.path.dirname(os.path.abspath(__file__))
+    with open(os.path.join(here, "golden", "golden_{}.json".format(name))) as fh:
+        fx = json.load(fh)
+    out = compute(name, fx["inputs"])
+    ref = fx["outputs"]
+    assert out["trainable"] == ref["trainable"]
+    np.testing.assert_allclose(out["f_data"], ref["f_data"], rtol=1e-12)
+    np.testing.assert_allclose(out["f_mc"], ref["f_mc"], rtol=1e-12)
+    assert abs(out["nll"] - ref["nll"]) < 1e-12 * abs(ref["nll"])
+    np.testing.assert_allclose(out["grad"], ref["grad"], rtol=1e-10, atol=1e-12)
+    np.testing.assert_allclose(out["hessian"], ref["hessian"], rtol=1e-9, atol=1e-10)

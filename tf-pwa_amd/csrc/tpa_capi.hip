@@ -37,7 +37,8 @@ bool g_timing = false;
 double g_timing_ms = 0.0;
 int64_t g_timing_n = 0;
 
-constexpr int kEvalGridMax = 1024;
+constexpr int kEvalGridMax = 8192;     // partial-sum slots per event set
+constexpr int kEvalGridDefault = 1024;
 
 // small_d_weight (tf_pwa/dfun.py:118-143) for 2J = 0..TPA_MAX_J2, tables concatenated.
 std::vector<double> wigner_weights() {
@@ -240,11 +241,17 @@ static int dispatch_shape(int lg, int kg, const EvalShapeArgs& a, int grid, hipS
   }
 }
 
+// Blocks of the evaluation grid: at most kEvalGridDefault (TFPWA_EVAL_GRID: 64 .. kEvalGridMax).
 static int eval_grid(int64_t n, int lanes_per_event) {
+  static const int gmax = [] {
+    const char* e = std::getenv("TFPWA_EVAL_GRID");
+    const int v = e ? std::atoi(e) : kEvalGridDefault;
+    return (v >= 64 && v <= kEvalGridMax) ? v : kEvalGridDefault;
+  }();
   const int64_t epb = 4 * (64 / lanes_per_event);
   int64_t g = (n + epb - 1) / epb;
   if (g < 1) g = 1;
-  return (int)std::min<int64_t>(g, kEvalGridMax);
+  return (int)std::min<int64_t>(g, gmax);
 }
 
 struct tpa_pool_s {
@@ -492,7 +499,7 @@ int tpa_events_create(tpa_model_t m, const double* p4, int64_t n, const double* 
   if ((e = hipMalloc(&ev->d_ang, sizeof(double) * ev->nf * n)) != hipSuccess)
     return bail(e, "alloc angles");
   if ((e = hipMalloc(&ev->d_w, sizeof(double) * n)) != hipSuccess) return bail(e, "alloc w");
-  if ((e = hipMalloc(&ev->d_part, sizeof(double) * kEvalGridMax * TPA_NACC_SHAPE_MAX)) !=
+  if ((e = hipMalloc(&ev->d_part, sizeof(double) * kEvalGridMax * (2 + 6 * std::max(P, 1)))) !=
       hipSuccess)
     return bail(e, "alloc partials");
   if ((e = hipMalloc(&ev->d_out, sizeof(double) * TPA_NACC_SHAPE_MAX)) != hipSuccess)

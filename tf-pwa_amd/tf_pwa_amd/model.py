@@ -256,6 +256,29 @@ class Model:
         H = 0.5 * (H + H.T)
         return nll, g, H
 
+    def grad_hessp_batch(self, p, data, mcdata, weight, mc_weight):
+        """Gradient and Hessian-vector product H p (BaseModel.grad_hessp_batch,
+        tf_pwa/model/model.py:451-511): exact for couplings; central difference of the gradient
+        along p when masses or widths float."""
+        wdata = WeightedData(_as_sets(data), np.concatenate([np.atleast_1d(w) for w in weight])
+                             if isinstance(weight, (list, tuple)) else weight)
+        mc_sets = _as_sets(mcdata)
+        mc_w = (np.concatenate([np.atleast_1d(w) for w in mc_weight])
+                if isinstance(mc_weight, (list, tuple)) else np.asarray(mc_weight))
+        p = np.asarray(p, dtype=np.float64)
+        if not self.Amp.shape_mode:
+            _, g, H = self._nll_grad_hess(wdata, mc_sets, mc_w)
+            return g, H @ p
+        _, g = self._nll_grad(wdata, mc_sets, mc_w)
+        x0 = np.array(self.vm.get_all_val())
+        h = 1e-6 * max(1.0, np.linalg.norm(x0)) / max(np.linalg.norm(p), 1e-300)
+        self.vm.set_all(x0 + h * p)
+        _, gp = self._nll_grad(wdata, mc_sets, mc_w)
+        self.vm.set_all(x0 - h * p)
+        _, gm = self._nll_grad(wdata, mc_sets, mc_w)
+        self.vm.set_all(x0)
+        return g, (gp - gm) / (2 * h)
+
     def set_params(self, var):
         self.Amp.set_params(var)
 
@@ -471,10 +494,14 @@ class FCN:
         return nll + cv, g + cg, H
 
     def grad_hessp(self, x, p, batch=None, eps=1e-6):
-        """Gradient and Hessian-vector product H p (central difference of the gradient
-        along p)."""
+        """Gradient and Hessian-vector product H p (FCN.grad_hessp, model.py:1267-1275): exact
+        for couplings; the central difference of the gradient along p when masses or widths
+        float."""
         x = np.asarray(x, dtype=np.float64)
         p = np.asarray(p, dtype=np.float64)
+        if not self.model.Amp.shape_mode:
+            _, g, H = self.nll_grad_hessian(x)
+            return g, H @ p
         _, g = self.nll_grad(x)
         h = eps * max(1.0, np.linalg.norm(x)) / max(np.linalg.norm(p), 1e-300)
         _, gp = self.get_nll_grad(x + h * p)
