@@ -26,6 +26,7 @@ sys.path.insert(0, ROOT)
 CONFIG = os.path.join(ROOT, "tf-pwa_amd", "tf_pwa_amd", "configs", "lc_pkpi_6res.yml")
 METRIC = "NLL+grad evals/s @1M data+10M MC, |dNLL|<1e-3 vs FP64; toy-fit p50/p99 ms"
 HBM_PEAK_GBS = 8000.0
+ANGLE_BYTES_PER_EVENT = 176
 TRAFFIC = os.path.join(ROOT, "profiles", "k_eval_fold_traffic.json")
 
 
@@ -180,6 +181,9 @@ def serving_leg(conf, mc, data_p4, x_true, args, seed=4242):
         "p99_ms": float(np.percentile(lat, 99)),
         "requests": c["requests"],
         "batches": c["batches"],
+        "events_evaluated": c["events"],
+        "k_eval_multi_bytes": c["events"] * 1416,
+        "achieved_GBps_in_graphs": c["events"] * 1416 / (c["gpu_ms"] * 1e-3) / 1e9,
         "mean_batch": c["requests"] / max(1, c["batches"]),
         "gpu_busy_frac": c["gpu_ms"] / 1e3 / span,
         "span_s": span,
@@ -323,8 +327,14 @@ def main():
         if traffic is not None:
             tr = traffic["data_1M"] if n_events_per_step == args.n_data else 0.5 * (
                 traffic["data_1M"] + traffic["mc_10M"])
+        # the angle-cache input of the same events (22 f64 per event: 3 topologies x 5 + 2 x 3
+        # aligned angles + weight), the bytes a recompute-from-angles kernel would read
+        survey = ANGLE_BYTES_PER_EVENT * n_events_per_step * args.steps / kernel_s / 1e9 \
+            if kernel_s > 0 else 0.0
         return {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": achieved / HBM_PEAK_GBS, "traffic": tr, "kernel": "k_eval_fold",
+                "angle_cache_bytes_per_event": ANGLE_BYTES_PER_EVENT,
+                "achieved_on_angle_cache_bytes": survey,
                 "launches": r["launches"], "kernel_ms_avg": r["kernel_ms"] / max(1, r["launches"]),
                 "bytes_per_event": bytes_per_event,
                 "bytes_per_launch": n_events_per_step * bytes_per_event / max(
@@ -374,6 +384,21 @@ def main():
                            "peak_TFLOPs": 157.3},
         },
     }
+    # exact Hessian (second-moment passes) of both paths, against the 2n gradient evaluations
+    # a central-difference Hessian needs
+    hess = {}
+    for name, f in (("fp64", FCN(conf.get_model(), data, mc)), ("mixed", fmix)):
+        f.nll_grad_hessian(x0)
+        sync()
+        t = time.perf_counter()
+        f.nll_grad_hessian(x0)
+        sync()
+        hess[name] = {"ms": 1e3 * (time.perf_counter() - t),
+                      "central_difference_ms": 2 * len(x0) * (r64 if name == "fp64" else rmix)[
+                          "ms_per_step"]}
+    res["paths"]["hessian"] = hess
+    log("hessian: " + ", ".join("{} {:.2f} ms (FD {:.1f} ms)".format(k, v["ms"], v[
+        "central_difference_ms"]) for k, v in hess.items()))
     if args.serve_fits > 0:
         del fmix, data
         res["serving"] = serving_leg(conf, mc, data_p4, x0, args)

@@ -247,9 +247,10 @@ int tpa_server_result(tpa_server_t server, int fit_id, double* y, double* nll, i
  * microseconds; *n receives the total count, at most max_n values are copied. */
 int tpa_server_latencies(tpa_server_t server, double* us, int64_t max_n, int64_t* n);
 
-/* Requests served, batches launched and summed batched-graph milliseconds. */
+/* Requests served, batches launched, events summed over all requests and summed
+ * batched-graph milliseconds (any pointer may be NULL). */
 int tpa_server_counters(tpa_server_t server, int64_t* requests, int64_t* batches,
-                        double* gpu_ms);
+                        int64_t* events, double* gpu_ms);
 
 /* ---- Single-fit likelihood session (fixed lineshapes).
  * NLL = -sum_data w clip_log f + sw log S_mc and its gradient in the trainable values y, with

@@ -44,7 +44,8 @@ def mocklib(tmp_path_factory):
         "tpa_server_poll": [vp, ip, ctypes.c_int, ip],
         "tpa_server_result": [vp, ctypes.c_int, dp, dp, ip, ctypes.POINTER(i64),
                               ctypes.POINTER(i64)],
-        "tpa_server_counters": [vp, ctypes.POINTER(i64), ctypes.POINTER(i64), dp],
+        "tpa_server_counters": [vp, ctypes.POINTER(i64), ctypes.POINTER(i64),
+                                ctypes.POINTER(i64), dp],
     }
     for name, args in sig.items():
         getattr(L, name).argtypes = args
@@ -245,10 +246,12 @@ def test_server_fits_converge_and_ignore_the_mix(mocklib, hess_start):
         _, g = _nll_grad(mocklib, s, slot, y)
         assert np.abs(g).max() < 1e-3
         assert 0.1 <= y[0] <= 3.0  # the bounded magnitude stays inside its bounds
-    r, b = ctypes.c_int64(), ctypes.c_int64()
+    r, b, ev = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int64()
     ms = ctypes.c_double()
-    mocklib.tpa_server_counters(s, ctypes.byref(r), ctypes.byref(b), ctypes.byref(ms))
+    mocklib.tpa_server_counters(s, ctypes.byref(r), ctypes.byref(b), ctypes.byref(ev),
+                                ctypes.byref(ms))
     assert b.value < r.value  # batches were shared between fits
+    assert ev.value > r.value
     mocklib.tpa_server_destroy(s)
 
 

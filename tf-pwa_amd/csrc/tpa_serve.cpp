@@ -75,7 +75,7 @@ struct tpa_server_s : tpa_cpl::CouplingMap {
   std::string batch_error;
   std::thread batcher;
   std::vector<double> lat_us;
-  int64_t n_requests = 0, n_batches = 0;
+  int64_t n_requests = 0, n_batches = 0, n_events = 0;
   double gpu_ms = 0.0;
 };
 
@@ -420,6 +420,7 @@ void batcher_main(tpa_server_s* S) {
     const auto now = Clock::now();
     S->n_batches += 1;
     S->n_requests += m;
+    for (int i = 0; i < m; ++i) S->n_events += ranges[2 * i + 1];
     S->gpu_ms += ms;
     if (rc != TPA_OK) S->batch_error = err;
     for (int i = 0; i < m; ++i) {
@@ -586,11 +587,13 @@ int tpa_server_latencies(tpa_server_t S, double* us, int64_t max_n, int64_t* n) 
   return TPA_OK;
 }
 
-int tpa_server_counters(tpa_server_t S, int64_t* requests, int64_t* batches, double* gpu_ms) {
+int tpa_server_counters(tpa_server_t S, int64_t* requests, int64_t* batches, int64_t* events,
+                        double* gpu_ms) {
   if (!S) return tpa_set_error(TPA_ERR_ARG, "null pointer");
   std::lock_guard<std::mutex> lk(S->mu);
   if (requests) *requests = S->n_requests;
   if (batches) *batches = S->n_batches;
+  if (events) *events = S->n_events;
   if (gpu_ms) *gpu_ms = S->gpu_ms;
   return TPA_OK;
 }

@@ -157,8 +157,8 @@ def lib():
         "tpa_server_result": ([vp, ctypes.c_int, dp, dp, ctypes.POINTER(ctypes.c_int),
                                ctypes.POINTER(i64), ctypes.POINTER(i64)], ctypes.c_int),
         "tpa_server_latencies": ([vp, dp, i64, ctypes.POINTER(i64)], ctypes.c_int),
-        "tpa_server_counters": ([vp, ctypes.POINTER(i64), ctypes.POINTER(i64), dp],
-                                ctypes.c_int),
+        "tpa_server_counters": ([vp, ctypes.POINTER(i64), ctypes.POINTER(i64),
+                                 ctypes.POINTER(i64), dp], ctypes.c_int),
     }
     for name, (args, res) in sig.items():
         fn = getattr(L, name)

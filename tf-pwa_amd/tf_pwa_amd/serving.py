@@ -309,11 +309,11 @@ class FitServer:
         return out[:n.value]
 
     def counters(self):
-        r, b = ctypes.c_int64(), ctypes.c_int64()
+        r, b, e = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int64()
         ms = ctypes.c_double()
         _lib.check(_lib.lib().tpa_server_counters(self._h, ctypes.byref(r), ctypes.byref(b),
-                                                  ctypes.byref(ms)))
-        return dict(requests=r.value, batches=b.value, gpu_ms=ms.value)
+                                                  ctypes.byref(e), ctypes.byref(ms)))
+        return dict(requests=r.value, batches=b.value, events=e.value, gpu_ms=ms.value)
 
     def close(self):
         h = getattr(self, "_h", None)
