@@ -36,6 +36,23 @@ def load_dat_file(fnames, n_final):
     return np.concatenate(arrs, axis=0)
 
 
+def inverse_hessian(h, force_pos=True):
+    """Error matrix from a Hessian: the inverse when it is positive definite; otherwise, as in
+    force_pos_def (tf_pwa/applications.py:249-277), eigenvalues that are negative only by
+    rounding (above -1e-5 of the largest) are lifted to positive, and larger negative
+    eigenvalues are replaced by their magnitude (a positive-definite stand-in)."""
+    h = 0.5 * (np.asarray(h) + np.asarray(h).T)
+    e, v = np.linalg.eigh(h)
+    if e.min() > 0 or not force_pos:
+        return np.linalg.inv(h) if e.min() > 0 else np.linalg.pinv(h)
+    emax = np.abs(e).max()
+    if e.min() > -1e-5 * emax:
+        e_new = np.where(e < 0, e - e.min() * 1.1, e)
+    else:
+        e_new = np.where(np.abs(e) > 1e-12 * emax, np.abs(e), 1e-12 * emax)
+    return (v / e_new) @ v.T
+
+
 class ConfigLoader:
     """Build the decay group, variables, data and likelihood from a config.yml dict/file."""
 
@@ -186,6 +203,33 @@ class ConfigLoader:
         data, phsp = all_data[0], all_data[1]
         bg = all_data[2] if len(all_data) > 2 else None
         return FCN(self.get_model(), data, phsp, bg=bg, batch=batch)
+
+    def get_params_error(self, params=None, data=None, phsp=None, bg=None, batch=10000,
+                         force_pos=True, fcn=None):
+        """Errors from the inverse Hessian (ConfigLoader.get_params_error,
+        config_loader.py:814-870 with cal_hesse_error / force_pos_def,
+        applications.py:249-311): exact Hessian for couplings. Returns {name: error} and keeps
+        the error matrix in ``self.inv_he``."""
+        if params is not None:
+            self.set_params(getattr(params, "params", params))
+        if fcn is None:
+            if data is None:
+                data, phsp, bg, _ = self.get_all_data()
+            fcn = FCN(self.get_model(), data, phsp, bg=bg, batch=batch)
+        _, _, h = fcn.nll_grad_hessian()
+        self.inv_he = inverse_hessian(h, force_pos=force_pos)
+        err = np.sqrt(np.fabs(self.inv_he.diagonal()))
+        return dict(zip(self.vm.trainable_vars, err.tolist()))
+
+    def cal_fitfractions(self, mc=None, res=None, gram=None, error_matrix=None):
+        """Fit fractions and their errors (tf_pwa/fitfractions.py:178-258) from the MC Gram
+        matrix; ``error_matrix`` defaults to the last get_params_error result."""
+        from .fitfractions import cal_fitfractions
+
+        if error_matrix is None:
+            error_matrix = getattr(self, "inv_he", None)
+        return cal_fitfractions(self.get_amplitude(), mc, res=res, gram=gram,
+                                error_matrix=error_matrix)
 
     def fit(self, data=None, phsp=None, bg=None, batch=65000, method="BFGS", maxiter=None,
             print_init_nll=False):
