@@ -322,7 +322,19 @@ TPA_DEV void neumaier_merge(double& s, double& c, double s2, double c2) {
 // helicity components of one event, and the KG groups split the partial waves
 // k = kg + KG * j (j < PML). A_lambda is completed by a reduction over the k-groups and f by a
 // reduction over lambda, so each lane keeps only PML amplitudes and PML gradient sums.
-template <int LG, int KG, int PML, int U>
+// NT: stream the cache with non-temporal loads (it is read once per evaluation).
+template <bool NT>
+TPA_DEV double2 load_pw(const double2* p) {
+  if constexpr (NT) {
+    typedef double d2v __attribute__((ext_vector_type(2)));
+    const d2v v = __builtin_nontemporal_load(reinterpret_cast<const d2v*>(p));
+    return make_double2(v.x, v.y);
+  } else {
+    return *p;
+  }
+}
+
+template <int LG, int KG, int PML, int U, bool NT = false>
 __global__ __launch_bounds__(256) void k_eval_fold(EvalArgs a) {
   constexpr int LPE = LG * KG;  // lanes per event
   constexpr int EPW = 64 / LPE;
@@ -356,7 +368,7 @@ __global__ __launch_bounds__(256) void k_eval_fold(EvalArgs a) {
 #pragma unroll
       for (int j = 0; j < PML; ++j) {
         const int k = kg + KG * j;
-        hv[uu][j] = (ok[uu] && k < a.P) ? a.h[((int64_t)k * a.N + ev[uu]) * LG + lam]
+        hv[uu][j] = (ok[uu] && k < a.P) ? load_pw<NT>(&a.h[((int64_t)k * a.N + ev[uu]) * LG + lam])
                                         : make_double2(0.0, 0.0);
       }
     }
@@ -544,7 +556,7 @@ __global__ __launch_bounds__(256) void k_gram_f32(const double2* __restrict__ h,
     double2 hv[32];
 #pragma unroll
     for (int k = 0; k < 32; ++k)
-      hv[k] = (k < P && row < R) ? h[(int64_t)k * R + row] : make_double2(0.0, 0.0);
+      hv[k] = (k < P && row < R) ? load_pw<true>(&h[(int64_t)k * R + row]) : make_double2(0.0, 0.0);
 #pragma unroll
     for (int k = 0; k < 32; ++k) {
       Xs[k * TPA_GRAM_LDS_STRIDE + t] = (float)hv[k].x;
@@ -693,7 +705,7 @@ __global__ __launch_bounds__(256) void k_eval_shape(EvalShapeArgs s) {
     for (int j = 0; j < PML; ++j) {
       const int k = kg + KG * j;
       const bool ld = valid && k < a.P;
-      hv[j] = ld ? a.h[((int64_t)k * a.N + e) * LG + lam] : make_double2(0.0, 0.0);
+      hv[j] = ld ? load_pw<true>(&a.h[((int64_t)k * a.N + e) * LG + lam]) : make_double2(0.0, 0.0);
       bv[j] = ld ? s.bw[(int64_t)kc[j] * a.N + e] : make_double2(0.0, 0.0);
     }
     double ar = 0.0, ai = 0.0;
@@ -859,7 +871,8 @@ __global__ __launch_bounds__(256) void k_eval_multi(MultiArgs a) {
 #pragma unroll
     for (int j = 0; j < PML; ++j) {
       const int k = kg + KG * j;
-      hv[j] = (valid && k < a.P) ? a.h[((int64_t)k * a.S + e) * LG + lam] : make_double2(0.0, 0.0);
+      hv[j] = (valid && k < a.P) ? load_pw<true>(&a.h[((int64_t)k * a.S + e) * LG + lam])
+                                 : make_double2(0.0, 0.0);
     }
     double ar = 0.0, ai = 0.0;
 #pragma unroll
@@ -1039,7 +1052,8 @@ __global__ __launch_bounds__(256) void k_hess(HessArgs a) {
     for (int idx = t; idx < P * TPA_HESS_PAIRS; idx += 256) {
       const int k = idx / TPA_HESS_PAIRS, p = idx % TPA_HESS_PAIRS;
       const int64_t e = e0 + p / LG;
-      hs[k][p] = (e < e_end) ? a.h[((int64_t)k * a.S + e) * LG + (p % LG)] : make_double2(0.0, 0.0);
+      hs[k][p] = (e < e_end) ? load_pw<true>(&a.h[((int64_t)k * a.S + e) * LG + (p % LG)])
+                             : make_double2(0.0, 0.0);
     }
     __syncthreads();
     if (t < TPA_HESS_PAIRS) {
