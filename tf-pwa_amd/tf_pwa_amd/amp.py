@@ -184,7 +184,15 @@ class AmplitudeModel:
         return out
 
     def barrier_constants(self):
-        """Per chain: sqrt(P_l(z0)) (or 1 when clamped) and the act flags of both vertices."""
+        """Per chain: sqrt(P_l(z0)) (or 1 when clamped) and the act flags of both vertices,
+        cached by the lineshape parameters they depend on."""
+        key = tuple(self.shape_params())
+        if getattr(self, "_bc_key", None) != key:
+            self._bc = self._barrier_constants()
+            self._bc_key = key
+        return self._bc
+
+    def _barrier_constants(self):
         top = self.decay_group.top
         out = []
         for ch, (m0, g0) in zip(self.chains, self.shape_params()):
@@ -218,7 +226,21 @@ class AmplitudeModel:
         return out
 
     def describe(self, fold=True):
-        """The TpaModel structure for the current parameter values."""
+        """The TpaModel structure for the current parameter values. The structure is cached:
+        when only masses and widths changed since the last call, their fields are updated."""
+        bc = self.barrier_constants()
+        flags = tuple((tuple(b["act1"]), tuple(b["act2"])) for b in bc)
+        cached = getattr(self, "_desc_cache", None)
+        if cached is not None and cached[0] == flags:
+            M = cached[1]
+            for c, (m0, g0) in enumerate(self.shape_params()):
+                M.chain[c].m0, M.chain[c].g0 = m0, g0
+            return M
+        M = self._describe()
+        self._desc_cache = (flags, M)
+        return M
+
+    def _describe(self):
         M = _lib.TpaModel()
         g = self.decay_group
         M.n_final = 3
@@ -353,7 +375,15 @@ class AmplitudeModel:
         return self._fidx
 
     def _factor_values(self):
-        """Complex value of every unique factor and its derivatives d/dr, d/di."""
+        """Complex value of every unique factor and its derivatives d/dr, d/di (cached by the
+        parameter store's version)."""
+        if getattr(self, "_fv_version", None) == self.vm.version:
+            return self._fv
+        self._fv = self._factor_values_now()
+        self._fv_version = self.vm.version
+        return self._fv
+
+    def _factor_values_now(self):
         fx = self._factor_index()
         vm = self.vm
         r = np.array([vm.variables[n + "r"] for n in fx["cnames"]])
@@ -497,21 +527,30 @@ class AmplitudeModel:
         d/dm0 = 2 Re sum_{k in c} [(d c_k/d m0) G_k + c_k Dm_k] (the barrier constants of both
         vertices depend on m0) and d/dg0 = 2 Re sum_{k in c} c_k Dg_k."""
         vm = self.vm
-        index = {n: i for i, n in enumerate(vm.trainable_vars)}
-        out = np.zeros(len(vm.trainable_vars))
+        n = len(vm.trainable_vars)
+        key = (tuple(vm.trainable_vars), tuple(self.shape_params()))
+        if getattr(self, "_sg_key", None) != key:
+            index = {nm: i for i, nm in enumerate(vm.trainable_vars)}
+            fx = self._factor_index()
+            bc = self.barrier_constants()
+            names = self._names()
+            dlog = np.array([bc[ch]["dlog1"][a] + bc[ch]["dlog2"][b]
+                             for ch, a, b in zip(fx["ch"], fx["s1"], fx["s2"])])
+            mrow = np.array([index.get(nm["mass"], -1) for nm in names])[fx["ch"]]
+            wrow = np.array([index.get(nm["width"], -1) for nm in names])[fx["ch"]]
+            self._sg = (dlog, mrow, wrow)
+            self._sg_key = key
+        dlog, mrow, wrow = self._sg
         c = self.coefficients()
-        bc = self.barrier_constants()
-        fx = self._factor_index()
-        names = self._names()
-        dlog = np.array([bc[ch]["dlog1"][a] + bc[ch]["dlog2"][b]
-                         for ch, a, b in zip(fx["ch"], fx["s1"], fx["s2"])])
-        for ci, nm in enumerate(names):
-            ks = fx["ch"] == ci
-            if nm["mass"] in index:
-                out[index[nm["mass"]]] += 2.0 * np.real(
-                    np.sum(c[ks] * dlog[ks] * G[ks]) + np.sum(c[ks] * Dm[ks]))
-            if nm["width"] in index:
-                out[index[nm["width"]]] += 2.0 * np.real(np.sum(c[ks] * Dg[ks]))
+        out = np.zeros(n)
+        tm = 2.0 * np.real(c * (dlog * G + Dm))
+        tw = 2.0 * np.real(c * Dg)
+        m = mrow >= 0
+        if m.any():
+            out += np.bincount(mrow[m], weights=tm[m], minlength=n)
+        m = wrow >= 0
+        if m.any():
+            out += np.bincount(wrow[m], weights=tw[m], minlength=n)
         return out
 
     # ---------------------------------------------------------------- parameters
