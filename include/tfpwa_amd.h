@@ -183,19 +183,22 @@ int tpa_pool_eval(tpa_pool_t pool, int n_items, const int64_t* ranges, const dou
 
 /* ---- The serving runtime: many BFGS fits at once over one pool.
  * The coupling map says how the partial-wave coefficients follow from a fit's trainable
- * parameters y (tf-pwa's trainable_vars order): c_k = wave_const[k] * v[f0] * v[f1] * v[f2]
- * with f = wave_factor[3k..3k+2] (the chain total and the g_ls of both vertices,
- * tf_pwa/amp/core.py:1000-1080) and v[f] = r e^{i phi} (factor_polar) or r + i phi, where r and
+ * parameters y (tf-pwa's trainable_vars order): c_k = wave_const[k] * prod_a v[f_a] over the
+ * TPA_CPL_SLOTS factors f = wave_factor[4k..4k+3] (the chain total and the g_ls of each vertex,
+ * tf_pwa/amp/core.py:1000-1080; a fixed factor 1 fills unused slots) and
+ * v[f] = r e^{i phi} (factor_polar) or r + i phi, where r and
  * phi are y[factor_var[2f]] and y[factor_var[2f+1]], or factor_fixed[2f], [2f+1] when the index
  * is -1. bound_kind[j] (0 none, 1 [a, b], 2 lower a, 3 upper b, with a, b = bound_ab[2j],
  * [2j+1]) is the bound transform of y[j] (tf_pwa/variable.py:1029-1135); BFGS runs in the
  * unbounded variables. */
+#define TPA_CPL_SLOTS 4
+
 typedef struct {
   int32_t P;
   int32_t n_factor;
   int32_t n_var;
   int32_t reserved;
-  const int32_t* wave_factor;   /* [P][3] */
+  const int32_t* wave_factor;   /* [P][TPA_CPL_SLOTS] */
   const double* wave_const;     /* [P] */
   const int32_t* factor_polar;  /* [n_factor] */
   const int32_t* factor_var;    /* [n_factor][2] */

@@ -70,11 +70,12 @@ class Problem:
         self.h[1] *= 0.4
         hm = rng.normal(size=(P, 20000, LG)) + 1j * rng.normal(size=(P, 20000, LG))
         self.gram = np.einsum("kel,mel->km", hm.conj(), hm) / 20000
-        # factors: 0..2 chain totals, 3..8 wave couplings g_k, 9 a fixed cartesian factor
-        self.nf = 10
-        self.wave_factor = np.array([[k // 2, 3 + k, 9] for k in range(P)], dtype=np.int32)
+        # factors: 0..2 chain totals, 3..8 wave couplings g_k, 9 a fixed cartesian factor,
+        # 10 the fixed unit factor of the unused fourth slot
+        self.nf = 11
+        self.wave_factor = np.array([[k // 2, 3 + k, 9, 10] for k in range(P)], dtype=np.int32)
         self.wave_const = np.linspace(0.8, 1.3, P)
-        self.factor_polar = np.array([1] * 9 + [0], dtype=np.int32)
+        self.factor_polar = np.array([1] * 9 + [0, 0], dtype=np.int32)
         fv = -np.ones((self.nf, 2), dtype=np.int32)
         nv = 0
         for f in (1, 2, 4, 5, 6, 7, 8):  # factor 0 (chain 0 total) and 3 (g_0) fixed
@@ -84,6 +85,7 @@ class Problem:
         self.factor_fixed = np.zeros((self.nf, 2))
         self.factor_fixed[:, 0] = 1.0
         self.factor_fixed[9] = (0.9, 0.2)
+        self.factor_fixed[10] = (1.0, 0.0)
         self.nv = nv
         self.bound_kind = np.zeros(nv, dtype=np.int32)
         self.bound_ab = np.zeros((nv, 2))
@@ -128,7 +130,7 @@ class Problem:
         i = np.where(fv[:, 1] >= 0, y[np.maximum(fv[:, 1], 0)], self.factor_fixed[:, 1])
         v = np.where(self.factor_polar == 1, r * np.exp(1j * i), r + 1j * i)
         wf = self.wave_factor
-        return self.wave_const * v[wf[:, 0]] * v[wf[:, 1]] * v[wf[:, 2]]
+        return self.wave_const * v[wf[:, 0]] * v[wf[:, 1]] * v[wf[:, 2]] * v[wf[:, 3]]
 
     def nll(self, slot, y):
         """numpy restatement: -sum log f + n log(c^dagger I c) (unit weights, f >> 1e-6)."""

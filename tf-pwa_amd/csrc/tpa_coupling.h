@@ -1,6 +1,6 @@
 // Coupling map of the partial-wave coefficients and the host-side chain rules shared by the
 // serving runtime (tpa_serve.cpp) and the single-fit likelihood session (tpa_fcn.cpp):
-// c_k = wave_const_k * v_f0 * v_f1 * v_f2 with polar (r e^{i phi}) or cartesian factors whose
+// c_k = wave_const_k * v_f0 * ... * v_f(NS-1) with polar (r e^{i phi}) or cartesian factors whose
 // real parts are trainable variables or fixed values, the "asin" bound transforms of
 // tf_pwa/variable.py:1029-1135, and the Wirtinger-calculus chain rules from derivatives in c to
 // derivatives in the variables.
@@ -17,10 +17,11 @@
 namespace tpa_cpl {
 
 using cplx = std::complex<double>;
+constexpr int NS = TPA_CPL_SLOTS;  // factors per partial wave (total + one per vertex)
 
 struct CouplingMap {
   int P = 0, nf = 0, nv = 0;
-  std::vector<int> wave_factor;  // [P][3]
+  std::vector<int> wave_factor;  // [P][NS]
   std::vector<double> wave_const;
   std::vector<int> factor_polar, factor_var;  // [nf], [nf][2]
   std::vector<double> factor_fixed;           // [nf][2]
@@ -39,7 +40,7 @@ struct CouplingMap {
     P = map->P;
     nf = map->n_factor;
     nv = map->n_var;
-    for (int i = 0; i < 3 * P; ++i)
+    for (int i = 0; i < NS * P; ++i)
       if (map->wave_factor[i] < 0 || map->wave_factor[i] >= nf)
         return tpa_set_error(TPA_ERR_ARG, "wave factor index out of range");
     for (int i = 0; i < 2 * nf; ++i)
@@ -48,7 +49,7 @@ struct CouplingMap {
     for (int j = 0; j < nv; ++j)
       if (map->bound_kind[j] < 0 || map->bound_kind[j] > 3)
         return tpa_set_error(TPA_ERR_ARG, "bad bound kind");
-    wave_factor.assign(map->wave_factor, map->wave_factor + 3 * P);
+    wave_factor.assign(map->wave_factor, map->wave_factor + NS * P);
     wave_const.assign(map->wave_const, map->wave_const + P);
     factor_polar.assign(map->factor_polar, map->factor_polar + nf);
     factor_var.assign(map->factor_var, map->factor_var + 2 * nf);
@@ -182,10 +183,10 @@ inline void grad_from_F1(const CouplingMap* S, const cplx* v, const cplx* dr, co
   std::fill(gy, gy + S->nv, 0.0);
   for (int k = 0; k < S->P; ++k) {
     const cplx comb = S->wave_const[k] * F1[k];
-    const int* w = &S->wave_factor[3 * k];
-    for (int a = 0; a < 3; ++a) {
+    const int* w = &S->wave_factor[NS * k];
+    for (int a = 0; a < NS; ++a) {
       cplx rest = comb;
-      for (int b = 0; b < 3; ++b)
+      for (int b = 0; b < NS; ++b)
         if (b != a) rest *= v[w[b]];
       const int f = w[a];
       const int vr = S->factor_var[2 * f], vi = S->factor_var[2 * f + 1];
@@ -203,8 +204,10 @@ inline void coefficients(const CouplingMap* S, const double* y, std::vector<cplx
   c.resize(S->P);
   factor_values(S, y, v.data(), dr.data(), di.data());
   for (int k = 0; k < S->P; ++k) {
-    const int* w = &S->wave_factor[3 * k];
-    c[k] = S->wave_const[k] * v[w[0]] * v[w[1]] * v[w[2]];
+    const int* w = &S->wave_factor[NS * k];
+    cplx ck = S->wave_const[k];
+    for (int a = 0; a < NS; ++a) ck *= v[w[a]];
+    c[k] = ck;
   }
 }
 
@@ -232,10 +235,10 @@ inline void hess_chain(const CouplingMap* S, const cplx* v, const cplx* dr, cons
   // J[a][k] = d c_k / d y_a
   std::vector<cplx> J((size_t)nv * P, 0.0);
   for (int k = 0; k < P; ++k) {
-    const int* w = &S->wave_factor[3 * k];
-    for (int a = 0; a < 3; ++a) {
+    const int* w = &S->wave_factor[NS * k];
+    for (int a = 0; a < NS; ++a) {
       cplx rest = S->wave_const[k];
-      for (int b = 0; b < 3; ++b)
+      for (int b = 0; b < NS; ++b)
         if (b != a) rest *= v[w[b]];
       const int f = w[a];
       const int vr = S->factor_var[2 * f], vi = S->factor_var[2 * f + 1];
@@ -266,12 +269,12 @@ inline void hess_chain(const CouplingMap* S, const cplx* v, const cplx* dr, cons
   // sum_k F1_k d2c_k/dy dy: one polar factor twice, or two different factors once each
   for (int k = 0; k < P; ++k) {
     const cplx CF = S->wave_const[k] * F1[k];
-    const int* w = &S->wave_factor[3 * k];
-    for (int a = 0; a < 3; ++a) {
+    const int* w = &S->wave_factor[NS * k];
+    for (int a = 0; a < NS; ++a) {
       const int f = w[a];
       const int fr = S->factor_var[2 * f], fi = S->factor_var[2 * f + 1];
       cplx rest = CF;
-      for (int b = 0; b < 3; ++b)
+      for (int b = 0; b < NS; ++b)
         if (b != a) rest *= v[w[b]];
       if (S->factor_polar[f]) {
         const cplx e = dr[f];  // e^{i phi}
@@ -281,11 +284,12 @@ inline void hess_chain(const CouplingMap* S, const cplx* v, const cplx* dr, cons
         }
         if (fi >= 0) M[(size_t)fi * nv + fi] += -v[f] * rest;
       }
-      for (int b = 0; b < 3; ++b) {
+      for (int b = 0; b < NS; ++b) {
         if (b == a) continue;
         const int g = w[b];
-        const int third = 3 - a - b;
-        const cplx rest2 = CF * v[w[third]];
+        cplx rest2 = CF;
+        for (int t = 0; t < NS; ++t)
+          if (t != a && t != b) rest2 *= v[w[t]];
         const int va[2] = {S->factor_var[2 * f], S->factor_var[2 * f + 1]};
         const int vb[2] = {S->factor_var[2 * g], S->factor_var[2 * g + 1]};
         const cplx da[2] = {dr[f], di[f]}, db[2] = {dr[g], di[g]};

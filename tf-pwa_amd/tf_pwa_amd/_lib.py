@@ -27,6 +27,7 @@ MODE_MC = 1
 FLAG_GRAD = 1
 FLAG_FOUT = 2
 FLAG_SHAPE = 4
+CPL_SLOTS = 4  # factor slots per partial wave in TpaCouplingMap (total + up to 3 vertices)
 
 _i32 = ctypes.c_int32
 

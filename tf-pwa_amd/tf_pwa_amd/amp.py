@@ -75,10 +75,20 @@ def _rel_p2_d_daughter(M, m1, m2, which):
 
 
 class PartialWave:
-    __slots__ = ("chain", "ls1", "ls2")
+    """One partial wave: a chain and the LS index at each of its vertices."""
 
-    def __init__(self, chain, ls1, ls2):
-        self.chain, self.ls1, self.ls2 = chain, ls1, ls2
+    __slots__ = ("chain", "ls")
+
+    def __init__(self, chain, *ls):
+        self.chain, self.ls = chain, tuple(ls)
+
+    @property
+    def ls1(self):
+        return self.ls[0]
+
+    @property
+    def ls2(self):
+        return self.ls[1]
 
 
 class AmplitudeModel:
@@ -164,11 +174,9 @@ class AmplitudeModel:
         """Variable names of every chain: total, the g_ls of both vertices, mass and width."""
         out = []
         for c, ch in zip(self.decay_group.chains, self.chains):
-            out.append(dict(total=c.params_head() + "_total_0",
-                            g1=[ch["d1"].params_head() + "_g_ls_{}".format(i)
-                                for i in range(len(ch["d1"].ls))],
-                            g2=[ch["d2"].params_head() + "_g_ls_{}".format(i)
-                                for i in range(len(ch["d2"].ls))],
+            g1 = [ch["d1"].params_head() + "_g_ls_{}".format(i) for i in range(len(ch["d1"].ls))]
+            g2 = [ch["d2"].params_head() + "_g_ls_{}".format(i) for i in range(len(ch["d2"].ls))]
+            out.append(dict(total=c.params_head() + "_total_0", g1=g1, g2=g2, g=[g1, g2],
                             mass=ch["R"].name + "_mass", width=ch["R"].name + "_width"))
         return out
 
@@ -222,7 +230,9 @@ class AmplitudeModel:
                 g2.append(bprime_poly_deriv(l, q02 * d * d) * d * d * dq02 / (2 * p0)
                           if p0 > 0 else 0.0)
             out.append(dict(c1=np.array(c1), c2=np.array(c2), act1=a1, act2=a2,
-                            q01=q01, q02=q02, dlog1=np.array(g1), dlog2=np.array(g2)))
+                            q01=q01, q02=q02, dlog1=np.array(g1), dlog2=np.array(g2),
+                            c=[np.array(c1), np.array(c2)], act=[a1, a2],
+                            dlog=[np.array(g1), np.array(g2)]))
         return out
 
     def describe(self, fold=True):
@@ -348,8 +358,9 @@ class AmplitudeModel:
 
     # ---------------------------------------------------------------- coefficients
     def _factor_index(self):
-        """Unique complex factors (chain totals and g_ls of both vertices) and, per partial
-        wave, the indices of its three factors."""
+        """Unique complex factors (chain totals and the g_ls of every vertex) and, per partial
+        wave, the index of each of its factors: slots[0] the chain total, slots[1 + v] the g_ls
+        of vertex v (three slots for three-body chains, four for four-body ones)."""
         if getattr(self, "_fidx", None) is None:
             names = self._names()
             cnames, pos = [], {}
@@ -360,18 +371,19 @@ class AmplitudeModel:
                     cnames.append(n)
                 return pos[n]
 
-            it, i1, i2, ch, s1, s2 = [], [], [], [], [], []
+            nv = len(self.waves[0].ls)
+            slots = [[] for _ in range(1 + nv)]
+            ch = []
             for w in self.waves:
                 nm = names[w.chain]
-                it.append(ix(nm["total"]))
-                i1.append(ix(nm["g1"][w.ls1]))
-                i2.append(ix(nm["g2"][w.ls2]))
+                slots[0].append(ix(nm["total"]))
+                for v in range(nv):
+                    slots[1 + v].append(ix(nm["g"][v][w.ls[v]]))
                 ch.append(w.chain)
-                s1.append(w.ls1)
-                s2.append(w.ls2)
-            self._fidx = dict(cnames=cnames, it=np.array(it), i1=np.array(i1),
-                              i2=np.array(i2), ch=np.array(ch), s1=np.array(s1),
-                              s2=np.array(s2))
+            slots = [np.array(x) for x in slots]
+            ls = np.array([w.ls for w in self.waves]).reshape(len(self.waves), nv)
+            self._fidx = dict(cnames=cnames, slots=slots, ch=np.array(ch), ls=ls,
+                              it=slots[0], i1=slots[1], i2=slots[2], s1=ls[:, 0], s2=ls[:, 1])
         return self._fidx
 
     def _factor_values(self):
@@ -400,8 +412,8 @@ class AmplitudeModel:
         if getattr(self, "_bpw_key", None) != key:
             fx = self._factor_index()
             bc = self.barrier_constants()
-            self._bpw = np.array([bc[c]["c1"][a] * bc[c]["c2"][b]
-                                  for c, a, b in zip(fx["ch"], fx["s1"], fx["s2"])])
+            self._bpw = np.array([np.prod([bc[c]["c"][v][l] for v, l in enumerate(ls)])
+                                  for c, ls in zip(fx["ch"], fx["ls"])])
             self._bpw_key = key
         return self._bpw
 
@@ -409,7 +421,18 @@ class AmplitudeModel:
         """c_k for the current parameters (complex array of length P)."""
         fx = self._factor_index()
         val, _, _ = self._factor_values()
-        return val[fx["it"]] * val[fx["i1"]] * val[fx["i2"]] * self._barrier_per_wave()
+        c = self._barrier_per_wave().astype(np.complex128)
+        for sl in fx["slots"]:
+            c = c * val[sl]
+        return c
+
+    @staticmethod
+    def _others(val, slots, skip, base):
+        out = base
+        for b, sl in enumerate(slots):
+            if b not in skip:
+                out = out * val[sl]
+        return out
 
     def jacobian(self):
         """J[i, k] = d c_k / d trainable_vars[i] (complex) for the coupling parameters.
@@ -424,16 +447,13 @@ class AmplitudeModel:
         fx = self._factor_index()
         val, d_r, d_i = self._factor_values()
         C = self._barrier_per_wave()
-        slots = (fx["it"], fx["i1"], fx["i2"])
+        slots = fx["slots"]
         J = np.zeros((len(vm.trainable_vars), self.P), dtype=np.complex128)
         ks = np.arange(self.P)
         row_r = np.array([index.get(n + "r", -1) for n in fx["cnames"]])
         row_i = np.array([index.get(n + "i", -1) for n in fx["cnames"]])
-        for a in range(3):
-            rest = C.astype(np.complex128)
-            for b in range(3):
-                if b != a:
-                    rest = rest * val[slots[b]]
+        for a in range(len(slots)):
+            rest = self._others(val, slots, (a,), C.astype(np.complex128))
             f = slots[a]
             for rows, d in ((row_r, d_r), (row_i, d_i)):
                 rr = rows[f]
@@ -459,15 +479,12 @@ class AmplitudeModel:
         fx = self._factor_index()
         val, d_r, d_i = self._factor_values()
         C = self._barrier_per_wave()
-        slots = (fx["it"], fx["i1"], fx["i2"])
+        slots = fx["slots"]
         n = len(self.vm.trainable_vars)
         out = np.zeros(n)
         CG = C * np.asarray(G)
-        for a in range(3):
-            rest = CG
-            for b in range(3):
-                if b != a:
-                    rest = rest * val[slots[b]]
+        for a in range(len(slots)):
+            rest = self._others(val, slots, (a,), CG)
             f = slots[a]
             for rows, d in ((row_r, d_r), (row_i, d_i)):
                 rr = rows[f]
@@ -489,16 +506,13 @@ class AmplitudeModel:
         val, d_r, d_i = self._factor_values()
         polar = np.array([self.vm.complex_vars.get(n, self.vm.polar) for n in fx["cnames"]])
         C = self._barrier_per_wave() * np.asarray(F1)
-        slots = (fx["it"], fx["i1"], fx["i2"])
+        slots = fx["slots"]
         n = H.shape[0]
         M = np.zeros((n, n), dtype=np.complex128)
         i_unit = 1j
-        for a in range(3):
+        for a in range(len(slots)):
             f = slots[a]
-            rest = C.copy().astype(np.complex128)
-            for b in range(3):
-                if b != a:
-                    rest = rest * val[slots[b]]
+            rest = self._others(val, slots, (a,), C.astype(np.complex128))
             # second derivatives of one polar factor: d2/dr dphi = i e^{i phi},
             # d2/dphi2 = -r e^{i phi} (cartesian factors are linear)
             e = d_r[f]
@@ -509,12 +523,11 @@ class AmplitudeModel:
             m = polar[f] & (ri >= 0)
             np.add.at(M, (ri[m], ri[m]), -val[f][m] * rest[m])
             # products of first derivatives of two different factors
-            for b in range(3):
+            for b in range(len(slots)):
                 if b == a:
                     continue
                 g = slots[b]
-                third = [s_ for s_ in range(3) if s_ not in (a, b)][0]
-                rest2 = C * val[slots[third]]
+                rest2 = self._others(val, slots, (a, b), C.astype(np.complex128))
                 for rows_a, da in ((row_r, d_r), (row_i, d_i)):
                     for rows_b, db in ((row_r, d_r), (row_i, d_i)):
                         ra, rb = rows_a[f], rows_b[g]
@@ -534,8 +547,8 @@ class AmplitudeModel:
             fx = self._factor_index()
             bc = self.barrier_constants()
             names = self._names()
-            dlog = np.array([bc[ch]["dlog1"][a] + bc[ch]["dlog2"][b]
-                             for ch, a, b in zip(fx["ch"], fx["s1"], fx["s2"])])
+            dlog = np.array([sum(bc[ch]["dlog"][v][l] for v, l in enumerate(ls))
+                             for ch, ls in zip(fx["ch"], fx["ls"])])
             mrow = np.array([index.get(nm["mass"], -1) for nm in names])[fx["ch"]]
             wrow = np.array([index.get(nm["width"], -1) for nm in names])[fx["ch"]]
             self._sg = (dlog, mrow, wrow)

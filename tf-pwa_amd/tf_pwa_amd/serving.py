@@ -138,16 +138,19 @@ def coupling_map(amp):
     fx = amp._factor_index()
     index = {n: i for i, n in enumerate(vm.trainable_vars)}
     names = fx["cnames"]
+    slots = list(fx["slots"])
+    unit = len(names)  # a fixed factor 1 + 0i fills the unused slots of three-body waves
+    while len(slots) < _lib.CPL_SLOTS:
+        slots.append(np.full(amp.P, unit))
     keep = dict(
-        wave_factor=np.ascontiguousarray(np.stack([fx["it"], fx["i1"], fx["i2"]], 1),
-                                         dtype=np.int32),
+        wave_factor=np.ascontiguousarray(np.stack(slots, 1), dtype=np.int32),
         wave_const=np.ascontiguousarray(amp._barrier_per_wave(), dtype=np.float64),
-        factor_polar=np.array([int(bool(vm.complex_vars.get(n, vm.polar))) for n in names],
+        factor_polar=np.array([int(bool(vm.complex_vars.get(n, vm.polar))) for n in names] + [0],
                               dtype=np.int32),
-        factor_var=np.array([[index.get(n + "r", -1), index.get(n + "i", -1)] for n in names],
-                            dtype=np.int32),
-        factor_fixed=np.array([[vm.variables[n + "r"], vm.variables[n + "i"]] for n in names],
-                              dtype=np.float64),
+        factor_var=np.array([[index.get(n + "r", -1), index.get(n + "i", -1)] for n in names]
+                            + [[-1, -1]], dtype=np.int32),
+        factor_fixed=np.array([[vm.variables[n + "r"], vm.variables[n + "i"]] for n in names]
+                              + [[1.0, 0.0]], dtype=np.float64),
     )
     kind, ab = [], []
     for n in vm.trainable_vars:
@@ -168,7 +171,7 @@ def coupling_map(amp):
     keep["bound_ab"] = np.array(ab, dtype=np.float64).reshape(-1, 2)
     i32p = ctypes.POINTER(ctypes.c_int32)
     m = _lib.TpaCouplingMap(
-        P=amp.P, n_factor=len(names), n_var=len(vm.trainable_vars), reserved=0,
+        P=amp.P, n_factor=len(names) + 1, n_var=len(vm.trainable_vars), reserved=0,
         wave_factor=keep["wave_factor"].ctypes.data_as(i32p),
         wave_const=_lib.dptr(keep["wave_const"]),
         factor_polar=keep["factor_polar"].ctypes.data_as(i32p),
@@ -188,7 +191,10 @@ def map_coefficients(keep, y):
     i = np.where(fv[:, 1] >= 0, y[np.maximum(fv[:, 1], 0)], fixed[:, 1])
     v = np.where(keep["factor_polar"] == 1, r * np.exp(1j * i), r + 1j * i)
     wf = keep["wave_factor"]
-    return keep["wave_const"] * v[wf[:, 0]] * v[wf[:, 1]] * v[wf[:, 2]]
+    c = keep["wave_const"].astype(np.complex128)
+    for a in range(wf.shape[1]):
+        c = c * v[wf[:, a]]
+    return c
 
 
 class FitServer:
