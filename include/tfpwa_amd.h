@@ -93,6 +93,16 @@ int tpa_events_build(tpa_events_t ev, int fold);
  * their next evaluation. */
 int tpa_model_update(tpa_model_t model, const void* desc, int64_t desc_bytes);
 
+/* Four-body final states of spin-0 particles (desc: TpaModel4, tpa_model.h): trees of three
+ * two-body vertices, quasi-two-body (A -> R1 R2) or sequential (A -> R1 f, R1 -> R2 f). The
+ * handle works with every event-set, evaluation, Gram, second-moment and pool call; the
+ * partial-wave cache always folds the lineshapes (masses and widths fixed). Replaces
+ * DecayGroup/DecayChain.get_amp with cal_angle_from_momentum for such decay groups
+ * (tf_pwa/amp/core.py:1392-1445, 1696-1728; tf_pwa/cal_angle.py:186-343, 762-865). */
+int tpa_model4_create(const void* desc, int64_t desc_bytes, const double* cg, int n_cg,
+                      tpa_model_t* out);
+int tpa_model4_update(tpa_model_t model, const void* desc, int64_t desc_bytes);
+
 /* Copy the partial-wave cache to the host: out [P][n][lam_pad] complex (interleaved). */
 int tpa_events_get_pw(tpa_events_t ev, double* out, int64_t out_len);
 

@@ -428,12 +428,19 @@ class Particle:
 
 
 class Decay:
-    def __init__(self, core, outs, p_break=False, **kw):
+    def __init__(self, core, outs, p_break=False, l_list=None, ls_list=None, **kw):
+        """LS couplings as HelicityDecay.get_ls_list (amp/core.py:1212-1233): ``ls_list``
+        replaces the allowed list, ``l_list`` keeps the couplings with those l."""
         self.core = core
         self.outs = tuple(outs)
         self.p_break = p_break
-        self.ls = ls_list(core.J, outs[0].J, outs[1].J, core.P, outs[0].P, outs[1].P,
-                          p_break=p_break)
+        if ls_list is not None:
+            self.ls = [(int(l), _num(sv)) for l, sv in ls_list]
+        else:
+            self.ls = globals()["ls_list"](core.J, outs[0].J, outs[1].J, core.P, outs[0].P,
+                                           outs[1].P, p_break=p_break)
+            if l_list is not None:
+                self.ls = [(l, sv) for l, sv in self.ls if l in set(int(x) for x in l_list)]
         self.d = 3.0
 
     def __repr__(self):

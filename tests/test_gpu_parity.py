@@ -283,3 +283,69 @@ def test_fit_fractions_from_the_gram_matrix(case2, gpu_available):
     frac32, _ = cal_fitfractions(amp, gram=mc.gram())
     for key, val in frac.items():
         assert abs(frac32[key] - val) < 1e-5
+
+
+@pytest.fixture(scope="module")
+def case4():
+    """J/psi -> K+ K- pi+ pi- (12 chains, 6 topologies): phase-space MC and weighted
+    phase-space "data" (the likelihood does not assume the data follow the model)."""
+    from conftest import config_path
+    from oracle import tfpwa_oracle as O
+    from tf_pwa_amd import ConfigLoader
+    from tf_pwa_amd.phasespace import generate_phsp
+
+    path = config_path("jpsi_kkpipi_12ch.yml")
+    om = O.Model(path)
+    om.set_params(O.random_couplings(om, 17))
+    conf = ConfigLoader(path, seed=17)
+    conf.set_params(om.values)
+    order = om.config["data"]["dat_order"]
+    masses = [om.particles[n].mass for n in order]
+    mc = dict(zip(order, generate_phsp(om.top.mass, masses, 4000, np.random.default_rng(1))))
+    data = dict(zip(order, generate_phsp(om.top.mass, masses, 700, np.random.default_rng(2))))
+    wd = np.random.default_rng(3).uniform(0.5, 1.5, 700)
+    return dict(om=om, conf=conf, mc=mc, data=data, wd=wd, O=O)
+
+
+def test_four_body_angles_match_oracle(case4, gpu_available):
+    conf, O, om = case4["conf"], case4["O"], case4["om"]
+    amp = conf.get_amplitude()
+    ev = conf.data_from_p4(case4["mc"])
+    got = ev.angles()
+    ang = O.cal_angle(om, case4["mc"])
+    names = [p.name for p in amp.finals]
+
+    def key(mask):
+        return frozenset(n for i, n in enumerate(names) if mask >> i & 1)
+
+    for t, (T, rep) in enumerate(zip(amp.topo, amp.reps)):
+        a = ang["topo"][rep.topology_key()]["ang"]
+        for v in range(3):
+            b = 15 * t + 5 * v
+            np.testing.assert_allclose(got[b], ang["mass"][key(T["core"][v])].numpy(), rtol=1e-12)
+            for o in range(2):
+                ref = a[key(T["core"][v])][key(T["out"][v][o])]
+                al, be = got[b + 1 + 2 * o], got[b + 2 + 2 * o]
+                assert np.abs(np.exp(0.5j * al) - np.exp(0.5j * ref["alpha"].numpy())).max() < 1e-9
+                np.testing.assert_allclose(be, ref["beta"].numpy(), atol=1e-9)
+
+
+def test_four_body_amplitude_nll_gradient_hessian_match_oracle(case4, gpu_available):
+    from tf_pwa_amd import FCN
+
+    conf = case4["conf"]
+    amp = conf.get_amplitude()
+    mc = conf.data_from_p4(case4["mc"])
+    ref_f, _ = oracle_f(case4, "mc")
+    np.testing.assert_allclose(amp(mc), ref_f, rtol=1e-9, atol=1e-12 * ref_f.max())
+    data = conf.data_from_p4(case4["data"], weight=case4["wd"])
+    fcn = FCN(conf.get_model(), data, mc)
+    nll, g = fcn.nll_grad()
+    ref_nll, ref_g, ref_H = oracle_nll_grad(case4, hessian=True)
+    assert abs(nll - ref_nll) < 1e-6, (nll, ref_nll)
+    assert np.linalg.norm(g - ref_g) <= 1e-7 * np.linalg.norm(ref_g)
+    _, _, H = fcn.nll_grad_hessian()
+    assert np.abs(H - ref_H).max() <= 1e-7 * np.abs(ref_H).max()
+    mix = FCN(conf.get_model(), data, mc, precision="mixed")
+    n2, g2 = mix.nll_grad()
+    assert abs(n2 - nll) < 1e-3 and np.linalg.norm(g2 - g) < 1e-4 * np.linalg.norm(g)

@@ -317,3 +317,49 @@ def test_second_moment_decode_layout():
     np.testing.assert_allclose(Dcc, np.einsum("e,ek,el->kl", v, q, q), rtol=1e-12, atol=1e-12)
     ref = np.einsum("e,ek,el->kl", v, q, q.conj()) + np.einsum("e,elk,elm->km", u, h, h.conj())
     np.testing.assert_allclose(Dcb, ref, rtol=1e-12, atol=1e-12)
+
+
+def test_genbod_against_the_three_body_generator():
+    """The n-body GENBOD generator, run for three bodies, against the independent m23-weight
+    three-body generator: the same m23^2 distribution (two-sample KS), exact four-momentum
+    conservation and masses; and for four bodies conservation, masses and kinematic limits."""
+    from scipy.stats import ks_2samp
+
+    from tf_pwa_amd.phasespace import _genbod, generate_phsp
+
+    m0, ms = 2.286, [0.938, 0.494, 0.140]
+    a = _genbod(m0, ms, 40000, np.random.default_rng(1))
+    b = generate_phsp(m0, ms, 40000, np.random.default_rng(2))
+
+    def m2(p, q):
+        s = p + q
+        return s[:, 0] ** 2 - (s[:, 1:] ** 2).sum(1)
+
+    assert ks_2samp(m2(a[1], a[2]), m2(b[1], b[2])).pvalue > 1e-3
+    assert ks_2samp(m2(a[0], a[1]), m2(b[0], b[1])).pvalue > 1e-3
+    ms4 = [0.493677, 0.493677, 0.13957, 0.13957]
+    p = generate_phsp(3.0969, ms4, 20000, np.random.default_rng(3))
+    tot = sum(p)
+    np.testing.assert_allclose(tot[:, 0], 3.0969, atol=1e-12)
+    np.testing.assert_allclose(tot[:, 1:], 0.0, atol=1e-12)
+    for x, m in zip(p, ms4):
+        np.testing.assert_allclose(np.sqrt(np.abs(x[:, 0] ** 2 - (x[:, 1:] ** 2).sum(1))), m,
+                                   atol=1e-12)
+    m01 = np.sqrt(m2(p[0], p[1]))
+    assert m01.min() > ms4[0] + ms4[1] - 1e-12 and m01.max() < 3.0969 - ms4[2] - ms4[3] + 1e-12
+
+
+def test_four_body_layout_matches_oracle():
+    from tf_pwa_amd import ConfigLoader
+    from tf_pwa_amd.serving import coupling_map, map_coefficients
+
+    c = ConfigLoader(config_path("jpsi_kkpipi_12ch.yml"), seed=0)
+    m = O.Model(config_path("jpsi_kkpipi_12ch.yml"))
+    assert c.vm.trainable_vars == m.trainable
+    amp = c.get_amplitude()
+    assert (amp.n_final, amp.P, amp.lam, len(amp.chains), len(amp.topo)) == (4, 25, 2, 12, 6)
+    _, keep = coupling_map(amp)
+    y = np.array(c.vm.get_all_val())
+    np.testing.assert_allclose(map_coefficients(keep, y), amp.coefficients(), rtol=1e-14)
+    M = amp.describe()
+    assert M.P == 25 and M.n_chain == 12

@@ -233,3 +233,40 @@ def test_oracle_reproduces_committed_golden_vectors(name):
     assert abs(out["nll"] - ref["nll"]) < 1e-12 * abs(ref["nll"])
     np.testing.assert_allclose(out["grad"], ref["grad"], rtol=1e-10, atol=1e-12)
     np.testing.assert_allclose(out["hessian"], ref["hessian"], rtol=1e-9, atol=1e-10)
+
+
+def test_oracle_four_body_rotation_invariance():
+    """J/psi -> K+ K- pi+ pi- through 12 chains in 6 topologies (quasi-two-body and sequential):
+    with every top helicity summed, |A|^2 must not change under a rotation of whole events,
+    which ties the frame conventions of all topologies together; with helicities +-1 only it is
+    invariant under rotations about the z axis alone."""
+    import yaml
+    from scipy.spatial.transform import Rotation
+
+    from tf_pwa_amd.phasespace import generate_phsp
+
+    cfg = yaml.safe_load(open(config_path("jpsi_kkpipi_12ch.yml")))
+    for full in (True, False):
+        c = dict(cfg)
+        c["particle"] = {k: (dict(v) if isinstance(v, dict) else v) for k, v in cfg["particle"].items()}
+        if full:
+            c["particle"]["$top"] = {"Jpsi": {k: v for k, v in cfg["particle"]["$top"]["Jpsi"].items()
+                                              if k != "spins"}}
+        om = O.Model(c)
+        om.set_params(O.random_couplings(om, 5))
+        order = om.config["data"]["dat_order"]
+        ps = generate_phsp(om.top.mass, [om.particles[n].mass for n in order], 400,
+                           np.random.default_rng(3))
+        p4 = dict(zip(order, ps))
+        f = O.sum_amp(om, O.cal_angle(om, p4), om.var_tensors()).numpy()
+        phi = 0.9
+        Rz = np.array([[np.cos(phi), -np.sin(phi), 0], [np.sin(phi), np.cos(phi), 0], [0, 0, 1]])
+        R = Rotation.random(random_state=2).as_matrix() if full else Rz
+        p4r = {k: np.concatenate([v[:, :1], v[:, 1:] @ R.T], 1) for k, v in p4.items()}
+        fr = O.sum_amp(om, O.cal_angle(om, p4r), om.var_tensors()).numpy()
+        assert np.abs(fr - f).max() < 1e-10 * f.max()
+        if not full:
+            Rx = Rotation.from_euler("x", 0.7).as_matrix()
+            p4x = {k: np.concatenate([v[:, :1], v[:, 1:] @ Rx.T], 1) for k, v in p4.items()}
+            fx = O.sum_amp(om, O.cal_angle(om, p4x), om.var_tensors()).numpy()
+            assert np.abs(fx - f).max() > 1e-3 * f.max()  # helicity +-1 only: not invariant

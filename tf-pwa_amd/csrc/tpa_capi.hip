@@ -15,6 +15,7 @@
 #include "tfpwa_amd.h"
 #include "tpa_internal.h"
 #include "tpa_kernels.h"
+#include "tpa_kernels4.h"
 
 namespace {
 
@@ -124,12 +125,76 @@ std::string check_model(const TpaModel& M, int n_cg) {
   return "";
 }
 
+std::string check_model4(const TpaModel4& M, int n_cg) {
+  if (M.n_final != 4) return "four-body description needs n_final = 4";
+  if (M.n_topo < 1 || M.n_topo > TPA4_MAX_TOPO) return "n_topo out of range";
+  if (M.n_chain < 1 || M.n_chain > TPA4_MAX_CHAIN) return "n_chain out of range";
+  if (M.P < 1 || M.P > TPA_MAX_P) return "P out of range";
+  if (M.top_j2 < 0 || M.top_j2 > TPA_MAX_J2) return "top spin out of range";
+  if (M.n_top < 1 || M.n_top > TPA_MAX_HEL) return "top helicities out of range";
+  if (M.lam != M.n_top) return "four-body finals must be spin 0 (lam = top helicities)";
+  if (M.lam_pad < M.lam || M.lam_pad > 16 || (M.lam_pad & (M.lam_pad - 1))) return "bad lam_pad";
+  if (M.n_cg != n_cg) return "n_cg mismatch";
+  for (int i = 0; i < M.n_top; ++i)
+    if (std::abs(M.hel_top[i]) > M.top_j2 || ((M.hel_top[i] - M.top_j2) & 1))
+      return "bad top helicity";
+  for (int t = 0; t < M.n_topo; ++t) {
+    const Tpa4Topo& T = M.topo[t];
+    if (T.core[0] != 15 || T.parent[0] != -1) return "vertex 0 must be the top decay";
+    for (int v = 0; v < 3; ++v) {
+      if ((T.out[v][0] | T.out[v][1]) != T.core[v] || (T.out[v][0] & T.out[v][1]) ||
+          !T.out[v][0] || !T.out[v][1])
+        return "vertex outs must split the core's finals";
+      if (v > 0) {
+        if (T.parent[v] < 0 || T.parent[v] >= v || T.parent_out[v] < 0 || T.parent_out[v] > 1)
+          return "bad vertex parent";
+        if (T.out[T.parent[v]][T.parent_out[v]] != T.core[v]) return "parent out is not the core";
+      }
+    }
+  }
+  int covered = 0;
+  for (int c = 0; c < M.n_chain; ++c) {
+    const Tpa4Chain& C = M.chain[c];
+    if (C.topo < 0 || C.topo >= M.n_topo) return "chain topology out of range";
+    int np = 1;
+    for (int v = 0; v < 3; ++v) {
+      if (C.n_ls[v] < 1 || C.n_ls[v] > TPA_MAX_LS) return "n_ls out of range";
+      for (int s = 0; s < C.n_ls[v]; ++s)
+        if (C.l[v][s] < 0 || C.l[v][s] > 5) return "l out of range";
+      np *= C.n_ls[v];
+      for (int o = 0; o < 2; ++o)
+        if (C.out_slot[v][o] < 1 || C.out_slot[v][o] > 6 || C.out_slot[v][o] == v)
+          return "bad out slot";
+    }
+    for (int r = 0; r < 2; ++r) {
+      if (C.r_j2[r] < 0 || C.r_j2[r] > TPA_MAX_J2) return "resonance spin out of range";
+      if (C.r_n[r] < 1 || C.r_n[r] > TPA_MAX_HEL) return "resonance helicities out of range";
+      for (int i = 0; i < C.r_n[r]; ++i)
+        if (std::abs(C.r_hel[r][i]) > C.r_j2[r] || ((C.r_hel[r][i] - C.r_j2[r]) & 1))
+          return "bad resonance helicity";
+      if (C.r_shape[r] < 0 || C.r_shape[r] > 2) return "bad lineshape";
+    }
+    auto nh = [&](int slot) { return slot == 0 ? M.n_top : slot <= 2 ? C.r_n[slot - 1] : 1; };
+    for (int v = 0; v < 3; ++v) {
+      const int sz = C.n_ls[v] * nh(C.out_slot[v][0]) * nh(C.out_slot[v][1]);
+      if (C.cg_off[v] < 0 || C.cg_off[v] + sz > n_cg) return "CG offset out of range";
+    }
+    if (C.k0 != covered) return "chains must cover the partial waves in order";
+    covered += np;
+  }
+  if (covered != M.P) return "partial-wave count mismatch";
+  return "";
+}
+
 }  // namespace
 
 struct tpa_model_s {
   int device = 0;
-  TpaModel h{};
+  int kind = 3;        // 3: three-body chains (TpaModel); 4: four-body trees (TpaModel4)
+  TpaModel h{};        // for kind 4 only the header fields (sizes, helicities) are set
   TpaModel* d = nullptr;
+  TpaModel4 h4{};
+  TpaModel4* d4 = nullptr;
   double* d_cg = nullptr;
   double* d_dw = nullptr;
   int* d_kchain = nullptr;
@@ -355,6 +420,33 @@ static int run_hess(const TpaModel& M, const double2* h, const double* w, int64_
 
 int tpa_set_error(int code, const std::string& msg) { return fail(code, msg); }
 
+// Per-event angle-cache fields and four-momentum rows of a model's final state.
+static int angle_fields(const tpa_model_s* m) {
+  return m->kind == 4 ? TPA4_NF_TOPO * m->h.n_topo + 5 : TPA_NF_TOPO * m->h.n_topo + 4;
+}
+static int p4_rows(const tpa_model_s* m) { return 4 * (m->kind == 4 ? 4 : 3); }
+
+// Angles of n events (p4 [rows][n] on the device) into columns off.. of a cache of stride S.
+static void launch_cal_angle(const tpa_model_s* m, const double* d_p4, int64_t n, int64_t off,
+                             int64_t S, double* ang, hipStream_t s) {
+  const dim3 grid((unsigned)((n + 255) / 256));
+  if (m->kind == 4)
+    hipLaunchKernelGGL(k_cal_angle4, grid, dim3(256), 0, s, m->d4, d_p4, n, off, S, ang);
+  else
+    hipLaunchKernelGGL(k_cal_angle3, grid, dim3(256), 0, s, m->d, d_p4, n, off, S, ang);
+}
+
+static void launch_build_pw(const tpa_model_s* m, const double* ang, int64_t n, int64_t off,
+                            int64_t S, int fold, double2* h, hipStream_t s) {
+  const dim3 grid((unsigned)((n + 127) / 128));
+  if (m->kind == 4)
+    hipLaunchKernelGGL(k_build_pw4, grid, dim3(128), 0, s, m->d4, ang, n, off, S, m->d_cg,
+                       m->d_dw, fold, h);
+  else
+    hipLaunchKernelGGL(k_build_pw, grid, dim3(128), 0, s, m->d, ang, n, off, S, m->d_cg,
+                       m->d_dw, fold, h);
+}
+
 template <int LG, int KG>
 static void launch_multi(const MultiArgs& a, int grid, hipStream_t s) {
   hipLaunchKernelGGL((k_eval_multi<LG, KG, 8>), dim3(grid), dim3(256), 0, s, a);
@@ -440,9 +532,73 @@ int tpa_model_create(const void* desc, int64_t desc_bytes, const double* cg, int
   return TPA_OK;
 }
 
+int tpa_model4_create(const void* desc, int64_t desc_bytes, const double* cg, int n_cg,
+                      tpa_model_t* out) {
+  if (!desc || !out || (n_cg > 0 && !cg)) return fail(TPA_ERR_ARG, "null pointer");
+  if (desc_bytes != (int64_t)sizeof(TpaModel4))
+    return fail(TPA_ERR_ARG, "model description size mismatch: got " +
+                                 std::to_string(desc_bytes) + " expected " +
+                                 std::to_string(sizeof(TpaModel4)));
+  tpa_model_s* m = new tpa_model_s();
+  m->kind = 4;
+  std::memcpy(&m->h4, desc, sizeof(TpaModel4));
+  std::string why = check_model4(m->h4, n_cg);
+  if (!why.empty()) {
+    delete m;
+    return fail(TPA_ERR_ARG, "invalid model: " + why);
+  }
+  // the header the generic passes (evaluation, Gram, second moments, pool) read
+  m->h.n_final = 4;
+  m->h.n_topo = m->h4.n_topo;
+  m->h.n_chain = m->h4.n_chain;
+  m->h.P = m->h4.P;
+  m->h.lam = m->h4.lam;
+  m->h.lam_pad = m->h4.lam_pad;
+  m->h.top_j2 = m->h4.top_j2;
+  m->h.n_top = m->h4.n_top;
+  for (int i = 0; i < TPA_MAX_HEL; ++i) m->h.hel_top[i] = m->h4.hel_top[i];
+  m->h.random_z = m->h4.random_z;
+  m->h.n_cg = n_cg;
+  hipError_t e;
+  std::vector<double> dw = wigner_weights();
+  if ((e = hipGetDevice(&m->device)) != hipSuccess ||
+      (e = hipStreamCreateWithFlags(&m->stream, hipStreamNonBlocking)) != hipSuccess ||
+      (e = hipMalloc(&m->d4, sizeof(TpaModel4))) != hipSuccess ||
+      (e = hipMalloc(&m->d_cg, sizeof(double) * std::max(1, n_cg))) != hipSuccess ||
+      (e = hipMalloc(&m->d_dw, sizeof(double) * dw.size())) != hipSuccess ||
+      (e = hipMemcpy(m->d4, &m->h4, sizeof(TpaModel4), hipMemcpyHostToDevice)) != hipSuccess ||
+      (n_cg > 0 &&
+       (e = hipMemcpy(m->d_cg, cg, sizeof(double) * n_cg, hipMemcpyHostToDevice)) != hipSuccess) ||
+      (e = hipMemcpy(m->d_dw, dw.data(), sizeof(double) * dw.size(), hipMemcpyHostToDevice)) !=
+          hipSuccess) {
+    tpa_model_destroy(m);
+    return fail(TPA_ERR_HIP, std::string("model upload: ") + hipGetErrorString(e));
+  }
+  *out = m;
+  return TPA_OK;
+}
+
+int tpa_model4_update(tpa_model_t m, const void* desc, int64_t desc_bytes) {
+  if (!m || !desc) return fail(TPA_ERR_ARG, "null pointer");
+  if (m->kind != 4) return fail(TPA_ERR_ARG, "not a four-body model");
+  if (desc_bytes != (int64_t)sizeof(TpaModel4))
+    return fail(TPA_ERR_ARG, "model description size mismatch");
+  TpaModel4 next;
+  std::memcpy(&next, desc, sizeof(TpaModel4));
+  if (next.P != m->h4.P || next.n_cg != m->h4.n_cg || next.lam_pad != m->h4.lam_pad)
+    return fail(TPA_ERR_ARG, "partial-wave layout cannot change");
+  std::string why = check_model4(next, next.n_cg);
+  if (!why.empty()) return fail(TPA_ERR_ARG, "invalid model: " + why);
+  HIPCHK(hipStreamSynchronize(m->stream));
+  m->h4 = next;
+  HIPCHK(hipMemcpy(m->d4, &m->h4, sizeof(TpaModel4), hipMemcpyHostToDevice));
+  return TPA_OK;
+}
+
 int tpa_model_destroy(tpa_model_t m) {
   if (!m) return TPA_OK;
   if (m->d) (void)hipFree(m->d);
+  if (m->d4) (void)hipFree(m->d4);
   if (m->d_cg) (void)hipFree(m->d_cg);
   if (m->d_dw) (void)hipFree(m->d_dw);
   if (m->d_kchain) (void)hipFree(m->d_kchain);
@@ -453,6 +609,7 @@ int tpa_model_destroy(tpa_model_t m) {
 
 int tpa_model_update(tpa_model_t m, const void* desc, int64_t desc_bytes) {
   if (!m || !desc) return fail(TPA_ERR_ARG, "null pointer");
+  if (m->kind != 3) return fail(TPA_ERR_ARG, "use tpa_model4_update for a four-body model");
   if (desc_bytes != (int64_t)sizeof(TpaModel))
     return fail(TPA_ERR_ARG, "model description size mismatch");
   TpaModel next;
@@ -494,7 +651,8 @@ int tpa_events_create(tpa_model_t m, const double* p4, int64_t n, const double* 
   tpa_events_s* ev = new tpa_events_s();
   ev->m = m;
   ev->n = n;
-  ev->nf = TPA_NF_TOPO * m->h.n_topo + 4;
+  ev->nf = angle_fields(m);
+  const int rows = p4_rows(m);
   const int P = m->h.P;
   double* d_p4 = nullptr;
   hipError_t e;
@@ -503,7 +661,7 @@ int tpa_events_create(tpa_model_t m, const double* p4, int64_t n, const double* 
     tpa_events_destroy(ev);
     return fail(TPA_ERR_HIP, std::string(what) + ": " + hipGetErrorString(er));
   };
-  if ((e = hipMalloc(&d_p4, sizeof(double) * 12 * n)) != hipSuccess) return bail(e, "alloc p4");
+  if ((e = hipMalloc(&d_p4, sizeof(double) * rows * n)) != hipSuccess) return bail(e, "alloc p4");
   if ((e = hipMalloc(&ev->d_ang, sizeof(double) * ev->nf * n)) != hipSuccess)
     return bail(e, "alloc angles");
   if ((e = hipMalloc(&ev->d_w, sizeof(double) * n)) != hipSuccess) return bail(e, "alloc w");
@@ -516,7 +674,7 @@ int tpa_events_create(tpa_model_t m, const double* p4, int64_t n, const double* 
     return bail(e, "alloc coef");
   if ((e = hipEventCreate(&ev->t0)) != hipSuccess || (e = hipEventCreate(&ev->t1)) != hipSuccess)
     return bail(e, "event create");
-  if ((e = hipMemcpyAsync(d_p4, p4, sizeof(double) * 12 * n, hipMemcpyHostToDevice, m->stream)) !=
+  if ((e = hipMemcpyAsync(d_p4, p4, sizeof(double) * rows * n, hipMemcpyHostToDevice, m->stream)) !=
       hipSuccess)
     return bail(e, "upload p4");
   if (weight) {
@@ -532,8 +690,9 @@ int tpa_events_create(tpa_model_t m, const double* p4, int64_t n, const double* 
   }
   const int block = 256;
   const int64_t grid = (n + block - 1) / block;
-  hipLaunchKernelGGL(k_cal_angle3, dim3((unsigned)grid), dim3(block), 0, m->stream, m->d, d_p4,
-                     n, (int64_t)0, n, ev->d_ang);
+  (void)block;
+  (void)grid;
+  launch_cal_angle(m, d_p4, n, 0, n, ev->d_ang, m->stream);
   if ((e = hipGetLastError()) != hipSuccess) return bail(e, "launch cal_angle");
   if ((e = hipStreamSynchronize(m->stream)) != hipSuccess) return bail(e, "cal_angle");
   (void)hipFree(d_p4);
@@ -565,11 +724,9 @@ int tpa_events_build(tpa_events_t ev, int fold) {
   if (!ev) return fail(TPA_ERR_ARG, "null pointer");
   const TpaModel& M = ev->m->h;
   if (!ev->d_h) HIPCHK(hipMalloc(&ev->d_h, sizeof(double2) * (size_t)M.P * ev->n * M.lam_pad));
-  const int block = 128;
-  const int64_t grid = (ev->n + block - 1) / block;
-  hipLaunchKernelGGL(k_build_pw, dim3((unsigned)grid), dim3(block), 0, ev->m->stream, ev->m->d,
-                     ev->d_ang, ev->n, (int64_t)0, ev->n, ev->m->d_cg, ev->m->d_dw, fold,
-                     ev->d_h);
+  if (ev->m->kind == 4 && !fold)
+    return fail(TPA_ERR_UNSUPPORTED, "four-body models fold fixed lineshapes into the cache");
+  launch_build_pw(ev->m, ev->d_ang, ev->n, 0, ev->n, fold, ev->d_h, ev->m->stream);
   HIPCHK(hipGetLastError());
   HIPCHK(hipStreamSynchronize(ev->m->stream));
   ev->built_fold = fold ? 1 : 0;
@@ -610,6 +767,8 @@ int tpa_eval(tpa_events_t ev, const double* coef, int mode, int flags, double* o
   a.mode = mode;
   a.grad = (flags & TPA_FLAG_GRAD) ? 1 : 0;
   const bool shape = (flags & TPA_FLAG_SHAPE) != 0;
+  if (shape && ev->m->kind == 4)
+    return fail(TPA_ERR_UNSUPPORTED, "floating lineshapes need a three-body model");
   if (shape && ev->built_fold != 0)
     return fail(TPA_ERR_ARG, "shape evaluation needs a cache built without lineshapes (fold=0)");
   if (!shape && ev->built_fold != 1)
@@ -746,7 +905,7 @@ int tpa_pool_create(tpa_model_t m, int64_t capacity, int max_items, tpa_pool_t* 
   p->max_items = max_items;
   p->max_chunks = (int)std::min<int64_t>((capacity + TPA_CHUNK - 1) / TPA_CHUNK + max_items,
                                          1 << 22);
-  const int nf = TPA_NF_TOPO * M.n_topo + 4;
+  const int nf = angle_fields(m);
   const int nacc = 2 + 2 * M.P;
   hipError_t e = hipSuccess;
   auto D = [&](void** ptr, size_t bytes) {
@@ -825,26 +984,25 @@ int tpa_pool_add(tpa_pool_t p, const double* p4, int64_t n, const double* weight
     p->d_stage = nullptr;
     p->h_stage = nullptr;
     p->stage_n = 0;
-    if ((e = hipMalloc(&p->d_stage, sizeof(double) * 13 * want)) != hipSuccess)
+    if ((e = hipMalloc(&p->d_stage, sizeof(double) * 17 * want)) != hipSuccess)
       return undo(e, "staging");
-    if ((e = hipHostMalloc(&p->h_stage, sizeof(double) * 13 * want, 0)) != hipSuccess)
+    if ((e = hipHostMalloc(&p->h_stage, sizeof(double) * 17 * want, 0)) != hipSuccess)
       return undo(e, "pinned staging");
     p->stage_n = want;
   }
-  std::memcpy(p->h_stage, p4, sizeof(double) * 12 * n);
-  std::memcpy(p->h_stage + 12 * n, weight, sizeof(double) * n);
   tpa_model_s* m = p->m;
+  const int rows = p4_rows(m);
+  std::memcpy(p->h_stage, p4, sizeof(double) * rows * n);
+  std::memcpy(p->h_stage + rows * n, weight, sizeof(double) * n);
   hipStream_t s = p->add_stream;
-  if ((e = hipMemcpyAsync(p->d_stage, p->h_stage, sizeof(double) * 13 * n, hipMemcpyHostToDevice,
-                          s)) != hipSuccess)
+  if ((e = hipMemcpyAsync(p->d_stage, p->h_stage, sizeof(double) * (rows + 1) * n,
+                          hipMemcpyHostToDevice, s)) != hipSuccess)
     return undo(e, "upload");
-  if ((e = hipMemcpyAsync(p->d_w + off, p->d_stage + 12 * n, sizeof(double) * n,
+  if ((e = hipMemcpyAsync(p->d_w + off, p->d_stage + rows * n, sizeof(double) * n,
                           hipMemcpyDeviceToDevice, s)) != hipSuccess)
     return undo(e, "weights");
-  hipLaunchKernelGGL(k_cal_angle3, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, m->d,
-                     p->d_stage, n, off, p->cap, p->d_ang);
-  hipLaunchKernelGGL(k_build_pw, dim3((unsigned)((n + 127) / 128)), dim3(128), 0, s, m->d,
-                     p->d_ang, n, off, p->cap, m->d_cg, m->d_dw, 1, p->d_h);
+  launch_cal_angle(m, p->d_stage, n, off, p->cap, p->d_ang, s);
+  launch_build_pw(m, p->d_ang, n, off, p->cap, 1, p->d_h, s);
   if ((e = hipGetLastError()) != hipSuccess) return undo(e, "launch");
   if ((e = hipStreamSynchronize(s)) != hipSuccess) return undo(e, "build");
   *offset = off;

@@ -70,3 +70,55 @@ typedef struct TpaModel {
   TpaTopo topo[TPA_MAX_TOPO];
   TpaChain chain[TPA_MAX_CHAIN];
 } TpaModel;
+
+// ---------------------------------------------------------------- four-body final states
+// A decay group of four spin-0 finals is lowered to trees of three two-body vertices:
+// vertex 0 is the top decay, vertices 1 and 2 the decays of the two resonances. Quasi-two-body
+// trees (A -> R1 R2, R1 -> f f, R2 -> f f) have both resonance vertices under vertex 0;
+// sequential trees (A -> R1 f, R1 -> R2 f, R2 -> f f) have vertex 2 under vertex 1.
+// Partial waves: k = k0(chain) + (ls0 * n_ls1 + ls1) * n_ls2 + ls2.
+#define TPA4_MAX_FINAL 4
+#define TPA4_MAX_TOPO 8
+#define TPA4_MAX_CHAIN 32
+
+// Angle cache fields per four-body topology: for vertex v of the representative tree,
+// 5v + 0: invariant mass of the core, 5v + 1..2: (alpha, beta) of outs[0] and 5v + 3..4:
+// (alpha, beta) of outs[1] in the core's helicity frame. After all topologies: m_top, then the
+// data mass of every final.
+#define TPA4_NF_TOPO 15
+
+typedef struct Tpa4Topo {
+  int32_t core[3];        // finals below each vertex's core (bit mask; vertex 0: all)
+  int32_t out[3][2];      // finals below each out of the representative decay (bit masks)
+  int32_t parent[3];      // vertex whose out is this vertex's core (-1 for vertex 0)
+  int32_t parent_out[3];  // which out (0 or 1) of the parent vertex
+} Tpa4Topo;
+
+// Node slots of a chain: 0 the top, 1 and 2 the resonances decaying at vertices 1 and 2,
+// 3 + f final f (spin 0, helicity 0).
+typedef struct Tpa4Chain {
+  int32_t topo;
+  int32_t swap[3];             // the chain lists vertex v's outs in the opposite order
+  int32_t out_slot[3][2];      // node slot of each out, in this chain's order
+  int32_t n_ls[3];
+  int32_t l[3][TPA_MAX_LS];
+  int32_t act[3][TPA_MAX_LS];  // 1: barrier uses 1/sqrt(P_l(z)); 0: clamped
+  int32_t cg_off[3];           // CG tables [ls][i_out0][i_out1] in this chain's order
+  int32_t k0;
+  int32_t r_j2[2], r_n[2];     // resonances of slots 1 and 2
+  int32_t r_hel[2][TPA_MAX_HEL];
+  int32_t r_shape[2], r_bw_l[2];
+  double r_m0[2], r_g0[2];
+  double r_q0m[2][2];          // nominal daughter masses of each resonance (q0 of its width)
+  double d;                    // barrier radius parameter
+} Tpa4Chain;
+
+typedef struct TpaModel4 {
+  int32_t n_final, n_topo, n_chain, P, lam, lam_pad;
+  int32_t top_j2, n_top;
+  int32_t hel_top[TPA_MAX_HEL];
+  int32_t random_z;
+  int32_t n_cg;
+  Tpa4Topo topo[TPA4_MAX_TOPO];
+  Tpa4Chain chain[TPA4_MAX_CHAIN];
+} TpaModel4;

@@ -78,6 +78,52 @@ class TpaModel(ctypes.Structure):
     ]
 
 
+MAX4_TOPO = 8
+MAX4_CHAIN = 32
+NF4_TOPO = 15
+
+
+class Tpa4Topo(ctypes.Structure):
+    _fields_ = [
+        ("core", _i32 * 3),
+        ("out", (_i32 * 2) * 3),
+        ("parent", _i32 * 3),
+        ("parent_out", _i32 * 3),
+    ]
+
+
+class Tpa4Chain(ctypes.Structure):
+    _fields_ = [
+        ("topo", _i32),
+        ("swap", _i32 * 3),
+        ("out_slot", (_i32 * 2) * 3),
+        ("n_ls", _i32 * 3),
+        ("l", (_i32 * MAX_LS) * 3),
+        ("act", (_i32 * MAX_LS) * 3),
+        ("cg_off", _i32 * 3),
+        ("k0", _i32),
+        ("r_j2", _i32 * 2), ("r_n", _i32 * 2),
+        ("r_hel", (_i32 * MAX_HEL) * 2),
+        ("r_shape", _i32 * 2), ("r_bw_l", _i32 * 2),
+        ("r_m0", ctypes.c_double * 2), ("r_g0", ctypes.c_double * 2),
+        ("r_q0m", (ctypes.c_double * 2) * 2),
+        ("d", ctypes.c_double),
+    ]
+
+
+class TpaModel4(ctypes.Structure):
+    _fields_ = [
+        ("n_final", _i32), ("n_topo", _i32), ("n_chain", _i32), ("P", _i32), ("lam", _i32),
+        ("lam_pad", _i32),
+        ("top_j2", _i32), ("n_top", _i32),
+        ("hel_top", _i32 * MAX_HEL),
+        ("random_z", _i32),
+        ("n_cg", _i32),
+        ("topo", Tpa4Topo * MAX4_TOPO),
+        ("chain", Tpa4Chain * MAX4_CHAIN),
+    ]
+
+
 class TpaCouplingMap(ctypes.Structure):
     _fields_ = [
         ("P", _i32), ("n_factor", _i32), ("n_var", _i32), ("reserved", _i32),
@@ -116,6 +162,8 @@ def lib():
         "tpa_model_create": ([vp, i64, dp, ctypes.c_int, ctypes.POINTER(vp)], ctypes.c_int),
         "tpa_model_destroy": ([vp], ctypes.c_int),
         "tpa_model_update": ([vp, vp, i64], ctypes.c_int),
+        "tpa_model4_create": ([vp, i64, dp, ctypes.c_int, ctypes.POINTER(vp)], ctypes.c_int),
+        "tpa_model4_update": ([vp, vp, i64], ctypes.c_int),
         "tpa_events_create": ([vp, dp, i64, dp, ctypes.POINTER(vp)], ctypes.c_int),
         "tpa_events_destroy": ([vp], ctypes.c_int),
         "tpa_events_size": ([vp], i64),
@@ -189,7 +237,8 @@ def exported_symbols():
     """Names declared in include/tfpwa_amd.h that the loaded library must export."""
     return [
         "tpa_abi_version", "tpa_last_error", "tpa_device_count", "tpa_model_create",
-        "tpa_model_destroy", "tpa_model_update", "tpa_events_create", "tpa_events_destroy",
+        "tpa_model_destroy", "tpa_model_update", "tpa_model4_create", "tpa_model4_update",
+        "tpa_events_create", "tpa_events_destroy",
         "tpa_events_size", "tpa_events_set_weight", "tpa_events_get_angles",
         "tpa_events_build", "tpa_events_get_pw", "tpa_eval", "tpa_timing_enable",
         "tpa_timing_read", "tpa_timing_reset", "tpa_device_synchronize", "tpa_calib_copy",

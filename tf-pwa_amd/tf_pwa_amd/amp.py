@@ -104,8 +104,11 @@ class AmplitudeModel:
     # ---------------------------------------------------------------- lowering
     def _lower(self):
         g = self.decay_group
-        if len(g.outs) != 3:
-            raise NotImplementedError("the engine supports three-body final states")
+        self.n_final = len(g.outs)
+        if self.n_final == 4:
+            return self._lower4()
+        if self.n_final != 3:
+            raise NotImplementedError("the engine supports three- and four-body final states")
         self.finals = list(g.outs)
         fidx = {p.name: i for i, p in enumerate(self.finals)}
         reps = g.topologies()
@@ -170,8 +173,185 @@ class AmplitudeModel:
             lp <<= 1
         self.lam_pad = lp
 
+    # ---------------------------------------------------------------- four-body lowering
+    def _lower4(self):
+        """Trees of three two-body vertices over four spin-0 finals: vertex 0 the top decay,
+        vertices 1 and 2 the decays of the two resonances, in the representative chain's
+        order (quasi-two-body A -> R1 R2 or sequential A -> R1 f, R1 -> R2 f)."""
+        g = self.decay_group
+        self.finals = list(g.outs)
+        for p in self.finals:
+            if p.J != 0:
+                raise NotImplementedError("four-body final states must be spin 0")
+        fidx = {p.name: i for i, p in enumerate(self.finals)}
+
+        def masks(chain):
+            return {p: sum(1 << fidx[n] for n in names)
+                    for p, names in chain.finals_below().items()}
+
+        reps = g.topologies()
+        self.reps = reps
+        topo = []
+        for rep in reps:
+            if len(rep) != 3:
+                raise NotImplementedError("four-body chains need three two-body vertices")
+            mk = masks(rep)
+            d0 = rep.decay_of(g.top)
+            inner0 = [o for o in d0.outs if o in rep.inner]
+            if len(inner0) == 2:
+                v1, v2 = d0.outs
+                parent, pout = [-1, 0, 0], [0, 0, 1]
+            elif len(inner0) == 1:
+                v1 = inner0[0]
+                d1 = rep.decay_of(v1)
+                inner1 = [o for o in d1.outs if o in rep.inner]
+                if len(inner1) != 1:
+                    raise NotImplementedError("unsupported four-body chain")
+                v2 = inner1[0]
+                parent, pout = [-1, 0, 1], [0, d0.outs.index(v1), d1.outs.index(v2)]
+            else:
+                raise NotImplementedError("unsupported four-body chain")
+            decs = [d0, rep.decay_of(v1), rep.decay_of(v2)]
+            topo.append(dict(core=[mk[d.core] for d in decs],
+                             out=[[mk[o] for o in d.outs] for d in decs],
+                             parent=parent, parent_out=pout))
+        self.topo = topo
+        chains, cg, waves, k0 = [], [], [], 0
+        for ci, c in enumerate(g.chains):
+            t = [i for i, r in enumerate(reps) if r.topology_key() == c.topology_key()][0]
+            T = topo[t]
+            mk = masks(c)
+            by_core = {mk[d.core]: d for d in c.decays}
+            decs = [by_core[m] for m in T["core"]]
+            R = [decs[1].core, decs[2].core]
+
+            def slot(p):
+                if p is R[0]:
+                    return 1
+                if p is R[1]:
+                    return 2
+                return 3 + fidx[p.name]
+
+            cg_off = []
+            for d in decs:
+                cg_off.append(len(cg))
+                cg.extend(d.cg_matrix().ravel().tolist())
+
+            def nominal(p):
+                return p.mass
+
+            res = []
+            for r in range(2):
+                Rr, dr = R[r], decs[1 + r]
+                shape = 0 if Rr.width is None else (1 if Rr.model in ("BW",) else 2)
+                res.append(dict(R=Rr, shape=shape,
+                                bw_l=min(Rr.decays[0].l_list) if Rr.decays else 0,
+                                q0m=[nominal(o) for o in dr.outs]))
+            n_ls = [len(d.ls) for d in decs]
+            chains.append(dict(topo=t, decs=decs, R=R, res=res, k0=k0, cg_off=cg_off,
+                               swap=[int(mk[decs[v].outs[0]] != T["out"][v][0])
+                                     for v in range(3)],
+                               out_slot=[[slot(o) for o in d.outs] for d in decs],
+                               n_ls=n_ls))
+            for s0 in range(n_ls[0]):
+                for s1 in range(n_ls[1]):
+                    for s2 in range(n_ls[2]):
+                        waves.append(PartialWave(ci, s0, s1, s2))
+            k0 += n_ls[0] * n_ls[1] * n_ls[2]
+        self.chains = chains
+        self.waves = waves
+        self.P = k0
+        self.cg = np.array(cg, dtype=np.float64)
+        self.lam = len(g.top.spins)
+        lp = 1
+        while lp < self.lam:
+            lp <<= 1
+        self.lam_pad = lp
+
+    def _describe4(self):
+        M = _lib.TpaModel4()
+        g = self.decay_group
+        M.n_final = 4
+        M.n_topo = len(self.topo)
+        M.n_chain = len(self.chains)
+        M.P = self.P
+        M.lam = self.lam
+        M.lam_pad = self.lam_pad
+        M.top_j2 = int(round(2 * g.top.J))
+        M.n_top = len(g.top.spins)
+        for i, sp in enumerate(g.top.spins):
+            M.hel_top[i] = int(round(2 * sp))
+        M.random_z = int(self.random_z)
+        M.n_cg = len(self.cg)
+        for t, T in enumerate(self.topo):
+            for v in range(3):
+                M.topo[t].core[v] = T["core"][v]
+                M.topo[t].out[v][0], M.topo[t].out[v][1] = T["out"][v]
+                M.topo[t].parent[v] = T["parent"][v]
+                M.topo[t].parent_out[v] = T["parent_out"][v]
+        bc = self.barrier_constants()
+        shapes = self.shape_params()
+        for c, ch in enumerate(self.chains):
+            C = M.chain[c]
+            C.topo = ch["topo"]
+            C.k0 = ch["k0"]
+            for v in range(3):
+                C.swap[v] = ch["swap"][v]
+                C.out_slot[v][0], C.out_slot[v][1] = ch["out_slot"][v]
+                C.n_ls[v] = ch["n_ls"][v]
+                for i, l in enumerate(ch["decs"][v].l_list):
+                    C.l[v][i] = l
+                    C.act[v][i] = bc[c]["act"][v][i]
+                C.cg_off[v] = ch["cg_off"][v]
+            for r, rr in enumerate(ch["res"]):
+                R = rr["R"]
+                C.r_j2[r] = int(round(2 * R.J))
+                C.r_n[r] = len(R.spins)
+                for i, sp in enumerate(R.spins):
+                    C.r_hel[r][i] = int(round(2 * sp))
+                C.r_shape[r] = rr["shape"]
+                C.r_bw_l[r] = rr["bw_l"]
+                C.r_m0[r], C.r_g0[r] = shapes[c][2 * r], shapes[c][2 * r + 1]
+                C.r_q0m[r][0], C.r_q0m[r][1] = rr["q0m"]
+            C.d = ch["decs"][0].d
+        return M
+
+    def _barrier_constants4(self):
+        """Per chain and vertex: sqrt(P_l(z0)) at the nominal masses (resonances at their
+        mass parameters), or 1 with act = 0 when P_l(z0) <= 0."""
+        top = self.decay_group.top
+        out = []
+        for ch, sh in zip(self.chains, self.shape_params()):
+            m0 = {id(ch["R"][0]): sh[0], id(ch["R"][1]): sh[2], id(top): top.mass}
+
+            def mass(p):
+                return m0.get(id(p), p.mass)
+
+            cs, acts = [], []
+            for d in ch["decs"]:
+                q02 = rel_p2(mass(d.core), mass(d.outs[0]), mass(d.outs[1]))
+                c_v, a_v = [], []
+                for l in d.l_list:
+                    p0 = bprime_poly(l, q02 * d.d * d.d)
+                    a_v.append(int(p0 > 0))
+                    c_v.append(np.sqrt(p0) if p0 > 0 else 1.0)
+                cs.append(np.array(c_v))
+                acts.append(a_v)
+            out.append(dict(c=cs, act=acts, dlog=[np.zeros(len(x)) for x in cs]))
+        return out
+
     def _names(self):
         """Variable names of every chain: total, the g_ls of both vertices, mass and width."""
+        if self.n_final == 4:
+            out = []
+            for c, ch in zip(self.decay_group.chains, self.chains):
+                g = [[d.params_head() + "_g_ls_{}".format(i) for i in range(len(d.ls))]
+                     for d in ch["decs"]]
+                R1, R2 = ch["R"]
+                out.append(dict(total=c.params_head() + "_total_0", g=g, g1=g[0], g2=g[1],
+                                mass=R1.name + "_mass", width=R1.name + "_width",
+                                mass2=R2.name + "_mass", width2=R2.name + "_width"))
+            return out
         out = []
         for c, ch in zip(self.decay_group.chains, self.chains):
             g1 = [ch["d1"].params_head() + "_g_ls_{}".format(i) for i in range(len(ch["d1"].ls))]
@@ -181,8 +361,19 @@ class AmplitudeModel:
         return out
 
     def shape_params(self):
-        """(m0, g0) of every chain's resonance from the parameter store."""
+        """(m0, g0) of every chain's resonance from the parameter store (four-body chains:
+        (m0, g0, m0', g0') of their two resonances)."""
         vm = self.vm
+        if self.n_final == 4:
+            out = []
+            for ch in self.chains:
+                t = ()
+                for R in ch["R"]:
+                    m0 = vm.variables.get(R.name + "_mass", R.mass)
+                    g0 = vm.variables.get(R.name + "_width", R.width)
+                    t += (float(m0), float(g0) if g0 is not None else 0.0)
+                out.append(t)
+            return out
         out = []
         for ch in self.chains:
             R = ch["R"]
@@ -201,6 +392,8 @@ class AmplitudeModel:
         return self._bc
 
     def _barrier_constants(self):
+        if self.n_final == 4:
+            return self._barrier_constants4()
         top = self.decay_group.top
         out = []
         for ch, (m0, g0) in zip(self.chains, self.shape_params()):
@@ -239,14 +432,18 @@ class AmplitudeModel:
         """The TpaModel structure for the current parameter values. The structure is cached:
         when only masses and widths changed since the last call, their fields are updated."""
         bc = self.barrier_constants()
-        flags = tuple((tuple(b["act1"]), tuple(b["act2"])) for b in bc)
+        flags = tuple(tuple(tuple(a) for a in b["act"]) for b in bc)
         cached = getattr(self, "_desc_cache", None)
         if cached is not None and cached[0] == flags:
             M = cached[1]
-            for c, (m0, g0) in enumerate(self.shape_params()):
-                M.chain[c].m0, M.chain[c].g0 = m0, g0
+            for c, sh in enumerate(self.shape_params()):
+                if self.n_final == 4:
+                    for r in range(2):
+                        M.chain[c].r_m0[r], M.chain[c].r_g0[r] = sh[2 * r], sh[2 * r + 1]
+                else:
+                    M.chain[c].m0, M.chain[c].g0 = sh
             return M
-        M = self._describe()
+        M = self._describe4() if self.n_final == 4 else self._describe()
         self._desc_cache = (flags, M)
         return M
 
@@ -310,7 +507,8 @@ class AmplitudeModel:
         key = self.vm.struct_version
         if getattr(self, "_sm_key", None) != key:
             tv = set(self.vm.trainable_vars)
-            self._sm = any(nm["mass"] in tv or nm["width"] in tv for nm in self._names())
+            self._sm = any(nm.get(k) in tv for nm in self._names()
+                           for k in ("mass", "width", "mass2", "width2"))
             self._sm_key = key
         return self._sm
 
@@ -323,7 +521,7 @@ class AmplitudeModel:
         if getattr(self, "_bs_version", None) == key:
             return self._bs
         bc = self.barrier_constants()
-        flags = tuple((tuple(b["act1"]), tuple(b["act2"])) for b in bc)
+        flags = tuple(tuple(tuple(a) for a in b["act"]) for b in bc)
         shapes = tuple(self.shape_params())
         fold = not self.shape_mode
         cache = (fold, flags, shapes if fold else None)
@@ -336,16 +534,19 @@ class AmplitudeModel:
         when the lineshape parameters or barrier clamp flags change."""
         cache, desc = self.build_state()
         L = _lib.lib()
+        four = self.n_final == 4
         if self._dev is None:
             M = self.describe()
             h = ctypes.c_void_p()
-            _lib.check(L.tpa_model_create(ctypes.byref(M), ctypes.sizeof(M),
-                                          _lib.dptr(self.cg), len(self.cg), ctypes.byref(h)))
+            create = L.tpa_model4_create if four else L.tpa_model_create
+            _lib.check(create(ctypes.byref(M), ctypes.sizeof(M), _lib.dptr(self.cg),
+                              len(self.cg), ctypes.byref(h)))
             self._dev = h
             self._desc_state = desc
         elif self._desc_state != desc:
             M = self.describe()
-            _lib.check(L.tpa_model_update(self._dev, ctypes.byref(M), ctypes.sizeof(M)))
+            update = L.tpa_model4_update if four else L.tpa_model_update
+            _lib.check(update(self._dev, ctypes.byref(M), ctypes.sizeof(M)))
             self._desc_state = desc
         self._dev_state = cache
         return self._dev

@@ -100,7 +100,10 @@ class EventSet:
         self._src_weight = src
 
     def angles(self):
-        nf = _lib.NF_TOPO * len(self.amp.topo) + 4
+        if self.amp.n_final == 4:
+            nf = _lib.NF4_TOPO * len(self.amp.topo) + 5
+        else:
+            nf = _lib.NF_TOPO * len(self.amp.topo) + 4
         out = np.empty((nf, self.n), dtype=np.float64)
         _lib.check(_lib.lib().tpa_events_get_angles(self._h, _lib.dptr(out), out.size))
         return out

@@ -78,7 +78,8 @@ class Particle:
 class Decay:
     """A two-body vertex core -> outs[0] + outs[1] (tf_pwa/amp/core.py:650-1233)."""
 
-    def __init__(self, core, outs, p_break=False, c_break=True, model="default", **kwargs):
+    def __init__(self, core, outs, p_break=False, c_break=True, model="default", l_list=None,
+                 ls_list=None, **kwargs):
         if len(outs) != 2:
             raise NotImplementedError("only two-body vertices are supported")
         self.core = core
@@ -89,8 +90,15 @@ class Decay:
         self.kwargs = kwargs
         self.d = 3.0
         ca = core.C if not c_break else None
-        self.ls = ls_couplings(core.J, outs[0].J, outs[1].J, core.P, outs[0].P, outs[1].P,
-                               p_break=self.p_break, ca=ca)
+        # ls_list replaces the allowed couplings, l_list keeps those with the listed l
+        # (HelicityDecay.get_ls_list, tf_pwa/amp/core.py:1212-1233)
+        if ls_list is not None:
+            self.ls = [(int(l), spin_value(sv)) for l, sv in ls_list]
+        else:
+            self.ls = ls_couplings(core.J, outs[0].J, outs[1].J, core.P, outs[0].P, outs[1].P,
+                                   p_break=self.p_break, ca=ca)
+            if l_list is not None:
+                self.ls = [(l, sv) for l, sv in self.ls if l in set(int(x) for x in l_list)]
         if not self.ls:
             raise ValueError("invalid spin parity for {}{}".format(
                 self, "" if self.p_break else ", maybe you should set `p_break: True`"))
