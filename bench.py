@@ -24,6 +24,7 @@ sys.path.insert(0, os.path.join(ROOT, "tf-pwa_amd"))
 sys.path.insert(0, ROOT)
 
 CONFIG = os.path.join(ROOT, "tf-pwa_amd", "tf_pwa_amd", "configs", "lc_pkpi_6res.yml")
+CONFIG4 = os.path.join(ROOT, "tf-pwa_amd", "tf_pwa_amd", "configs", "jpsi_kkpipi_12ch.yml")
 METRIC = "NLL+grad evals/s @1M data+10M MC, |dNLL|<1e-3 vs FP64; toy-fit p50/p99 ms"
 HBM_PEAK_GBS = 8000.0
 ANGLE_BYTES_PER_EVENT = 176
@@ -263,6 +264,70 @@ def multistart_leg(args, seed=3303, starts=8, n_data=10_000, n_mc=100_000):
             "ms_per_eval": 1e3 * wall / max(1, nfev)}
 
 
+def four_body_leg(args, seed=5505):
+    """J/psi -> K+ K- pi+ pi- through 12 chains (quasi-two-body and sequential, 25 partial
+    waves, 44 free couplings): 50M flat MC events generated on the device and 2M data events
+    accept-reject sampled from the model over device-generated candidates. Times FCN.nll_grad
+    on the FP64 path and the mixed path (MC Gram on FP32 MFMA), like the three-body headline."""
+    import gc
+
+    from tf_pwa_amd import FCN, ConfigLoader, EventSet, _lib
+
+    L = _lib.lib()
+    conf = ConfigLoader(CONFIG4, seed=seed)
+    conf.set_params(random_couplings(conf.vm, seed))
+    amp = conf.get_amplitude()
+    n_mc, n_data = args.four_body_mc, args.four_body_data
+    t = time.perf_counter()
+    mc = EventSet.generate(amp, n_mc, seed=seed)
+    f_mc = amp(mc)
+    fmax = float(f_mc.max())
+    log("four-body: {} MC events in {:.1f}s, <f>/fmax = {:.4f}".format(
+        n_mc, time.perf_counter() - t, f_mc.mean() / fmax))
+    del f_mc
+    t = time.perf_counter()
+    rng = np.random.default_rng(seed + 1)
+    data_sets, got, piece = [], 0, 0
+    while got < n_data:
+        cand = EventSet.generate(amp, args.four_body_pool, seed=seed + 7, first=piece * args.four_body_pool)
+        f = amp(cand)
+        idx = np.nonzero(f > rng.uniform(0.0, fmax, f.size))[0][:n_data - got]
+        if idx.size:
+            data_sets.append(cand.select(idx))
+            got += idx.size
+        del cand
+        piece += 1
+    gc.collect()
+    log("four-body: {} data events from {} candidates in {:.1f}s".format(
+        got, piece * args.four_body_pool, time.perf_counter() - t))
+    x0 = np.array(conf.vm.get_all_val())
+
+    def run(precision):
+        fcn = FCN(conf.get_model(), data_sets, mc, precision=precision)
+        for _ in range(args.warmup):
+            fcn.nll_grad(x0)
+        _lib.check(L.tpa_device_synchronize())
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            nll, g = fcn.nll_grad(x0)
+        _lib.check(L.tpa_device_synchronize())
+        return args.steps / (time.perf_counter() - t0), nll, g
+
+    e64, n64, g64 = run("fp64")
+    t = time.perf_counter()
+    emix, nmix, gmix = run("mixed")
+    dn = abs(nmix - n64)
+    ge = float(np.linalg.norm(gmix - g64) / np.linalg.norm(g64))
+    log("four-body: fp64 {:.1f} evals/s, mixed {:.1f} evals/s, |dNLL| {:.2e}".format(e64, emix, dn))
+    return {"workload": "J/psi -> K+ K- pi+ pi-, 12 chains in 6 topologies (quasi-two-body and "
+                        "sequential), 25 partial waves, {} data + {} MC, helicity +-1".format(
+                            got, n_mc),
+            "n_params": len(x0), "fp64_evals_per_s": e64, "mixed_evals_per_s": emix,
+            "speedup": emix / e64, "abs_dnll_vs_fp64": dn, "grad_rel_err_vs_fp64": ge,
+            "nll_fp64": n64, "data_candidates": piece * args.four_body_pool,
+            "mc_generated_on_device": True}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -276,6 +341,11 @@ def main():
     ap.add_argument("--serve-rate", type=float, default=512.0,
                     help="Poisson arrival rate of the serving leg's fits per second")
     ap.add_argument("--serve-budget-gb", type=float, default=200.0)
+    ap.add_argument("--four-body-mc", type=int, default=50_000_000,
+                    help="MC events of the four-body leg (0 skips it)")
+    ap.add_argument("--four-body-data", type=int, default=2_000_000)
+    ap.add_argument("--four-body-pool", type=int, default=20_000_000,
+                    help="device-generated candidates per accept-reject piece")
     ap.add_argument("--multistart", type=int, default=8,
                     help="BFGS fits of the floating-lineshape leg (0 skips it)")
     ap.add_argument("--calib-copy", action="store_true",
@@ -404,6 +474,12 @@ def main():
         res["serving"] = serving_leg(conf, mc, data_p4, x0, args)
     if args.multistart > 0:
         res["floating_lineshape_fits"] = multistart_leg(args, starts=args.multistart)
+    if args.four_body_mc > 0:
+        import gc
+
+        fmix = data = mc = None  # release the three-body sets' device memory first
+        gc.collect()
+        res["four_body"] = four_body_leg(args)
     if args.calib_copy:
         cms = ctypes.c_double(0)
         _lib.check(L.tpa_calib_copy(1 << 30, 3, ctypes.byref(cms)))

@@ -70,6 +70,22 @@ int tpa_model_destroy(tpa_model_t model);
  * of a fit (FCN.__init__, tf_pwa/model/model.py:1126-1164). weight: host [n] or NULL (1). */
 int tpa_events_create(tpa_model_t model, const double* p4, int64_t n, const double* weight,
                       tpa_events_t* out);
+/* Flat phase-space events of m_top -> masses[0..n_final) (the model's finals, in the engine's
+ * final order) generated on the device: GENBOD with a per-event Philox4x32 counter
+ * (seed, first + event index), so samples are reproducible and can be produced in pieces.
+ * The four-momenta stay on the device (copied to p4_out, host [n_final*4][n], when not NULL);
+ * the set's angles are computed from them. weight: host [n] or NULL (1).
+ * Replaces PhaseSpaceGenerator.generate (tf_pwa/phasespace.py:93-202) for MC samples. */
+int tpa_events_generate(tpa_model_t model, double m_top, const double* masses, int64_t n,
+                        uint64_t seed, uint64_t first, const double* weight, double* p4_out,
+                        tpa_events_t* out);
+
+/* A new event set made of events idx[0..n) of src (their angles; the partial-wave cache is
+ * built on demand), e.g. the accepted events of an accept-reject pass over a generated sample
+ * (tf_pwa/applications.py:430-453). weight: host [n] or NULL (1). */
+int tpa_events_select(tpa_events_t src, const int64_t* idx, int64_t n, const double* weight,
+                      tpa_events_t* out);
+
 int tpa_events_destroy(tpa_events_t ev);
 int64_t tpa_events_size(tpa_events_t ev);
 

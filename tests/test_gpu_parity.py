@@ -349,3 +349,51 @@ def test_four_body_amplitude_nll_gradient_hessian_match_oracle(case4, gpu_availa
     mix = FCN(conf.get_model(), data, mc, precision="mixed")
     n2, g2 = mix.nll_grad()
     assert abs(n2 - nll) < 1e-3 and np.linalg.norm(g2 - g) < 1e-4 * np.linalg.norm(g)
+
+
+@pytest.mark.parametrize("cfg", ["lc_pkpi_6res.yml", "jpsi_kkpipi_12ch.yml"])
+def test_device_phase_space_and_selection(cfg, gpu_available):
+    """Device GENBOD sample: four-momentum conservation and masses, the same invariant-mass
+    distributions as the host generators (two-sample KS), reproducible per (seed, index) and
+    in pieces, angles equal to those of an event set made from the same four-momenta; a
+    selected subset carries exactly the angles of its source events."""
+    from scipy.stats import ks_2samp
+
+    from conftest import config_path
+    from tf_pwa_amd import ConfigLoader, EventSet
+    from tf_pwa_amd.data import p4_array
+    from tf_pwa_amd.phasespace import generate_phsp
+
+    conf = ConfigLoader(config_path(cfg), seed=2)
+    amp = conf.get_amplitude()
+    nfin = len(amp.finals)
+    masses = [p.mass for p in amp.finals]
+    top = amp.decay_group.top.mass
+    n = 30000
+    ev, p4 = EventSet.generate(amp, n, seed=1234, return_p4=True)
+    p = [p4[4 * f:4 * f + 4].T for f in range(nfin)]
+    tot = sum(p)
+    np.testing.assert_allclose(tot[:, 0], top, atol=1e-12)
+    np.testing.assert_allclose(tot[:, 1:], 0.0, atol=1e-12)
+    for x, m in zip(p, masses):
+        np.testing.assert_allclose(np.sqrt(np.abs(x[:, 0] ** 2 - (x[:, 1:] ** 2).sum(1))), m,
+                                   atol=1e-10)
+    ref = generate_phsp(top, masses, n, np.random.default_rng(5))
+
+    def m2(a, b):
+        s = a + b
+        return s[:, 0] ** 2 - (s[:, 1:] ** 2).sum(1)
+
+    for i in range(nfin):
+        for j in range(i + 1, nfin):
+            assert ks_2samp(m2(p[i], p[j]), m2(ref[i], ref[j])).pvalue > 1e-4, (i, j)
+    # reproducible, and the same events when generated in two pieces
+    _, p4b = EventSet.generate(amp, 1000, seed=1234, first=500, return_p4=True)
+    np.testing.assert_array_equal(p4b, p4[:, 500:1500])
+    # the device sample's angles equal those of an event set uploaded from its four-momenta
+    up = EventSet(amp, {f.name: p4[4 * i:4 * i + 4].T[:2000] for i, f in enumerate(amp.finals)})
+    np.testing.assert_array_equal(up.angles(), ev.select(np.arange(2000)).angles())
+    idx = np.random.default_rng(0).choice(n, 777, replace=False)
+    sub = ev.select(idx, weight=np.full(777, 2.0))
+    np.testing.assert_array_equal(sub.angles(), ev.angles()[:, idx])
+    assert sub.n == 777 and np.all(sub.weight == 2.0)

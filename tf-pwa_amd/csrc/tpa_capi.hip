@@ -644,24 +644,21 @@ int tpa_events_destroy(tpa_events_t ev) {
   return TPA_OK;
 }
 
-int tpa_events_create(tpa_model_t m, const double* p4, int64_t n, const double* weight,
-                      tpa_events_t* out) {
-  if (!m || !p4 || !out) return fail(TPA_ERR_ARG, "null pointer");
-  if (n < 1) return fail(TPA_ERR_ARG, "event count must be positive");
+// An event set of n events: its buffers, weights (host, or 1 when null) and angle cache.
+// The angles come from device four-momenta (p4_dev, [rows][n]) when given; otherwise the
+// caller fills ev->d_ang.
+static int events_make(tpa_model_t m, int64_t n, const double* weight, const double* p4_dev,
+                       tpa_events_s** out) {
   tpa_events_s* ev = new tpa_events_s();
   ev->m = m;
   ev->n = n;
   ev->nf = angle_fields(m);
-  const int rows = p4_rows(m);
   const int P = m->h.P;
-  double* d_p4 = nullptr;
   hipError_t e;
   auto bail = [&](hipError_t er, const char* what) {
-    if (d_p4) (void)hipFree(d_p4);
     tpa_events_destroy(ev);
     return fail(TPA_ERR_HIP, std::string(what) + ": " + hipGetErrorString(er));
   };
-  if ((e = hipMalloc(&d_p4, sizeof(double) * rows * n)) != hipSuccess) return bail(e, "alloc p4");
   if ((e = hipMalloc(&ev->d_ang, sizeof(double) * ev->nf * n)) != hipSuccess)
     return bail(e, "alloc angles");
   if ((e = hipMalloc(&ev->d_w, sizeof(double) * n)) != hipSuccess) return bail(e, "alloc w");
@@ -674,9 +671,6 @@ int tpa_events_create(tpa_model_t m, const double* p4, int64_t n, const double* 
     return bail(e, "alloc coef");
   if ((e = hipEventCreate(&ev->t0)) != hipSuccess || (e = hipEventCreate(&ev->t1)) != hipSuccess)
     return bail(e, "event create");
-  if ((e = hipMemcpyAsync(d_p4, p4, sizeof(double) * rows * n, hipMemcpyHostToDevice, m->stream)) !=
-      hipSuccess)
-    return bail(e, "upload p4");
   if (weight) {
     if ((e = hipMemcpyAsync(ev->d_w, weight, sizeof(double) * n, hipMemcpyHostToDevice,
                             m->stream)) != hipSuccess)
@@ -686,17 +680,106 @@ int tpa_events_create(tpa_model_t m, const double* p4, int64_t n, const double* 
     if ((e = hipMemcpyAsync(ev->d_w, ones.data(), sizeof(double) * n, hipMemcpyHostToDevice,
                             m->stream)) != hipSuccess)
       return bail(e, "upload weights");
-    if ((e = hipStreamSynchronize(m->stream)) != hipSuccess) return bail(e, "sync");
   }
-  const int block = 256;
-  const int64_t grid = (n + block - 1) / block;
-  (void)block;
-  (void)grid;
-  launch_cal_angle(m, d_p4, n, 0, n, ev->d_ang, m->stream);
-  if ((e = hipGetLastError()) != hipSuccess) return bail(e, "launch cal_angle");
+  if (p4_dev) {
+    launch_cal_angle(m, p4_dev, n, 0, n, ev->d_ang, m->stream);
+    if ((e = hipGetLastError()) != hipSuccess) return bail(e, "launch cal_angle");
+  }
   if ((e = hipStreamSynchronize(m->stream)) != hipSuccess) return bail(e, "cal_angle");
+  *out = ev;
+  return TPA_OK;
+}
+
+int tpa_events_create(tpa_model_t m, const double* p4, int64_t n, const double* weight,
+                      tpa_events_t* out) {
+  if (!m || !p4 || !out) return fail(TPA_ERR_ARG, "null pointer");
+  if (n < 1) return fail(TPA_ERR_ARG, "event count must be positive");
+  const int rows = p4_rows(m);
+  double* d_p4 = nullptr;
+  HIPCHK(hipMalloc(&d_p4, sizeof(double) * rows * n));
+  hipError_t e = hipMemcpyAsync(d_p4, p4, sizeof(double) * rows * n, hipMemcpyHostToDevice,
+                                m->stream);
+  if (e != hipSuccess) {
+    (void)hipFree(d_p4);
+    return fail(TPA_ERR_HIP, std::string("upload p4: ") + hipGetErrorString(e));
+  }
+  const int rc = events_make(m, n, weight, d_p4, out);
   (void)hipFree(d_p4);
-  d_p4 = nullptr;
+  return rc;
+}
+
+int tpa_events_generate(tpa_model_t m, double m_top, const double* masses, int64_t n,
+                        uint64_t seed, uint64_t first, const double* weight, double* p4_out,
+                        tpa_events_t* out) {
+  if (!m || !masses || !out) return fail(TPA_ERR_ARG, "null pointer");
+  if (n < 1) return fail(TPA_ERR_ARG, "event count must be positive");
+  const int nf = m->kind == 4 ? 4 : 3;
+  double msum = 0.0;
+  for (int i = 0; i < nf; ++i) {
+    if (!(masses[i] >= 0.0)) return fail(TPA_ERR_ARG, "masses must be non-negative");
+    msum += masses[i];
+  }
+  if (!(m_top > msum)) return fail(TPA_ERR_ARG, "the top mass is below the final-state masses");
+  // GENBOD's weight maximum (emmin, emmax running bounds of the intermediate masses)
+  double emmax = m_top - msum + masses[0], emmin = 0.0, wtmax = 1.0;
+  for (int i = 1; i < nf; ++i) {
+    emmin += masses[i - 1];
+    emmax += masses[i];
+    const double s = emmin + masses[i], d = emmin - masses[i];
+    const double p2 = (emmax * emmax - s * s) * (emmax * emmax - d * d);
+    wtmax *= p2 > 0.0 ? std::sqrt(p2) / (2.0 * emmax) : 0.0;
+  }
+  const int rows = 4 * nf;
+  double* d_p4 = nullptr;
+  double* d_m = nullptr;
+  HIPCHK(hipMalloc(&d_p4, sizeof(double) * rows * n));
+  hipError_t e = hipMalloc(&d_m, sizeof(double) * 4);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(d_m, masses, sizeof(double) * nf, hipMemcpyHostToDevice, m->stream);
+  if (e == hipSuccess) {
+    hipLaunchKernelGGL(k_phsp, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, m->stream, nf,
+                       m_top, d_m, wtmax, n, (int64_t)first, (uint32_t)seed,
+                       (uint32_t)(seed >> 32), d_p4);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess && p4_out)
+    e = hipMemcpyAsync(p4_out, d_p4, sizeof(double) * rows * n, hipMemcpyDeviceToHost,
+                       m->stream);
+  int rc = TPA_OK;
+  if (e != hipSuccess)
+    rc = fail(TPA_ERR_HIP, std::string("phase space: ") + hipGetErrorString(e));
+  else
+    rc = events_make(m, n, weight, d_p4, out);
+  (void)hipFree(d_p4);
+  if (d_m) (void)hipFree(d_m);
+  return rc;
+}
+
+int tpa_events_select(tpa_events_t src, const int64_t* idx, int64_t n, const double* weight,
+                      tpa_events_t* out) {
+  if (!src || !idx || !out) return fail(TPA_ERR_ARG, "null pointer");
+  if (n < 1) return fail(TPA_ERR_ARG, "event count must be positive");
+  for (int64_t i = 0; i < n; ++i)
+    if (idx[i] < 0 || idx[i] >= src->n) return fail(TPA_ERR_ARG, "index out of range");
+  tpa_model_s* m = src->m;
+  tpa_events_s* ev = nullptr;
+  int rc = events_make(m, n, weight, nullptr, &ev);
+  if (rc != TPA_OK) return rc;
+  int64_t* d_idx = nullptr;
+  hipError_t e = hipMalloc(&d_idx, sizeof(int64_t) * n);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(d_idx, idx, sizeof(int64_t) * n, hipMemcpyHostToDevice, m->stream);
+  if (e == hipSuccess) {
+    hipLaunchKernelGGL(k_gather_cols, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, m->stream,
+                       src->d_ang, src->nf, src->n, d_idx, n, ev->d_ang);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipStreamSynchronize(m->stream);
+  if (d_idx) (void)hipFree(d_idx);
+  if (e != hipSuccess) {
+    tpa_events_destroy(ev);
+    return fail(TPA_ERR_HIP, std::string("select: ") + hipGetErrorString(e));
+  }
   *out = ev;
   return TPA_OK;
 }

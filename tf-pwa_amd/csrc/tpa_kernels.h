@@ -1177,3 +1177,141 @@ __global__ __launch_bounds__(256) void k_hess(HessArgs a) {
       }
   }
 }
+
+// ------------------------------------------------------------------ phase space on the device
+// Counter-based random numbers: Philox4x32 with 10 rounds (J. K. Salmon et al., "Parallel
+// random numbers: as easy as 1, 2, 3", SC11). Every event draws from its own counter, so a
+// sample is reproducible from (seed, event index) whatever the launch shape.
+struct philox_ctr {
+  uint32_t x[4];
+};
+
+TPA_DEV philox_ctr philox4x32(philox_ctr c, uint32_t k0, uint32_t k1) {
+  const uint32_t M0 = 0xD2511F53u, M1 = 0xCD9E8D57u, W0 = 0x9E3779B9u, W1 = 0xBB67AE85u;
+#pragma unroll
+  for (int r = 0; r < 10; ++r) {
+    const uint64_t p0 = (uint64_t)M0 * c.x[0], p1 = (uint64_t)M1 * c.x[2];
+    const uint32_t hi0 = (uint32_t)(p0 >> 32), lo0 = (uint32_t)p0;
+    const uint32_t hi1 = (uint32_t)(p1 >> 32), lo1 = (uint32_t)p1;
+    philox_ctr n;
+    n.x[0] = hi1 ^ c.x[1] ^ k0;
+    n.x[1] = lo1;
+    n.x[2] = hi0 ^ c.x[3] ^ k1;
+    n.x[3] = lo0;
+    c = n;
+    k0 += W0;
+    k1 += W1;
+  }
+  return c;
+}
+
+// Uniform doubles in [0, 1) with 53 random bits, drawn two per Philox block.
+struct uniform_stream {
+  uint32_t k0, k1;
+  uint64_t event;
+  uint32_t block;
+  double cache[2];
+  int left;
+  TPA_DEV double next() {
+    if (left == 0) {
+      philox_ctr c;
+      c.x[0] = (uint32_t)event;
+      c.x[1] = (uint32_t)(event >> 32);
+      c.x[2] = block++;
+      c.x[3] = 0x7f4a7c15u;
+      c = philox4x32(c, k0, k1);
+      const uint64_t a = ((uint64_t)c.x[0] << 32) | c.x[1];
+      const uint64_t b = ((uint64_t)c.x[2] << 32) | c.x[3];
+      cache[0] = (double)(a >> 11) * 0x1.0p-53;
+      cache[1] = (double)(b >> 11) * 0x1.0p-53;
+      left = 2;
+    }
+    return cache[--left];
+  }
+};
+
+TPA_DEV double pdk(double M, double m1, double m2) {
+  const double s = m1 + m2, d = m1 - m2;
+  const double p2 = (M * M - s * s) * (M * M - d * d);
+  return p2 > 0.0 ? sqrt(p2) / (2.0 * M) : 0.0;
+}
+
+// Back-to-back pair in the rest frame of M with an isotropic direction.
+TPA_DEV void iso_pair(uniform_stream& u, double M, double ma, double mb, v4* pa, v4* pb) {
+  const double c = 2.0 * u.next() - 1.0;
+  const double s = sqrt(fmax(0.0, 1.0 - c * c));
+  const double phi = 2.0 * M_PI * u.next();
+  const double q = pdk(M, ma, mb);
+  const double px = q * s * cos(phi), py = q * s * sin(phi), pz = q * c;
+  *pa = v4{sqrt(q * q + ma * ma), px, py, pz};
+  *pb = v4{sqrt(q * q + mb * mb), -px, -py, -pz};
+}
+
+// Boost p by the velocity of the four-vector frame (frame.t, frame.x, ...) given in the same
+// system (the momentum of a rest frame seen from outside).
+TPA_DEV v4 boost_by(v4 p, v4 frame) {
+  const v3 b = v_make(frame.x / frame.t, frame.y / frame.t, frame.z / frame.t);
+  return lv_boost(p, b);
+}
+
+// Flat phase space of m0 -> masses[0..nf) by GENBOD (F. James, CERN 68-15): sorted uniform
+// intermediate masses accepted with probability prod_i p_i / wtmax, then a chain of
+// back-to-back pairs boosted along. Writes p4 [f*4 + c][n]; one thread per event, each with
+// its own Philox counter (event index + first), retrying until accepted.
+__global__ __launch_bounds__(256) void k_phsp(int nf, double m0, const double* __restrict__ masses,
+                                              double wtmax, int64_t n, int64_t first, uint32_t k0,
+                                              uint32_t k1, double* __restrict__ p4) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n) return;
+  double m[4];
+  double msum = 0.0;
+  for (int i = 0; i < nf; ++i) {
+    m[i] = masses[i];
+    msum += m[i];
+  }
+  const double tecm = m0 - msum;
+  uniform_stream u{k0, k1, (uint64_t)(first + e), 0u, {0.0, 0.0}, 0};
+  double inv[4];
+  for (int attempt = 0; attempt < 4096; ++attempt) {
+    double r[2] = {u.next(), u.next()};
+    if (nf == 4 && r[0] > r[1]) {
+      const double t = r[0];
+      r[0] = r[1];
+      r[1] = t;
+    }
+    inv[0] = m[0];
+    double cs = m[0];
+    for (int i = 1; i < nf - 1; ++i) {
+      cs += m[i];
+      inv[i] = r[i - 1] * tecm + cs;
+    }
+    inv[nf - 1] = m0;
+    double w = 1.0;
+    for (int i = 1; i < nf; ++i) w *= pdk(inv[i], inv[i - 1], m[i]);
+    if (w > u.next() * wtmax) break;
+  }
+  v4 p[4];
+  iso_pair(u, inv[1], m[0], m[1], &p[0], &p[1]);
+  for (int i = 2; i < nf; ++i) {
+    v4 sys, pi;
+    iso_pair(u, inv[i], inv[i - 1], m[i], &sys, &pi);
+    for (int j = 0; j < i; ++j) p[j] = boost_by(p[j], sys);
+    p[i] = pi;
+  }
+  for (int f = 0; f < nf; ++f) {
+    p4[(int64_t)(4 * f + 0) * n + e] = p[f].t;
+    p4[(int64_t)(4 * f + 1) * n + e] = p[f].x;
+    p4[(int64_t)(4 * f + 2) * n + e] = p[f].y;
+    p4[(int64_t)(4 * f + 3) * n + e] = p[f].z;
+  }
+}
+
+// Gather columns idx[0..n) of an angle cache (nf rows, stride S_src) into a new cache.
+__global__ __launch_bounds__(256) void k_gather_cols(const double* __restrict__ src, int nf,
+                                                     int64_t S_src, const int64_t* __restrict__ idx,
+                                                     int64_t n, double* __restrict__ dst) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n) return;
+  const int64_t j = idx[e];
+  for (int r = 0; r < nf; ++r) dst[(int64_t)r * n + e] = src[(int64_t)r * S_src + j];
+}

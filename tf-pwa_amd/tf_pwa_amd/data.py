@@ -77,6 +77,42 @@ class EventSet:
         self._weight_version = 0
         self._state = None
 
+    @classmethod
+    def _wrap(cls, amp, h, n, weight):
+        self = cls.__new__(cls)
+        self.amp = amp
+        self.n = int(n)
+        self.weight = np.ones(self.n) if weight is None else np.array(weight, dtype=np.float64)
+        self._h = h
+        self._dev_weight = self.weight.copy()
+        self._weight_version = 0
+        self._state = None
+        return self
+
+    @classmethod
+    def generate(cls, amp, n, seed, weight=None, first=0, return_p4=False):
+        """n flat phase-space events generated on the device (tpa_events_generate): the
+        counter-based sample of (seed, first + index). With ``return_p4`` also returns the
+        four-momenta [n_final * 4][n] in the engine's final order."""
+        masses = np.array([p.mass for p in amp.finals], dtype=np.float64)
+        w = None if weight is None else np.ascontiguousarray(weight, dtype=np.float64)
+        p4 = np.empty((4 * len(masses), n)) if return_p4 else None
+        h = ctypes.c_void_p()
+        _lib.check(_lib.lib().tpa_events_generate(
+            amp.device(), float(amp.decay_group.top.mass), _lib.dptr(masses), int(n),
+            int(seed) & (2 ** 64 - 1), int(first), _lib.dptr(w), _lib.dptr(p4), ctypes.byref(h)))
+        ev = cls._wrap(amp, h, n, w)
+        return (ev, p4) if return_p4 else ev
+
+    def select(self, idx, weight=None):
+        """A new event set of the events ``idx`` of this one (their angles)."""
+        idx = np.ascontiguousarray(idx, dtype=np.int64)
+        w = None if weight is None else np.ascontiguousarray(weight, dtype=np.float64)
+        h = ctypes.c_void_p()
+        _lib.check(_lib.lib().tpa_events_select(self._h, idx.ctypes.data_as(
+            ctypes.POINTER(ctypes.c_int64)), idx.size, _lib.dptr(w), ctypes.byref(h)))
+        return EventSet._wrap(self.amp, h, idx.size, w)
+
     def __len__(self):
         return self.n
 
