@@ -150,6 +150,12 @@ int tpa_timing_reset(void);
  * ms (optional): the MFMA kernel's duration. */
 int tpa_gram(tpa_events_t ev, double w_mean, double* out, double* ms);
 
+/* The same Gram matrix on FP64 MFMA (v_mfma_f64_16x16x4f64): products and sums in FP64, for
+ * amplitudes whose dynamic range makes the FP32 partial sums of tpa_gram drift (narrow peaks
+ * over a broad MC sample). Same output layout and the same reference interface as tpa_gram
+ * (tf_pwa/experimental/opt_int.py:30-52). P <= 32. ms (optional): the MFMA kernel's duration. */
+int tpa_gram_f64(tpa_events_t ev, double* out, double* ms);
+
 /* Second moments for the exact Hessian (fold = 1 cache, P <= 32). out (host, 2 + 2P + Q(Q+1)
  * doubles, Q = 2P): out[0] the sum of the mode, out[2+2k], out[3+2k] Re/Im G_k as in tpa_eval,
  * then the packed upper triangles (row-major, i <= j) of

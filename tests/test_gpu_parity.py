@@ -177,7 +177,7 @@ def test_gram_matches_partial_waves(case2, gpu_available):
     mc.set_weight(w)
     h = mc.partial_waves()  # [P][n][lam_pad]
     ref = np.einsum("e,kel,jel->kj", w, np.conj(h), h)
-    got = mc.gram()
+    got = mc.gram("fp32")
     scale = np.abs(ref).max()
     assert np.abs(got - ref).max() < 1e-6 * scale
     np.testing.assert_allclose(got, np.conj(got.T), atol=1e-12 * scale)
@@ -190,6 +190,29 @@ def test_gram_matches_partial_waves(case2, gpu_available):
     np.testing.assert_allclose(np.conj(got @ c), G, rtol=0, atol=1e-6 * np.abs(G).max())
 
 
+@pytest.mark.parametrize("which", ["case1", "case2", "case4"])
+def test_gram_f64_matches_partial_waves(which, request, gpu_available):
+    """FP64 MFMA Gram (one to four 16-column tiles: 2, 6 resonances and the 12-chain four-body
+    model, ragged last row block) against the FP64 Gram of the same partial waves read back."""
+    from tf_pwa_amd import _lib
+
+    case = request.getfixturevalue(which)
+    conf = case["conf"]
+    mc = conf.data_from_p4(case["mc"])
+    w = np.random.default_rng(1).uniform(0.5, 1.5, mc.n)
+    mc.set_weight(w)
+    h = mc.partial_waves()
+    ref = np.einsum("e,kel,jel->kj", w, np.conj(h), h)
+    got = mc.gram("fp64")
+    scale = np.abs(ref).max()
+    assert np.abs(got - ref).max() < 1e-12 * scale
+    np.testing.assert_array_equal(got, np.conj(got.T))
+    c = conf.get_amplitude().coefficients()
+    v, G, _ = mc.eval(c, _lib.MODE_MC, grad=True)
+    assert abs(np.real(np.vdot(c, got @ c)) - v) < 1e-12 * abs(v)
+    np.testing.assert_allclose(np.conj(got @ c), G, rtol=0, atol=1e-12 * np.abs(G).max())
+
+
 def test_mixed_path_close_to_fp64(case2, gpu_available):
     from tf_pwa_amd import FCN
 
@@ -197,11 +220,17 @@ def test_mixed_path_close_to_fp64(case2, gpu_available):
     data = conf.data_from_p4(case2["data"], weight=case2["wd"])
     mc = conf.data_from_p4(case2["mc"])
     f64 = FCN(conf.get_model(), data, mc)
-    mix = FCN(conf.get_model(), data, mc, precision="mixed")
     n1, g1 = f64.nll_grad()
-    n2, g2 = mix.nll_grad()
-    assert abs(n1 - n2) < 1e-3
-    assert np.linalg.norm(g1 - g2) < 1e-4 * np.linalg.norm(g1)
+    # FP64-MFMA Gram (default): the same sums in another order
+    n2, g2 = FCN(conf.get_model(), data, mc, precision="mixed").nll_grad()
+    assert abs(n1 - n2) < 1e-9
+    assert np.linalg.norm(g1 - g2) < 1e-10 * np.linalg.norm(g1)
+    # FP32-MFMA Gram
+    model32 = conf.get_model()
+    model32.gram_precision = "fp32"
+    n3, g3 = FCN(model32, data, mc, precision="mixed").nll_grad()
+    assert abs(n1 - n3) < 1e-3
+    assert np.linalg.norm(g1 - g3) < 1e-4 * np.linalg.norm(g1)
 
 
 @pytest.mark.parametrize("cfg,res", [
@@ -280,7 +309,7 @@ def test_fit_fractions_from_the_gram_matrix(case2, gpu_available):
     assert all(e >= 0 and np.isfinite(e) for e in ferr.values())
     # the FP32-MFMA Gram gives the same fractions to the mixed path's accuracy
     mc.set_weight(np.ones(mc.n))
-    frac32, _ = cal_fitfractions(amp, gram=mc.gram())
+    frac32, _ = cal_fitfractions(amp, gram=mc.gram("fp32"))
     for key, val in frac.items():
         assert abs(frac32[key] - val) < 1e-5
 
@@ -346,9 +375,12 @@ def test_four_body_amplitude_nll_gradient_hessian_match_oracle(case4, gpu_availa
     assert np.linalg.norm(g - ref_g) <= 1e-7 * np.linalg.norm(ref_g)
     _, _, H = fcn.nll_grad_hessian()
     assert np.abs(H - ref_H).max() <= 1e-7 * np.abs(ref_H).max()
-    mix = FCN(conf.get_model(), data, mc, precision="mixed")
-    n2, g2 = mix.nll_grad()
+    model32 = conf.get_model()
+    model32.gram_precision = "fp32"
+    n2, g2 = FCN(model32, data, mc, precision="mixed").nll_grad()
     assert abs(n2 - nll) < 1e-3 and np.linalg.norm(g2 - g) < 1e-4 * np.linalg.norm(g)
+    n3, g3 = FCN(conf.get_model(), data, mc, precision="mixed").nll_grad()
+    assert abs(n3 - nll) < 1e-9 and np.linalg.norm(g3 - g) < 1e-10 * np.linalg.norm(g)
 
 
 @pytest.mark.parametrize("cfg", ["lc_pkpi_6res.yml", "jpsi_kkpipi_12ch.yml"])

@@ -918,8 +918,28 @@ int tpa_gram(tpa_events_t ev, double w_mean, double* out, double* ms) {
   if (!ev->d_gpart) HIPCHK(hipMalloc(&ev->d_gpart, sizeof(double) * 1024 * TPA_GRAM_SLOTS));
   if (!ev->d_gout) HIPCHK(hipMalloc(&ev->d_gout, sizeof(double) * TPA_GRAM_SLOTS));
   HIPCHK(hipEventRecord(ev->t0, s));
-  hipLaunchKernelGGL(k_gram_f32, dim3(grid), dim3(256), 0, s, ev->d_h, ev->d_w, wscale, ev->n,
-                     M.lam_pad, M.lam, M.P, ev->d_gpart);
+  static const int refold = [] {
+    const char* v = std::getenv("TFPWA_GRAM_REFOLD");
+    const int x = v ? std::atoi(v) : 8;
+    return (x == 1 || x == 2 || x == 4 || x == 8) ? x : 8;
+  }();
+  switch (refold) {
+    case 1:
+      hipLaunchKernelGGL(k_gram_f32<1>, dim3(grid), dim3(256), 0, s, ev->d_h, ev->d_w, wscale,
+                         ev->n, M.lam_pad, M.lam, M.P, ev->d_gpart);
+      break;
+    case 2:
+      hipLaunchKernelGGL(k_gram_f32<2>, dim3(grid), dim3(256), 0, s, ev->d_h, ev->d_w, wscale,
+                         ev->n, M.lam_pad, M.lam, M.P, ev->d_gpart);
+      break;
+    case 4:
+      hipLaunchKernelGGL(k_gram_f32<4>, dim3(grid), dim3(256), 0, s, ev->d_h, ev->d_w, wscale,
+                         ev->n, M.lam_pad, M.lam, M.P, ev->d_gpart);
+      break;
+    default:
+      hipLaunchKernelGGL(k_gram_f32<8>, dim3(grid), dim3(256), 0, s, ev->d_h, ev->d_w, wscale,
+                         ev->n, M.lam_pad, M.lam, M.P, ev->d_gpart);
+  }
   HIPCHK(hipGetLastError());
   HIPCHK(hipEventRecord(ev->t1, s));
   hipLaunchKernelGGL(k_reduce_slots, dim3(TPA_GRAM_SLOTS), dim3(256), 0, s, ev->d_gpart, grid,
@@ -949,6 +969,73 @@ int tpa_gram(tpa_events_t ev, double w_mean, double* out, double* ms) {
       out[2 * (k * P + l)] =
           w_mean * (0.5 * ((G(0, k, l) + G(2, k, l)) + (G(0, l, k) + G(2, l, k))));
       out[2 * (k * P + l) + 1] = w_mean * (G(1, k, l) - G(1, l, k));
+    }
+  return TPA_OK;
+}
+
+int tpa_gram_f64(tpa_events_t ev, double* out, double* ms) {
+  if (!ev || !out) return fail(TPA_ERR_ARG, "null pointer");
+  if (!ev->d_h) return fail(TPA_ERR_ARG, "partial-wave cache not built");
+  const TpaModel& M = ev->m->h;
+  const int P = M.P, lam_pad = M.lam_pad, lam = M.lam;
+  if (P > 32) return fail(TPA_ERR_UNSUPPORTED, "the FP64 MFMA Gram supports up to 32 partial waves");
+  const int QT = (2 * P + 15) / 16;
+  hipStream_t s = ev->m->stream;
+  const int64_t rows = ev->n * (int64_t)lam_pad;
+  const int grid = (int)std::min<int64_t>((rows + TPA_GRAM64_ROWS - 1) / TPA_GRAM64_ROWS, 1024);
+  if (!ev->d_gpart) HIPCHK(hipMalloc(&ev->d_gpart, sizeof(double) * 1024 * TPA_GRAM_SLOTS));
+  if (!ev->d_gout) HIPCHK(hipMalloc(&ev->d_gout, sizeof(double) * TPA_GRAM_SLOTS));
+  HIPCHK(hipEventRecord(ev->t0, s));
+  switch (QT) {
+    case 1:
+      hipLaunchKernelGGL(k_gram_f64<1>, dim3(grid), dim3(256), 0, s, ev->d_h, ev->d_w, ev->n,
+                         lam_pad, lam, P, ev->d_gpart);
+      break;
+    case 2:
+      hipLaunchKernelGGL(k_gram_f64<2>, dim3(grid), dim3(256), 0, s, ev->d_h, ev->d_w, ev->n,
+                         lam_pad, lam, P, ev->d_gpart);
+      break;
+    case 3:
+      hipLaunchKernelGGL(k_gram_f64<3>, dim3(grid), dim3(256), 0, s, ev->d_h, ev->d_w, ev->n,
+                         lam_pad, lam, P, ev->d_gpart);
+      break;
+    default:
+      hipLaunchKernelGGL(k_gram_f64<4>, dim3(grid), dim3(256), 0, s, ev->d_h, ev->d_w, ev->n,
+                         lam_pad, lam, P, ev->d_gpart);
+  }
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipEventRecord(ev->t1, s));
+  // only the NT = QT (QT + 1) / 2 tiles' 256 slots each are written
+  hipLaunchKernelGGL(k_reduce_slots, dim3(QT * (QT + 1) / 2 * 256), dim3(256), 0, s, ev->d_gpart,
+                     grid, TPA_GRAM64_SLOTS, ev->d_gout);
+  HIPCHK(hipGetLastError());
+  std::vector<double> raw(TPA_GRAM64_SLOTS);
+  HIPCHK(hipMemcpyAsync(raw.data(), ev->d_gout, sizeof(double) * TPA_GRAM64_SLOTS,
+                        hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+  if (ms) {
+    float f = 0.0f;
+    HIPCHK(hipEventElapsedTime(&f, ev->t0, ev->t1));
+    *ms = f;
+  }
+  // decode the 16x16 FP64 MFMA accumulator layout: element (i, j) of a tile sits in lane
+  // j + 16 * (i % 4), accumulator register i / 4; tiles q enumerate (ti <= tj) row-major
+  std::vector<int> tile(QT * QT, -1);
+  for (int ti = 0, q = 0; ti < QT; ++ti)
+    for (int tj = ti; tj < QT; ++tj, ++q) tile[ti * QT + tj] = q;
+  auto G = [&](int a, int b) {
+    if (a / 16 > b / 16) std::swap(a, b);
+    const int q = tile[(a / 16) * QT + b / 16];
+    const int i = a % 16, j = b % 16;
+    return raw[(q * 4 + i / 4) * 64 + j + 16 * (i % 4)];
+  };
+  for (int k = 0; k < P; ++k)
+    for (int l = 0; l < P; ++l) {
+      // the two positions of a symmetric entry round differently; their mean makes I
+      // exactly Hermitian
+      out[2 * (k * P + l)] =
+          0.5 * ((G(k, l) + G(P + k, P + l)) + (G(l, k) + G(P + l, P + k)));
+      out[2 * (k * P + l) + 1] = 0.5 * ((G(k, P + l) - G(P + k, l)) - (G(l, P + k) - G(P + l, k)));
     }
   return TPA_OK;
 }

@@ -525,6 +525,7 @@ typedef float tpa_floatx16 __attribute__((ext_vector_type(16)));
 #define TPA_GRAM_LDS_STRIDE (TPA_GRAM_ROWS + 1)
 #define TPA_GRAM_SLOTS (3 * 16 * 64)
 
+template <int RF>
 __global__ __launch_bounds__(256) void k_gram_f32(const double2* __restrict__ h,
                                                   const double* __restrict__ w, double wscale,
                                                   int64_t N, int lam_pad, int lam, int P,
@@ -566,12 +567,12 @@ __global__ __launch_bounds__(256) void k_gram_f32(const double2* __restrict__ h,
     const int i = lane & 31;
     // FP32 accumulation runs over 16 rows (8 MFMA steps) before each FP64 refold, which keeps
     // the FP32 rounding of the partial sums at the level of single products
-    for (int s0 = 0; s0 < 32; s0 += 8) {
+    for (int s0 = 0; s0 < 32; s0 += RF) {
       tpa_floatx16 cxx, cxy, cyy;
 #pragma unroll
       for (int v = 0; v < 16; ++v) cxx[v] = cxy[v] = cyy[v] = 0.0f;
 #pragma unroll
-      for (int s = s0; s < s0 + 8; ++s) {
+      for (int s = s0; s < s0 + RF; ++s) {
         const int rl = wave * 64 + 2 * s + (lane >> 5);
         const float x = Xs[i * TPA_GRAM_LDS_STRIDE + rl];
         const float y = Ys[i * TPA_GRAM_LDS_STRIDE + rl];
@@ -1314,4 +1315,89 @@ __global__ __launch_bounds__(256) void k_gather_cols(const double* __restrict__ 
   if (e >= n) return;
   const int64_t j = idx[e];
   for (int r = 0; r < nf; ++r) dst[(int64_t)r * n + e] = src[(int64_t)r * S_src + j];
+}
+
+// ------------------------------------------------------------------ FP64 MFMA Gram matrix
+// G = sum_rows w_row Z_row Z_row^T in FP64 on the matrix cores (v_mfma_f64_16x16x4f64), with
+// Z_row = [Re h_0 .. Re h_{P-1}, Im h_0 .. Im h_{P-1}] (Q = 2P <= 64 columns, QT = ceil(Q/16)
+// tiles). A block stages 128 rows in LDS; each wave multiplies its 32 rows into the upper
+// tiles (ti <= tj) of G. No refolds are needed: products and sums are FP64 throughout.
+typedef double tpa_doublex4 __attribute__((ext_vector_type(4)));
+
+#define TPA_GRAM64_ROWS 128
+#define TPA_GRAM64_STRIDE (TPA_GRAM64_ROWS + 1)
+#define TPA_GRAM64_SLOTS (10 * 256)
+
+template <int QT>
+__global__ __launch_bounds__(256) void k_gram_f64(const double2* __restrict__ h,
+                                                  const double* __restrict__ w, int64_t N,
+                                                  int lam_pad, int lam, int P,
+                                                  double* __restrict__ partials) {
+  constexpr int NT = QT * (QT + 1) / 2;
+  __shared__ double Zs[16 * QT * TPA_GRAM64_STRIDE];
+  __shared__ double Ws[TPA_GRAM64_ROWS];
+  const int t = threadIdx.x;
+  const int lane = t & 63;
+  const int wave = t >> 6;
+  const int Q = 2 * P;
+  const int64_t R = N * lam_pad;
+  tpa_doublex4 acc[NT];
+#pragma unroll
+  for (int q = 0; q < NT; ++q) acc[q] = tpa_doublex4{0.0, 0.0, 0.0, 0.0};
+  for (int64_t rb = (int64_t)blockIdx.x * TPA_GRAM64_ROWS; rb < R;
+       rb += (int64_t)gridDim.x * TPA_GRAM64_ROWS) {
+    // stage: thread t handles rows t and t - 128 ... (128 rows x Q columns)
+    for (int idx = t; idx < TPA_GRAM64_ROWS * P; idx += 256) {
+      const int r = idx % TPA_GRAM64_ROWS, k = idx / TPA_GRAM64_ROWS;
+      const int64_t row = rb + r;
+      const double2 v = row < R ? load_pw<true>(&h[(int64_t)k * R + row]) : make_double2(0.0, 0.0);
+      Zs[k * TPA_GRAM64_STRIDE + r] = v.x;
+      Zs[(P + k) * TPA_GRAM64_STRIDE + r] = v.y;
+    }
+    for (int idx = t; idx < (16 * QT - Q) * TPA_GRAM64_ROWS; idx += 256)
+      Zs[(Q + idx / TPA_GRAM64_ROWS) * TPA_GRAM64_STRIDE + idx % TPA_GRAM64_ROWS] = 0.0;
+    if (t < TPA_GRAM64_ROWS) {
+      const int64_t row = rb + t;
+      double wr = 0.0;
+      if (row < R) {
+        const int64_t e = row / lam_pad;
+        const int l = (int)(row - e * lam_pad);
+        wr = l < lam ? w[e] : 0.0;
+      }
+      Ws[t] = wr;
+    }
+    __syncthreads();
+    const int i = lane & 15, kk = lane >> 4;
+#pragma unroll
+    for (int s = 0; s < 8; ++s) {
+      const int rl = wave * 32 + 4 * s + kk;
+      const double ww = Ws[rl];
+      double a[QT];
+#pragma unroll
+      for (int ti = 0; ti < QT; ++ti) a[ti] = Zs[(16 * ti + i) * TPA_GRAM64_STRIDE + rl];
+      int q = 0;
+#pragma unroll
+      for (int ti = 0; ti < QT; ++ti)
+#pragma unroll
+        for (int tj = ti; tj < QT; ++tj, ++q)
+          acc[q] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[ti], ww * a[tj], acc[q], 0, 0, 0);
+    }
+    __syncthreads();
+  }
+  // block sum of the four waves in a fixed order (reusing the staging area)
+  double* red = Zs;  // NT * 256 doubles <= 2560 < 16 * QT * TPA_GRAM64_STRIDE for QT >= 2
+  for (int wv = 0; wv < 4; ++wv) {
+    if (wave == wv) {
+#pragma unroll
+      for (int q = 0; q < NT; ++q)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const int slot = (q * 4 + v) * 64 + lane;
+          red[slot] = (wv == 0) ? acc[q][v] : red[slot] + acc[q][v];
+        }
+    }
+    __syncthreads();
+  }
+  double* out = partials + (int64_t)blockIdx.x * TPA_GRAM64_SLOTS;
+  for (int j = t; j < NT * 256; j += blockDim.x) out[j] = red[j];
 }

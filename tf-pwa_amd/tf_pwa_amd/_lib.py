@@ -181,6 +181,7 @@ def lib():
         "tpa_timing_reset": ([], ctypes.c_int),
         "tpa_device_synchronize": ([], ctypes.c_int),
         "tpa_gram": ([vp, ctypes.c_double, dp, dp], ctypes.c_int),
+        "tpa_gram_f64": ([vp, dp, dp], ctypes.c_int),
         "tpa_calib_copy": ([i64, ctypes.c_int, dp], ctypes.c_int),
         "tpa_hess": ([vp, dp, ctypes.c_int, dp], ctypes.c_int),
         "tpa_pool_hess": ([vp, i64, i64, dp, dp], ctypes.c_int),
@@ -246,7 +247,7 @@ def exported_symbols():
         "tpa_events_size", "tpa_events_set_weight", "tpa_events_get_angles",
         "tpa_events_build", "tpa_events_get_pw", "tpa_eval", "tpa_timing_enable",
         "tpa_timing_read", "tpa_timing_reset", "tpa_device_synchronize", "tpa_calib_copy",
-        "tpa_gram", "tpa_hess", "tpa_pool_hess", "tpa_pool_create", "tpa_pool_destroy", "tpa_pool_add", "tpa_pool_release",
+        "tpa_gram", "tpa_gram_f64", "tpa_hess", "tpa_pool_hess", "tpa_pool_create", "tpa_pool_destroy", "tpa_pool_add", "tpa_pool_release",
         "tpa_pool_free_events", "tpa_pool_eval", "tpa_server_create", "tpa_server_destroy",
         "tpa_server_nll_grad", "tpa_server_nll_grad_hess", "tpa_server_submit", "tpa_server_poll", "tpa_server_result",
         "tpa_server_latencies", "tpa_server_counters", "tpa_fcn_create", "tpa_fcn_destroy",

@@ -190,17 +190,24 @@ class EventSet:
         _lib.check(_lib.lib().tpa_hess(self._h, _lib.dptr(c), mode, _lib.dptr(out)))
         return decode_second_moments(out, P)
 
-    def gram(self):
-        """Partial-wave Gram matrix I_kl = sum_e w_e sum_lambda conj(h_k) h_l (FP32 MFMA with
-        FP64 refolds), cached until the partial waves or the weights change."""
+    def gram(self, precision="fp64"):
+        """Partial-wave Gram matrix I_kl = sum_e w_e sum_lambda conj(h_k) h_l, cached until the
+        partial waves or the weights change. precision "fp32": FP32 MFMA with FP64 refolds (its
+        partial sums drift by up to ~1e-8 relative for narrow peaks over a broad MC sample);
+        "fp64": FP64 MFMA."""
+        if precision not in ("fp32", "fp64"):
+            raise ValueError("gram precision must be 'fp32' or 'fp64'")
         self.ensure_built()
-        key = (self._state, self._weight_version)
+        key = (self._state, self._weight_version, precision)
         if getattr(self, "_gram_key", None) != key:
             P = self.amp.P
             out = np.zeros(2 * P * P, dtype=np.float64)
             ms = ctypes.c_double(0)
-            _lib.check(_lib.lib().tpa_gram(self._h, float(np.mean(self._dev_weight)),
-                                           _lib.dptr(out), ctypes.byref(ms)))
+            if precision == "fp64":
+                _lib.check(_lib.lib().tpa_gram_f64(self._h, _lib.dptr(out), ctypes.byref(ms)))
+            else:
+                _lib.check(_lib.lib().tpa_gram(self._h, float(np.mean(self._dev_weight)),
+                                               _lib.dptr(out), ctypes.byref(ms)))
             self._gram = out.view(np.complex128).reshape(P, P).copy()
             self._gram_key = key
             self.gram_ms = ms.value

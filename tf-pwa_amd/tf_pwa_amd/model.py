@@ -65,13 +65,20 @@ class Model:
     """Model.nll / nll_grad / nll_grad_batch / nll_grad_hessian (model.py:576-790)."""
 
     def __init__(self, amp, w_bkg=1.0, resolution_size=1, extended=False, precision="fp64",
-                 **kwargs):
+                 gram_precision="fp64", **kwargs):
         """precision: "fp64" evaluates every MC event per call; "mixed" replaces the MC pass
         by the partial-wave Gram matrix built once on MFMA (valid while the partial waves,
-        i.e. lineshape parameters and MC weights, are unchanged; rebuilt otherwise)."""
+        i.e. lineshape parameters and MC weights, are unchanged; rebuilt otherwise).
+        gram_precision: the Gram's arithmetic in mixed precision, "fp64" (FP64 MFMA, the
+        default: the Gram is built once per lineshape state, so its cost is off the per-call
+        path) or "fp32" (FP32 MFMA with FP64 refolds; its partial sums drift by up to ~1e-8
+        relative for narrow peaks over a broad MC sample)."""
         if precision not in ("fp64", "mixed"):
             raise ValueError("precision must be 'fp64' or 'mixed'")
+        if gram_precision not in ("fp32", "fp64"):
+            raise ValueError("gram_precision must be 'fp32' or 'fp64'")
         self.precision = precision
+        self.gram_precision = gram_precision
         if resolution_size != 1:
             raise NotImplementedError("resolution folding is not supported")
         if extended:
@@ -124,7 +131,7 @@ class Model:
         for s, w in zip(sets, weights):
             s.set_weight(w)
             if mode == _lib.MODE_MC and self.precision == "mixed" and not shape:
-                I = s.gram()
+                I = s.gram(self.gram_precision)
                 Ic = I @ coef
                 v = float(np.real(np.vdot(coef, Ic)))
                 g = (np.conj(Ic),)
@@ -190,7 +197,7 @@ class Model:
         for s, w in zip(mc_sets, mw):
             s.set_weight(w)
             if self.precision == "mixed":
-                I = s.gram()
+                I = s.gram(self.gram_precision)
                 Ic = I @ coef
                 S += float(np.real(np.vdot(coef, Ic)))
                 Sk += np.conj(Ic)
@@ -306,7 +313,7 @@ class _NativeSession:
             I = np.zeros((amp.P, amp.P), dtype=np.complex128)
             for s, w in zip(mc_sets, mc_w):
                 s.set_weight(w)
-                I = I + s.gram()
+                I = I + s.gram(model.gram_precision)
             self._gram = np.ascontiguousarray(I).view(np.float64).ravel()
             gram = _lib.dptr(self._gram)
         else:
@@ -405,7 +412,7 @@ class FCN:
         sets = list(self.wdata.sets) + list(self.mc_sets)
         for st, w in zip(sets, list(dw) + list(mw)):
             st.set_weight(w)
-        key = (self.model.precision, self.vm.struct_version,
+        key = (self.model.precision, self.model.gram_precision, self.vm.struct_version,
                tuple(st._weight_version for st in sets))
         if self._native_key != key:
             if self._native is not None:

@@ -206,15 +206,16 @@ class FitServer:
     leaves (or ``capacity`` events when given). Fits whose data do not fit wait for others to end.
     ``hess_start``: each fit's BFGS starts from the inverse exact Hessian at its start point
     (one second-moment pass) rather than from the identity.
+    ``gram_precision``: "fp64" (FP64 MFMA) or "fp32" (FP32 MFMA with FP64 refolds) Gram.
     """
 
     def __init__(self, amp, mc, hbm_budget=200e9, capacity=None, max_batch=512,
-                 window_us=200.0, gtol=1e-5, maxiter=0, hess_start=False):
+                 window_us=200.0, gtol=1e-5, maxiter=0, hess_start=False, gram_precision="fp64"):
         self.amp = amp
         self.vm = amp.vm
         self.mc = mc
         mc.set_weight(mc.weight / mc.weight.sum())
-        self.gram = np.ascontiguousarray(mc.gram())
+        self.gram = np.ascontiguousarray(mc.gram(gram_precision))
         bpe = EventPool.bytes_per_event(amp)
         if capacity is None:
             capacity = int((hbm_budget - mc.n * bpe) // bpe)
