@@ -224,15 +224,16 @@ struct tpa_events_s {
 
 // --------------------------------------------------------------------------- dispatcher
 // Partial waves are split over KG lane groups so that each lane holds at most 8 of them.
-// Events per wave step: U event groups are loaded before any is processed, so more bytes
-// are in flight per wave. The cache is streamed with non-temporal loads (measured on the
-// MI355X: 5.97 against 5.55 TB/s at 10M events). TFPWA_EVAL_UNROLL selects U = 1, U = 2
-// (default) or, with 3, U = 2 with ordinary loads.
+// The default evaluation kernel is k_eval_pipe with PML = ceil(P / KG) rounded up to 4, 6 or
+// 8 (the register stages and accumulators of unused slots cost occupancy): on the MI355X it
+// streams 6.4 TB/s on 1M and 10M events, against 5.0-5.3 TB/s for k_eval_fold<.., U = 2>.
+// TFPWA_EVAL_UNROLL selects k_eval_fold instead, for comparison: 1 (U = 1), 2 (U = 2),
+// 3 (U = 2 with ordinary loads).
 static int eval_unroll() {
   static int u = [] {
     const char* v = getenv("TFPWA_EVAL_UNROLL");
-    int x = v ? atoi(v) : 2;
-    return (x == 1 || x == 2 || x == 3) ? x : 2;
+    int x = v ? atoi(v) : 0;
+    return (x >= 1 && x <= 3) ? x : 0;
   }();
   return u;
 }
@@ -240,6 +241,16 @@ static int eval_unroll() {
 template <int LG, int KG>
 static void launch_eval(const EvalArgs& a, int grid, hipStream_t s) {
   switch (eval_unroll()) {
+    case 0: {
+      const int need = (a.P + KG - 1) / KG;
+      if (need <= 4)
+        hipLaunchKernelGGL((k_eval_pipe<LG, KG, 4>), dim3(grid), dim3(256), 0, s, a);
+      else if (need <= 6)
+        hipLaunchKernelGGL((k_eval_pipe<LG, KG, 6>), dim3(grid), dim3(256), 0, s, a);
+      else
+        hipLaunchKernelGGL((k_eval_pipe<LG, KG, 8>), dim3(grid), dim3(256), 0, s, a);
+      break;
+    }
     case 1:
       hipLaunchKernelGGL((k_eval_fold<LG, KG, 8, 1, true>), dim3(grid), dim3(256), 0, s, a);
       break;
@@ -449,7 +460,13 @@ static void launch_build_pw(const tpa_model_s* m, const double* ang, int64_t n, 
 
 template <int LG, int KG>
 static void launch_multi(const MultiArgs& a, int grid, hipStream_t s) {
-  hipLaunchKernelGGL((k_eval_multi<LG, KG, 8>), dim3(grid), dim3(256), 0, s, a);
+  const int need = (a.P + KG - 1) / KG;  // PML as for the single-set pass
+  if (need <= 4)
+    hipLaunchKernelGGL((k_eval_multi<LG, KG, 4>), dim3(grid), dim3(256), 0, s, a);
+  else if (need <= 6)
+    hipLaunchKernelGGL((k_eval_multi<LG, KG, 6>), dim3(grid), dim3(256), 0, s, a);
+  else
+    hipLaunchKernelGGL((k_eval_multi<LG, KG, 8>), dim3(grid), dim3(256), 0, s, a);
 }
 
 template <int LG>

@@ -7,10 +7,11 @@
 //                  factor per LS coupling, the contraction over the resonance helicity and
 //                  the aligned D of spin>0 finals (amp/core.py:931-1164, 1392-1445,
 //                  dfun.py:118-266); optionally the Breit-Wigner is folded in.
-//   k_eval_fold    per evaluation: A_lambda = sum_k c_k h_k,lambda, f = sum |A|^2, the
+//   k_eval_pipe    per evaluation: A_lambda = sum_k c_k h_k,lambda, f = sum |A|^2, the
 //                  weighted log-sum (clip_log) or linear MC sum with a compensated
 //                  accumulator, and the adjoint sums G_k = sum_e u_e sum_lambda conj(A) h_k
-//                  (model/model.py:41-94, 283-292, 407-449).
+//                  (model/model.py:41-94, 283-292, 407-449); software-pipelined loads.
+//                  k_eval_fold is the same pass without the pipeline (kept for comparison).
 //   k_reduce       deterministic fixed-order reduction of the per-block partial sums.
 #pragma once
 #include <hip/hip_runtime.h>
@@ -473,6 +474,185 @@ __global__ __launch_bounds__(256) void k_eval_fold(EvalArgs a) {
   }
 }
 
+// Cross-lane exchange of a double with lane ^ M: DPP quad permutations for M = 1, 2 (no LDS
+// traffic), ds_swizzle's bit mode (xor within 32 lanes, no address operand) for M = 4 .. 16,
+// ds_bpermute beyond.
+template <int M>
+TPA_DEV double xor_lanes(double v) {
+  static_assert(M >= 1 && M < 64 && (M & (M - 1)) == 0, "M must be a power of two below 64");
+  int lo = __double2loint(v), hi = __double2hiint(v);
+  if constexpr (M == 1 || M == 2) {
+    constexpr int ctrl = M == 1 ? 0xB1 : 0x4E;  // quad_perm [1,0,3,2] / [2,3,0,1]
+    lo = __builtin_amdgcn_update_dpp(0, lo, ctrl, 0xF, 0xF, true);
+    hi = __builtin_amdgcn_update_dpp(0, hi, ctrl, 0xF, 0xF, true);
+  } else if constexpr (M < 32) {
+    constexpr int pattern = 0x1F | (M << 10);  // and_mask 31, or_mask 0, xor_mask M
+    lo = __builtin_amdgcn_ds_swizzle(lo, pattern);
+    hi = __builtin_amdgcn_ds_swizzle(hi, pattern);
+  } else {
+    return __shfl_xor(v, M, 64);
+  }
+  return __hiloint2double(hi, lo);
+}
+
+// Sum of v over the lanes lane ^ m for m = FROM, 2 FROM, .. < TO (a butterfly).
+template <int FROM, int TO>
+TPA_DEV double butterfly(double v) {
+  if constexpr (FROM < TO) {
+    v += xor_lanes<FROM>(v);
+    return butterfly<2 * FROM, TO>(v);
+  } else {
+    return v;
+  }
+}
+
+// One event group of the evaluation pass: A_lambda from the lane's PML partial waves, f, the
+// mode's value and derivative, the value into the compensated pair (vs, vc) of the event's
+// leader lane and the adjoint into (gr, gi). Lanes past the last event (valid = false) carry
+// finite stand-in waves and add exactly zero. Every lane of the wave must be active.
+template <int LG, int KG, int PML>
+TPA_DEV void eval_group(int mode, bool grad, double* __restrict__ f_out,
+                        const double2 (&hv)[PML], double w, int64_t e, bool valid, bool leader,
+                        const double (&cr)[PML], const double (&ci)[PML], double& vs, double& vc,
+                        double (&gr)[PML], double (&gi)[PML]) {
+  constexpr int LPE = LG * KG;
+  const double eps = 1.0e-6;
+  double ar = 0.0, ai = 0.0;
+#pragma unroll
+  for (int j = 0; j < PML; ++j) {
+    ar += cr[j] * hv[j].x - ci[j] * hv[j].y;
+    ai += cr[j] * hv[j].y + ci[j] * hv[j].x;
+  }
+  ar = butterfly<LG, LPE>(ar);
+  ai = butterfly<LG, LPE>(ai);
+  const double f = butterfly<1, LG>(ar * ar + ai * ai);
+  // lanes past the last event take weight zero (their f is finite: clamped waves); the weight
+  // is read outside any branch, so no path leaves its load pending
+  const double wv = valid ? w : 0.0;
+  double y, u;
+  if (mode == 0) {
+    // clip_log(f) (tf_pwa/model/model.py:283-292) weighted by w
+    double cl, dcl;
+    if (f > eps) {
+      cl = log(f);
+      dcl = 1.0 / f;
+    } else {
+      const double t = (f - eps) / eps;
+      cl = (log(eps) + t) - (t * t) / 2.0;
+      dcl = 1.0 / eps - t / eps;
+    }
+    y = wv * cl;
+    u = wv * dcl;
+  } else {
+    y = wv * f;
+    u = wv;
+  }
+  if (f_out != nullptr && valid && leader) f_out[e] = f;
+  neumaier_add(vs, vc, leader ? y : 0.0);
+  if (grad) {
+    const double tr = u * ar, ti = -(u * ai);
+#pragma unroll
+    for (int j = 0; j < PML; ++j) {
+      gr[j] += tr * hv[j].x - ti * hv[j].y;
+      gi[j] += tr * hv[j].y + ti * hv[j].x;
+    }
+  }
+}
+
+// The evaluation pass, software-pipelined: each wave holds two register stages of one event
+// group each (its partial waves and weights) and issues the loads of the next group before
+// it processes the current one, so its loads stay in flight while it computes. Loads are
+// unconditional (no branch around them, so the wait before a stage's first use counts only
+// the older loads): past the last event or partial wave they read a clamped, valid address,
+// whose values meet a zero coefficient or a zero weight. The loop bounds are wave-uniform
+// (scalar), so every lane stays active for the cross-lane exchanges. Requires N >= 1.
+template <int LG, int KG, int PML>
+__global__ __launch_bounds__(256) void k_eval_pipe(EvalArgs a) {
+  constexpr int LPE = LG * KG;
+  constexpr int EPW = 64 / LPE;
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  const int lam = lane & (LG - 1);
+  const int kg = (lane / LG) & (KG - 1);
+  const int el = lane / LPE;
+  const bool leader = lam == 0 && kg == 0;
+  double vs = 0.0, vc = 0.0;
+  double gr[PML], gi[PML], cr[PML], ci[PML];
+  const double2* base[PML];
+#pragma unroll
+  for (int j = 0; j < PML; ++j) {
+    const int k = kg + KG * j;
+    gr[j] = gi[j] = 0.0;
+    const double2 c = k < a.P ? a.coef[k] : make_double2(0.0, 0.0);
+    cr[j] = c.x;
+    ci[j] = c.y;
+    const int kc = k < a.P ? k : a.P - 1;
+    base[j] = a.h + (int64_t)kc * a.N * LG + lam;
+  }
+  const int64_t last = a.N - 1;
+  auto load = [&](double2 (&hv)[PML], double& wv, int64_t e) {
+    const int64_t ec = e < a.N ? e : last;
+#pragma unroll
+    for (int j = 0; j < PML; ++j) hv[j] = load_pw<true>(base[j] + ec * LG);
+    wv = a.w[ec];
+  };
+  const int64_t stride = (int64_t)gridDim.x * 4 * EPW;
+  double2 h0[PML], h1[PML];
+  double w0, w1;
+  const int wave_u = __builtin_amdgcn_readfirstlane(wave);
+  const int64_t first = ((int64_t)blockIdx.x * 4 + wave_u) * EPW;
+  int64_t e = first + el;
+  load(h0, w0, e);
+  for (int64_t b = first; b < a.N; b += 2 * stride, e += 2 * stride) {
+    load(h1, w1, e + stride);
+    // keep the next stage's loads ahead of this stage's first use (no scheduling across)
+    __builtin_amdgcn_sched_barrier(0);
+    eval_group<LG, KG, PML>(a.mode, a.grad, a.f_out, h0, w0, e, e < a.N, leader, cr, ci, vs, vc,
+                            gr, gi);
+    if (b + stride >= a.N) break;
+    load(h0, w0, e + 2 * stride);
+    __builtin_amdgcn_sched_barrier(0);
+    eval_group<LG, KG, PML>(a.mode, a.grad, a.f_out, h1, w1, e + stride, e + stride < a.N, leader,
+                            cr, ci, vs, vc, gr, gi);
+  }
+#pragma unroll
+  for (int m = 32; m >= 1; m >>= 1) {
+    const double s2 = __shfl_xor(vs, m, 64);
+    const double c2 = __shfl_xor(vc, m, 64);
+    neumaier_merge(vs, vc, s2, c2);
+  }
+  __shared__ double sh[4][TPA_NACC_MAX];
+  if (a.grad) {
+#pragma unroll
+    for (int j = 0; j < PML; ++j) {
+      double xr = butterfly<1, LG>(gr[j]), xi = butterfly<1, LG>(gi[j]);
+      xr = butterfly<LPE, 64>(xr);
+      xi = butterfly<LPE, 64>(xi);
+      const int k = kg + KG * j;
+      if (el == 0 && lam == 0 && k < a.P) {
+        sh[wave][2 + 2 * k] = xr;
+        sh[wave][3 + 2 * k] = xi;
+      }
+    }
+  }
+  if (lane == 0) {
+    sh[wave][0] = vs;
+    sh[wave][1] = vc;
+  }
+  __syncthreads();
+  double* out = a.partials + (int64_t)blockIdx.x * a.nacc;
+  if (threadIdx.x == 0) {
+    double s = sh[0][0], c = sh[0][1];
+    for (int wv = 1; wv < 4; ++wv) neumaier_merge(s, c, sh[wv][0], sh[wv][1]);
+    out[0] = s;
+    out[1] = c;
+  }
+  if (a.grad) {
+    for (int j = 2 + threadIdx.x; j < a.nacc; j += blockDim.x)
+      out[j] = ((sh[0][j] + sh[1][j]) + sh[2][j]) + sh[3][j];
+  }
+}
+
 // Fixed-order reduction over the per-block partial sums: one block per output slot. Slot 0/1
 // are the (sum, compensation) pair of the value and fold into out[0]; every other slot is a
 // compensated sum. Each thread reduces a fixed strided subset, then a fixed tree in LDS
@@ -850,13 +1030,13 @@ __global__ __launch_bounds__(256) void k_eval_multi(MultiArgs a) {
   const int lam = lane & (LG - 1);
   const int kg = (lane / LG) & (KG - 1);
   const int el = lane / LPE;
+  const bool leader = lam == 0 && kg == 0;
   const int item = a.chunk_item[chunk];
-  const int64_t end = a.chunk_end[chunk];
+  const int64_t end = a.chunk_end[chunk];  // > chunk_start: chunks are never empty
   const double2* __restrict__ coef = a.coef + (int64_t)item * a.P;
-  const double eps = 1.0e-6;
-  const double logeps = log(1.0e-6);
   double vs = 0.0, vc = 0.0;
   double gr[PML], gi[PML], cr[PML], ci[PML];
+  const double2* base[PML];
 #pragma unroll
   for (int j = 0; j < PML; ++j) {
     const int k = kg + KG * j;
@@ -864,61 +1044,35 @@ __global__ __launch_bounds__(256) void k_eval_multi(MultiArgs a) {
     const double2 c = k < a.P ? coef[k] : make_double2(0.0, 0.0);
     cr[j] = c.x;
     ci[j] = c.y;
+    const int kc = k < a.P ? k : a.P - 1;
+    base[j] = a.h + (int64_t)kc * a.S * LG + lam;
   }
-  for (int64_t base = a.chunk_start[chunk] + wave * EPW; base < end; base += 4 * EPW) {
-    const int64_t e = base + el;
-    const bool valid = e < end;
-    double2 hv[PML];
+  // the chunk's events, software-pipelined as in k_eval_pipe (two register stages, loads of
+  // the next group issued before the current one is processed, clamped unconditional loads)
+  const int64_t last = end - 1;
+  auto load = [&](double2 (&hv)[PML], double& wv, int64_t e) {
+    const int64_t ec = e < end ? e : last;
 #pragma unroll
-    for (int j = 0; j < PML; ++j) {
-      const int k = kg + KG * j;
-      hv[j] = (valid && k < a.P) ? load_pw<true>(&a.h[((int64_t)k * a.S + e) * LG + lam])
-                                 : make_double2(0.0, 0.0);
-    }
-    double ar = 0.0, ai = 0.0;
-#pragma unroll
-    for (int j = 0; j < PML; ++j) {
-      ar += cr[j] * hv[j].x - ci[j] * hv[j].y;
-      ai += cr[j] * hv[j].y + ci[j] * hv[j].x;
-    }
-#pragma unroll
-    for (int m = LG; m < LPE; m <<= 1) {
-      ar += __shfl_xor(ar, m, 64);
-      ai += __shfl_xor(ai, m, 64);
-    }
-    double f = ar * ar + ai * ai;
-#pragma unroll
-    for (int m = 1; m < LG; m <<= 1) f += __shfl_xor(f, m, 64);
-    double y = 0.0, u = 0.0;
-    if (valid) {
-      const double w = a.w[e];
-      if (a.mode == 0) {
-        const double dom = (w == 0.0) ? 1.0 : w;
-        const double x = (f * w) / dom;
-        double cl, dcl;
-        if (x > eps) {
-          cl = log(x);
-          dcl = 1.0 / x;
-        } else {
-          const double dx = x - eps;
-          const double t = dx / eps;
-          cl = (logeps + t) - (t * t) / 2.0;
-          dcl = 1.0 / eps - t / eps;
-        }
-        y = w * cl;
-        u = (w * dcl) * (w / dom);
-      } else {
-        y = w * f;
-        u = w;
-      }
-    }
-    if (lam == 0 && kg == 0) neumaier_add(vs, vc, y);
-    const double tr = u * ar, ti = -(u * ai);
-#pragma unroll
-    for (int j = 0; j < PML; ++j) {
-      gr[j] += tr * hv[j].x - ti * hv[j].y;
-      gi[j] += tr * hv[j].y + ti * hv[j].x;
-    }
+    for (int j = 0; j < PML; ++j) hv[j] = load_pw<true>(base[j] + ec * LG);
+    wv = a.w[ec];
+  };
+  constexpr int64_t stride = 4 * EPW;
+  double2 h0[PML], h1[PML];
+  double w0, w1;
+  const int wave_u = __builtin_amdgcn_readfirstlane(wave);
+  const int64_t first = a.chunk_start[chunk] + wave_u * EPW;
+  int64_t e = first + el;
+  load(h0, w0, e);
+  for (int64_t b = first; b < end; b += 2 * stride, e += 2 * stride) {
+    load(h1, w1, e + stride);
+    __builtin_amdgcn_sched_barrier(0);
+    eval_group<LG, KG, PML>(a.mode, true, nullptr, h0, w0, e, e < end, leader, cr, ci, vs, vc, gr,
+                            gi);
+    if (b + stride >= end) break;
+    load(h0, w0, e + 2 * stride);
+    __builtin_amdgcn_sched_barrier(0);
+    eval_group<LG, KG, PML>(a.mode, true, nullptr, h1, w1, e + stride, e + stride < end, leader,
+                            cr, ci, vs, vc, gr, gi);
   }
 #pragma unroll
   for (int m = 32; m >= 1; m >>= 1) {
@@ -929,17 +1083,9 @@ __global__ __launch_bounds__(256) void k_eval_multi(MultiArgs a) {
   __shared__ double sh[4][TPA_NACC_MAX];
 #pragma unroll
   for (int j = 0; j < PML; ++j) {
-    double xr = gr[j], xi = gi[j];
-#pragma unroll
-    for (int m = 1; m < LG; m <<= 1) {
-      xr += __shfl_xor(xr, m, 64);
-      xi += __shfl_xor(xi, m, 64);
-    }
-#pragma unroll
-    for (int m = LPE; m < 64; m <<= 1) {
-      xr += __shfl_xor(xr, m, 64);
-      xi += __shfl_xor(xi, m, 64);
-    }
+    double xr = butterfly<1, LG>(gr[j]), xi = butterfly<1, LG>(gi[j]);
+    xr = butterfly<LPE, 64>(xr);
+    xi = butterfly<LPE, 64>(xi);
     const int k = kg + KG * j;
     if (el == 0 && lam == 0 && k < a.P) {
       sh[wave][2 + 2 * k] = xr;
