@@ -28,11 +28,11 @@ CONFIG4 = os.path.join(ROOT, "tf-pwa_amd", "tf_pwa_amd", "configs", "jpsi_kkpipi
 METRIC = "NLL+grad evals/s @1M data+10M MC, |dNLL|<1e-3 vs FP64; toy-fit p50/p99 ms"
 HBM_PEAK_GBS = 8000.0
 ANGLE_BYTES_PER_EVENT = 176
-TRAFFIC = os.path.join(ROOT, "profiles", "k_eval_fold_traffic.json")
+TRAFFIC = os.path.join(ROOT, "profiles", "k_eval_pipe_traffic.json")
 
 
 def measured_traffic():
-    """HBM bytes per k_eval_fold launch from the committed FETCH_SIZE / WRITE_SIZE profile
+    """HBM bytes per k_eval_pipe launch from the committed FETCH_SIZE / WRITE_SIZE profile
     of this workload (counter values already scaled by the copy-kernel calibration)."""
     try:
         with open(TRAFFIC) as f:
@@ -306,6 +306,8 @@ def four_body_leg(args, seed=5505):
         del cand
         piece += 1
         log("four-body: candidate piece {}: {} data events so far".format(piece, got))
+    data = EventSet.concat(data_sets)  # one pass per evaluation over all accepted events
+    data_sets = None
     gc.collect()
     log("four-body: {} data events from {} candidates in {:.1f}s".format(
         got, piece * args.four_body_pool, time.perf_counter() - t))
@@ -314,7 +316,7 @@ def four_body_leg(args, seed=5505):
     def run(precision, gram="fp64"):
         model = conf.get_model()
         model.gram_precision = gram
-        fcn = FCN(model, data_sets, mc, precision=precision)
+        fcn = FCN(model, data, mc, precision=precision)
         for _ in range(args.warmup):
             fcn.nll_grad(x0)
         _lib.check(L.tpa_device_synchronize())
@@ -331,7 +333,7 @@ def four_body_leg(args, seed=5505):
     ge = float(np.linalg.norm(gmix - g64) / np.linalg.norm(g64))
     m32 = conf.get_model()
     m32.gram_precision = "fp32"
-    n32, gr32 = FCN(m32, data_sets, mc, precision="mixed").nll_grad(x0)
+    n32, gr32 = FCN(m32, data, mc, precision="mixed").nll_grad(x0)
     dn32 = abs(n32 - n64)
     ge32 = float(np.linalg.norm(gr32 - g64) / np.linalg.norm(g64))
     log("four-body: fp64 {:.1f} evals/s, mixed {:.1f} evals/s, |dNLL| {:.2e} (FP32-MFMA Gram: "
@@ -425,7 +427,7 @@ def main():
         survey = ANGLE_BYTES_PER_EVENT * n_events_per_step * args.steps / kernel_s / 1e9 \
             if kernel_s > 0 else 0.0
         return {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": achieved / HBM_PEAK_GBS, "traffic": tr, "kernel": "k_eval_fold",
+                "frac": achieved / HBM_PEAK_GBS, "traffic": tr, "kernel": "k_eval_pipe",
                 "angle_cache_bytes_per_event": ANGLE_BYTES_PER_EVENT,
                 "achieved_on_angle_cache_bytes": survey,
                 "launches": r["launches"], "kernel_ms_avg": r["kernel_ms"] / max(1, r["launches"]),

@@ -86,6 +86,11 @@ int tpa_events_generate(tpa_model_t model, double m_top, const double* masses, i
 int tpa_events_select(tpa_events_t src, const int64_t* idx, int64_t n, const double* weight,
                       tpa_events_t* out);
 
+/* A new event set made of all events of srcs[0..n_src) in order (their angles and weights;
+ * the partial-wave cache is built on demand), so that one pass covers them. Every source must
+ * belong to the same model. Replaces data_merge (tf_pwa/data.py:527-540) of event dicts. */
+int tpa_events_concat(const tpa_events_t* srcs, int n_src, tpa_events_t* out);
+
 int tpa_events_destroy(tpa_events_t ev);
 int64_t tpa_events_size(tpa_events_t ev);
 

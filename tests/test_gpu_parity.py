@@ -429,3 +429,31 @@ def test_device_phase_space_and_selection(cfg, gpu_available):
     sub = ev.select(idx, weight=np.full(777, 2.0))
     np.testing.assert_array_equal(sub.angles(), ev.angles()[:, idx])
     assert sub.n == 777 and np.all(sub.weight == 2.0)
+
+
+def test_concat_event_sets(case2, gpu_available):
+    """tpa_events_concat: the angles and weights of the sources side by side (bit-identical),
+    and one pass over the merged set equals the sum of the passes over its parts."""
+    from tf_pwa_amd import EventSet, _lib
+
+    conf = case2["conf"]
+    amp = conf.get_amplitude()
+    mc = conf.data_from_p4(case2["mc"])
+    rng = np.random.default_rng(5)
+    parts = [mc.select(np.arange(0, 1000), rng.uniform(0.5, 1.5, 1000)),
+             mc.select(np.arange(3000, 3001)),
+             mc.select(np.arange(1000, 2777), rng.uniform(-1.0, 1.0, 1777))]
+    merged = EventSet.concat(parts)
+    assert merged.n == 2778
+    np.testing.assert_array_equal(merged.angles(), np.concatenate([p.angles() for p in parts], 1))
+    np.testing.assert_array_equal(merged._dev_weight, np.concatenate([p._dev_weight for p in parts]))
+    c = amp.coefficients()
+    for mode in (_lib.MODE_DATA, _lib.MODE_MC):
+        v, G, _ = merged.eval(c, mode, grad=True)
+        ref = [p.eval(c, mode, grad=True) for p in parts]
+        vr = sum(r[0] for r in ref)
+        Gr = sum(r[1] for r in ref)
+        assert abs(v - vr) <= 1e-12 * sum(abs(r[0]) for r in ref)
+        assert np.abs(G - Gr).max() <= 1e-12 * sum(np.abs(r[1]).max() for r in ref)
+    with pytest.raises(Exception):
+        EventSet.concat([])

@@ -220,7 +220,13 @@ struct tpa_events_s {
   double2* d_hcoef = nullptr;
   int built_fold = -1;
   hipEvent_t t0 = nullptr, t1 = nullptr;
+  double* h_io = nullptr;  // pinned staging of the coefficients and the sums of tpa_eval
 };
+
+// Pinned staging of tpa_eval: 2 * TPA_MAX_P doubles of coefficients, then the sums (at most
+// 2 + 6 * TPA_MAX_P doubles). Copies from and to pageable caller memory would go through the
+// runtime's own staging on every call.
+#define TPA_IO_DOUBLES (2 * TPA_MAX_P + 2 + 6 * TPA_MAX_P)
 
 // --------------------------------------------------------------------------- dispatcher
 // Partial waves are split over KG lane groups so that each lane holds at most 8 of them.
@@ -337,6 +343,8 @@ static int eval_grid(int64_t n, int lanes_per_event) {
   if (g < 1) g = 1;
   return (int)std::min<int64_t>(g, gmax);
 }
+
+#define TPA_POOL_ALIGN 64  // events: the granule of pool ranges
 
 struct tpa_pool_s {
   std::mutex mu;  // one call at a time: the range maps, staging buffers and graphs are shared
@@ -657,6 +665,7 @@ int tpa_events_destroy(tpa_events_t ev) {
   if (ev->d_hcoef) (void)hipFree(ev->d_hcoef);
   if (ev->t0) (void)hipEventDestroy(ev->t0);
   if (ev->t1) (void)hipEventDestroy(ev->t1);
+  if (ev->h_io) (void)hipHostFree(ev->h_io);
   delete ev;
   return TPA_OK;
 }
@@ -801,6 +810,41 @@ int tpa_events_select(tpa_events_t src, const int64_t* idx, int64_t n, const dou
   return TPA_OK;
 }
 
+int tpa_events_concat(const tpa_events_t* srcs, int n_src, tpa_events_t* out) {
+  if (!srcs || !out) return fail(TPA_ERR_ARG, "null pointer");
+  if (n_src < 1) return fail(TPA_ERR_ARG, "no source event sets");
+  int64_t total = 0;
+  for (int i = 0; i < n_src; ++i) {
+    if (!srcs[i]) return fail(TPA_ERR_ARG, "null source event set");
+    if (srcs[i]->m != srcs[0]->m) return fail(TPA_ERR_ARG, "sources of different models");
+    total += srcs[i]->n;
+  }
+  tpa_model_s* m = srcs[0]->m;
+  tpa_events_s* ev = nullptr;
+  int rc = events_make(m, total, nullptr, nullptr, &ev);
+  if (rc != TPA_OK) return rc;
+  hipError_t e = hipSuccess;
+  int64_t off = 0;
+  for (int i = 0; i < n_src && e == hipSuccess; ++i) {
+    const tpa_events_s* src = srcs[i];
+    // angle cache [field][n]: a (nf x n_i) block at column off of the (nf x total) cache
+    e = hipMemcpy2DAsync(ev->d_ang + off, sizeof(double) * total, src->d_ang,
+                         sizeof(double) * src->n, sizeof(double) * src->n, ev->nf,
+                         hipMemcpyDeviceToDevice, m->stream);
+    if (e == hipSuccess)
+      e = hipMemcpyAsync(ev->d_w + off, src->d_w, sizeof(double) * src->n,
+                         hipMemcpyDeviceToDevice, m->stream);
+    off += src->n;
+  }
+  if (e == hipSuccess) e = hipStreamSynchronize(m->stream);
+  if (e != hipSuccess) {
+    tpa_events_destroy(ev);
+    return fail(TPA_ERR_HIP, std::string("concat: ") + hipGetErrorString(e));
+  }
+  *out = ev;
+  return TPA_OK;
+}
+
 int64_t tpa_events_size(tpa_events_t ev) { return ev ? ev->n : -1; }
 
 int tpa_events_set_weight(tpa_events_t ev, const double* weight) {
@@ -855,7 +899,11 @@ int tpa_eval(tpa_events_t ev, const double* coef, int mode, int flags, double* o
   const bool want_f = (flags & TPA_FLAG_FOUT) != 0;
   if (want_f && !f_out) return fail(TPA_ERR_ARG, "f_out required");
   if (want_f && !ev->d_f) HIPCHK(hipMalloc(&ev->d_f, sizeof(double) * ev->n));
-  HIPCHK(hipMemcpyAsync(ev->d_coef, coef, sizeof(double2) * M.P, hipMemcpyHostToDevice, s));
+  if (!ev->h_io) HIPCHK(hipHostMalloc((void**)&ev->h_io, sizeof(double) * TPA_IO_DOUBLES));
+  double* h_coef = ev->h_io;
+  double* h_out = ev->h_io + 2 * TPA_MAX_P;
+  std::memcpy(h_coef, coef, sizeof(double2) * M.P);
+  HIPCHK(hipMemcpyAsync(ev->d_coef, h_coef, sizeof(double2) * M.P, hipMemcpyHostToDevice, s));
   EvalArgs a;
   a.h = ev->d_h;
   a.w = ev->d_w;
@@ -908,10 +956,11 @@ int tpa_eval(tpa_events_t ev, const double* coef, int mode, int flags, double* o
   const int nout = a.grad ? a.nacc : 2;
   hipLaunchKernelGGL(k_reduce, dim3(nout), dim3(256), 0, s, ev->d_part, grid, a.nacc, ev->d_out);
   HIPCHK(hipGetLastError());
-  HIPCHK(hipMemcpyAsync(out, ev->d_out, sizeof(double) * nout, hipMemcpyDeviceToHost, s));
+  HIPCHK(hipMemcpyAsync(h_out, ev->d_out, sizeof(double) * nout, hipMemcpyDeviceToHost, s));
   if (want_f)
     HIPCHK(hipMemcpyAsync(f_out, ev->d_f, sizeof(double) * ev->n, hipMemcpyDeviceToHost, s));
   HIPCHK(hipStreamSynchronize(s));
+  std::memcpy(out, h_out, sizeof(double) * nout);
   if (timing) {
     float ms = 0.0f;
     HIPCHK(hipEventElapsedTime(&ms, ev->t0, ev->t1));
@@ -1154,9 +1203,13 @@ int tpa_pool_add(tpa_pool_t p, const double* p4, int64_t n, const double* weight
     if (it == p->free_ranges.end()) return fail(TPA_ERR_ARG, "pool full");
     off = it->first;
     const int64_t len = it->second;
+    // ranges are whole multiples of TPA_POOL_ALIGN events (only the pool's last range may
+    // end short of one), so every range and every 2048-event chunk starts on a cache-line
+    // boundary of each partial-wave row: the streaming loads of two chunks never share a line
+    const int64_t take = std::min<int64_t>(len, (n + TPA_POOL_ALIGN - 1) / TPA_POOL_ALIGN * TPA_POOL_ALIGN);
     p->free_ranges.erase(it);
-    if (len > n) p->free_ranges[off + n] = len - n;
-    p->used_ranges[off] = n;
+    if (len > take) p->free_ranges[off + take] = len - take;
+    p->used_ranges[off] = take;
   }
   auto undo = [&](hipError_t e, const char* what) {
     std::lock_guard<std::mutex> lk(p->mu);

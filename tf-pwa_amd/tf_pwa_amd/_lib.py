@@ -168,6 +168,7 @@ def lib():
         "tpa_events_destroy": ([vp], ctypes.c_int),
         "tpa_events_generate": ([vp, ctypes.c_double, dp, i64, ctypes.c_uint64, ctypes.c_uint64,
                                  dp, dp, ctypes.POINTER(vp)], ctypes.c_int),
+        "tpa_events_concat": ([ctypes.POINTER(vp), ctypes.c_int, ctypes.POINTER(vp)], ctypes.c_int),
         "tpa_events_select": ([vp, ctypes.POINTER(i64), i64, dp, ctypes.POINTER(vp)],
                               ctypes.c_int),
         "tpa_events_size": ([vp], i64),
@@ -243,7 +244,8 @@ def exported_symbols():
     return [
         "tpa_abi_version", "tpa_last_error", "tpa_device_count", "tpa_model_create",
         "tpa_model_destroy", "tpa_model_update", "tpa_model4_create", "tpa_model4_update",
-        "tpa_events_create", "tpa_events_generate", "tpa_events_select", "tpa_events_destroy",
+        "tpa_events_create", "tpa_events_generate", "tpa_events_select", "tpa_events_concat",
+        "tpa_events_destroy",
         "tpa_events_size", "tpa_events_set_weight", "tpa_events_get_angles",
         "tpa_events_build", "tpa_events_get_pw", "tpa_eval", "tpa_timing_enable",
         "tpa_timing_read", "tpa_timing_reset", "tpa_device_synchronize", "tpa_calib_copy",

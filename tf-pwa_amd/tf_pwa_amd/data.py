@@ -113,6 +113,23 @@ class EventSet:
             ctypes.POINTER(ctypes.c_int64)), idx.size, _lib.dptr(w), ctypes.byref(h)))
         return EventSet._wrap(self.amp, h, idx.size, w)
 
+    @classmethod
+    def concat(cls, sets):
+        """One event set holding the events of ``sets`` in order, with their weights
+        (tpa_events_concat; tf_pwa/data.py:527-540 data_merge): one pass per evaluation
+        instead of one per set."""
+        sets = list(sets)
+        if not sets:
+            raise ValueError("no event sets to concatenate")
+        amp = sets[0].amp
+        if any(s.amp is not amp for s in sets):
+            raise ValueError("event sets of different amplitude models")
+        handles = (ctypes.c_void_p * len(sets))(*[s._h.value for s in sets])
+        h = ctypes.c_void_p()
+        _lib.check(_lib.lib().tpa_events_concat(handles, len(sets), ctypes.byref(h)))
+        w = np.concatenate([s._dev_weight for s in sets])
+        return cls._wrap(amp, h, w.size, w)
+
     def __len__(self):
         return self.n
 
