@@ -14,15 +14,10 @@ matrix V the error of a fraction is sqrt(g^T V g) (FitFractions.get_frac, fitfra
 
 import numpy as np
 
-from . import _lib
-
 
 def gram_fp64(mc):
-    """FP64 Gram matrix I_kl = sum w conj(h_k) h_l of an event set (k_hess in MC mode; the
-    coefficients do not enter the second moment O2)."""
-    P = mc.amp.P
-    _, _, _, Dcb = mc.hess(np.ones(P, dtype=np.complex128), _lib.MODE_MC)
-    return Dcb.T.copy()  # Dcb_kl = sum w h_k conj(h_l) = I_lk
+    """FP64 Gram matrix I_kl = sum w conj(h_k) h_l of an event set (FP64 MFMA, tpa_gram_f64)."""
+    return mc.gram("fp64")
 
 
 def _groups(amp, res):
