@@ -10,6 +10,7 @@
 #include <vector>
 
 #include "tfpwa_amd.h"
+#include "tpa_internal.h"
 
 struct tpa_pool_s {
   int P = 0, lg = 0;
@@ -17,6 +18,9 @@ struct tpa_pool_s {
   std::vector<double> h;  // interleaved complex
   std::vector<double> w;
   int64_t evals = 0;
+  // batch slots of the pipelined evaluation: results are computed at submit time
+  std::vector<double> slot_out[2];
+  bool slot_busy[2] = {false, false};
 };
 
 static thread_local std::string g_err;
@@ -87,19 +91,17 @@ int tpa_pool_eval(tpa_pool_t p, int n_items, const int64_t* ranges, const double
         f += sr * sr + si * si;
       }
       const double w = p->w[e];
-      const double dom = (w == 0.0) ? 1.0 : w;
-      const double x = (f * w) / dom;
       double cl, dcl;
-      if (x > eps) {
-        cl = std::log(x);
-        dcl = 1.0 / x;
+      if (f > eps) {
+        cl = std::log(f);
+        dcl = 1.0 / f;
       } else {
-        const double t = (x - eps) / eps;
+        const double t = (f - eps) / eps;
         cl = (logeps + t) - (t * t) / 2.0;
         dcl = 1.0 / eps - t / eps;
       }
       o[0] += w * cl;
-      const double u = (w * dcl) * (w / dom);
+      const double u = w * dcl;
       for (int k = 0; k < P; ++k)
         for (int l = 0; l < lg; ++l) {
           const double* hv = &p->h[2 * (((size_t)k * p->cap + e) * lg + l)];
@@ -180,3 +182,30 @@ int tpa_pool_hess(tpa_pool_t p, int64_t off, int64_t n, const double* c, double*
 }
 
 }  // extern "C"
+
+// The pipelined evaluation of the serving batcher (declared in tpa_internal.h): the sums are
+// computed at submit time into the slot and handed out by finish.
+int tpa_pool_eval_submit(tpa_pool_t p, int* slot_io, int n_items, const int64_t* ranges,
+                         const double* coefs, int mode) {
+  int si = *slot_io;
+  if (si < 0)
+    for (int i = 0; i < 2 && si < 0; ++i)
+      if (!p->slot_busy[i]) si = i;
+  if (si < 0 || si > 1 || p->slot_busy[si]) return tpa_set_error(-1, "every batch slot is in flight");
+  p->slot_out[si].assign((size_t)n_items * (2 + 2 * p->P), 0.0);
+  const int rc = tpa_pool_eval(p, n_items, ranges, coefs, mode, p->slot_out[si].data(), nullptr);
+  if (rc != 0) return rc;
+  p->slot_busy[si] = true;
+  *slot_io = si;
+  return 0;
+}
+
+int tpa_pool_eval_ready(tpa_pool_t p, int slot) { return p->slot_busy[slot] ? 1 : -1; }
+
+int tpa_pool_eval_finish(tpa_pool_t p, int slot, double* out, double* ms) {
+  if (!p->slot_busy[slot]) return tpa_set_error(-1, "no batch in this slot");
+  std::memcpy(out, p->slot_out[slot].data(), sizeof(double) * p->slot_out[slot].size());
+  p->slot_busy[slot] = false;
+  if (ms) *ms = 0.0;
+  return 0;
+}

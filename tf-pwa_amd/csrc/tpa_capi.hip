@@ -345,6 +345,7 @@ static int eval_grid(int64_t n, int lanes_per_event) {
 }
 
 #define TPA_POOL_ALIGN 64  // events: the granule of pool ranges
+#define TPA_POOL_SLOTS 2   // batches that can be in flight on a pool at once
 
 struct tpa_pool_s {
   std::mutex mu;  // one call at a time: the range maps, staging buffers and graphs are shared
@@ -356,23 +357,29 @@ struct tpa_pool_s {
   std::map<int64_t, int64_t> free_ranges;  // offset -> length
   std::map<int64_t, int64_t> used_ranges;
   int max_items = 0, max_chunks = 0;
-  int* d_chunk_item = nullptr;
-  int64_t* d_chunk_start = nullptr;
-  int64_t* d_chunk_end = nullptr;
-  int* d_counts = nullptr;
-  int* d_item_chunk = nullptr;
-  double2* d_coef = nullptr;
-  double* d_part = nullptr;
-  double* d_out = nullptr;
-  int* h_chunk_item = nullptr;
-  int64_t* h_chunk_start = nullptr;
-  int64_t* h_chunk_end = nullptr;
-  int* h_counts = nullptr;
-  int* h_item_chunk = nullptr;
-  double2* h_coef = nullptr;
-  double* h_out = nullptr;
-  std::map<int, hipGraphExec_t> graphs;  // grid bucket -> instantiated graph
-  hipEvent_t t0 = nullptr, t1 = nullptr;
+  // Batched evaluations run through TPA_POOL_SLOTS independent sets of work lists, partials
+  // and results, so one batch can be queued on the stream while the previous one runs.
+  struct Slot {
+    int* d_chunk_item = nullptr;
+    int64_t* d_chunk_start = nullptr;
+    int64_t* d_chunk_end = nullptr;
+    int* d_counts = nullptr;
+    int* d_item_chunk = nullptr;
+    double2* d_coef = nullptr;
+    double* d_part = nullptr;
+    double* d_out = nullptr;
+    int* h_chunk_item = nullptr;
+    int64_t* h_chunk_start = nullptr;
+    int64_t* h_chunk_end = nullptr;
+    int* h_counts = nullptr;
+    int* h_item_chunk = nullptr;
+    double2* h_coef = nullptr;
+    double* h_out = nullptr;
+    std::map<int, hipGraphExec_t> graphs;  // grid bucket -> instantiated graph
+    hipEvent_t t0 = nullptr, t1 = nullptr, done = nullptr;
+    int n_items = 0;
+    bool busy = false;  // submitted and not yet finished
+  } slot[TPA_POOL_SLOTS];
   hipStream_t stream = nullptr;  // the pool's own stream (graph capture never meets other work)
   // adds: serialised among themselves, overlapping the batched evaluations on their own stream
   std::mutex add_mu;
@@ -1109,17 +1116,24 @@ int tpa_gram_f64(tpa_events_t ev, double* out, double* ms) {
 // ------------------------------------------------------------------------ serving pool
 int tpa_pool_destroy(tpa_pool_t p) {
   if (!p) return TPA_OK;
-  for (auto& g : p->graphs) (void)hipGraphExecDestroy(g.second);
-  void* dev[] = {p->d_ang, p->d_h, p->d_w, p->d_chunk_item, p->d_chunk_start, p->d_chunk_end,
-                 p->d_counts, p->d_item_chunk, p->d_coef, p->d_part, p->d_out};
+  if (p->stream) (void)hipStreamSynchronize(p->stream);
+  for (auto& q : p->slot) {
+    for (auto& g : q.graphs) (void)hipGraphExecDestroy(g.second);
+    void* dev[] = {q.d_chunk_item, q.d_chunk_start, q.d_chunk_end, q.d_counts, q.d_item_chunk,
+                   q.d_coef, q.d_part, q.d_out};
+    for (void* x : dev)
+      if (x) (void)hipFree(x);
+    void* host[] = {q.h_chunk_item, q.h_chunk_start, q.h_chunk_end, q.h_counts, q.h_item_chunk,
+                    q.h_coef, q.h_out};
+    for (void* x : host)
+      if (x) (void)hipHostFree(x);
+    hipEvent_t evs[] = {q.t0, q.t1, q.done};
+    for (hipEvent_t x : evs)
+      if (x) (void)hipEventDestroy(x);
+  }
+  void* dev[] = {p->d_ang, p->d_h, p->d_w};
   for (void* x : dev)
     if (x) (void)hipFree(x);
-  void* host[] = {p->h_chunk_item, p->h_chunk_start, p->h_chunk_end, p->h_counts,
-                  p->h_item_chunk, p->h_coef, p->h_out};
-  for (void* x : host)
-    if (x) (void)hipHostFree(x);
-  if (p->t0) (void)hipEventDestroy(p->t0);
-  if (p->t1) (void)hipEventDestroy(p->t1);
   if (p->stream) (void)hipStreamDestroy(p->stream);
   if (p->add_stream) (void)hipStreamDestroy(p->add_stream);
   if (p->d_stage) (void)hipFree(p->d_stage);
@@ -1153,23 +1167,26 @@ int tpa_pool_create(tpa_model_t m, int64_t capacity, int max_items, tpa_pool_t* 
   D((void**)&p->d_ang, sizeof(double) * nf * capacity);
   D((void**)&p->d_h, sizeof(double2) * (size_t)M.P * capacity * M.lam_pad);
   D((void**)&p->d_w, sizeof(double) * capacity);
-  D((void**)&p->d_chunk_item, sizeof(int) * p->max_chunks);
-  D((void**)&p->d_chunk_start, sizeof(int64_t) * p->max_chunks);
-  D((void**)&p->d_chunk_end, sizeof(int64_t) * p->max_chunks);
-  D((void**)&p->d_counts, sizeof(int) * 2);
-  D((void**)&p->d_item_chunk, sizeof(int) * (max_items + 1));
-  D((void**)&p->d_coef, sizeof(double2) * (size_t)max_items * M.P);
-  D((void**)&p->d_part, sizeof(double) * (size_t)p->max_chunks * nacc);
-  D((void**)&p->d_out, sizeof(double) * (size_t)max_items * nacc);
-  H((void**)&p->h_chunk_item, sizeof(int) * p->max_chunks);
-  H((void**)&p->h_chunk_start, sizeof(int64_t) * p->max_chunks);
-  H((void**)&p->h_chunk_end, sizeof(int64_t) * p->max_chunks);
-  H((void**)&p->h_counts, sizeof(int) * 2);
-  H((void**)&p->h_item_chunk, sizeof(int) * (max_items + 1));
-  H((void**)&p->h_coef, sizeof(double2) * (size_t)max_items * M.P);
-  H((void**)&p->h_out, sizeof(double) * (size_t)max_items * nacc);
-  if (e == hipSuccess) e = hipEventCreate(&p->t0);
-  if (e == hipSuccess) e = hipEventCreate(&p->t1);
+  for (auto& q : p->slot) {
+    D((void**)&q.d_chunk_item, sizeof(int) * p->max_chunks);
+    D((void**)&q.d_chunk_start, sizeof(int64_t) * p->max_chunks);
+    D((void**)&q.d_chunk_end, sizeof(int64_t) * p->max_chunks);
+    D((void**)&q.d_counts, sizeof(int) * 2);
+    D((void**)&q.d_item_chunk, sizeof(int) * (max_items + 1));
+    D((void**)&q.d_coef, sizeof(double2) * (size_t)max_items * M.P);
+    D((void**)&q.d_part, sizeof(double) * (size_t)p->max_chunks * nacc);
+    D((void**)&q.d_out, sizeof(double) * (size_t)max_items * nacc);
+    H((void**)&q.h_chunk_item, sizeof(int) * p->max_chunks);
+    H((void**)&q.h_chunk_start, sizeof(int64_t) * p->max_chunks);
+    H((void**)&q.h_chunk_end, sizeof(int64_t) * p->max_chunks);
+    H((void**)&q.h_counts, sizeof(int) * 2);
+    H((void**)&q.h_item_chunk, sizeof(int) * (max_items + 1));
+    H((void**)&q.h_coef, sizeof(double2) * (size_t)max_items * M.P);
+    H((void**)&q.h_out, sizeof(double) * (size_t)max_items * nacc);
+    if (e == hipSuccess) e = hipEventCreate(&q.t0);
+    if (e == hipSuccess) e = hipEventCreate(&q.t1);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&q.done, hipEventDisableTiming);
+  }
   if (e == hipSuccess) e = hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking);
   if (e == hipSuccess) e = hipStreamCreateWithFlags(&p->add_stream, hipStreamNonBlocking);
   if (e != hipSuccess) {
@@ -1278,12 +1295,24 @@ static void release_range(tpa_pool_s* p, int64_t offset) {
   p->free_ranges[off] = len;
 }
 
-int tpa_pool_eval(tpa_pool_t p, int n_items, const int64_t* ranges, const double* coefs,
-                  int mode, double* out, double* ms) {
-  if (!p || !ranges || !coefs || !out) return fail(TPA_ERR_ARG, "null pointer");
+}  // extern "C"
+
+// Queue one batched evaluation on a batch slot of the pool's stream and return without
+// waiting. *slot_io: the slot to use, or -1 for any free one (the chosen slot is returned).
+int tpa_pool_eval_submit(tpa_pool_t p, int* slot_io, int n_items, const int64_t* ranges,
+                         const double* coefs, int mode) {
+  if (!p || !slot_io || !ranges || !coefs) return fail(TPA_ERR_ARG, "null pointer");
+  if (*slot_io >= TPA_POOL_SLOTS) return fail(TPA_ERR_ARG, "bad batch slot");
   if (n_items < 1 || n_items > p->max_items) return fail(TPA_ERR_ARG, "bad request count");
   if (mode != TPA_MODE_DATA) return fail(TPA_ERR_UNSUPPORTED, "batched evaluation serves data sums");
   std::lock_guard<std::mutex> lk(p->mu);
+  int si = *slot_io;
+  if (si < 0)
+    for (int i = 0; i < TPA_POOL_SLOTS && si < 0; ++i)
+      if (!p->slot[i].busy) si = i;
+  if (si < 0) return fail(TPA_ERR_ARG, "every batch slot is in flight");
+  auto& q = p->slot[si];
+  if (q.busy) return fail(TPA_ERR_ARG, "batch slot still in flight");
   const TpaModel& M = p->m->h;
   const int nacc = 2 + 2 * M.P;
   int nch = 0;
@@ -1294,39 +1323,39 @@ int tpa_pool_eval(tpa_pool_t p, int n_items, const int64_t* ranges, const double
     --it;
     if (off < it->first || off + n > it->first + it->second)
       return fail(TPA_ERR_ARG, "range outside an allocation");
-    p->h_item_chunk[i] = nch;
+    q.h_item_chunk[i] = nch;
     for (int64_t st = off; st < off + n; st += TPA_CHUNK) {
       if (nch >= p->max_chunks) return fail(TPA_ERR_ARG, "too many chunks in one batch");
-      p->h_chunk_item[nch] = i;
-      p->h_chunk_start[nch] = st;
-      p->h_chunk_end[nch] = std::min<int64_t>(st + TPA_CHUNK, off + n);
+      q.h_chunk_item[nch] = i;
+      q.h_chunk_start[nch] = st;
+      q.h_chunk_end[nch] = std::min<int64_t>(st + TPA_CHUNK, off + n);
       ++nch;
     }
   }
-  p->h_item_chunk[n_items] = nch;
-  p->h_counts[0] = nch;
-  p->h_counts[1] = n_items;
-  std::memcpy(p->h_coef, coefs, sizeof(double2) * (size_t)n_items * M.P);
+  q.h_item_chunk[n_items] = nch;
+  q.h_counts[0] = nch;
+  q.h_counts[1] = n_items;
+  std::memcpy(q.h_coef, coefs, sizeof(double2) * (size_t)n_items * M.P);
   hipStream_t s = p->stream;
-  HIPCHK(hipMemcpyAsync(p->d_chunk_item, p->h_chunk_item, sizeof(int) * nch, hipMemcpyHostToDevice, s));
-  HIPCHK(hipMemcpyAsync(p->d_chunk_start, p->h_chunk_start, sizeof(int64_t) * nch, hipMemcpyHostToDevice, s));
-  HIPCHK(hipMemcpyAsync(p->d_chunk_end, p->h_chunk_end, sizeof(int64_t) * nch, hipMemcpyHostToDevice, s));
-  HIPCHK(hipMemcpyAsync(p->d_item_chunk, p->h_item_chunk, sizeof(int) * (n_items + 1), hipMemcpyHostToDevice, s));
-  HIPCHK(hipMemcpyAsync(p->d_counts, p->h_counts, sizeof(int) * 2, hipMemcpyHostToDevice, s));
-  HIPCHK(hipMemcpyAsync(p->d_coef, p->h_coef, sizeof(double2) * (size_t)n_items * M.P, hipMemcpyHostToDevice, s));
+  HIPCHK(hipMemcpyAsync(q.d_chunk_item, q.h_chunk_item, sizeof(int) * nch, hipMemcpyHostToDevice, s));
+  HIPCHK(hipMemcpyAsync(q.d_chunk_start, q.h_chunk_start, sizeof(int64_t) * nch, hipMemcpyHostToDevice, s));
+  HIPCHK(hipMemcpyAsync(q.d_chunk_end, q.h_chunk_end, sizeof(int64_t) * nch, hipMemcpyHostToDevice, s));
+  HIPCHK(hipMemcpyAsync(q.d_item_chunk, q.h_item_chunk, sizeof(int) * (n_items + 1), hipMemcpyHostToDevice, s));
+  HIPCHK(hipMemcpyAsync(q.d_counts, q.h_counts, sizeof(int) * 2, hipMemcpyHostToDevice, s));
+  HIPCHK(hipMemcpyAsync(q.d_coef, q.h_coef, sizeof(double2) * (size_t)n_items * M.P, hipMemcpyHostToDevice, s));
   int bucket = 64;
   while (bucket < nch) bucket <<= 1;
-  auto g = p->graphs.find(bucket);
-  if (g == p->graphs.end()) {
+  auto g = q.graphs.find(bucket);
+  if (g == q.graphs.end()) {
     MultiArgs a;
     a.h = p->d_h;
     a.w = p->d_w;
-    a.coef = p->d_coef;
-    a.chunk_item = p->d_chunk_item;
-    a.chunk_start = p->d_chunk_start;
-    a.chunk_end = p->d_chunk_end;
-    a.counts = p->d_counts;
-    a.partials = p->d_part;
+    a.coef = q.d_coef;
+    a.chunk_item = q.d_chunk_item;
+    a.chunk_start = q.d_chunk_start;
+    a.chunk_end = q.d_chunk_end;
+    a.counts = q.d_counts;
+    a.partials = q.d_part;
     a.S = p->cap;
     a.P = M.P;
     a.mode = mode;
@@ -1336,8 +1365,8 @@ int tpa_pool_eval(tpa_pool_t p, int n_items, const int64_t* ranges, const double
     hipGraph_t graph;
     HIPCHK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
     int rc = dispatch_multi(M.lam_pad, kg, a, bucket, s);
-    hipLaunchKernelGGL(k_reduce_items, dim3(p->max_items), dim3(256), 0, s, p->d_part,
-                       p->d_item_chunk, p->d_counts, nacc, p->d_out);
+    hipLaunchKernelGGL(k_reduce_items, dim3(p->max_items), dim3(256), 0, s, q.d_part,
+                       q.d_item_chunk, q.d_counts, nacc, q.d_out);
     hipError_t ce = hipStreamEndCapture(s, &graph);
     if (rc != TPA_OK) return fail(rc, "no batched kernel for this helicity count");
     if (ce != hipSuccess) return fail(TPA_ERR_HIP, std::string("capture: ") + hipGetErrorString(ce));
@@ -1345,20 +1374,59 @@ int tpa_pool_eval(tpa_pool_t p, int n_items, const int64_t* ranges, const double
     hipError_t ie = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
     (void)hipGraphDestroy(graph);
     if (ie != hipSuccess) return fail(TPA_ERR_HIP, std::string("instantiate: ") + hipGetErrorString(ie));
-    g = p->graphs.emplace(bucket, exec).first;
+    g = q.graphs.emplace(bucket, exec).first;
   }
-  HIPCHK(hipEventRecord(p->t0, s));
+  HIPCHK(hipEventRecord(q.t0, s));
   HIPCHK(hipGraphLaunch(g->second, s));
-  HIPCHK(hipEventRecord(p->t1, s));
-  HIPCHK(hipMemcpyAsync(p->h_out, p->d_out, sizeof(double) * (size_t)n_items * nacc, hipMemcpyDeviceToHost, s));
-  HIPCHK(hipStreamSynchronize(s));
-  if (ms) {
-    float f = 0.0f;
-    HIPCHK(hipEventElapsedTime(&f, p->t0, p->t1));
-    *ms = f;
-  }
-  std::memcpy(out, p->h_out, sizeof(double) * (size_t)n_items * nacc);
+  HIPCHK(hipEventRecord(q.t1, s));
+  HIPCHK(hipMemcpyAsync(q.h_out, q.d_out, sizeof(double) * (size_t)n_items * nacc, hipMemcpyDeviceToHost, s));
+  HIPCHK(hipEventRecord(q.done, s));
+  q.n_items = n_items;
+  q.busy = true;
+  *slot_io = si;
   return TPA_OK;
+}
+
+// 1 when slot `si`'s batch has completed (its results are on the host), 0 while it runs.
+int tpa_pool_eval_ready(tpa_pool_t p, int si) {
+  if (!p || si < 0 || si >= TPA_POOL_SLOTS) return fail(TPA_ERR_ARG, "bad batch slot");
+  auto& q = p->slot[si];
+  if (!q.busy) return fail(TPA_ERR_ARG, "no batch in this slot");
+  const hipError_t e = hipEventQuery(q.done);
+  if (e == hipSuccess) return 1;
+  if (e == hipErrorNotReady) return 0;
+  return fail(TPA_ERR_HIP, std::string("batch: ") + hipGetErrorString(e));
+}
+
+// Wait for slot `si`'s batch; its results (n_items x (2 + 2P)) go to out, the graph's GPU time
+// to ms. Frees the slot.
+int tpa_pool_eval_finish(tpa_pool_t p, int si, double* out, double* ms) {
+  if (!p || !out || si < 0 || si >= TPA_POOL_SLOTS) return fail(TPA_ERR_ARG, "bad batch slot");
+  auto& q = p->slot[si];
+  if (!q.busy) return fail(TPA_ERR_ARG, "no batch in this slot");
+  const hipError_t e = hipEventSynchronize(q.done);
+  float f = 0.0f;
+  const hipError_t et = e == hipSuccess ? hipEventElapsedTime(&f, q.t0, q.t1) : e;
+  const int nacc = 2 + 2 * p->m->h.P;
+  if (e == hipSuccess) std::memcpy(out, q.h_out, sizeof(double) * (size_t)q.n_items * nacc);
+  {
+    std::lock_guard<std::mutex> lk(p->mu);
+    q.busy = false;
+  }
+  if (e != hipSuccess) return fail(TPA_ERR_HIP, std::string("batch: ") + hipGetErrorString(e));
+  if (ms) *ms = et == hipSuccess ? f : 0.0;
+  return TPA_OK;
+}
+
+extern "C" {
+
+int tpa_pool_eval(tpa_pool_t p, int n_items, const int64_t* ranges, const double* coefs,
+                  int mode, double* out, double* ms) {
+  if (!p || !ranges || !coefs || !out) return fail(TPA_ERR_ARG, "null pointer");
+  int si = -1;  // any free slot (a serving batcher may hold the others)
+  const int rc = tpa_pool_eval_submit(p, &si, n_items, ranges, coefs, mode);
+  if (rc != TPA_OK) return rc;
+  return tpa_pool_eval_finish(p, si, out, ms);
 }
 
 int tpa_hess(tpa_events_t ev, const double* coef, int mode, double* out) {

@@ -1,6 +1,20 @@
 // Internal glue between the library's translation units (not part of the C ABI).
 #pragma once
+#include <stdint.h>
+
 #include <string>
+
+#include "tfpwa_amd.h"
 
 // Sets the calling thread's tpa_last_error() message and returns code.
 int tpa_set_error(int code, const std::string& msg);
+
+// Pipelined batched evaluation on an event pool (tpa_pool_eval split in two): submit queues
+// a batch on one of the pool's batch slots (*slot_io = -1: any free slot; the slot used is
+// returned) without waiting; ready polls it (1 done, 0 running); finish waits, copies the
+// n_items x (2 + 2P) sums to out and frees the slot. A slot belongs to its submitter until
+// finished.
+int tpa_pool_eval_submit(tpa_pool_t p, int* slot_io, int n_items, const int64_t* ranges,
+                         const double* coefs, int mode);
+int tpa_pool_eval_ready(tpa_pool_t p, int slot);
+int tpa_pool_eval_finish(tpa_pool_t p, int slot, double* out, double* ms);

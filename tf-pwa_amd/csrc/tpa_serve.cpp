@@ -5,7 +5,8 @@
 // (FCN.nll_grad, tf_pwa/model/model.py:1216-1240) computes the partial-wave coefficients
 // c_k = const_k * v_total * v_g1 * v_g2 from the fit's parameters, hands them to one batcher
 // thread and waits. The batcher gathers the pending requests of all fits and evaluates them in
-// ONE tpa_pool_eval launch (data sum and its adjoint per request); the MC normalisation is the
+// ONE batched pool launch (data sum and its adjoint per request), with the next batch queued
+// behind the running one; the MC normalisation is the
 // shared partial-wave Gram matrix I, so int_mc = c^dagger I c and its adjoint conj(I c) cost
 // O(P^2) on the host. The gradient in the fit's parameters follows from the adjoints by the
 // chain rule through the three complex factors of each c_k.
@@ -380,58 +381,132 @@ void fit_main(tpa_server_s* S, Fit* F) {
 }
 
 // ------------------------------------------------------------------ batcher
-void batcher_main(tpa_server_s* S) {
-  const int P = S->P;
-  const int nacc = 2 + 2 * P;
-  std::vector<Fit*> batch;
+// One batch on its way through the pool: the requests, their ranges and coefficients, and the
+// pool batch slot it occupies while it runs.
+struct Batch {
+  std::vector<Fit*> fits;
   std::vector<int64_t> ranges;
   std::vector<double> coefs, out;
+  int slot = -1;
+  bool failed = false;
+};
+
+// Hand a finished batch's sums to its fits (server mutex held).
+void deliver(tpa_server_s* S, Batch& B, double ms, const std::string& err) {
+  const int nacc = 2 + 2 * S->P;
+  const int m = (int)B.fits.size();
+  const auto now = Clock::now();
+  S->n_batches += 1;
+  S->n_requests += m;
+  for (int i = 0; i < m; ++i) S->n_events += B.ranges[2 * i + 1];
+  S->gpu_ms += ms;
+  if (B.failed) S->batch_error = err;
+  for (int i = 0; i < m; ++i) {
+    Fit* F = B.fits[i];
+    if (!B.failed)
+      F->out.assign(B.out.begin() + (size_t)nacc * i, B.out.begin() + (size_t)nacc * (i + 1));
+    F->failed = B.failed;
+    F->ready = true;
+    S->lat_us.push_back(std::chrono::duration<double, std::micro>(now - F->t_submit).count());
+    F->cv.notify_one();
+  }
+  B.fits.clear();
+  B.slot = -1;
+}
+
+// Wait for a running batch and deliver it (called and returning with the mutex held; the wait
+// itself runs unlocked).
+void complete(tpa_server_s* S, Batch& B, std::unique_lock<std::mutex>& lk) {
+  lk.unlock();
+  double ms = 0.0;
+  B.out.resize((size_t)(2 + 2 * S->P) * B.fits.size());
+  const int rc = tpa_pool_eval_finish(S->pool, B.slot, B.out.data(), &ms);
+  const std::string err = rc == TPA_OK ? std::string() : std::string(tpa_last_error());
+  lk.lock();
+  B.failed = rc != TPA_OK;
+  deliver(S, B, ms, err);
+}
+
+// The batcher keeps up to two batches on the GPU: while one runs, the requests of the other
+// fits are gathered and the next batch is queued behind it, so the GPU does not idle between
+// batches and a request does not wait for the batch ahead to return before it is launched.
+// A batch launches when every fit outside the running batch is waiting, when max_batch
+// requests are pending, or window_us after the first of them.
+void batcher_main(tpa_server_s* S) {
+  const int P = S->P;
+  Batch bt[2];
+  int run = -1;  // index in bt of the batch on the GPU (-1: none)
+  const auto poll = std::chrono::microseconds(20);
   std::unique_lock<std::mutex> lk(S->mu);
   for (;;) {
-    S->cv.wait(lk, [&] { return S->stop || !S->pending.empty(); });
-    if (S->stop) break;
-    // gather: until every client is waiting, the batch is full, or the window has passed
-    const auto deadline = Clock::now() + std::chrono::duration<double, std::micro>(S->window_us);
-    while (!S->stop && (int)S->pending.size() < S->clients &&
-           (int)S->pending.size() < S->max_batch) {
-      if (S->cv.wait_until(lk, deadline) == std::cv_status::timeout) break;
+    // wait for requests; meanwhile deliver the running batch as soon as it is done
+    while (!S->stop && S->pending.empty()) {
+      if (run >= 0) {
+        if (tpa_pool_eval_ready(S->pool, bt[run].slot) != 0) {
+          complete(S, bt[run], lk);
+          run = -1;
+          continue;
+        }
+        S->cv.wait_for(lk, poll);
+      } else {
+        S->cv.wait(lk);
+      }
     }
     if (S->stop) break;
-    batch.clear();
-    while (!S->pending.empty() && (int)batch.size() < S->max_batch) {
-      batch.push_back(S->pending.front());
+    // gather
+    const Clock::time_point deadline =
+        Clock::now() + std::chrono::duration_cast<Clock::duration>(
+                           std::chrono::duration<double, std::micro>(S->window_us));
+    for (;;) {
+      const int busy = run >= 0 ? (int)bt[run].fits.size() : 0;
+      if (S->stop || (int)S->pending.size() >= S->clients - busy ||
+          (int)S->pending.size() >= S->max_batch)
+        break;
+      if (run >= 0 && tpa_pool_eval_ready(S->pool, bt[run].slot) != 0) {
+        complete(S, bt[run], lk);
+        run = -1;
+        continue;
+      }
+      const Clock::time_point now = Clock::now();
+      if (now >= deadline) break;
+      const Clock::time_point until =
+          run >= 0 ? std::min<Clock::time_point>(deadline, now + poll) : deadline;
+      S->cv.wait_until(lk, until);
+    }
+    if (S->stop) break;
+    const int nb = run == 0 ? 1 : 0;
+    Batch& B = bt[nb];
+    B.fits.clear();
+    while (!S->pending.empty() && (int)B.fits.size() < S->max_batch) {
+      B.fits.push_back(S->pending.front());
       S->pending.pop_front();
     }
-    const int m = (int)batch.size();
-    ranges.resize(2 * m);
-    coefs.resize((size_t)2 * P * m);
+    const int m = (int)B.fits.size();
+    B.ranges.resize(2 * m);
+    B.coefs.resize((size_t)2 * P * m);
     for (int i = 0; i < m; ++i) {
-      ranges[2 * i] = batch[i]->off;
-      ranges[2 * i + 1] = batch[i]->n;
-      std::memcpy(&coefs[(size_t)2 * P * i], batch[i]->coef.data(), sizeof(double) * 2 * P);
+      B.ranges[2 * i] = B.fits[i]->off;
+      B.ranges[2 * i + 1] = B.fits[i]->n;
+      std::memcpy(&B.coefs[(size_t)2 * P * i], B.fits[i]->coef.data(), sizeof(double) * 2 * P);
     }
     lk.unlock();
-    out.resize((size_t)nacc * m);
-    double ms = 0.0;
-    const int rc = tpa_pool_eval(S->pool, m, ranges.data(), coefs.data(), TPA_MODE_DATA,
-                                 out.data(), &ms);
+    int slot = -1;
+    const int rc = tpa_pool_eval_submit(S->pool, &slot, m, B.ranges.data(), B.coefs.data(),
+                                        TPA_MODE_DATA);
     const std::string err = rc == TPA_OK ? std::string() : std::string(tpa_last_error());
     lk.lock();
-    const auto now = Clock::now();
-    S->n_batches += 1;
-    S->n_requests += m;
-    for (int i = 0; i < m; ++i) S->n_events += ranges[2 * i + 1];
-    S->gpu_ms += ms;
-    if (rc != TPA_OK) S->batch_error = err;
-    for (int i = 0; i < m; ++i) {
-      Fit* F = batch[i];
-      F->out.assign(out.begin() + (size_t)nacc * i, out.begin() + (size_t)nacc * (i + 1));
-      F->failed = rc != TPA_OK;
-      F->ready = true;
-      S->lat_us.push_back(std::chrono::duration<double, std::micro>(now - F->t_submit).count());
-      F->cv.notify_one();
+    if (rc != TPA_OK) {
+      B.failed = true;
+      deliver(S, B, 0.0, err);
+      continue;
     }
+    B.slot = slot;
+    B.failed = false;
+    // the new batch is queued behind the running one: deliver that one when it returns
+    if (run >= 0) complete(S, bt[run], lk);
+    run = nb;
   }
+  if (run >= 0) complete(S, bt[run], lk);
   // wake every waiting client so it sees the stop
   for (Fit* F : S->fits) F->cv.notify_all();
   for (Fit* F : S->pending) F->cv.notify_all();
