@@ -270,3 +270,91 @@ def test_oracle_four_body_rotation_invariance():
             p4x = {k: np.concatenate([v[:, :1], v[:, 1:] @ Rx.T], 1) for k, v in p4.items()}
             fx = O.sum_amp(om, O.cal_angle(om, p4x), om.var_tensors()).numpy()
             assert np.abs(fx - f).max() > 1e-3 * f.max()  # helicity +-1 only: not invariant
+
+
+def _boost_to_rest(p, frame):
+    """Pure boost of four-vectors p (n, 4) into the rest frame of `frame` (n, 4); the axes
+    keep their directions (no rotation)."""
+    b = frame[:, 1:] / frame[:, :1]
+    b2 = (b * b).sum(1)
+    g = 1.0 / np.sqrt(1.0 - b2)
+    bp = (b * p[:, 1:]).sum(1)
+    k = np.where(b2 > 0, (g - 1.0) / np.where(b2 > 0, b2, 1.0), 0.0)
+    out = np.empty_like(p)
+    out[:, 0] = g * (p[:, 0] - bp)
+    out[:, 1:] = p[:, 1:] + (k * bp - g * p[:, 0])[:, None] * b
+    return out
+
+
+def _rotate_pair_in_rest_frame(pa, pb, R):
+    """Rotate the back-to-back momenta of the pair (a, b) by R in the pair's rest frame (pure
+    boost there and back): the pair's four-momentum, every invariant mass built from the pair
+    or from it and other particles whole, stays; the pair's decay angles change."""
+    P = pa + pb
+    a = _boost_to_rest(pa, P)
+    b = _boost_to_rest(pb, P)
+    a[:, 1:] = a[:, 1:] @ R.T
+    b[:, 1:] = b[:, 1:] @ R.T
+    back = P.copy()
+    back[:, 1:] = -back[:, 1:]
+    return _boost_to_rest(a, back), _boost_to_rest(b, back)
+
+
+@pytest.mark.parametrize("kind", ["quasi-two-body", "sequential"])
+def test_oracle_four_body_single_chain_closed_form(kind):
+    """One four-body chain whose every vertex has a single LS coupling with a helicity-
+    independent helicity amplitude: J/psi(+-1) -> phi f0, phi -> K+ K- (l = 1), f0 -> pi pi;
+    and J/psi(+-1) -> K1+ K- (S wave), K1+ -> K*0 pi+ (S wave), K*0 -> K+ pi- (l = 1). The sum
+    over the intermediate helicities of the product of the vertices' D functions is then the D
+    function of the composed rotation, so sum_{lambda = +-1} |A|^2 = C(masses) sin^2(beta),
+    with beta the polar angle (to the J/psi z axis) of the K+ after the chain of pure boosts
+    into the last resonance's rest frame. Rotating the last pair inside its own rest frame keeps
+    every mass of the chain, so f'/f must equal sin^2(beta') / sin^2(beta)."""
+    from scipy.spatial.transform import Rotation
+
+    from tf_pwa_amd.phasespace import generate_phsp
+
+    cfg = O.yaml.safe_load(open(config_path("jpsi_kkpipi_12ch.yml")))
+    parts = cfg["particle"]
+    if kind == "quasi-two-body":
+        decay = {"Jpsi": [["phi", "f0pp", {"l_list": [0]}]], "phi": ["Kp", "Km"],
+                 "f0pp": ["pip", "pim"]}
+        keep = ["phi", "f0pp"]
+    else:
+        decay = {"Jpsi": [["K1p", "Km", {"l_list": [0]}]],
+                 "K1p": [["Kst0", "pip", {"l_list": [0]}]], "Kst0": ["Kp", "pim"]}
+        keep = ["K1p", "Kst0"]
+    c = {"data": cfg["data"], "decay": decay,
+         "particle": {"$top": parts["$top"], "$finals": parts["$finals"],
+                      **{k: parts[k] for k in keep}}}
+    om = O.Model(c)
+    assert len(om.chains) == 1
+    om.set_params(O.random_couplings(om, 11))
+    order = om.config["data"]["dat_order"]
+    ps = generate_phsp(om.top.mass, [om.particles[n].mass for n in order], 300,
+                       np.random.default_rng(7))
+    p4 = dict(zip(order, ps))
+
+    def f_of(p):
+        return O.sum_amp(om, O.cal_angle(om, p), om.var_tensors()).numpy()
+
+    def sin2beta(p):
+        if kind == "quasi-two-body":
+            k = _boost_to_rest(p["Kp"], p["Kp"] + p["Km"])
+        else:
+            k1 = p["Kp"] + p["pim"] + p["pip"]
+            kst = _boost_to_rest(p["Kp"] + p["pim"], k1)
+            k = _boost_to_rest(_boost_to_rest(p["Kp"], k1), kst)
+        n = k[:, 1:] / np.linalg.norm(k[:, 1:], axis=1, keepdims=True)
+        return 1.0 - n[:, 2] ** 2
+
+    R = Rotation.random(random_state=4).as_matrix()
+    a, b = ("Kp", "Km") if kind == "quasi-two-body" else ("Kp", "pim")
+    pr = dict(p4)
+    pr[a], pr[b] = _rotate_pair_in_rest_frame(p4[a], p4[b], R)
+    f, fr = f_of(p4), f_of(pr)
+    s, sr = sin2beta(p4), sin2beta(pr)
+    ok = (s > 1e-3) & (sr > 1e-3)
+    np.testing.assert_allclose(fr[ok] / f[ok], sr[ok] / s[ok], rtol=1e-9)
+    # the closed form is not trivially satisfied: the angles did change
+    assert np.abs(sr / s - 1.0)[ok].max() > 0.1
