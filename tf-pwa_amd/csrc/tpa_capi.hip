@@ -404,16 +404,40 @@ static void launch_hess(const HessArgs& a, int grid, hipStream_t s) {
   hipLaunchKernelGGL(k_hess<LG>, dim3(grid), dim3(256), 0, s, a);
 }
 
+// Partials of the MFMA second-moment kernel: value pair, G, then the O1 and O2 tiles.
+static int hess_mfma_nacc(int P) {
+  const int QT = (2 * P + 15) / 16;
+  return 2 + 2 * P + 2 * (QT * (QT + 1) / 2) * 256;
+}
+
+template <int LG>
+static int launch_hess_mfma(const HessArgs& a, int QT, int grid, hipStream_t s) {
+  switch (QT) {
+    case 1: hipLaunchKernelGGL((k_hess_mfma<LG, 1>), dim3(grid), dim3(256), 0, s, a); return TPA_OK;
+    case 2: hipLaunchKernelGGL((k_hess_mfma<LG, 2>), dim3(grid), dim3(256), 0, s, a); return TPA_OK;
+    case 3: hipLaunchKernelGGL((k_hess_mfma<LG, 3>), dim3(grid), dim3(256), 0, s, a); return TPA_OK;
+    case 4: hipLaunchKernelGGL((k_hess_mfma<LG, 4>), dim3(grid), dim3(256), 0, s, a); return TPA_OK;
+    default: return TPA_ERR_UNSUPPORTED;
+  }
+}
+
+// The second-moment pass: k_hess_mfma (default) or, with TFPWA_HESS_KERNEL=vec, the FP64 VALU
+// kernel k_hess; both deliver the packed layout of tpa_hess.
 static int run_hess(const TpaModel& M, const double2* h, const double* w, int64_t off, int64_t n,
                     int64_t S, const double* coef, int mode, hipStream_t s, double** d_part,
                     double** d_out, double2** d_coef, double* out) {
   if (M.P > TPA_HESS_PMAX) return fail(TPA_ERR_UNSUPPORTED, "second moments need P <= 32");
-  const int nacc = hess_nacc(M.P);
-  const int cap_nacc = hess_nacc(TPA_HESS_PMAX);
+  static const bool vec = [] {
+    const char* v = getenv("TFPWA_HESS_KERNEL");
+    return v && std::string(v) == "vec";
+  }();
+  const int P = M.P, Q = 2 * P;
+  const int nacc = vec ? hess_nacc(P) : hess_mfma_nacc(P);
+  const int cap_nacc = std::max(hess_nacc(TPA_HESS_PMAX), hess_mfma_nacc(TPA_HESS_PMAX));
   if (!*d_part) HIPCHK(hipMalloc(d_part, sizeof(double) * (size_t)kHessGridMax * cap_nacc));
   if (!*d_out) HIPCHK(hipMalloc(d_out, sizeof(double) * cap_nacc));
   if (!*d_coef) HIPCHK(hipMalloc(d_coef, sizeof(double2) * TPA_HESS_PMAX));
-  HIPCHK(hipMemcpyAsync(*d_coef, coef, sizeof(double2) * M.P, hipMemcpyHostToDevice, s));
+  HIPCHK(hipMemcpyAsync(*d_coef, coef, sizeof(double2) * P, hipMemcpyHostToDevice, s));
   HessArgs a;
   a.h = h;
   a.w = w;
@@ -422,25 +446,64 @@ static int run_hess(const TpaModel& M, const double2* h, const double* w, int64_
   a.off = off;
   a.n = n;
   a.S = S;
-  a.P = M.P;
+  a.P = P;
   a.mode = mode;
   a.nacc = nacc;
-  const int E = TPA_HESS_PAIRS / M.lam_pad;
+  const int E = (vec ? TPA_HESS_PAIRS : 128) / M.lam_pad;
   const int64_t tiles = (n + E - 1) / E;
   const int grid = (int)std::max<int64_t>(1, std::min<int64_t>(tiles, kHessGridMax));
-  switch (M.lam_pad) {
-    case 1: launch_hess<1>(a, grid, s); break;
-    case 2: launch_hess<2>(a, grid, s); break;
-    case 4: launch_hess<4>(a, grid, s); break;
-    case 8: launch_hess<8>(a, grid, s); break;
-    case 16: launch_hess<16>(a, grid, s); break;
-    default: return fail(TPA_ERR_UNSUPPORTED, "no second-moment kernel for this helicity count");
+  const int QT = (Q + 15) / 16;
+  int rc = TPA_OK;
+  if (vec) {
+    switch (M.lam_pad) {
+      case 1: launch_hess<1>(a, grid, s); break;
+      case 2: launch_hess<2>(a, grid, s); break;
+      case 4: launch_hess<4>(a, grid, s); break;
+      case 8: launch_hess<8>(a, grid, s); break;
+      case 16: launch_hess<16>(a, grid, s); break;
+      default: rc = TPA_ERR_UNSUPPORTED;
+    }
+  } else {
+    switch (M.lam_pad) {
+      case 1: rc = launch_hess_mfma<1>(a, QT, grid, s); break;
+      case 2: rc = launch_hess_mfma<2>(a, QT, grid, s); break;
+      case 4: rc = launch_hess_mfma<4>(a, QT, grid, s); break;
+      case 8: rc = launch_hess_mfma<8>(a, QT, grid, s); break;
+      case 16: rc = launch_hess_mfma<16>(a, QT, grid, s); break;
+      default: rc = TPA_ERR_UNSUPPORTED;
+    }
   }
+  if (rc != TPA_OK) return fail(rc, "no second-moment kernel for this helicity count");
   HIPCHK(hipGetLastError());
   hipLaunchKernelGGL(k_reduce, dim3(nacc), dim3(256), 0, s, *d_part, grid, nacc, *d_out);
   HIPCHK(hipGetLastError());
-  HIPCHK(hipMemcpyAsync(out, *d_out, sizeof(double) * nacc, hipMemcpyDeviceToHost, s));
+  if (vec) {
+    HIPCHK(hipMemcpyAsync(out, *d_out, sizeof(double) * nacc, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    return TPA_OK;
+  }
+  std::vector<double> raw(nacc);
+  HIPCHK(hipMemcpyAsync(raw.data(), *d_out, sizeof(double) * nacc, hipMemcpyDeviceToHost, s));
   HIPCHK(hipStreamSynchronize(s));
+  // value, G, then each matrix's upper triangle (row-major, i <= j) from the 16x16 FP64 MFMA
+  // accumulator layout: element (i, j) of tile q sits in lane j + 16 (i % 4), register i / 4
+  for (int j = 0; j < 2 + Q; ++j) out[j] = raw[j];
+  std::vector<int> tile(QT * QT, -1);
+  for (int ti = 0, q = 0; ti < QT; ++ti)
+    for (int tj = ti; tj < QT; ++tj, ++q) tile[ti * QT + tj] = q;
+  const int NT = QT * (QT + 1) / 2;
+  const int T = Q * (Q + 1) / 2;
+  for (int m = 0; m < 2; ++m) {
+    const double* src = raw.data() + 2 + Q + m * NT * 256;
+    double* dst = out + 2 + Q + m * T;
+    int idx = 0;
+    for (int r = 0; r < Q; ++r)
+      for (int c = r; c < Q; ++c, ++idx) {
+        const int q = tile[(r / 16) * QT + c / 16];
+        const int i = r % 16, j = c % 16;
+        dst[idx] = src[(q * 4 + i / 4) * 64 + j + 16 * (i % 4)];
+      }
+  }
   return TPA_OK;
 }
 

@@ -1325,6 +1325,191 @@ __global__ __launch_bounds__(256) void k_hess(HessArgs a) {
   }
 }
 
+// Four doubles: the accumulator of one v_mfma_f64_16x16x4f64 tile per lane.
+typedef double tpa_doublex4 __attribute__((ext_vector_type(4)));
+
+// Second moments on the matrix cores. The same sums as k_hess, reorganised as two weighted
+// Gram products per tile of R = 128 (event, helicity) rows staged in LDS:
+//   O2 = Z^T diag(u) Z with Z the rows' [Re h, Im h]      (every row, weight u of its event)
+//   O1 = Y^T diag(v) Y with Y the events' [Re q, Im q]    (every event)
+// each on v_mfma_f64_16x16x4f64 over the upper 16 x 16 tiles (QT = ceil(2P / 16) tiles a side;
+// wave w takes rows 32w .. 32w + 31 of the tile for O2 and every fourth group of 4 events for
+// O1). A_lambda (one thread per row), the mode's u, v and value (one thread per event), q (one
+// thread per event and wave) and G = sum u q are FP64 VALU work on the staged tile.
+// Partials per block: value, compensation, G (interleaved re / im), then the O1 and O2 tiles in
+// the accumulator layout (tile q, register r, lane l at (4 q + r) * 64 + l).
+template <int LG, int QT>
+__global__ __launch_bounds__(256) void k_hess_mfma(HessArgs a) {
+  constexpr int R = 128;
+  constexpr int E = R / LG;
+  constexpr int C = 16 * QT;
+  constexpr int NT = QT * (QT + 1) / 2;
+  constexpr int ZS = R + 1, YS = E + 1;
+  __shared__ double Zs[C * ZS];
+  __shared__ double Ys[C * YS];
+  __shared__ double2 As[R];
+  __shared__ double ur[R];
+  __shared__ double ve[E];
+  const int P = a.P, Q = 2 * P;
+  const int t = threadIdx.x;
+  const int lane = t & 63;
+  const int wave = t >> 6;
+  const int il = lane & 15, kl = lane >> 4;
+  const double eps = 1.0e-6;
+  // padding columns stay zero
+  for (int idx = t; idx < (C - Q) * ZS; idx += 256) Zs[Q * ZS + idx] = 0.0;
+  for (int idx = t; idx < (C - Q) * YS; idx += 256) Ys[Q * YS + idx] = 0.0;
+  tpa_doublex4 o1[NT], o2[NT];
+#pragma unroll
+  for (int q = 0; q < NT; ++q) o1[q] = o2[q] = tpa_doublex4{0.0, 0.0, 0.0, 0.0};
+  double gacc = 0.0;  // thread t < Q: component t of (Re G, Im G)
+  double vsum = 0.0, vcomp = 0.0;
+  const int64_t tiles = (a.n + E - 1) / E;
+  const int64_t e_end = a.off + a.n;
+  for (int64_t tile = blockIdx.x; tile < tiles; tile += gridDim.x) {
+    const int64_t e0 = a.off + tile * E;
+    // stage: for each wave k, R contiguous (event, helicity) pairs of the cache
+    for (int idx = t; idx < P * R; idx += 256) {
+      const int k = idx / R, r = idx - k * R;
+      const int64_t e = e0 + r / LG;
+      const double2 v = e < e_end ? load_pw<true>(&a.h[((int64_t)k * a.S + e) * LG + (r % LG)])
+                                  : make_double2(0.0, 0.0);
+      Zs[k * ZS + r] = v.x;
+      Zs[(P + k) * ZS + r] = v.y;
+    }
+    __syncthreads();
+    if (t < R) {
+      double ar = 0.0, ai = 0.0;
+      for (int k = 0; k < P; ++k) {
+        const double2 c = a.coef[k];
+        const double hr = Zs[k * ZS + t], hi = Zs[(P + k) * ZS + t];
+        ar += c.x * hr - c.y * hi;
+        ai += c.x * hi + c.y * hr;
+      }
+      As[t] = make_double2(ar, ai);
+    }
+    __syncthreads();
+    if (t < E) {
+      const int64_t e = e0 + t;
+      double u = 0.0, v = 0.0;
+      if (e < e_end) {
+        double f = 0.0;
+        for (int l = 0; l < LG; ++l) f += As[t * LG + l].x * As[t * LG + l].x + As[t * LG + l].y * As[t * LG + l].y;
+        const double w = a.w[e];
+        double y;
+        if (a.mode == 0) {
+          // clip_log(f) and its first two derivatives (tf_pwa/model/model.py:283-292)
+          double cl, d1, d2;
+          if (f > eps) {
+            cl = log(f);
+            d1 = 1.0 / f;
+            d2 = -1.0 / (f * f);
+          } else {
+            const double tt = (f - eps) / eps;
+            cl = (log(eps) + tt) - (tt * tt) / 2.0;
+            d1 = 1.0 / eps - tt / eps;
+            d2 = -1.0 / (eps * eps);
+          }
+          y = w * cl;
+          u = w * d1;
+          v = w * d2;
+        } else {
+          y = w * f;
+          u = w;
+        }
+        neumaier_add(vsum, vcomp, y);
+      }
+      for (int l = 0; l < LG; ++l) ur[t * LG + l] = u;
+      ve[t] = v;
+    }
+    // q_ek = sum_l conj(A_el) h_ekl into Y (every event row, zero past the end)
+    for (int idx = t; idx < E * P; idx += 256) {
+      const int e = idx / P, k = idx - e * P;
+      double qr = 0.0, qi = 0.0;
+      for (int l = 0; l < LG; ++l) {
+        const double2 A = As[e * LG + l];
+        const double hr = Zs[k * ZS + e * LG + l], hi = Zs[(P + k) * ZS + e * LG + l];
+        qr += A.x * hr + A.y * hi;
+        qi += A.x * hi - A.y * hr;
+      }
+      Ys[k * YS + e] = qr;
+      Ys[(P + k) * YS + e] = qi;
+    }
+    __syncthreads();
+    if (t < Q)
+      for (int e = 0; e < E; ++e) gacc += ur[e * LG] * Ys[t * YS + e];
+    // O2 over the wave's 32 rows, 4 per MFMA step
+#pragma unroll
+    for (int st = 0; st < 8; ++st) {
+      const int rl = wave * 32 + 4 * st + kl;
+      const double uw = ur[rl];
+      double z[QT];
+#pragma unroll
+      for (int ti = 0; ti < QT; ++ti) z[ti] = Zs[(16 * ti + il) * ZS + rl];
+      int q = 0;
+#pragma unroll
+      for (int ti = 0; ti < QT; ++ti)
+#pragma unroll
+        for (int tj = ti; tj < QT; ++tj, ++q)
+          o2[q] = __builtin_amdgcn_mfma_f64_16x16x4f64(z[ti], uw * z[tj], o2[q], 0, 0, 0);
+    }
+    if (a.mode == 0) {
+      // O1 over groups of 4 events, group g on wave g % 4
+      for (int g = wave; g < E / 4; g += 4) {
+        const int ev = 4 * g + kl;
+        const double vw = ve[ev];
+        double y[QT];
+#pragma unroll
+        for (int ti = 0; ti < QT; ++ti) y[ti] = Ys[(16 * ti + il) * YS + ev];
+        int q = 0;
+#pragma unroll
+        for (int ti = 0; ti < QT; ++ti)
+#pragma unroll
+          for (int tj = ti; tj < QT; ++tj, ++q)
+            o1[q] = __builtin_amdgcn_mfma_f64_16x16x4f64(y[ti], vw * y[tj], o1[q], 0, 0, 0);
+      }
+    }
+    __syncthreads();
+  }
+  // block partials: value pair, G, then the four waves' tiles summed in a fixed order
+  double* out = a.partials + (int64_t)blockIdx.x * a.nacc;
+  double* red = Zs;  // 2 * NT * 256 doubles fit in the staging area
+  for (int wv = 0; wv < 4; ++wv) {
+    if (wave == wv) {
+#pragma unroll
+      for (int q = 0; q < NT; ++q)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int s1 = (4 * q + r) * 64 + lane, s2 = NT * 256 + s1;
+          red[s1] = wv == 0 ? o1[q][r] : red[s1] + o1[q][r];
+          red[s2] = wv == 0 ? o2[q][r] : red[s2] + o2[q][r];
+        }
+    }
+    __syncthreads();
+  }
+  for (int j = t; j < 2 * NT * 256; j += 256) out[2 + Q + j] = red[j];
+  __syncthreads();
+  double* rs = Zs;
+  double* rc = Zs + 256;
+  rs[t] = vsum;
+  rc[t] = vcomp;
+  __syncthreads();
+  for (int m = 128; m > 0; m >>= 1) {
+    if (t < m) {
+      double s1 = rs[t], c1 = rc[t];
+      neumaier_merge(s1, c1, rs[t + m], rc[t + m]);
+      rs[t] = s1;
+      rc[t] = c1;
+    }
+    __syncthreads();
+  }
+  if (t == 0) {
+    out[0] = rs[0];
+    out[1] = rc[0];
+  }
+  if (t < Q) out[t < P ? 2 + 2 * t : 3 + 2 * (t - P)] = gacc;  // interleaved re / im
+}
+
 // ------------------------------------------------------------------ phase space on the device
 // Counter-based random numbers: Philox4x32 with 10 rounds (J. K. Salmon et al., "Parallel
 // random numbers: as easy as 1, 2, 3", SC11). Every event draws from its own counter, so a
@@ -1468,7 +1653,6 @@ __global__ __launch_bounds__(256) void k_gather_cols(const double* __restrict__ 
 // Z_row = [Re h_0 .. Re h_{P-1}, Im h_0 .. Im h_{P-1}] (Q = 2P <= 64 columns, QT = ceil(Q/16)
 // tiles). A block stages 128 rows in LDS; each wave multiplies its 32 rows into the upper
 // tiles (ti <= tj) of G. No refolds are needed: products and sums are FP64 throughout.
-typedef double tpa_doublex4 __attribute__((ext_vector_type(4)));
 
 #define TPA_GRAM64_ROWS 128
 #define TPA_GRAM64_STRIDE (TPA_GRAM64_ROWS + 1)
