@@ -94,14 +94,71 @@ def make_workload(n_data, n_mc, seed=20261203):
     return conf, data, mc, data_p4, mc
 
 
-def cpu_baseline(conf, data_p4, n_data_full, n_mc_full, seconds=15.0):
-    """The oracle (torch CPU float64 eager + autograd, the analogue of tf-pwa's TF eager CPU
-    path) on a bounded sample, scaled per event to the full workload."""
-    import torch
+def cpu_baseline(conf, data, mc, gram, data_p4, n_data_full, n_mc_full, seconds=12.0):
+    """The oracle's C/OpenMP restatement of the evaluation pass (oracle/cpu_eval.c) on the
+    host's cores, over the same partial waves as the GPU: the mixed path's per-call work (the
+    full 1M-event data pass plus the P x P Gram term) and, on a 1M-event MC sample scaled by
+    events, the FP64 path's MC pass. Also the torch eager + autograd oracle (the analogue of
+    tf-pwa's TF eager CPU path) on a small sample, scaled by events."""
+    import ctypes
 
+    from oracle import build_cpu
+
+    L = build_cpu.load()
+    dp = ctypes.POINTER(ctypes.c_double)
+    amp = conf.get_amplitude()
+    c = amp.coefficients()
+    cc = np.ascontiguousarray(c).view(np.float64)
+
+    def prep(ev, n):
+        sub = ev if n >= ev.n else ev.select(np.arange(n))
+        sub.set_weight(ev._dev_weight[:n] if n < ev.n else ev._dev_weight)
+        h = np.ascontiguousarray(sub.partial_waves())
+        return h, np.ascontiguousarray(sub._dev_weight, dtype=np.float64)
+
+    def timed(h, w, mode):
+        P, N, LG = h.shape
+        out = np.zeros(2 + 2 * P)
+        hp = h.view(np.float64).ctypes.data_as(dp)
+        args = (hp, w.ctypes.data_as(dp), N, P, LG, cc.ctypes.data_as(dp), mode,
+                out.ctypes.data_as(dp))
+        L.cpu_eval(*args)
+        t0, reps = time.perf_counter(), 0
+        while time.perf_counter() - t0 < seconds / 2 or reps < 2:
+            L.cpu_eval(*args)
+            reps += 1
+        return (time.perf_counter() - t0) / reps, reps
+
+    hd, wd = prep(data, data.n)
+    t_data, r_data = timed(hd, wd, 0)
+    del hd
+    t_gram = time.perf_counter()
+    for _ in range(100):
+        Ic = gram @ c
+        float(np.real(np.vdot(c, Ic)))
+    t_gram = (time.perf_counter() - t_gram) / 100
+    n_mc_s = min(mc.n, 1_000_000)
+    hm, wm = prep(mc, n_mc_s)
+    t_mc, r_mc = timed(hm, wm, 1)
+    del hm
+    mixed = 1.0 / (t_data + t_gram)
+    fp64 = 1.0 / (t_data + t_mc * n_mc_full / n_mc_s)
+    torch_rate = torch_oracle_rate(conf, data_p4, n_data_full, n_mc_full)
+    return {"value": mixed, "unit": "evals/s", "cores": int(L.cpu_eval_threads()),
+            "kind": "port",
+            "sample": "C/OpenMP FP64 restatement of the pass over the same partial waves: the "
+                      "full {}-event data pass ({:.1f} ms, {} runs) + the P x P Gram term, i.e. "
+                      "the mixed path's per-call work; FP64 path with a {}-event MC pass scaled "
+                      "by events to {}".format(n_data_full, 1e3 * t_data, r_data, n_mc_s,
+                                               n_mc_full),
+            "fp64_path_evals_per_s": fp64,
+            "torch_oracle_evals_per_s": torch_rate}
+
+
+def torch_oracle_rate(conf, data_p4, n_data_full, n_mc_full, seconds=6.0):
+    """The torch oracle (float64 eager + autograd) on 4k data + 40k MC, scaled by events."""
     from oracle import tfpwa_oracle as O
 
-    threads = torch.get_num_threads()
     om = O.Model(CONFIG)
     om.set_params(conf.vm.get_all_dic())
     nd, nm = 4000, 40000
@@ -116,11 +173,7 @@ def cpu_baseline(conf, data_p4, n_data_full, n_mc_full, seconds=15.0):
         O.nll_grad(om, ad, am, w, wm)
         reps += 1
     dt = (time.perf_counter() - t0) / reps
-    scale = (n_data_full + n_mc_full) / float(nd + nm)
-    return {"value": 1.0 / (dt * scale), "unit": "evals/s", "cores": threads, "kind": "port",
-            "sample": "{} data + {} MC events (oracle NLL+autograd gradient, {:.3f}s per "
-                      "eval over {} evals), scaled by events to {} + {}".format(
-                          nd, nm, dt, reps, n_data_full, n_mc_full)}
+    return 1.0 / (dt * (n_data_full + n_mc_full) / float(nd + nm))
 
 
 def serving_leg(conf, mc, data_p4, x_true, args, seed=4242):
@@ -520,6 +573,11 @@ def main():
     res["paths"]["hessian"] = hess
     log("hessian: " + ", ".join("{} {:.2f} ms (FD {:.1f} ms)".format(k, v["ms"], v[
         "central_difference_ms"]) for k, v in hess.items()))
+    cpu_baseline_saved = None
+    if not args.no_cpu_baseline:
+        log("cpu baseline (C/OpenMP restatement) ...")
+        cpu_baseline_saved = cpu_baseline(conf, data, mc, mc.gram("fp64"), data_p4, args.n_data,
+                                          args.n_mc)
     if args.serve_fits > 0:
         del fmix, data
         res["serving"] = serving_leg(conf, mc, data_p4, x0, args)
@@ -537,8 +595,12 @@ def main():
         res["calib_copy"] = {"bytes": 1 << 30, "ms": cms.value,
                              "GBps_read+write": 2 * (1 << 30) / cms.value / 1e6}
     if not args.no_cpu_baseline:
-        log("cpu baseline ...")
-        res["cpu_baseline"] = cpu_baseline(conf, data_p4, args.n_data, args.n_mc)
+        res["cpu_baseline"] = cpu_baseline_saved
+        log("cpu baseline: {:.2f} evals/s (mixed-path work, {} threads); FP64 path {:.3f} "
+            "evals/s; torch oracle {:.4f} evals/s".format(
+                cpu_baseline_saved["value"], cpu_baseline_saved["cores"],
+                cpu_baseline_saved["fp64_path_evals_per_s"],
+                cpu_baseline_saved["torch_oracle_evals_per_s"]))
     print(json.dumps(res), flush=True)
 
 

@@ -457,3 +457,33 @@ def test_concat_event_sets(case2, gpu_available):
         assert np.abs(G - Gr).max() <= 1e-12 * sum(np.abs(r[1]).max() for r in ref)
     with pytest.raises(Exception):
         EventSet.concat([])
+
+
+@pytest.mark.parametrize("mode_name", ["MODE_DATA", "MODE_MC"])
+def test_cpu_port_matches_engine_pass(case2, mode_name, gpu_available):
+    """The oracle's C restatement (the CPU baseline) and the engine's pass agree on the same
+    partial waves: the baseline measures the same computation."""
+    import ctypes
+
+    from oracle import build_cpu
+    from tf_pwa_amd import _lib
+
+    mode = getattr(_lib, mode_name)
+    L = build_cpu.load()
+    conf = case2["conf"]
+    mc = conf.data_from_p4(case2["mc"])
+    w = np.random.default_rng(9).uniform(-0.5, 1.5, mc.n)
+    mc.set_weight(w)
+    c = conf.get_amplitude().coefficients()
+    v, G, _ = mc.eval(c, mode, grad=True)
+    h = np.ascontiguousarray(mc.partial_waves())
+    P, N, LG = h.shape
+    out = np.zeros(2 + 2 * P)
+    dp = ctypes.POINTER(ctypes.c_double)
+    cc = np.ascontiguousarray(c).view(np.float64)
+    assert L.cpu_eval(h.view(np.float64).ctypes.data_as(dp), w.ctypes.data_as(dp), N, P, LG,
+                      cc.ctypes.data_as(dp), 0 if mode == _lib.MODE_DATA else 1,
+                      out.ctypes.data_as(dp)) == 0
+    assert abs(out[0] - v) <= 1e-12 * np.abs(w).sum() * max(1.0, abs(v) / np.abs(w).sum())
+    np.testing.assert_allclose(out[2:].view(np.complex128), G, rtol=0,
+                               atol=1e-11 * np.abs(G).max())

@@ -1369,6 +1369,7 @@ __global__ __launch_bounds__(256) void k_hess_mfma(HessArgs a) {
   for (int64_t tile = blockIdx.x; tile < tiles; tile += gridDim.x) {
     const int64_t e0 = a.off + tile * E;
     // stage: for each wave k, R contiguous (event, helicity) pairs of the cache
+#pragma unroll 2
     for (int idx = t; idx < P * R; idx += 256) {
       const int k = idx / R, r = idx - k * R;
       const int64_t e = e0 + r / LG;
@@ -1380,6 +1381,7 @@ __global__ __launch_bounds__(256) void k_hess_mfma(HessArgs a) {
     __syncthreads();
     if (t < R) {
       double ar = 0.0, ai = 0.0;
+#pragma unroll 2
       for (int k = 0; k < P; ++k) {
         const double2 c = a.coef[k];
         const double hr = Zs[k * ZS + t], hi = Zs[(P + k) * ZS + t];
@@ -1423,6 +1425,7 @@ __global__ __launch_bounds__(256) void k_hess_mfma(HessArgs a) {
       ve[t] = v;
     }
     // q_ek = sum_l conj(A_el) h_ekl into Y (every event row, zero past the end)
+#pragma unroll 1
     for (int idx = t; idx < E * P; idx += 256) {
       const int e = idx / P, k = idx - e * P;
       double qr = 0.0, qi = 0.0;
