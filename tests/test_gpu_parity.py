@@ -487,3 +487,38 @@ def test_cpu_port_matches_engine_pass(case2, mode_name, gpu_available):
     assert abs(out[0] - v) <= 1e-12 * np.abs(w).sum() * max(1.0, abs(v) / np.abs(w).sum())
     np.testing.assert_allclose(out[2:].view(np.complex128), G, rtol=0,
                                atol=1e-11 * np.abs(G).max())
+
+
+@pytest.mark.parametrize("n", [1, 3, 17, 255, 257, 4097])
+def test_eval_pass_ragged_event_counts(case2, n, gpu_available):
+    """The pipelined pass at event counts that end inside a wave's group, inside a block's step
+    and past one grid stride (clamped loads, the early exit of the second stage), in both modes,
+    against the oracle's C restatement on the same partial waves."""
+    import ctypes
+
+    from oracle import build_cpu
+    from tf_pwa_amd import _lib
+
+    L = build_cpu.load()
+    conf = case2["conf"]
+    full = conf.data_from_p4(case2["mc"])
+    ev = full.select(np.arange(n) % full.n)
+    w = np.random.default_rng(n).uniform(-0.5, 1.5, n)
+    ev.set_weight(w)
+    c = conf.get_amplitude().coefficients()
+    h = np.ascontiguousarray(ev.partial_waves())
+    P, N, LG = h.shape
+    dp = ctypes.POINTER(ctypes.c_double)
+    cc = np.ascontiguousarray(c).view(np.float64)
+    for mode in (_lib.MODE_DATA, _lib.MODE_MC):
+        v, G, f = ev.eval(c, mode, grad=True, fout=True)
+        out = np.zeros(2 + 2 * P)
+        assert L.cpu_eval(h.view(np.float64).ctypes.data_as(dp), w.ctypes.data_as(dp), N, P, LG,
+                          cc.ctypes.data_as(dp), 0 if mode == _lib.MODE_DATA else 1,
+                          out.ctypes.data_as(dp)) == 0
+        scale = max(1.0, np.abs(w).sum())
+        assert abs(out[0] - v) <= 1e-12 * scale * max(1.0, abs(v) / scale)
+        np.testing.assert_allclose(out[2:].view(np.complex128), G, rtol=0,
+                                   atol=1e-11 * max(1e-300, np.abs(G).max()))
+        A = np.einsum("k,kel->el", c, h)
+        np.testing.assert_allclose(f, (np.abs(A) ** 2).sum(1), rtol=1e-12)

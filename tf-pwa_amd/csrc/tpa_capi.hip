@@ -303,7 +303,13 @@ static int dispatch_eval(int lg, int kg, const EvalArgs& a, int grid, hipStream_
 
 template <int LG, int KG>
 static void launch_shape(const EvalShapeArgs& a, int grid, hipStream_t s) {
-  hipLaunchKernelGGL((k_eval_shape<LG, KG, 8>), dim3(grid), dim3(256), 0, s, a);
+  const int need = (a.a.P + KG - 1) / KG;  // PML fitted to P as for the folded pass
+  if (need <= 4)
+    hipLaunchKernelGGL((k_eval_shape<LG, KG, 4>), dim3(grid), dim3(256), 0, s, a);
+  else if (need <= 6)
+    hipLaunchKernelGGL((k_eval_shape<LG, KG, 6>), dim3(grid), dim3(256), 0, s, a);
+  else
+    hipLaunchKernelGGL((k_eval_shape<LG, KG, 8>), dim3(grid), dim3(256), 0, s, a);
 }
 
 template <int LG>
@@ -994,7 +1000,7 @@ int tpa_eval(tpa_events_t ev, const double* coef, int mode, int flags, double* o
   a.nacc = shape ? 2 + 6 * M.P : 2 + 2 * M.P;
   const int kg = eval_kgroups(M.lam_pad, M.P);
   if (kg < 0) return fail(TPA_ERR_UNSUPPORTED, "too many partial waves for this helicity count");
-  const int grid = eval_grid(ev->n, M.lam_pad * kg);
+  int grid = eval_grid(ev->n, M.lam_pad * kg);
   bool timing;
   {
     std::lock_guard<std::mutex> lk(g_timing_mu);
@@ -1016,7 +1022,13 @@ int tpa_eval(tpa_events_t ev, const double* coef, int mode, int flags, double* o
     sa.bw = ev->d_bw;
     sa.kchain = ev->m->d_kchain;
     sa.n_chain = M.n_chain;
-    rc = dispatch_shape(M.lam_pad, kg, sa, grid, s);
+    // the shape pass keeps four accumulators more per wave: at most 4 waves per lane
+    // (PML = 4) while the lanes of an event fit in the wave; the reduction below sums the
+    // partials of this launch's grid
+    int kgs = kg;
+    while (kgs * 4 < M.P && M.lam_pad * kgs * 2 <= 64) kgs <<= 1;
+    grid = eval_grid(ev->n, M.lam_pad * kgs);
+    rc = dispatch_shape(M.lam_pad, kgs, sa, grid, s);
   } else {
     rc = dispatch_eval(M.lam_pad, kg, a, grid, s);
   }

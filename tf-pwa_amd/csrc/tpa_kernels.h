@@ -855,6 +855,10 @@ struct EvalShapeArgs {
 
 template <int LG, int KG, int PML>
 __global__ __launch_bounds__(256) void k_eval_shape(EvalShapeArgs s) {
+  // The evaluation pass with the lineshapes per event (floating masses and widths): the cache
+  // holds the waves without their Breit-Wigner; bw holds B_c, dB_c/dm0 and dB_c/dg0 per chain
+  // and event (k_lineshape). Loads are unconditional (clamped to a valid event and wave) and
+  // issued together; lane exchanges as in k_eval_pipe.
   const EvalArgs& a = s.a;
   constexpr int LPE = LG * KG;
   constexpr int EPW = 64 / LPE;
@@ -863,11 +867,12 @@ __global__ __launch_bounds__(256) void k_eval_shape(EvalShapeArgs s) {
   const int lam = lane & (LG - 1);
   const int kg = (lane / LG) & (KG - 1);
   const int el = lane / LPE;
+  const bool leader = lam == 0 && kg == 0;
   const double eps = 1.0e-6;
-  const double logeps = log(1.0e-6);
   double vs = 0.0, vc = 0.0;
   double gr[PML], gi[PML], mr[PML], mi[PML], wr[PML], wi[PML], cr[PML], ci[PML];
-  int kc[PML];
+  const double2* hb[PML];
+  const double2* bb[PML];
 #pragma unroll
   for (int j = 0; j < PML; ++j) {
     const int k = kg + KG * j;
@@ -875,19 +880,31 @@ __global__ __launch_bounds__(256) void k_eval_shape(EvalShapeArgs s) {
     const double2 c = k < a.P ? a.coef[k] : make_double2(0.0, 0.0);
     cr[j] = c.x;
     ci[j] = c.y;
-    kc[j] = k < a.P ? s.kchain[k] : 0;
+    const int kc = k < a.P ? k : a.P - 1;
+    hb[j] = a.h + (int64_t)kc * a.N * LG + lam;
+    bb[j] = s.bw + (int64_t)s.kchain[kc] * a.N;
   }
+  const int64_t nc = (int64_t)s.n_chain * a.N;  // B, dB/dm0, dB/dg0 rows are nc apart
+  const int64_t last = a.N - 1;
   const int64_t stride = (int64_t)gridDim.x * 4 * EPW;
-  for (int64_t base = ((int64_t)blockIdx.x * 4 + wave) * EPW; base < a.N; base += stride) {
-    const int64_t e = base + el;
+  const int wave_u = __builtin_amdgcn_readfirstlane(wave);
+  for (int64_t b = ((int64_t)blockIdx.x * 4 + wave_u) * EPW; b < a.N; b += stride) {
+    const int64_t e = b + el;
     const bool valid = e < a.N;
-    double2 hv[PML], bv[PML];
+    const int64_t ec = valid ? e : last;
+    double2 hv[PML], bv[PML], dm[PML], dg[PML];
 #pragma unroll
     for (int j = 0; j < PML; ++j) {
-      const int k = kg + KG * j;
-      const bool ld = valid && k < a.P;
-      hv[j] = ld ? load_pw<true>(&a.h[((int64_t)k * a.N + e) * LG + lam]) : make_double2(0.0, 0.0);
-      bv[j] = ld ? s.bw[(int64_t)kc[j] * a.N + e] : make_double2(0.0, 0.0);
+      hv[j] = load_pw<true>(hb[j] + ec * LG);
+      bv[j] = bb[j][ec];
+    }
+    const double w = valid ? a.w[ec] : 0.0;
+    if (a.grad) {
+#pragma unroll
+      for (int j = 0; j < PML; ++j) {
+        dm[j] = bb[j][nc + ec];
+        dg[j] = bb[j][2 * nc + ec];
+      }
     }
     double ar = 0.0, ai = 0.0;
 #pragma unroll
@@ -897,55 +914,41 @@ __global__ __launch_bounds__(256) void k_eval_shape(EvalShapeArgs s) {
       ar += cbr * hv[j].x - cbi * hv[j].y;
       ai += cbr * hv[j].y + cbi * hv[j].x;
     }
-#pragma unroll
-    for (int m = LG; m < LPE; m <<= 1) {
-      ar += __shfl_xor(ar, m, 64);
-      ai += __shfl_xor(ai, m, 64);
-    }
-    double f = ar * ar + ai * ai;
-#pragma unroll
-    for (int m = 1; m < LG; m <<= 1) f += __shfl_xor(f, m, 64);
-    double y = 0.0, u = 0.0;
-    if (valid) {
-      const double w = a.w[e];
-      if (a.mode == 0) {
-        const double dom = (w == 0.0) ? 1.0 : w;
-        const double x = (f * w) / dom;
-        double cl, dcl;
-        if (x > eps) {
-          cl = log(x);
-          dcl = 1.0 / x;
-        } else {
-          const double dx = x - eps;
-          const double t = dx / eps;
-          cl = (logeps + t) - (t * t) / 2.0;
-          dcl = 1.0 / eps - t / eps;
-        }
-        y = w * cl;
-        u = (w * dcl) * (w / dom);
+    ar = butterfly<LG, LPE>(ar);
+    ai = butterfly<LG, LPE>(ai);
+    const double f = butterfly<1, LG>(ar * ar + ai * ai);
+    double y, u;
+    if (a.mode == 0) {
+      // clip_log(f) (tf_pwa/model/model.py:283-292) weighted by w
+      double cl, dcl;
+      if (f > eps) {
+        cl = log(f);
+        dcl = 1.0 / f;
       } else {
-        y = w * f;
-        u = w;
+        const double t = (f - eps) / eps;
+        cl = (log(eps) + t) - (t * t) / 2.0;
+        dcl = 1.0 / eps - t / eps;
       }
-      if (a.f_out != nullptr && lam == 0 && kg == 0) a.f_out[e] = f;
+      y = w * cl;
+      u = w * dcl;
+    } else {
+      y = w * f;
+      u = w;
     }
-    if (lam == 0 && kg == 0) neumaier_add(vs, vc, y);
+    if (a.f_out != nullptr && valid && leader) a.f_out[e] = f;
+    neumaier_add(vs, vc, leader ? y : 0.0);
     if (a.grad) {
 #pragma unroll
       for (int j = 0; j < PML; ++j) {
-        const int k = kg + KG * j;
-        if (!(valid && k < a.P)) continue;
-        // u * conj(A) h_k
+        // u * conj(A) h_k, then times B, dB/dm0, dB/dg0 of the wave's chain
         const double hr = u * (ar * hv[j].x + ai * hv[j].y);
         const double hi = u * (ar * hv[j].y - ai * hv[j].x);
         gr[j] += bv[j].x * hr - bv[j].y * hi;
         gi[j] += bv[j].x * hi + bv[j].y * hr;
-        const double2 dm = s.bw[(int64_t)(s.n_chain + kc[j]) * a.N + e];
-        const double2 dg = s.bw[(int64_t)(2 * s.n_chain + kc[j]) * a.N + e];
-        mr[j] += dm.x * hr - dm.y * hi;
-        mi[j] += dm.x * hi + dm.y * hr;
-        wr[j] += dg.x * hr - dg.y * hi;
-        wi[j] += dg.x * hi + dg.y * hr;
+        mr[j] += dm[j].x * hr - dm[j].y * hi;
+        mi[j] += dm[j].x * hi + dm[j].y * hr;
+        wr[j] += dg[j].x * hr - dg[j].y * hi;
+        wi[j] += dg[j].x * hi + dg[j].y * hr;
       }
     }
   }
@@ -961,12 +964,7 @@ __global__ __launch_bounds__(256) void k_eval_shape(EvalShapeArgs s) {
     for (int j = 0; j < PML; ++j) {
       double x[6] = {gr[j], gi[j], mr[j], mi[j], wr[j], wi[j]};
 #pragma unroll
-      for (int q = 0; q < 6; ++q) {
-#pragma unroll
-        for (int m = 1; m < LG; m <<= 1) x[q] += __shfl_xor(x[q], m, 64);
-#pragma unroll
-        for (int m = LPE; m < 64; m <<= 1) x[q] += __shfl_xor(x[q], m, 64);
-      }
+      for (int q = 0; q < 6; ++q) x[q] = butterfly<LPE, 64>(butterfly<1, LG>(x[q]));
       const int k = kg + KG * j;
       if (el == 0 && lam == 0 && k < a.P) {
         sh[wave][2 + 2 * k] = x[0];

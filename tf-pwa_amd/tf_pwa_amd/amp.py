@@ -341,7 +341,13 @@ class AmplitudeModel:
         return out
 
     def _names(self):
-        """Variable names of every chain: total, the g_ls of both vertices, mass and width."""
+        """Variable names of every chain: total, the g_ls of both vertices, mass and width
+        (fixed by the decay structure, so built once)."""
+        if getattr(self, "_names_cache", None) is None:
+            self._names_cache = self._names_now()
+        return self._names_cache
+
+    def _names_now(self):
         if self.n_final == 4:
             out = []
             for c, ch in zip(self.decay_group.chains, self.chains):
@@ -362,7 +368,15 @@ class AmplitudeModel:
 
     def shape_params(self):
         """(m0, g0) of every chain's resonance from the parameter store (four-body chains:
-        (m0, g0, m0', g0') of their two resonances)."""
+        (m0, g0, m0', g0') of their two resonances), cached by the store's version."""
+        vm = self.vm
+        if getattr(self, "_sp_version", None) == vm.version:
+            return self._sp
+        self._sp = self._shape_params_now()
+        self._sp_version = vm.version
+        return self._sp
+
+    def _shape_params_now(self):
         vm = self.vm
         if self.n_final == 4:
             out = []
@@ -608,13 +622,30 @@ class AmplitudeModel:
         d_i = np.where(polar, 1j * r * e, 1j)
         return val, d_r, d_i
 
+    def _wave_barrier_index(self):
+        """Per partial wave and vertex, the position of its barrier constant in the flat list
+        of every chain's vertex constants (chain-major, then vertex, then LS)."""
+        if getattr(self, "_wbi", None) is None:
+            fx = self._factor_index()
+            bc = self.barrier_constants()
+            start, pos = {}, 0
+            for c, b in enumerate(bc):
+                for v, arr in enumerate(b["c"]):
+                    start[(c, v)] = pos
+                    pos += len(arr)
+            self._wbi = np.array([[start[(c, v)] + l for v, l in enumerate(ls)]
+                                  for c, ls in zip(fx["ch"], fx["ls"])], dtype=np.int64)
+        return self._wbi
+
+    def _barrier_flat(self, field):
+        bc = self.barrier_constants()
+        return np.concatenate([np.asarray(a, dtype=np.float64) for b in bc for a in b[field]])
+
     def _barrier_per_wave(self):
         key = tuple(self.shape_params())
         if getattr(self, "_bpw_key", None) != key:
-            fx = self._factor_index()
-            bc = self.barrier_constants()
-            self._bpw = np.array([np.prod([bc[c]["c"][v][l] for v, l in enumerate(ls)])
-                                  for c, ls in zip(fx["ch"], fx["ls"])])
+            idx = self._wave_barrier_index()
+            self._bpw = np.prod(self._barrier_flat("c")[idx], axis=1)
             self._bpw_key = key
         return self._bpw
 
@@ -744,15 +775,16 @@ class AmplitudeModel:
         n = len(vm.trainable_vars)
         key = (tuple(vm.trainable_vars), tuple(self.shape_params()))
         if getattr(self, "_sg_key", None) != key:
-            index = {nm: i for i, nm in enumerate(vm.trainable_vars)}
-            fx = self._factor_index()
-            bc = self.barrier_constants()
-            names = self._names()
-            dlog = np.array([sum(bc[ch]["dlog"][v][l] for v, l in enumerate(ls))
-                             for ch, ls in zip(fx["ch"], fx["ls"])])
-            mrow = np.array([index.get(nm["mass"], -1) for nm in names])[fx["ch"]]
-            wrow = np.array([index.get(nm["width"], -1) for nm in names])[fx["ch"]]
-            self._sg = (dlog, mrow, wrow)
+            rkey = tuple(vm.trainable_vars)
+            if getattr(self, "_sg_rows_key", None) != rkey:
+                index = {nm: i for i, nm in enumerate(vm.trainable_vars)}
+                fx = self._factor_index()
+                names = self._names()
+                self._sg_rows = (np.array([index.get(nm["mass"], -1) for nm in names])[fx["ch"]],
+                                 np.array([index.get(nm["width"], -1) for nm in names])[fx["ch"]])
+                self._sg_rows_key = rkey
+            dlog = self._barrier_flat("dlog")[self._wave_barrier_index()].sum(axis=1)
+            self._sg = (dlog,) + self._sg_rows
             self._sg_key = key
         dlog, mrow, wrow = self._sg
         c = self.coefficients()
