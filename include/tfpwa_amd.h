@@ -31,6 +31,9 @@ typedef struct tpa_fcn_s* tpa_fcn_t;
 #define TPA_ERR_ARG -1
 #define TPA_ERR_HIP -2
 #define TPA_ERR_UNSUPPORTED -3
+/* The barrier clamp flags at the requested masses differ from those the partial-wave caches
+ * were built with: rebuild the caches (tpa_events_build) and the session. */
+#define TPA_ERR_STALE -4
 
 /* Evaluation modes (tpa_eval). DATA: sum_e w_e * clip_log(f_e) (tf_pwa/model/model.py:41-54
  * with trans=clip_log, 283-292). MC: sum_e w_e * f_e (the same function with identity). */
@@ -306,6 +309,44 @@ int tpa_fcn_destroy(tpa_fcn_t fcn);
 
 /* grad may be NULL (value only). */
 int tpa_fcn_nll_grad(tpa_fcn_t fcn, const double* y, double* nll, double* grad);
+
+/* Floating masses and widths of three-body chains (resonance R of chain c decaying at the top
+ * vertex top -> R + bachelor and at R -> y + z). Per evaluation the session takes m0, g0 of
+ * each chain from y (or the fixed values), recomputes the barrier constants sqrt(P_l(z0)) of
+ * both vertices (tf_pwa/amp/core.py:1072-1097 with q0 from the masses, core.py:311-334), sets
+ * the device lineshape parameters, evaluates with per-event lineshapes and their derivatives
+ * (TPA_FLAG_SHAPE) and adds the mass and width gradient:
+ *   dNLL/dm0_c = 2 Re sum_{k in c} c_k (dlog_k F_k + Dm_k),  dNLL/dg0_c = 2 Re sum_{k in c} c_k Dg_k
+ * with dlog_k the m0 derivative of log(wave constant). The caches must be built without
+ * lineshapes (fold 0) and with the clamp flags act1/act2; when the flags at the requested
+ * masses differ, tpa_fcn_nll_grad returns TPA_ERR_STALE. The MC normalisation is the full MC
+ * pass (a Gram matrix cannot carry per-event lineshapes). Replaces FCN.nll_grad of models with
+ * `float: mg` resonances (tf_pwa/model/model.py:1207-1240, the mass/width Variables of
+ * tf_pwa/amp/core.py:380-390). */
+#define TPA_SHAPE_MAX_CHAIN 32
+#define TPA_SHAPE_MAX_LS 8
+#define TPA_SHAPE_MAX_P 128
+typedef struct TpaShapeMap {
+  int32_t n_chain, P;
+  int32_t mass_var[TPA_SHAPE_MAX_CHAIN], width_var[TPA_SHAPE_MAX_CHAIN]; /* y index or -1 */
+  double mass_fixed[TPA_SHAPE_MAX_CHAIN], width_fixed[TPA_SHAPE_MAX_CHAIN];
+  double m_top;
+  double m_bach[TPA_SHAPE_MAX_CHAIN], m_y[TPA_SHAPE_MAX_CHAIN], m_z[TPA_SHAPE_MAX_CHAIN];
+  double d[TPA_SHAPE_MAX_CHAIN];                                     /* barrier radius */
+  int32_t n_l1[TPA_SHAPE_MAX_CHAIN], n_l2[TPA_SHAPE_MAX_CHAIN];
+  int32_t l1[TPA_SHAPE_MAX_CHAIN][TPA_SHAPE_MAX_LS], l2[TPA_SHAPE_MAX_CHAIN][TPA_SHAPE_MAX_LS];
+  int32_t act1[TPA_SHAPE_MAX_CHAIN][TPA_SHAPE_MAX_LS], act2[TPA_SHAPE_MAX_CHAIN][TPA_SHAPE_MAX_LS];
+  int32_t wave_chain[TPA_SHAPE_MAX_P], wave_ls1[TPA_SHAPE_MAX_P], wave_ls2[TPA_SHAPE_MAX_P];
+} TpaShapeMap;
+
+/* Attach a shape map to a session built without a Gram matrix; model: the event sets' model
+ * (its device lineshape parameters are set per evaluation). */
+int tpa_fcn_set_shape_map(tpa_fcn_t fcn, tpa_model_t model, const TpaShapeMap* map);
+
+/* Set the lineshape parameters (m0, g0) of every chain of a three-body model on the device,
+ * ordered with later work on the model's stream (the fields of TpaChain that
+ * tpa_model_update would replace; no other field changes). */
+int tpa_model_set_lineshapes(tpa_model_t model, int n_chain, const double* m0, const double* g0);
 
 #ifdef __cplusplus
 }

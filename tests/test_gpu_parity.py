@@ -522,3 +522,45 @@ def test_eval_pass_ragged_event_counts(case2, n, gpu_available):
                                    atol=1e-11 * max(1e-300, np.abs(G).max()))
         A = np.einsum("k,kel->el", c, h)
         np.testing.assert_allclose(f, (np.abs(A) ** 2).sum(1), rtol=1e-12)
+
+
+@pytest.mark.parametrize("cfg", ["lc_pkpi_2res.yml", "lc_pkpi_6res.yml"])
+def test_native_shape_session_matches_python_assembly(cfg, gpu_available):
+    """Floating masses and widths: the native session (barrier constants, lineshape parameters
+    and the shape gradient in C++) against the Python host assembly at several points,
+    including a mass step large enough to move q0 and every barrier constant."""
+    import yaml
+
+    from conftest import config_path
+    from tf_pwa_amd import FCN, ConfigLoader
+    from tf_pwa_amd.phasespace import generate_phsp
+
+    c = yaml.safe_load(open(config_path(cfg)))
+    for name, p in c["particle"].items():
+        if not name.startswith("$") and isinstance(p, dict) and "width" in p:
+            p["float"] = "mg"
+    conf = ConfigLoader(c, seed=4)
+    order = conf.dat_order
+    masses = [conf.particles[n].mass for n in order]
+    top = conf.particles[conf.top_name].mass
+    rng = np.random.default_rng(4)
+    data = conf.data_from_p4(dict(zip(order, generate_phsp(top, masses, 3000, rng))),
+                             weight=rng.uniform(0.5, 1.5, 3000))
+    mc = conf.data_from_p4(dict(zip(order, generate_phsp(top, masses, 20000, rng))))
+    fn = FCN(conf.get_model(), data, mc)
+    fp = FCN(conf.get_model(), data, mc)
+    fp.native = False
+    x0 = np.array(conf.vm.get_all_val())
+    names = conf.vm.trainable_vars
+    for step in range(4):
+        x = x0.copy()
+        for i, n in enumerate(names):
+            if n.endswith("_mass"):
+                x[i] += 0.004 * (step + 1) * (1 if i % 2 else -1)
+            elif n.endswith("_width"):
+                x[i] *= 1.0 + 0.05 * step
+        n1, g1 = fn.nll_grad(x)
+        assert fn._native is not None  # the native path ran
+        n2, g2 = fp.nll_grad(x)
+        assert abs(n1 - n2) <= 1e-11 * max(1.0, abs(n2)), (n1, n2)
+        np.testing.assert_allclose(g1, g2, rtol=0, atol=1e-9 * np.abs(g2).max())

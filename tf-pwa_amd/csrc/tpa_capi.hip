@@ -199,6 +199,7 @@ struct tpa_model_s {
   double* d_dw = nullptr;
   int* d_kchain = nullptr;
   hipStream_t stream = nullptr;
+  TpaModel* h_pin = nullptr;  // pinned staging of tpa_model_set_lineshapes
 };
 
 struct tpa_events_s {
@@ -703,6 +704,7 @@ int tpa_model_destroy(tpa_model_t m) {
   if (m->d_cg) (void)hipFree(m->d_cg);
   if (m->d_dw) (void)hipFree(m->d_dw);
   if (m->d_kchain) (void)hipFree(m->d_kchain);
+  if (m->h_pin) (void)hipHostFree(m->h_pin);
   if (m->stream) (void)hipStreamDestroy(m->stream);
   delete m;
   return TPA_OK;
@@ -721,6 +723,22 @@ int tpa_model_update(tpa_model_t m, const void* desc, int64_t desc_bytes) {
   HIPCHK(hipStreamSynchronize(m->stream));
   m->h = next;
   HIPCHK(hipMemcpy(m->d, &m->h, sizeof(TpaModel), hipMemcpyHostToDevice));
+  return TPA_OK;
+}
+
+int tpa_model_set_lineshapes(tpa_model_t m, int n_chain, const double* m0, const double* g0) {
+  if (!m || !m0 || !g0) return fail(TPA_ERR_ARG, "null pointer");
+  if (m->kind != 3) return fail(TPA_ERR_UNSUPPORTED, "lineshape parameters of a three-body model");
+  if (n_chain != m->h.n_chain) return fail(TPA_ERR_ARG, "chain count mismatch");
+  if (!m->h_pin) HIPCHK(hipHostMalloc((void**)&m->h_pin, sizeof(TpaModel)));
+  // the pinned copy is read by the asynchronous copy: wait for the previous one to finish
+  HIPCHK(hipStreamSynchronize(m->stream));
+  for (int c = 0; c < n_chain; ++c) {
+    m->h.chain[c].m0 = m0[c];
+    m->h.chain[c].g0 = g0[c];
+  }
+  std::memcpy(m->h_pin, &m->h, sizeof(TpaModel));
+  HIPCHK(hipMemcpyAsync(m->d, m->h_pin, sizeof(TpaModel), hipMemcpyHostToDevice, m->stream));
   return TPA_OK;
 }
 

@@ -4,6 +4,7 @@
 // (mixed path) the MC normalisation is the Gram form c^dagger I c, and the gradient in y follows
 // by the coupling chain rule. The same arithmetic as Model._nll_grad of the Python host, without
 // its per-call interpreter work.
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <vector>
@@ -18,7 +19,137 @@ struct tpa_fcn_s : CouplingMap {
   std::vector<tpa_events_t> data, mc;
   double sw = 0.0;
   std::vector<double> coef, out;
+  // floating lineshapes (tpa_fcn_set_shape_map)
+  bool shape = false;
+  tpa_model_t model = nullptr;
+  TpaShapeMap sm{};
 };
+
+namespace {
+
+// Host forms of the barrier polynomial |theta_l(i w)|^2 (tf_pwa/breit_wigner.py:332-355), its
+// derivative, and get_relative_p2 (tf_pwa/amp/core.py:324-334) with its mass derivatives.
+double bp_poly(int l, double z) {
+  static const double C[6][6] = {{1}, {1, 1}, {1, 3, 9}, {1, 6, 45, 225}, {1, 10, 135, 1575, 11025},
+                                 {1, 15, 315, 6300, 99225, 893025}};
+  const int L = std::min(l, 5);
+  double r = 0.0;
+  for (int i = 0; i <= L; ++i) r = r * z + C[L][i];
+  return r;
+}
+double bp_poly_deriv(int l, double z) {
+  static const double C[6][6] = {{1}, {1, 1}, {1, 3, 9}, {1, 6, 45, 225}, {1, 10, 135, 1575, 11025},
+                                 {1, 15, 315, 6300, 99225, 893025}};
+  const int L = std::min(l, 5);
+  double r = 0.0;
+  for (int i = 0; i < L; ++i) r = r * z + C[L][i] * (L - i);
+  return r;
+}
+double q2_of(double M, double m1, double m2) {
+  const double s = m1 + m2, d = m1 - m2;
+  return (M - s) * (M + s) * (M - d) * (M + d) / (4.0 * M * M);
+}
+double dq2_dparent(double M, double m1, double m2) {
+  const double s = m1 + m2, d = m1 - m2;
+  const double F = (M * M - s * s) * (M * M - d * d);
+  const double dF = 2.0 * M * (M * M - d * d) + 2.0 * M * (M * M - s * s);
+  return dF / (4.0 * M * M) - F / (2.0 * M * M * M);
+}
+double dq2_ddaughter(double M, double m1, double m2) {  // d / d m1
+  const double s = m1 + m2, d = m1 - m2;
+  const double dF = -2.0 * s * (M * M - d * d) - 2.0 * d * (M * M - s * s);
+  return dF / (4.0 * M * M);
+}
+
+// Per chain: sqrt(P_l(q0^2 d^2)) of every LS of both vertices (1 when clamped, P_l <= 0) and
+// d log / d m0 of it; false when a clamp flag differs from the cache's.
+bool barrier_constants(const TpaShapeMap& S, int c, double m0, double* c1, double* c2,
+                       double* g1, double* g2) {
+  const double d2 = S.d[c] * S.d[c];
+  const double q01 = q2_of(S.m_top, m0, S.m_bach[c]);
+  const double dq01 = dq2_ddaughter(S.m_top, m0, S.m_bach[c]);
+  const double q02 = q2_of(m0, S.m_y[c], S.m_z[c]);
+  const double dq02 = dq2_dparent(m0, S.m_y[c], S.m_z[c]);
+  for (int s = 0; s < S.n_l1[c]; ++s) {
+    const int l = S.l1[c][s];
+    const double p0 = bp_poly(l, q01 * d2);
+    if ((p0 > 0) != (S.act1[c][s] != 0)) return false;
+    c1[s] = p0 > 0 ? std::sqrt(p0) : 1.0;
+    g1[s] = p0 > 0 ? bp_poly_deriv(l, q01 * d2) * d2 * dq01 / (2.0 * p0) : 0.0;
+  }
+  for (int s = 0; s < S.n_l2[c]; ++s) {
+    const int l = S.l2[c][s];
+    const double p0 = bp_poly(l, q02 * d2);
+    if ((p0 > 0) != (S.act2[c][s] != 0)) return false;
+    c2[s] = p0 > 0 ? std::sqrt(p0) : 1.0;
+    g2[s] = p0 > 0 ? bp_poly_deriv(l, q02 * d2) * d2 * dq02 / (2.0 * p0) : 0.0;
+  }
+  return true;
+}
+
+// NLL and gradient with floating lineshapes (one data and one MC pass per event set, each
+// with TPA_FLAG_SHAPE).
+int shape_nll_grad(tpa_fcn_s* F, const double* y, double* nll, double* grad) {
+  const TpaShapeMap& S = F->sm;
+  const int P = F->P, nc = S.n_chain;
+  std::vector<double> m0(nc), g0(nc), dlog(P);
+  std::vector<double> c1((size_t)nc * TPA_SHAPE_MAX_LS), c2(c1.size()), g1(c1.size()), g2(c1.size());
+  for (int c = 0; c < nc; ++c) {
+    m0[c] = S.mass_var[c] >= 0 ? y[S.mass_var[c]] : S.mass_fixed[c];
+    g0[c] = S.width_var[c] >= 0 ? y[S.width_var[c]] : S.width_fixed[c];
+    if (!barrier_constants(S, c, m0[c], &c1[c * TPA_SHAPE_MAX_LS], &c2[c * TPA_SHAPE_MAX_LS],
+                           &g1[c * TPA_SHAPE_MAX_LS], &g2[c * TPA_SHAPE_MAX_LS]))
+      return tpa_set_error(TPA_ERR_STALE, "barrier clamp flags changed: rebuild the caches");
+  }
+  for (int k = 0; k < P; ++k) {
+    const int c = S.wave_chain[k], s1 = S.wave_ls1[k], s2 = S.wave_ls2[k];
+    F->wave_const[k] = c1[c * TPA_SHAPE_MAX_LS + s1] * c2[c * TPA_SHAPE_MAX_LS + s2];
+    dlog[k] = g1[c * TPA_SHAPE_MAX_LS + s1] + g2[c * TPA_SHAPE_MAX_LS + s2];
+  }
+  int rc = tpa_model_set_lineshapes(F->model, nc, m0.data(), g0.data());
+  if (rc != TPA_OK) return rc;
+  std::vector<cplx> v, dr, di, c;
+  coefficients(F, y, v, dr, di, c);
+  for (int k = 0; k < P; ++k) {
+    F->coef[2 * k] = c[k].real();
+    F->coef[2 * k + 1] = c[k].imag();
+  }
+  const int flags = TPA_FLAG_SHAPE | (grad ? TPA_FLAG_GRAD : 0);
+  std::vector<cplx> Gd(3 * P, 0.0), Gm(3 * P, 0.0);
+  double sd = 0.0, smc = 0.0;
+  for (int pass = 0; pass < 2; ++pass) {
+    auto& sets = pass == 0 ? F->data : F->mc;
+    auto& G = pass == 0 ? Gd : Gm;
+    double& acc = pass == 0 ? sd : smc;
+    for (tpa_events_t ev : sets) {
+      rc = tpa_eval(ev, F->coef.data(), pass == 0 ? TPA_MODE_DATA : TPA_MODE_MC, flags,
+                    F->out.data(), nullptr);
+      if (rc != TPA_OK) return rc;
+      acc += F->out[0];
+      if (grad)
+        for (int j = 0; j < 3 * P; ++j) G[j] += cplx(F->out[2 + 2 * j], F->out[3 + 2 * j]);
+    }
+  }
+  *nll = -sd + F->sw * std::log(smc);
+  if (!grad) return TPA_OK;
+  const double r = F->sw / smc;
+  std::vector<cplx> F1(P), Fm(P), Fg(P);
+  for (int k = 0; k < P; ++k) {
+    F1[k] = -Gd[k] + r * Gm[k];
+    Fm[k] = -Gd[P + k] + r * Gm[P + k];
+    Fg[k] = -Gd[2 * P + k] + r * Gm[2 * P + k];
+  }
+  grad_from_F1(F, v.data(), dr.data(), di.data(), F1.data(), grad);
+  for (int k = 0; k < P; ++k) {
+    const int ch = S.wave_chain[k];
+    if (S.mass_var[ch] >= 0)
+      grad[S.mass_var[ch]] += 2.0 * (c[k] * (dlog[k] * F1[k] + Fm[k])).real();
+    if (S.width_var[ch] >= 0) grad[S.width_var[ch]] += 2.0 * (c[k] * Fg[k]).real();
+  }
+  return TPA_OK;
+}
+
+}  // namespace
 
 extern "C" {
 
@@ -39,7 +170,7 @@ int tpa_fcn_create(const TpaCouplingMap* map, const tpa_events_t* data, int n_da
   if (!gram) F->mc.assign(mc, mc + n_mc);
   F->sw = sw;
   F->coef.resize(2 * F->P);
-  F->out.resize(2 + 2 * F->P);
+  F->out.resize(2 + 6 * F->P);  // room for the shape pass (G, Dm, Dg)
   *out = F;
   return TPA_OK;
 }
@@ -49,8 +180,34 @@ int tpa_fcn_destroy(tpa_fcn_t F) {
   return TPA_OK;
 }
 
+int tpa_fcn_set_shape_map(tpa_fcn_t F, tpa_model_t model, const TpaShapeMap* map) {
+  if (!F || !model || !map) return tpa_set_error(TPA_ERR_ARG, "null pointer");
+  if (!F->gram.empty() || F->mc.empty())
+    return tpa_set_error(TPA_ERR_ARG, "floating lineshapes need the MC event sets, not a Gram matrix");
+  if (map->P != F->P || map->n_chain < 1 || map->n_chain > TPA_SHAPE_MAX_CHAIN)
+    return tpa_set_error(TPA_ERR_ARG, "shape map sizes do not match the session");
+  for (int c = 0; c < map->n_chain; ++c) {
+    if (map->mass_var[c] >= F->nv || map->width_var[c] >= F->nv)
+      return tpa_set_error(TPA_ERR_ARG, "shape variable index out of range");
+    if (map->n_l1[c] < 1 || map->n_l1[c] > TPA_SHAPE_MAX_LS || map->n_l2[c] < 1 ||
+        map->n_l2[c] > TPA_SHAPE_MAX_LS)
+      return tpa_set_error(TPA_ERR_ARG, "bad LS counts in the shape map");
+  }
+  for (int k = 0; k < F->P; ++k) {
+    const int c = map->wave_chain[k];
+    if (c < 0 || c >= map->n_chain || map->wave_ls1[k] < 0 || map->wave_ls1[k] >= map->n_l1[c] ||
+        map->wave_ls2[k] < 0 || map->wave_ls2[k] >= map->n_l2[c])
+      return tpa_set_error(TPA_ERR_ARG, "bad wave entry in the shape map");
+  }
+  F->sm = *map;
+  F->model = model;
+  F->shape = true;
+  return TPA_OK;
+}
+
 int tpa_fcn_nll_grad(tpa_fcn_t F, const double* y, double* nll, double* grad) {
   if (!F || !y || !nll) return tpa_set_error(TPA_ERR_ARG, "null pointer");
+  if (F->shape) return shape_nll_grad(F, y, nll, grad);
   const int P = F->P;
   std::vector<cplx> v, dr, di, c, Sk, F1(P, 0.0);
   coefficients(F, y, v, dr, di, c);

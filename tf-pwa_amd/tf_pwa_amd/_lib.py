@@ -137,8 +137,36 @@ class TpaCouplingMap(ctypes.Structure):
     ]
 
 
+SHAPE_MAX_CHAIN = 32
+SHAPE_MAX_LS = 8
+SHAPE_MAX_P = 128
+ERR_STALE = -4
+
+
+class TpaShapeMap(ctypes.Structure):
+    _fields_ = [
+        ("n_chain", _i32), ("P", _i32),
+        ("mass_var", _i32 * SHAPE_MAX_CHAIN), ("width_var", _i32 * SHAPE_MAX_CHAIN),
+        ("mass_fixed", ctypes.c_double * SHAPE_MAX_CHAIN),
+        ("width_fixed", ctypes.c_double * SHAPE_MAX_CHAIN),
+        ("m_top", ctypes.c_double),
+        ("m_bach", ctypes.c_double * SHAPE_MAX_CHAIN), ("m_y", ctypes.c_double * SHAPE_MAX_CHAIN),
+        ("m_z", ctypes.c_double * SHAPE_MAX_CHAIN), ("d", ctypes.c_double * SHAPE_MAX_CHAIN),
+        ("n_l1", _i32 * SHAPE_MAX_CHAIN), ("n_l2", _i32 * SHAPE_MAX_CHAIN),
+        ("l1", (_i32 * SHAPE_MAX_LS) * SHAPE_MAX_CHAIN), ("l2", (_i32 * SHAPE_MAX_LS) * SHAPE_MAX_CHAIN),
+        ("act1", (_i32 * SHAPE_MAX_LS) * SHAPE_MAX_CHAIN),
+        ("act2", (_i32 * SHAPE_MAX_LS) * SHAPE_MAX_CHAIN),
+        ("wave_chain", _i32 * SHAPE_MAX_P), ("wave_ls1", _i32 * SHAPE_MAX_P),
+        ("wave_ls2", _i32 * SHAPE_MAX_P),
+    ]
+
+
 class EngineError(RuntimeError):
     pass
+
+
+class StaleCache(EngineError):
+    """The barrier clamp flags changed: the caches need a rebuild."""
 
 
 _lib = None
@@ -202,6 +230,8 @@ def lib():
                             ctypes.POINTER(vp)], ctypes.c_int),
         "tpa_fcn_destroy": ([vp], ctypes.c_int),
         "tpa_fcn_nll_grad": ([vp, dp, dp, dp], ctypes.c_int),
+        "tpa_fcn_set_shape_map": ([vp, vp, ctypes.POINTER(TpaShapeMap)], ctypes.c_int),
+        "tpa_model_set_lineshapes": ([vp, ctypes.c_int, dp, dp], ctypes.c_int),
         "tpa_server_nll_grad": ([vp, i64, i64, ctypes.c_double, dp, dp, dp], ctypes.c_int),
         "tpa_server_nll_grad_hess": ([vp, i64, i64, ctypes.c_double, dp, dp, dp, dp],
                                      ctypes.c_int),
@@ -228,7 +258,8 @@ def lib():
 def check(rc):
     if rc != 0:
         msg = lib().tpa_last_error().decode(errors="replace")
-        raise EngineError("engine call failed ({}): {}".format(rc, msg))
+        cls = StaleCache if rc == ERR_STALE else EngineError
+        raise cls("engine call failed ({}): {}".format(rc, msg))
 
 
 def dptr(a):
@@ -253,5 +284,5 @@ def exported_symbols():
         "tpa_pool_free_events", "tpa_pool_eval", "tpa_server_create", "tpa_server_destroy",
         "tpa_server_nll_grad", "tpa_server_nll_grad_hess", "tpa_server_submit", "tpa_server_poll", "tpa_server_result",
         "tpa_server_latencies", "tpa_server_counters", "tpa_fcn_create", "tpa_fcn_destroy",
-        "tpa_fcn_nll_grad",
+        "tpa_fcn_nll_grad", "tpa_fcn_set_shape_map", "tpa_model_set_lineshapes",
     ]

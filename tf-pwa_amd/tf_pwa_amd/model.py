@@ -294,22 +294,25 @@ class Model:
 
 
 class _NativeSession:
-    """One tpa_fcn session (fixed lineshapes): the coupling map of the current parameter store,
-    the event sets with their weights in place and, in mixed precision, the summed MC Gram
-    matrix. Valid while the store's fixed values, the trainable set and the weights are
-    unchanged (the caller keys it on those)."""
+    """One tpa_fcn session: the coupling map of the current parameter store, the event sets
+    with their weights in place and, in mixed precision, the summed MC Gram matrix; with
+    floating masses or widths (three-body chains) also the shape map (tpa_fcn_set_shape_map).
+    Valid while the store's fixed values, the trainable set, the weights and the barrier clamp
+    flags are unchanged (the caller keys it on those)."""
 
     def __init__(self, model, data_sets, data_w, mc_sets, mc_w, sw):
         import ctypes
 
-        from .serving import coupling_map
+        from .serving import coupling_map, shape_map
 
         amp = model.Amp
+        self.amp = amp
+        self.shape = amp.shape_mode
         for s, w in zip(data_sets, data_w):
             s.set_weight(w)
             s.ensure_built()
         gram = None
-        if model.precision == "mixed":
+        if model.precision == "mixed" and not self.shape:
             I = np.zeros((amp.P, amp.P), dtype=np.complex128)
             for s, w in zip(mc_sets, mc_w):
                 s.set_weight(w)
@@ -330,6 +333,9 @@ class _NativeSession:
                                              self._mc, len(mc_sets), gram, float(sw),
                                              ctypes.byref(h)))
         self._h = h
+        if self.shape:
+            self._shape = shape_map(amp)
+            _lib.check(_lib.lib().tpa_fcn_set_shape_map(h, amp.device(), ctypes.byref(self._shape)))
         self.n = len(self._keep["bound_kind"])
         self._g = np.zeros(self.n)
         self._nll = ctypes.c_double()
@@ -339,8 +345,13 @@ class _NativeSession:
 
         y = np.ascontiguousarray(y, dtype=np.float64)
         g = np.zeros(self.n) if grad else None
-        _lib.check(_lib.lib().tpa_fcn_nll_grad(self._h, _lib.dptr(y), ctypes.byref(self._nll),
-                                               _lib.dptr(g)))
+        rc = _lib.lib().tpa_fcn_nll_grad(self._h, _lib.dptr(y), ctypes.byref(self._nll),
+                                         _lib.dptr(g))
+        if self.shape:
+            # the session set the device lineshape parameters: the Python host must upload its
+            # own description again before it evaluates
+            self.amp._desc_state = None
+        _lib.check(rc)
         return self._nll.value, g
 
     def close(self):
@@ -403,10 +414,11 @@ class FCN:
         return self.vm.get_all_dic(trainable_only)
 
     def _native_session(self):
-        """The native session for fixed lineshapes (None when masses or widths float), rebuilt
-        when the precision, fixed values, trainable set or any set's weights change."""
+        """The native session (None for floating lineshapes of four-body chains), rebuilt when
+        the precision, fixed values, trainable set or any set's weights change, or after the
+        session reported changed barrier clamp flags."""
         amp = self.model.Amp
-        if amp.shape_mode or not self.native:
+        if not self.native or (amp.shape_mode and amp.n_final != 3):
             return None
         dw, mw, sw = self._split
         sets = list(self.wdata.sets) + list(self.mc_sets)
@@ -427,12 +439,26 @@ class FCN:
             return np.asarray(x, dtype=np.float64)
         return np.array(self.vm.get_all_val())
 
+    def _native_call(self, x, grad):
+        """(nll, g) from the native session, or None when there is none or its caches went
+        stale (the Python host then rebuilds them; the next call makes a new session)."""
+        nat = self._native_session()
+        if nat is None:
+            return None
+        try:
+            out = nat.nll_grad(self._y(x), grad=grad)
+        except _lib.StaleCache:
+            self._native.close()
+            self._native, self._native_key = None, None
+            return None
+        self.model.n_eval += 1
+        return out
+
     def get_nll(self, x=None):
         self._set(x)
-        nat = self._native_session()
-        if nat is not None:
-            nll, _ = nat.nll_grad(self._y(x), grad=False)
-            self.model.n_eval += 1
+        r = self._native_call(x, grad=False)
+        if r is not None:
+            nll = r[0]
         else:
             nll, _ = self.model._nll_grad(self.wdata, self.mc_sets, self.mc_weight, grad=False,
                                           split=self._split)
@@ -445,10 +471,9 @@ class FCN:
 
     def get_nll_grad(self, x=None):
         self._set(x)
-        nat = self._native_session()
-        if nat is not None:
-            nll, g = nat.nll_grad(self._y(x))
-            self.model.n_eval += 1
+        r = self._native_call(x, grad=True)
+        if r is not None:
+            nll, g = r
         else:
             nll, g = self.model._nll_grad(self.wdata, self.mc_sets, self.mc_weight,
                                           split=self._split)

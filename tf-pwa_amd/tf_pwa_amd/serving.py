@@ -130,6 +130,45 @@ class EventPool:
         self.close()
 
 
+def shape_map(amp):
+    """The TpaShapeMap of a three-body model with floating lineshapes: per chain its mass and
+    width variables (trainable index or fixed value), the masses and LS lists of both vertices
+    for the barrier constants, the clamp flags the caches were built with, and per partial
+    wave its chain and LS indices."""
+    if amp.n_final != 3:
+        raise NotImplementedError("native floating lineshapes cover three-body chains")
+    vm = amp.vm
+    index = {n: i for i, n in enumerate(vm.trainable_vars)}
+    names = amp._names()
+    bc = amp.barrier_constants()
+    fx = amp._factor_index()
+    m = _lib.TpaShapeMap()
+    m.n_chain = len(amp.chains)
+    m.P = amp.P
+    m.m_top = float(amp.decay_group.top.mass)
+    shapes = amp.shape_params()
+    for c, (ch, nm) in enumerate(zip(amp.chains, names)):
+        R, d1, d2 = ch["R"], ch["d1"], ch["d2"]
+        m.mass_var[c] = index.get(nm["mass"], -1)
+        m.width_var[c] = index.get(nm["width"], -1)
+        m.mass_fixed[c], m.width_fixed[c] = shapes[c]
+        bach = d1.outs[1] if d1.outs[0] is R else d1.outs[0]
+        m.m_bach[c] = float(bach.mass)
+        m.m_y[c], m.m_z[c] = float(d2.outs[0].mass), float(d2.outs[1].mass)
+        m.d[c] = float(d1.d)
+        m.n_l1[c], m.n_l2[c] = len(d1.l_list), len(d2.l_list)
+        for s, l in enumerate(d1.l_list):
+            m.l1[c][s] = int(l)
+            m.act1[c][s] = int(bc[c]["act"][0][s])
+        for s, l in enumerate(d2.l_list):
+            m.l2[c][s] = int(l)
+            m.act2[c][s] = int(bc[c]["act"][1][s])
+    for k, (c, ls) in enumerate(zip(fx["ch"], fx["ls"])):
+        m.wave_chain[k] = int(c)
+        m.wave_ls1[k], m.wave_ls2[k] = int(ls[0]), int(ls[1])
+    return m
+
+
 def coupling_map(amp):
     """The TpaCouplingMap of ``amp``'s current parameter store: per partial wave its three
     complex factors and barrier constant, per factor its trainable indices or fixed values,
