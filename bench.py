@@ -258,14 +258,14 @@ def serving_leg(conf, mc, data_p4, x_true, args, seed=4242):
 def multistart_leg(args, seed=3303, starts=8, n_data=10_000, n_mc=100_000):
     """Floating lineshapes: the six-resonance model with all 6 masses and widths trainable
     (34 coupling + 12 lineshape parameters, tf-pwa's factorised parameterisation), 10k
-    accept-reject data and 100k MC, FP64 path. A full scipy BFGS fit from each of ``starts``
-    seeded starting points (couplings |g| ~ U(0.5, 2), arg g ~ U(-pi, pi); masses and widths
-    at their nominal values, bounded to m0 +- 0.05 GeV and [g0 / 2, 2 g0]); reports NLL+grad
-    evals/s over all fits."""
+    accept-reject data and 100k MC, FP64 path. A full BFGS fit from each of ``starts`` seeded
+    starting points (couplings |g| ~ U(0.5, 2), arg g ~ U(-pi, pi); masses and widths at their
+    nominal values, bounded to m0 +- 0.05 GeV and [g0 / 2, 2 g0]) with the engine's BFGS, and
+    the first two again with scipy's; reports NLL+grad evals/s over the fits."""
     import yaml
 
     from tf_pwa_amd import FCN, ConfigLoader
-    from tf_pwa_amd.fit import fit_scipy
+    from tf_pwa_amd.fit import fit_native, fit_scipy
     from tf_pwa_amd.phasespace import generate_phsp
 
     cfg = yaml.safe_load(open(CONFIG))
@@ -296,17 +296,26 @@ def multistart_leg(args, seed=3303, starts=8, n_data=10_000, n_mc=100_000):
     data = conf.data_from_p4({n: np.concatenate(v)[:n_data] for n, v in keep.items()})
     fcn = FCN(conf.get_model(), data, mc)
     n_par = len(conf.vm.trainable_vars)
-    results, nfev, t0 = [], 0, time.perf_counter()
-    for k in range(starts):
-        conf.set_params(nominal)
-        conf.set_params(random_couplings(conf.vm, seed + 10 + k))
-        c0 = fcn.n_call
-        r = fit_scipy(fcn)
-        nfev += fcn.n_call - c0
-        results.append(r)
-        log("multistart {}: NLL {:.4f} ({} evals, {})".format(k, r.min_nll, fcn.n_call - c0,
-                                                             r.message))
-    wall = time.perf_counter() - t0
+
+    def fits(driver, n):
+        out, nf, t = [], 0, time.perf_counter()
+        for k in range(n):
+            conf.set_params(nominal)
+            conf.set_params(random_couplings(conf.vm, seed + 10 + k))
+            c0 = fcn.n_call
+            r = driver(fcn)
+            nf += fcn.n_call - c0
+            out.append(r)
+            log("multistart {} ({}): NLL {:.4f} ({} evals, {})".format(
+                k, driver.__name__, r.min_nll, fcn.n_call - c0, r.message))
+        return out, nf, time.perf_counter() - t
+
+    # the engine's BFGS (fit_native) for every start; scipy's (fit_scipy, the reference's
+    # driver) on the first two starts for comparison; one evaluation first loads the shape
+    # kernels' code
+    fcn.nll_grad()
+    results, nfev, wall = fits(fit_native, starts)
+    sres, snfev, swall = fits(fit_scipy, min(2, starts))
     best = min(r.min_nll for r in results)
     # FCN.nll_grad alone (every call at a new point, as in a fit): the engine's rate without
     # the minimiser's own host time
@@ -319,12 +328,15 @@ def multistart_leg(args, seed=3303, starts=8, n_data=10_000, n_mc=100_000):
     for xs in steps:
         fcn.nll_grad(xs)
     fcn_rate = len(steps) / (time.perf_counter() - tf)
-    log("floating lineshapes: {:.0f} evals/s in the fits, FCN.nll_grad alone {:.0f} evals/s".format(
-        nfev / wall, fcn_rate))
+    log("floating lineshapes: {:.0f} evals/s in native BFGS fits ({:.0f} with scipy's), "
+        "FCN.nll_grad alone {:.0f} evals/s".format(nfev / wall, snfev / swall, fcn_rate))
     return {"workload": "Lc->pKpi 6 resonances, all masses and widths floating ({} parameters: "
                         "34 couplings + 12 lineshape), {} data + {} MC, {} BFGS fits from seeded "
                         "starts, FP64".format(n_par, n_data, n_mc, starts),
             "evals_per_s": nfev / wall, "fcn_evals_per_s": fcn_rate, "evals": nfev,
+            "driver": "fit_native (engine BFGS)",
+            "scipy_driver": {"fits": len(sres), "evals": snfev, "evals_per_s": snfev / swall,
+                             "nll": [r.min_nll for r in sres]},
             "wall_s": wall, "n_params": n_par,
             "best_nll": best, "nll": [r.min_nll for r in results],
             "fits_at_best": int(sum(abs(r.min_nll - best) < 1e-3 for r in results)),

@@ -343,6 +343,16 @@ typedef struct TpaShapeMap {
  * (its device lineshape parameters are set per evaluation). */
 int tpa_fcn_set_shape_map(tpa_fcn_t fcn, tpa_model_t model, const TpaShapeMap* map);
 
+/* A whole fit on the session: BFGS from y0 (physical values) in the unbounded variables of the
+ * bound transforms, strong-Wolfe line search, until |grad|_inf <= gtol in those variables or
+ * maxiter iterations (<= 0: 200 n), every step one tpa_fcn_nll_grad call. The loop of
+ * fit_scipy / scipy.optimize.minimize(method="BFGS") (tf_pwa/fit.py:189-300) without a
+ * round trip to the host interpreter per evaluation. status: 0 converged, 1 iteration limit,
+ * 2 line search failed (precision loss). Returns the error of a failed evaluation (e.g.
+ * TPA_ERR_STALE) with nfev the evaluations done. */
+int tpa_fcn_minimize(tpa_fcn_t fcn, const double* y0, double gtol, int maxiter, double* y_out,
+                     double* nll_out, int* status, int64_t* nfev, int64_t* niter);
+
 /* Set the lineshape parameters (m0, g0) of every chain of a three-body model on the device,
  * ordered with later work on the model's stream (the fields of TpaChain that
  * tpa_model_update would replace; no other field changes). */

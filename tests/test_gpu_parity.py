@@ -564,3 +564,22 @@ def test_native_shape_session_matches_python_assembly(cfg, gpu_available):
         n2, g2 = fp.nll_grad(x)
         assert abs(n1 - n2) <= 1e-11 * max(1.0, abs(n2)), (n1, n2)
         np.testing.assert_allclose(g1, g2, rtol=0, atol=1e-9 * np.abs(g2).max())
+
+
+def test_native_fit_matches_scipy_fit(case1, gpu_available):
+    """fit_native (the engine's BFGS in tpa_fcn_minimize) and fit_scipy reach the same minimum
+    from the same start (couplings of the 2-resonance model, 800 weighted data events)."""
+    from tf_pwa_amd import FCN
+    from tf_pwa_amd.fit import fit_native, fit_scipy
+
+    conf = case1["conf"]
+    data = conf.data_from_p4(case1["data"])
+    mc = conf.data_from_p4(case1["mc"])
+    fcn = FCN(conf.get_model(), data, mc)
+    start = conf.vm.get_all_dic()
+    rn = fit_native(fcn)
+    conf.set_params(start)
+    rs = fit_scipy(fcn)
+    assert rn.success
+    assert abs(rn.min_nll - rs.min_nll) < 1e-6 * max(1.0, abs(rs.min_nll)), (rn.min_nll, rs.min_nll)
+    assert rn.nfev > 0

@@ -10,6 +10,7 @@
 #include <vector>
 
 #include "tfpwa_amd.h"
+#include "tpa_bfgs.h"
 #include "tpa_coupling.h"
 #include "tpa_internal.h"
 
@@ -243,6 +244,36 @@ int tpa_fcn_nll_grad(tpa_fcn_t F, const double* y, double* nll, double* grad) {
     for (int k = 0; k < P; ++k) F1[k] += r * Sk[k];
     grad_from_F1(F, v.data(), dr.data(), di.data(), F1.data(), grad);
   }
+  return TPA_OK;
+}
+
+int tpa_fcn_minimize(tpa_fcn_t F, const double* y0, double gtol, int maxiter, double* y_out,
+                     double* nll_out, int* status, int64_t* nfev, int64_t* niter) {
+  if (!F || !y0 || !y_out || !nll_out || !status) return tpa_set_error(TPA_ERR_ARG, "null pointer");
+  if (!(gtol > 0.0)) return tpa_set_error(TPA_ERR_ARG, "gtol must be positive");
+  struct Failed {
+    int rc;
+  };
+  int64_t calls = 0;
+  auto fy = [&](const double* y, double* gy) {
+    double f = 0.0;
+    const int rc = tpa_fcn_nll_grad(F, y, &f, gy);
+    if (rc != TPA_OK) throw Failed{rc};
+    ++calls;
+    return f;
+  };
+  BfgsResult r;
+  try {
+    r = bfgs_minimize(F, y0, fy, gtol, maxiter, nullptr);
+  } catch (const Failed& e) {
+    if (nfev) *nfev = calls;
+    return e.rc;  // the message is already set
+  }
+  std::copy(r.y.begin(), r.y.end(), y_out);
+  *nll_out = r.f;
+  *status = r.status;
+  if (nfev) *nfev = calls;
+  if (niter) *niter = r.niter;
   return TPA_OK;
 }
 

@@ -231,6 +231,9 @@ def lib():
         "tpa_fcn_destroy": ([vp], ctypes.c_int),
         "tpa_fcn_nll_grad": ([vp, dp, dp, dp], ctypes.c_int),
         "tpa_fcn_set_shape_map": ([vp, vp, ctypes.POINTER(TpaShapeMap)], ctypes.c_int),
+        "tpa_fcn_minimize": ([vp, dp, ctypes.c_double, ctypes.c_int, dp, dp,
+                              ctypes.POINTER(ctypes.c_int), ctypes.POINTER(i64),
+                              ctypes.POINTER(i64)], ctypes.c_int),
         "tpa_model_set_lineshapes": ([vp, ctypes.c_int, dp, dp], ctypes.c_int),
         "tpa_server_nll_grad": ([vp, i64, i64, ctypes.c_double, dp, dp, dp], ctypes.c_int),
         "tpa_server_nll_grad_hess": ([vp, i64, i64, ctypes.c_double, dp, dp, dp, dp],
@@ -284,5 +287,5 @@ def exported_symbols():
         "tpa_pool_free_events", "tpa_pool_eval", "tpa_server_create", "tpa_server_destroy",
         "tpa_server_nll_grad", "tpa_server_nll_grad_hess", "tpa_server_submit", "tpa_server_poll", "tpa_server_result",
         "tpa_server_latencies", "tpa_server_counters", "tpa_fcn_create", "tpa_fcn_destroy",
-        "tpa_fcn_nll_grad", "tpa_fcn_set_shape_map", "tpa_model_set_lineshapes",
+        "tpa_fcn_nll_grad", "tpa_fcn_set_shape_map", "tpa_model_set_lineshapes", "tpa_fcn_minimize",
     ]

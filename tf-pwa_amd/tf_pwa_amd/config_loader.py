@@ -233,9 +233,13 @@ class ConfigLoader:
 
     def fit(self, data=None, phsp=None, bg=None, batch=65000, method="BFGS", maxiter=None,
             print_init_nll=False):
-        from .fit import fit_scipy
+        """ConfigLoader.fit (config_loader.py:744-812). method: a scipy.optimize.minimize
+        method, or "native" for the engine's own BFGS (fit_native)."""
+        from .fit import fit_native, fit_scipy
 
         if data is None:
             data, phsp, bg, _ = self.get_all_data()
         fcn = FCN(self.get_model(), data, phsp, bg=bg, batch=batch)
+        if method == "native":
+            return fit_native(fcn, maxiter=maxiter)
         return fit_scipy(fcn, method=method, maxiter=maxiter)

@@ -102,3 +102,54 @@ def fit_scipy(fcn, method="BFGS", maxiter=None, callback=None, check_large=1e7):
                      message=res.message, hess_inv=getattr(res, "hess_inv", None),
                      nfev=res.nfev, ngrad=getattr(res, "njev", 0) or 0,
                      time_s=time.perf_counter() - t0)
+
+
+_NATIVE_MESSAGES = {0: "Optimization terminated successfully.",
+                    1: "Maximum number of iterations has been exceeded.",
+                    2: "Desired error not necessarily achieved due to precision loss."}
+
+
+def fit_native(fcn, maxiter=None, gtol=1e-5):
+    """Minimise ``fcn`` from the current parameter values with the engine's BFGS
+    (tpa_fcn_minimize): the scheme of fit_scipy's scipy BFGS (same bound transforms, strong-
+    Wolfe line search, |grad|_inf <= gtol in the unbounded variables) run natively, one engine
+    evaluation per step. Falls back to fit_scipy when the FCN has no native session (Gaussian
+    constraints, four-body floating lineshapes, ``fcn.native = False``) or its caches go stale
+    mid-fit (barrier clamp flags changed). Returns a FitResult."""
+    import ctypes
+
+    from . import _lib
+
+    vm = fcn.vm
+    if fcn.gauss_constr:
+        return fit_scipy(fcn, maxiter=maxiter)
+    fcn._set(None)
+    nat = fcn._native_session()
+    if nat is None:
+        return fit_scipy(fcn, maxiter=maxiter)
+    y0 = np.ascontiguousarray(vm.get_all_val(), dtype=np.float64)
+    y = np.zeros_like(y0)
+    nll = ctypes.c_double()
+    status = ctypes.c_int()
+    nfev = ctypes.c_int64()
+    niter = ctypes.c_int64()
+    t0 = time.perf_counter()
+    rc = _lib.lib().tpa_fcn_minimize(nat._h, _lib.dptr(y0), float(gtol),
+                                     int(maxiter or 0), _lib.dptr(y), ctypes.byref(nll),
+                                     ctypes.byref(status), ctypes.byref(nfev), ctypes.byref(niter))
+    if getattr(nat, "shape", False):
+        fcn.model.Amp._desc_state = None
+    fcn.n_call += int(nfev.value)
+    fcn.model.n_eval += int(nfev.value)
+    if rc == _lib.ERR_STALE:
+        fcn._native.close()
+        fcn._native, fcn._native_key = None, None
+        return fit_scipy(fcn, maxiter=maxiter)
+    _lib.check(rc)
+    vm.set_all(y)
+    vm.std_polar_all()
+    fcn.cached_nll = nll.value
+    return FitResult(vm.get_all_dic(), fcn, nll.value, ndf=len(y), success=status.value == 0,
+                     message=_NATIVE_MESSAGES.get(status.value, str(status.value)),
+                     nfev=int(nfev.value), ngrad=int(nfev.value),
+                     time_s=time.perf_counter() - t0)
