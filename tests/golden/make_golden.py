@@ -3,7 +3,7 @@
 outputs on them (per-event |A|^2 of data and MC, NLL, gradient and autograd Hessian in
 trainable_vars order). The GPU tests compare the engine with these committed numbers; a CPU test
 re-runs the oracle on the same inputs so that a change of the oracle shows up as a fixture
-mismatch. Run in the build container: python tests/golden/make_golden.py"""
+mismatch. Run in the build container: python tests/golden/make_golden.py [case names]"""
 
 import json
 import os
@@ -15,8 +15,11 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(os.path.dirname(HERE))
 sys.path.insert(0, ROOT)
 
-CASES = {"lc_pkpi_2res": 101, "lc_pkpi_6res": 202}
+CASES = {"lc_pkpi_2res": 101, "lc_pkpi_6res": 202, "jpsi_kkpipi_12ch": 303}
 N_DATA, N_MC = 64, 192
+# four-body cases: weighted flat phase space as data (the likelihood does not assume the data
+# follow the model), from the host GENBOD generator
+FOUR_BODY = {"jpsi_kkpipi_12ch"}
 
 
 def config_file(name):
@@ -44,15 +47,28 @@ def compute(name, inputs):
             "trainable": list(om.trainable)}
 
 
-def main():
+def main(names=None):
     from oracle import tfpwa_oracle as O
 
     for name, seed in CASES.items():
+        if names and name not in names:
+            continue
         om = O.Model(config_file(name))
         params = O.random_couplings(om, seed)
         om.set_params(params)
-        data = O.gen_data(om, N_DATA, seed + 1, batch=20000)
-        mc = O.gen_mc(om, N_MC, seed + 2)
+        if name in FOUR_BODY:
+            sys.path.insert(0, os.path.join(ROOT, "tf-pwa_amd"))
+            from tf_pwa_amd.phasespace import generate_phsp
+
+            order = om.config["data"]["dat_order"]
+            masses = [om.particles[n].mass for n in order]
+            data = dict(zip(order, generate_phsp(om.top.mass, masses, N_DATA,
+                                                 np.random.default_rng(seed + 1))))
+            mc = dict(zip(order, generate_phsp(om.top.mass, masses, N_MC,
+                                               np.random.default_rng(seed + 2))))
+        else:
+            data = O.gen_data(om, N_DATA, seed + 1, batch=20000)
+            mc = O.gen_mc(om, N_MC, seed + 2)
         wd = np.random.default_rng(seed + 3).uniform(0.5, 1.5, N_DATA)
         inputs = {"config": name + ".yml", "seed": seed,
                   "params": {k: float(v) for k, v in om.values.items()},
@@ -67,4 +83,4 @@ def main():
 
 
 if __name__ == "__main__":
-    sys.exit(main())
+    sys.exit(main(sys.argv[1:]))

@@ -131,7 +131,7 @@ def test_native_session_matches_python_assembly(case2, precision, gpu_available)
     conf.vm.set_all(x0)
 
 
-@pytest.mark.parametrize("name", ["lc_pkpi_2res", "lc_pkpi_6res"])
+@pytest.mark.parametrize("name", ["lc_pkpi_2res", "lc_pkpi_6res", "jpsi_kkpipi_12ch"])
 def test_engine_matches_committed_golden_vectors(name, gpu_available):
     """The FP64 engine on the committed golden inputs against the oracle's committed outputs:
     per-event |A|^2 (1e-9 relative), NLL (1e-6 absolute), gradient (1e-7 relative) and the

@@ -213,7 +213,7 @@ def test_oracle_gradient_finite_difference(small_model):
         assert abs(fd - g[i]) < 1e-5 * max(1.0, abs(g[i])), (i, fd, g[i])
 
 
-@pytest.mark.parametrize("name", ["lc_pkpi_2res", "lc_pkpi_6res"])
+@pytest.mark.parametrize("name", ["lc_pkpi_2res", "lc_pkpi_6res", "jpsi_kkpipi_12ch"])
 def test_oracle_reproduces_committed_golden_vectors(name):
     """The committed golden vectors are the oracle's outputs on their inputs: re-running the
     oracle must give the same f, NLL, gradient and Hessian (a regression pin of the oracle)."""
