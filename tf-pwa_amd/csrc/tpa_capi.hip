@@ -231,9 +231,10 @@ struct tpa_events_s {
 
 // --------------------------------------------------------------------------- dispatcher
 // Partial waves are split over KG lane groups so that each lane holds at most 8 of them.
-// The default evaluation kernel is k_eval_pipe with PML = ceil(P / KG) rounded up to 4, 6 or
-// 8 (the register stages and accumulators of unused slots cost occupancy): on the MI355X it
-// streams 6.4 TB/s on 1M and 10M events, against 5.0-5.3 TB/s for k_eval_fold<.., U = 2>.
+// The default evaluation kernel is k_eval_pipe with PML = ceil(P / KG) rounded up to 4, 6, 7
+// or 8 (the register stages and accumulators of unused slots cost occupancy; up to 6 it runs
+// three waves per SIMD): on the MI355X it streams 6.5-6.7 TB/s on 1M and 10M events, against
+// 5.0-5.3 TB/s for k_eval_fold<.., U = 2>.
 // TFPWA_EVAL_UNROLL selects k_eval_fold instead, for comparison: 1 (U = 1), 2 (U = 2),
 // 3 (U = 2 with ordinary loads).
 static int eval_unroll() {
@@ -254,6 +255,8 @@ static void launch_eval(const EvalArgs& a, int grid, hipStream_t s) {
         hipLaunchKernelGGL((k_eval_pipe<LG, KG, 4>), dim3(grid), dim3(256), 0, s, a);
       else if (need <= 6)
         hipLaunchKernelGGL((k_eval_pipe<LG, KG, 6>), dim3(grid), dim3(256), 0, s, a);
+      else if (need <= 7)
+        hipLaunchKernelGGL((k_eval_pipe<LG, KG, 7>), dim3(grid), dim3(256), 0, s, a);
       else
         hipLaunchKernelGGL((k_eval_pipe<LG, KG, 8>), dim3(grid), dim3(256), 0, s, a);
       break;
@@ -270,6 +273,10 @@ static void launch_eval(const EvalArgs& a, int grid, hipStream_t s) {
   }
 }
 
+// k-groups of the evaluation passes: the fewest that leave at most 8 waves per lane, with an
+// event's lanes in one wave. (More k-groups for fewer waves per lane raise the occupancy but
+// cut the loads each wave keeps in flight: at P = 25, two helicities, 8 groups of 4 waves ran
+// 13% slower than 4 groups of 7.)
 static int eval_kgroups(int lg, int P) {
   int kg = 1;
   while (kg * 8 < P) kg <<= 1;

@@ -567,7 +567,7 @@ TPA_DEV void eval_group(int mode, bool grad, double* __restrict__ f_out,
 // whose values meet a zero coefficient or a zero weight. The loop bounds are wave-uniform
 // (scalar), so every lane stays active for the cross-lane exchanges. Requires N >= 1.
 template <int LG, int KG, int PML>
-__global__ __launch_bounds__(256) void k_eval_pipe(EvalArgs a) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(PML <= 6 ? 3 : 1))) void k_eval_pipe(EvalArgs a) {
   constexpr int LPE = LG * KG;
   constexpr int EPW = 64 / LPE;
   const int lane = threadIdx.x & 63;
