@@ -557,6 +557,8 @@ static void launch_multi(const MultiArgs& a, int grid, hipStream_t s) {
     hipLaunchKernelGGL((k_eval_multi<LG, KG, 4>), dim3(grid), dim3(256), 0, s, a);
   else if (need <= 6)
     hipLaunchKernelGGL((k_eval_multi<LG, KG, 6>), dim3(grid), dim3(256), 0, s, a);
+  else if (need <= 7)
+    hipLaunchKernelGGL((k_eval_multi<LG, KG, 7>), dim3(grid), dim3(256), 0, s, a);
   else
     hipLaunchKernelGGL((k_eval_multi<LG, KG, 8>), dim3(grid), dim3(256), 0, s, a);
 }

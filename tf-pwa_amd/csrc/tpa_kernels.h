@@ -1018,7 +1018,7 @@ struct MultiArgs {
 };
 
 template <int LG, int KG, int PML>
-__global__ __launch_bounds__(256) void k_eval_multi(MultiArgs a) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(PML <= 6 ? 3 : 1))) void k_eval_multi(MultiArgs a) {
   const int chunk = blockIdx.x;
   if (chunk >= a.counts[0]) return;  // uniform per block, before any barrier
   constexpr int LPE = LG * KG;
