@@ -31,9 +31,6 @@ typedef struct tpa_fcn_s* tpa_fcn_t;
 #define TPA_ERR_ARG -1
 #define TPA_ERR_HIP -2
 #define TPA_ERR_UNSUPPORTED -3
-/* The barrier clamp flags at the requested masses differ from those the partial-wave caches
- * were built with: rebuild the caches (tpa_events_build) and the session. */
-#define TPA_ERR_STALE -4
 
 /* Evaluation modes (tpa_eval). DATA: sum_e w_e * clip_log(f_e) (tf_pwa/model/model.py:41-54
  * with trans=clip_log, 283-292). MC: sum_e w_e * f_e (the same function with identity). */
@@ -318,8 +315,10 @@ int tpa_fcn_nll_grad(tpa_fcn_t fcn, const double* y, double* nll, double* grad);
  * (TPA_FLAG_SHAPE) and adds the mass and width gradient:
  *   dNLL/dm0_c = 2 Re sum_{k in c} c_k (dlog_k F_k + Dm_k),  dNLL/dg0_c = 2 Re sum_{k in c} c_k Dg_k
  * with dlog_k the m0 derivative of log(wave constant). The caches must be built without
- * lineshapes (fold 0) and with the clamp flags act1/act2; when the flags at the requested
- * masses differ, tpa_fcn_nll_grad returns TPA_ERR_STALE. The MC normalisation is the full MC
+ * lineshapes (fold 0) and with the clamp flags act1/act2; when a mass crosses the point where
+ * a flag changes, the session sets the new flags on the device and rebuilds the caches of its
+ * event sets (tpa_fcn_cache_rebuilds counts this, so that a host keeping its own view of the
+ * caches can refresh it). The MC normalisation is the full MC
  * pass (a Gram matrix cannot carry per-event lineshapes). Replaces FCN.nll_grad of models with
  * `float: mg` resonances (tf_pwa/model/model.py:1207-1240, the mass/width Variables of
  * tf_pwa/amp/core.py:380-390). */
@@ -343,13 +342,17 @@ typedef struct TpaShapeMap {
  * (its device lineshape parameters are set per evaluation). */
 int tpa_fcn_set_shape_map(tpa_fcn_t fcn, tpa_model_t model, const TpaShapeMap* map);
 
+/* Cache rebuilds the session made after barrier clamp flags changed (no reference
+ * counterpart: a bookkeeping counter for hosts that track the caches' state). */
+int tpa_fcn_cache_rebuilds(tpa_fcn_t fcn, int64_t* n);
+
 /* A whole fit on the session: BFGS from y0 (physical values) in the unbounded variables of the
  * bound transforms, strong-Wolfe line search, until |grad|_inf <= gtol in those variables or
  * maxiter iterations (<= 0: 200 n), every step one tpa_fcn_nll_grad call. The loop of
  * fit_scipy / scipy.optimize.minimize(method="BFGS") (tf_pwa/fit.py:189-300) without a
  * round trip to the host interpreter per evaluation. status: 0 converged, 1 iteration limit,
- * 2 line search failed (precision loss). Returns the error of a failed evaluation (e.g.
- * TPA_ERR_STALE) with nfev the evaluations done. */
+ * 2 line search failed (precision loss). Returns the error of a failed evaluation with nfev
+ * the evaluations done. */
 int tpa_fcn_minimize(tpa_fcn_t fcn, const double* y0, double gtol, int maxiter, double* y_out,
                      double* nll_out, int* status, int64_t* nfev, int64_t* niter);
 

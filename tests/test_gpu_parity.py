@@ -566,6 +566,60 @@ def test_native_shape_session_matches_python_assembly(cfg, gpu_available):
         np.testing.assert_allclose(g1, g2, rtol=0, atol=1e-9 * np.abs(g2).max())
 
 
+def test_native_session_rebuilds_on_clamp_flip(gpu_available):
+    """K*(1430)'s production-vertex barrier flag flips between m = 1.4481 and 1.4482 GeV
+    (P_l(q0^2 d^2) changes sign): the native session rebuilds the caches itself when a mass
+    crosses it, and the Python host, sharing the same event sets, sees its caches as stale.
+    Order P(lo), N(lo), N(hi), P(lo): the last Python pass reads caches the native session
+    rebuilt with the other flags unless it rebuilds them."""
+    import yaml
+
+    from conftest import config_path
+    from tf_pwa_amd import FCN, ConfigLoader
+    from tf_pwa_amd.phasespace import generate_phsp
+
+    c = yaml.safe_load(open(config_path("lc_pkpi_6res.yml")))
+    for name, p in c["particle"].items():
+        if not name.startswith("$") and isinstance(p, dict) and "width" in p:
+            p["float"] = "mg"
+    conf = ConfigLoader(c, seed=4)
+    order = conf.dat_order
+    masses = [conf.particles[n].mass for n in order]
+    top = conf.particles[conf.top_name].mass
+    rng = np.random.default_rng(5)
+    data = conf.data_from_p4(dict(zip(order, generate_phsp(top, masses, 3000, rng))))
+    mc = conf.data_from_p4(dict(zip(order, generate_phsp(top, masses, 20000, rng))))
+    fn = FCN(conf.get_model(), data, mc)
+    fp = FCN(conf.get_model(), data, mc)
+    fp.native = False
+    names = conf.vm.trainable_vars
+    i = names.index("K0st_mass")
+    lo = np.array(conf.vm.get_all_val())
+    lo[i] = 1.440
+    hi = lo.copy()
+    hi[i] = 1.456
+    amp = conf.get_amplitude()
+    flags = []
+    for x in (lo, hi):
+        conf.vm.set_all(x)
+        flags.append([tuple(b["act1"]) for b in amp.barrier_constants()])
+    assert flags[0] != flags[1]  # the two points sit on either side of the flip
+
+    def close(a, b):
+        assert abs(a[0] - b[0]) <= 1e-11 * max(1.0, abs(b[0])), (a[0], b[0])
+        np.testing.assert_allclose(a[1], b[1], rtol=0, atol=1e-9 * np.abs(b[1]).max())
+
+    p_lo = fp.nll_grad(lo)
+    n_lo = fn.nll_grad(lo)
+    close(n_lo, p_lo)
+    n_hi = fn.nll_grad(hi)
+    assert fn._native._seen_rebuilds == 1
+    close(fp.nll_grad(lo), p_lo)
+    close(n_hi, fp.nll_grad(hi))
+    close(fn.nll_grad(lo), p_lo)
+    assert fn._native._seen_rebuilds == 2
+
+
 def test_native_fit_matches_scipy_fit(case1, gpu_available):
     """fit_native (the engine's BFGS in tpa_fcn_minimize) and fit_scipy reach the same minimum
     from the same start (couplings of the 2-resonance model, 800 weighted data events)."""

@@ -751,6 +751,22 @@ int tpa_model_set_lineshapes(tpa_model_t m, int n_chain, const double* m0, const
   return TPA_OK;
 }
 
+int tpa_model_set_barrier_flags(tpa_model_t m, int n_chain, const int32_t* act1,
+                                const int32_t* act2) {
+  if (!m || !act1 || !act2) return fail(TPA_ERR_ARG, "null pointer");
+  if (m->kind != 3) return fail(TPA_ERR_UNSUPPORTED, "barrier flags of a three-body model");
+  if (n_chain != m->h.n_chain) return fail(TPA_ERR_ARG, "chain count mismatch");
+  if (!m->h_pin) HIPCHK(hipHostMalloc((void**)&m->h_pin, sizeof(TpaModel)));
+  HIPCHK(hipStreamSynchronize(m->stream));
+  for (int c = 0; c < n_chain; ++c) {
+    for (int s = 0; s < m->h.chain[c].n_ls1; ++s) m->h.chain[c].act1[s] = act1[c * TPA_MAX_LS + s];
+    for (int s = 0; s < m->h.chain[c].n_ls2; ++s) m->h.chain[c].act2[s] = act2[c * TPA_MAX_LS + s];
+  }
+  std::memcpy(m->h_pin, &m->h, sizeof(TpaModel));
+  HIPCHK(hipMemcpyAsync(m->d, m->h_pin, sizeof(TpaModel), hipMemcpyHostToDevice, m->stream));
+  return TPA_OK;
+}
+
 int tpa_events_destroy(tpa_events_t ev) {
   if (!ev) return TPA_OK;
   if (ev->d_ang) (void)hipFree(ev->d_ang);

@@ -13,6 +13,7 @@
 #include "tpa_bfgs.h"
 #include "tpa_coupling.h"
 #include "tpa_internal.h"
+#include "tpa_model.h"
 
 using namespace tpa_cpl;
 
@@ -24,6 +25,7 @@ struct tpa_fcn_s : CouplingMap {
   bool shape = false;
   tpa_model_t model = nullptr;
   TpaShapeMap sm{};
+  int64_t rebuilds = 0;  // cache rebuilds after barrier clamp flags changed
 };
 
 namespace {
@@ -62,6 +64,15 @@ double dq2_ddaughter(double M, double m1, double m2) {  // d / d m1
   return dF / (4.0 * M * M);
 }
 
+// The clamp flags (P_l(q0^2 d^2) > 0) of both vertices of chain c at mass m0.
+void barrier_flags(const TpaShapeMap& S, int c, double m0, int32_t* a1, int32_t* a2) {
+  const double d2 = S.d[c] * S.d[c];
+  const double q01 = q2_of(S.m_top, m0, S.m_bach[c]);
+  const double q02 = q2_of(m0, S.m_y[c], S.m_z[c]);
+  for (int s = 0; s < S.n_l1[c]; ++s) a1[s] = bp_poly(S.l1[c][s], q01 * d2) > 0 ? 1 : 0;
+  for (int s = 0; s < S.n_l2[c]; ++s) a2[s] = bp_poly(S.l2[c][s], q02 * d2) > 0 ? 1 : 0;
+}
+
 // Per chain: sqrt(P_l(q0^2 d^2)) of every LS of both vertices (1 when clamped, P_l <= 0) and
 // d log / d m0 of it; false when a clamp flag differs from the cache's.
 bool barrier_constants(const TpaShapeMap& S, int c, double m0, double* c1, double* c2,
@@ -95,12 +106,32 @@ int shape_nll_grad(tpa_fcn_s* F, const double* y, double* nll, double* grad) {
   const int P = F->P, nc = S.n_chain;
   std::vector<double> m0(nc), g0(nc), dlog(P);
   std::vector<double> c1((size_t)nc * TPA_SHAPE_MAX_LS), c2(c1.size()), g1(c1.size()), g2(c1.size());
+  bool stale = false;
   for (int c = 0; c < nc; ++c) {
     m0[c] = S.mass_var[c] >= 0 ? y[S.mass_var[c]] : S.mass_fixed[c];
     g0[c] = S.width_var[c] >= 0 ? y[S.width_var[c]] : S.width_fixed[c];
     if (!barrier_constants(S, c, m0[c], &c1[c * TPA_SHAPE_MAX_LS], &c2[c * TPA_SHAPE_MAX_LS],
                            &g1[c * TPA_SHAPE_MAX_LS], &g2[c * TPA_SHAPE_MAX_LS]))
-      return tpa_set_error(TPA_ERR_STALE, "barrier clamp flags changed: rebuild the caches");
+      stale = true;
+  }
+  if (stale) {
+    // a resonance mass crossed the point where P_l(q0^2 d^2) changes sign: new flags on the
+    // device, every cache rebuilt with them (the barrier factor per event depends on the flag)
+    std::vector<int32_t> a1((size_t)TPA_MAX_CHAIN * TPA_MAX_LS, 0), a2(a1.size(), 0);
+    for (int c = 0; c < nc; ++c) {
+      barrier_flags(S, c, m0[c], F->sm.act1[c], F->sm.act2[c]);
+      for (int s = 0; s < S.n_l1[c]; ++s) a1[c * TPA_MAX_LS + s] = F->sm.act1[c][s];
+      for (int s = 0; s < S.n_l2[c]; ++s) a2[c * TPA_MAX_LS + s] = F->sm.act2[c][s];
+    }
+    int rc = tpa_model_set_barrier_flags(F->model, nc, a1.data(), a2.data());
+    for (int pass = 0; pass < 2 && rc == TPA_OK; ++pass)
+      for (tpa_events_t ev : pass == 0 ? F->data : F->mc)
+        if (rc == TPA_OK) rc = tpa_events_build(ev, 0);
+    if (rc != TPA_OK) return rc;
+    F->rebuilds += 1;
+    for (int c = 0; c < nc; ++c)
+      barrier_constants(S, c, m0[c], &c1[c * TPA_SHAPE_MAX_LS], &c2[c * TPA_SHAPE_MAX_LS],
+                        &g1[c * TPA_SHAPE_MAX_LS], &g2[c * TPA_SHAPE_MAX_LS]);
   }
   for (int k = 0; k < P; ++k) {
     const int c = S.wave_chain[k], s1 = S.wave_ls1[k], s2 = S.wave_ls2[k];
@@ -203,6 +234,12 @@ int tpa_fcn_set_shape_map(tpa_fcn_t F, tpa_model_t model, const TpaShapeMap* map
   F->sm = *map;
   F->model = model;
   F->shape = true;
+  return TPA_OK;
+}
+
+int tpa_fcn_cache_rebuilds(tpa_fcn_t F, int64_t* n) {
+  if (!F || !n) return tpa_set_error(TPA_ERR_ARG, "null pointer");
+  *n = F->rebuilds;
   return TPA_OK;
 }
 

@@ -18,3 +18,8 @@ int tpa_pool_eval_submit(tpa_pool_t p, int* slot_io, int n_items, const int64_t*
                          const double* coefs, int mode);
 int tpa_pool_eval_ready(tpa_pool_t p, int slot);
 int tpa_pool_eval_finish(tpa_pool_t p, int slot, double* out, double* ms);
+
+// Replace the barrier clamp flags (act1 / act2, [n_chain][TPA_MAX_LS]) of a three-body model's
+// chains on the device; the event sets' partial-wave caches must be rebuilt afterwards.
+extern "C" int tpa_model_set_barrier_flags(tpa_model_t model, int n_chain,
+                                           const int32_t* act1, const int32_t* act2);

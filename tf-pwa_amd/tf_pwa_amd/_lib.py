@@ -140,7 +140,6 @@ class TpaCouplingMap(ctypes.Structure):
 SHAPE_MAX_CHAIN = 32
 SHAPE_MAX_LS = 8
 SHAPE_MAX_P = 128
-ERR_STALE = -4
 
 
 class TpaShapeMap(ctypes.Structure):
@@ -163,10 +162,6 @@ class TpaShapeMap(ctypes.Structure):
 
 class EngineError(RuntimeError):
     pass
-
-
-class StaleCache(EngineError):
-    """The barrier clamp flags changed: the caches need a rebuild."""
 
 
 _lib = None
@@ -231,6 +226,7 @@ def lib():
         "tpa_fcn_destroy": ([vp], ctypes.c_int),
         "tpa_fcn_nll_grad": ([vp, dp, dp, dp], ctypes.c_int),
         "tpa_fcn_set_shape_map": ([vp, vp, ctypes.POINTER(TpaShapeMap)], ctypes.c_int),
+        "tpa_fcn_cache_rebuilds": ([vp, ctypes.POINTER(ctypes.c_int64)], ctypes.c_int),
         "tpa_fcn_minimize": ([vp, dp, ctypes.c_double, ctypes.c_int, dp, dp,
                               ctypes.POINTER(ctypes.c_int), ctypes.POINTER(i64),
                               ctypes.POINTER(i64)], ctypes.c_int),
@@ -261,8 +257,7 @@ def lib():
 def check(rc):
     if rc != 0:
         msg = lib().tpa_last_error().decode(errors="replace")
-        cls = StaleCache if rc == ERR_STALE else EngineError
-        raise cls("engine call failed ({}): {}".format(rc, msg))
+        raise EngineError("engine call failed ({}): {}".format(rc, msg))
 
 
 def dptr(a):
@@ -288,4 +283,5 @@ def exported_symbols():
         "tpa_server_nll_grad", "tpa_server_nll_grad_hess", "tpa_server_submit", "tpa_server_poll", "tpa_server_result",
         "tpa_server_latencies", "tpa_server_counters", "tpa_fcn_create", "tpa_fcn_destroy",
         "tpa_fcn_nll_grad", "tpa_fcn_set_shape_map", "tpa_model_set_lineshapes", "tpa_fcn_minimize",
+        "tpa_fcn_cache_rebuilds",
     ]

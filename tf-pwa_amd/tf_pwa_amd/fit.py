@@ -113,9 +113,9 @@ def fit_native(fcn, maxiter=None, gtol=1e-5):
     """Minimise ``fcn`` from the current parameter values with the engine's BFGS
     (tpa_fcn_minimize): the scheme of fit_scipy's scipy BFGS (same bound transforms, strong-
     Wolfe line search, |grad|_inf <= gtol in the unbounded variables) run natively, one engine
-    evaluation per step. Falls back to fit_scipy when the FCN has no native session (Gaussian
-    constraints, four-body floating lineshapes, ``fcn.native = False``) or its caches go stale
-    mid-fit (barrier clamp flags changed). Returns a FitResult."""
+    evaluation per step (a mass that crosses a barrier clamp point rebuilds the caches inside
+    the session). Falls back to fit_scipy when the FCN has no native session (Gaussian
+    constraints, four-body floating lineshapes, ``fcn.native = False``). Returns a FitResult."""
     import ctypes
 
     from . import _lib
@@ -137,14 +137,9 @@ def fit_native(fcn, maxiter=None, gtol=1e-5):
     rc = _lib.lib().tpa_fcn_minimize(nat._h, _lib.dptr(y0), float(gtol),
                                      int(maxiter or 0), _lib.dptr(y), ctypes.byref(nll),
                                      ctypes.byref(status), ctypes.byref(nfev), ctypes.byref(niter))
-    if getattr(nat, "shape", False):
-        fcn.model.Amp._desc_state = None
+    nat.after_call()
     fcn.n_call += int(nfev.value)
     fcn.model.n_eval += int(nfev.value)
-    if rc == _lib.ERR_STALE:
-        fcn._native.close()
-        fcn._native, fcn._native_key = None, None
-        return fit_scipy(fcn, maxiter=maxiter)
     _lib.check(rc)
     vm.set_all(y)
     vm.std_polar_all()

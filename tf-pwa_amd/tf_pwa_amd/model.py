@@ -339,6 +339,24 @@ class _NativeSession:
         self.n = len(self._keep["bound_kind"])
         self._g = np.zeros(self.n)
         self._nll = ctypes.c_double()
+        self._rebuilds = ctypes.c_int64(0)
+        self._seen_rebuilds = 0
+
+    def after_call(self):
+        """Bring the Python host's view in line with what the session did on the device: it
+        set the lineshape parameters (and, when a mass crossed a barrier clamp point, the
+        clamp flags and every cache), so the description is uploaded again and the sets
+        rebuild before the next Python-side pass."""
+        import ctypes
+
+        if not self.shape:
+            return
+        self.amp._desc_state = None
+        _lib.check(_lib.lib().tpa_fcn_cache_rebuilds(self._h, ctypes.byref(self._rebuilds)))
+        if self._rebuilds.value != self._seen_rebuilds:
+            self._seen_rebuilds = self._rebuilds.value
+            for s in self._sets:
+                s._state = None
 
     def nll_grad(self, y, grad=True):
         import ctypes
@@ -347,10 +365,7 @@ class _NativeSession:
         g = np.zeros(self.n) if grad else None
         rc = _lib.lib().tpa_fcn_nll_grad(self._h, _lib.dptr(y), ctypes.byref(self._nll),
                                          _lib.dptr(g))
-        if self.shape:
-            # the session set the device lineshape parameters: the Python host must upload its
-            # own description again before it evaluates
-            self.amp._desc_state = None
+        self.after_call()
         _lib.check(rc)
         return self._nll.value, g
 
@@ -415,8 +430,7 @@ class FCN:
 
     def _native_session(self):
         """The native session (None for floating lineshapes of four-body chains), rebuilt when
-        the precision, fixed values, trainable set or any set's weights change, or after the
-        session reported changed barrier clamp flags."""
+        the precision, fixed values, trainable set or any set's weights change."""
         amp = self.model.Amp
         if not self.native or (amp.shape_mode and amp.n_final != 3):
             return None
@@ -440,17 +454,11 @@ class FCN:
         return np.array(self.vm.get_all_val())
 
     def _native_call(self, x, grad):
-        """(nll, g) from the native session, or None when there is none or its caches went
-        stale (the Python host then rebuilds them; the next call makes a new session)."""
+        """(nll, g) from the native session, or None when there is none."""
         nat = self._native_session()
         if nat is None:
             return None
-        try:
-            out = nat.nll_grad(self._y(x), grad=grad)
-        except _lib.StaleCache:
-            self._native.close()
-            self._native, self._native_key = None, None
-            return None
+        out = nat.nll_grad(self._y(x), grad=grad)
         self.model.n_eval += 1
         return out
 
