@@ -37,7 +37,10 @@ def measured_traffic():
     try:
         with open(TRAFFIC) as f:
             t = json.load(f)
-        return {k: t[k]["read_bytes"] + t[k]["write_bytes"] for k in ("data_1M", "mc_10M")}
+        out = {k: t[k]["read_bytes"] + t[k]["write_bytes"] for k in ("data_1M", "mc_10M")}
+        if "data_1M_f32" in t:
+            out["data_1M_f32"] = t["data_1M_f32"]["read_bytes"] + t["data_1M_f32"]["write_bytes"]
+        return out
     except (OSError, KeyError, ValueError):
         return None
 
@@ -392,9 +395,10 @@ def four_body_leg(args, seed=5505):
         got, piece * args.four_body_pool, time.perf_counter() - t))
     x0 = np.array(conf.vm.get_all_val())
 
-    def run(precision, gram="fp64"):
+    def run(precision, gram="fp64", storage="fp64"):
         model = conf.get_model()
         model.gram_precision = gram
+        model.data_storage = storage
         fcn = FCN(model, data, mc, precision=precision)
         for _ in range(args.warmup):
             fcn.nll_grad(x0)
@@ -410,6 +414,12 @@ def four_body_leg(args, seed=5505):
     g64ms = mc.gram_ms
     dn = abs(nmix - n64)
     ge = float(np.linalg.norm(gmix - g64) / np.linalg.norm(g64))
+    es32, ns32, gs32 = run("mixed", storage="fp32")
+    dns32 = abs(ns32 - n64)
+    ges32 = float(np.linalg.norm(gs32 - g64) / np.linalg.norm(g64))
+    log("four-body: mixed with FP32-stored data waves {:.1f} evals/s, |dNLL| {:.2e}".format(
+        es32, dns32))
+    data.keep_f32(False)
     m32 = conf.get_model()
     m32.gram_precision = "fp32"
     n32, gr32 = FCN(m32, data, mc, precision="mixed").nll_grad(x0)
@@ -426,6 +436,8 @@ def four_body_leg(args, seed=5505):
             "speedup": emix / e64, "abs_dnll_vs_fp64": dn, "grad_rel_err_vs_fp64": ge,
             "gram": "fp64", "gram_ms": {"k_gram_f64": g64ms, "k_gram_f32": mc.gram_ms},
             "fp32_gram": {"abs_dnll_vs_fp64": dn32, "grad_rel_err_vs_fp64": ge32},
+            "mixed_f32_data": {"evals_per_s": es32, "speedup": es32 / e64,
+                               "abs_dnll_vs_fp64": dns32, "grad_rel_err_vs_fp64": ges32},
             "nll_fp64": n64, "data_candidates": piece * args.four_body_pool,
             "clipped_share": clipped, "mc_generated_on_device": True}
 
@@ -490,28 +502,34 @@ def main():
                     kernel_ms=ms.value, launches=int(nl.value), nll=nll, g=g)
 
     bytes_per_event = amp.P * amp.lam_pad * 16 + 8
+    bytes_per_event_f32 = amp.P * amp.lam_pad * 8 + 8
 
     traffic = measured_traffic() if (args.n_data, args.n_mc) == (1_000_000, 10_000_000) else None
 
-    def roofline(r, n_events_per_step):
+    def roofline(r, n_events_per_step, f32=False):
+        bpe = bytes_per_event_f32 if f32 else bytes_per_event
         kernel_s = r["kernel_ms"] / 1e3
-        bytes_total = n_events_per_step * bytes_per_event * args.steps
+        bytes_total = n_events_per_step * bpe * args.steps
         achieved = bytes_total / kernel_s / 1e9 if kernel_s > 0 else 0.0
         tr = None
         if traffic is not None:
-            tr = traffic["data_1M"] if n_events_per_step == args.n_data else 0.5 * (
-                traffic["data_1M"] + traffic["mc_10M"])
+            if f32:
+                tr = traffic.get("data_1M_f32")
+            else:
+                tr = traffic["data_1M"] if n_events_per_step == args.n_data else 0.5 * (
+                    traffic["data_1M"] + traffic["mc_10M"])
         # the angle-cache input of the same events (22 f64 per event: 3 topologies x 5 + 2 x 3
         # aligned angles + weight), the bytes a recompute-from-angles kernel would read
         survey = ANGLE_BYTES_PER_EVENT * n_events_per_step * args.steps / kernel_s / 1e9 \
             if kernel_s > 0 else 0.0
         return {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": achieved / HBM_PEAK_GBS, "traffic": tr, "kernel": "k_eval_pipe",
+                "frac": achieved / HBM_PEAK_GBS, "traffic": tr,
+                "kernel": "k_eval_pipe32" if f32 else "k_eval_pipe",
                 "angle_cache_bytes_per_event": ANGLE_BYTES_PER_EVENT,
                 "achieved_on_angle_cache_bytes": survey,
                 "launches": r["launches"], "kernel_ms_avg": r["kernel_ms"] / max(1, r["launches"]),
-                "bytes_per_event": bytes_per_event,
-                "bytes_per_launch": n_events_per_step * bytes_per_event / max(
+                "bytes_per_event": bpe,
+                "bytes_per_launch": n_events_per_step * bpe / max(
                     1, r["launches"] // args.steps)}
 
     log("fp64 path ...")
@@ -537,8 +555,26 @@ def main():
     dnll = abs(rmix["nll"] - r64["nll"])
     grel = float(np.linalg.norm(rmix["g"] - r64["g"]) / np.linalg.norm(r64["g"]))
     log("mixed: {:.1f} evals/s, |dNLL| {:.2e}, grad rel {:.2e}".format(rmix["evals"], dnll, grel))
-    ok = dnll < 1e-3
-    head = rmix if ok else r64
+    # the same mixed path reading FP32 copies of the data partial waves (FP64 arithmetic)
+    ms32 = conf.get_model()
+    ms32.data_storage = "fp32"
+    fmix32 = FCN(ms32, data, mc, precision="mixed")
+    rmix32 = time_path(fmix32)
+    dnll_s32 = abs(rmix32["nll"] - r64["nll"])
+    grel_s32 = float(np.linalg.norm(rmix32["g"] - r64["g"]) / np.linalg.norm(r64["g"]))
+    log("mixed, FP32-stored data waves: {:.1f} evals/s, |dNLL| {:.2e}, grad rel {:.2e}".format(
+        rmix32["evals"], dnll_s32, grel_s32))
+    if dnll_s32 < 1e-3:
+        path, head, head_events, head_f32 = "mixed_f32_data", rmix32, args.n_data, True
+        dtype = ("f64 arithmetic (data partial waves stored f32, widened on load; data log-sum "
+                 "in f64; MC normalisation through the FP64-MFMA Gram)")
+    elif dnll < 1e-3:
+        path, head, head_events, head_f32 = "mixed", rmix, args.n_data, False
+        dtype = "f64 (data log-sum; MC normalisation through the FP64-MFMA Gram)"
+    else:
+        path, head, head_events, head_f32 = "fp64", r64, args.n_data + args.n_mc, False
+        dtype = "f64"
+    ok = path != "fp64"
     # MFMA work of the Gram kernels: FP64, (2P/16)(2P/16+1)/2 16x16 tiles of 2*16*16 flops per
     # row (event x helicity slot); FP32, three 32x32 tiles of 2*32*32 flops per row
     rows = args.n_mc * amp.lam_pad
@@ -557,19 +593,24 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64 (data log-sum; MC normalisation through the FP64-MFMA Gram)" if ok
-        else "f64",
+        "dtype": dtype,
         "data": "synthetic",
         "config": {"workload": "Lc->pKpi 6 resonances, 22 partial waves, 1M data + 10M MC",
                    "n_data": args.n_data, "n_mc": args.n_mc, "n_params": len(x0),
-                   "path": "mixed" if ok else "fp64"},
-        "roofline": roofline(head, args.n_data if ok else args.n_data + args.n_mc),
+                   "path": path},
+        "roofline": roofline(head, head_events, head_f32),
         "paths": {
             "fp64": {"evals_per_s": r64["evals"], "ms_per_step": r64["ms_per_step"],
                      "nll": r64["nll"], "roofline": roofline(r64, args.n_data + args.n_mc)},
             "mixed": {"evals_per_s": rmix["evals"], "ms_per_step": rmix["ms_per_step"],
                       "nll": rmix["nll"], "abs_dnll_vs_fp64": dnll, "grad_rel_err_vs_fp64": grel,
-                      "speedup_vs_fp64": rmix["evals"] / r64["evals"], "gram": "fp64"},
+                      "speedup_vs_fp64": rmix["evals"] / r64["evals"], "gram": "fp64",
+                      "roofline": roofline(rmix, args.n_data)},
+            "mixed_f32_data": {"evals_per_s": rmix32["evals"], "ms_per_step": rmix32["ms_per_step"],
+                               "nll": rmix32["nll"], "abs_dnll_vs_fp64": dnll_s32,
+                               "grad_rel_err_vs_fp64": grel_s32,
+                               "speedup_vs_fp64": rmix32["evals"] / r64["evals"],
+                               "gram": "fp64", "data_storage": "fp32"},
             "mixed_fp32_gram": {"abs_dnll_vs_fp64": dnll32, "grad_rel_err_vs_fp64": grel32,
                                 "evals_per_s": "same per-call work as mixed"},
             "gram_build": {
@@ -605,14 +646,14 @@ def main():
         cpu_baseline_saved = cpu_baseline(conf, data, mc, mc.gram("fp64"), data_p4, args.n_data,
                                           args.n_mc)
     if args.serve_fits > 0:
-        del fmix, data
+        del fmix, fmix32, data
         res["serving"] = serving_leg(conf, mc, data_p4, x0, args)
     if args.multistart > 0:
         res["floating_lineshape_fits"] = multistart_leg(args, starts=args.multistart)
     if args.four_body_mc > 0:
         import gc
 
-        fmix = data = mc = None  # release the three-body sets' device memory first
+        fmix = fmix32 = data = mc = None  # release the three-body sets' device memory first
         gc.collect()
         res["four_body"] = four_body_leg(args)
     if args.calib_copy:

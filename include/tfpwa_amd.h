@@ -44,6 +44,10 @@ typedef struct tpa_fcn_s* tpa_fcn_t;
  * Dm_k = sum_e u_e dB/dm0 H_k at out[2+2P+2k..] and Dg_k (dB/dg0) at out[2+4P+2k..],
  * H_k = sum_lambda conj(A_lambda) h_k,lambda, so out needs 2+6P doubles. */
 #define TPA_FLAG_SHAPE 4
+/* Read the FP32 copy of a folded cache (tpa_events_keep_f32) instead of the FP64 cache: half
+ * the bytes per event; the values are widened to FP64 on load and every operation after it is
+ * FP64. Not with TPA_FLAG_SHAPE. */
+#define TPA_FLAG_F32 8
 
 /* Version of this ABI. */
 int tpa_abi_version(void);
@@ -108,6 +112,13 @@ int tpa_events_get_angles(tpa_events_t ev, double* out, int64_t out_len);
  * parameter-free factors of HelicityDecay.get_amp / DecayChain.get_amp
  * (tf_pwa/amp/core.py:931-1164, 1392-1445). */
 int tpa_events_build(tpa_events_t ev, int fold);
+
+/* on != 0: keep an FP32 copy (round to nearest) of the folded partial-wave cache beside the
+ * FP64 one, made now if the cache is built and refreshed by every later folded build, for
+ * tpa_eval with TPA_FLAG_F32; on = 0 frees it. No reference counterpart: the storage option
+ * of the data pass of the mixed-precision likelihood (the reference evaluates
+ * sum_gradient, tf_pwa/model/model.py:41-94, in FP64 from the angle cache). */
+int tpa_events_keep_f32(tpa_events_t ev, int on);
 
 /* Replace the model description (same chains and CG table; new lineshape parameters or
  * barrier clamp flags). Event sets of the model must be rebuilt (tpa_events_build) before
@@ -341,6 +352,11 @@ typedef struct TpaShapeMap {
 /* Attach a shape map to a session built without a Gram matrix; model: the event sets' model
  * (its device lineshape parameters are set per evaluation). */
 int tpa_fcn_set_shape_map(tpa_fcn_t fcn, tpa_model_t model, const TpaShapeMap* map);
+
+/* data_f32 != 0: the session's data passes read the FP32 copies of the data caches
+ * (TPA_FLAG_F32; the copies are kept from here on, tpa_events_keep_f32). Fixed lineshapes
+ * only. The MC side is unchanged (FP64 passes or the Gram matrix). */
+int tpa_fcn_set_data_storage(tpa_fcn_t fcn, int data_f32);
 
 /* Cache rebuilds the session made after barrier clamp flags changed (no reference
  * counterpart: a bookkeeping counter for hosts that track the caches' state). */

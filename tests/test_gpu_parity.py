@@ -231,6 +231,64 @@ def test_mixed_path_close_to_fp64(case2, gpu_available):
     n3, g3 = FCN(model32, data, mc, precision="mixed").nll_grad()
     assert abs(n1 - n3) < 1e-3
     assert np.linalg.norm(g1 - g3) < 1e-4 * np.linalg.norm(g1)
+    # FP32-stored data waves (FP64 sums): the bar of the mixed path, |dNLL| < 1e-3, with the
+    # rounding's expected size (~1e-7 per event, random signs) well inside it
+    m32 = conf.get_model()
+    m32.data_storage = "fp32"
+    fs = FCN(m32, data, mc, precision="mixed")
+    n4, g4 = fs.nll_grad()
+    assert fs._native is not None
+    assert abs(n1 - n4) < 1e-5, abs(n1 - n4)
+    assert np.linalg.norm(g1 - g4) < 1e-6 * np.linalg.norm(g1)
+    fs.native = False  # the Python host assembly reads the same FP32 copies
+    n5, g5 = fs.nll_grad()
+    assert abs(n5 - n4) <= 1e-12 * abs(n4)
+    assert np.abs(g5 - g4).max() <= 1e-10 * np.abs(g4).max()
+
+
+@pytest.mark.parametrize("n", [1, 257, 20000])
+def test_fp32_cache_pass_matches_oracle_on_rounded_waves(case2, n, gpu_available):
+    """tpa_eval with TPA_FLAG_F32 computes in FP64 from the FP32-rounded partial waves: the
+    oracle's C restatement on the same rounded waves gives the same sums to FP64 rounding.
+    The copy follows a rebuild of the cache (a fixed mass moved)."""
+    import ctypes
+
+    from oracle import build_cpu
+    from tf_pwa_amd import _lib
+
+    L = build_cpu.load()
+    conf = case2["conf"]
+    full = conf.data_from_p4(case2["mc"])
+    ev = full.select(np.arange(n) % full.n)
+    w = np.random.default_rng(n + 1).uniform(-0.5, 1.5, n)
+    ev.set_weight(w)
+    ev.keep_f32(True)
+    amp = conf.get_amplitude()
+    dp = ctypes.POINTER(ctypes.c_double)
+    name = [v for v in conf.vm.variables if v.endswith("_mass")][0]
+    m0 = conf.vm.get(name)
+    try:
+        for shift in (0.0, 0.01):
+            conf.vm.set(name, m0 + shift)
+            c = amp.coefficients()
+            h = ev.partial_waves().astype(np.complex64).astype(np.complex128)
+            h = np.ascontiguousarray(h)
+            P, N, LG = h.shape
+            cc = np.ascontiguousarray(c).view(np.float64)
+            for mode in (_lib.MODE_DATA, _lib.MODE_MC):
+                v, G, f = ev.eval(c, mode, grad=True, fout=True, f32=True)
+                out = np.zeros(2 + 2 * P)
+                assert L.cpu_eval(h.view(np.float64).ctypes.data_as(dp), w.ctypes.data_as(dp), N,
+                                  P, LG, cc.ctypes.data_as(dp), 0 if mode == _lib.MODE_DATA else 1,
+                                  out.ctypes.data_as(dp)) == 0
+                scale = max(1.0, np.abs(w).sum())
+                assert abs(out[0] - v) <= 1e-12 * scale * max(1.0, abs(v) / scale)
+                np.testing.assert_allclose(out[2:].view(np.complex128), G, rtol=0,
+                                           atol=1e-11 * max(1e-300, np.abs(G).max()))
+                A = np.einsum("k,kel->el", c, h)
+                np.testing.assert_allclose(f, (np.abs(A) ** 2).sum(1), rtol=1e-12)
+    finally:
+        conf.vm.set(name, m0)
 
 
 @pytest.mark.parametrize("cfg,res", [

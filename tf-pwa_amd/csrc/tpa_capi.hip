@@ -222,6 +222,9 @@ struct tpa_events_s {
   int built_fold = -1;
   hipEvent_t t0 = nullptr, t1 = nullptr;
   double* h_io = nullptr;  // pinned staging of the coefficients and the sums of tpa_eval
+  float2* d_h32 = nullptr;  // FP32 copy of a folded cache (tpa_events_keep_f32)
+  bool keep32 = false;      // refresh d_h32 after every folded build
+  bool h32_valid = false;
 };
 
 // Pinned staging of tpa_eval: 2 * TPA_MAX_P doubles of coefficients, then the sums (at most
@@ -246,21 +249,50 @@ static int eval_unroll() {
   return u;
 }
 
+template <int LG, int KG, typename HT>
+static void launch_pipe(const EvalArgs& a, int grid, hipStream_t s) {
+  const int need = (a.P + KG - 1) / KG;
+  if (need <= 4)
+    hipLaunchKernelGGL((k_eval_pipe<LG, KG, 4, HT>), dim3(grid), dim3(256), 0, s, a);
+  else if (need <= 6)
+    hipLaunchKernelGGL((k_eval_pipe<LG, KG, 6, HT>), dim3(grid), dim3(256), 0, s, a);
+  else if (need <= 7)
+    hipLaunchKernelGGL((k_eval_pipe<LG, KG, 7, HT>), dim3(grid), dim3(256), 0, s, a);
+  else
+    hipLaunchKernelGGL((k_eval_pipe<LG, KG, 8, HT>), dim3(grid), dim3(256), 0, s, a);
+}
+
+template <int LG, int KG>
+static void launch_pipe32(const EvalArgs& a, int grid, hipStream_t s) {
+  if constexpr (LG >= 2) {
+    const int need = (a.P + KG - 1) / KG;
+    if (need <= 4)
+      hipLaunchKernelGGL((k_eval_pipe32<LG, KG, 4>), dim3(grid), dim3(256), 0, s, a);
+    else if (need <= 6)
+      hipLaunchKernelGGL((k_eval_pipe32<LG, KG, 6>), dim3(grid), dim3(256), 0, s, a);
+    else if (need <= 7)
+      hipLaunchKernelGGL((k_eval_pipe32<LG, KG, 7>), dim3(grid), dim3(256), 0, s, a);
+    else
+      hipLaunchKernelGGL((k_eval_pipe32<LG, KG, 8>), dim3(grid), dim3(256), 0, s, a);
+  } else {
+    launch_pipe<LG, KG, float2>(a, grid, s);
+  }
+}
+
+// Lanes per event of the FP32-cache pass: two helicity components per lane when LG >= 2.
+static int f32_lanes_per_event(int lg, int kg) { return (lg >= 2 ? lg / 2 : 1) * kg; }
+
+// a.h32 set: the pass reads the FP32 copy (k_eval_pipe32; k_eval_pipe over float2 for LG 1).
 template <int LG, int KG>
 static void launch_eval(const EvalArgs& a, int grid, hipStream_t s) {
+  if (a.h32 != nullptr) {
+    launch_pipe32<LG, KG>(a, grid, s);
+    return;
+  }
   switch (eval_unroll()) {
-    case 0: {
-      const int need = (a.P + KG - 1) / KG;
-      if (need <= 4)
-        hipLaunchKernelGGL((k_eval_pipe<LG, KG, 4>), dim3(grid), dim3(256), 0, s, a);
-      else if (need <= 6)
-        hipLaunchKernelGGL((k_eval_pipe<LG, KG, 6>), dim3(grid), dim3(256), 0, s, a);
-      else if (need <= 7)
-        hipLaunchKernelGGL((k_eval_pipe<LG, KG, 7>), dim3(grid), dim3(256), 0, s, a);
-      else
-        hipLaunchKernelGGL((k_eval_pipe<LG, KG, 8>), dim3(grid), dim3(256), 0, s, a);
+    case 0:
+      launch_pipe<LG, KG, double2>(a, grid, s);
       break;
-    }
     case 1:
       hipLaunchKernelGGL((k_eval_fold<LG, KG, 8, 1, true>), dim3(grid), dim3(256), 0, s, a);
       break;
@@ -771,6 +803,7 @@ int tpa_events_destroy(tpa_events_t ev) {
   if (!ev) return TPA_OK;
   if (ev->d_ang) (void)hipFree(ev->d_ang);
   if (ev->d_h) (void)hipFree(ev->d_h);
+  if (ev->d_h32) (void)hipFree(ev->d_h32);
   if (ev->d_w) (void)hipFree(ev->d_w);
   if (ev->d_part) (void)hipFree(ev->d_part);
   if (ev->d_out) (void)hipFree(ev->d_out);
@@ -983,6 +1016,19 @@ int tpa_events_get_angles(tpa_events_t ev, double* out, int64_t out_len) {
   return TPA_OK;
 }
 
+// Refresh the FP32 copy of ev's folded cache (queued on the model's stream).
+static int convert_f32(tpa_events_s* ev) {
+  const TpaModel& M = ev->m->h;
+  const int64_t n = (int64_t)M.P * ev->n * M.lam_pad;
+  if (!ev->d_h32) HIPCHK(hipMalloc(&ev->d_h32, sizeof(float2) * (size_t)n));
+  const int64_t grid = std::min<int64_t>((n + 255) / 256, 8192);
+  hipLaunchKernelGGL(k_pw_to_f32, dim3((unsigned)grid), dim3(256), 0, ev->m->stream, ev->d_h,
+                     ev->d_h32, n);
+  HIPCHK(hipGetLastError());
+  ev->h32_valid = true;
+  return TPA_OK;
+}
+
 int tpa_events_build(tpa_events_t ev, int fold) {
   if (!ev) return fail(TPA_ERR_ARG, "null pointer");
   const TpaModel& M = ev->m->h;
@@ -991,8 +1037,31 @@ int tpa_events_build(tpa_events_t ev, int fold) {
     return fail(TPA_ERR_UNSUPPORTED, "four-body models fold fixed lineshapes into the cache");
   launch_build_pw(ev->m, ev->d_ang, ev->n, 0, ev->n, fold, ev->d_h, ev->m->stream);
   HIPCHK(hipGetLastError());
-  HIPCHK(hipStreamSynchronize(ev->m->stream));
   ev->built_fold = fold ? 1 : 0;
+  ev->h32_valid = false;
+  if (ev->keep32 && fold) {
+    const int rc = convert_f32(ev);
+    if (rc != TPA_OK) return rc;
+  }
+  HIPCHK(hipStreamSynchronize(ev->m->stream));
+  return TPA_OK;
+}
+
+int tpa_events_keep_f32(tpa_events_t ev, int on) {
+  if (!ev) return fail(TPA_ERR_ARG, "null pointer");
+  if (!on) {
+    HIPCHK(hipStreamSynchronize(ev->m->stream));
+    if (ev->d_h32) HIPCHK(hipFree(ev->d_h32));
+    ev->d_h32 = nullptr;
+    ev->keep32 = ev->h32_valid = false;
+    return TPA_OK;
+  }
+  ev->keep32 = true;
+  if (ev->d_h && ev->built_fold == 1 && !ev->h32_valid) {
+    const int rc = convert_f32(ev);
+    if (rc != TPA_OK) return rc;
+    HIPCHK(hipStreamSynchronize(ev->m->stream));
+  }
   return TPA_OK;
 }
 
@@ -1023,8 +1092,14 @@ int tpa_eval(tpa_events_t ev, const double* coef, int mode, int flags, double* o
   double* h_out = ev->h_io + 2 * TPA_MAX_P;
   std::memcpy(h_coef, coef, sizeof(double2) * M.P);
   HIPCHK(hipMemcpyAsync(ev->d_coef, h_coef, sizeof(double2) * M.P, hipMemcpyHostToDevice, s));
+  const bool shape = (flags & TPA_FLAG_SHAPE) != 0;
+  const bool f32 = (flags & TPA_FLAG_F32) != 0;
+  if (f32 && shape) return fail(TPA_ERR_UNSUPPORTED, "FP32 cache reads need a folded cache");
+  if (f32 && !ev->h32_valid)
+    return fail(TPA_ERR_ARG, "no FP32 copy of the cache (tpa_events_keep_f32)");
   EvalArgs a;
   a.h = ev->d_h;
+  a.h32 = f32 ? ev->d_h32 : nullptr;
   a.w = ev->d_w;
   a.coef = ev->d_coef;
   a.partials = ev->d_part;
@@ -1033,7 +1108,6 @@ int tpa_eval(tpa_events_t ev, const double* coef, int mode, int flags, double* o
   a.P = M.P;
   a.mode = mode;
   a.grad = (flags & TPA_FLAG_GRAD) ? 1 : 0;
-  const bool shape = (flags & TPA_FLAG_SHAPE) != 0;
   if (shape && ev->m->kind == 4)
     return fail(TPA_ERR_UNSUPPORTED, "floating lineshapes need a three-body model");
   if (shape && ev->built_fold != 0)
@@ -1043,7 +1117,7 @@ int tpa_eval(tpa_events_t ev, const double* coef, int mode, int flags, double* o
   a.nacc = shape ? 2 + 6 * M.P : 2 + 2 * M.P;
   const int kg = eval_kgroups(M.lam_pad, M.P);
   if (kg < 0) return fail(TPA_ERR_UNSUPPORTED, "too many partial waves for this helicity count");
-  int grid = eval_grid(ev->n, M.lam_pad * kg);
+  int grid = eval_grid(ev->n, f32 ? f32_lanes_per_event(M.lam_pad, kg) : M.lam_pad * kg);
   bool timing;
   {
     std::lock_guard<std::mutex> lk(g_timing_mu);

@@ -26,6 +26,7 @@ struct tpa_fcn_s : CouplingMap {
   tpa_model_t model = nullptr;
   TpaShapeMap sm{};
   int64_t rebuilds = 0;  // cache rebuilds after barrier clamp flags changed
+  int data_flags = 0;    // extra tpa_eval flags of the data passes (TPA_FLAG_F32)
 };
 
 namespace {
@@ -216,6 +217,8 @@ int tpa_fcn_set_shape_map(tpa_fcn_t F, tpa_model_t model, const TpaShapeMap* map
   if (!F || !model || !map) return tpa_set_error(TPA_ERR_ARG, "null pointer");
   if (!F->gram.empty() || F->mc.empty())
     return tpa_set_error(TPA_ERR_ARG, "floating lineshapes need the MC event sets, not a Gram matrix");
+  if (F->data_flags)
+    return tpa_set_error(TPA_ERR_UNSUPPORTED, "floating lineshapes read the FP64 data caches");
   if (map->P != F->P || map->n_chain < 1 || map->n_chain > TPA_SHAPE_MAX_CHAIN)
     return tpa_set_error(TPA_ERR_ARG, "shape map sizes do not match the session");
   for (int c = 0; c < map->n_chain; ++c) {
@@ -234,6 +237,18 @@ int tpa_fcn_set_shape_map(tpa_fcn_t F, tpa_model_t model, const TpaShapeMap* map
   F->sm = *map;
   F->model = model;
   F->shape = true;
+  return TPA_OK;
+}
+
+int tpa_fcn_set_data_storage(tpa_fcn_t F, int data_f32) {
+  if (!F) return tpa_set_error(TPA_ERR_ARG, "null pointer");
+  if (data_f32 && F->shape)
+    return tpa_set_error(TPA_ERR_UNSUPPORTED, "FP32 data caches need fixed lineshapes");
+  for (tpa_events_t ev : F->data) {
+    const int rc = tpa_events_keep_f32(ev, data_f32 ? 1 : 0);
+    if (rc != TPA_OK) return rc;
+  }
+  F->data_flags = data_f32 ? TPA_FLAG_F32 : 0;
   return TPA_OK;
 }
 
@@ -256,7 +271,8 @@ int tpa_fcn_nll_grad(tpa_fcn_t F, const double* y, double* nll, double* grad) {
   const int flags = grad ? TPA_FLAG_GRAD : 0;
   double sd = 0.0;
   for (tpa_events_t ev : F->data) {
-    const int rc = tpa_eval(ev, F->coef.data(), TPA_MODE_DATA, flags, F->out.data(), nullptr);
+    const int rc = tpa_eval(ev, F->coef.data(), TPA_MODE_DATA, flags | F->data_flags,
+                            F->out.data(), nullptr);
     if (rc != TPA_OK) return rc;
     sd += F->out[0];
     if (grad)

@@ -287,6 +287,7 @@ __global__ __launch_bounds__(128) void k_build_pw(const TpaModel* __restrict__ M
 // ------------------------------------------------------------------------- evaluation
 struct EvalArgs {
   const double2* __restrict__ h;     // [P][N][LG]
+  const float2* __restrict__ h32;    // [P][N][LG], FP32 copy of h (passes that read FP32)
   const double* __restrict__ w;      // [N]
   const double2* __restrict__ coef;  // [P]
   double* __restrict__ partials;     // [grid][nacc]
@@ -333,6 +334,28 @@ TPA_DEV double2 load_pw(const double2* p) {
   } else {
     return *p;
   }
+}
+
+TPA_DEV float2 load_pw32(const float2* p) {
+  typedef float f2v __attribute__((ext_vector_type(2)));
+  const f2v v = __builtin_nontemporal_load(reinterpret_cast<const f2v*>(p));
+  return make_float2(v.x, v.y);
+}
+
+// The cache an evaluation pass reads: FP64 (h) or its FP32 copy (h32).
+template <typename HT>
+TPA_DEV const HT* pw_cache(const EvalArgs& a) {
+  if constexpr (sizeof(HT) == sizeof(float2))
+    return a.h32;
+  else
+    return a.h;
+}
+template <typename HT>
+TPA_DEV HT load_stream(const HT* p) {
+  if constexpr (sizeof(HT) == sizeof(float2))
+    return load_pw32(p);
+  else
+    return load_pw<true>(p);
 }
 
 template <int LG, int KG, int PML, int U, bool NT = false>
@@ -509,14 +532,19 @@ TPA_DEV double butterfly(double v) {
 // One event group of the evaluation pass: A_lambda from the lane's PML partial waves, f, the
 // mode's value and derivative, the value into the compensated pair (vs, vc) of the event's
 // leader lane and the adjoint into (gr, gi). Lanes past the last event (valid = false) carry
-// finite stand-in waves and add exactly zero. Every lane of the wave must be active.
-template <int LG, int KG, int PML>
+// finite stand-in waves and add exactly zero. Every lane of the wave must be active. HT is
+// the cache's element type (double2, or float2 for an FP32 copy: its values are widened to
+// FP64 exactly and every operation after the load is FP64).
+template <int LG, int KG, int PML, typename HT>
 TPA_DEV void eval_group(int mode, bool grad, double* __restrict__ f_out,
-                        const double2 (&hv)[PML], double w, int64_t e, bool valid, bool leader,
+                        const HT (&hs)[PML], double w, int64_t e, bool valid, bool leader,
                         const double (&cr)[PML], const double (&ci)[PML], double& vs, double& vc,
                         double (&gr)[PML], double (&gi)[PML]) {
   constexpr int LPE = LG * KG;
   const double eps = 1.0e-6;
+  double2 hv[PML];
+#pragma unroll
+  for (int j = 0; j < PML; ++j) hv[j] = make_double2(hs[j].x, hs[j].y);
   double ar = 0.0, ai = 0.0;
 #pragma unroll
   for (int j = 0; j < PML; ++j) {
@@ -566,8 +594,12 @@ TPA_DEV void eval_group(int mode, bool grad, double* __restrict__ f_out,
 // the older loads): past the last event or partial wave they read a clamped, valid address,
 // whose values meet a zero coefficient or a zero weight. The loop bounds are wave-uniform
 // (scalar), so every lane stays active for the cross-lane exchanges. Requires N >= 1.
-template <int LG, int KG, int PML>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(PML <= 6 ? 3 : 1))) void k_eval_pipe(EvalArgs a) {
+// HT: the cache read (double2: a.h; float2: the FP32 copy a.h32). WPE: the waves per SIMD
+// the register budget is fitted to (without spills: FP32 stages at PML 6 need 162 VGPRs,
+// at PML 4 128, at PML 7-8 179-196).
+template <int LG, int KG, int PML, typename HT = double2,
+          int WPE = (PML <= 4 && sizeof(HT) == 8) ? 4 : PML <= 6 ? 3 : (sizeof(HT) == 8 ? 2 : 1)>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE))) void k_eval_pipe(EvalArgs a) {
   constexpr int LPE = LG * KG;
   constexpr int EPW = 64 / LPE;
   const int lane = threadIdx.x & 63;
@@ -578,7 +610,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(PML <= 6 ? 
   const bool leader = lam == 0 && kg == 0;
   double vs = 0.0, vc = 0.0;
   double gr[PML], gi[PML], cr[PML], ci[PML];
-  const double2* base[PML];
+  const HT* base[PML];
+  const HT* cache = pw_cache<HT>(a);
 #pragma unroll
   for (int j = 0; j < PML; ++j) {
     const int k = kg + KG * j;
@@ -587,17 +620,17 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(PML <= 6 ? 
     cr[j] = c.x;
     ci[j] = c.y;
     const int kc = k < a.P ? k : a.P - 1;
-    base[j] = a.h + (int64_t)kc * a.N * LG + lam;
+    base[j] = cache + (int64_t)kc * a.N * LG + lam;
   }
   const int64_t last = a.N - 1;
-  auto load = [&](double2 (&hv)[PML], double& wv, int64_t e) {
+  auto load = [&](HT (&hv)[PML], double& wv, int64_t e) {
     const int64_t ec = e < a.N ? e : last;
 #pragma unroll
-    for (int j = 0; j < PML; ++j) hv[j] = load_pw<true>(base[j] + ec * LG);
+    for (int j = 0; j < PML; ++j) hv[j] = load_stream<HT>(base[j] + ec * LG);
     wv = a.w[ec];
   };
   const int64_t stride = (int64_t)gridDim.x * 4 * EPW;
-  double2 h0[PML], h1[PML];
+  HT h0[PML], h1[PML];
   double w0, w1;
   const int wave_u = __builtin_amdgcn_readfirstlane(wave);
   const int64_t first = ((int64_t)blockIdx.x * 4 + wave_u) * EPW;
@@ -630,6 +663,164 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(PML <= 6 ? 
       xi = butterfly<LPE, 64>(xi);
       const int k = kg + KG * j;
       if (el == 0 && lam == 0 && k < a.P) {
+        sh[wave][2 + 2 * k] = xr;
+        sh[wave][3 + 2 * k] = xi;
+      }
+    }
+  }
+  if (lane == 0) {
+    sh[wave][0] = vs;
+    sh[wave][1] = vc;
+  }
+  __syncthreads();
+  double* out = a.partials + (int64_t)blockIdx.x * a.nacc;
+  if (threadIdx.x == 0) {
+    double s = sh[0][0], c = sh[0][1];
+    for (int wv = 1; wv < 4; ++wv) neumaier_merge(s, c, sh[wv][0], sh[wv][1]);
+    out[0] = s;
+    out[1] = c;
+  }
+  if (a.grad) {
+    for (int j = 2 + threadIdx.x; j < a.nacc; j += blockDim.x)
+      out[j] = ((sh[0][j] + sh[1][j]) + sh[2][j]) + sh[3][j];
+  }
+}
+
+// One event group of the FP32-cache pass with two helicity components per lane (one 16-byte
+// load of two float2 per partial wave): as eval_group, with the lane's two A_lambda summed
+// into f before the reduction over the LG/2 lanes of a k-group.
+template <int LGL, int KG, int PML>
+TPA_DEV void eval_group2(int mode, bool grad, double* __restrict__ f_out,
+                         const float4 (&hs)[PML], double w, int64_t e, bool valid, bool leader,
+                         const double (&cr)[PML], const double (&ci)[PML], double& vs,
+                         double& vc, double (&gr)[PML], double (&gi)[PML]) {
+  constexpr int LPE = LGL * KG;
+  const double eps = 1.0e-6;
+  double ar0 = 0.0, ai0 = 0.0, ar1 = 0.0, ai1 = 0.0;
+#pragma unroll
+  for (int j = 0; j < PML; ++j) {
+    const double x0 = hs[j].x, y0 = hs[j].y, x1 = hs[j].z, y1 = hs[j].w;
+    ar0 += cr[j] * x0 - ci[j] * y0;
+    ai0 += cr[j] * y0 + ci[j] * x0;
+    ar1 += cr[j] * x1 - ci[j] * y1;
+    ai1 += cr[j] * y1 + ci[j] * x1;
+  }
+  ar0 = butterfly<LGL, LPE>(ar0);
+  ai0 = butterfly<LGL, LPE>(ai0);
+  ar1 = butterfly<LGL, LPE>(ar1);
+  ai1 = butterfly<LGL, LPE>(ai1);
+  const double f = butterfly<1, LGL>((ar0 * ar0 + ai0 * ai0) + (ar1 * ar1 + ai1 * ai1));
+  const double wv = valid ? w : 0.0;
+  double y, u;
+  if (mode == 0) {
+    double cl, dcl;
+    if (f > eps) {
+      cl = log(f);
+      dcl = 1.0 / f;
+    } else {
+      const double t = (f - eps) / eps;
+      cl = (log(eps) + t) - (t * t) / 2.0;
+      dcl = 1.0 / eps - t / eps;
+    }
+    y = wv * cl;
+    u = wv * dcl;
+  } else {
+    y = wv * f;
+    u = wv;
+  }
+  if (f_out != nullptr && valid && leader) f_out[e] = f;
+  neumaier_add(vs, vc, leader ? y : 0.0);
+  if (grad) {
+    const double tr0 = u * ar0, ti0 = -(u * ai0), tr1 = u * ar1, ti1 = -(u * ai1);
+#pragma unroll
+    for (int j = 0; j < PML; ++j) {
+      // widen again rather than keep the first loop's FP64 values live (an empty asm makes
+      // the FP32 registers opaque to common-subexpression elimination; no instruction)
+      float4 q = hs[j];
+      asm("" : "+v"(q.x), "+v"(q.y), "+v"(q.z), "+v"(q.w));
+      const double x0 = q.x, y0 = q.y, x1 = q.z, y1 = q.w;
+      gr[j] += (tr0 * x0 - ti0 * y0) + (tr1 * x1 - ti1 * y1);
+      gi[j] += (tr0 * y0 + ti0 * x0) + (tr1 * y1 + ti1 * x1);
+    }
+  }
+}
+
+TPA_DEV float4 load_pw32x2(const float4* p) {
+  typedef float f4v __attribute__((ext_vector_type(4)));
+  const f4v v = __builtin_nontemporal_load(reinterpret_cast<const f4v*>(p));
+  return make_float4(v.x, v.y, v.z, v.w);
+}
+
+// The evaluation pass over the FP32 copy of the cache (a.h32), for LG >= 2: the pipelined
+// scheme of k_eval_pipe with each lane holding two helicity components of a partial wave
+// (lane = ev * (LG/2 * KG) + kg * LG/2 + lambda/2), so a lane's loads stay 16 bytes wide and
+// a wave covers twice the events of the FP64 pass. The FP32 values are widened to FP64 on
+// load; every sum is FP64. Requires N >= 1.
+template <int LG, int KG, int PML>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(PML <= 6 ? 3 : 1))) void k_eval_pipe32(EvalArgs a) {
+  static_assert(LG >= 2, "two helicity components per lane");
+  constexpr int LGL = LG / 2;
+  constexpr int LPE = LGL * KG;
+  constexpr int EPW = 64 / LPE;
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  const int lamg = lane & (LGL - 1);
+  const int kg = (lane / LGL) & (KG - 1);
+  const int el = lane / LPE;
+  const bool leader = lamg == 0 && kg == 0;
+  double vs = 0.0, vc = 0.0;
+  double gr[PML], gi[PML], cr[PML], ci[PML];
+  const float4* base[PML];
+#pragma unroll
+  for (int j = 0; j < PML; ++j) {
+    const int k = kg + KG * j;
+    gr[j] = gi[j] = 0.0;
+    const double2 c = k < a.P ? a.coef[k] : make_double2(0.0, 0.0);
+    cr[j] = c.x;
+    ci[j] = c.y;
+    const int kc = k < a.P ? k : a.P - 1;
+    base[j] = reinterpret_cast<const float4*>(a.h32 + (int64_t)kc * a.N * LG) + lamg;
+  }
+  const int64_t last = a.N - 1;
+  auto load = [&](float4 (&hv)[PML], double& wv, int64_t e) {
+    const int64_t ec = e < a.N ? e : last;
+#pragma unroll
+    for (int j = 0; j < PML; ++j) hv[j] = load_pw32x2(base[j] + ec * LGL);
+    wv = a.w[ec];
+  };
+  const int64_t stride = (int64_t)gridDim.x * 4 * EPW;
+  float4 h0[PML], h1[PML];
+  double w0, w1;
+  const int wave_u = __builtin_amdgcn_readfirstlane(wave);
+  const int64_t first = ((int64_t)blockIdx.x * 4 + wave_u) * EPW;
+  int64_t e = first + el;
+  load(h0, w0, e);
+  for (int64_t b = first; b < a.N; b += 2 * stride, e += 2 * stride) {
+    load(h1, w1, e + stride);
+    __builtin_amdgcn_sched_barrier(0);
+    eval_group2<LGL, KG, PML>(a.mode, a.grad, a.f_out, h0, w0, e, e < a.N, leader, cr, ci, vs,
+                              vc, gr, gi);
+    if (b + stride >= a.N) break;
+    load(h0, w0, e + 2 * stride);
+    __builtin_amdgcn_sched_barrier(0);
+    eval_group2<LGL, KG, PML>(a.mode, a.grad, a.f_out, h1, w1, e + stride, e + stride < a.N,
+                              leader, cr, ci, vs, vc, gr, gi);
+  }
+#pragma unroll
+  for (int m = 32; m >= 1; m >>= 1) {
+    const double s2 = __shfl_xor(vs, m, 64);
+    const double c2 = __shfl_xor(vc, m, 64);
+    neumaier_merge(vs, vc, s2, c2);
+  }
+  __shared__ double sh[4][TPA_NACC_MAX];
+  if (a.grad) {
+#pragma unroll
+    for (int j = 0; j < PML; ++j) {
+      double xr = butterfly<1, LGL>(gr[j]), xi = butterfly<1, LGL>(gi[j]);
+      xr = butterfly<LPE, 64>(xr);
+      xi = butterfly<LPE, 64>(xi);
+      const int k = kg + KG * j;
+      if (el == 0 && lamg == 0 && k < a.P) {
         sh[wave][2 + 2 * k] = xr;
         sh[wave][3 + 2 * k] = xi;
       }
@@ -690,6 +881,16 @@ __global__ __launch_bounds__(256) void k_copy(const double2* __restrict__ src,
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
     dst[i] = src[i];
+}
+
+// FP32 copy of a partial-wave cache (round to nearest), for the passes that read FP32.
+__global__ __launch_bounds__(256) void k_pw_to_f32(const double2* __restrict__ src,
+                                                   float2* __restrict__ dst, int64_t n) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+    const double2 v = src[i];
+    dst[i] = make_float2((float)v.x, (float)v.y);
+  }
 }
 
 // ------------------------------------------------------------- MFMA Gram (MC normalisation)

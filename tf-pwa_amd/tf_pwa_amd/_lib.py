@@ -27,6 +27,7 @@ MODE_MC = 1
 FLAG_GRAD = 1
 FLAG_FOUT = 2
 FLAG_SHAPE = 4
+FLAG_F32 = 8
 CPL_SLOTS = 4  # factor slots per partial wave in TpaCouplingMap (total + up to 3 vertices)
 
 _i32 = ctypes.c_int32
@@ -198,6 +199,7 @@ def lib():
         "tpa_events_set_weight": ([vp, dp], ctypes.c_int),
         "tpa_events_get_angles": ([vp, dp, i64], ctypes.c_int),
         "tpa_events_build": ([vp, ctypes.c_int], ctypes.c_int),
+        "tpa_events_keep_f32": ([vp, ctypes.c_int], ctypes.c_int),
         "tpa_events_get_pw": ([vp, dp, i64], ctypes.c_int),
         "tpa_eval": ([vp, dp, ctypes.c_int, ctypes.c_int, dp, dp], ctypes.c_int),
         "tpa_timing_enable": ([ctypes.c_int], ctypes.c_int),
@@ -227,6 +229,7 @@ def lib():
         "tpa_fcn_nll_grad": ([vp, dp, dp, dp], ctypes.c_int),
         "tpa_fcn_set_shape_map": ([vp, vp, ctypes.POINTER(TpaShapeMap)], ctypes.c_int),
         "tpa_fcn_cache_rebuilds": ([vp, ctypes.POINTER(ctypes.c_int64)], ctypes.c_int),
+        "tpa_fcn_set_data_storage": ([vp, ctypes.c_int], ctypes.c_int),
         "tpa_fcn_minimize": ([vp, dp, ctypes.c_double, ctypes.c_int, dp, dp,
                               ctypes.POINTER(ctypes.c_int), ctypes.POINTER(i64),
                               ctypes.POINTER(i64)], ctypes.c_int),
@@ -283,5 +286,5 @@ def exported_symbols():
         "tpa_server_nll_grad", "tpa_server_nll_grad_hess", "tpa_server_submit", "tpa_server_poll", "tpa_server_result",
         "tpa_server_latencies", "tpa_server_counters", "tpa_fcn_create", "tpa_fcn_destroy",
         "tpa_fcn_nll_grad", "tpa_fcn_set_shape_map", "tpa_model_set_lineshapes", "tpa_fcn_minimize",
-        "tpa_fcn_cache_rebuilds",
+        "tpa_fcn_cache_rebuilds", "tpa_events_keep_f32", "tpa_fcn_set_data_storage",
     ]

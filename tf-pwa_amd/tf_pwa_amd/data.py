@@ -175,17 +175,26 @@ class EventSet:
         _lib.check(_lib.lib().tpa_events_get_pw(self._h, _lib.dptr(out), out.size))
         return out.view(np.complex128).reshape(self.amp.P, self.n, self.amp.lam_pad)
 
-    def eval(self, coef, mode, grad=False, fout=False):
+    def keep_f32(self, on=True):
+        """Keep an FP32 copy of the (folded) partial-wave cache beside the FP64 one
+        (tpa_events_keep_f32), refreshed by every rebuild, for passes with ``f32=True``."""
+        _lib.check(_lib.lib().tpa_events_keep_f32(self._h, 1 if on else 0))
+        self._keep32 = bool(on)
+
+    def eval(self, coef, mode, grad=False, fout=False, f32=False):
         """One pass: (value, G or None, f or None). When the model floats lineshape
-        parameters, G is the tuple (G, Dm, Dg) of coupling and lineshape adjoints."""
+        parameters, G is the tuple (G, Dm, Dg) of coupling and lineshape adjoints. ``f32``:
+        read the FP32 copy of the cache (keep_f32; values widened to FP64 on load)."""
         self.ensure_built()
         P = self.amp.P
         shape = not self._state[0]
+        if f32 and not getattr(self, "_keep32", False):
+            self.keep_f32(True)
         c = np.ascontiguousarray(np.asarray(coef, dtype=np.complex128)).view(np.float64)
         out = np.zeros(2 + (6 if shape else 2) * P, dtype=np.float64)
         f = np.empty(self.n, dtype=np.float64) if fout else None
         flags = ((_lib.FLAG_GRAD if grad else 0) | (_lib.FLAG_FOUT if fout else 0)
-                 | (_lib.FLAG_SHAPE if shape else 0))
+                 | (_lib.FLAG_SHAPE if shape else 0) | (_lib.FLAG_F32 if f32 else 0))
         _lib.check(_lib.lib().tpa_eval(self._h, _lib.dptr(c), mode, flags, _lib.dptr(out),
                                        _lib.dptr(f)))
         G = None

@@ -65,20 +65,26 @@ class Model:
     """Model.nll / nll_grad / nll_grad_batch / nll_grad_hessian (model.py:576-790)."""
 
     def __init__(self, amp, w_bkg=1.0, resolution_size=1, extended=False, precision="fp64",
-                 gram_precision="fp64", **kwargs):
+                 gram_precision="fp64", data_storage="fp64", **kwargs):
         """precision: "fp64" evaluates every MC event per call; "mixed" replaces the MC pass
         by the partial-wave Gram matrix built once on MFMA (valid while the partial waves,
         i.e. lineshape parameters and MC weights, are unchanged; rebuilt otherwise).
         gram_precision: the Gram's arithmetic in mixed precision, "fp64" (FP64 MFMA, the
         default: the Gram is built once per lineshape state, so its cost is off the per-call
         path) or "fp32" (FP32 MFMA with FP64 refolds; its partial sums drift by up to ~1e-8
-        relative for narrow peaks over a broad MC sample)."""
+        relative for narrow peaks over a broad MC sample).
+        data_storage: the data partial-wave cache the mixed path reads, "fp64" or "fp32"
+        (an FP32 copy kept beside the FP64 cache, values widened to FP64 on load, every sum
+        in FP64: half the bytes per data event; with fixed lineshapes, else FP64)."""
         if precision not in ("fp64", "mixed"):
             raise ValueError("precision must be 'fp64' or 'mixed'")
         if gram_precision not in ("fp32", "fp64"):
             raise ValueError("gram_precision must be 'fp32' or 'fp64'")
+        if data_storage not in ("fp32", "fp64"):
+            raise ValueError("data_storage must be 'fp32' or 'fp64'")
         self.precision = precision
         self.gram_precision = gram_precision
+        self.data_storage = data_storage
         if resolution_size != 1:
             raise NotImplementedError("resolution folding is not supported")
         if extended:
@@ -136,7 +142,7 @@ class Model:
                 v = float(np.real(np.vdot(coef, Ic)))
                 g = (np.conj(Ic),)
             else:
-                v, g, _ = s.eval(coef, mode, grad=grad)
+                v, g, _ = s.eval(coef, mode, grad=grad, f32=self._data_f32(mode))
                 if grad and not shape:
                     g = (g,)
             value += v
@@ -144,6 +150,11 @@ class Model:
                 for acc, x in zip(G, g):
                     acc += x
         return value, G
+
+    def _data_f32(self, mode):
+        """Whether a pass of this mode reads the FP32 data caches."""
+        return (mode == _lib.MODE_DATA and self.precision == "mixed"
+                and self.data_storage == "fp32" and not self.Amp.shape_mode)
 
     @staticmethod
     def _split(sets, w):
@@ -336,6 +347,10 @@ class _NativeSession:
         if self.shape:
             self._shape = shape_map(amp)
             _lib.check(_lib.lib().tpa_fcn_set_shape_map(h, amp.device(), ctypes.byref(self._shape)))
+        elif model._data_f32(_lib.MODE_DATA):
+            _lib.check(_lib.lib().tpa_fcn_set_data_storage(h, 1))
+            for s in data_sets:
+                s._keep32 = True
         self.n = len(self._keep["bound_kind"])
         self._g = np.zeros(self.n)
         self._nll = ctypes.c_double()
@@ -438,7 +453,8 @@ class FCN:
         sets = list(self.wdata.sets) + list(self.mc_sets)
         for st, w in zip(sets, list(dw) + list(mw)):
             st.set_weight(w)
-        key = (self.model.precision, self.model.gram_precision, self.vm.struct_version,
+        key = (self.model.precision, self.model.gram_precision, self.model.data_storage,
+               self.vm.struct_version,
                tuple(st._weight_version for st in sets))
         if self._native_key != key:
             if self._native is not None:
