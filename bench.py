@@ -445,8 +445,8 @@ def four_body_leg(args, seed=5505):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--n-data", type=int, default=1_000_000)
     ap.add_argument("--n-mc", type=int, default=10_000_000)
     ap.add_argument("--no-cpu-baseline", action="store_true")

@@ -1090,9 +1090,18 @@ int tpa_eval(tpa_events_t ev, const double* coef, int mode, int flags, double* o
   if (!ev->h_io) HIPCHK(hipHostMalloc((void**)&ev->h_io, sizeof(double) * TPA_IO_DOUBLES));
   double* h_coef = ev->h_io;
   double* h_out = ev->h_io + 2 * TPA_MAX_P;
-  std::memcpy(h_coef, coef, sizeof(double2) * M.P);
-  HIPCHK(hipMemcpyAsync(ev->d_coef, h_coef, sizeof(double2) * M.P, hipMemcpyHostToDevice, s));
   const bool shape = (flags & TPA_FLAG_SHAPE) != 0;
+  // the folded passes take P <= 32 coefficients in their kernel arguments and k_reduce writes
+  // the sums straight into the pinned staging: no copy before or after the pass
+  static const bool direct_io = [] {
+    const char* v = std::getenv("TFPWA_EVAL_IO");
+    return !(v && std::atoi(v) == 0);
+  }();
+  const bool args_coef = direct_io && !shape && M.P <= TPA_EVAL_CIN && eval_unroll() == 0;
+  if (!args_coef) {
+    std::memcpy(h_coef, coef, sizeof(double2) * M.P);
+    HIPCHK(hipMemcpyAsync(ev->d_coef, h_coef, sizeof(double2) * M.P, hipMemcpyHostToDevice, s));
+  }
   const bool f32 = (flags & TPA_FLAG_F32) != 0;
   if (f32 && shape) return fail(TPA_ERR_UNSUPPORTED, "FP32 cache reads need a folded cache");
   if (f32 && !ev->h32_valid)
@@ -1101,7 +1110,8 @@ int tpa_eval(tpa_events_t ev, const double* coef, int mode, int flags, double* o
   a.h = ev->d_h;
   a.h32 = f32 ? ev->d_h32 : nullptr;
   a.w = ev->d_w;
-  a.coef = ev->d_coef;
+  a.coef = args_coef ? nullptr : ev->d_coef;
+  if (args_coef) std::memcpy(a.cin, coef, sizeof(double2) * M.P);
   a.partials = ev->d_part;
   a.f_out = want_f ? ev->d_f : nullptr;
   a.N = ev->n;
@@ -1153,9 +1163,11 @@ int tpa_eval(tpa_events_t ev, const double* coef, int mode, int flags, double* o
   HIPCHK(hipGetLastError());
   if (timing) HIPCHK(hipEventRecord(ev->t1, s));
   const int nout = a.grad ? a.nacc : 2;
-  hipLaunchKernelGGL(k_reduce, dim3(nout), dim3(256), 0, s, ev->d_part, grid, a.nacc, ev->d_out);
+  hipLaunchKernelGGL(k_reduce, dim3(nout), dim3(256), 0, s, ev->d_part, grid, a.nacc,
+                     direct_io ? h_out : ev->d_out);
   HIPCHK(hipGetLastError());
-  HIPCHK(hipMemcpyAsync(h_out, ev->d_out, sizeof(double) * nout, hipMemcpyDeviceToHost, s));
+  if (!direct_io)
+    HIPCHK(hipMemcpyAsync(h_out, ev->d_out, sizeof(double) * nout, hipMemcpyDeviceToHost, s));
   if (want_f)
     HIPCHK(hipMemcpyAsync(f_out, ev->d_f, sizeof(double) * ev->n, hipMemcpyDeviceToHost, s));
   HIPCHK(hipStreamSynchronize(s));

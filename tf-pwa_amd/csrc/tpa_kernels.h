@@ -285,11 +285,13 @@ __global__ __launch_bounds__(128) void k_build_pw(const TpaModel* __restrict__ M
 }
 
 // ------------------------------------------------------------------------- evaluation
+#define TPA_EVAL_CIN 32  // coefficients passed in the kernel arguments (P <= 32)
+
 struct EvalArgs {
   const double2* __restrict__ h;     // [P][N][LG]
   const float2* __restrict__ h32;    // [P][N][LG], FP32 copy of h (passes that read FP32)
   const double* __restrict__ w;      // [N]
-  const double2* __restrict__ coef;  // [P]
+  const double2* __restrict__ coef;  // [P] in device memory, or null: cin holds them
   double* __restrict__ partials;     // [grid][nacc]
   double* __restrict__ f_out;        // [N] or null
   int64_t N;
@@ -297,7 +299,11 @@ struct EvalArgs {
   int mode;  // 0: data, sum w * clip_log(f); 1: MC, sum w * f
   int grad;
   int nacc;  // 2 + 2P
+  double2 cin[TPA_EVAL_CIN];  // coefficients by value (k_eval_pipe / k_eval_pipe32, coef null)
 };
+
+// Coefficient k of an evaluation pass (from the kernel arguments when a.coef is null).
+TPA_DEV double2 eval_coef(const EvalArgs& a, int k) { return a.coef ? a.coef[k] : a.cin[k]; }
 
 TPA_DEV void neumaier_add(double& s, double& c, double y) {
   const double t = s + y;
@@ -616,7 +622,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE))) void
   for (int j = 0; j < PML; ++j) {
     const int k = kg + KG * j;
     gr[j] = gi[j] = 0.0;
-    const double2 c = k < a.P ? a.coef[k] : make_double2(0.0, 0.0);
+    const double2 c = k < a.P ? eval_coef(a, k) : make_double2(0.0, 0.0);
     cr[j] = c.x;
     ci[j] = c.y;
     const int kc = k < a.P ? k : a.P - 1;
@@ -775,7 +781,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(PML <= 6 ? 
   for (int j = 0; j < PML; ++j) {
     const int k = kg + KG * j;
     gr[j] = gi[j] = 0.0;
-    const double2 c = k < a.P ? a.coef[k] : make_double2(0.0, 0.0);
+    const double2 c = k < a.P ? eval_coef(a, k) : make_double2(0.0, 0.0);
     cr[j] = c.x;
     ci[j] = c.y;
     const int kc = k < a.P ? k : a.P - 1;
@@ -871,7 +877,11 @@ __global__ __launch_bounds__(256) void k_reduce(const double* __restrict__ parti
     }
     __syncthreads();
   }
-  if (t == 0) out[j] = (j == 1) ? 0.0 : ss[0] + cc[0];
+  if (t == 0) {
+    out[j] = (j == 1) ? 0.0 : ss[0] + cc[0];
+    // out may be pinned host memory read right after the stream synchronises
+    __threadfence_system();
+  }
 }
 
 // Plain streaming copy (16-byte loads and stores), the calibration kernel for HBM byte

@@ -352,8 +352,13 @@ class _NativeSession:
             for s in data_sets:
                 s._keep32 = True
         self.n = len(self._keep["bound_kind"])
-        self._g = np.zeros(self.n)
+        # argument buffers with their pointers taken once (a ctypes pointer per call costs
+        # ~3 us of the host's time per evaluation)
+        self._ybuf = np.zeros(self.n)
+        self._gbuf = np.zeros(self.n)
+        self._yp, self._gp = _lib.dptr(self._ybuf), _lib.dptr(self._gbuf)
         self._nll = ctypes.c_double()
+        self._nllp = ctypes.byref(self._nll)
         self._rebuilds = ctypes.c_int64(0)
         self._seen_rebuilds = 0
 
@@ -374,15 +379,12 @@ class _NativeSession:
                 s._state = None
 
     def nll_grad(self, y, grad=True):
-        import ctypes
-
-        y = np.ascontiguousarray(y, dtype=np.float64)
-        g = np.zeros(self.n) if grad else None
-        rc = _lib.lib().tpa_fcn_nll_grad(self._h, _lib.dptr(y), ctypes.byref(self._nll),
-                                         _lib.dptr(g))
+        self._ybuf[:] = y
+        rc = _lib.lib().tpa_fcn_nll_grad(self._h, self._yp, self._nllp,
+                                         self._gp if grad else None)
         self.after_call()
         _lib.check(rc)
-        return self._nll.value, g
+        return self._nll.value, (self._gbuf.copy() if grad else None)
 
     def close(self):
         h = getattr(self, "_h", None)
@@ -424,6 +426,7 @@ class FCN:
         self.native = True
         self._native = None
         self._native_key = None
+        self._set_weights = None  # (event set, its weights) of the data and MC sets
 
     def _set(self, x):
         if x is None or (isinstance(x, dict) and not x):
@@ -450,12 +453,14 @@ class FCN:
         if not self.native or (amp.shape_mode and amp.n_final != 3):
             return None
         dw, mw, sw = self._split
-        sets = list(self.wdata.sets) + list(self.mc_sets)
-        for st, w in zip(sets, list(dw) + list(mw)):
+        pairs = self._set_weights
+        if pairs is None:
+            pairs = self._set_weights = list(zip(list(self.wdata.sets) + list(self.mc_sets),
+                                                 list(dw) + list(mw)))
+        for st, w in pairs:
             st.set_weight(w)
         key = (self.model.precision, self.model.gram_precision, self.model.data_storage,
-               self.vm.struct_version,
-               tuple(st._weight_version for st in sets))
+               self.vm.struct_version, tuple([st._weight_version for st, _ in pairs]))
         if self._native_key != key:
             if self._native is not None:
                 self._native.close()

@@ -138,12 +138,13 @@ class VarsManager:
                 if k in self.variables:
                     self.variables[k] = float(v)
         else:
-            vals = list(vals)
+            # ndarray.tolist gives Python floats in one call (float() per numpy scalar costs
+            # ~0.2 us each, on every likelihood call)
+            vals = vals.tolist() if isinstance(vals, np.ndarray) else [float(v) for v in vals]
             if len(vals) != len(self.trainable_vars):
                 raise ValueError("expected {} values, got {}".format(
                     len(self.trainable_vars), len(vals)))
-            for n, v in zip(self.trainable_vars, vals):
-                self.variables[n] = float(v)
+            self.variables.update(zip(self.trainable_vars, vals))
 
     def complex_value(self, name):
         """Variable.__call__ (variable.py:1715-1765)."""
