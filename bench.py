@@ -207,12 +207,15 @@ def serving_leg(conf, mc, data_p4, x_true, args, seed=4242):
                              scale=0.1, size=x_true.size)))
     log("serving: {} samples drawn in {:.1f}s".format(m, time.perf_counter() - t))
     t = time.perf_counter()
+    storage = args.serve_storage
     srv = FitServer(conf.get_amplitude(), mc, hbm_budget=args.serve_budget_gb * 1e9,
-                    max_batch=m)
+                    max_batch=m, data_storage=storage)
     setup_s = time.perf_counter() - t
-    log("serving: pool {} events ({:.1f} GB) in {:.1f}s; {} fits, {} events".format(
-        srv.pool.capacity, srv.pool.capacity * srv.pool.bytes_per_event(srv.amp) / 1e9, setup_s,
-        m, int(sizes.sum())))
+    bpe_pool = srv.pool.bytes_per_event(srv.amp, storage)
+    bpe_read = amp.P * amp.lam_pad * (8 if storage == "fp32" else 16) + 8
+    log("serving: pool {} events ({:.1f} GB) in {:.1f}s; {} fits, {} events; data waves {}".format(
+        srv.pool.capacity, srv.pool.capacity * bpe_pool / 1e9, setup_s, m, int(sizes.sum()),
+        storage))
     t0 = time.perf_counter()
     results, times = srv.serve(jobs, arrivals, timeout_s=600)
     wall = time.perf_counter() - t0
@@ -231,16 +234,20 @@ def serving_leg(conf, mc, data_p4, x_true, args, seed=4242):
     out = {
         "workload": "{} toy fits of the Lc->pKpi 6-resonance model (34 free couplings), data "
                     "sizes lognormal median 30k clipped [5k, 200k] (total {}), shared 10M MC "
-                    "Gram, Poisson arrivals at {}/s, HBM budget {} GB".format(
-                        m, int(sizes.sum()), args.serve_rate, args.serve_budget_gb),
+                    "Gram, Poisson arrivals at {}/s, HBM budget {} GB, data waves stored {} "
+                    "(FP64 arithmetic)".format(
+                        m, int(sizes.sum()), args.serve_rate, args.serve_budget_gb, storage),
+        "data_storage": storage,
         "evals_per_s": c["requests"] / span,
         "p50_ms": float(np.percentile(lat, 50)),
         "p99_ms": float(np.percentile(lat, 99)),
         "requests": c["requests"],
         "batches": c["batches"],
         "events_evaluated": c["events"],
-        "k_eval_multi_bytes": c["events"] * 1416,
-        "achieved_GBps_in_graphs": c["events"] * 1416 / (c["gpu_ms"] * 1e-3) / 1e9,
+        "kernel": "k_eval_multi32" if storage == "fp32" else "k_eval_multi",
+        "bytes_per_event": bpe_read,
+        "k_eval_multi_bytes": c["events"] * bpe_read,
+        "achieved_GBps_in_graphs": c["events"] * bpe_read / (c["gpu_ms"] * 1e-3) / 1e9,
         "mean_batch": c["requests"] / max(1, c["batches"]),
         "gpu_busy_frac": c["gpu_ms"] / 1e3 / span,
         "span_s": span,
@@ -248,7 +255,7 @@ def serving_leg(conf, mc, data_p4, x_true, args, seed=4242):
         "fits_per_s": m / span,
         "fit_s_p50": float(np.median([t["done"] - t["admitted"] for t in times])),
         "admission_wait_s_max": float(max(t["admitted"] - t["arrival"] for t in times)),
-        "peak_pool_GB": srv.peak_events * srv.pool.bytes_per_event(srv.amp) / 1e9,
+        "peak_pool_GB": srv.peak_events * bpe_pool / 1e9,
         "status": status,
         "setup_s": setup_s,
     }
@@ -455,6 +462,8 @@ def main():
     ap.add_argument("--serve-rate", type=float, default=512.0,
                     help="Poisson arrival rate of the serving leg's fits per second")
     ap.add_argument("--serve-budget-gb", type=float, default=200.0)
+    ap.add_argument("--serve-storage", choices=("fp32", "fp64"), default="fp32",
+                    help="data partial waves the serving pool's batched passes read")
     ap.add_argument("--four-body-mc", type=int, default=50_000_000,
                     help="MC events of the four-body leg (0 skips it)")
     ap.add_argument("--four-body-data", type=int, default=2_000_000)

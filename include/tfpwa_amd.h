@@ -214,6 +214,13 @@ int tpa_pool_destroy(tpa_pool_t pool);
 int tpa_pool_add(tpa_pool_t pool, const double* p4, int64_t n, const double* weight,
                  int64_t* offset);
 
+/* on != 0: keep an FP32 copy (round to nearest) of the pool's partial-wave cache, made now for
+ * every allocated range and by every later tpa_pool_add; the batched passes (tpa_pool_eval and
+ * the serving runtime) then read it (half the bytes per event, FP64 arithmetic after the load).
+ * Needs >= 2 helicity slots and no batch in flight. tpa_pool_hess keeps reading the FP64 cache.
+ * No reference counterpart: the storage option of the mixed-precision data pass. */
+int tpa_pool_keep_f32(tpa_pool_t pool, int on);
+
 /* Return the range that starts at offset to the free list (merged with free neighbours). */
 int tpa_pool_release(tpa_pool_t pool, int64_t offset);
 

@@ -80,6 +80,50 @@ def test_pool_batch_matches_single_sets(case6, gpu_available):
     pool.close()
 
 
+def test_pool_fp32_batch_matches_fp32_single_sets(case6, gpu_available):
+    """The pool's FP32 copy (tpa_pool_keep_f32), made for the ranges already allocated when it
+    is switched on and by every later add: a batched request equals the single-set pass over
+    the set's own FP32 copy (TPA_FLAG_F32) to 1e-12, alone or in the mix bit-identically, and
+    the FP64 pool's result to the FP32 rounding (1e-6 relative at these sizes)."""
+    import ctypes
+
+    from tf_pwa_amd import EventPool
+    from tf_pwa_amd import _lib
+
+    conf = case6["conf"]
+    amp = conf.get_amplitude()
+    pool = EventPool(amp, capacity=20000, max_items=16)
+    rng = np.random.default_rng(6)
+    slots, sets, a = [], [], 0
+    for i, n in enumerate(SIZES):
+        if i == 2:  # switched on with two ranges allocated
+            _lib.check(_lib.lib().tpa_pool_keep_f32(pool._h, 1))
+        p4 = _slice(case6["data"], a, a + n)
+        w = rng.uniform(0.5, 1.5, n)
+        slots.append(pool.add(p4, w, order=conf.dat_order))
+        ev = conf.data_from_p4(p4)
+        ev.set_weight(w * (w.sum() / (w * w).sum()))
+        sets.append(ev)
+        a += n
+    coefs = _random_coefs(amp, conf.vm, rng, len(SIZES))
+    val, G, _ = pool.eval(slots, coefs)
+    for i, ev in enumerate(sets):
+        v, g, _ = ev.eval(coefs[i], _lib.MODE_DATA, grad=True, f32=True)
+        assert abs(val[i] - v) <= 1e-12 * max(1.0, abs(v)), (i, val[i], v)
+        np.testing.assert_allclose(G[i], g, rtol=1e-11, atol=1e-11 * np.abs(g).max())
+        v64, g64, _ = ev.eval(coefs[i], _lib.MODE_DATA, grad=True)
+        assert abs(val[i] - v64) <= 1e-6 * max(1.0, abs(v64)), (i, val[i], v64)
+    for i in range(len(SIZES)):
+        v1, g1, _ = pool.eval([slots[i]], coefs[i:i + 1])
+        assert v1[0] == val[i] and np.array_equal(g1[0], G[i])
+    # switched off: the batched passes read the FP64 cache again
+    _lib.check(_lib.lib().tpa_pool_keep_f32(pool._h, 0))
+    v0, _, _ = pool.eval(slots[:1], coefs[:1])
+    ref, _, _ = sets[0].eval(coefs[0], _lib.MODE_DATA, grad=True)
+    assert abs(v0[0] - ref) <= 1e-12 * max(1.0, abs(ref))
+    pool.close()
+
+
 def test_pool_admission_and_release(case6, gpu_available):
     from tf_pwa_amd import EventPool, PoolFull
 
@@ -205,6 +249,25 @@ def test_served_fit_reaches_scipy_minimum(case2res, gpu_available):
     n_at, _ = srv.nll_grad(slot, y)
     assert abs(n_at - res.min_nll) < 1e-6
     srv.close()
+
+
+def test_served_fit_with_fp32_pool_matches_fp64_pool(case2res, gpu_available):
+    """A fit served from the pool's FP32 copy ends at the FP64 pool's minimum to the FP32
+    rounding of the data waves (|dNLL| < 1e-4 at 6000 events)."""
+    conf = case2res["conf"]
+    p4 = _slice(case2res["data"], 0, 6000)
+    y0 = _starts(conf, 1, 9, scale=0.1)[0]
+    out = {}
+    for storage in ("fp64", "fp32"):
+        srv, _ = _server(case2res, data_storage=storage)
+        slot = srv.add_data(p4, order=conf.dat_order)
+        fid = srv.submit(slot, y0)
+        _wait(srv, 1)
+        out[storage] = srv.result(fid)
+        srv.close()
+    assert out["fp32"].message in ("converged", "line search failed (precision loss)")
+    assert abs(out["fp32"].min_nll - out["fp64"].min_nll) < 1e-4, (
+        out["fp32"].min_nll, out["fp64"].min_nll)
 
 
 def test_serve_loop_admission_and_results(case2res, gpu_available):

@@ -436,6 +436,7 @@ struct tpa_pool_s {
   double* d_hpart = nullptr;  // second-moment pass (tpa_pool_hess)
   double* d_hout = nullptr;
   double2* d_hcoef = nullptr;
+  float2* d_h32 = nullptr;  // FP32 copy of the cache (tpa_pool_keep_f32): batched passes read it
 };
 
 static void release_range(tpa_pool_s* p, int64_t offset);
@@ -585,6 +586,19 @@ static void launch_build_pw(const tpa_model_s* m, const double* ang, int64_t n, 
 template <int LG, int KG>
 static void launch_multi(const MultiArgs& a, int grid, hipStream_t s) {
   const int need = (a.P + KG - 1) / KG;  // PML as for the single-set pass
+  if constexpr (LG >= 2) {
+    if (a.h32 != nullptr) {  // the pool's FP32 copy
+      if (need <= 4)
+        hipLaunchKernelGGL((k_eval_multi32<LG, KG, 4>), dim3(grid), dim3(256), 0, s, a);
+      else if (need <= 6)
+        hipLaunchKernelGGL((k_eval_multi32<LG, KG, 6>), dim3(grid), dim3(256), 0, s, a);
+      else if (need <= 7)
+        hipLaunchKernelGGL((k_eval_multi32<LG, KG, 7>), dim3(grid), dim3(256), 0, s, a);
+      else
+        hipLaunchKernelGGL((k_eval_multi32<LG, KG, 8>), dim3(grid), dim3(256), 0, s, a);
+      return;
+    }
+  }
   if (need <= 4)
     hipLaunchKernelGGL((k_eval_multi<LG, KG, 4>), dim3(grid), dim3(256), 0, s, a);
   else if (need <= 6)
@@ -1318,6 +1332,15 @@ int tpa_gram_f64(tpa_events_t ev, double* out, double* ms) {
 }
 
 // ------------------------------------------------------------------------ serving pool
+// FP32 copy of the pool events [off, off + n) of every partial-wave row.
+static void launch_rows_to_f32(tpa_pool_s* p, int64_t off, int64_t n, hipStream_t s) {
+  const TpaModel& M = p->m->h;
+  const int64_t total = (int64_t)M.P * n * M.lam_pad;
+  const int64_t grid = std::min<int64_t>((total + 255) / 256, 8192);
+  hipLaunchKernelGGL(k_pw_rows_to_f32, dim3((unsigned)grid), dim3(256), 0, s, p->d_h, p->d_h32,
+                     M.P, p->cap * M.lam_pad, off * M.lam_pad, n * M.lam_pad);
+}
+
 int tpa_pool_destroy(tpa_pool_t p) {
   if (!p) return TPA_OK;
   if (p->stream) (void)hipStreamSynchronize(p->stream);
@@ -1335,7 +1358,7 @@ int tpa_pool_destroy(tpa_pool_t p) {
     for (hipEvent_t x : evs)
       if (x) (void)hipEventDestroy(x);
   }
-  void* dev[] = {p->d_ang, p->d_h, p->d_w};
+  void* dev[] = {p->d_ang, p->d_h, p->d_w, p->d_h32};
   for (void* x : dev)
     if (x) (void)hipFree(x);
   if (p->stream) (void)hipStreamDestroy(p->stream);
@@ -1464,9 +1487,37 @@ int tpa_pool_add(tpa_pool_t p, const double* p4, int64_t n, const double* weight
     return undo(e, "weights");
   launch_cal_angle(m, p->d_stage, n, off, p->cap, p->d_ang, s);
   launch_build_pw(m, p->d_ang, n, off, p->cap, 1, p->d_h, s);
+  if (p->d_h32) launch_rows_to_f32(p, off, n, s);
   if ((e = hipGetLastError()) != hipSuccess) return undo(e, "launch");
   if ((e = hipStreamSynchronize(s)) != hipSuccess) return undo(e, "build");
   *offset = off;
+  return TPA_OK;
+}
+
+int tpa_pool_keep_f32(tpa_pool_t p, int on) {
+  if (!p) return fail(TPA_ERR_ARG, "null pointer");
+  std::lock_guard<std::mutex> add_lk(p->add_mu);
+  std::lock_guard<std::mutex> lk(p->mu);
+  for (auto& q : p->slot)
+    if (q.busy) return fail(TPA_ERR_ARG, "a batch is in flight");
+  const TpaModel& M = p->m->h;
+  if (on && M.lam_pad < 2) return fail(TPA_ERR_UNSUPPORTED, "FP32 pool caches need >= 2 helicity slots");
+  if ((p->d_h32 != nullptr) == (on != 0)) return TPA_OK;
+  HIPCHK(hipStreamSynchronize(p->stream));
+  // the batched graphs captured the cache pointer: capture again on the next batch
+  for (auto& q : p->slot) {
+    for (auto& g : q.graphs) (void)hipGraphExecDestroy(g.second);
+    q.graphs.clear();
+  }
+  if (!on) {
+    HIPCHK(hipFree(p->d_h32));
+    p->d_h32 = nullptr;
+    return TPA_OK;
+  }
+  HIPCHK(hipMalloc(&p->d_h32, sizeof(float2) * (size_t)M.P * p->cap * M.lam_pad));
+  for (auto& r : p->used_ranges) launch_rows_to_f32(p, r.first, r.second, p->add_stream);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(p->add_stream));
   return TPA_OK;
 }
 
@@ -1553,6 +1604,7 @@ int tpa_pool_eval_submit(tpa_pool_t p, int* slot_io, int n_items, const int64_t*
   if (g == q.graphs.end()) {
     MultiArgs a;
     a.h = p->d_h;
+    a.h32 = p->d_h32;
     a.w = p->d_w;
     a.coef = q.d_coef;
     a.chunk_item = q.d_chunk_item;

@@ -1215,6 +1215,7 @@ __global__ __launch_bounds__(256) void k_eval_shape(EvalShapeArgs s) {
 
 struct MultiArgs {
   const double2* __restrict__ h;     // pool cache [P][S][LG]
+  const float2* __restrict__ h32;    // its FP32 copy [P][S][LG] (k_eval_multi32), or null
   const double* __restrict__ w;      // pool weights [S]
   const double2* __restrict__ coef;  // [n_items][P]
   const int* __restrict__ chunk_item;
@@ -1315,6 +1316,113 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(PML <= 6 ? 
   }
   for (int j = 2 + threadIdx.x; j < a.nacc; j += blockDim.x)
     out[j] = ((sh[0][j] + sh[1][j]) + sh[2][j]) + sh[3][j];
+}
+
+// k_eval_multi over the pool's FP32 copy (a.h32), two helicity components per lane as in
+// k_eval_pipe32 (LG >= 2); FP64 arithmetic after the load.
+template <int LG, int KG, int PML>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(PML <= 6 ? 3 : 1))) void k_eval_multi32(MultiArgs a) {
+  static_assert(LG >= 2, "two helicity components per lane");
+  const int chunk = blockIdx.x;
+  if (chunk >= a.counts[0]) return;  // uniform per block, before any barrier
+  constexpr int LGL = LG / 2;
+  constexpr int LPE = LGL * KG;
+  constexpr int EPW = 64 / LPE;
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  const int lamg = lane & (LGL - 1);
+  const int kg = (lane / LGL) & (KG - 1);
+  const int el = lane / LPE;
+  const bool leader = lamg == 0 && kg == 0;
+  const int item = a.chunk_item[chunk];
+  const int64_t end = a.chunk_end[chunk];  // > chunk_start: chunks are never empty
+  const double2* __restrict__ coef = a.coef + (int64_t)item * a.P;
+  double vs = 0.0, vc = 0.0;
+  double gr[PML], gi[PML], cr[PML], ci[PML];
+  const float4* base[PML];
+#pragma unroll
+  for (int j = 0; j < PML; ++j) {
+    const int k = kg + KG * j;
+    gr[j] = gi[j] = 0.0;
+    const double2 c = k < a.P ? coef[k] : make_double2(0.0, 0.0);
+    cr[j] = c.x;
+    ci[j] = c.y;
+    const int kc = k < a.P ? k : a.P - 1;
+    base[j] = reinterpret_cast<const float4*>(a.h32 + (int64_t)kc * a.S * LG) + lamg;
+  }
+  const int64_t last = end - 1;
+  auto load = [&](float4 (&hv)[PML], double& wv, int64_t e) {
+    const int64_t ec = e < end ? e : last;
+#pragma unroll
+    for (int j = 0; j < PML; ++j) hv[j] = load_pw32x2(base[j] + ec * LGL);
+    wv = a.w[ec];
+  };
+  constexpr int64_t stride = 4 * EPW;
+  float4 h0[PML], h1[PML];
+  double w0, w1;
+  const int wave_u = __builtin_amdgcn_readfirstlane(wave);
+  const int64_t first = a.chunk_start[chunk] + wave_u * EPW;
+  int64_t e = first + el;
+  load(h0, w0, e);
+  for (int64_t b = first; b < end; b += 2 * stride, e += 2 * stride) {
+    load(h1, w1, e + stride);
+    __builtin_amdgcn_sched_barrier(0);
+    eval_group2<LGL, KG, PML>(a.mode, true, nullptr, h0, w0, e, e < end, leader, cr, ci, vs, vc,
+                              gr, gi);
+    if (b + stride >= end) break;
+    load(h0, w0, e + 2 * stride);
+    __builtin_amdgcn_sched_barrier(0);
+    eval_group2<LGL, KG, PML>(a.mode, true, nullptr, h1, w1, e + stride, e + stride < end,
+                              leader, cr, ci, vs, vc, gr, gi);
+  }
+#pragma unroll
+  for (int m = 32; m >= 1; m >>= 1) {
+    const double s2 = __shfl_xor(vs, m, 64);
+    const double c2 = __shfl_xor(vc, m, 64);
+    neumaier_merge(vs, vc, s2, c2);
+  }
+  __shared__ double sh[4][TPA_NACC_MAX];
+#pragma unroll
+  for (int j = 0; j < PML; ++j) {
+    double xr = butterfly<1, LGL>(gr[j]), xi = butterfly<1, LGL>(gi[j]);
+    xr = butterfly<LPE, 64>(xr);
+    xi = butterfly<LPE, 64>(xi);
+    const int k = kg + KG * j;
+    if (el == 0 && lamg == 0 && k < a.P) {
+      sh[wave][2 + 2 * k] = xr;
+      sh[wave][3 + 2 * k] = xi;
+    }
+  }
+  if (lane == 0) {
+    sh[wave][0] = vs;
+    sh[wave][1] = vc;
+  }
+  __syncthreads();
+  double* out = a.partials + (int64_t)chunk * a.nacc;
+  if (threadIdx.x == 0) {
+    double s0 = sh[0][0], c0 = sh[0][1];
+    for (int wv = 1; wv < 4; ++wv) neumaier_merge(s0, c0, sh[wv][0], sh[wv][1]);
+    out[0] = s0;
+    out[1] = c0;
+  }
+  for (int j = 2 + threadIdx.x; j < a.nacc; j += blockDim.x)
+    out[j] = ((sh[0][j] + sh[1][j]) + sh[2][j]) + sh[3][j];
+}
+
+// FP32 copy of rows [start, start + count) of every partial-wave row k < P of a pool cache
+// (row stride `stride` complex values), rounded to nearest.
+__global__ __launch_bounds__(256) void k_pw_rows_to_f32(const double2* __restrict__ src,
+                                                        float2* __restrict__ dst, int P,
+                                                        int64_t stride, int64_t start,
+                                                        int64_t count) {
+  const int64_t total = (int64_t)P * count;
+  const int64_t step = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += step) {
+    const int64_t k = i / count, r = i - k * count;
+    const int64_t x = k * stride + start + r;
+    const double2 v = src[x];
+    dst[x] = make_float2((float)v.x, (float)v.y);
+  }
 }
 
 // Per request: the compensated, fixed-order sum of its chunks (item_chunk[i] .. item_chunk[i+1]).

@@ -57,24 +57,32 @@ class EventPool:
     """Device event store shared by many fits (tpa_pool_*). Lineshapes are folded into the
     cache when a sample is added, so the amplitude model must not float masses or widths."""
 
-    def __init__(self, amp, capacity, max_items=512):
+    def __init__(self, amp, capacity, max_items=512, data_storage="fp64"):
+        """data_storage "fp32": the batched passes read an FP32 copy of the pooled partial
+        waves (tpa_pool_keep_f32; FP64 arithmetic), "fp64": the FP64 cache."""
         if amp.shape_mode:
             raise NotImplementedError("the event pool caches fixed lineshapes; masses and "
                                       "widths must not be trainable")
+        if data_storage not in ("fp32", "fp64"):
+            raise ValueError("data_storage must be 'fp32' or 'fp64'")
         self.amp = amp
         self.capacity = int(capacity)
         self.max_items = int(max_items)
+        self.data_storage = data_storage
         h = ctypes.c_void_p()
         _lib.check(_lib.lib().tpa_pool_create(amp.device(), self.capacity, self.max_items,
                                               ctypes.byref(h)))
         self._h = h
+        if data_storage == "fp32":
+            _lib.check(_lib.lib().tpa_pool_keep_f32(h, 1))
 
     @staticmethod
-    def bytes_per_event(amp):
-        """Device bytes one pooled event occupies: the partial-wave cache, the angles and the
-        weight."""
+    def bytes_per_event(amp, data_storage="fp64"):
+        """Device bytes one pooled event occupies: the partial-wave cache (and its FP32 copy),
+        the angles and the weight."""
         nf = _lib.NF_TOPO * len(amp.topo) + 4
-        return 16 * amp.P * amp.lam_pad + 8 * nf + 8
+        extra = 8 * amp.P * amp.lam_pad if data_storage == "fp32" else 0
+        return 16 * amp.P * amp.lam_pad + extra + 8 * nf + 8
 
     def add(self, p4, weight=None, order=None, engine_layout=False):
         """Upload a sample into a free range. ``p4`` as for EventSet (dict or array in
@@ -246,21 +254,24 @@ class FitServer:
     ``hess_start``: each fit's BFGS starts from the inverse exact Hessian at its start point
     (one second-moment pass) rather than from the identity.
     ``gram_precision``: "fp64" (FP64 MFMA) or "fp32" (FP32 MFMA with FP64 refolds) Gram.
+    ``data_storage``: "fp32" keeps an FP32 copy of the pooled data waves for the batched passes
+    (half the bytes per event; FP64 arithmetic), "fp64" reads the FP64 cache.
     """
 
     def __init__(self, amp, mc, hbm_budget=200e9, capacity=None, max_batch=512,
-                 window_us=200.0, gtol=1e-5, maxiter=0, hess_start=False, gram_precision="fp64"):
+                 window_us=200.0, gtol=1e-5, maxiter=0, hess_start=False, gram_precision="fp64",
+                 data_storage="fp64"):
         self.amp = amp
         self.vm = amp.vm
         self.mc = mc
         mc.set_weight(mc.weight / mc.weight.sum())
         self.gram = np.ascontiguousarray(mc.gram(gram_precision))
-        bpe = EventPool.bytes_per_event(amp)
         if capacity is None:
-            capacity = int((hbm_budget - mc.n * bpe) // bpe)
+            bpe = EventPool.bytes_per_event(amp, data_storage)
+            capacity = int((hbm_budget - mc.n * EventPool.bytes_per_event(amp)) // bpe)
         if capacity < 1:
             raise ValueError("the HBM budget leaves no room for data")
-        self.pool = EventPool(amp, capacity, max_batch)
+        self.pool = EventPool(amp, capacity, max_batch, data_storage)
         self._map, self._keep = coupling_map(amp)
         self.n_var = len(self.vm.trainable_vars)
         h = ctypes.c_void_p()
