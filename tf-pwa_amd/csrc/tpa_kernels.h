@@ -18,6 +18,7 @@
 #include <math.h>
 #include <stdint.h>
 
+#include "tpa_log.h"
 #include "tpa_math.h"
 
 // ----------------------------------------------------------------------------- cal_angle
@@ -535,6 +536,12 @@ TPA_DEV double butterfly(double v) {
   }
 }
 
+// The hardware's reciprocal approximation (v_rcp_f64), the seed of tpa_recip_pos.
+struct HwRcp {
+  TPA_DEV double operator()(double d) const { return __builtin_amdgcn_rcp(d); }
+};
+constexpr HwRcp hw_rcp{};
+
 // One event group of the evaluation pass: A_lambda from the lane's PML partial waves, f, the
 // mode's value and derivative, the value into the compensated pair (vs, vc) of the event's
 // leader lane and the adjoint into (gr, gi). Lanes past the last event (valid = false) carry
@@ -568,8 +575,8 @@ TPA_DEV void eval_group(int mode, bool grad, double* __restrict__ f_out,
     // clip_log(f) (tf_pwa/model/model.py:283-292) weighted by w
     double cl, dcl;
     if (f > eps) {
-      cl = log(f);
-      dcl = 1.0 / f;
+      cl = tpa_log_pos(f, hw_rcp);
+      dcl = tpa_recip_pos(f, hw_rcp);
     } else {
       const double t = (f - eps) / eps;
       cl = (log(eps) + t) - (t * t) / 2.0;
@@ -695,6 +702,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE))) void
 // One event group of the FP32-cache pass with two helicity components per lane (one 16-byte
 // load of two float2 per partial wave): as eval_group, with the lane's two A_lambda summed
 // into f before the reduction over the LG/2 lanes of a k-group.
+// FL: clip_log's log and reciprocal from tpa_log.h (about 45 fewer FP64 instructions per event
+// group) or from the math library. The FP32 passes at PML 6 sit at the 168-VGPR limit of three
+// waves per SIMD; tpa_log.h's form fits there only with two helicities per lane and 2 to 8
+// k-groups (the P = 17 .. 24 three-body shapes), so the other PML-6 shapes keep the library's.
+template <int LGL, int KG, int PML>
+constexpr bool fast_log2 = PML != 6 || (LGL == 2 && KG >= 2 && KG <= 8);
+
 template <int LGL, int KG, int PML>
 TPA_DEV void eval_group2(int mode, bool grad, double* __restrict__ f_out,
                          const float4 (&hs)[PML], double w, int64_t e, bool valid, bool leader,
@@ -721,8 +735,13 @@ TPA_DEV void eval_group2(int mode, bool grad, double* __restrict__ f_out,
   if (mode == 0) {
     double cl, dcl;
     if (f > eps) {
-      cl = log(f);
-      dcl = 1.0 / f;
+      if constexpr (fast_log2<LGL, KG, PML>) {
+        dcl = tpa_recip_pos(f, hw_rcp);
+        cl = tpa_log_pos(f, hw_rcp);
+      } else {
+        cl = log(f);
+        dcl = 1.0 / f;
+      }
     } else {
       const double t = (f - eps) / eps;
       cl = (log(eps) + t) - (t * t) / 2.0;
