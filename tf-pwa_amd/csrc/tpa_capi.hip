@@ -4,6 +4,8 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
+#include <chrono>
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
@@ -225,6 +227,9 @@ struct tpa_events_s {
   float2* d_h32 = nullptr;  // FP32 copy of a folded cache (tpa_events_keep_f32)
   bool keep32 = false;      // refresh d_h32 after every folded build
   bool h32_valid = false;
+  unsigned* d_cnt = nullptr;  // k_reduce's finishing-block counter (zero between calls)
+  uint64_t* h_flag = nullptr; // pinned: the sequence number of the last completed reduction
+  uint64_t seq = 0;
 };
 
 // Pinned staging of tpa_eval: 2 * TPA_MAX_P doubles of coefficients, then the sums (at most
@@ -522,7 +527,8 @@ static int run_hess(const TpaModel& M, const double2* h, const double* w, int64_
   }
   if (rc != TPA_OK) return fail(rc, "no second-moment kernel for this helicity count");
   HIPCHK(hipGetLastError());
-  hipLaunchKernelGGL(k_reduce, dim3(nacc), dim3(256), 0, s, *d_part, grid, nacc, *d_out);
+  hipLaunchKernelGGL(k_reduce, dim3(nacc), dim3(256), 0, s, *d_part, grid, nacc, *d_out,
+                     nullptr, nullptr, (uint64_t)0);
   HIPCHK(hipGetLastError());
   if (vec) {
     HIPCHK(hipMemcpyAsync(out, *d_out, sizeof(double) * nacc, hipMemcpyDeviceToHost, s));
@@ -818,6 +824,8 @@ int tpa_events_destroy(tpa_events_t ev) {
   if (ev->d_ang) (void)hipFree(ev->d_ang);
   if (ev->d_h) (void)hipFree(ev->d_h);
   if (ev->d_h32) (void)hipFree(ev->d_h32);
+  if (ev->d_cnt) (void)hipFree(ev->d_cnt);
+  if (ev->h_flag) (void)hipHostFree(ev->h_flag);
   if (ev->d_w) (void)hipFree(ev->d_w);
   if (ev->d_part) (void)hipFree(ev->d_part);
   if (ev->d_out) (void)hipFree(ev->d_out);
@@ -1177,16 +1185,43 @@ int tpa_eval(tpa_events_t ev, const double* coef, int mode, int flags, double* o
   HIPCHK(hipGetLastError());
   if (timing) HIPCHK(hipEventRecord(ev->t1, s));
   const int nout = a.grad ? a.nacc : 2;
+  // the sums land in pinned memory; without f_out the host polls the reduction's flag rather
+  // than waiting on the stream (a bounded spin: after 2 s it waits on the stream, which also
+  // reports a failed launch)
+  const bool poll = direct_io && !want_f;
+  if (poll && !ev->d_cnt) {
+    HIPCHK(hipMalloc(&ev->d_cnt, sizeof(unsigned)));
+    HIPCHK(hipMemsetAsync(ev->d_cnt, 0, sizeof(unsigned), s));
+    HIPCHK(hipHostMalloc((void**)&ev->h_flag, 64, 0));
+    *(volatile uint64_t*)ev->h_flag = 0;
+  }
+  const uint64_t seq = poll ? ++ev->seq : 0;
   hipLaunchKernelGGL(k_reduce, dim3(nout), dim3(256), 0, s, ev->d_part, grid, a.nacc,
-                     direct_io ? h_out : ev->d_out);
+                     direct_io ? h_out : ev->d_out, poll ? ev->d_cnt : nullptr,
+                     poll ? (volatile uint64_t*)ev->h_flag : nullptr, seq);
   HIPCHK(hipGetLastError());
   if (!direct_io)
     HIPCHK(hipMemcpyAsync(h_out, ev->d_out, sizeof(double) * nout, hipMemcpyDeviceToHost, s));
   if (want_f)
     HIPCHK(hipMemcpyAsync(f_out, ev->d_f, sizeof(double) * ev->n, hipMemcpyDeviceToHost, s));
-  HIPCHK(hipStreamSynchronize(s));
+  bool done = false;
+  if (poll) {
+    const volatile uint64_t* flag = ev->h_flag;
+    const auto t_start = std::chrono::steady_clock::now();
+    for (unsigned it = 0; !done; ++it) {
+      if (*flag == seq) {
+        done = true;
+      } else if ((it & 1023u) == 1023u &&
+                 std::chrono::steady_clock::now() - t_start > std::chrono::seconds(2)) {
+        break;
+      }
+    }
+    std::atomic_thread_fence(std::memory_order_acquire);
+  }
+  if (!done) HIPCHK(hipStreamSynchronize(s));
   std::memcpy(out, h_out, sizeof(double) * nout);
   if (timing) {
+    HIPCHK(hipEventSynchronize(ev->t1));
     float ms = 0.0f;
     HIPCHK(hipEventElapsedTime(&ms, ev->t0, ev->t1));
     std::lock_guard<std::mutex> lk(g_timing_mu);

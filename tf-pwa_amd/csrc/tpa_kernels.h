@@ -873,8 +873,14 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(PML <= 6 ? 
 // are the (sum, compensation) pair of the value and fold into out[0]; every other slot is a
 // compensated sum. Each thread reduces a fixed strided subset, then a fixed tree in LDS
 // combines the threads, so the result does not depend on scheduling.
+// With cnt non-null, the block that finishes last (counted on cnt, which it resets to zero)
+// writes seq to *flag once every slot is written: flag and out in pinned host memory let the
+// host poll for the result instead of waiting on the stream.
 __global__ __launch_bounds__(256) void k_reduce(const double* __restrict__ partials, int nblocks,
-                                                int nacc, double* __restrict__ out) {
+                                                int nacc, double* __restrict__ out,
+                                                unsigned* __restrict__ cnt,
+                                                volatile uint64_t* __restrict__ flag,
+                                                uint64_t seq) {
   const int j = blockIdx.x;
   const int t = threadIdx.x;
   __shared__ double ss[256], cc[256];
@@ -898,8 +904,14 @@ __global__ __launch_bounds__(256) void k_reduce(const double* __restrict__ parti
   }
   if (t == 0) {
     out[j] = (j == 1) ? 0.0 : ss[0] + cc[0];
-    // out may be pinned host memory read right after the stream synchronises
+    // out may be pinned host memory that the host reads once the flag (or the stream) says so
     __threadfence_system();
+    if (cnt != nullptr && atomicAdd(cnt, 1u) == gridDim.x - 1) {
+      atomicExch(cnt, 0u);
+      __threadfence_system();
+      *flag = seq;
+      __threadfence_system();
+    }
   }
 }
 
