@@ -528,7 +528,7 @@ static int run_hess(const TpaModel& M, const double2* h, const double* w, int64_
   if (rc != TPA_OK) return fail(rc, "no second-moment kernel for this helicity count");
   HIPCHK(hipGetLastError());
   hipLaunchKernelGGL(k_reduce, dim3(nacc), dim3(256), 0, s, *d_part, grid, nacc, *d_out,
-                     nullptr, nullptr, (uint64_t)0);
+                     nullptr, nullptr, nullptr, (uint64_t)0);
   HIPCHK(hipGetLastError());
   if (vec) {
     HIPCHK(hipMemcpyAsync(out, *d_out, sizeof(double) * nacc, hipMemcpyDeviceToHost, s));
@@ -1113,13 +1113,16 @@ int tpa_eval(tpa_events_t ev, const double* coef, int mode, int flags, double* o
   double* h_coef = ev->h_io;
   double* h_out = ev->h_io + 2 * TPA_MAX_P;
   const bool shape = (flags & TPA_FLAG_SHAPE) != 0;
-  // the folded passes take P <= 32 coefficients in their kernel arguments and k_reduce writes
-  // the sums straight into the pinned staging: no copy before or after the pass
-  static const bool direct_io = [] {
+  // the folded passes take P <= 32 coefficients in their kernel arguments, and k_reduce writes
+  // the sums into the pinned staging and raises a flag the host polls: no copy before or after
+  // the pass and no stream synchronisation. The shape pass keeps the copies and the stream
+  // wait (with the flag it ran 10-15% slower per call, measured; cause not isolated).
+  static const bool io_env = [] {
     const char* v = std::getenv("TFPWA_EVAL_IO");
     return !(v && std::atoi(v) == 0);
   }();
-  const bool args_coef = direct_io && !shape && M.P <= TPA_EVAL_CIN && eval_unroll() == 0;
+  const bool direct_io = io_env && !shape;
+  const bool args_coef = direct_io && M.P <= TPA_EVAL_CIN && eval_unroll() == 0;
   if (!args_coef) {
     std::memcpy(h_coef, coef, sizeof(double2) * M.P);
     HIPCHK(hipMemcpyAsync(ev->d_coef, h_coef, sizeof(double2) * M.P, hipMemcpyHostToDevice, s));
@@ -1197,7 +1200,7 @@ int tpa_eval(tpa_events_t ev, const double* coef, int mode, int flags, double* o
   }
   const uint64_t seq = poll ? ++ev->seq : 0;
   hipLaunchKernelGGL(k_reduce, dim3(nout), dim3(256), 0, s, ev->d_part, grid, a.nacc,
-                     direct_io ? h_out : ev->d_out, poll ? ev->d_cnt : nullptr,
+                     ev->d_out, direct_io ? h_out : nullptr, poll ? ev->d_cnt : nullptr,
                      poll ? (volatile uint64_t*)ev->h_flag : nullptr, seq);
   HIPCHK(hipGetLastError());
   if (!direct_io)

@@ -873,17 +873,21 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(PML <= 6 ? 
 // are the (sum, compensation) pair of the value and fold into out[0]; every other slot is a
 // compensated sum. Each thread reduces a fixed strided subset, then a fixed tree in LDS
 // combines the threads, so the result does not depend on scheduling.
-// With cnt non-null, the block that finishes last (counted on cnt, which it resets to zero)
-// writes seq to *flag once every slot is written: flag and out in pinned host memory let the
-// host poll for the result instead of waiting on the stream.
+// host_out: pinned host memory for the results. With cnt null every block writes its slot
+// there (and fences at system scope); with cnt non-null the slots go to out (device memory)
+// and the block that finishes last (counted on cnt, which it resets to zero) copies them to
+// host_out in one coalesced write and then writes seq to *flag, so the host can poll for the
+// result instead of waiting on the stream.
 __global__ __launch_bounds__(256) void k_reduce(const double* __restrict__ partials, int nblocks,
                                                 int nacc, double* __restrict__ out,
+                                                double* __restrict__ host_out,
                                                 unsigned* __restrict__ cnt,
                                                 volatile uint64_t* __restrict__ flag,
                                                 uint64_t seq) {
   const int j = blockIdx.x;
   const int t = threadIdx.x;
   __shared__ double ss[256], cc[256];
+  __shared__ int last;
   double s = 0.0, c = 0.0;
   if (j != 1) {
     for (int b = t; b < nblocks; b += 256) neumaier_add(s, c, partials[(int64_t)b * nacc + j]);
@@ -902,16 +906,34 @@ __global__ __launch_bounds__(256) void k_reduce(const double* __restrict__ parti
     }
     __syncthreads();
   }
-  if (t == 0) {
-    out[j] = (j == 1) ? 0.0 : ss[0] + cc[0];
-    // out may be pinned host memory that the host reads once the flag (or the stream) says so
-    __threadfence_system();
-    if (cnt != nullptr && atomicAdd(cnt, 1u) == gridDim.x - 1) {
-      atomicExch(cnt, 0u);
-      __threadfence_system();
-      *flag = seq;
-      __threadfence_system();
+  const double v = (j == 1) ? 0.0 : ss[0] + cc[0];
+  if (cnt == nullptr) {
+    if (t == 0) {
+      if (host_out != nullptr) {
+        host_out[j] = v;
+        __threadfence_system();
+      } else {
+        out[j] = v;
+      }
     }
+    return;
+  }
+  if (t == 0) {
+    out[j] = v;
+    __threadfence();
+    last = atomicAdd(cnt, 1u) == gridDim.x - 1;
+  }
+  __syncthreads();
+  if (!last) return;  // uniform per block
+  __threadfence();
+  const volatile double* vout = out;  // the other blocks' slots, past this CU's caches
+  for (int i = t; i < gridDim.x; i += 256) host_out[i] = vout[i];
+  __syncthreads();
+  if (t == 0) {
+    atomicExch(cnt, 0u);
+    __threadfence_system();
+    *flag = seq;
+    __threadfence_system();
   }
 }
 
