@@ -230,6 +230,8 @@ struct tpa_events_s {
   unsigned* d_cnt = nullptr;  // k_reduce's finishing-block counter (zero between calls)
   uint64_t* h_flag = nullptr; // pinned: the sequence number of the last completed reduction
   uint64_t seq = 0;
+  int pending_nout = 0;                 // a queued evaluation (tpa_eval_enqueue) not collected
+  hipStream_t pending_stream = nullptr;
 };
 
 // Pinned staging of tpa_eval: 2 * TPA_MAX_P doubles of coefficients, then the sums (at most
@@ -1099,14 +1101,17 @@ int tpa_events_get_pw(tpa_events_t ev, double* out, int64_t out_len) {
   return TPA_OK;
 }
 
-int tpa_eval(tpa_events_t ev, const double* coef, int mode, int flags, double* out,
-             double* f_out) {
-  if (!ev || !coef || !out) return fail(TPA_ERR_ARG, "null pointer");
+// One evaluation pass on stream s. async: queue it and return (shape passes without f_out
+// only); eval_collect then waits for the stream and copies the sums.
+static int eval_impl(tpa_events_s* ev, const double* coef, int mode, int flags, double* out,
+                     double* f_out, hipStream_t s, bool async) {
+  if (!ev || !coef || (!out && !async)) return fail(TPA_ERR_ARG, "null pointer");
   if (!ev->d_h) return fail(TPA_ERR_ARG, "partial-wave cache not built");
   if (mode != TPA_MODE_DATA && mode != TPA_MODE_MC) return fail(TPA_ERR_ARG, "bad mode");
   const TpaModel& M = ev->m->h;
-  hipStream_t s = ev->m->stream;
   const bool want_f = (flags & TPA_FLAG_FOUT) != 0;
+  if (async && (want_f || !(flags & TPA_FLAG_SHAPE)))
+    return fail(TPA_ERR_UNSUPPORTED, "queued evaluation serves shape passes without f_out");
   if (want_f && !f_out) return fail(TPA_ERR_ARG, "f_out required");
   if (want_f && !ev->d_f) HIPCHK(hipMalloc(&ev->d_f, sizeof(double) * ev->n));
   if (!ev->h_io) HIPCHK(hipHostMalloc((void**)&ev->h_io, sizeof(double) * TPA_IO_DOUBLES));
@@ -1153,8 +1158,8 @@ int tpa_eval(tpa_events_t ev, const double* coef, int mode, int flags, double* o
   const int kg = eval_kgroups(M.lam_pad, M.P);
   if (kg < 0) return fail(TPA_ERR_UNSUPPORTED, "too many partial waves for this helicity count");
   int grid = eval_grid(ev->n, f32 ? f32_lanes_per_event(M.lam_pad, kg) : M.lam_pad * kg);
-  bool timing;
-  {
+  bool timing = false;
+  if (!async) {
     std::lock_guard<std::mutex> lk(g_timing_mu);
     timing = g_timing;
   }
@@ -1207,6 +1212,11 @@ int tpa_eval(tpa_events_t ev, const double* coef, int mode, int flags, double* o
     HIPCHK(hipMemcpyAsync(h_out, ev->d_out, sizeof(double) * nout, hipMemcpyDeviceToHost, s));
   if (want_f)
     HIPCHK(hipMemcpyAsync(f_out, ev->d_f, sizeof(double) * ev->n, hipMemcpyDeviceToHost, s));
+  if (async) {
+    ev->pending_nout = nout;
+    ev->pending_stream = s;
+    return TPA_OK;
+  }
   bool done = false;
   if (poll) {
     const volatile uint64_t* flag = ev->h_flag;
@@ -1231,6 +1241,34 @@ int tpa_eval(tpa_events_t ev, const double* coef, int mode, int flags, double* o
     g_timing_ms += ms;
     g_timing_n += 1;
   }
+  return TPA_OK;
+}
+
+int tpa_eval(tpa_events_t ev, const double* coef, int mode, int flags, double* out,
+             double* f_out) {
+  if (!ev) return fail(TPA_ERR_ARG, "null pointer");
+  return eval_impl(ev, coef, mode, flags, out, f_out, ev->m->stream, false);
+}
+
+int tpa_eval_enqueue(tpa_events_t ev, const double* coef, int mode, int flags, void* stream) {
+  if (!ev) return fail(TPA_ERR_ARG, "null pointer");
+  return eval_impl(ev, coef, mode, flags, nullptr, nullptr,
+                   stream ? (hipStream_t)stream : ev->m->stream, true);
+}
+
+int tpa_eval_collect(tpa_events_t ev, double* out) {
+  if (!ev || !out) return fail(TPA_ERR_ARG, "null pointer");
+  if (ev->pending_nout <= 0) return fail(TPA_ERR_ARG, "no queued evaluation");
+  const int nout = ev->pending_nout;
+  ev->pending_nout = 0;
+  HIPCHK(hipStreamSynchronize(ev->pending_stream));
+  std::memcpy(out, ev->h_io + 2 * TPA_MAX_P, sizeof(double) * nout);
+  return TPA_OK;
+}
+
+int tpa_model_stream(tpa_model_t m, void** stream) {
+  if (!m || !stream) return fail(TPA_ERR_ARG, "null pointer");
+  *stream = (void*)m->stream;
   return TPA_OK;
 }
 

@@ -4,6 +4,8 @@
 // (mixed path) the MC normalisation is the Gram form c^dagger I c, and the gradient in y follows
 // by the coupling chain rule. The same arithmetic as Model._nll_grad of the Python host, without
 // its per-call interpreter work.
+#include <hip/hip_runtime.h>
+
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -27,6 +29,13 @@ struct tpa_fcn_s : CouplingMap {
   TpaShapeMap sm{};
   int64_t rebuilds = 0;  // cache rebuilds after barrier clamp flags changed
   int data_flags = 0;    // extra tpa_eval flags of the data passes (TPA_FLAG_F32)
+  // shape passes: the MC sets run on their own stream, overlapping the data sets' passes
+  hipStream_t mc_stream = nullptr;
+  hipEvent_t shapes_set = nullptr;  // the lineshape parameters are on the device
+  ~tpa_fcn_s() {
+    if (mc_stream) (void)hipStreamDestroy(mc_stream);
+    if (shapes_set) (void)hipEventDestroy(shapes_set);
+  }
 };
 
 namespace {
@@ -148,6 +157,25 @@ int shape_nll_grad(tpa_fcn_s* F, const double* y, double* nll, double* grad) {
     F->coef[2 * k + 1] = c[k].imag();
   }
   const int flags = TPA_FLAG_SHAPE | (grad ? TPA_FLAG_GRAD : 0);
+  // data passes on the model's stream, MC passes on the session's second stream once the
+  // lineshape parameters are on the device: the two overlap; then every pass is collected in
+  // the order of the sums below
+  void* ms = nullptr;
+  if ((rc = tpa_model_stream(F->model, &ms)) != TPA_OK) return rc;
+  if (!F->mc_stream) {
+    if (hipStreamCreateWithFlags(&F->mc_stream, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&F->shapes_set, hipEventDisableTiming) != hipSuccess)
+      return tpa_set_error(TPA_ERR_HIP, "session stream");
+  }
+  if (hipEventRecord(F->shapes_set, (hipStream_t)ms) != hipSuccess ||
+      hipStreamWaitEvent(F->mc_stream, F->shapes_set, 0) != hipSuccess)
+    return tpa_set_error(TPA_ERR_HIP, "session stream order");
+  int queued = TPA_OK;
+  for (int pass = 0; pass < 2 && queued == TPA_OK; ++pass)
+    for (tpa_events_t ev : pass == 0 ? F->data : F->mc)
+      if (queued == TPA_OK)
+        queued = tpa_eval_enqueue(ev, F->coef.data(), pass == 0 ? TPA_MODE_DATA : TPA_MODE_MC,
+                                  flags, pass == 0 ? ms : (void*)F->mc_stream);
   std::vector<cplx> Gd(3 * P, 0.0), Gm(3 * P, 0.0);
   double sd = 0.0, smc = 0.0;
   for (int pass = 0; pass < 2; ++pass) {
@@ -155,13 +183,21 @@ int shape_nll_grad(tpa_fcn_s* F, const double* y, double* nll, double* grad) {
     auto& G = pass == 0 ? Gd : Gm;
     double& acc = pass == 0 ? sd : smc;
     for (tpa_events_t ev : sets) {
-      rc = tpa_eval(ev, F->coef.data(), pass == 0 ? TPA_MODE_DATA : TPA_MODE_MC, flags,
-                    F->out.data(), nullptr);
-      if (rc != TPA_OK) return rc;
+      // every queued pass is collected (waited for) even after a failure
+      const int rcv = tpa_eval_collect(ev, F->out.data());
+      if (queued != TPA_OK || rcv != TPA_OK) {
+        if (rc == TPA_OK) rc = queued != TPA_OK ? queued : rcv;
+        continue;
+      }
       acc += F->out[0];
       if (grad)
         for (int j = 0; j < 3 * P; ++j) G[j] += cplx(F->out[2 + 2 * j], F->out[3 + 2 * j]);
     }
+  }
+  if (rc != TPA_OK) {
+    (void)hipStreamSynchronize((hipStream_t)ms);
+    (void)hipStreamSynchronize(F->mc_stream);
+    return rc;
   }
   *nll = -sd + F->sw * std::log(smc);
   if (!grad) return TPA_OK;

@@ -23,3 +23,13 @@ int tpa_pool_eval_finish(tpa_pool_t p, int slot, double* out, double* ms);
 // chains on the device; the event sets' partial-wave caches must be rebuilt afterwards.
 extern "C" int tpa_model_set_barrier_flags(tpa_model_t model, int n_chain,
                                            const int32_t* act1, const int32_t* act2);
+
+// Queue one shape-mode evaluation pass (TPA_FLAG_SHAPE, no f_out) on `stream` (a hipStream_t;
+// null: the model's stream) without waiting; tpa_eval_collect waits for that stream and copies
+// the sums (tpa_eval's layout). Passes of different event sets may be queued on different
+// streams and overlap.
+extern "C" int tpa_eval_enqueue(tpa_events_t ev, const double* coef, int mode, int flags,
+                                void* stream);
+extern "C" int tpa_eval_collect(tpa_events_t ev, double* out);
+// The model's stream (a hipStream_t): the lineshape parameters are set on it.
+extern "C" int tpa_model_stream(tpa_model_t model, void** stream);
