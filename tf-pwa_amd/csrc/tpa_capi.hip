@@ -529,7 +529,7 @@ static int run_hess(const TpaModel& M, const double2* h, const double* w, int64_
   }
   if (rc != TPA_OK) return fail(rc, "no second-moment kernel for this helicity count");
   HIPCHK(hipGetLastError());
-  hipLaunchKernelGGL(k_reduce, dim3(nacc), dim3(256), 0, s, *d_part, grid, nacc, *d_out,
+  hipLaunchKernelGGL(k_reduce, dim3(nacc), dim3(256), 0, s, *d_part, grid, nacc, 0, *d_out,
                      nullptr, nullptr, nullptr, (uint64_t)0);
   HIPCHK(hipGetLastError());
   if (vec) {
@@ -1204,7 +1204,7 @@ static int eval_impl(tpa_events_s* ev, const double* coef, int mode, int flags, 
     *(volatile uint64_t*)ev->h_flag = 0;
   }
   const uint64_t seq = poll ? ++ev->seq : 0;
-  hipLaunchKernelGGL(k_reduce, dim3(nout), dim3(256), 0, s, ev->d_part, grid, a.nacc,
+  hipLaunchKernelGGL(k_reduce, dim3(nout), dim3(256), 0, s, ev->d_part, grid, a.nacc, 1,
                      ev->d_out, direct_io ? h_out : nullptr, poll ? ev->d_cnt : nullptr,
                      poll ? (volatile uint64_t*)ev->h_flag : nullptr, seq);
   HIPCHK(hipGetLastError());

@@ -491,16 +491,18 @@ __global__ __launch_bounds__(256) void k_eval_fold(EvalArgs a) {
     sh[wave][1] = vc;
   }
   __syncthreads();
-  double* out = a.partials + (int64_t)blockIdx.x * a.nacc;
+  // slot-major partials: slot j of this block at out[j * gridDim.x] (k_reduce reads each
+  // slot's blocks contiguously)
+  double* out = a.partials + blockIdx.x;
   if (threadIdx.x == 0) {
     double s = sh[0][0], c = sh[0][1];
     for (int wv = 1; wv < 4; ++wv) neumaier_merge(s, c, sh[wv][0], sh[wv][1]);
     out[0] = s;
-    out[1] = c;
+    out[gridDim.x] = c;
   }
   if (a.grad) {
     for (int j = 2 + threadIdx.x; j < a.nacc; j += blockDim.x)
-      out[j] = ((sh[0][j] + sh[1][j]) + sh[2][j]) + sh[3][j];
+      out[(int64_t)j * gridDim.x] = ((sh[0][j] + sh[1][j]) + sh[2][j]) + sh[3][j];
   }
 }
 
@@ -686,16 +688,18 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE))) void
     sh[wave][1] = vc;
   }
   __syncthreads();
-  double* out = a.partials + (int64_t)blockIdx.x * a.nacc;
+  // slot-major partials: slot j of this block at out[j * gridDim.x] (k_reduce reads each
+  // slot's blocks contiguously)
+  double* out = a.partials + blockIdx.x;
   if (threadIdx.x == 0) {
     double s = sh[0][0], c = sh[0][1];
     for (int wv = 1; wv < 4; ++wv) neumaier_merge(s, c, sh[wv][0], sh[wv][1]);
     out[0] = s;
-    out[1] = c;
+    out[gridDim.x] = c;
   }
   if (a.grad) {
     for (int j = 2 + threadIdx.x; j < a.nacc; j += blockDim.x)
-      out[j] = ((sh[0][j] + sh[1][j]) + sh[2][j]) + sh[3][j];
+      out[(int64_t)j * gridDim.x] = ((sh[0][j] + sh[1][j]) + sh[2][j]) + sh[3][j];
   }
 }
 
@@ -856,16 +860,18 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(PML <= 6 ? 
     sh[wave][1] = vc;
   }
   __syncthreads();
-  double* out = a.partials + (int64_t)blockIdx.x * a.nacc;
+  // slot-major partials: slot j of this block at out[j * gridDim.x] (k_reduce reads each
+  // slot's blocks contiguously)
+  double* out = a.partials + blockIdx.x;
   if (threadIdx.x == 0) {
     double s = sh[0][0], c = sh[0][1];
     for (int wv = 1; wv < 4; ++wv) neumaier_merge(s, c, sh[wv][0], sh[wv][1]);
     out[0] = s;
-    out[1] = c;
+    out[gridDim.x] = c;
   }
   if (a.grad) {
     for (int j = 2 + threadIdx.x; j < a.nacc; j += blockDim.x)
-      out[j] = ((sh[0][j] + sh[1][j]) + sh[2][j]) + sh[3][j];
+      out[(int64_t)j * gridDim.x] = ((sh[0][j] + sh[1][j]) + sh[2][j]) + sh[3][j];
   }
 }
 
@@ -879,7 +885,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(PML <= 6 ? 
 // host_out in one coalesced write and then writes seq to *flag, so the host can poll for the
 // result instead of waiting on the stream.
 __global__ __launch_bounds__(256) void k_reduce(const double* __restrict__ partials, int nblocks,
-                                                int nacc, double* __restrict__ out,
+                                                int nacc, int slot_major,
+                                                double* __restrict__ out,
                                                 double* __restrict__ host_out,
                                                 unsigned* __restrict__ cnt,
                                                 volatile uint64_t* __restrict__ flag,
@@ -889,10 +896,13 @@ __global__ __launch_bounds__(256) void k_reduce(const double* __restrict__ parti
   __shared__ double ss[256], cc[256];
   __shared__ int last;
   double s = 0.0, c = 0.0;
+  // partial of block b, slot q: partials[q * nblocks + b] (slot-major, contiguous per slot) or
+  // partials[b * nacc + q]
+  const int64_t bs = slot_major ? 1 : nacc, qs = slot_major ? nblocks : 1;
   if (j != 1) {
-    for (int b = t; b < nblocks; b += 256) neumaier_add(s, c, partials[(int64_t)b * nacc + j]);
+    for (int b = t; b < nblocks; b += 256) neumaier_add(s, c, partials[b * bs + j * qs]);
     if (j == 0)
-      for (int b = t; b < nblocks; b += 256) neumaier_add(s, c, partials[(int64_t)b * nacc + 1]);
+      for (int b = t; b < nblocks; b += 256) neumaier_add(s, c, partials[b * bs + qs]);
   }
   ss[t] = s;
   cc[t] = c;
@@ -1245,16 +1255,18 @@ __global__ __launch_bounds__(256) void k_eval_shape(EvalShapeArgs s) {
     sh[wave][1] = vc;
   }
   __syncthreads();
-  double* out = a.partials + (int64_t)blockIdx.x * a.nacc;
+  // slot-major partials: slot j of this block at out[j * gridDim.x] (k_reduce reads each
+  // slot's blocks contiguously)
+  double* out = a.partials + blockIdx.x;
   if (threadIdx.x == 0) {
     double s0 = sh[0][0], c0 = sh[0][1];
     for (int wv = 1; wv < 4; ++wv) neumaier_merge(s0, c0, sh[wv][0], sh[wv][1]);
     out[0] = s0;
-    out[1] = c0;
+    out[gridDim.x] = c0;
   }
   if (a.grad) {
     for (int j = 2 + threadIdx.x; j < a.nacc; j += blockDim.x)
-      out[j] = ((sh[0][j] + sh[1][j]) + sh[2][j]) + sh[3][j];
+      out[(int64_t)j * gridDim.x] = ((sh[0][j] + sh[1][j]) + sh[2][j]) + sh[3][j];
   }
 }
 
