@@ -362,7 +362,8 @@ int tpa_fcn_set_shape_map(tpa_fcn_t fcn, tpa_model_t model, const TpaShapeMap* m
 
 /* data_f32 != 0: the session's data passes read the FP32 copies of the data caches
  * (TPA_FLAG_F32; the copies are kept from here on, tpa_events_keep_f32). Fixed lineshapes
- * only. The MC side is unchanged (FP64 passes or the Gram matrix). */
+ * only. The MC side is unchanged (FP64 passes or the Gram matrix). No reference counterpart:
+ * the storage option of the data pass behind FCN.nll_grad (tf_pwa/model/model.py:1207-1229). */
 int tpa_fcn_set_data_storage(tpa_fcn_t fcn, int data_f32);
 
 /* Cache rebuilds the session made after barrier clamp flags changed (no reference
