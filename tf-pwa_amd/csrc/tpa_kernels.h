@@ -1491,23 +1491,39 @@ __global__ __launch_bounds__(256) void k_pw_rows_to_f32(const double2* __restric
 }
 
 // Per request: the compensated, fixed-order sum of its chunks (item_chunk[i] .. item_chunk[i+1]).
+// R = blockDim.x / nacc threads share a slot: thread (slot j, replica r) sums chunks r, r + R, ..
+// of the request in order (the loads of one thread are independent, so several stay in flight),
+// and the R partial sums are merged in r order. R depends only on nacc, so a request's result
+// does not depend on the batch around it.
 __global__ __launch_bounds__(256) void k_reduce_items(const double* __restrict__ partials,
                                                       const int* __restrict__ item_chunk,
                                                       const int* __restrict__ counts, int nacc,
                                                       double* __restrict__ out) {
   const int item = blockIdx.x;
-  if (item >= counts[1]) return;
+  if (item >= counts[1]) return;  // uniform per block, before any barrier
   const int c0 = item_chunk[item], c1 = item_chunk[item + 1];
-  for (int j = threadIdx.x; j < nacc; j += blockDim.x) {
-    if (j == 1) {
-      out[(int64_t)item * nacc + 1] = 0.0;
-      continue;
-    }
+  const int R = nacc <= (int)blockDim.x ? (int)blockDim.x / nacc : 1;
+  __shared__ double ss[TPA_NACC_MAX > 256 ? TPA_NACC_MAX : 256];
+  __shared__ double sc[TPA_NACC_MAX > 256 ? TPA_NACC_MAX : 256];
+  for (int x = threadIdx.x; x < R * nacc; x += blockDim.x) {
+    const int j = x % nacc, r = x / nacc;
     double s = 0.0, c = 0.0;
-    for (int b = c0; b < c1; ++b) neumaier_add(s, c, partials[(int64_t)b * nacc + j]);
-    if (j == 0)
-      for (int b = c0; b < c1; ++b) neumaier_add(s, c, partials[(int64_t)b * nacc + 1]);
-    out[(int64_t)item * nacc + j] = s + c;
+    if (j != 1) {
+#pragma unroll 4
+      for (int b = c0 + r; b < c1; b += R) neumaier_add(s, c, partials[(int64_t)b * nacc + j]);
+      if (j == 0) {
+#pragma unroll 4
+        for (int b = c0 + r; b < c1; b += R) neumaier_add(s, c, partials[(int64_t)b * nacc + 1]);
+      }
+    }
+    ss[x] = s;
+    sc[x] = c;
+  }
+  __syncthreads();
+  for (int j = threadIdx.x; j < nacc; j += blockDim.x) {
+    double s = ss[j], c = sc[j];
+    for (int r = 1; r < R; ++r) neumaier_merge(s, c, ss[r * nacc + j], sc[r * nacc + j]);
+    out[(int64_t)item * nacc + j] = (j == 1) ? 0.0 : s + c;
   }
 }
 
