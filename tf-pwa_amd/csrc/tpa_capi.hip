@@ -413,20 +413,10 @@ struct tpa_pool_s {
   // Batched evaluations run through TPA_POOL_SLOTS independent sets of work lists, partials
   // and results, so one batch can be queued on the stream while the previous one runs.
   struct Slot {
-    int* d_chunk_item = nullptr;
-    int64_t* d_chunk_start = nullptr;
-    int64_t* d_chunk_end = nullptr;
-    int* d_counts = nullptr;
-    int* d_item_chunk = nullptr;
-    double2* d_coef = nullptr;
+    int64_t* d_blob = nullptr;  // the batch's work list and coefficients (BatchView)
     double* d_part = nullptr;
     double* d_out = nullptr;
-    int* h_chunk_item = nullptr;
-    int64_t* h_chunk_start = nullptr;
-    int64_t* h_chunk_end = nullptr;
-    int* h_counts = nullptr;
-    int* h_item_chunk = nullptr;
-    double2* h_coef = nullptr;
+    int64_t* h_blob = nullptr;  // its pinned mirror, copied in one piece per batch
     double* h_out = nullptr;
     std::map<int, hipGraphExec_t> graphs;  // grid bucket -> instantiated graph
     hipEvent_t t0 = nullptr, t1 = nullptr, done = nullptr;
@@ -1422,12 +1412,10 @@ int tpa_pool_destroy(tpa_pool_t p) {
   if (p->stream) (void)hipStreamSynchronize(p->stream);
   for (auto& q : p->slot) {
     for (auto& g : q.graphs) (void)hipGraphExecDestroy(g.second);
-    void* dev[] = {q.d_chunk_item, q.d_chunk_start, q.d_chunk_end, q.d_counts, q.d_item_chunk,
-                   q.d_coef, q.d_part, q.d_out};
+    void* dev[] = {q.d_blob, q.d_part, q.d_out};
     for (void* x : dev)
       if (x) (void)hipFree(x);
-    void* host[] = {q.h_chunk_item, q.h_chunk_start, q.h_chunk_end, q.h_counts, q.h_item_chunk,
-                    q.h_coef, q.h_out};
+    void* host[] = {q.h_blob, q.h_out};
     for (void* x : host)
       if (x) (void)hipHostFree(x);
     hipEvent_t evs[] = {q.t0, q.t1, q.done};
@@ -1470,21 +1458,14 @@ int tpa_pool_create(tpa_model_t m, int64_t capacity, int max_items, tpa_pool_t* 
   D((void**)&p->d_ang, sizeof(double) * nf * capacity);
   D((void**)&p->d_h, sizeof(double2) * (size_t)M.P * capacity * M.lam_pad);
   D((void**)&p->d_w, sizeof(double) * capacity);
+  // blob words: header 2, three chunk arrays, item_chunk m + 1, one alignment word, 2 P per request
+  const size_t blob_words = 2 + 3 * (size_t)p->max_chunks + (size_t)max_items + 2 +
+                            2 * (size_t)max_items * M.P;
   for (auto& q : p->slot) {
-    D((void**)&q.d_chunk_item, sizeof(int) * p->max_chunks);
-    D((void**)&q.d_chunk_start, sizeof(int64_t) * p->max_chunks);
-    D((void**)&q.d_chunk_end, sizeof(int64_t) * p->max_chunks);
-    D((void**)&q.d_counts, sizeof(int) * 2);
-    D((void**)&q.d_item_chunk, sizeof(int) * (max_items + 1));
-    D((void**)&q.d_coef, sizeof(double2) * (size_t)max_items * M.P);
+    D((void**)&q.d_blob, sizeof(int64_t) * blob_words);
     D((void**)&q.d_part, sizeof(double) * (size_t)p->max_chunks * nacc);
     D((void**)&q.d_out, sizeof(double) * (size_t)max_items * nacc);
-    H((void**)&q.h_chunk_item, sizeof(int) * p->max_chunks);
-    H((void**)&q.h_chunk_start, sizeof(int64_t) * p->max_chunks);
-    H((void**)&q.h_chunk_end, sizeof(int64_t) * p->max_chunks);
-    H((void**)&q.h_counts, sizeof(int) * 2);
-    H((void**)&q.h_item_chunk, sizeof(int) * (max_items + 1));
-    H((void**)&q.h_coef, sizeof(double2) * (size_t)max_items * M.P);
+    H((void**)&q.h_blob, sizeof(int64_t) * blob_words);
     H((void**)&q.h_out, sizeof(double) * (size_t)max_items * nacc);
     if (e == hipSuccess) e = hipEventCreate(&q.t0);
     if (e == hipSuccess) e = hipEventCreate(&q.t1);
@@ -1646,7 +1627,7 @@ int tpa_pool_eval_submit(tpa_pool_t p, int* slot_io, int n_items, const int64_t*
   if (q.busy) return fail(TPA_ERR_ARG, "batch slot still in flight");
   const TpaModel& M = p->m->h;
   const int nacc = 2 + 2 * M.P;
-  int nch = 0;
+  int64_t nch = 0;
   for (int i = 0; i < n_items; ++i) {
     const int64_t off = ranges[2 * i], n = ranges[2 * i + 1];
     auto it = p->used_ranges.upper_bound(off);
@@ -1654,26 +1635,34 @@ int tpa_pool_eval_submit(tpa_pool_t p, int* slot_io, int n_items, const int64_t*
     --it;
     if (off < it->first || off + n > it->first + it->second)
       return fail(TPA_ERR_ARG, "range outside an allocation");
-    q.h_item_chunk[i] = nch;
-    for (int64_t st = off; st < off + n; st += TPA_CHUNK) {
-      if (nch >= p->max_chunks) return fail(TPA_ERR_ARG, "too many chunks in one batch");
-      q.h_chunk_item[nch] = i;
-      q.h_chunk_start[nch] = st;
-      q.h_chunk_end[nch] = std::min<int64_t>(st + TPA_CHUNK, off + n);
-      ++nch;
+    nch += (n + TPA_CHUNK - 1) / TPA_CHUNK;
+  }
+  if (nch > p->max_chunks) return fail(TPA_ERR_ARG, "too many chunks in one batch");
+  // the blob (BatchView's layout): header, chunk starts, ends and requests, each request's
+  // first chunk, then the coefficients at an even word
+  int64_t* hb = q.h_blob;
+  int64_t* cs = hb + 2;
+  int64_t* ce = cs + nch;
+  int64_t* ci = ce + nch;
+  int64_t* ic = ci + nch;
+  hb[0] = nch;
+  hb[1] = n_items;
+  int64_t c = 0;
+  for (int i = 0; i < n_items; ++i) {
+    const int64_t off = ranges[2 * i], n = ranges[2 * i + 1];
+    ic[i] = c;
+    for (int64_t st = off; st < off + n; st += TPA_CHUNK, ++c) {
+      cs[c] = st;
+      ce[c] = std::min<int64_t>(st + TPA_CHUNK, off + n);
+      ci[c] = i;
     }
   }
-  q.h_item_chunk[n_items] = nch;
-  q.h_counts[0] = nch;
-  q.h_counts[1] = n_items;
-  std::memcpy(q.h_coef, coefs, sizeof(double2) * (size_t)n_items * M.P);
+  ic[n_items] = c;
+  const int64_t coef_word = (2 + 3 * nch + n_items + 1 + 1) & ~int64_t(1);
+  std::memcpy(hb + coef_word, coefs, sizeof(double2) * (size_t)n_items * M.P);
+  const size_t blob_bytes = sizeof(int64_t) * (size_t)(coef_word + 2 * (int64_t)n_items * M.P);
   hipStream_t s = p->stream;
-  HIPCHK(hipMemcpyAsync(q.d_chunk_item, q.h_chunk_item, sizeof(int) * nch, hipMemcpyHostToDevice, s));
-  HIPCHK(hipMemcpyAsync(q.d_chunk_start, q.h_chunk_start, sizeof(int64_t) * nch, hipMemcpyHostToDevice, s));
-  HIPCHK(hipMemcpyAsync(q.d_chunk_end, q.h_chunk_end, sizeof(int64_t) * nch, hipMemcpyHostToDevice, s));
-  HIPCHK(hipMemcpyAsync(q.d_item_chunk, q.h_item_chunk, sizeof(int) * (n_items + 1), hipMemcpyHostToDevice, s));
-  HIPCHK(hipMemcpyAsync(q.d_counts, q.h_counts, sizeof(int) * 2, hipMemcpyHostToDevice, s));
-  HIPCHK(hipMemcpyAsync(q.d_coef, q.h_coef, sizeof(double2) * (size_t)n_items * M.P, hipMemcpyHostToDevice, s));
+  HIPCHK(hipMemcpyAsync(q.d_blob, q.h_blob, blob_bytes, hipMemcpyHostToDevice, s));
   int bucket = 64;
   while (bucket < nch) bucket <<= 1;
   auto g = q.graphs.find(bucket);
@@ -1682,11 +1671,7 @@ int tpa_pool_eval_submit(tpa_pool_t p, int* slot_io, int n_items, const int64_t*
     a.h = p->d_h;
     a.h32 = p->d_h32;
     a.w = p->d_w;
-    a.coef = q.d_coef;
-    a.chunk_item = q.d_chunk_item;
-    a.chunk_start = q.d_chunk_start;
-    a.chunk_end = q.d_chunk_end;
-    a.counts = q.d_counts;
+    a.blob = q.d_blob;
     a.partials = q.d_part;
     a.S = p->cap;
     a.P = M.P;
@@ -1697,8 +1682,8 @@ int tpa_pool_eval_submit(tpa_pool_t p, int* slot_io, int n_items, const int64_t*
     hipGraph_t graph;
     HIPCHK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
     int rc = dispatch_multi(M.lam_pad, kg, a, bucket, s);
-    hipLaunchKernelGGL(k_reduce_items, dim3(p->max_items), dim3(256), 0, s, q.d_part,
-                       q.d_item_chunk, q.d_counts, nacc, q.d_out);
+    hipLaunchKernelGGL(k_reduce_items, dim3(p->max_items), dim3(256), 0, s, q.d_part, q.d_blob,
+                       nacc, q.d_out);
     hipError_t ce = hipStreamEndCapture(s, &graph);
     if (rc != TPA_OK) return fail(rc, "no batched kernel for this helicity count");
     if (ce != hipSuccess) return fail(TPA_ERR_HIP, std::string("capture: ") + hipGetErrorString(ce));

@@ -1278,15 +1278,35 @@ __global__ __launch_bounds__(256) void k_eval_shape(EvalShapeArgs s) {
 // its grid size.
 #define TPA_CHUNK 2048
 
+// A batch's work list, one blob (one copy to the device per batch), in int64 words:
+// [0] chunks n, [1] requests m, then chunk_start[n], chunk_end[n], chunk_item[n],
+// item_chunk[m + 1] (each request's first chunk, and the end), then at the next even word the
+// coefficients [m][P] (double2).
+struct BatchView {
+  int64_t n, m;
+  const int64_t* start;
+  const int64_t* end;
+  const int64_t* item;
+  const int64_t* item_chunk;
+  const double2* coef;
+};
+TPA_DEV BatchView batch_view(const int64_t* __restrict__ blob) {
+  BatchView v;
+  v.n = blob[0];
+  v.m = blob[1];
+  v.start = blob + 2;
+  v.end = v.start + v.n;
+  v.item = v.end + v.n;
+  v.item_chunk = v.item + v.n;
+  v.coef = reinterpret_cast<const double2*>(blob + ((2 + 3 * v.n + v.m + 1 + 1) & ~int64_t(1)));
+  return v;
+}
+
 struct MultiArgs {
   const double2* __restrict__ h;     // pool cache [P][S][LG]
   const float2* __restrict__ h32;    // its FP32 copy [P][S][LG] (k_eval_multi32), or null
   const double* __restrict__ w;      // pool weights [S]
-  const double2* __restrict__ coef;  // [n_items][P]
-  const int* __restrict__ chunk_item;
-  const int64_t* __restrict__ chunk_start;
-  const int64_t* __restrict__ chunk_end;
-  const int* __restrict__ counts;    // [0] = chunks, [1] = items
+  const int64_t* __restrict__ blob;  // the batch (BatchView)
   double* __restrict__ partials;     // [chunk][nacc]
   int64_t S;
   int P;
@@ -1297,7 +1317,8 @@ struct MultiArgs {
 template <int LG, int KG, int PML>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(PML <= 6 ? 3 : 1))) void k_eval_multi(MultiArgs a) {
   const int chunk = blockIdx.x;
-  if (chunk >= a.counts[0]) return;  // uniform per block, before any barrier
+  const BatchView bv = batch_view(a.blob);
+  if (chunk >= bv.n) return;  // uniform per block, before any barrier
   constexpr int LPE = LG * KG;
   constexpr int EPW = 64 / LPE;
   const int lane = threadIdx.x & 63;
@@ -1306,9 +1327,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(PML <= 6 ? 
   const int kg = (lane / LG) & (KG - 1);
   const int el = lane / LPE;
   const bool leader = lam == 0 && kg == 0;
-  const int item = a.chunk_item[chunk];
-  const int64_t end = a.chunk_end[chunk];  // > chunk_start: chunks are never empty
-  const double2* __restrict__ coef = a.coef + (int64_t)item * a.P;
+  const int64_t item = bv.item[chunk];
+  const int64_t end = bv.end[chunk];  // > start: chunks are never empty
+  const double2* __restrict__ coef = bv.coef + item * a.P;
   double vs = 0.0, vc = 0.0;
   double gr[PML], gi[PML], cr[PML], ci[PML];
   const double2* base[PML];
@@ -1335,7 +1356,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(PML <= 6 ? 
   double2 h0[PML], h1[PML];
   double w0, w1;
   const int wave_u = __builtin_amdgcn_readfirstlane(wave);
-  const int64_t first = a.chunk_start[chunk] + wave_u * EPW;
+  const int64_t first = bv.start[chunk] + wave_u * EPW;
   int64_t e = first + el;
   load(h0, w0, e);
   for (int64_t b = first; b < end; b += 2 * stride, e += 2 * stride) {
@@ -1389,7 +1410,8 @@ template <int LG, int KG, int PML>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(PML <= 6 ? 3 : 1))) void k_eval_multi32(MultiArgs a) {
   static_assert(LG >= 2, "two helicity components per lane");
   const int chunk = blockIdx.x;
-  if (chunk >= a.counts[0]) return;  // uniform per block, before any barrier
+  const BatchView bv = batch_view(a.blob);
+  if (chunk >= bv.n) return;  // uniform per block, before any barrier
   constexpr int LGL = LG / 2;
   constexpr int LPE = LGL * KG;
   constexpr int EPW = 64 / LPE;
@@ -1399,9 +1421,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(PML <= 6 ? 
   const int kg = (lane / LGL) & (KG - 1);
   const int el = lane / LPE;
   const bool leader = lamg == 0 && kg == 0;
-  const int item = a.chunk_item[chunk];
-  const int64_t end = a.chunk_end[chunk];  // > chunk_start: chunks are never empty
-  const double2* __restrict__ coef = a.coef + (int64_t)item * a.P;
+  const int64_t item = bv.item[chunk];
+  const int64_t end = bv.end[chunk];  // > start: chunks are never empty
+  const double2* __restrict__ coef = bv.coef + item * a.P;
   double vs = 0.0, vc = 0.0;
   double gr[PML], gi[PML], cr[PML], ci[PML];
   const float4* base[PML];
@@ -1426,7 +1448,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(PML <= 6 ? 
   float4 h0[PML], h1[PML];
   double w0, w1;
   const int wave_u = __builtin_amdgcn_readfirstlane(wave);
-  const int64_t first = a.chunk_start[chunk] + wave_u * EPW;
+  const int64_t first = bv.start[chunk] + wave_u * EPW;
   int64_t e = first + el;
   load(h0, w0, e);
   for (int64_t b = first; b < end; b += 2 * stride, e += 2 * stride) {
@@ -1496,12 +1518,12 @@ __global__ __launch_bounds__(256) void k_pw_rows_to_f32(const double2* __restric
 // and the R partial sums are merged in r order. R depends only on nacc, so a request's result
 // does not depend on the batch around it.
 __global__ __launch_bounds__(256) void k_reduce_items(const double* __restrict__ partials,
-                                                      const int* __restrict__ item_chunk,
-                                                      const int* __restrict__ counts, int nacc,
-                                                      double* __restrict__ out) {
+                                                      const int64_t* __restrict__ blob,
+                                                      int nacc, double* __restrict__ out) {
   const int item = blockIdx.x;
-  if (item >= counts[1]) return;  // uniform per block, before any barrier
-  const int c0 = item_chunk[item], c1 = item_chunk[item + 1];
+  const BatchView bv = batch_view(blob);
+  if (item >= bv.m) return;  // uniform per block, before any barrier
+  const int64_t c0 = bv.item_chunk[item], c1 = bv.item_chunk[item + 1];
   const int R = nacc <= (int)blockDim.x ? (int)blockDim.x / nacc : 1;
   __shared__ double ss[TPA_NACC_MAX > 256 ? TPA_NACC_MAX : 256];
   __shared__ double sc[TPA_NACC_MAX > 256 ? TPA_NACC_MAX : 256];
@@ -1510,10 +1532,10 @@ __global__ __launch_bounds__(256) void k_reduce_items(const double* __restrict__
     double s = 0.0, c = 0.0;
     if (j != 1) {
 #pragma unroll 4
-      for (int b = c0 + r; b < c1; b += R) neumaier_add(s, c, partials[(int64_t)b * nacc + j]);
+      for (int64_t b = c0 + r; b < c1; b += R) neumaier_add(s, c, partials[b * nacc + j]);
       if (j == 0) {
 #pragma unroll 4
-        for (int b = c0 + r; b < c1; b += R) neumaier_add(s, c, partials[(int64_t)b * nacc + 1]);
+        for (int64_t b = c0 + r; b < c1; b += R) neumaier_add(s, c, partials[b * nacc + 1]);
       }
     }
     ss[x] = s;
