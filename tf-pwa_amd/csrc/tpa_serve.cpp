@@ -259,12 +259,19 @@ void complete(tpa_server_s* S, Batch& B, std::unique_lock<std::mutex>& lk) {
 // batches and a request does not wait for the batch ahead to return before it is launched.
 // A batch launches when every fit outside the running batch is waiting, when max_batch
 // requests are pending, or window_us after the first of them.
+// While a batch runs the batcher checks it between yields rather than sleeping on a timed
+// wait: a timed wait oversleeps by the kernel's timer slack (~50 us by default), which would
+// land on every request's latency. The spinning costs one host core while the GPU works.
 void batcher_main(tpa_server_s* S) {
   const int P = S->P;
   Batch bt[2];
   int run = -1;  // index in bt of the batch on the GPU (-1: none)
-  const auto poll = std::chrono::microseconds(20);
   std::unique_lock<std::mutex> lk(S->mu);
+  auto nap = [&lk]() {
+    lk.unlock();
+    std::this_thread::yield();
+    lk.lock();
+  };
   for (;;) {
     // wait for requests; meanwhile deliver the running batch as soon as it is done
     while (!S->stop && S->pending.empty()) {
@@ -274,7 +281,7 @@ void batcher_main(tpa_server_s* S) {
           run = -1;
           continue;
         }
-        S->cv.wait_for(lk, poll);
+        nap();
       } else {
         S->cv.wait(lk);
       }
@@ -296,9 +303,10 @@ void batcher_main(tpa_server_s* S) {
       }
       const Clock::time_point now = Clock::now();
       if (now >= deadline) break;
-      const Clock::time_point until =
-          run >= 0 ? std::min<Clock::time_point>(deadline, now + poll) : deadline;
-      S->cv.wait_until(lk, until);
+      if (run >= 0)
+        nap();
+      else
+        S->cv.wait_until(lk, deadline);
     }
     if (S->stop) break;
     const int nb = run == 0 ? 1 : 0;
