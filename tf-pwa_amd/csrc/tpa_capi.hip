@@ -1116,8 +1116,10 @@ static int eval_impl(tpa_events_s* ev, const double* coef, int mode, int flags, 
     const char* v = std::getenv("TFPWA_EVAL_IO");
     return !(v && std::atoi(v) == 0);
   }();
-  const bool direct_io = io_env && !shape;
-  const bool args_coef = direct_io && M.P <= TPA_EVAL_CIN && eval_unroll() == 0;
+  // (a pass that also returns f waits on the stream for that copy anyway: its sums take the
+  // device-memory route, one copy, rather than a system-scope fence in every k_reduce block)
+  const bool direct_io = io_env && !shape && !want_f;
+  const bool args_coef = io_env && !shape && M.P <= TPA_EVAL_CIN && eval_unroll() == 0;
   if (!args_coef) {
     std::memcpy(h_coef, coef, sizeof(double2) * M.P);
     HIPCHK(hipMemcpyAsync(ev->d_coef, h_coef, sizeof(double2) * M.P, hipMemcpyHostToDevice, s));
