@@ -678,6 +678,49 @@ def test_native_session_rebuilds_on_clamp_flip(gpu_available):
     assert fn._native._seen_rebuilds == 2
 
 
+def test_torch_autograd_gradient_matches_central_differences(gpu_available):
+    """The NLL as a torch operation (tf_pwa_amd.autograd.nll): its backward pass, the engine's
+    adjoint for couplings, masses and widths (46 parameters of the six-resonance model), equals
+    central differences of the NLL with h = 1e-6 relative, and composes with a torch term."""
+    import torch
+    import yaml
+
+    from conftest import config_path
+    from tf_pwa_amd import FCN, ConfigLoader
+    from tf_pwa_amd.autograd import nll
+    from tf_pwa_amd.phasespace import generate_phsp
+
+    c = yaml.safe_load(open(config_path("lc_pkpi_6res.yml")))
+    for name, p in c["particle"].items():
+        if not name.startswith("$") and isinstance(p, dict) and "width" in p:
+            p["float"] = "mg"
+    conf = ConfigLoader(c, seed=6)
+    order = conf.dat_order
+    masses = [conf.particles[n].mass for n in order]
+    top = conf.particles[conf.top_name].mass
+    rng = np.random.default_rng(6)
+    data = conf.data_from_p4(dict(zip(order, generate_phsp(top, masses, 3000, rng))),
+                             weight=rng.uniform(0.5, 1.5, 3000))
+    mc = conf.data_from_p4(dict(zip(order, generate_phsp(top, masses, 20000, rng))))
+    fcn = FCN(conf.get_model(), data, mc)
+    x0 = np.array(conf.vm.get_all_val())
+    assert len(x0) == 46
+    x = torch.tensor(x0, requires_grad=True)
+    y = nll(fcn, x)
+    (y + 0.5 * (x * x).sum()).backward()
+    g = x.grad.numpy() - x0
+    fd = np.zeros_like(x0)
+    for i in range(len(x0)):
+        h = 1e-6 * max(abs(x0[i]), 1e-3)
+        xp, xm = x0.copy(), x0.copy()
+        xp[i] += h
+        xm[i] -= h
+        fd[i] = (fcn(xp) - fcn(xm)) / (2 * h)
+    conf.vm.set_all(x0)
+    err = np.abs(g - fd)
+    assert err.max() <= 1e-5 * np.abs(g).max() + 1e-5, (err.max(), np.abs(g).max())
+
+
 def test_native_fit_matches_scipy_fit(case1, gpu_available):
     """fit_native (the engine's BFGS in tpa_fcn_minimize) and fit_scipy reach the same minimum
     from the same start (couplings of the 2-resonance model, 800 weighted data events)."""
