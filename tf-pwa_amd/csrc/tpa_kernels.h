@@ -560,11 +560,15 @@ TPA_DEV void eval_group(int mode, bool grad, double* __restrict__ f_out,
   double2 hv[PML];
 #pragma unroll
   for (int j = 0; j < PML; ++j) hv[j] = make_double2(hs[j].x, hs[j].y);
+  // complex multiply-accumulates as chained FMAs into the accumulators (two per real part
+  // instead of a multiply, an FMA and an add)
   double ar = 0.0, ai = 0.0;
 #pragma unroll
   for (int j = 0; j < PML; ++j) {
-    ar += cr[j] * hv[j].x - ci[j] * hv[j].y;
-    ai += cr[j] * hv[j].y + ci[j] * hv[j].x;
+    ar = fma(cr[j], hv[j].x, ar);
+    ar = fma(-ci[j], hv[j].y, ar);
+    ai = fma(cr[j], hv[j].y, ai);
+    ai = fma(ci[j], hv[j].x, ai);
   }
   ar = butterfly<LG, LPE>(ar);
   ai = butterfly<LG, LPE>(ai);
@@ -596,8 +600,10 @@ TPA_DEV void eval_group(int mode, bool grad, double* __restrict__ f_out,
     const double tr = u * ar, ti = -(u * ai);
 #pragma unroll
     for (int j = 0; j < PML; ++j) {
-      gr[j] += tr * hv[j].x - ti * hv[j].y;
-      gi[j] += tr * hv[j].y + ti * hv[j].x;
+      gr[j] = fma(tr, hv[j].x, gr[j]);
+      gr[j] = fma(-ti, hv[j].y, gr[j]);
+      gi[j] = fma(tr, hv[j].y, gi[j]);
+      gi[j] = fma(ti, hv[j].x, gi[j]);
     }
   }
 }
@@ -724,10 +730,14 @@ TPA_DEV void eval_group2(int mode, bool grad, double* __restrict__ f_out,
 #pragma unroll
   for (int j = 0; j < PML; ++j) {
     const double x0 = hs[j].x, y0 = hs[j].y, x1 = hs[j].z, y1 = hs[j].w;
-    ar0 += cr[j] * x0 - ci[j] * y0;
-    ai0 += cr[j] * y0 + ci[j] * x0;
-    ar1 += cr[j] * x1 - ci[j] * y1;
-    ai1 += cr[j] * y1 + ci[j] * x1;
+    ar0 = fma(cr[j], x0, ar0);
+    ar0 = fma(-ci[j], y0, ar0);
+    ai0 = fma(cr[j], y0, ai0);
+    ai0 = fma(ci[j], x0, ai0);
+    ar1 = fma(cr[j], x1, ar1);
+    ar1 = fma(-ci[j], y1, ar1);
+    ai1 = fma(cr[j], y1, ai1);
+    ai1 = fma(ci[j], x1, ai1);
   }
   ar0 = butterfly<LGL, LPE>(ar0);
   ai0 = butterfly<LGL, LPE>(ai0);
@@ -768,8 +778,14 @@ TPA_DEV void eval_group2(int mode, bool grad, double* __restrict__ f_out,
       float4 q = hs[j];
       asm("" : "+v"(q.x), "+v"(q.y), "+v"(q.z), "+v"(q.w));
       const double x0 = q.x, y0 = q.y, x1 = q.z, y1 = q.w;
-      gr[j] += (tr0 * x0 - ti0 * y0) + (tr1 * x1 - ti1 * y1);
-      gi[j] += (tr0 * y0 + ti0 * x0) + (tr1 * y1 + ti1 * x1);
+      gr[j] = fma(tr0, x0, gr[j]);
+      gr[j] = fma(-ti0, y0, gr[j]);
+      gr[j] = fma(tr1, x1, gr[j]);
+      gr[j] = fma(-ti1, y1, gr[j]);
+      gi[j] = fma(tr0, y0, gi[j]);
+      gi[j] = fma(ti0, x0, gi[j]);
+      gi[j] = fma(tr1, y1, gi[j]);
+      gi[j] = fma(ti1, x1, gi[j]);
     }
   }
 }
