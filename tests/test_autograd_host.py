@@ -59,3 +59,17 @@ def test_dtype_and_shape_checks():
     np.testing.assert_allclose(x.grad.numpy(), f.b.astype(np.float32), rtol=1e-6)
     with pytest.raises(ValueError):
         nll(f, torch.zeros(5))
+
+
+def test_device_tensors_are_refused():
+    """Parameter vectors stay on the host: a tensor on another device is refused, not moved
+    (the engine alone puts work on the GPU of its process)."""
+    from tf_pwa_amd.autograd import hessian, nll
+
+    f = _QuadraticFCN(3)
+    x = torch.zeros(3, dtype=torch.float64, device="meta")
+    with pytest.raises(ValueError, match="CPU tensor"):
+        nll(f, x)
+    with pytest.raises(ValueError, match="CPU tensor"):
+        hessian(f, x)
+    assert f.calls == 0

@@ -8,6 +8,11 @@ gradient is computed with the value and held for ``backward``, which scales it b
 incoming gradient. PyTorch carries the parameter vector and the autograd graph around the
 call, so the NLL composes with other torch terms (penalties, priors) and torch optimisers.
 ``x`` is the vector of trainable values in ``fcn.vm.trainable_vars`` order.
+
+``x`` must be a CPU tensor. The parameters are host values (the engine takes them as a host
+array per call), and the engine owns the GPU work of its process: a torch tensor on the GPU
+would put torch's own kernels (copies, fills) on the device beside the engine's, so such
+tensors are refused rather than moved.
 """
 
 import numpy as np
@@ -19,11 +24,11 @@ class NLLFunction(torch.autograd.Function):
 
     @staticmethod
     def forward(ctx, x, fcn):
-        xs = x.detach().to("cpu", torch.float64).numpy()
+        xs = x.detach().to(torch.float64).numpy()
         value, grad = fcn.nll_grad(np.ascontiguousarray(xs))
         ctx.save_for_backward(torch.as_tensor(np.asarray(grad, dtype=np.float64)).to(
-            device=x.device, dtype=x.dtype))
-        return torch.tensor(float(value), dtype=x.dtype, device=x.device)
+            dtype=x.dtype))
+        return torch.tensor(float(value), dtype=x.dtype)
 
     @staticmethod
     def backward(ctx, grad_output):
@@ -31,9 +36,16 @@ class NLLFunction(torch.autograd.Function):
         return grad_output * grad, None
 
 
+def _check_host(x):
+    if x.device.type != "cpu":
+        raise ValueError("x must be a CPU tensor (got one on {}): the parameters are host "
+                         "values and the engine alone runs work on the GPU".format(x.device))
+
+
 def nll(fcn, x):
-    """NLL of ``fcn`` at the trainable values ``x`` (a 1-d tensor) as a differentiable torch
-    scalar."""
+    """NLL of ``fcn`` at the trainable values ``x`` (a 1-d CPU tensor) as a differentiable
+    torch scalar."""
+    _check_host(x)
     if x.dim() != 1 or x.numel() != len(fcn.vm.trainable_vars):
         raise ValueError("x must hold one value per trainable variable ({})".format(
             len(fcn.vm.trainable_vars)))
@@ -42,7 +54,8 @@ def nll(fcn, x):
 
 def hessian(fcn, x):
     """The exact Hessian of the NLL at ``x`` as a tensor (FCN.nll_grad_hessian: second-moment
-    passes for couplings, central differences of the analytic gradient when lineshapes float)."""
-    xs = x.detach().to("cpu", torch.float64).numpy()
+    passes for couplings and lineshape parameters)."""
+    _check_host(x)
+    xs = x.detach().to(torch.float64).numpy()
     _, _, H = fcn.nll_grad_hessian(np.ascontiguousarray(xs))
-    return torch.as_tensor(np.asarray(H, dtype=np.float64)).to(device=x.device, dtype=x.dtype)
+    return torch.as_tensor(np.asarray(H, dtype=np.float64)).to(dtype=x.dtype)
