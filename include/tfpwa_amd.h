@@ -150,6 +150,20 @@ int tpa_events_get_pw(tpa_events_t ev, double* out, int64_t out_len);
 int tpa_eval(tpa_events_t ev, const double* coef, int mode, int flags, double* out,
              double* f_out);
 
+/* tpa_eval without allocation, for callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc). No counterpart in the
+ * reference. The pass of tpa_eval over the event set's resident cache (flags: TPA_FLAG_GRAD,
+ * TPA_FLAG_F32; the cache built with fold = 1), with its per-block partial sums in a static
+ * device array of the library (1024 blocks x 66 doubles, 528 KiB on each GPU the library runs
+ * on, held until the process ends) instead of the event set's buffer. Writes tpa_eval's sums
+ * (2 doubles, or 2 + 2P with TPA_FLAG_GRAD) to d_out, which must be device memory, on `stream`
+ * (a hipStream_t; NULL: the model's stream), and returns once the work is queued: no
+ * allocation, copy or synchronisation. Results equal tpa_eval's bit for bit. Rejects P > 32
+ * and grids beyond the scratch (TFPWA_EVAL_GRID above 1024). All calls on one GPU share its
+ * scratch array: calls that overlap on one GPU race, so queue them on one stream. */
+int tpa_eval_noalloc(tpa_events_t ev, const double* coef, int mode, int flags, double* d_out,
+                     void* stream);
+
 /* Kernel timing with HIP events on the engine's stream: when enabled, every likelihood
  * kernel launch is bracketed by events; read returns the summed milliseconds and the
  * number of launches since the last reset. */

@@ -1258,6 +1258,112 @@ int tpa_eval_collect(tpa_events_t ev, double* out) {
   return TPA_OK;
 }
 
+}  // extern "C"
+
+// ---- the allocation-free pass (tpa_eval_noalloc)
+// Per-block partial sums of the pass live in this file-scope device array instead of the event
+// set's buffer: kNoAllocBlocks blocks of at most kNoAllocSlots slots (2 + 2 * 32 doubles), slot-
+// major as k_reduce reads them, 528 KiB on every GPU the library runs on, held until the process
+// ends. Every block of a pass writes each slot that k_reduce then reads, so nothing needs to
+// start at zero. In host code `scratch` names a host-side stand-in: kernels get the device
+// address that hipGetSymbolAddress returns for the current device (noalloc_scratch).
+constexpr int kNoAllocBlocks = kEvalGridDefault;
+constexpr int kNoAllocSlots = 2 + 2 * TPA_EVAL_CIN;
+static __device__ __align__(256) double scratch[kNoAllocBlocks * kNoAllocSlots];
+
+constexpr int kNoAllocMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static double* g_scratch_ptr[kNoAllocMaxDevices] = {};
+
+// Device address of `scratch` on device `dev` (current when called), resolved once per device.
+static int noalloc_scratch(int dev, double** ptr) {
+  if (dev < 0 || dev >= kNoAllocMaxDevices) return fail(TPA_ERR_UNSUPPORTED, "device index");
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      return fail(TPA_ERR_HIP, std::string("hipGetSymbolAddress(scratch): ") +
+                                   hipGetErrorString(e));
+    g_scratch_ptr[dev] = static_cast<double*>(p);
+  }
+  *ptr = g_scratch_ptr[dev];
+  return TPA_OK;
+}
+
+static int eval_noalloc_impl(tpa_events_s* ev, const double* coef, int mode, int flags,
+                             double* d_out, hipStream_t s) {
+  const TpaModel& M = ev->m->h;
+  if (mode != TPA_MODE_DATA && mode != TPA_MODE_MC) return fail(TPA_ERR_ARG, "bad mode");
+  if (flags & ~(TPA_FLAG_GRAD | TPA_FLAG_F32))
+    return fail(TPA_ERR_UNSUPPORTED, "tpa_eval_noalloc takes TPA_FLAG_GRAD and TPA_FLAG_F32 only");
+  if (M.P > TPA_EVAL_CIN)
+    return fail(TPA_ERR_UNSUPPORTED, "tpa_eval_noalloc: P = " + std::to_string(M.P) +
+                                         " partial waves exceed its maximum of " +
+                                         std::to_string(TPA_EVAL_CIN));
+  if (eval_unroll() != 0)
+    return fail(TPA_ERR_UNSUPPORTED, "tpa_eval_noalloc runs the default pass only "
+                                     "(TFPWA_EVAL_UNROLL is set)");
+  if (!ev->d_h || ev->built_fold != 1)
+    return fail(TPA_ERR_ARG, "tpa_eval_noalloc needs a folded cache (tpa_events_build fold=1)");
+  const bool f32 = (flags & TPA_FLAG_F32) != 0;
+  if (f32 && !ev->h32_valid)
+    return fail(TPA_ERR_ARG, "no FP32 copy of the cache (tpa_events_keep_f32)");
+  const int kg = eval_kgroups(M.lam_pad, M.P);
+  if (kg < 0) return fail(TPA_ERR_UNSUPPORTED, "too many partial waves for this helicity count");
+  // the same grid as tpa_eval, so both passes sum in the same order; it is at most
+  // kEvalGridDefault blocks unless TFPWA_EVAL_GRID raises it past the scratch
+  const int grid = eval_grid(ev->n, f32 ? f32_lanes_per_event(M.lam_pad, kg) : M.lam_pad * kg);
+  if (grid > kNoAllocBlocks)
+    return fail(TPA_ERR_UNSUPPORTED, "tpa_eval_noalloc: a grid of " + std::to_string(grid) +
+                                         " blocks exceeds its scratch of " +
+                                         std::to_string(kNoAllocBlocks));
+  double* part = nullptr;
+  int rc = noalloc_scratch(ev->m->device, &part);
+  if (rc != TPA_OK) return rc;
+  EvalArgs a;
+  a.h = ev->d_h;
+  a.h32 = f32 ? ev->d_h32 : nullptr;
+  a.w = ev->d_w;
+  a.coef = nullptr;  // coefficients travel in the kernel arguments
+  std::memcpy(a.cin, coef, sizeof(double2) * M.P);
+  a.partials = part;
+  a.f_out = nullptr;
+  a.N = ev->n;
+  a.P = M.P;
+  a.mode = mode;
+  a.grad = (flags & TPA_FLAG_GRAD) ? 1 : 0;
+  a.nacc = 2 + 2 * M.P;
+  rc = dispatch_eval(M.lam_pad, kg, a, grid, s);
+  if (rc != TPA_OK) return fail(rc, "no evaluation kernel for this helicity/partial-wave count");
+  HIPCHK(hipGetLastError());
+  const int nout = a.grad ? a.nacc : 2;
+  hipLaunchKernelGGL(k_reduce, dim3(nout), dim3(256), 0, s, part, grid, a.nacc, 1, d_out,
+                     nullptr, nullptr, nullptr, (uint64_t)0);
+  HIPCHK(hipGetLastError());
+  return TPA_OK;
+}
+
+extern "C" {
+
+int tpa_eval_noalloc(tpa_events_t ev, const double* coef, int mode, int flags, double* d_out,
+                     void* stream) {
+  if (!ev || !coef || !d_out) return fail(TPA_ERR_ARG, "null pointer");
+  // the scratch array of the event set's device, with that device current for the launches
+  int cur = 0;
+  HIPCHK(hipGetDevice(&cur));
+  const int dev = ev->m->device;
+  if (cur != dev) HIPCHK(hipSetDevice(dev));
+  const int rc = eval_noalloc_impl(ev, coef, mode, flags, d_out,
+                                   stream ? (hipStream_t)stream : ev->m->stream);
+  if (cur != dev) {
+    const hipError_t e = hipSetDevice(cur);
+    if (rc == TPA_OK && e != hipSuccess)
+      return fail(TPA_ERR_HIP, std::string("hipSetDevice: ") + hipGetErrorString(e));
+  }
+  return rc;
+}
+
 int tpa_model_stream(tpa_model_t m, void** stream) {
   if (!m || !stream) return fail(TPA_ERR_ARG, "null pointer");
   *stream = (void*)m->stream;

@@ -203,6 +203,7 @@ def lib():
         "tpa_pool_keep_f32": ([vp, ctypes.c_int], ctypes.c_int),
         "tpa_events_get_pw": ([vp, dp, i64], ctypes.c_int),
         "tpa_eval": ([vp, dp, ctypes.c_int, ctypes.c_int, dp, dp], ctypes.c_int),
+        "tpa_eval_noalloc": ([vp, dp, ctypes.c_int, ctypes.c_int, vp, vp], ctypes.c_int),
         "tpa_timing_enable": ([ctypes.c_int], ctypes.c_int),
         "tpa_timing_read": ([dp, ctypes.POINTER(i64)], ctypes.c_int),
         "tpa_timing_reset": ([], ctypes.c_int),
@@ -288,5 +289,5 @@ def exported_symbols():
         "tpa_server_latencies", "tpa_server_counters", "tpa_fcn_create", "tpa_fcn_destroy",
         "tpa_fcn_nll_grad", "tpa_fcn_set_shape_map", "tpa_model_set_lineshapes", "tpa_fcn_minimize",
         "tpa_fcn_cache_rebuilds", "tpa_events_keep_f32", "tpa_fcn_set_data_storage",
-        "tpa_pool_keep_f32",
+        "tpa_pool_keep_f32", "tpa_eval_noalloc",
     ]
