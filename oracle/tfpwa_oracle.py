@@ -18,13 +18,22 @@ Parity pins (what checks each part independently of this file, see ``tests/``):
                          Gamma(m0)=Gamma0, reverse-Bessel |theta_l(iw)|^2 polynomials.
   * clip_log ........... closed form; sum_hessian closed form (tests/test_model_new.py).
   * cal_angle .......... invariants: gamma=0 for helicity angles, reconstruction of the
-                         boosted momenta from (m, theta, phi), Wigner-rotation composition
-                         identities, rotation invariance of |A|^2.
-  * amplitude / NLL .... |A|^2 >= 0, rotation invariance, single-wave closed form
-                         angular distributions, finite differences of the gradient.
-Parity unpinned (source reading only, no reference vector exists without TensorFlow):
-the exact aligned-angle SU(2) branch conventions of ``SU2M.get_euler_angle`` and the alpha
-range folding of ``cal_helicity_angle``; the oracle follows the source line by line there.
+                         boosted momenta from (m, theta, phi). Aligned angles: the Wigner
+                         rotation of 4x4 Lorentz helicity frames built from the momenta is a
+                         pure rotation equal to the aligned Euler angles' SO(3) matrix; the
+                         SL(2, C) frame product in the reference's step convention equals
+                         D^1/2 of the aligned angles sign included; |A|^2 is invariant under a
+                         2 pi shift of any vertex's rotation branch and under a change of the
+                         proton's reference topology (tests/test_oracle_alignment.py).
+  * amplitude / NLL .... |A|^2 >= 0, rotation invariance, single-chain closed forms
+                         (Lc -> K*(892) p -> K pi p: a cos^2 th_K + b sin^2 th_K;
+                         Lc -> L(1520) pi -> p K pi: d^3/2 squares) with helicity couplings
+                         from sympy's CG, four-body single-chain closed forms, finite
+                         differences of the gradient.
+Parity unpinned (source reading only): the per-vertex alpha folding ranges of
+``cal_helicity_angle`` ([-pi, pi) for outs[0], [-2 pi, 0) for outs[1]), a convention restated
+from cal_angle.py:291-303 that every pin above takes as given; the pins fix everything that
+follows from it (frames, aligned angles with their SU(2) sign, the assembly).
 """
 
 import functools
@@ -638,13 +647,20 @@ class Model:
 # --------------------------------------------------------------------------------------
 # Helicity angles: cal_angle.py:138-566
 # --------------------------------------------------------------------------------------
-def cal_angle(model, p4, r_boost=True, final_rest=True, random_z=True):
+def cal_angle(model, p4, r_boost=True, final_rest=True, random_z=True, alpha_shift=None,
+              ref_topology=None):
     """p4: dict final name -> (N, 4) tensor. Returns per-topology angle data, the masses of
     every particle keyed by the frozenset of finals below it, and per-decay |q|^2.
 
     Follows cal_angle_from_momentum_single (cal_angle.py:825-865) with
     cal_angle_from_particle (408-490), cal_helicity_angle (262-343), cal_chain_boost
     (186-240) and aligned_angle_ref_rule1 (346-378).
+
+    Two switches exist for the invariance pins only (the reference has neither):
+    ``alpha_shift`` maps (topology key, core key, out key) to an angle added to that out's
+    folded alpha (a shift by 2 pi moves it to the other SU(2) branch, in the vertex D
+    function and in the frame products alike); ``ref_topology`` maps a final's name to the
+    topology key whose helicity frame is its reference instead of rule 1's choice.
     """
     p4 = {k: torch.as_tensor(v, dtype=DT) for k, v in p4.items()}
     reps = model.topologies()
@@ -721,6 +737,9 @@ def cal_angle(model, p4, r_boost=True, final_rest=True, random_z=True):
                         set_x[key(j)] = x
                         set_z[key(j)] = z2
                         a["alpha"] = torch.remainder(a["alpha"] - bias, 2 * np.pi) + bias
+                        if alpha_shift:
+                            a["alpha"] = a["alpha"] + alpha_shift.get(
+                                (c.topology_key(), kc, key(j)), 0.0)
                         bias -= np.pi
                         ang[kc][key(j)] = a
                         b_mat[key(j)] = SU2.boost_z_from_p(p_rest)
@@ -746,6 +765,8 @@ def cal_angle(model, p4, r_boost=True, final_rest=True, random_z=True):
     for f in model.finals:
         if f.name not in ref:
             ref[f.name] = reps[0].topology_key()
+    if ref_topology:
+        ref.update(ref_topology)
     aligned = {}
     for c in reps:
         tk = c.topology_key()
@@ -773,6 +794,13 @@ def cal_angle(model, p4, r_boost=True, final_rest=True, random_z=True):
 # --------------------------------------------------------------------------------------
 def _letters():
     return iter("abcdefghjklmnopqrstuvwxyz")
+
+
+def aligned_D(al, f):
+    """The aligned-angle factor D^{J_f*}_{lambda, lambda'}(alpha, beta, gamma) of a spin>0
+    final (amp/core.py:1414-1429), indexed [event][lambda in the chain's frame][lambda' in
+    the reference frame]."""
+    return get_D_matrix_lambda(al, f.J, f.spins, f.spins)
 
 
 def chain_amplitude(model, chain, angles, var):
@@ -852,7 +880,7 @@ def chain_amplitude(model, chain, angles, var):
         al = angles["aligned"][tk].get(f.name)
         if al is None:
             continue
-        dt = get_D_matrix_lambda(al, f.J, f.spins, f.spins)
+        dt = aligned_D(al, f)
         amps.append(dt)
         lo = letter[f.name]
         idxs.append(lo + lo.upper())
