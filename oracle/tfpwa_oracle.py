@@ -430,6 +430,11 @@ class Particle:
         self.width = width
         self.model = model
         self.float = kw.get("float", "")
+        # options of the default particle model (amp/core.py:366-377)
+        self.running_width = kw.get("running_width", True)
+        self.bw_l = kw.get("bw_l", None)
+        self.width_norm = kw.get("width_norm", False)
+        self.d = 3.0  # Particle.init_params (core.py:380); decay_d may replace it
         self.decays = []
 
     def __repr__(self):
@@ -599,8 +604,21 @@ class Model:
                 for i, n in enumerate(d.gls_names):
                     self._add(n + "r", 1.0 if i == 0 else 1.0, i != 0)
                     self._add(n + "i", 0.0, i != 0)
-        # add_decay_constraints (config_loader.py:358-366)
+        # add_decay_constraints (config_loader.py:358-389)
         cons = (self.config.get("constrains") or {}).get("decay") or {}
+        decay_d = cons.get("decay_d")
+        if decay_d is not None:
+            for c in self.chains:
+                if isinstance(decay_d, (int, float)):
+                    pairs = [(float(decay_d), d) for d in c.decays]
+                elif isinstance(decay_d, (list, tuple)):
+                    pairs = list(zip([float(x) for x in decay_d], c.decays))
+                else:
+                    pairs = [(float(decay_d[d.core.name]), d) for _, d in zip(decay_d, c.decays)
+                             if d.core.name in decay_d]
+                for v, d in pairs:
+                    d.d = v
+                    d.core.d = v
         idx = cons.get("fix_chain_idx", 0)
         val = cons.get("fix_chain_val", 1.0)
         name = self.chains[idx].total_name
@@ -796,6 +814,28 @@ def _letters():
     return iter("abcdefghjklmnopqrstuvwxyz")
 
 
+def particle_amp(r, m, m0, width, q, q0):
+    """Particle.get_amp (amp/core.py:399-418): 1 without a width; BW (constant width) when
+    running_width is false, else BWR with L = bw_l or the smallest l of the particle's first
+    decay and its barrier radius d; times the width when width_norm. ParticleBW (model BW,
+    amp/base.py:262-266) is BW alone. Other particle models are not restated."""
+    model = r.model or "default"
+    if model not in ("default", "BWR", "BW"):
+        raise NotImplementedError("particle model {}".format(model))
+    if r.width is None:
+        return torch.ones_like(m)
+    if model == "BW":
+        return BW(m, m0, width)
+    if not r.running_width:
+        ret = BW(m, m0, width)
+    else:
+        L = r.bw_l if r.bw_l is not None else min(l for l, s in r.decays[0].ls)
+        ret = BWR(m, m0, width, q, q0, L, r.d)
+    if r.width_norm:
+        ret = torch.complex(width, torch.zeros_like(width)) * ret
+    return ret
+
+
 def aligned_D(al, f):
     """The aligned-angle factor D^{J_f*}_{lambda, lambda'}(alpha, beta, gamma) of a spin>0
     final (amp/core.py:1414-1429), indexed [event][lambda in the chain's frame][lambda' in
@@ -856,13 +896,7 @@ def chain_amplitude(model, chain, angles, var):
         q = get_relative_p(m, mass[fs[y]], mass[fs[z]])
         m0 = var[r.name + "_mass"]
         q0 = get_relative_p(m0, torch.tensor(y.mass, dtype=DT), torch.tensor(z.mass, dtype=DT))
-        if r.width is None:
-            amp_r = torch.ones_like(m)
-        elif r.model == "BW":
-            amp_r = BW(m, m0, var[r.name + "_width"])
-        else:
-            L = min(l for l, s in r.decays[0].ls)
-            amp_r = BWR(m, m0, var[r.name + "_width"], q, q0, L, 3.0)
+        amp_r = particle_amp(r, m, m0, var.get(r.name + "_width"), q, q0)
         if not torch.is_complex(amp_r):
             amp_r = torch.complex(amp_r, torch.zeros_like(amp_r))
         rs = amp_r if rs is None else rs * amp_r
@@ -908,6 +942,35 @@ def sum_amp(model, angles, var):
 # --------------------------------------------------------------------------------------
 # Likelihood: model/model.py:41-94, 407-449, 603-641, 1126-1229
 # --------------------------------------------------------------------------------------
+def loaded_weight(idx, n, weight_option):
+    """The weight a config-loaded sample carries (SimpleData.load_extra_var / load_data,
+    config_loader/data.py:216-251): ``<idx>_weight`` as one number for every event or the
+    values of weight files, default 1, times -1 for names matching "bg*" (get_weight_sign,
+    149-155; re.match of "bg*" is any name that starts with b)."""
+    if weight_option is None:
+        w = np.ones(n)
+    elif isinstance(weight_option, (int, float)):
+        w = np.ones(n) * weight_option
+    else:
+        files = [weight_option] if isinstance(weight_option, str) else weight_option
+        w = np.concatenate([np.loadtxt(f).reshape((-1,)) for f in files])[:n]
+    sign = -1 if re.match("bg*", idx) else 1
+    return sign * w
+
+
+def get_weight_data(n_data, weight=None, bg_n=None, bg_weight=None, w_bkg=0.0, alpha=True):
+    """Model.get_weight_data (model/model.py:603-641): data weights (1 when None), then the
+    background's own weights, or -w_bkg for each background event when it has none, and
+    the alpha = sum w / sum w^2 rescale."""
+    w = np.ones(n_data) if weight is None else np.asarray(weight, dtype=np.float64)
+    if bg_n is not None:
+        bw = np.full(bg_n, -w_bkg) if bg_weight is None else np.asarray(bg_weight, float)
+        w = np.concatenate([w, bw])
+    if alpha:
+        w = w * (w.sum() / (w * w).sum())
+    return torch.as_tensor(w, dtype=DT)
+
+
 def fcn_weights(n_data, weight=None, n_mc=None, mc_weight=None):
     """FCN.__init__ (model.py:1126-1164) + get_weight_data (603-641)."""
     w = np.ones(n_data) if weight is None else np.asarray(weight, dtype=np.float64)

@@ -113,7 +113,8 @@ std::string check_model(const TpaModel& M, int n_cg) {
       if (C.l2[s2] < 0 || C.l2[s2] > 5) return "l out of range";
     if (C.k0 != next_k) return "chains must tile the partial-wave index range";
     next_k += C.n_ls1 * C.n_ls2;
-    if (C.shape < 0 || C.shape > 2 || C.bw_l < 0 || C.bw_l > 5) return "bad lineshape";
+    if (C.shape < 0 || C.shape > 7 || (C.shape & 3) == 3 || C.bw_l < 0 || C.bw_l > 5)
+      return "bad lineshape";
     const TpaTopo& T = M.topo[C.topo];
     int nX = M.n_fh[T.x];
     int n1 = C.n_ls1 * C.n_r * nX;
@@ -174,7 +175,9 @@ std::string check_model4(const TpaModel4& M, int n_cg) {
       for (int i = 0; i < C.r_n[r]; ++i)
         if (std::abs(C.r_hel[r][i]) > C.r_j2[r] || ((C.r_hel[r][i] - C.r_j2[r]) & 1))
           return "bad resonance helicity";
-      if (C.r_shape[r] < 0 || C.r_shape[r] > 2) return "bad lineshape";
+      if (C.r_shape[r] < 0 || C.r_shape[r] > 7 || (C.r_shape[r] & 3) == 3 || C.r_bw_l[r] < 0 ||
+          C.r_bw_l[r] > 5)
+        return "bad lineshape";
     }
     auto nh = [&](int slot) { return slot == 0 ? M.n_top : slot <= 2 ? C.r_n[slot - 1] : 1; };
     for (int v = 0; v < 3; ++v) {

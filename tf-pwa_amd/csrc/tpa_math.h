@@ -213,11 +213,18 @@ TPA_DEV double rel_p(double m0, double m1, double m2) {
 }
 
 // BWR (breit_wigner.py:74-91) with Gamma (231-245) and Bprime (290-306); BW (56-71).
+// Lineshape codes: bits 0-1 the propagator (0 none: 1; 1 BW, constant width; 2 BWR, running
+// width), bit 2 (width_norm) multiplies the propagator by the width g0 (Particle.get_amp,
+// tf_pwa/amp/core.py:399-418).
+#define TPA_SHAPE_KIND(s) ((s) & 3)
+#define TPA_SHAPE_NORM 4
+
 TPA_DEV cd lineshape(int shape, double m, double m0, double g0, double q, double q0, int L,
                      double d) {
-  if (shape == 0) return cd{1.0, 0.0};
+  const int kind = TPA_SHAPE_KIND(shape);
+  if (kind == 0) return cd{1.0, 0.0};
   double gamma = g0;
-  if (shape == 2) {
+  if (kind == 2) {
     double qq0 = q0 > 1.0e-15 ? pow(q / q0, (double)(2 * L + 1)) : 1.0;
     double mm0 = m0 / m;
     double qd0 = q0 * d, qd = q * d;
@@ -227,7 +234,8 @@ TPA_DEV cd lineshape(int shape, double m, double m0, double g0, double q, double
   double x = m0 * m0 - m * m;
   double y = m0 * gamma;
   double s = x * x + y * y;
-  return cd{x / s, y / s};
+  const double nf = (shape & TPA_SHAPE_NORM) ? g0 : 1.0;
+  return cd{nf * x / s, nf * y / s};
 }
 
 // Lineshape with its derivatives in the pole mass m0 and width g0 (the reverse-mode
@@ -235,13 +243,14 @@ TPA_DEV cd lineshape(int shape, double m, double m0, double g0, double q, double
 // get_relative_p's m_eff = max(m0, m1+m2) clamp, whose clamped branch has zero derivative).
 TPA_DEV void lineshape_grad(int shape, double m, double m0, double g0, double q, double my,
                             double mz, int L, double d, cd* B, cd* dBm, cd* dBg) {
-  if (shape == 0) {
+  const int kind = TPA_SHAPE_KIND(shape);
+  if (kind == 0) {
     *B = cd{1.0, 0.0};
     *dBm = *dBg = cd{0.0, 0.0};
     return;
   }
   double gamma = g0, dgm = 0.0, dgg = 1.0;
-  if (shape == 2) {
+  if (kind == 2) {
     const double s = my + mz, df = my - mz;
     const double me = m0 > s ? m0 : s;
     const double p = (me - s) * (me + s) * (me - df) * (me + df);
@@ -279,4 +288,9 @@ TPA_DEV void lineshape_grad(int shape, double m, double m0, double g0, double q,
   const cd dDg = cd{0.0, -(m0 * dgg)};
   *dBm = c_neg(c_mul(B2, dDm));
   *dBg = c_neg(c_mul(B2, dDg));
+  if (shape & TPA_SHAPE_NORM) {  // g0 B: d/dm0 = g0 dB/dm0, d/dg0 = B + g0 dB/dg0
+    *dBg = c_add(*B, c_scale(*dBg, g0));
+    *dBm = c_scale(*dBm, g0);
+    *B = c_scale(*B, g0);
+  }
 }

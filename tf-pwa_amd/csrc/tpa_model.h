@@ -49,7 +49,7 @@ typedef struct TpaChain {
   int32_t l1[TPA_MAX_LS], l2[TPA_MAX_LS];
   int32_t act1[TPA_MAX_LS], act2[TPA_MAX_LS];  // 1: barrier uses 1/sqrt(P_l(z)); 0: clamped
   int32_t k0;                        // first partial-wave index of the chain
-  int32_t shape;                     // 0: none (1), 1: BW, 2: BWR
+  int32_t shape;                     // bits 0-1: 0 none (1), 1 BW, 2 BWR; bit 2: times g0
   int32_t bw_l;                      // orbital L of the running width
   int32_t cg1_off, cg2_off;          // CG tables [ls][i_b][i_c] in this chain's out order
   double m0, g0;                     // lineshape mass and width (folded builds)
@@ -107,7 +107,7 @@ typedef struct Tpa4Chain {
   int32_t k0;
   int32_t r_j2[2], r_n[2];     // resonances of slots 1 and 2
   int32_t r_hel[2][TPA_MAX_HEL];
-  int32_t r_shape[2], r_bw_l[2];
+  int32_t r_shape[2], r_bw_l[2];   // lineshape codes as TpaChain.shape
   double r_m0[2], r_g0[2];
   double r_q0m[2][2];          // nominal daughter masses of each resonance (q0 of its width)
   double d;                    // barrier radius parameter

@@ -74,6 +74,40 @@ def _rel_p2_d_daughter(M, m1, m2, which):
     return dF / (4 * M * M)
 
 
+# Lineshape codes of the device description (TpaChain.shape, Tpa4Chain.r_shape): bits 0-1 the
+# propagator (1 BW with constant width, 2 BWR with running width), bit 2 the width_norm factor.
+SHAPE_NONE, SHAPE_BW, SHAPE_BWR, SHAPE_WIDTH_NORM = 0, 1, 2, 4
+LINESHAPE_MODELS = ("default", "BWR", "BW")
+
+
+def lineshape_code(R):
+    """The lineshape of resonance R as the default particle model computes it
+    (Particle.get_amp, tf_pwa/amp/core.py:399-418): 1 without a width; BW when
+    ``running_width`` is false, else BWR with the running width; times the width when
+    ``width_norm``. ``model: BW`` (ParticleBW, tf_pwa/amp/base.py:245-274) is BW and ignores
+    both options. Other particle models are not implemented and raise."""
+    model = R.model or "default"
+    if model not in LINESHAPE_MODELS:
+        raise NotImplementedError("particle model {!r} of {} (lineshapes implemented: {})".format(
+            model, R.name, ", ".join(LINESHAPE_MODELS)))
+    if R.width is None:
+        return SHAPE_NONE
+    if model == "BW":
+        return SHAPE_BW
+    code = SHAPE_BWR if R.kwargs.get("running_width", True) else SHAPE_BW
+    if R.kwargs.get("width_norm", False):
+        code |= SHAPE_WIDTH_NORM
+    return code
+
+
+def running_width_l(R):
+    """Orbital L of the running width: ``bw_l`` when given, else the smallest l of R's first
+    decay (core.py:409-411)."""
+    if R.kwargs.get("bw_l") is not None:
+        return int(R.kwargs["bw_l"])
+    return min(R.decays[0].l_list) if R.decays else 0
+
+
 class PartialWave:
     """One partial wave: a chain and the LS index at each of its vertices."""
 
@@ -140,18 +174,12 @@ class AmplitudeModel:
             d1 = c.decay_of(g.top)
             R = [o for o in d1.outs if o in c.inner][0]
             d2 = c.decay_of(R)
-            if R.width is None:
-                shape = 0
-            elif R.model in ("BW",):
-                shape = 1
-            else:
-                shape = 2
+            shape = lineshape_code(R)
             cg1 = d1.cg_matrix()
             cg2 = d2.cg_matrix()
             ch = dict(topo=t, top_r_first=int(d1.outs[0] is R),
                       r_y_first=int(fidx[d2.outs[0].name] == topo[t]["y"]),
-                      R=R, d1=d1, d2=d2, k0=k0, shape=shape,
-                      bw_l=min(R.decays[0].l_list) if R.decays else 0,
+                      R=R, d1=d1, d2=d2, k0=k0, shape=shape, bw_l=running_width_l(R),
                       cg1_off=len(cg), cg2_off=len(cg) + cg1.size)
             cg.extend(cg1.ravel().tolist())
             cg.extend(cg2.ravel().tolist())
@@ -243,9 +271,7 @@ class AmplitudeModel:
             res = []
             for r in range(2):
                 Rr, dr = R[r], decs[1 + r]
-                shape = 0 if Rr.width is None else (1 if Rr.model in ("BW",) else 2)
-                res.append(dict(R=Rr, shape=shape,
-                                bw_l=min(Rr.decays[0].l_list) if Rr.decays else 0,
+                res.append(dict(R=Rr, shape=lineshape_code(Rr), bw_l=running_width_l(Rr),
                                 q0m=[nominal(o) for o in dr.outs]))
             n_ls = [len(d.ls) for d in decs]
             chains.append(dict(topo=t, decs=decs, R=R, res=res, k0=k0, cg_off=cg_off,
@@ -520,7 +546,7 @@ class AmplitudeModel:
         lineshapes, which are evaluated per event with their derivatives."""
         key = self.vm.struct_version
         if getattr(self, "_sm_key", None) != key:
-            tv = set(self.vm.trainable_vars)
+            tv = set(self.vm.index_map())
             self._sm = any(nm.get(k) in tv for nm in self._names()
                            for k in ("mass", "width", "mass2", "width2"))
             self._sm_key = key
@@ -670,7 +696,7 @@ class AmplitudeModel:
         """J[i, k] = d c_k / d trainable_vars[i] (complex) for the coupling parameters.
         Raises if a lineshape parameter is trainable (needs the shape-gradient path)."""
         vm = self.vm
-        index = {n: i for i, n in enumerate(vm.trainable_vars)}
+        index = vm.index_map()
         for nm in self._names():
             for key in ("mass", "width"):
                 if nm[key] in index:
@@ -697,7 +723,7 @@ class AmplitudeModel:
         vm = self.vm
         key = tuple(vm.trainable_vars)
         if getattr(self, "_rows_key", None) != key:
-            index = {n: i for i, n in enumerate(vm.trainable_vars)}
+            index = vm.index_map()
             fx = self._factor_index()
             self._rows = (np.array([index.get(n + "r", -1) for n in fx["cnames"]]),
                           np.array([index.get(n + "i", -1) for n in fx["cnames"]]))
@@ -777,7 +803,7 @@ class AmplitudeModel:
         if getattr(self, "_sg_key", None) != key:
             rkey = tuple(vm.trainable_vars)
             if getattr(self, "_sg_rows_key", None) != rkey:
-                index = {nm: i for i, nm in enumerate(vm.trainable_vars)}
+                index = vm.index_map()
                 fx = self._factor_index()
                 names = self._names()
                 self._sg_rows = (np.array([index.get(nm["mass"], -1) for nm in names])[fx["ch"]],

@@ -58,10 +58,13 @@ def decode_second_moments(out, P):
 
 class EventSet:
     """One sample on the device. ``amp`` is the AmplitudeModel whose description the
-    kernels use; ``weight`` defaults to 1."""
+    kernels use; ``weight`` defaults to 1. ``has_weight`` says whether weights were given (a
+    sample dict's "weight" entry in the reference: background sets without one take -w_bkg
+    in Model.get_weight_data)."""
 
     def __init__(self, amp, p4, weight=None, order=None):
         self.amp = amp
+        self.has_weight = weight is not None
         arr = p4_array(p4, amp.finals, order)
         self.n = arr.shape[1]
         self.weight = np.ones(self.n) if weight is None else np.ascontiguousarray(
@@ -81,6 +84,7 @@ class EventSet:
     def _wrap(cls, amp, h, n, weight):
         self = cls.__new__(cls)
         self.amp = amp
+        self.has_weight = weight is not None
         self.n = int(n)
         self.weight = np.ones(self.n) if weight is None else np.array(weight, dtype=np.float64)
         self._h = h
@@ -128,7 +132,9 @@ class EventSet:
         h = ctypes.c_void_p()
         _lib.check(_lib.lib().tpa_events_concat(handles, len(sets), ctypes.byref(h)))
         w = np.concatenate([s._dev_weight for s in sets])
-        return cls._wrap(amp, h, w.size, w)
+        ev = cls._wrap(amp, h, w.size, w)
+        ev.has_weight = any(s.has_weight for s in sets)
+        return ev
 
     def __len__(self):
         return self.n

@@ -96,8 +96,10 @@ class Model:
         self.n_eval = 0
 
     def get_weight_data(self, data, weight=None, bg=None, alpha=True):
-        """Concatenate data (+ background with weight -w_bkg) and apply the alpha rescale
-        (model.py:603-641). Returns (WeightedData, weight)."""
+        """Concatenate data and background and apply the alpha rescale (model.py:603-641). A
+        background set keeps the weights it was given (the config loader gives every
+        background sample -bg_weight or minus its weight file); one without weights takes
+        -w_bkg per event. Returns (WeightedData, weight)."""
         sets = _as_sets(data)
         if weight is None:
             weight = np.concatenate([s.weight for s in sets])
@@ -108,10 +110,8 @@ class Model:
             bsets = _as_sets(bg)
             bw = []
             for s in bsets:
-                if np.all(s.weight == 1.0):
-                    bw.append(np.full(s.n, -self.w_bkg))
-                else:
-                    bw.append(s.weight)
+                bw.append(s.weight if getattr(s, "has_weight", True) else
+                          np.full(s.n, -self.w_bkg))
             sets = sets + bsets
             weight = np.concatenate([weight] + bw)
         if alpha:

@@ -146,7 +146,7 @@ def shape_map(amp):
     if amp.n_final != 3:
         raise NotImplementedError("native floating lineshapes cover three-body chains")
     vm = amp.vm
-    index = {n: i for i, n in enumerate(vm.trainable_vars)}
+    index = vm.index_map()
     names = amp._names()
     bc = amp.barrier_constants()
     fx = amp._factor_index()
@@ -183,7 +183,7 @@ def coupling_map(amp):
     per trainable variable its bound. Returns (map, arrays kept alive for the map)."""
     vm = amp.vm
     fx = amp._factor_index()
-    index = {n: i for i, n in enumerate(vm.trainable_vars)}
+    index = vm.index_map()
     names = fx["cnames"]
     slots = list(fx["slots"])
     unit = len(names)  # a fixed factor 1 + 0i fills the unused slots of three-body waves

@@ -13,6 +13,8 @@ class Bound:
     """Bound transform y(x) of tf_pwa/variable.py:1029-1135 ("asin" style)."""
 
     def __init__(self, a=None, b=None):
+        if a is not None and b is not None and a > b:
+            raise ValueError("bound lower {} above upper {}".format(a, b))
         self.lower = a
         self.upper = b
 
@@ -27,12 +29,16 @@ class Bound:
         return x
 
     def get_y2x(self, y):
+        """Inverse transform; y is clamped into the bound first, as the reference does, so a
+        start value outside a one-sided range maps to its edge instead of NaN."""
         a, b = self.lower, self.upper
         if a is not None and b is not None:
             return np.arcsin(np.clip(2 * (y - a) / (b - a) - 1, -1, 1))
         if a is not None:
+            y = max(y, a)
             return np.sqrt((y - a + 1) ** 2 - 1)
         if b is not None:
+            y = min(y, b)
             return np.sqrt((b - y + 1) ** 2 - 1)
         return y
 
@@ -57,6 +63,9 @@ class VarsManager:
         self.complex_vars = {}
         self.bnd_dic = {}
         self.init_val = {}
+        # shared variables (set_same): every alias name reads and writes its canonical name's
+        # value, and only the canonical one can be trainable
+        self.same = {}
         # bumped on every change of a value or of the trainable set, so derived quantities
         # (coefficients, lineshape state) can be cached by version
         self.version = 0
@@ -98,13 +107,56 @@ class VarsManager:
         """variable.py:415-440: fixing removes from trainable_vars, freeing appends."""
         self.version += 1
         self.struct_version += 1
+        name = self.same.get(name, name)
         if value is not None:
             self.variables[name] = float(value)
+            self._mirror()
         if unfix:
             if name not in self.trainable_vars:
                 self.trainable_vars.append(name)
         elif name in self.trainable_vars:
             self.trainable_vars.remove(name)
+
+    def set_same(self, names):
+        """Make the listed real variables one (VarsManager.set_same, variable.py:500-546):
+        the first name is canonical and takes its current value; the others leave
+        trainable_vars, and if any of them is fixed the canonical one is fixed too. A name
+        already shared joins with its whole group."""
+        names = [n for n in names if n in self.variables]
+        if len(names) < 2:
+            return
+        group = []
+        for n in names:
+            c = self.same.get(n, n)
+            for m in [c] + [a for a, cc in self.same.items() if cc == c]:
+                if m not in group:
+                    group.append(m)
+        canon = self.same.get(names[0], names[0])
+        group.remove(canon)
+        group.insert(0, canon)
+        fixed = any(n not in self.trainable_vars for n in group)
+        for n in group[1:]:
+            self.same[n] = canon
+            if n in self.trainable_vars:
+                self.trainable_vars.remove(n)
+        if fixed and canon in self.trainable_vars:
+            self.trainable_vars.remove(canon)
+        self._mirror()
+        self.version += 1
+        self.struct_version += 1
+
+    def _mirror(self):
+        for a, c in self.same.items():
+            self.variables[a] = self.variables[c]
+
+    def index_map(self):
+        """Name -> position in trainable_vars for every name that moves with a trainable
+        variable, shared aliases included (their derivatives add into the canonical row)."""
+        index = {n: i for i, n in enumerate(self.trainable_vars)}
+        for a, c in self.same.items():
+            if c in index:
+                index[a] = index[c]
+        return index
 
     def set_bound(self, bound_dic):
         self.struct_version += 1
@@ -116,7 +168,10 @@ class VarsManager:
         return self.variables[name]
 
     def set(self, name, value):
+        name = self.same.get(name, name)
         self.variables[name] = float(value)
+        if self.same:
+            self._mirror()
         self.version += 1
         if name not in self.trainable_vars:
             self.struct_version += 1
@@ -136,7 +191,7 @@ class VarsManager:
             self.struct_version += 1
             for k, v in vals.items():
                 if k in self.variables:
-                    self.variables[k] = float(v)
+                    self.variables[self.same.get(k, k)] = float(v)
         else:
             # ndarray.tolist gives Python floats in one call (float() per numpy scalar costs
             # ~0.2 us each, on every likelihood call)
@@ -145,6 +200,8 @@ class VarsManager:
                 raise ValueError("expected {} values, got {}".format(
                     len(self.trainable_vars), len(vals)))
             self.variables.update(zip(self.trainable_vars, vals))
+        if self.same:
+            self._mirror()
 
     def complex_value(self, name):
         """Variable.__call__ (variable.py:1715-1765)."""
