@@ -27,6 +27,7 @@ CONFIG = os.path.join(ROOT, "tf-pwa_amd", "tf_pwa_amd", "configs", "lc_pkpi_6res
 CONFIG4 = os.path.join(ROOT, "tf-pwa_amd", "tf_pwa_amd", "configs", "jpsi_kkpipi_12ch.yml")
 METRIC = "NLL+grad evals/s @1M data+10M MC, |dNLL|<1e-3 vs FP64; toy-fit p50/p99 ms"
 HBM_PEAK_GBS = 8000.0
+FP64_PEAK_TFLOPS = 78.6  # MI355X FP64 vector peak
 ANGLE_BYTES_PER_EVENT = 176
 TRAFFIC = os.path.join(ROOT, "profiles", "k_eval_pipe_traffic.json")
 
@@ -340,6 +341,15 @@ def multistart_leg(args, seed=3303, starts=8, n_data=10_000, n_mc=100_000):
     fcn_rate = len(steps) / (time.perf_counter() - tf)
     log("floating lineshapes: {:.0f} evals/s in native BFGS fits ({:.0f} with scipy's), "
         "FCN.nll_grad alone {:.0f} evals/s".format(nfev / wall, snfev / swall, fcn_rate))
+    # exact Hessian in all 46 parameters (lineshape second-moment passes) at the best fit,
+    # against the 2 n gradient calls of a central-difference Hessian
+    best_r = min(results, key=lambda r: r.min_nll)
+    conf.set_params(best_r.params)
+    fcn.nll_grad_hessian()
+    th = time.perf_counter()
+    _, _, Hbest = fcn.nll_grad_hessian()
+    hess_ms = 1e3 * (time.perf_counter() - th)
+    ev = np.linalg.eigvalsh(0.5 * (Hbest + Hbest.T))
     return {"workload": "Lc->pKpi 6 resonances, all masses and widths floating ({} parameters: "
                         "34 couplings + 12 lineshape), {} data + {} MC, {} BFGS fits from seeded "
                         "starts, FP64".format(n_par, n_data, n_mc, starts),
@@ -350,7 +360,10 @@ def multistart_leg(args, seed=3303, starts=8, n_data=10_000, n_mc=100_000):
             "wall_s": wall, "n_params": n_par,
             "best_nll": best, "nll": [r.min_nll for r in results],
             "fits_at_best": int(sum(abs(r.min_nll - best) < 1e-3 for r in results)),
-            "ms_per_eval": 1e3 * wall / max(1, nfev)}
+            "ms_per_eval": 1e3 * wall / max(1, nfev),
+            "exact_hessian": {"ms": hess_ms, "central_difference_ms": 2 * n_par * 1e3 / fcn_rate,
+                              "min_eigenvalue_at_best": float(ev.min()),
+                              "max_eigenvalue_at_best": float(ev.max())}}
 
 
 def four_body_leg(args, seed=5505):
@@ -515,11 +528,34 @@ def main():
 
     traffic = measured_traffic() if (args.n_data, args.n_mc) == (1_000_000, 10_000_000) else None
 
-    def roofline(r, n_events_per_step, f32=False):
+    kg = 1
+    while kg * 8 < amp.P:
+        kg <<= 1
+
+    def flops_per_event(mode_data):
+        """Algorithmic FP64 operations of one event of the evaluation pass, FMA = 2: A_lambda
+        = sum_k c_k h_k (8 P Lam), the k-group reductions of A (2 Lam (KG - 1)), f = sum |A|^2
+        (3 Lam - 1), the weighted value and its compensated sum (6), the adjoint prefactors
+        u conj(A) (2 Lam) and G_k += u conj(A) h_k (8 P Lam). Special functions counted apart:
+        clip_log's log and reciprocal (data passes)."""
+        lam = amp.lam
+        return (16 * amp.P * lam + 2 * lam * (kg - 1) + (3 * lam - 1) + 6 + 2 * lam,
+                {"log": 1, "reciprocal": 1} if mode_data else {})
+
+    def roofline(r, n_events_per_step, f32=False, n_data_per_step=None):
+        """Roofline of a timed path's pass kernel. achieved: the 176 B/event angle cache (the
+        path's input per event: 3 topologies x 5 f64, 2 x 3 aligned angles, the weight) of
+        the events one launch covers, over the kernel's average launch time; the bytes the
+        kernel itself streams (stored partial waves) are kept beside it, as are its counted
+        FP64 operations against the FP64 vector peak."""
         bpe = bytes_per_event_f32 if f32 else bytes_per_event
         kernel_s = r["kernel_ms"] / 1e3
-        bytes_total = n_events_per_step * bpe * args.steps
-        achieved = bytes_total / kernel_s / 1e9 if kernel_s > 0 else 0.0
+        launches = max(1, r["launches"])
+        per_launch_s = kernel_s / launches
+        events_per_launch = n_events_per_step * args.steps / launches
+        achieved = ANGLE_BYTES_PER_EVENT * events_per_launch / per_launch_s / 1e9 \
+            if kernel_s > 0 else 0.0
+        streamed = bpe * events_per_launch / per_launch_s / 1e9 if kernel_s > 0 else 0.0
         tr = None
         if traffic is not None:
             if f32:
@@ -527,19 +563,25 @@ def main():
             else:
                 tr = traffic["data_1M"] if n_events_per_step == args.n_data else 0.5 * (
                     traffic["data_1M"] + traffic["mc_10M"])
-        # the angle-cache input of the same events (22 f64 per event: 3 topologies x 5 + 2 x 3
-        # aligned angles + weight), the bytes a recompute-from-angles kernel would read
-        survey = ANGLE_BYTES_PER_EVENT * n_events_per_step * args.steps / kernel_s / 1e9 \
-            if kernel_s > 0 else 0.0
+        nd = n_events_per_step if n_data_per_step is None else n_data_per_step
+        fd, special = flops_per_event(True)
+        fm, _ = flops_per_event(False)
+        flops_step = nd * fd + (n_events_per_step - nd) * fm
+        tflops = flops_step * args.steps / kernel_s / 1e12 if kernel_s > 0 else 0.0
         return {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": achieved / HBM_PEAK_GBS, "traffic": tr,
                 "kernel": "k_eval_pipe32" if f32 else "k_eval_pipe",
-                "angle_cache_bytes_per_event": ANGLE_BYTES_PER_EVENT,
-                "achieved_on_angle_cache_bytes": survey,
-                "launches": r["launches"], "kernel_ms_avg": r["kernel_ms"] / max(1, r["launches"]),
-                "bytes_per_event": bpe,
-                "bytes_per_launch": n_events_per_step * bpe / max(
-                    1, r["launches"] // args.steps)}
+                "basis": "angle cache, {} B/event".format(ANGLE_BYTES_PER_EVENT),
+                "events_per_launch": events_per_launch,
+                "launches": r["launches"], "kernel_ms_avg": r["kernel_ms"] / launches,
+                "stored_waves": {"bytes_per_event": bpe, "achieved": streamed,
+                                 "frac": streamed / HBM_PEAK_GBS,
+                                 "bytes_per_launch": bpe * events_per_launch},
+                "fp64_valu": {"flops_per_event_data": fd, "flops_per_event_mc": fm,
+                              "special_per_data_event": special,
+                              "rule": "FMA = 2 flops; log and reciprocal apart",
+                              "achieved_TFLOPs": tflops, "peak_TFLOPs": FP64_PEAK_TFLOPS,
+                              "frac": tflops / FP64_PEAK_TFLOPS}}
 
     log("fp64 path ...")
     r64 = time_path(FCN(conf.get_model(), data, mc))
@@ -610,7 +652,8 @@ def main():
         "roofline": roofline(head, head_events, head_f32),
         "paths": {
             "fp64": {"evals_per_s": r64["evals"], "ms_per_step": r64["ms_per_step"],
-                     "nll": r64["nll"], "roofline": roofline(r64, args.n_data + args.n_mc)},
+                     "nll": r64["nll"],
+                     "roofline": roofline(r64, args.n_data + args.n_mc, n_data_per_step=args.n_data)},
             "mixed": {"evals_per_s": rmix["evals"], "ms_per_step": rmix["ms_per_step"],
                       "nll": rmix["nll"], "abs_dnll_vs_fp64": dnll, "grad_rel_err_vs_fp64": grel,
                       "speedup_vs_fp64": rmix["evals"] / r64["evals"], "gram": "fp64",

@@ -198,6 +198,20 @@ int tpa_gram_f64(tpa_events_t ev, double* out, double* ms);
  * BaseModel.nll_grad_hessian (513-560) for the coupling parameters. */
 int tpa_hess(tpa_events_t ev, const double* coef, int mode, double* out);
 
+/* Moments of the exact Hessian when masses and widths float (three-body, fold = 0 cache, the
+ * model's current lineshape parameters, P <= 24). With X_(t,k) = b^t h_k the extended waves
+ * (t = 0, 1, 2: the wave's lineshape B, dB/dm0 and dB/dg0), A = sum_k c_k X_(0,k), and u, v
+ * the mode's first and second derivative weights, out (host, 2 + 12P + Q(Q+1) doubles,
+ * Q = 6P) receives: out[0] the sum of the mode; then F_j = sum_e u z_j, z_j = sum_lambda
+ * conj(A) X_j (j < 3P, interleaved re/im; F_(0,k) is tpa_eval's G_k); then
+ * F2_(t,k) = sum_e u d2B_t q_k (t: d2/dm0^2, d2/dm0dg0, d2/dg0^2; q_k = sum_lambda conj(A) h_k;
+ * 3P complex); then the packed upper triangles of O1 = sum_e v y y^T, y = (Re z, Im z), and
+ * O2 = sum_e u sum_lambda x x^T, x = (Re X, Im X). Replaces the nested tf.GradientTape
+ * Hessians (sum_hessian, tf_pwa/model/model.py:97-150) of BaseModel.nll_grad_hessian
+ * (513-560) for models with `float: mg` resonances (the mass and width Variables of
+ * tf_pwa/amp/core.py:380-390). */
+int tpa_hess_shape(tpa_events_t ev, const double* coef, int mode, double* out);
+
 /* The same for the data range [offset, offset+n) of a pool allocation (TPA_MODE_DATA). */
 int tpa_pool_hess(tpa_pool_t pool, int64_t offset, int64_t n, const double* coef, double* out);
 

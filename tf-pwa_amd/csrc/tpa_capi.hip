@@ -224,6 +224,9 @@ struct tpa_events_s {
   double* d_hpart = nullptr;  // second-moment pass: partials and result
   double* d_hout = nullptr;
   double2* d_hcoef = nullptr;
+  double2* d_bw2 = nullptr;    // lineshapes with second derivatives (tpa_hess_shape)
+  double* d_hspart = nullptr;  // its pass's partials and result
+  double* d_hsout = nullptr;
   int built_fold = -1;
   hipEvent_t t0 = nullptr, t1 = nullptr;
   double* h_io = nullptr;  // pinned staging of the coefficients and the sums of tpa_eval
@@ -832,6 +835,9 @@ int tpa_events_destroy(tpa_events_t ev) {
   if (ev->d_hpart) (void)hipFree(ev->d_hpart);
   if (ev->d_hout) (void)hipFree(ev->d_hout);
   if (ev->d_hcoef) (void)hipFree(ev->d_hcoef);
+  if (ev->d_bw2) (void)hipFree(ev->d_bw2);
+  if (ev->d_hspart) (void)hipFree(ev->d_hspart);
+  if (ev->d_hsout) (void)hipFree(ev->d_hsout);
   if (ev->t0) (void)hipEventDestroy(ev->t0);
   if (ev->t1) (void)hipEventDestroy(ev->t1);
   if (ev->h_io) (void)hipHostFree(ev->h_io);
@@ -1863,6 +1869,93 @@ int tpa_hess(tpa_events_t ev, const double* coef, int mode, double* out) {
   if (ev->built_fold != 1) return fail(TPA_ERR_ARG, "second moments need a fold = 1 cache");
   return run_hess(ev->m->h, ev->d_h, ev->d_w, 0, ev->n, ev->n, coef, mode, ev->m->stream,
                   &ev->d_hpart, &ev->d_hout, &ev->d_hcoef, out);
+}
+
+int tpa_hess_shape(tpa_events_t ev, const double* coef, int mode, double* out) {
+  if (!ev || !coef || !out) return fail(TPA_ERR_ARG, "null pointer");
+  if (mode != TPA_MODE_DATA && mode != TPA_MODE_MC) return fail(TPA_ERR_ARG, "bad mode");
+  if (ev->m->kind != 3)
+    return fail(TPA_ERR_UNSUPPORTED, "floating lineshapes need a three-body model");
+  if (ev->built_fold != 0)
+    return fail(TPA_ERR_ARG, "lineshape second moments need a cache built with fold = 0");
+  const TpaModel& M = ev->m->h;
+  const int P = M.P, Q = 6 * P;
+  if (P > TPA_HESS_SHAPE_PMAX)
+    return fail(TPA_ERR_UNSUPPORTED, "lineshape second moments need P <= " +
+                                         std::to_string(TPA_HESS_SHAPE_PMAX));
+  const int QT = Q <= 48 ? 3 : Q <= 96 ? 6 : 9;
+  const int nacc = 2 + 12 * P + 2 * (QT * (QT + 1) / 2) * 256;
+  const int LG = M.lam_pad;
+  if (LG != 1 && LG != 2 && LG != 4)
+    return fail(TPA_ERR_UNSUPPORTED, "lineshape second moments need <= 4 helicity slots");
+  const int E = 32 / LG;
+  const int64_t tiles = (ev->n + E - 1) / E;
+  const int grid = (int)std::max<int64_t>(1, std::min<int64_t>(tiles, kHessGridMax));
+  hipStream_t s = ev->m->stream;
+  if (!ev->d_bw2)
+    HIPCHK(hipMalloc(&ev->d_bw2, sizeof(double2) * 6 * (size_t)M.n_chain * ev->n));
+  if (!ev->d_hspart) {
+    const int cap = 2 + 12 * TPA_HESS_SHAPE_PMAX + 2 * 45 * 256;
+    HIPCHK(hipMalloc(&ev->d_hspart, sizeof(double) * (size_t)kHessGridMax * cap));
+    HIPCHK(hipMalloc(&ev->d_hsout, sizeof(double) * cap));
+  }
+  if (!ev->d_hcoef) HIPCHK(hipMalloc(&ev->d_hcoef, sizeof(double2) * TPA_HESS_PMAX));
+  HIPCHK(hipMemcpyAsync(ev->d_hcoef, coef, sizeof(double2) * P, hipMemcpyHostToDevice, s));
+  const int64_t lgrid = (ev->n + 255) / 256;
+  hipLaunchKernelGGL(k_lineshape_hess, dim3((unsigned)lgrid), dim3(256), 0, s, ev->m->d,
+                     ev->d_ang, ev->n, ev->d_bw2);
+  HIPCHK(hipGetLastError());
+  HessShapeArgs a;
+  a.h = ev->d_h;
+  a.w = ev->d_w;
+  a.coef = ev->d_hcoef;
+  a.bw = ev->d_bw2;
+  a.kchain = ev->m->d_kchain;
+  a.partials = ev->d_hspart;
+  a.N = ev->n;
+  a.P = P;
+  a.n_chain = M.n_chain;
+  a.mode = mode;
+  a.nacc = nacc;
+#define TPA_HS_LAUNCH(lg, qt) \
+  hipLaunchKernelGGL((k_hess_shape<lg, qt>), dim3(grid), dim3(256), 0, s, a)
+  if (LG == 1) {
+    if (QT == 3) TPA_HS_LAUNCH(1, 3); else if (QT == 6) TPA_HS_LAUNCH(1, 6); else TPA_HS_LAUNCH(1, 9);
+  } else if (LG == 2) {
+    if (QT == 3) TPA_HS_LAUNCH(2, 3); else if (QT == 6) TPA_HS_LAUNCH(2, 6); else TPA_HS_LAUNCH(2, 9);
+  } else {
+    if (QT == 3) TPA_HS_LAUNCH(4, 3); else if (QT == 6) TPA_HS_LAUNCH(4, 6); else TPA_HS_LAUNCH(4, 9);
+  }
+#undef TPA_HS_LAUNCH
+  HIPCHK(hipGetLastError());
+  hipLaunchKernelGGL(k_reduce, dim3(nacc), dim3(256), 0, s, ev->d_hspart, grid, nacc, 0,
+                     ev->d_hsout, nullptr, nullptr, nullptr, (uint64_t)0);
+  HIPCHK(hipGetLastError());
+  std::vector<double> raw(nacc);
+  HIPCHK(hipMemcpyAsync(raw.data(), ev->d_hsout, sizeof(double) * nacc, hipMemcpyDeviceToHost,
+                        s));
+  HIPCHK(hipStreamSynchronize(s));
+  // value, F, F2, then each matrix's upper triangle (row-major, i <= j) from the 16x16 FP64
+  // MFMA accumulator layout: element (i, j) of tile q sits in lane j + 16 (i % 4), register i / 4
+  const int head = 2 + 12 * P;
+  for (int j = 0; j < head; ++j) out[j] = raw[j];
+  std::vector<int> tile(QT * QT, -1);
+  for (int ti = 0, q = 0; ti < QT; ++ti)
+    for (int tj = ti; tj < QT; ++tj, ++q) tile[ti * QT + tj] = q;
+  const int NT = QT * (QT + 1) / 2;
+  const int T = Q * (Q + 1) / 2;
+  for (int mtx = 0; mtx < 2; ++mtx) {
+    const double* src = raw.data() + head + (size_t)mtx * NT * 256;
+    double* dst = out + head + (size_t)mtx * T;
+    int idx = 0;
+    for (int r = 0; r < Q; ++r)
+      for (int c = r; c < Q; ++c, ++idx) {
+        const int q = tile[(r / 16) * QT + c / 16];
+        const int i = r % 16, j = c % 16;
+        dst[idx] = src[(q * 4 + i / 4) * 64 + j + 16 * (i % 4)];
+      }
+  }
+  return TPA_OK;
 }
 
 int tpa_pool_hess(tpa_pool_t p, int64_t offset, int64_t n, const double* coef, double* out) {

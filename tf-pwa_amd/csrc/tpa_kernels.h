@@ -1949,6 +1949,262 @@ __global__ __launch_bounds__(256) void k_hess_mfma(HessArgs a) {
   if (t < Q) out[t < P ? 2 + 2 * t : 3 + 2 * (t - P)] = gacc;  // interleaved re / im
 }
 
+// ------------------------------------------ second derivatives with floating lineshapes
+// Per chain and event: the lineshape and its first and second derivatives in (m0, g0),
+// bw[(q * n_chain + c) * N + e] for q = 0 .. 5 (B, dB/dm0, dB/dg0, d2B/dm0^2, d2B/dm0dg0,
+// d2B/dg0^2; lineshape_hess).
+__global__ __launch_bounds__(256) void k_lineshape_hess(const TpaModel* __restrict__ M,
+                                                        const double* __restrict__ ang,
+                                                        int64_t N, double2* __restrict__ bw) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= N) return;
+  const int nt = M->n_topo;
+  const int nc = M->n_chain;
+  const int64_t gbase = (int64_t)TPA_NF_TOPO * nt;
+  double mf[3];
+  for (int f = 0; f < 3; ++f) mf[f] = ang[(gbase + 1 + f) * N + e];
+  for (int c = 0; c < nc; ++c) {
+    const TpaChain& C = M->chain[c];
+    const TpaTopo T = M->topo[C.topo];
+    const double mR = ang[(int64_t)(TPA_NF_TOPO * C.topo + TPA_F_MR) * N + e];
+    const int fy = C.r_y_first ? T.y : T.z;
+    const int fz = C.r_y_first ? T.z : T.y;
+    const double q = rel_p(mR, mf[fy], mf[fz]);
+    cd out[6];
+    lineshape_hess(C.shape, mR, C.m0, C.g0, q, C.m_y, C.m_z, C.bw_l, C.d, out);
+    for (int k = 0; k < 6; ++k) bw[(int64_t)(k * nc + c) * N + e] = make_double2(out[k].r, out[k].i);
+  }
+}
+
+// One pass over a fold-0 cache (waves without their lineshapes) that gives every moment the
+// exact Hessian in couplings, masses and widths needs. Per event and helicity it forms the
+// extended waves X_(t,k) = b^t h_k, t = 0, 1, 2 (b^0 = B, b^1 = dB/dm0, b^2 = dB/dg0 of wave
+// k's chain), the amplitude A = sum_k c_k X_(0,k), and with u, v the first and second
+// derivative weights of the mode (as k_hess_mfma) accumulates
+//   F_j       = sum_e u z_ej,           z_ej = sum_l conj(A_el) X_j,el  (j < 3P; F_(0,k) = G_k)
+//   F2_(t,k)  = sum_e u b^(3+t) q_ek,   q_ek = sum_l conj(A_el) h_ekl   (t: mm, mg, gg)
+//   O1        = sum_e v y_e y_e^T,      y_e  = (Re z_e, Im z_e)            (6P columns)
+//   O2        = sum_e u sum_l x x^T,    x_el = (Re X_.,el, Im X_.,el)
+// as FP64 MFMA Gram products over tiles of R (event, helicity) rows staged in LDS. The
+// QT(QT+1)/2 16x16 tiles of each matrix are owned by the block's four waves in turn (tile q
+// by wave q % 4), which run every row of the staged tile through their own tiles. Partials per
+// block: value pair, F (interleaved), F2, then O1 and O2 as raw MFMA accumulator tiles.
+struct HessShapeArgs {
+  const double2* __restrict__ h;     // [P][N][LG], fold-0 cache
+  const double* __restrict__ w;      // [N]
+  const double2* __restrict__ coef;  // [P]
+  const double2* __restrict__ bw;    // [6][n_chain][N] (k_lineshape_hess)
+  const int* __restrict__ kchain;    // [P]
+  double* __restrict__ partials;     // [grid][nacc]
+  int64_t N;
+  int P, n_chain, mode, nacc;
+};
+
+#define TPA_HESS_SHAPE_PMAX 24
+
+template <int LG, int QT>
+__global__ __launch_bounds__(256) void k_hess_shape(HessShapeArgs a) {
+  constexpr int R = 32;
+  constexpr int E = R / LG;
+  constexpr int C = 16 * QT;
+  constexpr int NT = QT * (QT + 1) / 2;
+  constexpr int NTW = (NT + 3) / 4;
+  constexpr int ZS = R + 1, YS = E + 1;
+  __shared__ double Zs[C * ZS];
+  __shared__ double Ys[C * YS];
+  __shared__ double2 As[R];
+  __shared__ double ur[R];
+  __shared__ double ve[E];
+  const int P = a.P, P3 = 3 * P, Q = 6 * P;
+  const int nc = a.n_chain;
+  const int t = threadIdx.x;
+  const int lane = t & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(t >> 6);
+  const int il = lane & 15, kl = lane >> 4;
+  const double eps = 1.0e-6;
+  for (int idx = t; idx < (C - Q) * ZS; idx += 256) Zs[Q * ZS + idx] = 0.0;
+  for (int idx = t; idx < (C - Q) * YS; idx += 256) Ys[Q * YS + idx] = 0.0;
+  tpa_doublex4 o1[NTW], o2[NTW];
+#pragma unroll
+  for (int q = 0; q < NTW; ++q) o1[q] = o2[q] = tpa_doublex4{0.0, 0.0, 0.0, 0.0};
+  double gacc = 0.0;                        // thread t < 6P: component t of (Re F, Im F)
+  double f2[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};  // thread t < P: F2_(mm|mg|gg, t) re, im
+  double vsum = 0.0, vcomp = 0.0;
+  const int64_t tiles = (a.N + E - 1) / E;
+  for (int64_t tile = blockIdx.x; tile < tiles; tile += gridDim.x) {
+    const int64_t e0 = tile * E;
+    // stage the extended waves: columns t P + k (real) and 3P + t P + k (imaginary)
+    for (int idx = t; idx < P * R; idx += 256) {
+      const int k = idx / R, r = idx - k * R;
+      const int64_t e = e0 + r / LG;
+      const int c = a.kchain[k];
+      double2 hv = make_double2(0.0, 0.0);
+      double2 b[3] = {hv, hv, hv};
+      if (e < a.N) {
+        hv = a.h[((int64_t)k * a.N + e) * LG + (r % LG)];
+#pragma unroll
+        for (int tt = 0; tt < 3; ++tt) b[tt] = a.bw[(int64_t)(tt * nc + c) * a.N + e];
+      }
+#pragma unroll
+      for (int tt = 0; tt < 3; ++tt) {
+        Zs[(tt * P + k) * ZS + r] = b[tt].x * hv.x - b[tt].y * hv.y;
+        Zs[(P3 + tt * P + k) * ZS + r] = b[tt].x * hv.y + b[tt].y * hv.x;
+      }
+    }
+    __syncthreads();
+    if (t < R) {
+      double ar = 0.0, ai = 0.0;
+      for (int k = 0; k < P; ++k) {
+        const double2 c = a.coef[k];
+        const double xr = Zs[k * ZS + t], xi = Zs[(P3 + k) * ZS + t];
+        ar += c.x * xr - c.y * xi;
+        ai += c.x * xi + c.y * xr;
+      }
+      As[t] = make_double2(ar, ai);
+    }
+    __syncthreads();
+    if (t < E) {
+      const int64_t e = e0 + t;
+      double u = 0.0, v = 0.0;
+      if (e < a.N) {
+        double f = 0.0;
+        for (int l = 0; l < LG; ++l)
+          f += As[t * LG + l].x * As[t * LG + l].x + As[t * LG + l].y * As[t * LG + l].y;
+        const double w = a.w[e];
+        double y;
+        if (a.mode == 0) {
+          double cl, d1, d2;
+          if (f > eps) {
+            cl = log(f);
+            d1 = 1.0 / f;
+            d2 = -1.0 / (f * f);
+          } else {
+            const double tt = (f - eps) / eps;
+            cl = (log(eps) + tt) - (tt * tt) / 2.0;
+            d1 = 1.0 / eps - tt / eps;
+            d2 = -1.0 / (eps * eps);
+          }
+          y = w * cl;
+          u = w * d1;
+          v = w * d2;
+        } else {
+          y = w * f;
+          u = w;
+        }
+        neumaier_add(vsum, vcomp, y);
+      }
+      for (int l = 0; l < LG; ++l) ur[t * LG + l] = u;
+      ve[t] = v;
+    }
+    // z_ej = sum_l conj(A_el) X_j,el for the 3P extended waves (zero past the end)
+    for (int idx = t; idx < E * P3; idx += 256) {
+      const int e = idx / P3, j = idx - e * P3;
+      double zr = 0.0, zi = 0.0;
+      for (int l = 0; l < LG; ++l) {
+        const double2 A = As[e * LG + l];
+        const double xr = Zs[j * ZS + e * LG + l], xi = Zs[(P3 + j) * ZS + e * LG + l];
+        zr += A.x * xr + A.y * xi;
+        zi += A.x * xi - A.y * xr;
+      }
+      Ys[j * YS + e] = zr;
+      Ys[(P3 + j) * YS + e] = zi;
+    }
+    __syncthreads();
+    if (t < Q)
+      for (int e = 0; e < E; ++e) gacc += ur[e * LG] * Ys[t * YS + e];
+    if (t < P) {
+      // F2 from q_ek = z_(0,k) / B of the wave's chain
+      const int c = a.kchain[t];
+      for (int e = 0; e < E; ++e) {
+        const int64_t ev = e0 + e;
+        if (ev >= a.N) break;
+        const double u = ur[e * LG];
+        const double2 B = a.bw[(int64_t)c * a.N + ev];
+        const double zr = Ys[t * YS + e], zi = Ys[(P3 + t) * YS + e];
+        const double den = B.x * B.x + B.y * B.y;
+        const double qr = (zr * B.x + zi * B.y) / den, qi = (zi * B.x - zr * B.y) / den;
+#pragma unroll
+        for (int tt = 0; tt < 3; ++tt) {
+          const double2 b2 = a.bw[(int64_t)((3 + tt) * nc + c) * a.N + ev];
+          f2[2 * tt] += u * (b2.x * qr - b2.y * qi);
+          f2[2 * tt + 1] += u * (b2.x * qi + b2.y * qr);
+        }
+      }
+    }
+    // O2 over the tile's R rows, 4 per MFMA step; tile q of the upper triangle on wave q % 4
+#pragma unroll
+    for (int st = 0; st < R / 4; ++st) {
+      const int rl = 4 * st + kl;
+      const double uw = ur[rl];
+      double z[QT];
+#pragma unroll
+      for (int ti = 0; ti < QT; ++ti) z[ti] = Zs[(16 * ti + il) * ZS + rl];
+      int q = 0;
+#pragma unroll
+      for (int ti = 0; ti < QT; ++ti)
+#pragma unroll
+        for (int tj = ti; tj < QT; ++tj, ++q)
+          if (q % 4 == wave)
+            o2[q / 4] = __builtin_amdgcn_mfma_f64_16x16x4f64(z[ti], uw * z[tj], o2[q / 4], 0, 0, 0);
+    }
+    if (a.mode == 0) {
+#pragma unroll
+      for (int g = 0; g < E / 4; ++g) {
+        const int ev = 4 * g + kl;
+        const double vw = ve[ev];
+        double y[QT];
+#pragma unroll
+        for (int ti = 0; ti < QT; ++ti) y[ti] = Ys[(16 * ti + il) * YS + ev];
+        int q = 0;
+#pragma unroll
+        for (int ti = 0; ti < QT; ++ti)
+#pragma unroll
+          for (int tj = ti; tj < QT; ++tj, ++q)
+            if (q % 4 == wave)
+              o1[q / 4] = __builtin_amdgcn_mfma_f64_16x16x4f64(y[ti], vw * y[tj], o1[q / 4], 0, 0, 0);
+      }
+    }
+    __syncthreads();
+  }
+  double* out = a.partials + (int64_t)blockIdx.x * a.nacc;
+  const int64_t base1 = 2 + 4 * P3, base2 = base1 + (int64_t)NT * 256;
+#pragma unroll
+  for (int q = 0; q < NT; ++q)
+    if (q % 4 == wave) {
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        out[base1 + (4 * q + r) * 64 + lane] = o1[q / 4][r];
+        out[base2 + (4 * q + r) * 64 + lane] = o2[q / 4][r];
+      }
+    }
+  double* rs = Zs;
+  double* rc = Zs + 256;
+  rs[t] = vsum;
+  rc[t] = vcomp;
+  __syncthreads();
+  for (int m = 128; m > 0; m >>= 1) {
+    if (t < m) {
+      double s1 = rs[t], c1 = rc[t];
+      neumaier_merge(s1, c1, rs[t + m], rc[t + m]);
+      rs[t] = s1;
+      rc[t] = c1;
+    }
+    __syncthreads();
+  }
+  if (t == 0) {
+    out[0] = rs[0];
+    out[1] = rc[0];
+  }
+  if (t < Q) out[t < P3 ? 2 + 2 * t : 3 + 2 * (t - P3)] = gacc;
+  if (t < P) {
+#pragma unroll
+    for (int tt = 0; tt < 3; ++tt) {
+      out[2 + 2 * P3 + 2 * (tt * P + t)] = f2[2 * tt];
+      out[3 + 2 * P3 + 2 * (tt * P + t)] = f2[2 * tt + 1];
+    }
+  }
+}
+
 // ------------------------------------------------------------------ phase space on the device
 // Counter-based random numbers: Philox4x32 with 10 rounds (J. K. Salmon et al., "Parallel
 // random numbers: as easy as 1, 2, 3", SC11). Every event draws from its own counter, so a

@@ -238,6 +238,113 @@ TPA_DEV cd lineshape(int shape, double m, double m0, double g0, double q, double
   return cd{nf * x / s, nf * y / s};
 }
 
+// Second-order forward derivatives in two variables (a: the pole mass m0, b: the width g0):
+// value, gradient and Hessian of a real function, propagated exactly through each operation.
+struct jet2 {
+  double v, a, b, aa, ab, bb;
+};
+TPA_DEV jet2 j2_const(double x) { return jet2{x, 0.0, 0.0, 0.0, 0.0, 0.0}; }
+TPA_DEV jet2 j2_add(jet2 u, jet2 w) {
+  return jet2{u.v + w.v, u.a + w.a, u.b + w.b, u.aa + w.aa, u.ab + w.ab, u.bb + w.bb};
+}
+TPA_DEV jet2 j2_sub(jet2 u, jet2 w) {
+  return jet2{u.v - w.v, u.a - w.a, u.b - w.b, u.aa - w.aa, u.ab - w.ab, u.bb - w.bb};
+}
+TPA_DEV jet2 j2_addc(jet2 u, double c) {
+  u.v += c;
+  return u;
+}
+TPA_DEV jet2 j2_scale(jet2 u, double c) {
+  return jet2{u.v * c, u.a * c, u.b * c, u.aa * c, u.ab * c, u.bb * c};
+}
+TPA_DEV jet2 j2_mul(jet2 u, jet2 w) {
+  return jet2{u.v * w.v,
+              u.a * w.v + u.v * w.a,
+              u.b * w.v + u.v * w.b,
+              u.aa * w.v + 2.0 * u.a * w.a + u.v * w.aa,
+              u.ab * w.v + u.a * w.b + u.b * w.a + u.v * w.ab,
+              u.bb * w.v + 2.0 * u.b * w.b + u.v * w.bb};
+}
+// g(u) from g(u.v), g'(u.v), g''(u.v) (the chain rule to second order)
+TPA_DEV jet2 j2_apply(jet2 u, double g, double g1, double g2) {
+  return jet2{g, g1 * u.a, g1 * u.b, g2 * u.a * u.a + g1 * u.aa, g2 * u.a * u.b + g1 * u.ab,
+              g2 * u.b * u.b + g1 * u.bb};
+}
+TPA_DEV jet2 j2_recip(jet2 u) {
+  const double r = 1.0 / u.v;
+  return j2_apply(u, r, -r * r, 2.0 * r * r * r);
+}
+TPA_DEV jet2 j2_sqrt(jet2 u) {
+  const double r = sqrt(u.v);
+  return j2_apply(u, r, 0.5 / r, -0.25 / (r * u.v));
+}
+TPA_DEV jet2 j2_powi(jet2 u, int n) {
+  const double x = u.v;
+  double p2 = 1.0;  // x^(n-2)
+  for (int i = 0; i < n - 2; ++i) p2 *= x;
+  if (n == 0) return j2_const(1.0);
+  if (n == 1) return u;
+  return j2_apply(u, p2 * x * x, n * p2 * x, n * (n - 1.0) * p2);
+}
+TPA_DEV jet2 j2_bprime_poly(int l, jet2 z) {
+  static constexpr double cof[6][6] = {{1.0},
+                                       {1.0, 1.0},
+                                       {1.0, 3.0, 9.0},
+                                       {1.0, 6.0, 45.0, 225.0},
+                                       {1.0, 10.0, 135.0, 1575.0, 11025.0},
+                                       {1.0, 15.0, 315.0, 6300.0, 99225.0, 893025.0}};
+  const int L = l < 5 ? l : 5;
+  jet2 r = j2_const(cof[L][0]);
+  for (int i = 1; i <= L; ++i) r = j2_addc(j2_mul(r, z), cof[L][i]);
+  return r;
+}
+
+// The lineshape as lineshape_grad computes it, with its first and second derivatives in
+// (m0, g0): out[0] = B, out[1] = dB/dm0, out[2] = dB/dg0, out[3] = d2B/dm0^2,
+// out[4] = d2B/dm0 dg0, out[5] = d2B/dg0^2 (the q0 clamp m_eff = max(m0, m1 + m2) of
+// get_relative_p has zero derivatives on its clamped branch, as in the reference's tape).
+TPA_DEV void lineshape_hess(int shape, double m, double m0v, double g0v, double q, double my,
+                            double mz, int L, double d, cd* out) {
+  const int kind = TPA_SHAPE_KIND(shape);
+  if (kind == 0) {
+    out[0] = cd{1.0, 0.0};
+    for (int i = 1; i < 6; ++i) out[i] = cd{0.0, 0.0};
+    return;
+  }
+  const jet2 m0 = jet2{m0v, 1.0, 0.0, 0.0, 0.0, 0.0};
+  const jet2 g0 = jet2{g0v, 0.0, 1.0, 0.0, 0.0, 0.0};
+  jet2 gamma = g0;
+  if (kind == 2) {
+    const double s = my + mz, df = my - mz;
+    const jet2 me = m0v > s ? m0 : j2_const(s);
+    const jet2 p = j2_mul(j2_mul(j2_addc(me, -s), j2_addc(me, s)),
+                          j2_mul(j2_addc(me, -df), j2_addc(me, df)));
+    const jet2 q0 = p.v > 0.0 ? j2_mul(j2_sqrt(p), j2_recip(j2_scale(me, 2.0)))
+                              : j2_const(0.0);
+    jet2 qq0 = j2_const(1.0);
+    if (q0.v > 1.0e-15) qq0 = j2_powi(j2_scale(j2_recip(q0), q), 2 * L + 1);
+    const jet2 qd0 = j2_scale(q0, d);
+    const jet2 p0 = j2_bprime_poly(L, j2_mul(qd0, qd0));
+    const double pz = bprime_poly(L, q * d * q * d);
+    const jet2 bp2 = j2_scale(p0, 1.0 / pz);
+    gamma = j2_mul(j2_mul(g0, qq0), j2_mul(j2_scale(m0, 1.0 / m), bp2));
+  }
+  const jet2 x = j2_addc(j2_mul(m0, m0), -m * m);
+  const jet2 y = j2_mul(m0, gamma);
+  const jet2 rs = j2_recip(j2_add(j2_mul(x, x), j2_mul(y, y)));
+  jet2 br = j2_mul(x, rs), bi = j2_mul(y, rs);
+  if (shape & TPA_SHAPE_NORM) {
+    br = j2_mul(br, g0);
+    bi = j2_mul(bi, g0);
+  }
+  out[0] = cd{br.v, bi.v};
+  out[1] = cd{br.a, bi.a};
+  out[2] = cd{br.b, bi.b};
+  out[3] = cd{br.aa, bi.aa};
+  out[4] = cd{br.ab, bi.ab};
+  out[5] = cd{br.bb, bi.bb};
+}
+
 // Lineshape with its derivatives in the pole mass m0 and width g0 (the reverse-mode
 // gradient of BWR / BW, tf_pwa/breit_wigner.py:56-91, 231-245, 290-306, with q0 from
 // get_relative_p's m_eff = max(m0, m1+m2) clamp, whose clamped branch has zero derivative).

@@ -212,6 +212,7 @@ def lib():
         "tpa_gram_f64": ([vp, dp, dp], ctypes.c_int),
         "tpa_calib_copy": ([i64, ctypes.c_int, dp], ctypes.c_int),
         "tpa_hess": ([vp, dp, ctypes.c_int, dp], ctypes.c_int),
+        "tpa_hess_shape": ([vp, dp, ctypes.c_int, dp], ctypes.c_int),
         "tpa_pool_hess": ([vp, i64, i64, dp, dp], ctypes.c_int),
         "tpa_pool_create": ([vp, i64, ctypes.c_int, ctypes.POINTER(vp)], ctypes.c_int),
         "tpa_pool_destroy": ([vp], ctypes.c_int),
@@ -289,5 +290,5 @@ def exported_symbols():
         "tpa_server_latencies", "tpa_server_counters", "tpa_fcn_create", "tpa_fcn_destroy",
         "tpa_fcn_nll_grad", "tpa_fcn_set_shape_map", "tpa_model_set_lineshapes", "tpa_fcn_minimize",
         "tpa_fcn_cache_rebuilds", "tpa_events_keep_f32", "tpa_fcn_set_data_storage",
-        "tpa_pool_keep_f32", "tpa_eval_noalloc",
+        "tpa_pool_keep_f32", "tpa_eval_noalloc", "tpa_hess_shape",
     ]

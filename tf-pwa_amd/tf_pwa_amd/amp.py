@@ -58,6 +58,25 @@ def bprime_poly_deriv(l, z):
     return r
 
 
+def bprime_poly_deriv2(l, z):
+    c = _BPRIME[l]
+    n = len(c) - 1
+    return sum(c[i] * (n - i) * (n - i - 1) * z ** (n - i - 2) for i in range(n - 1))
+
+
+def _rel_p2_d2_parent(M, m1, m2):
+    """d2 q^2 / dM^2 of rel_p2(M, m1, m2) = M^2/4 - (s^2 + d^2)/4 + s^2 d^2 / (4 M^2)."""
+    s, d = m1 + m2, m1 - m2
+    return 0.5 + 1.5 * s * s * d * d / M ** 4
+
+
+def _rel_p2_d2_daughter(M, m1, m2, which):
+    """d2 q^2 / dm_which^2 of rel_p2(M, m1, m2) (which 0: m1, 1: m2)."""
+    s, d = m1 + m2, m1 - m2
+    sig = 1.0 if which == 0 else -1.0
+    return (-2.0 * (M * M - d * d) - 2.0 * (M * M - s * s) + 8.0 * sig * s * d) / (4.0 * M * M)
+
+
 def _rel_p2_d_parent(M, m1, m2):
     """d rel_p2 / d M."""
     s, d = m1 + m2, m1 - m2
@@ -449,23 +468,31 @@ class AmplitudeModel:
             r_pos = 0 if d1.outs[0] is ch["R"] else 1
             dq01 = _rel_p2_d_daughter(top.mass, pm(d1.outs[0]), pm(d1.outs[1]), r_pos)
             dq02 = _rel_p2_d_parent(m0, d2.outs[0].mass, d2.outs[1].mass)
-            c1, a1, c2, a2, g1, g2 = [], [], [], [], [], []
-            for l in d1.l_list:
-                p0 = bprime_poly(l, q01 * d * d)
-                a1.append(int(p0 > 0))
-                c1.append(np.sqrt(p0) if p0 > 0 else 1.0)
-                g1.append(bprime_poly_deriv(l, q01 * d * d) * d * d * dq01 / (2 * p0)
-                          if p0 > 0 else 0.0)
-            for l in d2.l_list:
-                p0 = bprime_poly(l, q02 * d * d)
-                a2.append(int(p0 > 0))
-                c2.append(np.sqrt(p0) if p0 > 0 else 1.0)
-                g2.append(bprime_poly_deriv(l, q02 * d * d) * d * d * dq02 / (2 * p0)
-                          if p0 > 0 else 0.0)
-            out.append(dict(c1=np.array(c1), c2=np.array(c2), act1=a1, act2=a2,
-                            q01=q01, q02=q02, dlog1=np.array(g1), dlog2=np.array(g2),
-                            c=[np.array(c1), np.array(c2)], act=[a1, a2],
-                            dlog=[np.array(g1), np.array(g2)]))
+            d2q01 = _rel_p2_d2_daughter(top.mass, pm(d1.outs[0]), pm(d1.outs[1]), r_pos)
+            d2q02 = _rel_p2_d2_parent(m0, d2.outs[0].mass, d2.outs[1].mass)
+
+            def vertex(ls, q0, dq0, d2q0):
+                # sqrt(P_l(z0)), z0 = q0^2 d^2, and the first two m0 derivatives of its log
+                c, a, g, h = [], [], [], []
+                z0, z1, z2 = q0 * d * d, dq0 * d * d, d2q0 * d * d
+                for l in ls:
+                    p0 = bprime_poly(l, z0)
+                    a.append(int(p0 > 0))
+                    c.append(np.sqrt(p0) if p0 > 0 else 1.0)
+                    if p0 > 0:
+                        p1, p2 = bprime_poly_deriv(l, z0), bprime_poly_deriv2(l, z0)
+                        g.append(0.5 * p1 * z1 / p0)
+                        h.append(0.5 * ((p2 * z1 * z1 + p1 * z2) / p0 - (p1 * z1 / p0) ** 2))
+                    else:
+                        g.append(0.0)
+                        h.append(0.0)
+                return np.array(c), a, np.array(g), np.array(h)
+
+            c1, a1, g1, h1 = vertex(d1.l_list, q01, dq01, d2q01)
+            c2, a2, g2, h2 = vertex(d2.l_list, q02, dq02, d2q02)
+            out.append(dict(c1=c1, c2=c2, act1=a1, act2=a2, q01=q01, q02=q02, dlog1=g1,
+                            dlog2=g2, c=[c1, c2], act=[a1, a2], dlog=[g1, g2],
+                            d2log=[h1, h2]))
         return out
 
     def describe(self, fold=True):
@@ -692,14 +719,15 @@ class AmplitudeModel:
                 out = out * val[sl]
         return out
 
-    def jacobian(self):
+    def jacobian(self, couplings_only=False):
         """J[i, k] = d c_k / d trainable_vars[i] (complex) for the coupling parameters.
-        Raises if a lineshape parameter is trainable (needs the shape-gradient path)."""
+        Raises if a lineshape parameter is trainable (needs the shape-gradient path), unless
+        ``couplings_only``: then the rows of masses and widths are left zero."""
         vm = self.vm
         index = vm.index_map()
         for nm in self._names():
             for key in ("mass", "width"):
-                if nm[key] in index:
+                if nm[key] in index and not couplings_only:
                     raise NotImplementedError(
                         "gradients with respect to floating masses/widths use the shape path")
         fx = self._factor_index()
@@ -759,13 +787,18 @@ class AmplitudeModel:
         H = 2 Re[J Fcc J^T + J Fcb J^H + sum_k F1_k d2c_k/dtheta dtheta] (J = jacobian())."""
         J = self.jacobian()
         H = 2.0 * np.real(J @ Fcc @ J.T + J @ Fcb @ J.conj().T)
+        return H + 2.0 * np.real(self._coupling_second(F1, H.shape[0]))
+
+    def _coupling_second(self, F1, n):
+        """sum_k F1_k d2 c_k / dtheta dphi over the coupling parameters (complex [n, n]):
+        the second derivatives of each c_k's polar factors and the products of the first
+        derivatives of two of its factors."""
         row_r, row_i = self._grad_rows()
         fx = self._factor_index()
         val, d_r, d_i = self._factor_values()
         polar = np.array([self.vm.complex_vars.get(n, self.vm.polar) for n in fx["cnames"]])
         C = self._barrier_per_wave() * np.asarray(F1)
         slots = fx["slots"]
-        n = H.shape[0]
         M = np.zeros((n, n), dtype=np.complex128)
         i_unit = 1j
         for a in range(len(slots)):
@@ -791,7 +824,77 @@ class AmplitudeModel:
                         ra, rb = rows_a[f], rows_b[g]
                         m = (ra >= 0) & (rb >= 0)
                         np.add.at(M, (ra[m], rb[m]), da[f][m] * db[g][m] * rest2[m])
-        return H + 2.0 * np.real(M)
+        return M
+
+    def _shape_rows(self):
+        """Per partial wave: the trainable row of its chain's mass and width (-1 if fixed),
+        the m0 derivative of its log barrier constant and the second derivative."""
+        fx = self._factor_index()
+        index = self.vm.index_map()
+        names = self._names()
+        mrow = np.array([index.get(nm["mass"], -1) for nm in names])[fx["ch"]]
+        wrow = np.array([index.get(nm["width"], -1) for nm in names])[fx["ch"]]
+        idx = self._wave_barrier_index()
+        dlog = self._barrier_flat("dlog")[idx].sum(axis=1)
+        d2log = self._barrier_flat("d2log")[idx].sum(axis=1)
+        return mrow, wrow, dlog, d2log
+
+    def hess_shape_from_adjoint(self, F1, Fcc, Fcb):
+        """Hessian in every trainable parameter (couplings, masses, widths) of a real function
+        S of the amplitude when the lineshapes float, from its Wirtinger derivatives in the
+        extended coefficients alpha of A = sum_j alpha_j X_j, X_(t,k) = b^t h_k with the
+        lineshape factors b = (B, dB/dm0, dB/dg0, d2B/dm0^2, d2B/dm0dg0, d2B/dg0^2): F1 (6P),
+        Fcc and Fcb ([3P, 3P], the first three blocks; the others enter A at second order
+        only). At the current point alpha = (c, 0, ...) and
+          H = 2 Re[J Fcc J^T + J Fcb J^H + sum_j F1_j d2 alpha_j / dtheta dphi],
+        with J the first derivatives: d alpha_(0,k) = d c_k (couplings, and c_k dlog_k for the
+        chain's mass through the barrier constants), d alpha_(1,k) / dm0 = d alpha_(2,k) / dg0
+        = c_k; second derivatives: d2 c_k in the couplings, dlog_k dc_k / dtheta (mass x
+        coupling), c_k (dlog_k^2 + d2log_k) (mass x mass), and for the lineshape blocks the
+        products of their factors (tf_pwa/model/model.py:97-150 computes the same with nested
+        tapes)."""
+        P = self.P
+        n = len(self.vm.trainable_vars)
+        F1 = np.asarray(F1)
+        F0, Fm, Fg, Fmm, Fmg, Fgg = (F1[i * P:(i + 1) * P] for i in range(6))
+        c = self.coefficients()
+        mrow, wrow, dlog, d2log = self._shape_rows()
+        Jc = self.jacobian(couplings_only=True)  # coupling rows of d c_k
+        Jf = Jc.copy()                           # every row of d c_k (mass rows added)
+        ks = np.arange(P)
+        m = mrow >= 0
+        np.add.at(Jf, (mrow[m], ks[m]), c[m] * dlog[m])
+        J = np.zeros((n, 3 * P), dtype=np.complex128)
+        J[:, :P] = Jf
+        np.add.at(J, (mrow[m], P + ks[m]), c[m])
+        mw = wrow >= 0
+        np.add.at(J, (wrow[mw], 2 * P + ks[mw]), c[mw])
+        H = 2.0 * np.real(J @ Fcc @ J.T + J @ Fcb @ J.conj().T)
+        M = self._coupling_second(F0, n)
+        # block 0: the barrier constants' m0 dependence
+        for k in np.nonzero(m)[0]:
+            r = mrow[k]
+            x = F0[k] * dlog[k] * Jc[:, k]
+            M[r, :] += x
+            M[:, r] += x
+            M[r, r] += F0[k] * c[k] * (dlog[k] ** 2 + d2log[k])
+        # blocks dB/dm0 and dB/dg0: d2 (c_k delta) = dc_k/dphi [theta = m0] + (theta <-> phi)
+        for k in range(P):
+            if mrow[k] >= 0:
+                x = Fm[k] * Jf[:, k]
+                M[mrow[k], :] += x
+                M[:, mrow[k]] += x
+                M[mrow[k], mrow[k]] += Fmm[k] * c[k]
+            if wrow[k] >= 0:
+                x = Fg[k] * Jf[:, k]
+                M[wrow[k], :] += x
+                M[:, wrow[k]] += x
+                M[wrow[k], wrow[k]] += Fgg[k] * c[k]
+            if mrow[k] >= 0 and wrow[k] >= 0:
+                M[mrow[k], wrow[k]] += Fmg[k] * c[k]
+                M[wrow[k], mrow[k]] += Fmg[k] * c[k]
+        H = H + 2.0 * np.real(M)
+        return 0.5 * (H + H.T)
 
     def shape_grad(self, G, Dm, Dg):
         """Gradient of the lineshape parameters: for the resonance of chain c,

@@ -222,6 +222,31 @@ class EventSet:
         _lib.check(_lib.lib().tpa_hess(self._h, _lib.dptr(c), mode, _lib.dptr(out)))
         return decode_second_moments(out, P)
 
+    def hess_shape(self, coef, mode):
+        """Second-moment pass with per-event lineshapes (tpa_hess_shape, fold-0 cache):
+        (value, F, F2, Fcc, Fcb) over the extended waves X_(t,k) = b^t h_k (b^t the wave's
+        lineshape B, dB/dm0, dB/dg0): F_j = d sum / d alpha_j (3P), F2 the moments of the
+        second lineshape derivatives (3P: d2/dm0^2, d2/dm0dg0, d2/dg0^2 blocks), Fcc and Fcb
+        the Wirtinger second derivatives in the extended coefficients ([3P, 3P])."""
+        self.ensure_built()
+        if self._state[0]:
+            raise ValueError("lineshape second moments need a fold-0 cache (floating shapes)")
+        P = self.amp.P
+        P3, Q = 3 * P, 6 * P
+        T = Q * (Q + 1) // 2
+        c = np.ascontiguousarray(np.asarray(coef, dtype=np.complex128)).view(np.float64)
+        out = np.zeros(2 + 12 * P + 2 * T, dtype=np.float64)
+        _lib.check(_lib.lib().tpa_hess_shape(self._h, _lib.dptr(c), mode, _lib.dptr(out)))
+        F = out[2:2 + 6 * P].view(np.complex128).copy()
+        F2 = out[2 + 6 * P:2 + 12 * P].view(np.complex128).copy()
+        O1 = _unpack_upper(out[2 + 12 * P:2 + 12 * P + T], Q)
+        O2 = _unpack_upper(out[2 + 12 * P + T:2 + 12 * P + 2 * T], Q)
+        a, b = slice(0, P3), slice(P3, Q)
+        Fcc = (O1[a, a] - O1[b, b]) + 1j * (O1[a, b] + O1[b, a])
+        Fcb = ((O1[a, a] + O1[b, b]) + 1j * (O1[b, a] - O1[a, b])
+               + (O2[a, a] + O2[b, b]) + 1j * (O2[b, a] - O2[a, b]))
+        return out[0], F, F2, Fcc, Fcb
+
     def gram(self, precision="fp64"):
         """Partial-wave Gram matrix I_kl = sum_e w_e sum_lambda conj(h_k) h_l, cached until the
         partial waves or the weights change. precision "fp32": FP32 MFMA with FP64 refolds (its

@@ -225,6 +225,53 @@ class Model:
         H = amp.hess_from_adjoint(F1, Fcc, Fcb)
         return nll, g, 0.5 * (H + H.T)
 
+    def _nll_grad_hess_shape(self, wdata, mc_sets, mc_w, split=None):
+        """NLL, gradient and exact Hessian when masses or widths float: one lineshape
+        second-moment pass per data and MC set (tpa_hess_shape) gives the value and the
+        Wirtinger derivatives in the extended coefficients, combined as in _nll_grad_hess
+        (h = -h_data + sw (h_mc / S - g_mc g_mc^T / S^2), tf_pwa/model/model.py:513-560) and
+        chain-ruled to couplings, masses and widths (AmplitudeModel.hess_shape_from_adjoint)."""
+        amp = self.Amp
+        coef = amp.coefficients()
+        if split is None:
+            split = (wdata.split(), self._split(mc_sets, mc_w), wdata.weight.sum())
+        dw, mw, sw = split
+        P3 = 3 * amp.P
+
+        def zero():
+            return [0.0, np.zeros(2 * P3, dtype=np.complex128),
+                    np.zeros((P3, P3), dtype=np.complex128),
+                    np.zeros((P3, P3), dtype=np.complex128)]
+
+        def run(sets, weights, mode):
+            acc = zero()
+            for st, w in zip(sets, weights):
+                st.set_weight(w)
+                v, F, F2, cc, cb = st.hess_shape(coef, mode)
+                acc[0] += v
+                acc[1] += np.concatenate([F, F2])
+                acc[2] += cc
+                acc[3] += cb
+            return acc
+
+        D, DF, Dcc, Dcb = run(wdata.sets, dw, _lib.MODE_DATA)
+        S, SF, _, Scb = run(mc_sets, mw, _lib.MODE_MC)
+        self.n_eval += 1
+        nll = -D + sw * np.log(S)
+        F1 = -DF + (sw / S) * SF
+        s1 = SF[:P3]
+        Fcc = -Dcc - (sw / S ** 2) * np.outer(s1, s1)
+        Fcb = -Dcb + sw * (Scb / S - np.outer(s1, np.conj(s1)) / S ** 2)
+        P = amp.P
+        g = amp.grad_from_adjoint(F1[:P]) + amp.shape_grad(F1[:P], F1[P:2 * P], F1[2 * P:P3])
+        H = amp.hess_shape_from_adjoint(F1, Fcc, Fcb)
+        return nll, g, H
+
+    def _hess_any(self, wdata, mc_sets, mc_w, split=None):
+        if self.Amp.shape_mode:
+            return self._nll_grad_hess_shape(wdata, mc_sets, mc_w, split=split)
+        return self._nll_grad_hess(wdata, mc_sets, mc_w, split=split)
+
     def nll(self, data, mcdata, weight=1.0, batch=None, bg=None, mc_weight=1.0):
         wdata, _ = self.get_weight_data(data, None if np.isscalar(weight) and weight == 1.0
                                         else weight, bg=bg)
@@ -247,55 +294,26 @@ class Model:
                 if isinstance(mc_weight, (list, tuple)) else np.asarray(mc_weight))
         return self._nll_grad(wdata, mc_sets, mc_w)
 
-    def nll_grad_hessian(self, data, mcdata, weight=1.0, batch=24000, bg=None, mc_weight=1.0,
-                         eps=1e-5):
-        """NLL, gradient and Hessian (model.py:513-560). Exact for coupling parameters; with
-        floating masses or widths the Hessian is the central difference of the analytic
-        gradient with a relative step ``eps`` per trainable parameter."""
-        if not self.Amp.shape_mode:
-            wdata, _ = self.get_weight_data(data, None if np.isscalar(weight) and weight == 1.0
-                                            else weight, bg=bg)
-            mc_sets, mc_w = self._mc_weight(mcdata,
-                                            None if np.isscalar(mc_weight) else mc_weight)
-            return self._nll_grad_hess(wdata, mc_sets, mc_w)
-        nll, g = self.nll_grad(data, mcdata, weight=weight, bg=bg, mc_weight=mc_weight)
-        x0 = np.array(self.vm.get_all_val())
-        n = len(x0)
-        H = np.zeros((n, n))
-        for i in range(n):
-            h = eps * max(1.0, abs(x0[i]))
-            for sgn in (1, -1):
-                x = x0.copy()
-                x[i] += sgn * h
-                self.vm.set_all(x)
-                _, gi = self.nll_grad(data, mcdata, weight=weight, bg=bg, mc_weight=mc_weight)
-                H[:, i] += sgn * gi / (2 * h)
-        self.vm.set_all(x0)
-        H = 0.5 * (H + H.T)
-        return nll, g, H
+    def nll_grad_hessian(self, data, mcdata, weight=1.0, batch=24000, bg=None, mc_weight=1.0):
+        """NLL, gradient and exact Hessian (model.py:513-560) in every trainable parameter:
+        second-moment passes for couplings, lineshape second-moment passes when masses or
+        widths float."""
+        wdata, _ = self.get_weight_data(data, None if np.isscalar(weight) and weight == 1.0
+                                        else weight, bg=bg)
+        mc_sets, mc_w = self._mc_weight(mcdata, None if np.isscalar(mc_weight) else mc_weight)
+        return self._hess_any(wdata, mc_sets, mc_w)
 
     def grad_hessp_batch(self, p, data, mcdata, weight, mc_weight):
         """Gradient and Hessian-vector product H p (BaseModel.grad_hessp_batch,
-        tf_pwa/model/model.py:451-511): exact for couplings; central difference of the gradient
-        along p when masses or widths float."""
+        tf_pwa/model/model.py:451-511) from the exact Hessian."""
         wdata = WeightedData(_as_sets(data), np.concatenate([np.atleast_1d(w) for w in weight])
                              if isinstance(weight, (list, tuple)) else weight)
         mc_sets = _as_sets(mcdata)
         mc_w = (np.concatenate([np.atleast_1d(w) for w in mc_weight])
                 if isinstance(mc_weight, (list, tuple)) else np.asarray(mc_weight))
         p = np.asarray(p, dtype=np.float64)
-        if not self.Amp.shape_mode:
-            _, g, H = self._nll_grad_hess(wdata, mc_sets, mc_w)
-            return g, H @ p
-        _, g = self._nll_grad(wdata, mc_sets, mc_w)
-        x0 = np.array(self.vm.get_all_val())
-        h = 1e-6 * max(1.0, np.linalg.norm(x0)) / max(np.linalg.norm(p), 1e-300)
-        self.vm.set_all(x0 + h * p)
-        _, gp = self._nll_grad(wdata, mc_sets, mc_w)
-        self.vm.set_all(x0 - h * p)
-        _, gm = self._nll_grad(wdata, mc_sets, mc_w)
-        self.vm.set_all(x0)
-        return g, (gp - gm) / (2 * h)
+        _, g, H = self._hess_any(wdata, mc_sets, mc_w)
+        return g, H @ p
 
     def set_params(self, var):
         self.Amp.set_params(var)
@@ -520,33 +538,14 @@ class FCN:
     def grad(self, x=None):
         return self.nll_grad(x)[1]
 
-    def nll_grad_hessian(self, x=None, batch=None, eps=1e-5):
-        """Exact Hessian for couplings (second-moment passes); central differences of the
-        gradient when masses or widths float (FCN.nll_grad_hessian, model.py:1258-1265)."""
+    def nll_grad_hessian(self, x=None, batch=None):
+        """NLL, gradient and exact Hessian in every trainable parameter (FCN.nll_grad_hessian,
+        model.py:1258-1265): second-moment passes for couplings, lineshape second-moment
+        passes when masses or widths float, plus the Gaussian constraints' curvature."""
         self._set(x)
-        if not self.model.Amp.shape_mode:
-            nll, g, H = self.model._nll_grad_hess(self.wdata, self.mc_sets, self.mc_weight,
-                                                  split=self._split)
-            self.n_call += 1
-            cv, cg = self._constr()
-            for name, (mu, sigma) in self.gauss_constr.items():
-                if name in self.vm.trainable_vars:
-                    i = self.vm.trainable_vars.index(name)
-                    H[i, i] += 1.0 / sigma ** 2
-            return nll + cv, g + cg, H
-        nll, g = self.get_nll_grad()
-        x0 = np.array(self.vm.get_all_val())
-        n = len(x0)
-        H = np.zeros((n, n))
-        for i in range(n):
-            h = eps * max(1.0, abs(x0[i]))
-            for sgn in (1, -1):
-                xx = x0.copy()
-                xx[i] += sgn * h
-                _, gi = self.get_nll_grad(xx)
-                H[:, i] += sgn * gi / (2 * h)
-        self.vm.set_all(x0)
-        H = 0.5 * (H + H.T)
+        nll, g, H = self.model._hess_any(self.wdata, self.mc_sets, self.mc_weight,
+                                         split=self._split)
+        self.n_call += 1
         cv, cg = self._constr()
         for name, (mu, sigma) in self.gauss_constr.items():
             if name in self.vm.trainable_vars:
@@ -554,18 +553,10 @@ class FCN:
                 H[i, i] += 1.0 / sigma ** 2
         return nll + cv, g + cg, H
 
-    def grad_hessp(self, x, p, batch=None, eps=1e-6):
-        """Gradient and Hessian-vector product H p (FCN.grad_hessp, model.py:1267-1275): exact
-        for couplings; the central difference of the gradient along p when masses or widths
-        float."""
+    def grad_hessp(self, x, p, batch=None):
+        """Gradient and Hessian-vector product H p (FCN.grad_hessp, model.py:1267-1275) from
+        the exact Hessian."""
         x = np.asarray(x, dtype=np.float64)
         p = np.asarray(p, dtype=np.float64)
-        if not self.model.Amp.shape_mode:
-            _, g, H = self.nll_grad_hessian(x)
-            return g, H @ p
-        _, g = self.nll_grad(x)
-        h = eps * max(1.0, np.linalg.norm(x)) / max(np.linalg.norm(p), 1e-300)
-        _, gp = self.get_nll_grad(x + h * p)
-        _, gm = self.get_nll_grad(x - h * p)
-        self.vm.set_all(x)
-        return g, (gp - gm) / (2 * h)
+        _, g, H = self.nll_grad_hessian(x)
+        return g, H @ p
