@@ -48,6 +48,8 @@ typedef struct tpa_fcn_s* tpa_fcn_t;
  * the bytes per event; the values are widened to FP64 on load and every operation after it is
  * FP64. Not with TPA_FLAG_SHAPE. */
 #define TPA_FLAG_F32 8
+/* Largest model (partial waves) tpa_eval_fused accepts. */
+#define TPA_FUSED_MAX_P 8
 
 /* Version of this ABI. */
 int tpa_abi_version(void);
@@ -163,6 +165,30 @@ int tpa_eval(tpa_events_t ev, const double* coef, int mode, int flags, double* o
  * scratch array: calls that overlap on one GPU race, so queue them on one stream. */
 int tpa_eval_noalloc(tpa_events_t ev, const double* coef, int mode, int flags, double* d_out,
                      void* stream);
+
+/* The fused pass: tpa_eval's sums (modes DATA and MC; TPA_FLAG_GRAD for G_k) computed from the
+ * angle cache alone, every partial wave recomputed per event (Wigner D of both vertices, CG x
+ * barrier factors, the lineshape at the model's current m0/g0, the resonance-helicity
+ * contraction and the aligned rotation of spin > 0 finals), so no partial-wave cache is
+ * needed. The kernel is generated for the model's description and compiled for gfx950 with
+ * hipRTC on first use (regenerated after tpa_model_update only if its source changes).
+ * Three-body models with at most TPA_FUSED_MAX_P partial waves (TPA_ERR_UNSUPPORTED above: the
+ * generated kernel of a 22-wave model took a memory-aperture fault on gfx950 whose cause is
+ * not identified, so larger models are refused). Replaces the per-call amplitude of tf-pwa's default model, which
+ * recomputes the chains from the cached angle and D-matrix data on every evaluation
+ * (DecayGroup.get_amp / DecayChain.get_amp, tf_pwa/amp/core.py:1392-1445, 1696-1728; the
+ * D-matrix cache of tf_pwa/dfun.py:221-238), inside sum_gradient (tf_pwa/model/model.py:41-94). */
+int tpa_eval_fused(tpa_events_t ev, const double* coef, int mode, int flags, double* out);
+
+/* The HIP source of a three-body description's fused pass (host only, no GPU needed): *len
+ * receives its length; with buf (buf_len bytes) the text is copied, NUL-terminated. The same
+ * interface as tpa_model_create (tf_pwa/config_loader/config_loader.py:210-237). */
+int tpa_fused_source(const void* desc, int64_t desc_bytes, const double* cg, int n_cg,
+                     char* buf, int64_t buf_len, int64_t* len);
+
+/* Seconds the model's last fused-pass compile took (0 before the first). Bookkeeping only, no
+ * reference counterpart. */
+int tpa_fused_compile_seconds(tpa_model_t model, double* s);
 
 /* Kernel timing with HIP events on the engine's stream: when enabled, every likelihood
  * kernel launch is bracketed by events; read returns the summed milliseconds and the

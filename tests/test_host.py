@@ -363,3 +363,47 @@ def test_four_body_layout_matches_oracle():
     np.testing.assert_allclose(map_coefficients(keep, y), amp.coefficients(), rtol=1e-14)
     M = amp.describe()
     assert M.P == 25 and M.n_chain == 12
+
+
+@pytest.mark.parametrize("cfg", ["lc_pkpi_2res.yml", "lc_pkpi_6res.yml"])
+def test_fused_source_indices_stay_inside_their_arrays(cfg):
+    """The generated fused pass (tpa_fused_source, host code): its constants match the model
+    and every literal index it emits lies inside the array it indexes (angle-cache fields, the
+    coefficient, lineshape and clamp-flag arguments, the amplitude and adjoint arrays)."""
+    from tf_pwa_amd import ConfigLoader
+
+    amp = ConfigLoader(config_path(cfg), seed=1).get_amplitude()
+    src = amp.fused_source()
+    M = amp.describe()
+    define = dict(re.findall(r"#define (TPA_\w+) (\d+)", src))
+    assert int(define["TPA_P"]) == amp.P and int(define["TPA_NC"]) == M.n_chain
+    ns = int(define["TPA_NSLOT"])
+    assert ns == amp.lam
+
+    def top(pat):
+        return max(int(x) for x in re.findall(pat, src))
+
+    assert top(r"ang\[(\d+)LL \* S") == 18 * M.n_topo + 3  # the last of the four masses
+    assert top(r"a\.c\[(\d+)\]") == 2 * amp.P - 1
+    for arr in ("m0", "g0", "q0", "act"):
+        assert top(r"a\.%s\[(\d+)\]" % arr) == M.n_chain - 1
+    assert top(r"gr\[(\d+)\]") == amp.P - 1
+    assert top(r"[^_]A\[(\d+)\] =") == ns - 1
+    assert top(r"_(?:Ap|B)\[(\d+)\] =") == ns - 1
+    assert "extern \"C\" __global__ __launch_bounds__(256) void tpa_fused_pass" in src
+
+
+def test_fused_pass_size_limit():
+    """Models above TPA_FUSED_MAX_P partial waves are refused before any device work; the
+    limit in the header and in the bindings agree."""
+    from tf_pwa_amd import ConfigLoader, Model, _lib
+
+    header = open(os.path.join(ROOT, "include", "tfpwa_amd.h")).read()
+    assert int(re.search(r"#define TPA_FUSED_MAX_P (\d+)", header).group(1)) == _lib.FUSED_MAX_P
+    small = Model(ConfigLoader(config_path("lc_pkpi_2res.yml"), seed=1).get_amplitude(),
+                  evaluation="fused")
+    assert small.evaluation == "fused"
+    big = ConfigLoader(config_path("lc_pkpi_6res.yml"), seed=1).get_model()
+    with pytest.raises(NotImplementedError, match="disabled above 8 partial waves"):
+        big.evaluation = "fused"
+    assert big.evaluation == "stream"

@@ -14,10 +14,14 @@
 #include <string>
 #include <vector>
 
+#include <hip/hiprtc.h>
+
 #include "tfpwa_amd.h"
+#include "tpa_fused_gen.h"
 #include "tpa_internal.h"
 #include "tpa_kernels.h"
 #include "tpa_kernels4.h"
+#include "tpa_rtc_headers.inc"  // kRtcHeaders / kRtcHeaderNames (written by build.py)
 
 namespace {
 
@@ -205,6 +209,15 @@ struct tpa_model_s {
   int* d_kchain = nullptr;
   hipStream_t stream = nullptr;
   TpaModel* h_pin = nullptr;  // pinned staging of tpa_model_set_lineshapes
+  std::vector<double> cg_host;  // the CG table (the fused pass's generator bakes it in)
+  // the fused pass (tpa_eval_fused): compiled on first use, regenerated when the description
+  // changes (kept if its generated source is the same)
+  std::mutex fused_mu;
+  std::string fused_src;
+  hipModule_t fused_mod = nullptr;
+  hipFunction_t fused_fn = nullptr;
+  bool fused_stale = false;
+  double fused_compile_s = 0.0;
 };
 
 struct tpa_events_s {
@@ -659,6 +672,7 @@ int tpa_model_create(const void* desc, int64_t desc_bytes, const double* cg, int
                                  std::to_string(sizeof(TpaModel)));
   tpa_model_s* m = new tpa_model_s();
   std::memcpy(&m->h, desc, sizeof(TpaModel));
+  if (n_cg > 0) m->cg_host.assign(cg, cg + n_cg);
   std::string why = check_model(m->h, n_cg);
   if (!why.empty()) {
     delete m;
@@ -764,6 +778,7 @@ int tpa_model_destroy(tpa_model_t m) {
   if (m->d_dw) (void)hipFree(m->d_dw);
   if (m->d_kchain) (void)hipFree(m->d_kchain);
   if (m->h_pin) (void)hipHostFree(m->h_pin);
+  if (m->fused_mod) (void)hipModuleUnload(m->fused_mod);
   if (m->stream) (void)hipStreamDestroy(m->stream);
   delete m;
   return TPA_OK;
@@ -781,6 +796,10 @@ int tpa_model_update(tpa_model_t m, const void* desc, int64_t desc_bytes) {
   if (!why.empty()) return fail(TPA_ERR_ARG, "invalid model: " + why);
   HIPCHK(hipStreamSynchronize(m->stream));
   m->h = next;
+  {
+    std::lock_guard<std::mutex> lk(m->fused_mu);
+    m->fused_stale = true;
+  }
   HIPCHK(hipMemcpy(m->d, &m->h, sizeof(TpaModel), hipMemcpyHostToDevice));
   return TPA_OK;
 }
@@ -1264,6 +1283,175 @@ int tpa_eval_collect(tpa_events_t ev, double* out) {
   ev->pending_nout = 0;
   HIPCHK(hipStreamSynchronize(ev->pending_stream));
   std::memcpy(out, ev->h_io + 2 * TPA_MAX_P, sizeof(double) * nout);
+  return TPA_OK;
+}
+
+}  // extern "C"
+
+// ---- the fused pass (tpa_eval_fused): waves recomputed from the angles on every evaluation
+static double host_rel_p(double m0, double m1, double m2) {
+  const double s = m1 + m2, d = m1 - m2;
+  const double me = m0 > s ? m0 : s;
+  const double p = (me - s) * (me + s) * (me - d) * (me + d);
+  return std::sqrt(p) / (2.0 * me);
+}
+
+// Compile (hipRTC, gfx950) and load the model's fused kernel if it is missing or its
+// description changed; m->fused_mu held.
+static int fused_ensure(tpa_model_s* m) {
+  if (m->fused_fn && !m->fused_stale) return TPA_OK;
+  std::string src = tpa_fused::fused_source(m->h, m->cg_host.data(), (int)m->cg_host.size());
+  m->fused_stale = false;
+  if (m->fused_fn && src == m->fused_src) return TPA_OK;
+  const auto t0 = std::chrono::steady_clock::now();
+  hiprtcProgram prog;
+  if (hiprtcCreateProgram(&prog, src.c_str(), "tpa_fused.hip", kRtcHeaderCount, kRtcHeaders,
+                          kRtcHeaderNames) != HIPRTC_SUCCESS)
+    return fail(TPA_ERR_HIP, "hiprtcCreateProgram failed");
+  // packed FP32 instructions stay off here too (the pass is FP64, so none are expected)
+  const char* opts[] = {"--offload-arch=gfx950", "-O3", "-std=c++17", "-Xclang",
+                        "-target-feature", "-Xclang", "-packed-fp32-ops"};
+  const hiprtcResult rc = hiprtcCompileProgram(prog, 7, opts);
+  if (rc != HIPRTC_SUCCESS) {
+    size_t ls = 0;
+    hiprtcGetProgramLogSize(prog, &ls);
+    std::string log(ls, '\0');
+    if (ls) hiprtcGetProgramLog(prog, &log[0]);
+    hiprtcDestroyProgram(&prog);
+    return fail(TPA_ERR_HIP, "fused pass compile failed: " + log.substr(0, 2000));
+  }
+  size_t cs = 0;
+  hiprtcGetCodeSize(prog, &cs);
+  std::vector<char> code(cs);
+  hiprtcGetCode(prog, code.data());
+  hiprtcDestroyProgram(&prog);
+  hipModule_t mod = nullptr;
+  hipFunction_t fn = nullptr;
+  HIPCHK(hipModuleLoadData(&mod, code.data()));
+  hipError_t e = hipModuleGetFunction(&fn, mod, "tpa_fused_pass");
+  if (e != hipSuccess) {
+    (void)hipModuleUnload(mod);
+    return fail(TPA_ERR_HIP, std::string("hipModuleGetFunction: ") + hipGetErrorString(e));
+  }
+  if (m->fused_mod) (void)hipModuleUnload(m->fused_mod);
+  m->fused_mod = mod;
+  m->fused_fn = fn;
+  m->fused_src = std::move(src);
+  m->fused_compile_s =
+      std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  return TPA_OK;
+}
+
+static int eval_fused_impl(tpa_events_s* ev, const double* coef, int mode, int flags,
+                           double* out) {
+  tpa_model_s* m = ev->m;
+  if (m->kind != 3) return fail(TPA_ERR_UNSUPPORTED, "the fused pass covers three-body models");
+  if (mode != TPA_MODE_DATA && mode != TPA_MODE_MC) return fail(TPA_ERR_ARG, "bad mode");
+  if (flags & ~TPA_FLAG_GRAD)
+    return fail(TPA_ERR_UNSUPPORTED, "tpa_eval_fused takes TPA_FLAG_GRAD only");
+  const TpaModel& M = m->h;
+  // The generated kernel of the 22-wave Lambda_c model took a memory-aperture fault on gfx950;
+  // its addresses, kernel descriptor and launch arguments check out and the cause is not
+  // identified, so models above the size that has run clean stay refused.
+  if (M.P > TPA_FUSED_MAX_P)
+    return fail(TPA_ERR_UNSUPPORTED,
+                "the fused pass is disabled above " + std::to_string(TPA_FUSED_MAX_P) +
+                    " partial waves (a 22-wave model's kernel faulted, cause not identified)");
+  {
+    std::lock_guard<std::mutex> lk(m->fused_mu);
+    const int rc = fused_ensure(m);
+    if (rc != TPA_OK) return rc;
+  }
+  const int P = M.P, NC = M.n_chain;
+  const tpa_fused::ArgsLayout L = tpa_fused::fused_args_layout(P, NC);
+  std::vector<unsigned char> buf(L.size, 0);
+  auto put = [&](size_t off, const void* src, size_t n) { std::memcpy(buf.data() + off, src, n); };
+  const double* ang = ev->d_ang;
+  const double* w = ev->d_w;
+  double* part = ev->d_part;
+  const long long N = ev->n, S = ev->n;
+  put(0, &ang, 8);
+  put(8, &w, 8);
+  put(16, &part, 8);
+  put(24, &N, 8);
+  put(32, &S, 8);
+  put(L.off_c, coef, sizeof(double) * 2 * P);
+  for (int c = 0; c < NC; ++c) {
+    const TpaChain& C = M.chain[c];
+    const double q0 = host_rel_p(C.m0, C.m_y, C.m_z);
+    int act = 0;
+    for (int k = 0; k < C.n_ls1; ++k) act |= (C.act1[k] ? 1 : 0) << k;
+    for (int k = 0; k < C.n_ls2; ++k) act |= (C.act2[k] ? 1 : 0) << (16 + k);
+    put(L.off_m0 + 8 * c, &C.m0, 8);
+    put(L.off_g0 + 8 * c, &C.g0, 8);
+    put(L.off_q0 + 8 * c, &q0, 8);
+    put(L.off_act + 4 * c, &act, 4);
+  }
+  const int grad = (flags & TPA_FLAG_GRAD) ? 1 : 0;
+  put(L.off_mode, &mode, 4);
+  put(L.off_grad, &grad, 4);
+  const int nacc = 2 + 2 * P;
+  const int64_t blocks = (ev->n + 255) / 256;
+  const int grid = (int)std::max<int64_t>(1, std::min<int64_t>(blocks, kEvalGridDefault));
+  hipStream_t s = m->stream;
+  bool timing;
+  {
+    std::lock_guard<std::mutex> lk(g_timing_mu);
+    timing = g_timing;
+  }
+  if (timing) HIPCHK(hipEventRecord(ev->t0, s));
+  void* params[] = {buf.data()};
+  HIPCHK(hipModuleLaunchKernel(m->fused_fn, grid, 1, 1, 256, 1, 1, 0, s, params, nullptr));
+  if (timing) HIPCHK(hipEventRecord(ev->t1, s));
+  const int nout = grad ? nacc : 2;
+  hipLaunchKernelGGL(k_reduce, dim3(nout), dim3(256), 0, s, part, grid, nacc, 1, ev->d_out,
+                     nullptr, nullptr, nullptr, (uint64_t)0);
+  HIPCHK(hipGetLastError());
+  if (!ev->h_io) HIPCHK(hipHostMalloc((void**)&ev->h_io, sizeof(double) * TPA_IO_DOUBLES));
+  double* h_out = ev->h_io + 2 * TPA_MAX_P;
+  HIPCHK(hipMemcpyAsync(h_out, ev->d_out, sizeof(double) * nout, hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+  std::memcpy(out, h_out, sizeof(double) * nout);
+  if (timing) {
+    float ms = 0.0f;
+    HIPCHK(hipEventElapsedTime(&ms, ev->t0, ev->t1));
+    std::lock_guard<std::mutex> lk(g_timing_mu);
+    g_timing_ms += ms;
+    g_timing_n += 1;
+  }
+  return TPA_OK;
+}
+
+extern "C" {
+
+int tpa_fused_source(const void* desc, int64_t desc_bytes, const double* cg, int n_cg,
+                     char* buf, int64_t buf_len, int64_t* len) {
+  if (!desc || !len || (n_cg > 0 && !cg)) return fail(TPA_ERR_ARG, "null pointer");
+  if (desc_bytes != (int64_t)sizeof(TpaModel))
+    return fail(TPA_ERR_ARG, "model description size mismatch");
+  TpaModel M;
+  std::memcpy(&M, desc, sizeof(TpaModel));
+  const std::string why = check_model(M, n_cg);
+  if (!why.empty()) return fail(TPA_ERR_ARG, "invalid model: " + why);
+  const std::string src = tpa_fused::fused_source(M, cg, n_cg);
+  *len = (int64_t)src.size();
+  if (buf && buf_len > 0) {
+    const size_t n = std::min<size_t>(src.size(), (size_t)buf_len - 1);
+    std::memcpy(buf, src.data(), n);
+    buf[n] = '\0';
+  }
+  return TPA_OK;
+}
+
+int tpa_eval_fused(tpa_events_t ev, const double* coef, int mode, int flags, double* out) {
+  if (!ev || !coef || !out) return fail(TPA_ERR_ARG, "null pointer");
+  return eval_fused_impl(ev, coef, mode, flags, out);
+}
+
+int tpa_fused_compile_seconds(tpa_model_t m, double* s) {
+  if (!m || !s) return fail(TPA_ERR_ARG, "null pointer");
+  std::lock_guard<std::mutex> lk(m->fused_mu);
+  *s = m->fused_compile_s;
   return TPA_OK;
 }
 

@@ -306,24 +306,6 @@ struct EvalArgs {
 // Coefficient k of an evaluation pass (from the kernel arguments when a.coef is null).
 TPA_DEV double2 eval_coef(const EvalArgs& a, int k) { return a.coef ? a.coef[k] : a.cin[k]; }
 
-TPA_DEV void neumaier_add(double& s, double& c, double y) {
-  const double t = s + y;
-  if (fabs(s) >= fabs(y))
-    c += (s - t) + y;
-  else
-    c += (y - t) + s;
-  s = t;
-}
-TPA_DEV void neumaier_merge(double& s, double& c, double s2, double c2) {
-  const double t = s + s2;
-  double err;
-  if (fabs(s) >= fabs(s2))
-    err = (s - t) + s2;
-  else
-    err = (s2 - t) + s;
-  c = (c + c2) + err;
-  s = t;
-}
 
 #define TPA_NACC_MAX (2 + 2 * TPA_MAX_P)
 

@@ -4,8 +4,10 @@
 // Each function states the tf-pwa function whose arithmetic it reproduces; the operation
 // order follows the reference so results agree to rounding.
 #pragma once
+#ifndef __HIPCC_RTC__  // hipRTC (the fused pass's runtime compile) provides these itself
 #include <hip/hip_runtime.h>
 #include <math.h>
+#endif
 
 #include "tpa_model.h"
 
@@ -400,4 +402,25 @@ TPA_DEV void lineshape_grad(int shape, double m, double m0, double g0, double q,
     *dBm = c_scale(*dBm, g0);
     *B = c_scale(*B, g0);
   }
+}
+
+// Neumaier-compensated summation: s + c carries the sum of the added terms to about twice the
+// working precision.
+TPA_DEV void neumaier_add(double& s, double& c, double y) {
+  const double t = s + y;
+  if (fabs(s) >= fabs(y))
+    c += (s - t) + y;
+  else
+    c += (y - t) + s;
+  s = t;
+}
+TPA_DEV void neumaier_merge(double& s, double& c, double s2, double c2) {
+  const double t = s + s2;
+  double err;
+  if (fabs(s) >= fabs(s2))
+    err = (s - t) + s2;
+  else
+    err = (s2 - t) + s;
+  c = (c + c2) + err;
+  s = t;
 }

@@ -28,6 +28,7 @@ FLAG_GRAD = 1
 FLAG_FOUT = 2
 FLAG_SHAPE = 4
 FLAG_F32 = 8
+FUSED_MAX_P = 8  # largest model (partial waves) tpa_eval_fused accepts
 CPL_SLOTS = 4  # factor slots per partial wave in TpaCouplingMap (total + up to 3 vertices)
 
 _i32 = ctypes.c_int32
@@ -213,6 +214,10 @@ def lib():
         "tpa_calib_copy": ([i64, ctypes.c_int, dp], ctypes.c_int),
         "tpa_hess": ([vp, dp, ctypes.c_int, dp], ctypes.c_int),
         "tpa_hess_shape": ([vp, dp, ctypes.c_int, dp], ctypes.c_int),
+        "tpa_eval_fused": ([vp, dp, ctypes.c_int, ctypes.c_int, dp], ctypes.c_int),
+        "tpa_fused_source": ([vp, i64, dp, ctypes.c_int, ctypes.c_char_p, i64,
+                              ctypes.POINTER(i64)], ctypes.c_int),
+        "tpa_fused_compile_seconds": ([vp, dp], ctypes.c_int),
         "tpa_pool_hess": ([vp, i64, i64, dp, dp], ctypes.c_int),
         "tpa_pool_create": ([vp, i64, ctypes.c_int, ctypes.POINTER(vp)], ctypes.c_int),
         "tpa_pool_destroy": ([vp], ctypes.c_int),
@@ -290,5 +295,6 @@ def exported_symbols():
         "tpa_server_latencies", "tpa_server_counters", "tpa_fcn_create", "tpa_fcn_destroy",
         "tpa_fcn_nll_grad", "tpa_fcn_set_shape_map", "tpa_model_set_lineshapes", "tpa_fcn_minimize",
         "tpa_fcn_cache_rebuilds", "tpa_events_keep_f32", "tpa_fcn_set_data_storage",
-        "tpa_pool_keep_f32", "tpa_eval_noalloc", "tpa_hess_shape",
+        "tpa_pool_keep_f32", "tpa_eval_noalloc", "tpa_hess_shape", "tpa_eval_fused",
+        "tpa_fused_source", "tpa_fused_compile_seconds",
     ]

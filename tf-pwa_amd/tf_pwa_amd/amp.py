@@ -618,6 +618,20 @@ class AmplitudeModel:
         self._dev_state = cache
         return self._dev
 
+    def fused_source(self):
+        """The HIP source of this model's fused pass (tpa_fused_source; three-body)."""
+        if self.n_final != 3:
+            raise NotImplementedError("the fused pass covers three-body models")
+        M = self.describe()
+        n = ctypes.c_int64(0)
+        L = _lib.lib()
+        _lib.check(L.tpa_fused_source(ctypes.byref(M), ctypes.sizeof(M), _lib.dptr(self.cg),
+                                      len(self.cg), None, 0, ctypes.byref(n)))
+        buf = ctypes.create_string_buffer(n.value + 1)
+        _lib.check(L.tpa_fused_source(ctypes.byref(M), ctypes.sizeof(M), _lib.dptr(self.cg),
+                                      len(self.cg), buf, n.value + 1, ctypes.byref(n)))
+        return buf.value.decode()
+
     def __del__(self):
         h = getattr(self, "_dev", None)
         if h is not None and _lib._lib is not None:

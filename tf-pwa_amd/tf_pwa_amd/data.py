@@ -209,6 +209,17 @@ class EventSet:
             G = (z[:P].copy(), z[P:2 * P].copy(), z[2 * P:].copy()) if shape else z.copy()
         return out[0], G, f
 
+    def eval_fused(self, coef, mode, grad=False):
+        """One pass of the fused kernel (tpa_eval_fused): the waves recomputed from the angle
+        cache on every call, no partial-wave cache. (value, G or None)."""
+        self.amp.device()
+        P = self.amp.P
+        c = np.ascontiguousarray(np.asarray(coef, dtype=np.complex128)).view(np.float64)
+        out = np.zeros(2 + 2 * P, dtype=np.float64)
+        _lib.check(_lib.lib().tpa_eval_fused(self._h, _lib.dptr(c), mode,
+                                             _lib.FLAG_GRAD if grad else 0, _lib.dptr(out)))
+        return out[0], (out[2:].view(np.complex128).copy() if grad else None)
+
     def hess(self, coef, mode):
         """Second-moment pass: (value, G, Dcc, Dcb) with G_k = d sum / dc_k,
         Dcc_kl = d2 sum / dc_k dc_l and Dcb_kl = d2 sum / dc_k dconj(c_l) (complex [P, P])."""

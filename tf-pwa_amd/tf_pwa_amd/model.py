@@ -65,7 +65,7 @@ class Model:
     """Model.nll / nll_grad / nll_grad_batch / nll_grad_hessian (model.py:576-790)."""
 
     def __init__(self, amp, w_bkg=1.0, resolution_size=1, extended=False, precision="fp64",
-                 gram_precision="fp64", data_storage="fp64", **kwargs):
+                 gram_precision="fp64", data_storage="fp64", evaluation="stream", **kwargs):
         """precision: "fp64" evaluates every MC event per call; "mixed" replaces the MC pass
         by the partial-wave Gram matrix built once on MFMA (valid while the partial waves,
         i.e. lineshape parameters and MC weights, are unchanged; rebuilt otherwise).
@@ -75,13 +75,19 @@ class Model:
         relative for narrow peaks over a broad MC sample).
         data_storage: the data partial-wave cache the mixed path reads, "fp64" or "fp32"
         (an FP32 copy kept beside the FP64 cache, values widened to FP64 on load, every sum
-        in FP64: half the bytes per data event; with fixed lineshapes, else FP64)."""
+        in FP64: half the bytes per data event; with fixed lineshapes, else FP64).
+        evaluation: how a pass gets its partial waves, "stream" (read the cache built once,
+        k_eval_pipe / k_eval_pipe32) or "fused" (recompute every wave from the angle cache per
+        event in one generated kernel, tpa_eval_fused: no partial-wave cache; three-body,
+        fixed lineshapes)."""
         if precision not in ("fp64", "mixed"):
             raise ValueError("precision must be 'fp64' or 'mixed'")
         if gram_precision not in ("fp32", "fp64"):
             raise ValueError("gram_precision must be 'fp32' or 'fp64'")
         if data_storage not in ("fp32", "fp64"):
             raise ValueError("data_storage must be 'fp32' or 'fp64'")
+        self.Amp = amp
+        self.evaluation = evaluation
         self.precision = precision
         self.gram_precision = gram_precision
         self.data_storage = data_storage
@@ -89,7 +95,6 @@ class Model:
             raise NotImplementedError("resolution folding is not supported")
         if extended:
             raise NotImplementedError("extended likelihood is not supported")
-        self.Amp = amp
         self.vm = amp.vm
         self.w_bkg = w_bkg
         self.resolution_size = 1
@@ -127,6 +132,22 @@ class Model:
             w = np.asarray(mc_weight, dtype=np.float64)
         return sets, w / w.sum()
 
+    @property
+    def evaluation(self):
+        """"stream" or "fused" (see __init__)."""
+        return self._evaluation
+
+    @evaluation.setter
+    def evaluation(self, value):
+        if value not in ("stream", "fused"):
+            raise ValueError("evaluation must be 'stream' or 'fused'")
+        if value == "fused" and self.Amp.P > _lib.FUSED_MAX_P:
+            raise NotImplementedError(
+                "the fused pass is disabled above {} partial waves (this model has {}): the "
+                "kernel generated for a 22-wave model faulted, cause not identified".format(
+                    _lib.FUSED_MAX_P, self.Amp.P))
+        self._evaluation = value
+
     def _pass(self, sets, weights, coef, mode, grad):
         """Sum of one mode over event sets: value and adjoints (G, or (G, Dm, Dg) when the
         lineshapes float)."""
@@ -141,6 +162,11 @@ class Model:
                 Ic = I @ coef
                 v = float(np.real(np.vdot(coef, Ic)))
                 g = (np.conj(Ic),)
+            elif self.evaluation == "fused":
+                if shape:
+                    raise NotImplementedError("the fused pass needs fixed lineshapes")
+                v, g = s.eval_fused(coef, mode, grad=grad)
+                g = (g,)
             else:
                 v, g, _ = s.eval(coef, mode, grad=grad, f32=self._data_f32(mode))
                 if grad and not shape:
@@ -470,6 +496,8 @@ class FCN:
         amp = self.model.Amp
         if not self.native or (amp.shape_mode and amp.n_final != 3):
             return None
+        if getattr(self.model, "evaluation", "stream") == "fused":
+            return None  # the fused passes run through the host assembly (Model._pass)
         dw, mw, sw = self._split
         pairs = self._set_weights
         if pairs is None:
