@@ -258,7 +258,9 @@ int tpa_calib_copy(int64_t bytes, int reps, double* ms);
  * fit_scipy's minimize loop, tf_pwa/fit.py:189-300) when many fits run at once.
  * The MC normalisation of these fits is shared and comes from tpa_gram on an event set. */
 
-/* capacity: events the pool can hold; max_items: the most requests one tpa_pool_eval serves. */
+/* capacity: events the pool can hold, rounded up to a multiple of 64 (the row stride of the
+ * pool's per-event arrays, kept on 512-byte boundaries; tpa_pool_free_events of a new pool
+ * gives the rounded value); max_items: the most requests one tpa_pool_eval serves. */
 int tpa_pool_create(tpa_model_t model, int64_t capacity, int max_items, tpa_pool_t* out);
 int tpa_pool_destroy(tpa_pool_t pool);
 

@@ -130,19 +130,21 @@ def test_pool_admission_and_release(case6, gpu_available):
     conf = case6["conf"]
     amp = conf.get_amplitude()
     pool = EventPool(amp, capacity=5000, max_items=4)
+    cap = pool.capacity
+    assert cap == 5056  # rounded up to the 64-event range granule (the row stride)
     p4 = case6["data"]
     # ranges are granted in multiples of 64 events (aligned chunk starts)
     a = pool.add(_slice(p4, 0, 2000), order=conf.dat_order)
     b = pool.add(_slice(p4, 2000, 4000), order=conf.dat_order)
-    assert (a.offset, b.offset) == (0, 2048) and pool.free_events() == 5000 - 4096
+    assert (a.offset, b.offset) == (0, 2048) and pool.free_events() == cap - 4096
     with pytest.raises(PoolFull):
         pool.add(_slice(p4, 0, 1500), order=conf.dat_order)
     pool.release(a)
     c = pool.add(_slice(p4, 0, 1500), order=conf.dat_order)
-    assert c.offset == 0 and pool.free_events() == (2048 - 1536) + (5000 - 4096)
+    assert c.offset == 0 and pool.free_events() == (2048 - 1536) + (cap - 4096)
     pool.release(b)
     pool.release(c)
-    assert pool.free_events() == 5000
+    assert pool.free_events() == cap
     d = pool.add(_slice(p4, 0, 5000), order=conf.dat_order)  # free ranges merged again
     assert d.offset == 0
     with pytest.raises(Exception):

@@ -1744,6 +1744,11 @@ int tpa_pool_destroy(tpa_pool_t p) {
 int tpa_pool_create(tpa_model_t m, int64_t capacity, int max_items, tpa_pool_t* out) {
   if (!m || !out) return fail(TPA_ERR_ARG, "null pointer");
   if (capacity < 1 || max_items < 1) return fail(TPA_ERR_ARG, "bad pool size");
+  // The capacity is also the row stride of every per-event array of the pool ([row][cap]):
+  // rounded up to the range granule it puts each row on a 512-byte boundary. With an odd stride
+  // the rows of the FP32 copy (32 B per event) start off the 128-byte line grid, each wave's
+  // 256-byte group spans three lines and a full batch reads up to 1.24x its bytes.
+  capacity = (capacity + TPA_POOL_ALIGN - 1) / TPA_POOL_ALIGN * TPA_POOL_ALIGN;
   const TpaModel& M = m->h;
   tpa_pool_s* p = new tpa_pool_s();
   p->m = m;
@@ -1809,9 +1814,9 @@ int tpa_pool_add(tpa_pool_t p, const double* p4, int64_t n, const double* weight
     if (it == p->free_ranges.end()) return fail(TPA_ERR_ARG, "pool full");
     off = it->first;
     const int64_t len = it->second;
-    // ranges are whole multiples of TPA_POOL_ALIGN events (only the pool's last range may
-    // end short of one), so every range and every 2048-event chunk starts on a cache-line
-    // boundary of each partial-wave row: the streaming loads of two chunks never share a line
+    // ranges are whole multiples of TPA_POOL_ALIGN events, like the capacity (the row
+    // stride), so every range and every 2048-event chunk starts on a cache-line boundary of
+    // each partial-wave row: the streaming loads of two chunks never share a line
     const int64_t take = std::min<int64_t>(len, (n + TPA_POOL_ALIGN - 1) / TPA_POOL_ALIGN * TPA_POOL_ALIGN);
     p->free_ranges.erase(it);
     if (len > take) p->free_ranges[off + take] = len - take;

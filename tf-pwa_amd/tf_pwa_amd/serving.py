@@ -73,6 +73,7 @@ class EventPool:
         _lib.check(_lib.lib().tpa_pool_create(amp.device(), self.capacity, self.max_items,
                                               ctypes.byref(h)))
         self._h = h
+        self.capacity = self.free_events()  # the library rounds it up to a multiple of 64
         if data_storage == "fp32":
             _lib.check(_lib.lib().tpa_pool_keep_f32(h, 1))
 
