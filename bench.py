@@ -25,6 +25,7 @@ sys.path.insert(0, ROOT)
 
 CONFIG = os.path.join(ROOT, "tf-pwa_amd", "tf_pwa_amd", "configs", "lc_pkpi_6res.yml")
 CONFIG4 = os.path.join(ROOT, "tf-pwa_amd", "tf_pwa_amd", "configs", "jpsi_kkpipi_12ch.yml")
+SERVE_CONFIG = os.path.join(ROOT, "tf-pwa_amd", "tf_pwa_amd", "configs", "lc_pkpi_2res.yml")
 METRIC = "NLL+grad evals/s @1M data+10M MC, |dNLL|<1e-3 vs FP64; toy-fit p50/p99 ms"
 HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFLOPS = 78.6  # MI355X FP64 vector peak
@@ -180,64 +181,77 @@ def torch_oracle_rate(conf, data_p4, n_data_full, n_mc_full, seconds=6.0):
     return 1.0 / (dt * (n_data_full + n_mc_full) / float(nd + nm))
 
 
-def serving_leg(conf, mc, data_p4, x_true, args, seed=4242):
-    """Toy-fit serving: ``args.serve_fits`` independent BFGS fits of the six-resonance model
-    arriving as a Poisson stream, sharing the 10M-event MC (its Gram matrix) and an event pool
-    sized by the HBM budget. Per-fit data sizes are lognormal (median 30k, sigma 0.8) clipped to
-    [5k, 200k]; each fit's sample is a seeded resample of the accept-reject data set (drawn
-    before the clock starts: it arrives with the request), and each fit starts from the
-    generating couplings plus N(0, 0.1) noise."""
-    from tf_pwa_amd import FitServer
+def device_toy(amp, n, seed, fmax, eff):
+    """One fit's toy sample made on the device: flat candidates (tpa_events_generate with the
+    fit's seed), accept-reject of f = |A|^2 against fmax with the fit's own generator, the
+    accepted events gathered on the device (tpa_events_select); more candidates are drawn (the
+    next counters of the same seed) until n are accepted. Returns (EventSet, candidates)."""
+    from tf_pwa_amd import EventSet
 
     rng = np.random.default_rng(seed)
-    m = args.serve_fits
-    sizes = np.clip(np.round(np.exp(np.log(30000) + 0.8 * rng.normal(size=m))), 5000,
-                    200000).astype(np.int64)
-    arrivals = np.cumsum(rng.exponential(1.0 / args.serve_rate, m))
-    from tf_pwa_amd.data import p4_array
+    parts, got, first = [], 0, 0
+    while got < n:
+        n_cand = int((n - got) / eff * 1.2) + 2000
+        cand = EventSet.generate(amp, n_cand, seed=seed, first=first)
+        f = amp(cand)
+        idx = np.nonzero(f > rng.uniform(0.0, fmax, f.size))[0][: n - got]
+        parts.append(cand.select(idx))
+        got += idx.size
+        first += n_cand
+        del cand
+    return (parts[0] if len(parts) == 1 else EventSet.concat(parts)), first
 
-    amp = conf.get_amplitude()
-    pool_p4 = p4_array(data_p4, amp.finals)  # [n_final*4][n], the engine's input layout
-    n_pool = pool_p4.shape[1]
+
+def serve_once(amp, mc, toys, y0s, arrivals, storage, args):
+    """One serving run of the toy fits from a pool of the given storage, then an untimed
+    diagnosis of every fit's end: the gradient of the fit's own objective and the exact
+    Hessian at the final point (Newton decrement 0.5 g^T H^-1 g, the decrease still available)
+    against the resolution of the NLL value (its FP64 spacing)."""
+    from tf_pwa_amd import FitServer
+
+    m = len(toys)
     t = time.perf_counter()
-    jobs = []
-    for i, n in enumerate(sizes):  # each fit's sample arrives with its request
-        idx = np.random.default_rng(seed + 1 + i).integers(0, n_pool, int(n))
-        jobs.append(dict(p4=pool_p4[:, idx], engine_layout=True,
-                         y0=x_true + np.random.default_rng(seed + 100000 + i).normal(
-                             scale=0.1, size=x_true.size)))
-    log("serving: {} samples drawn in {:.1f}s".format(m, time.perf_counter() - t))
-    t = time.perf_counter()
-    storage = args.serve_storage
-    srv = FitServer(conf.get_amplitude(), mc, hbm_budget=args.serve_budget_gb * 1e9,
-                    max_batch=m, data_storage=storage)
+    srv = FitServer(amp, mc, hbm_budget=args.serve_budget_gb * 1e9, max_batch=m,
+                    data_storage=storage)
     setup_s = time.perf_counter() - t
-    bpe_pool = srv.pool.bytes_per_event(srv.amp, storage)
-    bpe_read = amp.P * amp.lam_pad * (8 if storage == "fp32" else 16) + 8
-    log("serving: pool {} events ({:.1f} GB) in {:.1f}s; {} fits, {} events; data waves {}".format(
-        srv.pool.capacity, srv.pool.capacity * bpe_pool / 1e9, setup_s, m, int(sizes.sum()),
-        storage))
+    jobs = [dict(events=toys[i], y0=y0s[i]) for i in range(m)]
     t0 = time.perf_counter()
     results, times = srv.serve(jobs, arrivals, timeout_s=600)
     wall = time.perf_counter() - t0
     lat = srv.latencies_us() / 1e3
     c = srv.counters()
-    status = {}
-    for r in results:
-        st = status.setdefault(r.message, {"fits": 0, "nfev_mean": 0.0, "niter_mean": 0.0})
-        st["fits"] += 1
-        st["nfev_mean"] += r.nfev
-        st["niter_mean"] += r.niter
-    for st in status.values():
-        st["nfev_mean"] /= st["fits"]
-        st["niter_mean"] /= st["fits"]
     span = max(t["done"] for t in times) - min(t["arrival"] for t in times)
+    bpe_read = amp.P * amp.lam_pad * (8 if storage == "fp32" else 16) + 8
+    names = amp.vm.trainable_vars
+    diag = []
+    for i, r in enumerate(results):
+        y = np.array([r.params[v] for v in names])
+        slot = srv.pool.add_events(toys[i])
+        nll, g = srv.nll_grad(slot, y)
+        _, _, H = srv.nll_grad_hessian(slot, y)
+        srv.release(slot)
+        try:
+            np.linalg.cholesky(H)
+            pd, dec = True, 0.5 * float(g @ np.linalg.solve(H, g))
+        except np.linalg.LinAlgError:
+            pd, dec = False, 0.5 * float(g @ np.linalg.lstsq(H, g, rcond=None)[0])
+        diag.append(dict(status=r.message, gmax=float(np.abs(g).max()), dec=dec, pd=pd,
+                         ulp=float(np.spacing(abs(nll))), nfev=r.nfev, niter=r.niter))
+    status = {}
+    for msg in sorted({d["status"] for d in diag}):
+        ds = [d for d in diag if d["status"] == msg]
+        gm = np.array([d["gmax"] for d in ds])
+        dec = np.array([d["dec"] for d in ds])
+        ulp = np.array([d["ulp"] for d in ds])
+        status[msg] = {"fits": len(ds), "nfev_mean": float(np.mean([d["nfev"] for d in ds])),
+                       "niter_mean": float(np.mean([d["niter"] for d in ds])),
+                       "final_gmax_median": float(np.median(gm)), "final_gmax_max": float(gm.max()),
+                       "newton_decrement_median": float(np.median(dec)),
+                       "newton_decrement_max": float(dec.max()),
+                       "decrement_over_nll_spacing_median": float(np.median(dec / ulp)),
+                       "decrement_over_nll_spacing_max": float((dec / ulp).max()),
+                       "hessian_not_positive_definite": int(sum(not d["pd"] for d in ds))}
     out = {
-        "workload": "{} toy fits of the Lc->pKpi 6-resonance model (34 free couplings), data "
-                    "sizes lognormal median 30k clipped [5k, 200k] (total {}), shared 10M MC "
-                    "Gram, Poisson arrivals at {}/s, HBM budget {} GB, data waves stored {} "
-                    "(FP64 arithmetic)".format(
-                        m, int(sizes.sum()), args.serve_rate, args.serve_budget_gb, storage),
         "data_storage": storage,
         "evals_per_s": c["requests"] / span,
         "p50_ms": float(np.percentile(lat, 50)),
@@ -256,13 +270,76 @@ def serving_leg(conf, mc, data_p4, x_true, args, seed=4242):
         "fits_per_s": m / span,
         "fit_s_p50": float(np.median([t["done"] - t["admitted"] for t in times])),
         "admission_wait_s_max": float(max(t["admitted"] - t["arrival"] for t in times)),
-        "peak_pool_GB": srv.peak_events * bpe_pool / 1e9,
+        "pool_capacity_events": srv.pool.capacity,
+        "peak_pool_GB": srv.peak_events * EventPool_bpe(amp, storage) / 1e9,
         "status": status,
         "setup_s": setup_s,
     }
-    log("serving: {:.0f} evals/s, p50 {:.2f} ms, p99 {:.2f} ms, mean batch {:.1f}".format(
-        out["evals_per_s"], out["p50_ms"], out["p99_ms"], out["mean_batch"]))
+    log("serving ({}): {:.0f} evals/s, p50 {:.2f} ms, p99 {:.2f} ms, mean batch {:.1f}; {}".format(
+        storage, out["evals_per_s"], out["p50_ms"], out["p99_ms"], out["mean_batch"],
+        ", ".join("{}: {}".format(k, v["fits"]) for k, v in status.items())))
     srv.close()
+    return out
+
+
+def EventPool_bpe(amp, storage):
+    from tf_pwa_amd import EventPool
+
+    return EventPool.bytes_per_event(amp, storage)
+
+
+def serving_leg(args, seed=4242):
+    """Toy-fit serving of the two-resonance Lc->pKpi model (K*(892) in K pi, Lambda(1520) in
+    p K; 8 partial waves): ``args.serve_fits`` independent BFGS fits arriving as a Poisson
+    stream, sharing a flat MC sample generated on the device (its Gram matrix) and an event pool
+    sized by the HBM budget. Per-fit data sizes are lognormal (median 30k, sigma 0.8) clipped to
+    [5k, 200k]; each fit's toy is generated on the device with its own seed (device_toy) before
+    the clock starts (it arrives with the request) and enters the pool on admission without a
+    host round trip (tpa_pool_add_events); each fit starts from the generating couplings plus
+    N(0, 0.1) noise. The same fits are served from an FP32 pool (the serving default) and an
+    FP64 pool."""
+    from tf_pwa_amd import ConfigLoader, EventSet
+
+    conf = ConfigLoader(SERVE_CONFIG, seed=seed)
+    conf.set_params(random_couplings(conf.vm, seed))
+    x_true = np.array(conf.vm.get_all_val())
+    amp = conf.get_amplitude()
+    rng = np.random.default_rng(seed)
+    m = args.serve_fits
+    sizes = np.clip(np.round(np.exp(np.log(30000) + 0.8 * rng.normal(size=m))), 5000,
+                    200000).astype(np.int64)
+    arrivals = np.cumsum(rng.exponential(1.0 / args.serve_rate, m))
+    t = time.perf_counter()
+    mc = EventSet.generate(amp, args.serve_mc, seed=seed + 7)
+    f_mc = amp(mc)
+    fmax = 1.1 * float(f_mc.max())
+    eff = float(f_mc.mean()) / fmax
+    mc_s = time.perf_counter() - t
+    t = time.perf_counter()
+    toys, n_cand = [], 0
+    for i, n in enumerate(sizes):
+        ev, nc = device_toy(amp, int(n), seed + 1 + i, fmax, eff)
+        toys.append(ev)
+        n_cand += nc
+    toys_s = time.perf_counter() - t
+    log("serving: {} device toys ({} events from {} candidates, efficiency {:.3f}) in {:.1f}s; "
+        "MC {} in {:.1f}s".format(m, int(sizes.sum()), n_cand, eff, toys_s, args.serve_mc, mc_s))
+    y0s = [x_true + np.random.default_rng(seed + 100000 + i).normal(scale=0.1, size=x_true.size)
+           for i in range(m)]
+    pools = {st: serve_once(amp, mc, toys, y0s, arrivals, st, args) for st in ("fp32", "fp64")}
+    out = dict(pools["fp32"])
+    out["workload"] = (
+        "{} toy fits of the Lc->pKpi two-resonance model (K*(892), Lambda(1520); {} partial "
+        "waves, {} free parameters), per-fit toys generated on the device (flat GENBOD "
+        "candidates with per-fit Philox seeds, accept-reject on |A|^2; {} events from {} "
+        "candidates), sizes lognormal median 30k clipped [5k, 200k], shared {} device MC Gram, "
+        "Poisson arrivals at {}/s, HBM budget {} GB; FP32 pool (the top-level figures) and FP64 "
+        "pool, FP64 arithmetic in both".format(
+            m, amp.P, len(x_true), int(sizes.sum()), n_cand, args.serve_mc, args.serve_rate,
+            args.serve_budget_gb))
+    out["pools"] = pools
+    out["toys"] = {"seconds": toys_s, "events": int(sizes.sum()), "candidates": n_cand,
+                   "efficiency": eff, "mc_events": args.serve_mc, "mc_seconds": mc_s}
     return out
 
 
@@ -475,8 +552,8 @@ def main():
     ap.add_argument("--serve-rate", type=float, default=512.0,
                     help="Poisson arrival rate of the serving leg's fits per second")
     ap.add_argument("--serve-budget-gb", type=float, default=200.0)
-    ap.add_argument("--serve-storage", choices=("fp32", "fp64"), default="fp32",
-                    help="data partial waves the serving pool's batched passes read")
+    ap.add_argument("--serve-mc", type=int, default=10_000_000,
+                    help="flat MC events generated on the device for the serving leg's Gram")
     ap.add_argument("--four-body-mc", type=int, default=50_000_000,
                     help="MC events of the four-body leg (0 skips it)")
     ap.add_argument("--four-body-data", type=int, default=2_000_000)
@@ -698,8 +775,12 @@ def main():
         cpu_baseline_saved = cpu_baseline(conf, data, mc, mc.gram("fp64"), data_p4, args.n_data,
                                           args.n_mc)
     if args.serve_fits > 0:
-        del fmix, fmix32, data
-        res["serving"] = serving_leg(conf, mc, data_p4, x0, args)
+        import gc
+
+        del fmix, fmix32, data, f
+        mc = None  # the serving leg makes its own samples: release the headline sets first
+        gc.collect()
+        res["serving"] = serving_leg(args)
     if args.multistart > 0:
         res["floating_lineshape_fits"] = multistart_leg(args, starts=args.multistart)
     if args.four_body_mc > 0:

@@ -270,6 +270,14 @@ int tpa_pool_destroy(tpa_pool_t pool);
 int tpa_pool_add(tpa_pool_t pool, const double* p4, int64_t n, const double* weight,
                  int64_t* offset);
 
+/* tpa_pool_add for the events of a device event set of the same model (their angles, copied on
+ * the device; weight host [n]): a sample generated on the GPU (tpa_events_generate +
+ * tpa_events_select) enters the pool without a host round trip. Replaces the hand-off of each
+ * toy sample of tf-pwa's toy studies (gen_data's accept-reject sample, tf_pwa/applications.py:
+ * 394-487, written out and read back as the fit's data). */
+int tpa_pool_add_events(tpa_pool_t pool, tpa_events_t ev, const double* weight,
+                        int64_t* offset);
+
 /* on != 0: keep an FP32 copy (round to nearest) of the pool's partial-wave cache, made now for
  * every allocated range and by every later tpa_pool_add; the batched passes (tpa_pool_eval and
  * the serving runtime) then read it (half the bytes per event, FP64 arithmetic after the load).

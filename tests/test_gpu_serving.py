@@ -299,3 +299,38 @@ def test_serve_loop_admission_and_results(case2res, gpu_available):
     lat = srv.latencies_us()
     assert len(lat) == srv.counters()["requests"] and np.all(lat > 0)
     srv.close()
+
+
+def test_pool_add_events_equals_add_from_momenta(case2res, gpu_available):
+    """A sample generated on the device enters the pool through tpa_pool_add_events (its angles
+    copied on the device) and gives bit-identical batched sums to the same events uploaded as
+    four-momenta (tpa_pool_add), for a whole generated set with weights and for a selected
+    subset (the accept-reject path), from FP64 and FP32 pools. A set of another model is
+    refused."""
+    from conftest import config_path
+    from tf_pwa_amd import ConfigLoader, EventPool, EventSet
+
+    conf = case2res["conf"]
+    amp = conf.get_amplitude()
+    n = 3100
+    ev, p4 = EventSet.generate(amp, n, seed=77, return_p4=True)
+    idx = np.sort(np.random.default_rng(3).choice(n, 1234, replace=False))
+    sub = ev.select(idx)
+    w = np.random.default_rng(4).uniform(0.5, 1.5, n)
+    c = _random_coefs(amp, conf.vm, np.random.default_rng(5), 2)
+    other = ConfigLoader(config_path("lc_pkpi_2res.yml"), seed=1).get_amplitude()
+    foreign = EventSet.generate(other, 100, seed=1)
+    for storage in ("fp64", "fp32"):
+        pool = EventPool(amp, capacity=20000, max_items=8, data_storage=storage)
+        a = pool.add(p4, weight=w, engine_layout=True)
+        b = pool.add_events(ev, weight=w)
+        s1 = pool.add(np.ascontiguousarray(p4[:, idx]), engine_layout=True)
+        s2 = pool.add_events(sub)
+        assert (b.n, s2.n) == (n, idx.size) and b.sw == a.sw and s2.sw == s1.sw
+        v, G, _ = pool.eval([a, b, s1, s2], np.array([c[0], c[0], c[1], c[1]]))
+        assert v[0] == v[1] and np.array_equal(G[0], G[1]), storage
+        assert v[2] == v[3] and np.array_equal(G[2], G[3]), storage
+        assert v[0] != v[2]
+        with pytest.raises(Exception, match="another model"):
+            pool.add_events(foreign)
+        pool.close()

@@ -1800,6 +1800,30 @@ int64_t tpa_pool_free_events(tpa_pool_t p) {
   return n;
 }
 
+}  // extern "C"
+
+// First free range of at least n events, taken from the free list (p->mu held): its offset,
+// or -1 when none is long enough. Ranges are whole multiples of TPA_POOL_ALIGN events, like
+// the capacity (the row stride), so every range and every 2048-event chunk starts on a
+// cache-line boundary of each partial-wave row: the streaming loads of two chunks never share
+// a line.
+static int64_t pool_take_range(tpa_pool_s* p, int64_t n) {
+  auto it = p->free_ranges.begin();
+  for (; it != p->free_ranges.end(); ++it)
+    if (it->second >= n) break;
+  if (it == p->free_ranges.end()) return -1;
+  const int64_t off = it->first;
+  const int64_t len = it->second;
+  const int64_t take =
+      std::min<int64_t>(len, (n + TPA_POOL_ALIGN - 1) / TPA_POOL_ALIGN * TPA_POOL_ALIGN);
+  p->free_ranges.erase(it);
+  if (len > take) p->free_ranges[off + take] = len - take;
+  p->used_ranges[off] = take;
+  return off;
+}
+
+extern "C" {
+
 int tpa_pool_add(tpa_pool_t p, const double* p4, int64_t n, const double* weight,
                  int64_t* offset) {
   if (!p || !p4 || !weight || !offset) return fail(TPA_ERR_ARG, "null pointer");
@@ -1808,19 +1832,8 @@ int tpa_pool_add(tpa_pool_t p, const double* p4, int64_t n, const double* weight
   int64_t off = 0;
   {
     std::lock_guard<std::mutex> lk(p->mu);
-    auto it = p->free_ranges.begin();
-    for (; it != p->free_ranges.end(); ++it)
-      if (it->second >= n) break;
-    if (it == p->free_ranges.end()) return fail(TPA_ERR_ARG, "pool full");
-    off = it->first;
-    const int64_t len = it->second;
-    // ranges are whole multiples of TPA_POOL_ALIGN events, like the capacity (the row
-    // stride), so every range and every 2048-event chunk starts on a cache-line boundary of
-    // each partial-wave row: the streaming loads of two chunks never share a line
-    const int64_t take = std::min<int64_t>(len, (n + TPA_POOL_ALIGN - 1) / TPA_POOL_ALIGN * TPA_POOL_ALIGN);
-    p->free_ranges.erase(it);
-    if (len > take) p->free_ranges[off + take] = len - take;
-    p->used_ranges[off] = take;
+    off = pool_take_range(p, n);
+    if (off < 0) return fail(TPA_ERR_ARG, "pool full");
   }
   auto undo = [&](hipError_t e, const char* what) {
     std::lock_guard<std::mutex> lk(p->mu);
@@ -1853,6 +1866,43 @@ int tpa_pool_add(tpa_pool_t p, const double* p4, int64_t n, const double* weight
                           hipMemcpyDeviceToDevice, s)) != hipSuccess)
     return undo(e, "weights");
   launch_cal_angle(m, p->d_stage, n, off, p->cap, p->d_ang, s);
+  launch_build_pw(m, p->d_ang, n, off, p->cap, 1, p->d_h, s);
+  if (p->d_h32) launch_rows_to_f32(p, off, n, s);
+  if ((e = hipGetLastError()) != hipSuccess) return undo(e, "launch");
+  if ((e = hipStreamSynchronize(s)) != hipSuccess) return undo(e, "build");
+  *offset = off;
+  return TPA_OK;
+}
+
+int tpa_pool_add_events(tpa_pool_t p, tpa_events_t ev, const double* weight,
+                        int64_t* offset) {
+  if (!p || !ev || !weight || !offset) return fail(TPA_ERR_ARG, "null pointer");
+  if (ev->m != p->m) return fail(TPA_ERR_ARG, "the event set belongs to another model");
+  const int64_t n = ev->n;
+  std::lock_guard<std::mutex> add_lk(p->add_mu);
+  int64_t off = 0;
+  {
+    std::lock_guard<std::mutex> lk(p->mu);
+    off = pool_take_range(p, n);
+    if (off < 0) return fail(TPA_ERR_ARG, "pool full");
+  }
+  auto undo = [&](hipError_t e, const char* what) {
+    std::lock_guard<std::mutex> lk(p->mu);
+    release_range(p, off);
+    return fail(TPA_ERR_HIP, std::string("pool add: ") + what + ": " + hipGetErrorString(e));
+  };
+  tpa_model_s* m = p->m;
+  hipStream_t s = p->add_stream;
+  hipError_t e = hipStreamSynchronize(m->stream);  // the set's angles are complete
+  if (e != hipSuccess) return undo(e, "event set");
+  // angles [field][n] of the set into rows [field][cap] of the pool at off
+  if ((e = hipMemcpy2DAsync(p->d_ang + off, sizeof(double) * p->cap, ev->d_ang,
+                            sizeof(double) * n, sizeof(double) * n, ev->nf,
+                            hipMemcpyDeviceToDevice, s)) != hipSuccess)
+    return undo(e, "angles");
+  if ((e = hipMemcpyAsync(p->d_w + off, weight, sizeof(double) * n, hipMemcpyHostToDevice,
+                          s)) != hipSuccess)
+    return undo(e, "weights");
   launch_build_pw(m, p->d_ang, n, off, p->cap, 1, p->d_h, s);
   if (p->d_h32) launch_rows_to_f32(p, off, n, s);
   if ((e = hipGetLastError()) != hipSuccess) return undo(e, "launch");

@@ -214,6 +214,7 @@ def lib():
         "tpa_calib_copy": ([i64, ctypes.c_int, dp], ctypes.c_int),
         "tpa_hess": ([vp, dp, ctypes.c_int, dp], ctypes.c_int),
         "tpa_hess_shape": ([vp, dp, ctypes.c_int, dp], ctypes.c_int),
+        "tpa_pool_add_events": ([vp, vp, dp, ctypes.POINTER(i64)], ctypes.c_int),
         "tpa_eval_fused": ([vp, dp, ctypes.c_int, ctypes.c_int, dp], ctypes.c_int),
         "tpa_fused_source": ([vp, i64, dp, ctypes.c_int, ctypes.c_char_p, i64,
                               ctypes.POINTER(i64)], ctypes.c_int),
@@ -296,5 +297,5 @@ def exported_symbols():
         "tpa_fcn_nll_grad", "tpa_fcn_set_shape_map", "tpa_model_set_lineshapes", "tpa_fcn_minimize",
         "tpa_fcn_cache_rebuilds", "tpa_events_keep_f32", "tpa_fcn_set_data_storage",
         "tpa_pool_keep_f32", "tpa_eval_noalloc", "tpa_hess_shape", "tpa_eval_fused",
-        "tpa_fused_source", "tpa_fused_compile_seconds",
+        "tpa_fused_source", "tpa_fused_compile_seconds", "tpa_pool_add_events",
     ]

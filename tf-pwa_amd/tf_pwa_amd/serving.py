@@ -105,6 +105,22 @@ class EventPool:
             _lib.check(rc)
         return DataSlot(off.value, n, w.sum())
 
+    def add_events(self, events, weight=None):
+        """Add the events of a device EventSet of the same model (tpa_pool_add_events): its
+        angles are copied on the device, so a sample generated on the GPU needs no host
+        round trip. ``weight``: host [n], default the set's own weights."""
+        events.amp.device()
+        n = events.n
+        w = _alpha_weights(n, events.weight if weight is None else weight)
+        off = ctypes.c_int64()
+        rc = _lib.lib().tpa_pool_add_events(self._h, events._h, _lib.dptr(w), ctypes.byref(off))
+        if rc != 0:
+            msg = _lib.lib().tpa_last_error().decode(errors="replace")
+            if "pool full" in msg:
+                raise PoolFull(msg)
+            _lib.check(rc)
+        return DataSlot(off.value, n, w.sum())
+
     def release(self, slot):
         _lib.check(_lib.lib().tpa_pool_release(self._h, slot.offset))
 
@@ -387,7 +403,8 @@ class FitServer:
     # -------------------------------------------------------------- the serving loop
     def serve(self, jobs, arrivals, poll_s=2e-4, timeout_s=None):
         """Run ``jobs`` (each a dict with "p4", "y0" and optional "weight", "order",
-        "engine_layout"; "p4" may be a callable that draws the sample when the fit is admitted)
+        "engine_layout"; "p4" may be a callable that draws the sample when the fit is admitted;
+        "events", a device EventSet of the model or a callable making one, in place of "p4")
         arriving at ``arrivals`` seconds from now. Admission is first come, first served: the head of the
         queue waits until the pool has a free range for its sample. Returns (results in job
         order, per-fit dict of times: arrival, admitted, done)."""
@@ -408,11 +425,15 @@ class FitServer:
             while waiting:
                 j = waiting[0]
                 if j not in staged:
-                    p4 = jobs[j]["p4"]
-                    staged[j] = p4() if callable(p4) else p4
+                    src = jobs[j]["events"] if "events" in jobs[j] else jobs[j]["p4"]
+                    staged[j] = src() if callable(src) else src
                 try:
-                    slot = self.add_data(staged[j], jobs[j].get("weight"), jobs[j].get("order"),
-                                         jobs[j].get("engine_layout", False))
+                    if "events" in jobs[j]:
+                        slot = self.pool.add_events(staged[j], jobs[j].get("weight"))
+                    else:
+                        slot = self.add_data(staged[j], jobs[j].get("weight"),
+                                             jobs[j].get("order"),
+                                             jobs[j].get("engine_layout", False))
                 except PoolFull:
                     if not running:
                         raise
