@@ -1757,7 +1757,7 @@ typedef double tpa_doublex4 __attribute__((ext_vector_type(4)));
 // Partials per block: value, compensation, G (interleaved re / im), then the O1 and O2 tiles in
 // the accumulator layout (tile q, register r, lane l at (4 q + r) * 64 + l).
 template <int LG, int QT>
-__global__ __launch_bounds__(256) void k_hess_mfma(HessArgs a) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_hess_mfma(HessArgs a) {
   constexpr int R = 128;
   constexpr int E = R / LG;
   constexpr int C = 16 * QT;
@@ -1784,19 +1784,37 @@ __global__ __launch_bounds__(256) void k_hess_mfma(HessArgs a) {
   double vsum = 0.0, vcomp = 0.0;
   const int64_t tiles = (a.n + E - 1) / E;
   const int64_t e_end = a.off + a.n;
+  // staging map: thread t holds (event, helicity) row t % R of waves k = t / R + 2 j; a tile's
+  // loads are issued into registers during the previous tile's work and written to LDS at the
+  // start of its own (without this overlap 68% of the wave cycles went to waiting on them)
+  static_assert(R == 128, "the staging map assumes 128 rows per tile");
+  constexpr int NPF = 4 * QT;  // R x P <= 8 QT waves over 256 threads
+  const int r_st = t & (R - 1), k_st = t >> 7;
+  double2 pf[NPF];
+  auto fetch = [&](int64_t tile) {
+    const int64_t e = a.off + tile * E + r_st / LG;
+#pragma unroll
+    for (int j = 0; j < NPF; ++j) {
+      const int k = k_st + 2 * j;
+      pf[j] = (k < P && e < e_end)
+                  ? load_pw<true>(&a.h[((int64_t)k * a.S + e) * LG + (r_st % LG)])
+                  : make_double2(0.0, 0.0);
+    }
+  };
+  if ((int64_t)blockIdx.x < tiles) fetch(blockIdx.x);
   for (int64_t tile = blockIdx.x; tile < tiles; tile += gridDim.x) {
     const int64_t e0 = a.off + tile * E;
     // stage: for each wave k, R contiguous (event, helicity) pairs of the cache
-#pragma unroll 2
-    for (int idx = t; idx < P * R; idx += 256) {
-      const int k = idx / R, r = idx - k * R;
-      const int64_t e = e0 + r / LG;
-      const double2 v = e < e_end ? load_pw<true>(&a.h[((int64_t)k * a.S + e) * LG + (r % LG)])
-                                  : make_double2(0.0, 0.0);
-      Zs[k * ZS + r] = v.x;
-      Zs[(P + k) * ZS + r] = v.y;
+#pragma unroll
+    for (int j = 0; j < NPF; ++j) {
+      const int k = k_st + 2 * j;
+      if (k < P) {
+        Zs[k * ZS + r_st] = pf[j].x;
+        Zs[(P + k) * ZS + r_st] = pf[j].y;
+      }
     }
     __syncthreads();
+    if (tile + gridDim.x < tiles) fetch(tile + gridDim.x);
     if (t < R) {
       double ar = 0.0, ai = 0.0;
 #pragma unroll 2
@@ -2330,17 +2348,24 @@ __global__ __launch_bounds__(256) void k_gather_cols(const double* __restrict__ 
 // Z_row = [Re h_0 .. Re h_{P-1}, Im h_0 .. Im h_{P-1}] (Q = 2P <= 64 columns, QT = ceil(Q/16)
 // tiles). A block stages 128 rows in LDS; each wave multiplies its 32 rows into the upper
 // tiles (ti <= tj) of G. No refolds are needed: products and sums are FP64 throughout.
+// The loads of a block's next 128 rows are issued into registers before the MFMAs of the
+// current ones and written to LDS after them, so HBM latency overlaps the matrix work (without
+// it 64% of the wave cycles went to waiting on the loads).
 
 #define TPA_GRAM64_ROWS 128
 #define TPA_GRAM64_STRIDE (TPA_GRAM64_ROWS + 1)
 #define TPA_GRAM64_SLOTS (10 * 256)
 
 template <int QT>
-__global__ __launch_bounds__(256) void k_gram_f64(const double2* __restrict__ h,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(QT <= 3 ? 3 : 2))) void k_gram_f64(const double2* __restrict__ h,
                                                   const double* __restrict__ w, int64_t N,
                                                   int lam_pad, int lam, int P,
                                                   double* __restrict__ partials) {
   constexpr int NT = QT * (QT + 1) / 2;
+  // staged values per thread: 128 rows x P <= 8 QT waves over 256 threads; thread t holds row
+  // t % 128 of waves k = t / 128 + 2 j
+  constexpr int NPF = 4 * QT;
+  static_assert(TPA_GRAM64_ROWS == 128, "the staging map assumes 128 rows per step");
   __shared__ double Zs[16 * QT * TPA_GRAM64_STRIDE];
   __shared__ double Ws[TPA_GRAM64_ROWS];
   const int t = threadIdx.x;
@@ -2348,32 +2373,46 @@ __global__ __launch_bounds__(256) void k_gram_f64(const double2* __restrict__ h,
   const int wave = t >> 6;
   const int Q = 2 * P;
   const int64_t R = N * lam_pad;
+  const int r_st = t & (TPA_GRAM64_ROWS - 1), k_st = t >> 7;
   tpa_doublex4 acc[NT];
 #pragma unroll
   for (int q = 0; q < NT; ++q) acc[q] = tpa_doublex4{0.0, 0.0, 0.0, 0.0};
-  for (int64_t rb = (int64_t)blockIdx.x * TPA_GRAM64_ROWS; rb < R;
-       rb += (int64_t)gridDim.x * TPA_GRAM64_ROWS) {
-    // stage: thread t handles rows t and t - 128 ... (128 rows x Q columns)
-    for (int idx = t; idx < TPA_GRAM64_ROWS * P; idx += 256) {
-      const int r = idx % TPA_GRAM64_ROWS, k = idx / TPA_GRAM64_ROWS;
-      const int64_t row = rb + r;
-      const double2 v = row < R ? load_pw<true>(&h[(int64_t)k * R + row]) : make_double2(0.0, 0.0);
-      Zs[k * TPA_GRAM64_STRIDE + r] = v.x;
-      Zs[(P + k) * TPA_GRAM64_STRIDE + r] = v.y;
+  // the padding columns Q .. 16 QT - 1 are never written by the staging: zero them once
+  for (int idx = t; idx < (16 * QT - Q) * TPA_GRAM64_ROWS; idx += 256)
+    Zs[(Q + idx / TPA_GRAM64_ROWS) * TPA_GRAM64_STRIDE + idx % TPA_GRAM64_ROWS] = 0.0;
+  double2 pf[NPF];
+  double wpf = 0.0;
+  auto fetch = [&](int64_t rb) {
+    const int64_t row = rb + r_st;
+#pragma unroll
+    for (int j = 0; j < NPF; ++j) {
+      const int k = k_st + 2 * j;
+      pf[j] = (k < P && row < R) ? load_pw<true>(&h[(int64_t)k * R + row])
+                                 : make_double2(0.0, 0.0);
     }
-    for (int idx = t; idx < (16 * QT - Q) * TPA_GRAM64_ROWS; idx += 256)
-      Zs[(Q + idx / TPA_GRAM64_ROWS) * TPA_GRAM64_STRIDE + idx % TPA_GRAM64_ROWS] = 0.0;
-    if (t < TPA_GRAM64_ROWS) {
-      const int64_t row = rb + t;
-      double wr = 0.0;
-      if (row < R) {
-        const int64_t e = row / lam_pad;
-        const int l = (int)(row - e * lam_pad);
-        wr = l < lam ? w[e] : 0.0;
+    wpf = 0.0;
+    if (t < TPA_GRAM64_ROWS && row < R) {
+      const int64_t e = row / lam_pad;
+      const int l = (int)(row - e * lam_pad);
+      wpf = l < lam ? w[e] : 0.0;
+    }
+  };
+  const int64_t step = (int64_t)gridDim.x * TPA_GRAM64_ROWS;
+  int64_t rb = (int64_t)blockIdx.x * TPA_GRAM64_ROWS;
+  if (rb < R) fetch(rb);
+  for (; rb < R; rb += step) {
+    // stage the rows fetched during the previous step (128 rows x Q columns)
+#pragma unroll
+    for (int j = 0; j < NPF; ++j) {
+      const int k = k_st + 2 * j;
+      if (k < P) {
+        Zs[k * TPA_GRAM64_STRIDE + r_st] = pf[j].x;
+        Zs[(P + k) * TPA_GRAM64_STRIDE + r_st] = pf[j].y;
       }
-      Ws[t] = wr;
     }
+    if (t < TPA_GRAM64_ROWS) Ws[t] = wpf;
     __syncthreads();
+    if (rb + step < R) fetch(rb + step);  // in flight during this step's MFMAs
     const int i = lane & 15, kk = lane >> 4;
 #pragma unroll
     for (int s = 0; s < 8; ++s) {
