@@ -108,12 +108,20 @@ inline void to_y(const CouplingMap* S, const double* x, double* y, double* dydx)
   }
 }
 
+// Inverse of to_y. y is clamped into the bound first (tf_pwa/variable.py Bound.get_y2x), so a
+// start value outside a one-sided range maps to its edge instead of NaN.
 inline double to_x(const CouplingMap* S, int j, double y) {
   const double a = S->bound_ab[2 * j], b = S->bound_ab[2 * j + 1];
   switch (S->bound_kind[j]) {
     case 1: return std::asin(std::min(1.0, std::max(-1.0, 2.0 * (y - a) / (b - a) - 1.0)));
-    case 2: return std::sqrt((y - a + 1.0) * (y - a + 1.0) - 1.0);
-    case 3: return std::sqrt((b - y + 1.0) * (b - y + 1.0) - 1.0);
+    case 2: {
+      const double u = std::max(y, a) - a + 1.0;
+      return std::sqrt(u * u - 1.0);
+    }
+    case 3: {
+      const double u = b - std::min(y, b) + 1.0;
+      return std::sqrt(u * u - 1.0);
+    }
     default: return y;
   }
 }

@@ -35,6 +35,16 @@ namespace {
 
 using namespace tpa_cpl;
 using Clock = std::chrono::steady_clock;
+// The batcher's one timed wait (the gathering window) runs on the monotonic clock. GCC 11's
+// ThreadSanitizer runtime does not intercept pthread_cond_clockwait, which libstdc++ uses for
+// steady_clock waits, so it misses the mutex being released during them and reports every
+// such wait as a double lock; the ThreadSanitizer build of the host tests defines
+// TPA_SERVE_WAIT_SYSTEM_CLOCK to wait on the system clock there (pthread_cond_timedwait).
+#ifdef TPA_SERVE_WAIT_SYSTEM_CLOCK
+using WaitClock = std::chrono::system_clock;
+#else
+using WaitClock = std::chrono::steady_clock;
+#endif
 
 struct Cancelled {};
 struct EngineFailure {};
@@ -306,7 +316,7 @@ void batcher_main(tpa_server_s* S) {
       if (run >= 0)
         nap();
       else
-        S->cv.wait_until(lk, deadline);
+        S->cv.wait_until(lk, WaitClock::now() + (deadline - now));
     }
     if (S->stop) break;
     const int nb = run == 0 ? 1 : 0;
