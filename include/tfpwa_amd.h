@@ -48,6 +48,11 @@ typedef struct tpa_fcn_s* tpa_fcn_t;
  * the bytes per event; the values are widened to FP64 on load and every operation after it is
  * FP64. Not with TPA_FLAG_SHAPE. */
 #define TPA_FLAG_F32 8
+/* Read the helicity-halved FP32 copy (tpa_events_keep_half) instead of the FP64 cache: the
+ * slots of one helicity of each mirror pair in FP32 plus one phase per chain and event, from
+ * which the mirrored slots follow (h_k,-o = eta_k s_o rho conj(h_k,o)); every operation after
+ * the load is FP64. At most 32 partial waves; not with TPA_FLAG_SHAPE or TPA_FLAG_F32. */
+#define TPA_FLAG_F32H 16
 /* Largest model (partial waves) tpa_eval_fused accepts. */
 #define TPA_FUSED_MAX_P 8
 
@@ -121,6 +126,32 @@ int tpa_events_build(tpa_events_t ev, int fold);
  * of the data pass of the mixed-precision likelihood (the reference evaluates
  * sum_gradient, tf_pwa/model/model.py:41-94, in FP64 from the angle cache). */
 int tpa_events_keep_f32(tpa_events_t ev, int on);
+
+/* on != 0: keep the helicity-halved FP32 copy of the folded cache for tpa_eval with
+ * TPA_FLAG_F32H, made now if the cache is built and refreshed by every later folded build;
+ * on = 0 frees it. Flipping every helicity of a partial wave conjugates its angular part, so
+ * for each mirror pair of outer helicity slots (o, -o) only o is stored (FP32), with the phase
+ * B/conj(B) of each chain's lineshape product per event (FP32): 408 instead of 712 B per event
+ * for the six-resonance model. Making the copy checks on every event that the stored half
+ * reproduces the mirrored slots of the FP64 cache; a relative residual above 1e-9 (a model
+ * whose helicity lists are not symmetric) refuses the copy with TPA_ERR_UNSUPPORTED.
+ * No reference counterpart: a storage option of the data pass of the mixed-precision
+ * likelihood (the reference evaluates sum_gradient, tf_pwa/model/model.py:41-94, in FP64 from
+ * the angle cache; the helicity amplitudes follow tf_pwa/amp/core.py:931-1164). */
+int tpa_events_keep_half(tpa_events_t ev, int on);
+
+/* The largest relative residual the last helicity-halved copy of ev measured
+ * (max |h_k,-o - eta_k s_o rho conj(h_k,o)| / max |h| per event, in FP64 before rounding);
+ * -1 if no copy was made. No reference counterpart (a diagnostic of tpa_events_keep_half). */
+int tpa_events_half_residual(tpa_events_t ev, double* out);
+
+/* Copy the helicity-halved copy to the host: g [P][n][nh] and rho [n_chain][n] as interleaved
+ * FP32 complex, and info: info[0] = nh, info[1+q] the stored slot q, info[5+q] its mirror,
+ * info[9+q] its sign s_o (q < nh), info[13+k] = 1 where wave k is stored times i (phi_k = -i),
+ * info[13+P+k] the chain of wave k (info_len >= 13 + 2P). So h_k,o = phi_k g_k,o and
+ * h_k,-o = phi_k s_o rho_c conj(g_k,o). No reference counterpart (a diagnostic). */
+int tpa_events_get_half(tpa_events_t ev, float* g, int64_t g_len, float* rho, int64_t rho_len,
+                        int* info, int info_len);
 
 /* Replace the model description (same chains and CG table; new lineshape parameters or
  * barrier clamp flags). Event sets of the model must be rebuilt (tpa_events_build) before
@@ -429,6 +460,12 @@ int tpa_fcn_set_shape_map(tpa_fcn_t fcn, tpa_model_t model, const TpaShapeMap* m
  * only. The MC side is unchanged (FP64 passes or the Gram matrix). No reference counterpart:
  * the storage option of the data pass behind FCN.nll_grad (tf_pwa/model/model.py:1207-1229). */
 int tpa_fcn_set_data_storage(tpa_fcn_t fcn, int data_f32);
+
+/* The session's data passes read the helicity-halved FP32 copies of the data caches
+ * (TPA_FLAG_F32H, tpa_events_keep_half) when on != 0, the FP64 caches when on = 0. Fixed
+ * lineshapes only. No reference counterpart: a storage option of the data pass behind
+ * FCN.nll_grad (tf_pwa/model/model.py:1207-1229). */
+int tpa_fcn_set_data_half(tpa_fcn_t fcn, int on);
 
 /* Cache rebuilds the session made after barrier clamp flags changed (no reference
  * counterpart: a bookkeeping counter for hosts that track the caches' state). */

@@ -10,12 +10,14 @@ OUT = os.path.join(HERE, "lib", "libtfpwa_cpu.so")
 
 def build(verbose=False):
     os.makedirs(os.path.dirname(OUT), exist_ok=True)
+    # a per-process temporary, so concurrent builds (parallel test workers) never share one
+    tmp = "{}.tmp{:x}".format(OUT, os.getpid())
     cmd = ["gcc", "-O3", "-march=x86-64-v3", "-fopenmp", "-fPIC", "-shared", "-std=c11",
-           "-o", OUT + ".tmp", os.path.join(HERE, "cpu_eval.c"), "-lm"]
+           "-o", tmp, os.path.join(HERE, "cpu_eval.c"), "-lm"]
     if verbose:
         print(" ".join(cmd), flush=True)
     subprocess.run(cmd, check=True)
-    os.replace(OUT + ".tmp", OUT)
+    os.replace(tmp, OUT)
     return OUT
 
 

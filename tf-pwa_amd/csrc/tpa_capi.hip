@@ -21,6 +21,7 @@
 #include "tpa_internal.h"
 #include "tpa_kernels.h"
 #include "tpa_kernels4.h"
+#include "tpa_half.h"
 #include "tpa_rtc_headers.inc"  // kRtcHeaders / kRtcHeaderNames (written by build.py)
 
 namespace {
@@ -246,6 +247,19 @@ struct tpa_events_s {
   float2* d_h32 = nullptr;  // FP32 copy of a folded cache (tpa_events_keep_f32)
   bool keep32 = false;      // refresh d_h32 after every folded build
   bool h32_valid = false;
+  // helicity-halved FP32 copy (tpa_events_keep_half): stored half [P][n][nh], chain phases
+  // [n_chain][n], the phi_k of each wave (1: -i) and the chain of each wave
+  float2* d_g32h = nullptr;
+  float2* d_rho = nullptr;
+  unsigned long long* d_resid = nullptr;
+  bool keep32h = false;
+  bool h32h_valid = false;
+  int half_nh = 0;
+  int half_slot[TPA_HALF_MAX_NH] = {}, half_mirror[TPA_HALF_MAX_NH] = {},
+      half_sgn[TPA_HALF_MAX_NH] = {};
+  double half_resid = -1.0;
+  std::vector<unsigned char> half_rot;
+  std::vector<signed char> half_kch;
   unsigned* d_cnt = nullptr;  // k_reduce's finishing-block counter (zero between calls)
   uint64_t* h_flag = nullptr; // pinned: the sequence number of the last completed reduction
   uint64_t seq = 0;
@@ -354,6 +368,38 @@ static int dispatch_kg(int kg, const EvalArgs& a, int grid, hipStream_t s) {
              return TPA_ERR_UNSUPPORTED;
     default: return TPA_ERR_UNSUPPORTED;
   }
+}
+
+// The helicity-halved pass (k_eval_half32): KG lanes per event, PML = ceil(P / KG) rounded up
+// to 4, 6 or 8 (P <= 32: the coefficients travel in the kernel arguments).
+static int half_kgroups(int P) { return P <= 8 ? 1 : P <= 16 ? 2 : 4; }
+
+template <int NH, int KG>
+static void launch_half(const EvalArgs& a, int grid, hipStream_t s) {
+  const int need = (a.P + KG - 1) / KG;
+  if (need <= 4)
+    hipLaunchKernelGGL((k_eval_half32<NH, KG, 4>), dim3(grid), dim3(256), 0, s, a);
+  else if (need <= 6)
+    hipLaunchKernelGGL((k_eval_half32<NH, KG, 6>), dim3(grid), dim3(256), 0, s, a);
+  else
+    hipLaunchKernelGGL((k_eval_half32<NH, KG, 8>), dim3(grid), dim3(256), 0, s, a);
+}
+
+static int dispatch_half(int nh, int kg, const EvalArgs& a, int grid, hipStream_t s) {
+  if (a.P > 4 * 8) return TPA_ERR_UNSUPPORTED;
+  if (nh == 1) {
+    if (kg == 1) launch_half<1, 1>(a, grid, s);
+    else if (kg == 2) launch_half<1, 2>(a, grid, s);
+    else launch_half<1, 4>(a, grid, s);
+    return TPA_OK;
+  }
+  if (nh == 2) {
+    if (kg == 1) launch_half<2, 1>(a, grid, s);
+    else if (kg == 2) launch_half<2, 2>(a, grid, s);
+    else launch_half<2, 4>(a, grid, s);
+    return TPA_OK;
+  }
+  return TPA_ERR_UNSUPPORTED;
 }
 
 static int dispatch_eval(int lg, int kg, const EvalArgs& a, int grid, hipStream_t s) {
@@ -841,6 +887,9 @@ int tpa_events_destroy(tpa_events_t ev) {
   if (ev->d_ang) (void)hipFree(ev->d_ang);
   if (ev->d_h) (void)hipFree(ev->d_h);
   if (ev->d_h32) (void)hipFree(ev->d_h32);
+  if (ev->d_g32h) (void)hipFree(ev->d_g32h);
+  if (ev->d_rho) (void)hipFree(ev->d_rho);
+  if (ev->d_resid) (void)hipFree(ev->d_resid);
   if (ev->d_cnt) (void)hipFree(ev->d_cnt);
   if (ev->h_flag) (void)hipHostFree(ev->h_flag);
   if (ev->d_w) (void)hipFree(ev->d_w);
@@ -1071,6 +1120,163 @@ static int convert_f32(tpa_events_s* ev) {
   return TPA_OK;
 }
 
+// The helicity-halved copy of ev's folded cache (see tpa_half.h). The host finds each slot's
+// mirror and sign s_o from the helicity lists, and each wave's sign relative to its chain's
+// reference wave from a sample of events (the sign of sum Re[h_k,-o conj(h_r,-o) h_k,o
+// conj(h_r,o)] = eta_k eta_r sum |h_k,o|^2 |h_r,o|^2); k_pw_to_half writes the copy and
+// measures the residual of the relation on every event, which the host checks.
+#define TPA_HALF_SAMPLE 512
+#define TPA_HALF_TOL 1e-9
+static int convert_half(tpa_events_s* ev) {
+  const tpa_model_s* m = ev->m;
+  const TpaModel& M = m->h;
+  const int L = M.lam;
+  const int Lp = M.lam_pad;
+  if (M.P > TPA_EVAL_CIN)
+    return fail(TPA_ERR_UNSUPPORTED, "the helicity-halved copy serves at most 32 partial waves");
+  // helicity (2 lambda) of every outer slot: the top's, then the finals' (three-body); the
+  // top's alone (four-body finals are spin 0)
+  std::vector<std::vector<int>> hel(L);
+  if (m->kind == 4) {
+    if (L != M.n_top) return fail(TPA_ERR_UNSUPPORTED, "unexpected four-body slot count");
+    for (int i = 0; i < L; ++i) hel[i] = {M.hel_top[i]};
+  } else {
+    int o = 0;
+    for (int iA = 0; iA < M.n_top; ++iA)
+      for (int j0 = 0; j0 < M.n_fh[0]; ++j0)
+        for (int j1 = 0; j1 < M.n_fh[1]; ++j1)
+          for (int j2 = 0; j2 < M.n_fh[2]; ++j2, ++o) {
+            if (o >= L) return fail(TPA_ERR_UNSUPPORTED, "unexpected slot count");
+            hel[o] = {M.hel_top[iA], M.hel_f[0][j0], M.hel_f[1][j1], M.hel_f[2][j2]};
+          }
+    if (o != L) return fail(TPA_ERR_UNSUPPORTED, "unexpected slot count");
+  }
+  HalfCopyArgs a{};
+  int nh = 0;
+  for (int o = 0; o < L; ++o) {
+    std::vector<int> neg(hel[o].size());
+    int sum = 0;
+    for (size_t i = 0; i < neg.size(); ++i) {
+      neg[i] = -hel[o][i];
+      sum += hel[o][i];
+    }
+    int mo = -1;
+    for (int q = 0; q < L; ++q)
+      if (hel[q] == neg) mo = q;
+    if (mo < 0 || mo == o || (sum % 2) != 0)
+      return fail(TPA_ERR_UNSUPPORTED, "helicity lists without a mirror for every slot");
+    if (o < mo) {
+      if (nh >= TPA_HALF_MAX_NH) return fail(TPA_ERR_UNSUPPORTED, "too many helicity slots");
+      a.slot[nh] = o;
+      a.mirror[nh] = mo;
+      a.sgn[nh] = ((sum / 2) % 2 == 0) ? 1 : -1;  // (-1)^(sum of helicities)
+      ++nh;
+    }
+  }
+  if (nh != 1 && nh != 2)
+    return fail(TPA_ERR_UNSUPPORTED, "the helicity-halved pass serves 2 or 4 helicity slots");
+  const int n_chain = m->kind == 4 ? m->h4.n_chain : M.n_chain;
+  std::vector<int> k0(n_chain), kn(n_chain);
+  for (int c = 0; c < n_chain; ++c) {
+    if (m->kind == 4) {
+      const Tpa4Chain& C = m->h4.chain[c];
+      k0[c] = C.k0;
+      kn[c] = C.n_ls[0] * C.n_ls[1] * C.n_ls[2];
+    } else {
+      const TpaChain& C = M.chain[c];
+      k0[c] = C.k0;
+      kn[c] = C.n_ls1 * C.n_ls2;
+    }
+  }
+  // wave signs relative to each chain's reference wave, from a sample of events
+  const int64_t ns = std::min<int64_t>(ev->n, TPA_HALF_SAMPLE);
+  std::vector<double2> smp((size_t)M.P * ns * Lp);
+  HIPCHK(hipMemcpy2DAsync(smp.data(), sizeof(double2) * ns * Lp, ev->d_h,
+                          sizeof(double2) * ev->n * Lp, sizeof(double2) * ns * Lp, M.P,
+                          hipMemcpyDeviceToHost, m->stream));
+  HIPCHK(hipStreamSynchronize(m->stream));
+  auto H = [&](int k, int64_t e, int o) { return smp[((size_t)k * ns + e) * Lp + o]; };
+  ev->half_rot.assign(M.P, 0);
+  ev->half_kch.assign(TPA_EVAL_CIN, 0);
+  for (int c = 0; c < n_chain; ++c) {
+    int r = k0[c];
+    double rn = -1.0;
+    for (int k = k0[c]; k < k0[c] + kn[c]; ++k) {
+      double nk = 0.0;
+      for (int64_t e = 0; e < ns; ++e)
+        for (int q = 0; q < nh; ++q) {
+          const double2 v = H(k, e, a.slot[q]);
+          nk += v.x * v.x + v.y * v.y;
+        }
+      if (nk > rn) {
+        rn = nk;
+        r = k;
+      }
+    }
+    for (int k = k0[c]; k < k0[c] + kn[c]; ++k) {
+      double sk = 0.0;
+      for (int64_t e = 0; e < ns; ++e)
+        for (int q = 0; q < nh; ++q) {
+          const double2 hm = H(k, e, a.mirror[q]), rm = H(r, e, a.mirror[q]);
+          const double2 hk = H(k, e, a.slot[q]), rk = H(r, e, a.slot[q]);
+          // x = h_k,-o conj(h_r,-o), y = h_k,o conj(h_r,o); Re(x y)
+          const double xr = hm.x * rm.x + hm.y * rm.y, xi = hm.y * rm.x - hm.x * rm.y;
+          const double yr = hk.x * rk.x + hk.y * rk.y, yi = hk.y * rk.x - hk.x * rk.y;
+          sk += xr * yr - xi * yi;
+        }
+      ev->half_rot[k] = sk < 0.0 ? 1 : 0;
+      ev->half_kch[k] = (signed char)c;
+    }
+  }
+  const size_t ng = (size_t)M.P * ev->n * nh, nr = (size_t)n_chain * ev->n;
+  if (ev->half_nh != nh && ev->d_g32h) {
+    HIPCHK(hipFree(ev->d_g32h));
+    ev->d_g32h = nullptr;
+  }
+  if (!ev->d_g32h) HIPCHK(hipMalloc(&ev->d_g32h, sizeof(float2) * ng));
+  if (!ev->d_rho) HIPCHK(hipMalloc(&ev->d_rho, sizeof(float2) * nr));
+  if (!ev->d_resid) HIPCHK(hipMalloc(&ev->d_resid, sizeof(unsigned long long)));
+  ev->half_nh = nh;
+  for (int q = 0; q < nh; ++q) {
+    ev->half_slot[q] = a.slot[q];
+    ev->half_mirror[q] = a.mirror[q];
+    ev->half_sgn[q] = a.sgn[q];
+  }
+  a.h = ev->d_h;
+  a.g = ev->d_g32h;
+  a.rho = ev->d_rho;
+  a.resid = ev->d_resid;
+  a.n = ev->n;
+  a.S = ev->n;
+  a.P = M.P;
+  a.lam_pad = Lp;
+  a.nh = nh;
+  a.n_chain = n_chain;
+  for (int c = 0; c < n_chain; ++c) {
+    a.ch_k0[c] = k0[c];
+    a.ch_n[c] = kn[c];
+  }
+  for (int k = 0; k < M.P; ++k) a.rot[k] = ev->half_rot[k];
+  unsigned long long zero = 0, bits = 0;
+  HIPCHK(hipMemcpyAsync(ev->d_resid, &zero, sizeof(zero), hipMemcpyHostToDevice, m->stream));
+  const int64_t grid = std::max<int64_t>(1, std::min<int64_t>((ev->n + 255) / 256, 4096));
+  hipLaunchKernelGGL(k_pw_to_half, dim3((unsigned)grid), dim3(256), 0, m->stream, a);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(&bits, ev->d_resid, sizeof(bits), hipMemcpyDeviceToHost, m->stream));
+  HIPCHK(hipStreamSynchronize(m->stream));
+  double r;
+  std::memcpy(&r, &bits, sizeof(r));
+  ev->half_resid = r;
+  if (!(r <= TPA_HALF_TOL)) {
+    ev->h32h_valid = false;
+    return fail(TPA_ERR_UNSUPPORTED,
+                "the helicity-flip relation does not hold for this model's cache (relative "
+                "residual " + std::to_string(r) + ")");
+  }
+  ev->h32h_valid = true;
+  return TPA_OK;
+}
+
 int tpa_events_build(tpa_events_t ev, int fold) {
   if (!ev) return fail(TPA_ERR_ARG, "null pointer");
   const TpaModel& M = ev->m->h;
@@ -1081,8 +1287,13 @@ int tpa_events_build(tpa_events_t ev, int fold) {
   HIPCHK(hipGetLastError());
   ev->built_fold = fold ? 1 : 0;
   ev->h32_valid = false;
+  ev->h32h_valid = false;
   if (ev->keep32 && fold) {
     const int rc = convert_f32(ev);
+    if (rc != TPA_OK) return rc;
+  }
+  if (ev->keep32h && fold) {
+    const int rc = convert_half(ev);
     if (rc != TPA_OK) return rc;
   }
   HIPCHK(hipStreamSynchronize(ev->m->stream));
@@ -1104,6 +1315,61 @@ int tpa_events_keep_f32(tpa_events_t ev, int on) {
     if (rc != TPA_OK) return rc;
     HIPCHK(hipStreamSynchronize(ev->m->stream));
   }
+  return TPA_OK;
+}
+
+int tpa_events_keep_half(tpa_events_t ev, int on) {
+  if (!ev) return fail(TPA_ERR_ARG, "null pointer");
+  if (!on) {
+    HIPCHK(hipStreamSynchronize(ev->m->stream));
+    if (ev->d_g32h) HIPCHK(hipFree(ev->d_g32h));
+    if (ev->d_rho) HIPCHK(hipFree(ev->d_rho));
+    ev->d_g32h = nullptr;
+    ev->d_rho = nullptr;
+    ev->keep32h = ev->h32h_valid = false;
+    return TPA_OK;
+  }
+  ev->keep32h = true;
+  if (ev->d_h && ev->built_fold == 1 && !ev->h32h_valid) {
+    const int rc = convert_half(ev);
+    if (rc != TPA_OK) {
+      ev->keep32h = false;
+      return rc;
+    }
+  }
+  return TPA_OK;
+}
+
+int tpa_events_get_half(tpa_events_t ev, float* g, int64_t g_len, float* rho, int64_t rho_len,
+                        int* info, int info_len) {
+  if (!ev || !g || !rho || !info) return fail(TPA_ERR_ARG, "null pointer");
+  if (!ev->h32h_valid) return fail(TPA_ERR_ARG, "no helicity-halved copy");
+  const tpa_model_s* m = ev->m;
+  const int P = m->h.P;
+  const int n_chain = m->kind == 4 ? m->h4.n_chain : m->h.n_chain;
+  const int64_t ng = 2 * (int64_t)P * ev->n * ev->half_nh, nr = 2 * (int64_t)n_chain * ev->n;
+  if (g_len < ng || rho_len < nr || info_len < 13 + 2 * P)
+    return fail(TPA_ERR_ARG, "output buffer too small");
+  HIPCHK(hipMemcpyAsync(g, ev->d_g32h, sizeof(float) * ng, hipMemcpyDeviceToHost, m->stream));
+  HIPCHK(hipMemcpyAsync(rho, ev->d_rho, sizeof(float) * nr, hipMemcpyDeviceToHost, m->stream));
+  HIPCHK(hipStreamSynchronize(m->stream));
+  std::fill(info, info + 13 + 2 * P, 0);
+  info[0] = ev->half_nh;
+  for (int q = 0; q < ev->half_nh; ++q) {
+    info[1 + q] = ev->half_slot[q];
+    info[5 + q] = ev->half_mirror[q];
+    info[9 + q] = ev->half_sgn[q];
+  }
+  for (int k = 0; k < P; ++k) {
+    info[13 + k] = ev->half_rot[k];
+    info[13 + P + k] = ev->half_kch[k];
+  }
+  return TPA_OK;
+}
+
+int tpa_events_half_residual(tpa_events_t ev, double* out) {
+  if (!ev || !out) return fail(TPA_ERR_ARG, "null pointer");
+  *out = ev->half_resid;
   return TPA_OK;
 }
 
@@ -1156,12 +1422,27 @@ static int eval_impl(tpa_events_s* ev, const double* coef, int mode, int flags, 
   if (f32 && shape) return fail(TPA_ERR_UNSUPPORTED, "FP32 cache reads need a folded cache");
   if (f32 && !ev->h32_valid)
     return fail(TPA_ERR_ARG, "no FP32 copy of the cache (tpa_events_keep_f32)");
+  const bool half = (flags & TPA_FLAG_F32H) != 0;
+  if (half && (shape || f32))
+    return fail(TPA_ERR_UNSUPPORTED, "TPA_FLAG_F32H excludes TPA_FLAG_SHAPE and TPA_FLAG_F32");
+  if (half && !ev->h32h_valid)
+    return fail(TPA_ERR_ARG, "no helicity-halved copy of the cache (tpa_events_keep_half)");
+  if (half && !args_coef)
+    return fail(TPA_ERR_UNSUPPORTED, "TPA_FLAG_F32H takes at most 32 coefficients");
   EvalArgs a;
   a.h = ev->d_h;
-  a.h32 = f32 ? ev->d_h32 : nullptr;
+  a.h32 = f32 ? ev->d_h32 : half ? ev->d_g32h : nullptr;
+  a.rho = half ? ev->d_rho : nullptr;
   a.w = ev->d_w;
   a.coef = args_coef ? nullptr : ev->d_coef;
   if (args_coef) std::memcpy(a.cin, coef, sizeof(double2) * M.P);
+  if (half) {
+    // c~_k = phi_k c_k: phi = -i multiplies (re, im) into (im, -re)
+    for (int k = 0; k < M.P; ++k) {
+      if (ev->half_rot[k]) a.cin[k] = make_double2(coef[2 * k + 1], -coef[2 * k]);
+      a.kch[k] = ev->half_kch[k];
+    }
+  }
   a.partials = ev->d_part;
   a.f_out = want_f ? ev->d_f : nullptr;
   a.N = ev->n;
@@ -1177,7 +1458,9 @@ static int eval_impl(tpa_events_s* ev, const double* coef, int mode, int flags, 
   a.nacc = shape ? 2 + 6 * M.P : 2 + 2 * M.P;
   const int kg = eval_kgroups(M.lam_pad, M.P);
   if (kg < 0) return fail(TPA_ERR_UNSUPPORTED, "too many partial waves for this helicity count");
-  int grid = eval_grid(ev->n, f32 ? f32_lanes_per_event(M.lam_pad, kg) : M.lam_pad * kg);
+  const int kgh = half_kgroups(M.P);
+  int grid = half ? eval_grid(ev->n, kgh)
+                  : eval_grid(ev->n, f32 ? f32_lanes_per_event(M.lam_pad, kg) : M.lam_pad * kg);
   bool timing = false;
   if (!async) {
     std::lock_guard<std::mutex> lk(g_timing_mu);
@@ -1207,7 +1490,8 @@ static int eval_impl(tpa_events_s* ev, const double* coef, int mode, int flags, 
     grid = eval_grid(ev->n, M.lam_pad * kgs);
     rc = dispatch_shape(M.lam_pad, kgs, sa, grid, s);
   } else {
-    rc = dispatch_eval(M.lam_pad, kg, a, grid, s);
+    rc = half ? dispatch_half(ev->half_nh, kgh, a, grid, s)
+              : dispatch_eval(M.lam_pad, kg, a, grid, s);
   }
   if (rc != TPA_OK) return fail(rc, "no evaluation kernel for this helicity/partial-wave count");
   HIPCHK(hipGetLastError());
@@ -1253,6 +1537,15 @@ static int eval_impl(tpa_events_s* ev, const double* coef, int mode, int flags, 
   }
   if (!done) HIPCHK(hipStreamSynchronize(s));
   std::memcpy(out, h_out, sizeof(double) * nout);
+  if (half && a.grad) {
+    // G_k = phi_k G~_k: phi = -i maps (re, im) to (im, -re)
+    for (int k = 0; k < M.P; ++k)
+      if (ev->half_rot[k]) {
+        const double re = out[2 + 2 * k], im = out[3 + 2 * k];
+        out[2 + 2 * k] = im;
+        out[3 + 2 * k] = -re;
+      }
+  }
   if (timing) {
     HIPCHK(hipEventSynchronize(ev->t1));
     float ms = 0.0f;

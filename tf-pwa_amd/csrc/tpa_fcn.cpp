@@ -288,6 +288,18 @@ int tpa_fcn_set_data_storage(tpa_fcn_t F, int data_f32) {
   return TPA_OK;
 }
 
+int tpa_fcn_set_data_half(tpa_fcn_t F, int on) {
+  if (!F) return tpa_set_error(TPA_ERR_ARG, "null pointer");
+  if (on && F->shape)
+    return tpa_set_error(TPA_ERR_UNSUPPORTED, "halved data caches need fixed lineshapes");
+  for (tpa_events_t ev : F->data) {
+    const int rc = tpa_events_keep_half(ev, on ? 1 : 0);
+    if (rc != TPA_OK) return rc;
+  }
+  F->data_flags = on ? TPA_FLAG_F32H : 0;
+  return TPA_OK;
+}
+
 int tpa_fcn_cache_rebuilds(tpa_fcn_t F, int64_t* n) {
   if (!F || !n) return tpa_set_error(TPA_ERR_ARG, "null pointer");
   *n = F->rebuilds;

@@ -1,0 +1,345 @@
+// Helicity-halved FP32 copy of a folded partial-wave cache, and the evaluation pass over it.
+//
+// Flipping the sign of every helicity in a partial wave conjugates its angular part: the
+// Wigner functions obey D^j_{-m,-n} = (-1)^(m-n) conj(D^j_{m,n}) (d^j is real), the LS
+// couplings obey the CG symmetry <j1 -m1 j2 -m2|J -M> = (-1)^(j1+j2-J) <j1 m1 j2 m2|J M>, and
+// the barrier factors are real. With the lineshape product B of the wave's chain folded in,
+// the folded wave h_k,o (o an outer helicity slot, -o the slot with every helicity negated)
+// satisfies, per event,
+//
+//     h_k,-o = eta_k s_o rho conj(h_k,o),   rho = B / conj(B),  eta_k, s_o in {+1, -1},
+//
+// where s_o depends only on the slot and eta_k only on the wave. So half of the slots carry
+// the whole cache. The copy stores, for each wave and event, the NH = Lambda/2 slots of H (one
+// slot of each mirror pair) in FP32, each wave multiplied by conj(phi_k) with phi_k in {1, -i}
+// chosen so that conj(phi_k)^2 eta_k is the same for all waves of a chain, and per chain and
+// event the phase rho' = g_-o / (s_o conj(g_o)) (FP32), with g = conj(phi) h. Then
+//
+//     A_o = sum_k c~_k g_k,o,   A_-o = s_o rho' sum_k c~_k conj(g_k,o),   c~_k = phi_k c_k,
+//
+// and f = sum_o |A_o|^2 + |A_-o|^2. The copy kernel measures, on every event, how far the
+// stored half and rho' are from reproducing the mirrored slots of the FP64 cache; the host
+// refuses the copy when that residual exceeds FP64 rounding (a model whose helicity lists
+// are not symmetric, say).
+//
+// Bytes per event at six resonances (P = 22, Lambda = 4): 22 x 2 x 8 B of waves + 6 x 8 B of
+// phases + the weight = 408 B, against 712 B for the full FP32 copy.
+#pragma once
+#include "tpa_kernels.h"
+
+#define TPA_HALF_MAX_NH 4
+
+struct HalfCopyArgs {
+  const double2* __restrict__ h;        // [P][S][lam_pad] folded FP64 cache
+  float2* __restrict__ g;               // [P][n][nh] stored half, times conj(phi_k)
+  float2* __restrict__ rho;             // [n_chain][n] chain phases rho'
+  unsigned long long* __restrict__ resid;  // largest relative residual (bits of a double >= 0)
+  int64_t n, S;
+  int P, lam_pad, nh, n_chain;
+  int slot[TPA_HALF_MAX_NH], mirror[TPA_HALF_MAX_NH], sgn[TPA_HALF_MAX_NH];
+  int ch_k0[TPA_MAX_CHAIN], ch_n[TPA_MAX_CHAIN];
+  unsigned char rot[TPA_MAX_P];         // 1: phi_k = -i (the wave is stored times i)
+};
+
+// conj(phi) v: v for phi = 1, i v for phi = -i
+TPA_DEV cd half_rot(double2 v, int r) {
+  return r ? c_make(-v.y, v.x) : c_make(v.x, v.y);
+}
+
+// One thread per event: every chain's reference phase from its largest stored wave, the
+// FP32 half and phases, and the event's residual max |g_-o - s_o rho' conj(g_o)| relative to
+// the event's largest wave; one atomic max per wave of threads.
+__global__ __launch_bounds__(256) void k_pw_to_half(HalfCopyArgs a) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  double worst = 0.0;
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < a.n; e += stride) {
+    double res = 0.0, scale = 0.0;
+    for (int c = 0; c < a.n_chain; ++c) {
+      const int k0 = a.ch_k0[c], k1 = a.ch_k0[c] + a.ch_n[c];
+      double best = 0.0;
+      cd gb = c_make(0.0, 0.0), gm = c_make(0.0, 0.0);
+      int sb = 1;
+      for (int k = k0; k < k1; ++k) {
+        const double2* hk = a.h + ((int64_t)k * a.S + e) * a.lam_pad;
+        for (int o = 0; o < a.nh; ++o) {
+          const double2 v = hk[a.slot[o]];
+          const double m = v.x * v.x + v.y * v.y;
+          if (m > best) {
+            best = m;
+            gb = half_rot(v, a.rot[k]);
+            gm = half_rot(hk[a.mirror[o]], a.rot[k]);
+            sb = a.sgn[o];
+          }
+        }
+      }
+      // rho' = g_-o / (s_o conj(g_o)) = g_-o g_o / (s_o |g_o|^2); any phase where the chain
+      // vanishes (it multiplies zero)
+      const cd rho = best > 0.0 ? c_scale(c_mul(gm, gb), (double)sb / best) : c_make(1.0, 0.0);
+      a.rho[(int64_t)c * a.n + e] = make_float2((float)rho.r, (float)rho.i);
+      for (int k = k0; k < k1; ++k) {
+        const double2* hk = a.h + ((int64_t)k * a.S + e) * a.lam_pad;
+        float2* gk = a.g + ((int64_t)k * a.n + e) * a.nh;
+        for (int o = 0; o < a.nh; ++o) {
+          const cd x = half_rot(hk[a.slot[o]], a.rot[k]);
+          const cd xm = half_rot(hk[a.mirror[o]], a.rot[k]);
+          gk[o] = make_float2((float)x.r, (float)x.i);
+          const cd pred = c_scale(c_mul(rho, c_conj(x)), (double)a.sgn[o]);
+          const double dr = xm.r - pred.r, di = xm.i - pred.i;
+          res = fmax(res, dr * dr + di * di);
+          scale = fmax(scale, fmax(x.r * x.r + x.i * x.i, xm.r * xm.r + xm.i * xm.i));
+        }
+      }
+    }
+    // a NaN anywhere makes the residual NaN, whose bits order above every finite value
+    const double rel = scale > 0.0 ? sqrt(res / scale) : (res == 0.0 ? 0.0 : res);
+    worst = (rel > worst || rel != rel) ? rel : worst;
+  }
+#pragma unroll
+  for (int m = 32; m >= 1; m >>= 1) {
+    const double o = __shfl_xor(worst, m, 64);
+    worst = (o > worst || o != o) ? o : worst;
+  }
+  if ((threadIdx.x & 63) == 0)
+    atomicMax(a.resid, (unsigned long long)__double_as_longlong(fabs(worst)) |
+                           (worst != worst ? 0x7ff8000000000000ull : 0ull));
+}
+
+template <int NH>
+struct HalfVec;
+template <>
+struct HalfVec<1> {
+  typedef float2 T;
+  TPA_DEV static T load(const T* p) {
+    typedef float f2v __attribute__((ext_vector_type(2)));
+    const f2v v = __builtin_nontemporal_load(reinterpret_cast<const f2v*>(p));
+    return make_float2(v.x, v.y);
+  }
+  TPA_DEV static void get(const T& v, int o, double& x, double& y) {
+    (void)o;
+    x = v.x;
+    y = v.y;
+  }
+};
+template <>
+struct HalfVec<2> {
+  typedef float4 T;
+  TPA_DEV static T load(const T* p) {
+    typedef float f4v __attribute__((ext_vector_type(4)));
+    const f4v v = __builtin_nontemporal_load(reinterpret_cast<const f4v*>(p));
+    return make_float4(v.x, v.y, v.z, v.w);
+  }
+  TPA_DEV static void get(const T& v, int o, double& x, double& y) {
+    x = o == 0 ? v.x : v.z;
+    y = o == 0 ? v.y : v.w;
+  }
+};
+
+// Empty asm that makes the FP32 registers opaque to common-subexpression elimination, so the
+// adjoint widens them again rather than keeping the first loop's FP64 values live.
+TPA_DEV void half_opaque(float2& v) { asm("" : "+v"(v.x), "+v"(v.y)); }
+TPA_DEV void half_opaque(float4& v) { asm("" : "+v"(v.x), "+v"(v.y), "+v"(v.z), "+v"(v.w)); }
+
+// One event group of the halved pass. A lane holds PML waves of one event (k = kg + KG j) and
+// the chain phase of each; A_o and B_o = A_-o / s_o are summed over the KG lanes of the event.
+// The adjoint per wave: G~_k += u sum_o [conj(A_o) g_k,o + conj(B_o) rho' conj(g_k,o)]
+//                            = sum_o conj(u A_o) g_k,o + rho' conj(sum_o u B_o g_k,o).
+template <int NH, int KG, int PML>
+TPA_DEV void eval_half_group(int mode, bool grad, double* __restrict__ f_out,
+                             const typename HalfVec<NH>::T (&hs)[PML], const float2 (&rs)[PML],
+                             double w, int64_t e, bool valid, bool leader,
+                             const double (&cr)[PML], const double (&ci)[PML], double& vs,
+                             double& vc, double (&gr)[PML], double (&gi)[PML]) {
+  typedef HalfVec<NH> V;
+  const double eps = 1.0e-6;
+  double ar[NH], ai[NH], br[NH], bi[NH];
+#pragma unroll
+  for (int o = 0; o < NH; ++o) ar[o] = ai[o] = br[o] = bi[o] = 0.0;
+#pragma unroll
+  for (int j = 0; j < PML; ++j) {
+    const double rr = rs[j].x, ri = rs[j].y;
+    // t = c~ rho'
+    const double tr = fma(cr[j], rr, -(ci[j] * ri));
+    const double ti = fma(cr[j], ri, ci[j] * rr);
+#pragma unroll
+    for (int o = 0; o < NH; ++o) {
+      double x, y;
+      V::get(hs[j], o, x, y);
+      ar[o] = fma(cr[j], x, ar[o]);
+      ar[o] = fma(-ci[j], y, ar[o]);
+      ai[o] = fma(cr[j], y, ai[o]);
+      ai[o] = fma(ci[j], x, ai[o]);
+      // B_o += t conj(g)
+      br[o] = fma(tr, x, br[o]);
+      br[o] = fma(ti, y, br[o]);
+      bi[o] = fma(ti, x, bi[o]);
+      bi[o] = fma(-tr, y, bi[o]);
+    }
+  }
+  double f2 = 0.0;
+#pragma unroll
+  for (int o = 0; o < NH; ++o) {
+    ar[o] = butterfly<1, KG>(ar[o]);
+    ai[o] = butterfly<1, KG>(ai[o]);
+    br[o] = butterfly<1, KG>(br[o]);
+    bi[o] = butterfly<1, KG>(bi[o]);
+    f2 += (ar[o] * ar[o] + ai[o] * ai[o]) + (br[o] * br[o] + bi[o] * bi[o]);
+  }
+  const double f = f2;
+  const double wv = valid ? w : 0.0;
+  double y, u;
+  if (mode == 0) {
+    // clip_log(f) (tf_pwa/model/model.py:283-292) weighted by w
+    double cl, dcl;
+    if (f > eps) {
+      dcl = tpa_recip_pos(f, hw_rcp);
+      cl = tpa_log_pos(f, hw_rcp);
+    } else {
+      const double t = (f - eps) / eps;
+      cl = (log(eps) + t) - (t * t) / 2.0;
+      dcl = 1.0 / eps - t / eps;
+    }
+    y = wv * cl;
+    u = wv * dcl;
+  } else {
+    y = wv * f;
+    u = wv;
+  }
+  if (f_out != nullptr && valid && leader) f_out[e] = f;
+  neumaier_add(vs, vc, leader ? y : 0.0);
+  if (grad) {
+    double uar[NH], uai[NH], ubr[NH], ubi[NH];
+#pragma unroll
+    for (int o = 0; o < NH; ++o) {
+      uar[o] = u * ar[o];
+      uai[o] = u * ai[o];
+      ubr[o] = u * br[o];
+      ubi[o] = u * bi[o];
+    }
+#pragma unroll
+    for (int j = 0; j < PML; ++j) {
+      typename V::T q = hs[j];
+      half_opaque(q);
+      float2 rq = rs[j];
+      half_opaque(rq);
+      const double rr = rq.x, ri = rq.y;
+      double qr = 0.0, qi = 0.0;
+#pragma unroll
+      for (int o = 0; o < NH; ++o) {
+        double x, yy;
+        V::get(q, o, x, yy);
+        // conj(u A_o) g
+        gr[j] = fma(uar[o], x, gr[j]);
+        gr[j] = fma(uai[o], yy, gr[j]);
+        gi[j] = fma(uar[o], yy, gi[j]);
+        gi[j] = fma(-uai[o], x, gi[j]);
+        // Q += u B_o g
+        qr = fma(ubr[o], x, qr);
+        qr = fma(-ubi[o], yy, qr);
+        qi = fma(ubr[o], yy, qi);
+        qi = fma(ubi[o], x, qi);
+      }
+      // rho' conj(Q)
+      gr[j] = fma(rr, qr, gr[j]);
+      gr[j] = fma(ri, qi, gr[j]);
+      gi[j] = fma(ri, qr, gi[j]);
+      gi[j] = fma(-rr, qi, gi[j]);
+    }
+  }
+}
+
+// The evaluation pass over the helicity-halved FP32 copy (a.h32: [P][N][NH] float2; a.rho:
+// [chain][N] float2; a.kch[k]: the chain of wave k; coefficients c~ in a.cin). Lane layout:
+// lane = ev * KG + kg, the wave k = kg + KG j in slot j < PML; the NH stored slots of a wave
+// come in one 16-byte (NH = 2) or 8-byte (NH = 1) load. Software-pipelined as k_eval_pipe32:
+// two register stages, loads of the next event group issued before the current one is used,
+// unconditional loads at clamped addresses past the last event or wave. Per-block partials
+// slot-major as the other passes (k_reduce). Requires N >= 1.
+template <int NH, int KG, int PML>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_eval_half32(EvalArgs a) {
+  typedef HalfVec<NH> V;
+  typedef typename V::T HV;
+  constexpr int EPW = 64 / KG;
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  const int kg = lane & (KG - 1);
+  const int el = lane / KG;
+  const bool leader = kg == 0;
+  double vs = 0.0, vc = 0.0;
+  double gr[PML], gi[PML], cr[PML], ci[PML];
+  const HV* base[PML];
+  const float2* rbase[PML];
+#pragma unroll
+  for (int j = 0; j < PML; ++j) {
+    const int k = kg + KG * j;
+    gr[j] = gi[j] = 0.0;
+    const double2 c = k < a.P ? eval_coef(a, k) : make_double2(0.0, 0.0);
+    cr[j] = c.x;
+    ci[j] = c.y;
+    const int kc = k < a.P ? k : a.P - 1;
+    base[j] = reinterpret_cast<const HV*>(a.h32 + (int64_t)kc * a.N * NH);
+    rbase[j] = a.rho + (int64_t)a.kch[kc] * a.N;
+  }
+  const int64_t last = a.N - 1;
+  auto load = [&](HV (&hv)[PML], float2 (&rv)[PML], double& wv, int64_t e) {
+    const int64_t ec = e < a.N ? e : last;
+#pragma unroll
+    for (int j = 0; j < PML; ++j) {
+      hv[j] = V::load(base[j] + ec);
+      rv[j] = rbase[j][ec];
+    }
+    wv = a.w[ec];
+  };
+  const int64_t stride = (int64_t)gridDim.x * 4 * EPW;
+  HV h0[PML], h1[PML];
+  float2 r0[PML], r1[PML];
+  double w0, w1;
+  const int wave_u = __builtin_amdgcn_readfirstlane(wave);
+  const int64_t first = ((int64_t)blockIdx.x * 4 + wave_u) * EPW;
+  int64_t e = first + el;
+  load(h0, r0, w0, e);
+  for (int64_t b = first; b < a.N; b += 2 * stride, e += 2 * stride) {
+    load(h1, r1, w1, e + stride);
+    __builtin_amdgcn_sched_barrier(0);
+    eval_half_group<NH, KG, PML>(a.mode, a.grad, a.f_out, h0, r0, w0, e, e < a.N, leader, cr,
+                                 ci, vs, vc, gr, gi);
+    if (b + stride >= a.N) break;
+    load(h0, r0, w0, e + 2 * stride);
+    __builtin_amdgcn_sched_barrier(0);
+    eval_half_group<NH, KG, PML>(a.mode, a.grad, a.f_out, h1, r1, w1, e + stride,
+                                 e + stride < a.N, leader, cr, ci, vs, vc, gr, gi);
+  }
+#pragma unroll
+  for (int m = 32; m >= 1; m >>= 1) {
+    const double s2 = __shfl_xor(vs, m, 64);
+    const double c2 = __shfl_xor(vc, m, 64);
+    neumaier_merge(vs, vc, s2, c2);
+  }
+  __shared__ double sh[4][TPA_NACC_MAX];
+  if (a.grad) {
+#pragma unroll
+    for (int j = 0; j < PML; ++j) {
+      const double xr = butterfly<KG, 64>(gr[j]), xi = butterfly<KG, 64>(gi[j]);
+      const int k = kg + KG * j;
+      if (el == 0 && k < a.P) {
+        sh[wave][2 + 2 * k] = xr;
+        sh[wave][3 + 2 * k] = xi;
+      }
+    }
+  }
+  if (lane == 0) {
+    sh[wave][0] = vs;
+    sh[wave][1] = vc;
+  }
+  __syncthreads();
+  double* out = a.partials + blockIdx.x;
+  if (threadIdx.x == 0) {
+    double s = sh[0][0], c = sh[0][1];
+    for (int wv = 1; wv < 4; ++wv) neumaier_merge(s, c, sh[wv][0], sh[wv][1]);
+    out[0] = s;
+    out[gridDim.x] = c;
+  }
+  if (a.grad) {
+    for (int j = 2 + threadIdx.x; j < a.nacc; j += blockDim.x)
+      out[(int64_t)j * gridDim.x] = ((sh[0][j] + sh[1][j]) + sh[2][j]) + sh[3][j];
+  }
+}

@@ -301,6 +301,10 @@ struct EvalArgs {
   int grad;
   int nacc;  // 2 + 2P
   double2 cin[TPA_EVAL_CIN];  // coefficients by value (k_eval_pipe / k_eval_pipe32, coef null)
+  // the helicity-halved pass (k_eval_half32): chain phases [chain][N] and the chain of each
+  // wave (h32 then holds the stored half, [P][N][NH])
+  const float2* __restrict__ rho;
+  signed char kch[TPA_EVAL_CIN];
 };
 
 // Coefficient k of an evaluation pass (from the kernel arguments when a.coef is null).

@@ -28,6 +28,7 @@ FLAG_GRAD = 1
 FLAG_FOUT = 2
 FLAG_SHAPE = 4
 FLAG_F32 = 8
+FLAG_F32H = 16
 FUSED_MAX_P = 8  # largest model (partial waves) tpa_eval_fused accepts
 CPL_SLOTS = 4  # factor slots per partial wave in TpaCouplingMap (total + up to 3 vertices)
 
@@ -201,6 +202,11 @@ def lib():
         "tpa_events_get_angles": ([vp, dp, i64], ctypes.c_int),
         "tpa_events_build": ([vp, ctypes.c_int], ctypes.c_int),
         "tpa_events_keep_f32": ([vp, ctypes.c_int], ctypes.c_int),
+        "tpa_events_keep_half": ([vp, ctypes.c_int], ctypes.c_int),
+        "tpa_events_half_residual": ([vp, dp], ctypes.c_int),
+        "tpa_events_get_half": ([vp, ctypes.POINTER(ctypes.c_float), i64,
+                                 ctypes.POINTER(ctypes.c_float), i64,
+                                 ctypes.POINTER(ctypes.c_int), ctypes.c_int], ctypes.c_int),
         "tpa_pool_keep_f32": ([vp, ctypes.c_int], ctypes.c_int),
         "tpa_events_get_pw": ([vp, dp, i64], ctypes.c_int),
         "tpa_eval": ([vp, dp, ctypes.c_int, ctypes.c_int, dp, dp], ctypes.c_int),
@@ -239,6 +245,7 @@ def lib():
         "tpa_fcn_set_shape_map": ([vp, vp, ctypes.POINTER(TpaShapeMap)], ctypes.c_int),
         "tpa_fcn_cache_rebuilds": ([vp, ctypes.POINTER(ctypes.c_int64)], ctypes.c_int),
         "tpa_fcn_set_data_storage": ([vp, ctypes.c_int], ctypes.c_int),
+        "tpa_fcn_set_data_half": ([vp, ctypes.c_int], ctypes.c_int),
         "tpa_fcn_minimize": ([vp, dp, ctypes.c_double, ctypes.c_int, dp, dp,
                               ctypes.POINTER(ctypes.c_int), ctypes.POINTER(i64),
                               ctypes.POINTER(i64)], ctypes.c_int),
@@ -298,4 +305,6 @@ def exported_symbols():
         "tpa_fcn_cache_rebuilds", "tpa_events_keep_f32", "tpa_fcn_set_data_storage",
         "tpa_pool_keep_f32", "tpa_eval_noalloc", "tpa_hess_shape", "tpa_eval_fused",
         "tpa_fused_source", "tpa_fused_compile_seconds", "tpa_pool_add_events",
+        "tpa_events_keep_half", "tpa_events_half_residual", "tpa_fcn_set_data_half",
+        "tpa_events_get_half",
     ]

@@ -187,20 +187,64 @@ class EventSet:
         _lib.check(_lib.lib().tpa_events_keep_f32(self._h, 1 if on else 0))
         self._keep32 = bool(on)
 
-    def eval(self, coef, mode, grad=False, fout=False, f32=False):
+    def keep_half(self, on=True):
+        """Keep the helicity-halved FP32 copy of the (folded) cache (tpa_events_keep_half):
+        one helicity slot of each mirror pair in FP32 and one lineshape phase per chain and
+        event, refreshed by every rebuild, for passes with ``half=True``. Making it checks the
+        helicity-flip relation on every event (the residual is ``half_residual()``)."""
+        self.ensure_built()
+        _lib.check(_lib.lib().tpa_events_keep_half(self._h, 1 if on else 0))
+        self._keep32h = bool(on)
+
+    def half_copy(self):
+        """The helicity-halved copy as the host sees it: (g [P, n, nh] complex64, rho
+        [n_chain, n] complex64, slots, mirrors, signs, phi_rot [P], chain of each wave [P]),
+        with h_k,o = phi_k g_k,o and h_k,-o = phi_k s_o rho conj(g_k,o), phi = -i where
+        phi_rot is 1."""
+        import ctypes
+        P = self.amp.P
+        nc = self.amp.n_chain if hasattr(self.amp, "n_chain") else None
+        info = np.zeros(13 + 2 * P, dtype=np.int32)
+        n_chain_max = 64
+        g = np.zeros(2 * P * self.n * 4, dtype=np.float32)
+        rho = np.zeros(2 * n_chain_max * self.n, dtype=np.float32)
+        fp = ctypes.POINTER(ctypes.c_float)
+        _lib.check(_lib.lib().tpa_events_get_half(
+            self._h, g.ctypes.data_as(fp), g.size, rho.ctypes.data_as(fp), rho.size,
+            info.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), info.size))
+        nh = int(info[0])
+        kch = info[13 + P:13 + 2 * P].copy()
+        n_chain = int(kch.max()) + 1 if nc is None else nc
+        g = g[:2 * P * self.n * nh].view(np.complex64).reshape(P, self.n, nh)
+        rho = rho[:2 * n_chain * self.n].view(np.complex64).reshape(n_chain, self.n)
+        return (g, rho, info[1:1 + nh].copy(), info[5:5 + nh].copy(), info[9:9 + nh].copy(),
+                info[13:13 + P].copy(), kch)
+
+    def half_residual(self):
+        """Largest relative residual of the helicity-flip relation measured by the last
+        halved copy (-1 if none was made)."""
+        r = np.zeros(1)
+        _lib.check(_lib.lib().tpa_events_half_residual(self._h, _lib.dptr(r)))
+        return float(r[0])
+
+    def eval(self, coef, mode, grad=False, fout=False, f32=False, half=False):
         """One pass: (value, G or None, f or None). When the model floats lineshape
         parameters, G is the tuple (G, Dm, Dg) of coupling and lineshape adjoints. ``f32``:
-        read the FP32 copy of the cache (keep_f32; values widened to FP64 on load)."""
+        read the FP32 copy of the cache (keep_f32; values widened to FP64 on load); ``half``:
+        read the helicity-halved FP32 copy (keep_half)."""
         self.ensure_built()
         P = self.amp.P
         shape = not self._state[0]
         if f32 and not getattr(self, "_keep32", False):
             self.keep_f32(True)
+        if half and not getattr(self, "_keep32h", False):
+            self.keep_half(True)
         c = np.ascontiguousarray(np.asarray(coef, dtype=np.complex128)).view(np.float64)
         out = np.zeros(2 + (6 if shape else 2) * P, dtype=np.float64)
         f = np.empty(self.n, dtype=np.float64) if fout else None
         flags = ((_lib.FLAG_GRAD if grad else 0) | (_lib.FLAG_FOUT if fout else 0)
-                 | (_lib.FLAG_SHAPE if shape else 0) | (_lib.FLAG_F32 if f32 else 0))
+                 | (_lib.FLAG_SHAPE if shape else 0) | (_lib.FLAG_F32 if f32 else 0)
+                 | (_lib.FLAG_F32H if half else 0))
         _lib.check(_lib.lib().tpa_eval(self._h, _lib.dptr(c), mode, flags, _lib.dptr(out),
                                        _lib.dptr(f)))
         G = None

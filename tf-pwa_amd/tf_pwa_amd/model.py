@@ -73,9 +73,12 @@ class Model:
         default: the Gram is built once per lineshape state, so its cost is off the per-call
         path) or "fp32" (FP32 MFMA with FP64 refolds; its partial sums drift by up to ~1e-8
         relative for narrow peaks over a broad MC sample).
-        data_storage: the data partial-wave cache the mixed path reads, "fp64" or "fp32"
+        data_storage: the data partial-wave cache the mixed path reads, "fp64", "fp32"
         (an FP32 copy kept beside the FP64 cache, values widened to FP64 on load, every sum
-        in FP64: half the bytes per data event; with fixed lineshapes, else FP64).
+        in FP64: half the bytes per data event) or "fp32h" (the helicity-halved FP32 copy:
+        one helicity slot of each mirror pair plus one lineshape phase per chain and event,
+        from which the mirrored slots follow; 408 instead of 712 B per event at six
+        resonances); the FP32 options with fixed lineshapes, else FP64.
         evaluation: how a pass gets its partial waves, "stream" (read the cache built once,
         k_eval_pipe / k_eval_pipe32) or "fused" (recompute every wave from the angle cache per
         event in one generated kernel, tpa_eval_fused: no partial-wave cache; three-body,
@@ -84,8 +87,8 @@ class Model:
             raise ValueError("precision must be 'fp64' or 'mixed'")
         if gram_precision not in ("fp32", "fp64"):
             raise ValueError("gram_precision must be 'fp32' or 'fp64'")
-        if data_storage not in ("fp32", "fp64"):
-            raise ValueError("data_storage must be 'fp32' or 'fp64'")
+        if data_storage not in ("fp32", "fp32h", "fp64"):
+            raise ValueError("data_storage must be 'fp32', 'fp32h' or 'fp64'")
         self.Amp = amp
         self.evaluation = evaluation
         self.precision = precision
@@ -168,7 +171,8 @@ class Model:
                 v, g = s.eval_fused(coef, mode, grad=grad)
                 g = (g,)
             else:
-                v, g, _ = s.eval(coef, mode, grad=grad, f32=self._data_f32(mode))
+                v, g, _ = s.eval(coef, mode, grad=grad, f32=self._data_f32(mode),
+                                 half=self._data_half(mode))
                 if grad and not shape:
                     g = (g,)
             value += v
@@ -181,6 +185,11 @@ class Model:
         """Whether a pass of this mode reads the FP32 data caches."""
         return (mode == _lib.MODE_DATA and self.precision == "mixed"
                 and self.data_storage == "fp32" and not self.Amp.shape_mode)
+
+    def _data_half(self, mode):
+        """Whether a pass of this mode reads the helicity-halved FP32 data caches."""
+        return (mode == _lib.MODE_DATA and self.precision == "mixed"
+                and self.data_storage == "fp32h" and not self.Amp.shape_mode)
 
     @staticmethod
     def _split(sets, w):
@@ -395,6 +404,10 @@ class _NativeSession:
             _lib.check(_lib.lib().tpa_fcn_set_data_storage(h, 1))
             for s in data_sets:
                 s._keep32 = True
+        elif model._data_half(_lib.MODE_DATA):
+            _lib.check(_lib.lib().tpa_fcn_set_data_half(h, 1))
+            for s in data_sets:
+                s._keep32h = True
         self.n = len(self._keep["bound_kind"])
         # argument buffers with their pointers taken once (a ctypes pointer per call costs
         # ~3 us of the host's time per evaluation)
