@@ -616,6 +616,7 @@ def main():
     # helicity-halved FP32 copy: half the slots of every wave, one phase per chain, the weight
     n_chain = len(amp.chains) if hasattr(amp, "chains") else None
     bytes_per_event_f32h = amp.P * (amp.lam // 2) * 8 + 8 * n_chain + 8
+    bytes_per_event_f64h = amp.P * (amp.lam // 2) * 16 + 16 * n_chain + 8
 
     traffic = measured_traffic() if (args.n_data, args.n_mc) == (1_000_000, 10_000_000) else None
 
@@ -646,13 +647,15 @@ def main():
         return (16 * amp.P * lam + 2 * lam * (kg - 1) + (3 * lam - 1) + 6 + 2 * lam,
                 {"log": 1, "reciprocal": 1} if mode_data else {})
 
-    def roofline(r, n_events_per_step, f32=False, n_data_per_step=None, half=False):
+    def roofline(r, n_events_per_step, f32=False, n_data_per_step=None, half=False,
+                 half64=False):
         """Roofline of a timed path's pass kernel. achieved: the 176 B/event angle cache (the
         path's input per event: 3 topologies x 5 f64, 2 x 3 aligned angles, the weight) of
         the events one launch covers, over the kernel's average launch time; the bytes the
         kernel itself streams (stored partial waves) are kept beside it, as are its counted
         FP64 operations against the FP64 vector peak."""
-        bpe = bytes_per_event_f32h if half else bytes_per_event_f32 if f32 else bytes_per_event
+        bpe = (bytes_per_event_f64h if half64 else bytes_per_event_f32h if half
+               else bytes_per_event_f32 if f32 else bytes_per_event)
         kernel_s = r["kernel_ms"] / 1e3
         launches = max(1, r["launches"])
         per_launch_s = kernel_s / launches
@@ -670,13 +673,14 @@ def main():
                 tr = traffic["data_1M"] if n_events_per_step == args.n_data else 0.5 * (
                     traffic["data_1M"] + traffic["mc_10M"])
         nd = n_events_per_step if n_data_per_step is None else n_data_per_step
-        fd, special = flops_per_event_half() if half else flops_per_event(True)
-        fm, _ = flops_per_event(False)
+        fd, special = flops_per_event_half() if (half or half64) else flops_per_event(True)
+        fm, _ = flops_per_event_half() if half64 else flops_per_event(False)
         flops_step = nd * fd + (n_events_per_step - nd) * fm
         tflops = flops_step * args.steps / kernel_s / 1e12 if kernel_s > 0 else 0.0
         return {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": achieved / HBM_PEAK_GBS, "traffic": tr,
-                "kernel": "k_eval_half32" if half else "k_eval_pipe32" if f32 else "k_eval_pipe",
+                "kernel": ("k_eval_half<double>" if half64 else "k_eval_half<float>" if half
+                           else "k_eval_pipe32" if f32 else "k_eval_pipe"),
                 "basis": "angle cache, {} B/event".format(ANGLE_BYTES_PER_EVENT),
                 "events_per_launch": events_per_launch,
                 "launches": r["launches"], "kernel_ms_avg": r["kernel_ms"] / launches,
@@ -692,6 +696,20 @@ def main():
     log("fp64 path ...")
     r64 = time_path(FCN(conf.get_model(), data, mc))
     log("fp64: {:.1f} evals/s".format(r64["evals"]))
+    # the FP64 path over the helicity-halved FP64 copies of the data and MC caches (the FP64
+    # cache's values to rounding; 808 instead of 1416 B per event)
+    m64h = conf.get_model()
+    m64h.data_storage = "fp64h"
+    m64h.mc_storage = "fp64h"
+    f64h = FCN(m64h, data, mc)
+    r64h = time_path(f64h)
+    dnll_64h = abs(r64h["nll"] - r64["nll"])
+    grel_64h = float(np.linalg.norm(r64h["g"] - r64["g"]) / np.linalg.norm(r64["g"]))
+    log("fp64, halved FP64 copies: {:.1f} evals/s, |dNLL| {:.2e}, grad rel {:.2e}".format(
+        r64h["evals"], dnll_64h, grel_64h))
+    del f64h
+    for s_ in (data, mc):
+        s_.keep_half(False)  # release the FP64 halved copies before the next paths
     # the FP32-MFMA Gram variant: its accuracy at the start point (the per-call cost is the
     # same for both Gram precisions, the Gram being built once)
     m32 = conf.get_model()
@@ -779,6 +797,12 @@ def main():
             "fp64": {"evals_per_s": r64["evals"], "ms_per_step": r64["ms_per_step"],
                      "nll": r64["nll"],
                      "roofline": roofline(r64, args.n_data + args.n_mc, n_data_per_step=args.n_data)},
+            "fp64_halved": {"evals_per_s": r64h["evals"], "ms_per_step": r64h["ms_per_step"],
+                            "nll": r64h["nll"], "abs_dnll_vs_fp64": dnll_64h,
+                            "grad_rel_err_vs_fp64": grel_64h,
+                            "data_storage": "fp64h", "mc_storage": "fp64h",
+                            "roofline": roofline(r64h, args.n_data + args.n_mc,
+                                                 n_data_per_step=args.n_data, half64=True)},
             "mixed": {"evals_per_s": rmix["evals"], "ms_per_step": rmix["ms_per_step"],
                       "nll": rmix["nll"], "abs_dnll_vs_fp64": dnll, "grad_rel_err_vs_fp64": grel,
                       "speedup_vs_fp64": rmix["evals"] / r64["evals"], "gram": "fp64",

@@ -53,6 +53,9 @@ typedef struct tpa_fcn_s* tpa_fcn_t;
  * which the mirrored slots follow (h_k,-o = eta_k s_o rho conj(h_k,o)); every operation after
  * the load is FP64. At most 32 partial waves; not with TPA_FLAG_SHAPE or TPA_FLAG_F32. */
 #define TPA_FLAG_F32H 16
+/* The same from the helicity-halved FP64 copy (tpa_events_keep_half with bit 1): the FP64
+ * cache's values to rounding, 808 instead of 1416 B per event for the six-resonance model. */
+#define TPA_FLAG_F64H 32
 /* Largest model (partial waves) tpa_eval_fused accepts. */
 #define TPA_FUSED_MAX_P 8
 
@@ -127,18 +130,19 @@ int tpa_events_build(tpa_events_t ev, int fold);
  * sum_gradient, tf_pwa/model/model.py:41-94, in FP64 from the angle cache). */
 int tpa_events_keep_f32(tpa_events_t ev, int on);
 
-/* on != 0: keep the helicity-halved FP32 copy of the folded cache for tpa_eval with
- * TPA_FLAG_F32H, made now if the cache is built and refreshed by every later folded build;
- * on = 0 frees it. Flipping every helicity of a partial wave conjugates its angular part, so
- * for each mirror pair of outer helicity slots (o, -o) only o is stored (FP32), with the phase
- * B/conj(B) of each chain's lineshape product per event (FP32): 408 instead of 712 B per event
- * for the six-resonance model. Making the copy checks on every event that the stored half
+/* kinds bit 0: keep the helicity-halved FP32 copy of the folded cache for tpa_eval with
+ * TPA_FLAG_F32H; bit 1: the FP64 copy for TPA_FLAG_F64H; each made now if the cache is built
+ * and refreshed by every later folded build; a copy whose bit is clear is freed. Flipping every
+ * helicity of a partial wave conjugates its angular part, so for each mirror pair of outer
+ * helicity slots (o, -o) only o is stored, with the phase B/conj(B) of each chain's lineshape
+ * product per event: 408 instead of 712 B per event (FP32), 808 instead of 1416 (FP64), for
+ * the six-resonance model. Making a copy checks on every event that the stored half
  * reproduces the mirrored slots of the FP64 cache; a relative residual above 1e-9 (a model
  * whose helicity lists are not symmetric) refuses the copy with TPA_ERR_UNSUPPORTED.
  * No reference counterpart: a storage option of the data pass of the mixed-precision
  * likelihood (the reference evaluates sum_gradient, tf_pwa/model/model.py:41-94, in FP64 from
  * the angle cache; the helicity amplitudes follow tf_pwa/amp/core.py:931-1164). */
-int tpa_events_keep_half(tpa_events_t ev, int on);
+int tpa_events_keep_half(tpa_events_t ev, int kinds);
 
 /* The largest relative residual the last helicity-halved copy of ev measured
  * (max |h_k,-o - eta_k s_o rho conj(h_k,o)| / max |h| per event, in FP64 before rounding);
@@ -461,11 +465,17 @@ int tpa_fcn_set_shape_map(tpa_fcn_t fcn, tpa_model_t model, const TpaShapeMap* m
  * the storage option of the data pass behind FCN.nll_grad (tf_pwa/model/model.py:1207-1229). */
 int tpa_fcn_set_data_storage(tpa_fcn_t fcn, int data_f32);
 
-/* The session's data passes read the helicity-halved FP32 copies of the data caches
- * (TPA_FLAG_F32H, tpa_events_keep_half) when on != 0, the FP64 caches when on = 0. Fixed
- * lineshapes only. No reference counterpart: a storage option of the data pass behind
- * FCN.nll_grad (tf_pwa/model/model.py:1207-1229). */
-int tpa_fcn_set_data_half(tpa_fcn_t fcn, int on);
+/* The session's data passes read the helicity-halved copies of the data caches
+ * (tpa_events_keep_half): FP32 (TPA_FLAG_F32H) when kind = 1, FP64 (TPA_FLAG_F64H) when
+ * kind = 2, the FP64 caches when kind = 0. Fixed lineshapes only. No reference counterpart: a
+ * storage option of the data pass behind FCN.nll_grad (tf_pwa/model/model.py:1207-1229). */
+int tpa_fcn_set_data_half(tpa_fcn_t fcn, int kind);
+
+/* on != 0: the session's MC passes (FP64 path) read the helicity-halved FP64 copies of the MC
+ * caches (TPA_FLAG_F64H); on = 0: the FP64 caches. Fixed lineshapes only. No reference
+ * counterpart: a storage option of the MC pass behind FCN.nll_grad
+ * (tf_pwa/model/model.py:1207-1229). */
+int tpa_fcn_set_mc_half(tpa_fcn_t fcn, int on);
 
 /* Cache rebuilds the session made after barrier clamp flags changed (no reference
  * counterpart: a bookkeeping counter for hosts that track the caches' state). */

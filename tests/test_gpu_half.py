@@ -155,3 +155,71 @@ def test_half_storage_mixed_path_close_to_fp64_and_oracle(which, request, gpu_av
         o_nll, o_g = oracle_nll_grad(case)
         assert abs(n2 - o_nll) < 1e-5
         assert np.linalg.norm(g2 - o_g) <= 1e-6 * np.linalg.norm(o_g)
+
+
+@pytest.mark.parametrize("which,n", [("case2", 1), ("case2", 257), ("case2", 20000),
+                                     ("case1", 1000), ("case4", 3001)])
+def test_half64_pass_matches_the_fp64_cache_pass(which, n, request, gpu_available):
+    """TPA_FLAG_F64H: the halved FP64 copy reproduces the FP64 cache's pass to rounding
+    (value 1e-12 relative, G 1e-11 of its largest entry, f 1e-12), both modes, after a rebuild."""
+    from tf_pwa_amd import _lib
+
+    case = request.getfixturevalue(which)
+    conf = case["conf"]
+    full = conf.data_from_p4(case["mc"])
+    ev = full.select(np.arange(n) % full.n)
+    w = np.random.default_rng(n + 5).uniform(-0.5, 1.5, n)
+    ev.set_weight(w)
+    ev.keep_half(False, fp64=True)
+    amp = conf.get_amplitude()
+    name = [v for v in conf.vm.variables if v.endswith("_mass")][0]
+    m0 = conf.vm.get(name)
+    try:
+        for shift in (0.0, 0.01):
+            conf.vm.set(name, m0 + shift)
+            c = amp.coefficients()
+            for mode in (_lib.MODE_DATA, _lib.MODE_MC):
+                v, G, f = ev.eval(c, mode, grad=True, fout=True, half64=True)
+                assert 0.0 <= ev.half_residual() <= 1e-12
+                v0, G0, f0 = ev.eval(c, mode, grad=True, fout=True)
+                scale = max(1.0, np.abs(w).sum())
+                assert abs(v - v0) <= 1e-12 * scale * max(1.0, abs(v0) / scale)
+                np.testing.assert_allclose(G, G0, rtol=0, atol=1e-11 * np.abs(G0).max())
+                np.testing.assert_allclose(f, f0, rtol=1e-12)
+    finally:
+        conf.vm.set(name, m0)
+
+
+@pytest.mark.parametrize("which", ["case1", "case2", "case4"])
+def test_half64_storage_fp64_and_mixed_paths(which, request, gpu_available):
+    """FP64 path with halved FP64 data and MC copies, and the mixed path with halved FP64
+    data: the FP64 cache's results to rounding (|dNLL| < 1e-8, gradient 1e-10 relative), the
+    oracle at the FP64 bar (1e-6, 1e-7)."""
+    from tf_pwa_amd import FCN
+
+    case = request.getfixturevalue(which)
+    conf = case["conf"]
+    data = conf.data_from_p4(case["data"], weight=case["wd"])
+    mc = conf.data_from_p4(case["mc"])
+    n1, g1 = FCN(conf.get_model(), data, mc).nll_grad()
+    m = conf.get_model()
+    m.data_storage = "fp64h"
+    m.mc_storage = "fp64h"
+    fs = FCN(m, data, mc)
+    n2, g2 = fs.nll_grad()
+    assert fs._native is not None
+    assert abs(n1 - n2) < 1e-8, abs(n1 - n2)
+    assert np.linalg.norm(g1 - g2) <= 1e-10 * np.linalg.norm(g1)
+    fs.native = False  # the Python host assembly reads the same halved copies
+    n3, g3 = fs.nll_grad()
+    assert abs(n3 - n2) <= 1e-12 * abs(n2)
+    assert np.abs(g3 - g2).max() <= 1e-10 * np.abs(g2).max()
+    mm = conf.get_model()
+    mm.data_storage = "fp64h"
+    n4, g4 = FCN(mm, data, mc, precision="mixed").nll_grad()
+    assert abs(n1 - n4) < 1e-8
+    assert np.linalg.norm(g1 - g4) <= 1e-10 * np.linalg.norm(g1)
+    if which != "case4":
+        o_nll, o_g = oracle_nll_grad(case)
+        assert abs(n2 - o_nll) < 1e-6
+        assert np.linalg.norm(g2 - o_g) <= 1e-7 * np.linalg.norm(o_g)

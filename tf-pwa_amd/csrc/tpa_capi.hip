@@ -251,9 +251,13 @@ struct tpa_events_s {
   // [n_chain][n], the phi_k of each wave (1: -i) and the chain of each wave
   float2* d_g32h = nullptr;
   float2* d_rho = nullptr;
+  double2* d_g64h = nullptr;  // the same layout in FP64 (TPA_FLAG_F64H)
+  double2* d_rho64 = nullptr;
   unsigned long long* d_resid = nullptr;
   bool keep32h = false;
   bool h32h_valid = false;
+  bool keep64h = false;
+  bool h64h_valid = false;
   int half_nh = 0;
   int half_slot[TPA_HALF_MAX_NH] = {}, half_mirror[TPA_HALF_MAX_NH] = {},
       half_sgn[TPA_HALF_MAX_NH] = {};
@@ -370,19 +374,50 @@ static int dispatch_kg(int kg, const EvalArgs& a, int grid, hipStream_t s) {
   }
 }
 
-// The helicity-halved pass (k_eval_half32): KG lanes per event, PML = ceil(P / KG) rounded up
-// to 4, 6 or 8 (P <= 32: the coefficients travel in the kernel arguments).
+// The helicity-halved FP32 pass (k_eval_half<.., float>): KG lanes per event, PML =
+// ceil(P / KG) rounded up to 4, 6 or 8 (P <= 32: the coefficients travel in the kernel
+// arguments).
 static int half_kgroups(int P) { return P <= 8 ? 1 : P <= 16 ? 2 : 4; }
 
 template <int NH, int KG>
 static void launch_half(const EvalArgs& a, int grid, hipStream_t s) {
   const int need = (a.P + KG - 1) / KG;
   if (need <= 4)
-    hipLaunchKernelGGL((k_eval_half32<NH, KG, 4>), dim3(grid), dim3(256), 0, s, a);
+    hipLaunchKernelGGL((k_eval_half<NH, KG, 4, float>), dim3(grid), dim3(256), 0, s, a);
   else if (need <= 6)
-    hipLaunchKernelGGL((k_eval_half32<NH, KG, 6>), dim3(grid), dim3(256), 0, s, a);
+    hipLaunchKernelGGL((k_eval_half<NH, KG, 6, float>), dim3(grid), dim3(256), 0, s, a);
   else
-    hipLaunchKernelGGL((k_eval_half32<NH, KG, 8>), dim3(grid), dim3(256), 0, s, a);
+    hipLaunchKernelGGL((k_eval_half<NH, KG, 8, float>), dim3(grid), dim3(256), 0, s, a);
+}
+
+// The helicity-halved FP64 pass (k_eval_half<.., double>): a lane's register stage holds twice
+// the bytes of the FP32 pass per wave, so the waves split over twice the k-groups (at most 4
+// per lane; 3 at P = 17 .. 24).
+static int half64_kgroups(int P) { return P <= 8 ? 2 : P <= 16 ? 4 : 8; }
+
+template <int NH>
+static void launch_half64(const EvalArgs& a, int grid, hipStream_t s) {
+  if (a.P <= 8)
+    hipLaunchKernelGGL((k_eval_half<NH, 2, 4, double>), dim3(grid), dim3(256), 0, s, a);
+  else if (a.P <= 16)
+    hipLaunchKernelGGL((k_eval_half<NH, 4, 4, double>), dim3(grid), dim3(256), 0, s, a);
+  else if (a.P <= 24)
+    hipLaunchKernelGGL((k_eval_half<NH, 8, 3, double>), dim3(grid), dim3(256), 0, s, a);
+  else
+    hipLaunchKernelGGL((k_eval_half<NH, 8, 4, double>), dim3(grid), dim3(256), 0, s, a);
+}
+
+static int dispatch_half64(int nh, const EvalArgs& a, int grid, hipStream_t s) {
+  if (a.P > 32) return TPA_ERR_UNSUPPORTED;
+  if (nh == 1) {
+    launch_half64<1>(a, grid, s);
+    return TPA_OK;
+  }
+  if (nh == 2) {
+    launch_half64<2>(a, grid, s);
+    return TPA_OK;
+  }
+  return TPA_ERR_UNSUPPORTED;
 }
 
 static int dispatch_half(int nh, int kg, const EvalArgs& a, int grid, hipStream_t s) {
@@ -889,6 +924,8 @@ int tpa_events_destroy(tpa_events_t ev) {
   if (ev->d_h32) (void)hipFree(ev->d_h32);
   if (ev->d_g32h) (void)hipFree(ev->d_g32h);
   if (ev->d_rho) (void)hipFree(ev->d_rho);
+  if (ev->d_g64h) (void)hipFree(ev->d_g64h);
+  if (ev->d_rho64) (void)hipFree(ev->d_rho64);
   if (ev->d_resid) (void)hipFree(ev->d_resid);
   if (ev->d_cnt) (void)hipFree(ev->d_cnt);
   if (ev->h_flag) (void)hipHostFree(ev->h_flag);
@@ -1127,7 +1164,7 @@ static int convert_f32(tpa_events_s* ev) {
 // measures the residual of the relation on every event, which the host checks.
 #define TPA_HALF_SAMPLE 512
 #define TPA_HALF_TOL 1e-9
-static int convert_half(tpa_events_s* ev) {
+static int convert_half(tpa_events_s* ev, bool fp64) {
   const tpa_model_s* m = ev->m;
   const TpaModel& M = m->h;
   const int L = M.lam;
@@ -1229,12 +1266,19 @@ static int convert_half(tpa_events_s* ev) {
     }
   }
   const size_t ng = (size_t)M.P * ev->n * nh, nr = (size_t)n_chain * ev->n;
-  if (ev->half_nh != nh && ev->d_g32h) {
-    HIPCHK(hipFree(ev->d_g32h));
+  if (ev->half_nh != nh) {
+    if (ev->d_g32h) HIPCHK(hipFree(ev->d_g32h));
+    if (ev->d_g64h) HIPCHK(hipFree(ev->d_g64h));
     ev->d_g32h = nullptr;
+    ev->d_g64h = nullptr;
   }
-  if (!ev->d_g32h) HIPCHK(hipMalloc(&ev->d_g32h, sizeof(float2) * ng));
-  if (!ev->d_rho) HIPCHK(hipMalloc(&ev->d_rho, sizeof(float2) * nr));
+  if (fp64) {
+    if (!ev->d_g64h) HIPCHK(hipMalloc(&ev->d_g64h, sizeof(double2) * ng));
+    if (!ev->d_rho64) HIPCHK(hipMalloc(&ev->d_rho64, sizeof(double2) * nr));
+  } else {
+    if (!ev->d_g32h) HIPCHK(hipMalloc(&ev->d_g32h, sizeof(float2) * ng));
+    if (!ev->d_rho) HIPCHK(hipMalloc(&ev->d_rho, sizeof(float2) * nr));
+  }
   if (!ev->d_resid) HIPCHK(hipMalloc(&ev->d_resid, sizeof(unsigned long long)));
   ev->half_nh = nh;
   for (int q = 0; q < nh; ++q) {
@@ -1243,8 +1287,8 @@ static int convert_half(tpa_events_s* ev) {
     ev->half_sgn[q] = a.sgn[q];
   }
   a.h = ev->d_h;
-  a.g = ev->d_g32h;
-  a.rho = ev->d_rho;
+  a.g = fp64 ? (void*)ev->d_g64h : (void*)ev->d_g32h;
+  a.rho = fp64 ? (void*)ev->d_rho64 : (void*)ev->d_rho;
   a.resid = ev->d_resid;
   a.n = ev->n;
   a.S = ev->n;
@@ -1260,20 +1304,24 @@ static int convert_half(tpa_events_s* ev) {
   unsigned long long zero = 0, bits = 0;
   HIPCHK(hipMemcpyAsync(ev->d_resid, &zero, sizeof(zero), hipMemcpyHostToDevice, m->stream));
   const int64_t grid = std::max<int64_t>(1, std::min<int64_t>((ev->n + 255) / 256, 4096));
-  hipLaunchKernelGGL(k_pw_to_half, dim3((unsigned)grid), dim3(256), 0, m->stream, a);
+  if (fp64)
+    hipLaunchKernelGGL(k_pw_to_half<double>, dim3((unsigned)grid), dim3(256), 0, m->stream, a);
+  else
+    hipLaunchKernelGGL(k_pw_to_half<float>, dim3((unsigned)grid), dim3(256), 0, m->stream, a);
   HIPCHK(hipGetLastError());
   HIPCHK(hipMemcpyAsync(&bits, ev->d_resid, sizeof(bits), hipMemcpyDeviceToHost, m->stream));
   HIPCHK(hipStreamSynchronize(m->stream));
   double r;
   std::memcpy(&r, &bits, sizeof(r));
   ev->half_resid = r;
+  bool& valid = fp64 ? ev->h64h_valid : ev->h32h_valid;
   if (!(r <= TPA_HALF_TOL)) {
-    ev->h32h_valid = false;
+    valid = false;
     return fail(TPA_ERR_UNSUPPORTED,
                 "the helicity-flip relation does not hold for this model's cache (relative "
                 "residual " + std::to_string(r) + ")");
   }
-  ev->h32h_valid = true;
+  valid = true;
   return TPA_OK;
 }
 
@@ -1287,13 +1335,17 @@ int tpa_events_build(tpa_events_t ev, int fold) {
   HIPCHK(hipGetLastError());
   ev->built_fold = fold ? 1 : 0;
   ev->h32_valid = false;
-  ev->h32h_valid = false;
+  ev->h32h_valid = ev->h64h_valid = false;
   if (ev->keep32 && fold) {
     const int rc = convert_f32(ev);
     if (rc != TPA_OK) return rc;
   }
   if (ev->keep32h && fold) {
-    const int rc = convert_half(ev);
+    const int rc = convert_half(ev, false);
+    if (rc != TPA_OK) return rc;
+  }
+  if (ev->keep64h && fold) {
+    const int rc = convert_half(ev, true);
     if (rc != TPA_OK) return rc;
   }
   HIPCHK(hipStreamSynchronize(ev->m->stream));
@@ -1318,23 +1370,35 @@ int tpa_events_keep_f32(tpa_events_t ev, int on) {
   return TPA_OK;
 }
 
-int tpa_events_keep_half(tpa_events_t ev, int on) {
+int tpa_events_keep_half(tpa_events_t ev, int kinds) {
   if (!ev) return fail(TPA_ERR_ARG, "null pointer");
-  if (!on) {
-    HIPCHK(hipStreamSynchronize(ev->m->stream));
+  if (kinds & ~3) return fail(TPA_ERR_ARG, "kinds: bit 0 the FP32 copy, bit 1 the FP64 copy");
+  HIPCHK(hipStreamSynchronize(ev->m->stream));
+  if (!(kinds & 1)) {
     if (ev->d_g32h) HIPCHK(hipFree(ev->d_g32h));
     if (ev->d_rho) HIPCHK(hipFree(ev->d_rho));
     ev->d_g32h = nullptr;
     ev->d_rho = nullptr;
     ev->keep32h = ev->h32h_valid = false;
-    return TPA_OK;
   }
-  ev->keep32h = true;
-  if (ev->d_h && ev->built_fold == 1 && !ev->h32h_valid) {
-    const int rc = convert_half(ev);
-    if (rc != TPA_OK) {
-      ev->keep32h = false;
-      return rc;
+  if (!(kinds & 2)) {
+    if (ev->d_g64h) HIPCHK(hipFree(ev->d_g64h));
+    if (ev->d_rho64) HIPCHK(hipFree(ev->d_rho64));
+    ev->d_g64h = nullptr;
+    ev->d_rho64 = nullptr;
+    ev->keep64h = ev->h64h_valid = false;
+  }
+  for (int fp64 = 0; fp64 < 2; ++fp64) {
+    if (!(kinds & (1 << fp64))) continue;
+    bool& keep = fp64 ? ev->keep64h : ev->keep32h;
+    const bool valid = fp64 ? ev->h64h_valid : ev->h32h_valid;
+    keep = true;
+    if (ev->d_h && ev->built_fold == 1 && !valid) {
+      const int rc = convert_half(ev, fp64 != 0);
+      if (rc != TPA_OK) {
+        keep = false;
+        return rc;
+      }
     }
   }
   return TPA_OK;
@@ -1422,17 +1486,20 @@ static int eval_impl(tpa_events_s* ev, const double* coef, int mode, int flags, 
   if (f32 && shape) return fail(TPA_ERR_UNSUPPORTED, "FP32 cache reads need a folded cache");
   if (f32 && !ev->h32_valid)
     return fail(TPA_ERR_ARG, "no FP32 copy of the cache (tpa_events_keep_f32)");
-  const bool half = (flags & TPA_FLAG_F32H) != 0;
-  if (half && (shape || f32))
-    return fail(TPA_ERR_UNSUPPORTED, "TPA_FLAG_F32H excludes TPA_FLAG_SHAPE and TPA_FLAG_F32");
-  if (half && !ev->h32h_valid)
+  const bool half32 = (flags & TPA_FLAG_F32H) != 0, half64 = (flags & TPA_FLAG_F64H) != 0;
+  const bool half = half32 || half64;
+  if (half && (shape || f32 || (half32 && half64)))
+    return fail(TPA_ERR_UNSUPPORTED,
+                "TPA_FLAG_F32H / TPA_FLAG_F64H exclude TPA_FLAG_SHAPE, TPA_FLAG_F32 and each other");
+  if ((half32 && !ev->h32h_valid) || (half64 && !ev->h64h_valid))
     return fail(TPA_ERR_ARG, "no helicity-halved copy of the cache (tpa_events_keep_half)");
   if (half && !args_coef)
-    return fail(TPA_ERR_UNSUPPORTED, "TPA_FLAG_F32H takes at most 32 coefficients");
+    return fail(TPA_ERR_UNSUPPORTED, "the helicity-halved passes take at most 32 coefficients");
   EvalArgs a;
-  a.h = ev->d_h;
-  a.h32 = f32 ? ev->d_h32 : half ? ev->d_g32h : nullptr;
-  a.rho = half ? ev->d_rho : nullptr;
+  a.h = half64 ? ev->d_g64h : ev->d_h;
+  a.h32 = f32 ? ev->d_h32 : half32 ? ev->d_g32h : nullptr;
+  a.rho = half32 ? ev->d_rho : nullptr;
+  a.rho64 = half64 ? ev->d_rho64 : nullptr;
   a.w = ev->d_w;
   a.coef = args_coef ? nullptr : ev->d_coef;
   if (args_coef) std::memcpy(a.cin, coef, sizeof(double2) * M.P);
@@ -1458,7 +1525,7 @@ static int eval_impl(tpa_events_s* ev, const double* coef, int mode, int flags, 
   a.nacc = shape ? 2 + 6 * M.P : 2 + 2 * M.P;
   const int kg = eval_kgroups(M.lam_pad, M.P);
   if (kg < 0) return fail(TPA_ERR_UNSUPPORTED, "too many partial waves for this helicity count");
-  const int kgh = half_kgroups(M.P);
+  const int kgh = half64 ? half64_kgroups(M.P) : half_kgroups(M.P);
   int grid = half ? eval_grid(ev->n, kgh)
                   : eval_grid(ev->n, f32 ? f32_lanes_per_event(M.lam_pad, kg) : M.lam_pad * kg);
   bool timing = false;
@@ -1490,8 +1557,9 @@ static int eval_impl(tpa_events_s* ev, const double* coef, int mode, int flags, 
     grid = eval_grid(ev->n, M.lam_pad * kgs);
     rc = dispatch_shape(M.lam_pad, kgs, sa, grid, s);
   } else {
-    rc = half ? dispatch_half(ev->half_nh, kgh, a, grid, s)
-              : dispatch_eval(M.lam_pad, kg, a, grid, s);
+    rc = half64 ? dispatch_half64(ev->half_nh, a, grid, s)
+         : half ? dispatch_half(ev->half_nh, kgh, a, grid, s)
+                : dispatch_eval(M.lam_pad, kg, a, grid, s);
   }
   if (rc != TPA_OK) return fail(rc, "no evaluation kernel for this helicity/partial-wave count");
   HIPCHK(hipGetLastError());

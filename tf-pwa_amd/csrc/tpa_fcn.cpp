@@ -28,7 +28,8 @@ struct tpa_fcn_s : CouplingMap {
   tpa_model_t model = nullptr;
   TpaShapeMap sm{};
   int64_t rebuilds = 0;  // cache rebuilds after barrier clamp flags changed
-  int data_flags = 0;    // extra tpa_eval flags of the data passes (TPA_FLAG_F32)
+  int data_flags = 0;    // extra tpa_eval flags of the data passes (TPA_FLAG_F32, ..F32H, ..F64H)
+  int mc_flags = 0;      // extra tpa_eval flags of the MC passes (TPA_FLAG_F64H)
   // shape passes: the MC sets run on their own stream, overlapping the data sets' passes
   hipStream_t mc_stream = nullptr;
   hipEvent_t shapes_set = nullptr;  // the lineshape parameters are on the device
@@ -288,15 +289,28 @@ int tpa_fcn_set_data_storage(tpa_fcn_t F, int data_f32) {
   return TPA_OK;
 }
 
-int tpa_fcn_set_data_half(tpa_fcn_t F, int on) {
+int tpa_fcn_set_data_half(tpa_fcn_t F, int kind) {
   if (!F) return tpa_set_error(TPA_ERR_ARG, "null pointer");
-  if (on && F->shape)
+  if (kind < 0 || kind > 2) return tpa_set_error(TPA_ERR_ARG, "kind: 0, 1 (FP32) or 2 (FP64)");
+  if (kind && F->shape)
     return tpa_set_error(TPA_ERR_UNSUPPORTED, "halved data caches need fixed lineshapes");
   for (tpa_events_t ev : F->data) {
-    const int rc = tpa_events_keep_half(ev, on ? 1 : 0);
+    const int rc = tpa_events_keep_half(ev, kind);
     if (rc != TPA_OK) return rc;
   }
-  F->data_flags = on ? TPA_FLAG_F32H : 0;
+  F->data_flags = kind == 1 ? TPA_FLAG_F32H : kind == 2 ? TPA_FLAG_F64H : 0;
+  return TPA_OK;
+}
+
+int tpa_fcn_set_mc_half(tpa_fcn_t F, int on) {
+  if (!F) return tpa_set_error(TPA_ERR_ARG, "null pointer");
+  if (on && F->shape)
+    return tpa_set_error(TPA_ERR_UNSUPPORTED, "halved MC caches need fixed lineshapes");
+  for (tpa_events_t ev : F->mc) {
+    const int rc = tpa_events_keep_half(ev, on ? 2 : 0);
+    if (rc != TPA_OK) return rc;
+  }
+  F->mc_flags = on ? TPA_FLAG_F64H : 0;
   return TPA_OK;
 }
 
@@ -332,7 +346,8 @@ int tpa_fcn_nll_grad(tpa_fcn_t F, const double* y, double* nll, double* grad) {
   } else {
     Sk.assign(P, 0.0);
     for (tpa_events_t ev : F->mc) {
-      const int rc = tpa_eval(ev, F->coef.data(), TPA_MODE_MC, flags, F->out.data(), nullptr);
+      const int rc = tpa_eval(ev, F->coef.data(), TPA_MODE_MC, flags | F->mc_flags,
+                              F->out.data(), nullptr);
       if (rc != TPA_OK) return rc;
       smc += F->out[0];
       if (grad)

@@ -1,4 +1,5 @@
-// Helicity-halved FP32 copy of a folded partial-wave cache, and the evaluation pass over it.
+// Helicity-halved copies (FP32 or FP64) of a folded partial-wave cache, and the evaluation
+// pass over them.
 //
 // Flipping the sign of every helicity in a partial wave conjugates its angular part: the
 // Wigner functions obey D^j_{-m,-n} = (-1)^(m-n) conj(D^j_{m,n}) (d^j is real), the LS
@@ -23,7 +24,8 @@
 // are not symmetric, say).
 //
 // Bytes per event at six resonances (P = 22, Lambda = 4): 22 x 2 x 8 B of waves + 6 x 8 B of
-// phases + the weight = 408 B, against 712 B for the full FP32 copy.
+// phases + the weight = 408 B, against 712 B for the full FP32 copy. The same layout in FP64
+// (S = double: 808 B against 1416 B for the FP64 cache) reproduces the FP64 cache to rounding.
 #pragma once
 #include "tpa_kernels.h"
 
@@ -31,8 +33,8 @@
 
 struct HalfCopyArgs {
   const double2* __restrict__ h;        // [P][S][lam_pad] folded FP64 cache
-  float2* __restrict__ g;               // [P][n][nh] stored half, times conj(phi_k)
-  float2* __restrict__ rho;             // [n_chain][n] chain phases rho'
+  void* __restrict__ g;                 // [P][n][nh] stored half, times conj(phi_k) (S pairs)
+  void* __restrict__ rho;               // [n_chain][n] chain phases rho' (S pairs)
   unsigned long long* __restrict__ resid;  // largest relative residual (bits of a double >= 0)
   int64_t n, S;
   int P, lam_pad, nh, n_chain;
@@ -46,10 +48,29 @@ TPA_DEV cd half_rot(double2 v, int r) {
   return r ? c_make(-v.y, v.x) : c_make(v.x, v.y);
 }
 
+// A complex value stored as a pair of S (float: rounded to nearest; double: exact).
+template <typename S>
+struct HalfPair;
+template <>
+struct HalfPair<float> {
+  typedef float2 T;
+  TPA_DEV static T make(const cd& v) { return make_float2((float)v.r, (float)v.i); }
+};
+template <>
+struct HalfPair<double> {
+  typedef double2 T;
+  TPA_DEV static T make(const cd& v) { return make_double2(v.r, v.i); }
+};
+
 // One thread per event: every chain's reference phase from its largest stored wave, the
-// FP32 half and phases, and the event's residual max |g_-o - s_o rho' conj(g_o)| relative to
-// the event's largest wave; one atomic max per wave of threads.
+// stored half and phases in S, and the event's residual max |g_-o - s_o rho' conj(g_o)|
+// relative to the event's largest wave (FP64, before rounding to S); one atomic max per wave
+// of threads.
+template <typename S>
 __global__ __launch_bounds__(256) void k_pw_to_half(HalfCopyArgs a) {
+  typedef HalfPair<S> HP;
+  typename HP::T* const gout = static_cast<typename HP::T*>(a.g);
+  typename HP::T* const rout = static_cast<typename HP::T*>(a.rho);
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
   double worst = 0.0;
   for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < a.n; e += stride) {
@@ -75,14 +96,14 @@ __global__ __launch_bounds__(256) void k_pw_to_half(HalfCopyArgs a) {
       // rho' = g_-o / (s_o conj(g_o)) = g_-o g_o / (s_o |g_o|^2); any phase where the chain
       // vanishes (it multiplies zero)
       const cd rho = best > 0.0 ? c_scale(c_mul(gm, gb), (double)sb / best) : c_make(1.0, 0.0);
-      a.rho[(int64_t)c * a.n + e] = make_float2((float)rho.r, (float)rho.i);
+      rout[(int64_t)c * a.n + e] = HP::make(rho);
       for (int k = k0; k < k1; ++k) {
         const double2* hk = a.h + ((int64_t)k * a.S + e) * a.lam_pad;
-        float2* gk = a.g + ((int64_t)k * a.n + e) * a.nh;
+        typename HP::T* gk = gout + ((int64_t)k * a.n + e) * a.nh;
         for (int o = 0; o < a.nh; ++o) {
           const cd x = half_rot(hk[a.slot[o]], a.rot[k]);
           const cd xm = half_rot(hk[a.mirror[o]], a.rot[k]);
-          gk[o] = make_float2((float)x.r, (float)x.i);
+          gk[o] = HP::make(x);
           const cd pred = c_scale(c_mul(rho, c_conj(x)), (double)a.sgn[o]);
           const double dr = xm.r - pred.r, di = xm.i - pred.i;
           res = fmax(res, dr * dr + di * di);
@@ -104,10 +125,12 @@ __global__ __launch_bounds__(256) void k_pw_to_half(HalfCopyArgs a) {
                            (worst != worst ? 0x7ff8000000000000ull : 0ull));
 }
 
-template <int NH>
+// The NH stored slots of one wave and event: FP32 pairs in one 8- or 16-byte load, FP64 pairs
+// in one or two 16-byte loads; get() widens slot o to FP64.
+template <int NH, typename S>
 struct HalfVec;
 template <>
-struct HalfVec<1> {
+struct HalfVec<1, float> {
   typedef float2 T;
   TPA_DEV static T load(const T* p) {
     typedef float f2v __attribute__((ext_vector_type(2)));
@@ -121,7 +144,7 @@ struct HalfVec<1> {
   }
 };
 template <>
-struct HalfVec<2> {
+struct HalfVec<2, float> {
   typedef float4 T;
   TPA_DEV static T load(const T* p) {
     typedef float f4v __attribute__((ext_vector_type(4)));
@@ -133,23 +156,57 @@ struct HalfVec<2> {
     y = o == 0 ? v.y : v.w;
   }
 };
+typedef double tpa_d2v __attribute__((ext_vector_type(2)));
+template <>
+struct HalfVec<1, double> {
+  typedef double2 T;
+  TPA_DEV static T load(const T* p) {
+    const tpa_d2v v = __builtin_nontemporal_load(reinterpret_cast<const tpa_d2v*>(p));
+    return make_double2(v.x, v.y);
+  }
+  TPA_DEV static void get(const T& v, int o, double& x, double& y) {
+    (void)o;
+    x = v.x;
+    y = v.y;
+  }
+};
+struct HalfD2x2 {
+  double2 a, b;
+};
+template <>
+struct HalfVec<2, double> {
+  typedef HalfD2x2 T;
+  TPA_DEV static T load(const T* p) {
+    const tpa_d2v* q = reinterpret_cast<const tpa_d2v*>(p);
+    const tpa_d2v v0 = __builtin_nontemporal_load(q), v1 = __builtin_nontemporal_load(q + 1);
+    return HalfD2x2{make_double2(v0.x, v0.y), make_double2(v1.x, v1.y)};
+  }
+  TPA_DEV static void get(const T& v, int o, double& x, double& y) {
+    x = o == 0 ? v.a.x : v.b.x;
+    y = o == 0 ? v.a.y : v.b.y;
+  }
+};
 
 // Empty asm that makes the FP32 registers opaque to common-subexpression elimination, so the
-// adjoint widens them again rather than keeping the first loop's FP64 values live.
+// adjoint widens them again rather than keeping the first loop's FP64 values live (FP64
+// storage needs no widening: nothing to do).
 TPA_DEV void half_opaque(float2& v) { asm("" : "+v"(v.x), "+v"(v.y)); }
 TPA_DEV void half_opaque(float4& v) { asm("" : "+v"(v.x), "+v"(v.y), "+v"(v.z), "+v"(v.w)); }
+TPA_DEV void half_opaque(double2&) {}
+TPA_DEV void half_opaque(HalfD2x2&) {}
 
 // One event group of the halved pass. A lane holds PML waves of one event (k = kg + KG j) and
 // the chain phase of each; A_o and B_o = A_-o / s_o are summed over the KG lanes of the event.
 // The adjoint per wave: G~_k += u sum_o [conj(A_o) g_k,o + conj(B_o) rho' conj(g_k,o)]
 //                            = sum_o conj(u A_o) g_k,o + rho' conj(sum_o u B_o g_k,o).
-template <int NH, int KG, int PML>
+template <int NH, int KG, int PML, typename S>
 TPA_DEV void eval_half_group(int mode, bool grad, double* __restrict__ f_out,
-                             const typename HalfVec<NH>::T (&hs)[PML], const float2 (&rs)[PML],
+                             const typename HalfVec<NH, S>::T (&hs)[PML],
+                             const typename HalfPair<S>::T (&rs)[PML],
                              double w, int64_t e, bool valid, bool leader,
                              const double (&cr)[PML], const double (&ci)[PML], double& vs,
                              double& vc, double (&gr)[PML], double (&gi)[PML]) {
-  typedef HalfVec<NH> V;
+  typedef HalfVec<NH, S> V;
   const double eps = 1.0e-6;
   double ar[NH], ai[NH], br[NH], bi[NH];
 #pragma unroll
@@ -219,7 +276,7 @@ TPA_DEV void eval_half_group(int mode, bool grad, double* __restrict__ f_out,
     for (int j = 0; j < PML; ++j) {
       typename V::T q = hs[j];
       half_opaque(q);
-      float2 rq = rs[j];
+      typename HalfPair<S>::T rq = rs[j];
       half_opaque(rq);
       const double rr = rq.x, ri = rq.y;
       double qr = 0.0, qi = 0.0;
@@ -247,17 +304,23 @@ TPA_DEV void eval_half_group(int mode, bool grad, double* __restrict__ f_out,
   }
 }
 
-// The evaluation pass over the helicity-halved FP32 copy (a.h32: [P][N][NH] float2; a.rho:
-// [chain][N] float2; a.kch[k]: the chain of wave k; coefficients c~ in a.cin). Lane layout:
-// lane = ev * KG + kg, the wave k = kg + KG j in slot j < PML; the NH stored slots of a wave
-// come in one 16-byte (NH = 2) or 8-byte (NH = 1) load. Software-pipelined as k_eval_pipe32:
+// The evaluation pass over a helicity-halved copy: FP32 (S = float; a.h32: [P][N][NH] float2,
+// a.rho: [chain][N] float2) or FP64 (S = double; a.h: [P][N][NH] double2, a.rho64); a.kch[k]:
+// the chain of wave k; coefficients c~ in a.cin. Lane layout: lane = ev * KG + kg, the wave
+// k = kg + KG j in slot j < PML; the NH stored slots of a wave come in one 8- or 16-byte load
+// (FP32) or one or two 16-byte loads (FP64). Software-pipelined as k_eval_pipe32:
 // two register stages, loads of the next event group issued before the current one is used,
 // unconditional loads at clamped addresses past the last event or wave. Per-block partials
 // slot-major as the other passes (k_reduce). Requires N >= 1.
-template <int NH, int KG, int PML>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_eval_half32(EvalArgs a) {
-  typedef HalfVec<NH> V;
+template <int NH, int KG, int PML, typename S>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_eval_half(EvalArgs a) {
+  typedef HalfVec<NH, S> V;
   typedef typename V::T HV;
+  typedef typename HalfPair<S>::T RV;
+  const HV* const hdata = sizeof(S) == 4 ? reinterpret_cast<const HV*>(a.h32)
+                                         : reinterpret_cast<const HV*>(a.h);
+  const RV* const rdata = sizeof(S) == 4 ? reinterpret_cast<const RV*>(a.rho)
+                                         : reinterpret_cast<const RV*>(a.rho64);
   constexpr int EPW = 64 / KG;
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
@@ -267,7 +330,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
   double vs = 0.0, vc = 0.0;
   double gr[PML], gi[PML], cr[PML], ci[PML];
   const HV* base[PML];
-  const float2* rbase[PML];
+  const RV* rbase[PML];
 #pragma unroll
   for (int j = 0; j < PML; ++j) {
     const int k = kg + KG * j;
@@ -276,11 +339,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
     cr[j] = c.x;
     ci[j] = c.y;
     const int kc = k < a.P ? k : a.P - 1;
-    base[j] = reinterpret_cast<const HV*>(a.h32 + (int64_t)kc * a.N * NH);
-    rbase[j] = a.rho + (int64_t)a.kch[kc] * a.N;
+    base[j] = hdata + (int64_t)kc * a.N;
+    rbase[j] = rdata + (int64_t)a.kch[kc] * a.N;
   }
   const int64_t last = a.N - 1;
-  auto load = [&](HV (&hv)[PML], float2 (&rv)[PML], double& wv, int64_t e) {
+  auto load = [&](HV (&hv)[PML], RV (&rv)[PML], double& wv, int64_t e) {
     const int64_t ec = e < a.N ? e : last;
 #pragma unroll
     for (int j = 0; j < PML; ++j) {
@@ -291,7 +354,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
   };
   const int64_t stride = (int64_t)gridDim.x * 4 * EPW;
   HV h0[PML], h1[PML];
-  float2 r0[PML], r1[PML];
+  RV r0[PML], r1[PML];
   double w0, w1;
   const int wave_u = __builtin_amdgcn_readfirstlane(wave);
   const int64_t first = ((int64_t)blockIdx.x * 4 + wave_u) * EPW;
@@ -300,12 +363,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
   for (int64_t b = first; b < a.N; b += 2 * stride, e += 2 * stride) {
     load(h1, r1, w1, e + stride);
     __builtin_amdgcn_sched_barrier(0);
-    eval_half_group<NH, KG, PML>(a.mode, a.grad, a.f_out, h0, r0, w0, e, e < a.N, leader, cr,
+    eval_half_group<NH, KG, PML, S>(a.mode, a.grad, a.f_out, h0, r0, w0, e, e < a.N, leader, cr,
                                  ci, vs, vc, gr, gi);
     if (b + stride >= a.N) break;
     load(h0, r0, w0, e + 2 * stride);
     __builtin_amdgcn_sched_barrier(0);
-    eval_half_group<NH, KG, PML>(a.mode, a.grad, a.f_out, h1, r1, w1, e + stride,
+    eval_half_group<NH, KG, PML, S>(a.mode, a.grad, a.f_out, h1, r1, w1, e + stride,
                                  e + stride < a.N, leader, cr, ci, vs, vc, gr, gi);
   }
 #pragma unroll

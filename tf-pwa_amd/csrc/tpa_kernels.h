@@ -301,9 +301,11 @@ struct EvalArgs {
   int grad;
   int nacc;  // 2 + 2P
   double2 cin[TPA_EVAL_CIN];  // coefficients by value (k_eval_pipe / k_eval_pipe32, coef null)
-  // the helicity-halved pass (k_eval_half32): chain phases [chain][N] and the chain of each
-  // wave (h32 then holds the stored half, [P][N][NH])
+  // the helicity-halved passes (k_eval_half): chain phases [chain][N] of the FP32 copy (rho)
+  // or the FP64 copy (rho64) and the chain of each wave (h32, resp. h, then holds the stored
+  // half, [P][N][NH])
   const float2* __restrict__ rho;
+  const double2* __restrict__ rho64;
   signed char kch[TPA_EVAL_CIN];
 };
 

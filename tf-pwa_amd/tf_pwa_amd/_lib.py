@@ -29,6 +29,7 @@ FLAG_FOUT = 2
 FLAG_SHAPE = 4
 FLAG_F32 = 8
 FLAG_F32H = 16
+FLAG_F64H = 32
 FUSED_MAX_P = 8  # largest model (partial waves) tpa_eval_fused accepts
 CPL_SLOTS = 4  # factor slots per partial wave in TpaCouplingMap (total + up to 3 vertices)
 
@@ -246,6 +247,7 @@ def lib():
         "tpa_fcn_cache_rebuilds": ([vp, ctypes.POINTER(ctypes.c_int64)], ctypes.c_int),
         "tpa_fcn_set_data_storage": ([vp, ctypes.c_int], ctypes.c_int),
         "tpa_fcn_set_data_half": ([vp, ctypes.c_int], ctypes.c_int),
+        "tpa_fcn_set_mc_half": ([vp, ctypes.c_int], ctypes.c_int),
         "tpa_fcn_minimize": ([vp, dp, ctypes.c_double, ctypes.c_int, dp, dp,
                               ctypes.POINTER(ctypes.c_int), ctypes.POINTER(i64),
                               ctypes.POINTER(i64)], ctypes.c_int),
@@ -306,5 +308,5 @@ def exported_symbols():
         "tpa_pool_keep_f32", "tpa_eval_noalloc", "tpa_hess_shape", "tpa_eval_fused",
         "tpa_fused_source", "tpa_fused_compile_seconds", "tpa_pool_add_events",
         "tpa_events_keep_half", "tpa_events_half_residual", "tpa_fcn_set_data_half",
-        "tpa_events_get_half",
+        "tpa_events_get_half", "tpa_fcn_set_mc_half",
     ]

@@ -187,14 +187,16 @@ class EventSet:
         _lib.check(_lib.lib().tpa_events_keep_f32(self._h, 1 if on else 0))
         self._keep32 = bool(on)
 
-    def keep_half(self, on=True):
-        """Keep the helicity-halved FP32 copy of the (folded) cache (tpa_events_keep_half):
-        one helicity slot of each mirror pair in FP32 and one lineshape phase per chain and
-        event, refreshed by every rebuild, for passes with ``half=True``. Making it checks the
-        helicity-flip relation on every event (the residual is ``half_residual()``)."""
+    def keep_half(self, on=True, fp64=False):
+        """Keep the helicity-halved copies of the (folded) cache (tpa_events_keep_half): one
+        helicity slot of each mirror pair and one lineshape phase per chain and event, in FP32
+        (``on``, for passes with ``half=True``) and/or FP64 (``fp64``, for ``half64=True``),
+        refreshed by every rebuild. Making one checks the helicity-flip relation on every event
+        (the residual is ``half_residual()``)."""
         self.ensure_built()
-        _lib.check(_lib.lib().tpa_events_keep_half(self._h, 1 if on else 0))
+        _lib.check(_lib.lib().tpa_events_keep_half(self._h, (1 if on else 0) | (2 if fp64 else 0)))
         self._keep32h = bool(on)
+        self._keep64h = bool(fp64)
 
     def half_copy(self):
         """The helicity-halved copy as the host sees it: (g [P, n, nh] complex64, rho
@@ -227,24 +229,26 @@ class EventSet:
         _lib.check(_lib.lib().tpa_events_half_residual(self._h, _lib.dptr(r)))
         return float(r[0])
 
-    def eval(self, coef, mode, grad=False, fout=False, f32=False, half=False):
+    def eval(self, coef, mode, grad=False, fout=False, f32=False, half=False, half64=False):
         """One pass: (value, G or None, f or None). When the model floats lineshape
         parameters, G is the tuple (G, Dm, Dg) of coupling and lineshape adjoints. ``f32``:
         read the FP32 copy of the cache (keep_f32; values widened to FP64 on load); ``half``:
-        read the helicity-halved FP32 copy (keep_half)."""
+        read the helicity-halved FP32 copy (keep_half); ``half64``: the halved FP64 copy."""
         self.ensure_built()
         P = self.amp.P
         shape = not self._state[0]
         if f32 and not getattr(self, "_keep32", False):
             self.keep_f32(True)
-        if half and not getattr(self, "_keep32h", False):
-            self.keep_half(True)
+        if (half and not getattr(self, "_keep32h", False)) or (
+                half64 and not getattr(self, "_keep64h", False)):
+            self.keep_half(half or getattr(self, "_keep32h", False),
+                           half64 or getattr(self, "_keep64h", False))
         c = np.ascontiguousarray(np.asarray(coef, dtype=np.complex128)).view(np.float64)
         out = np.zeros(2 + (6 if shape else 2) * P, dtype=np.float64)
         f = np.empty(self.n, dtype=np.float64) if fout else None
         flags = ((_lib.FLAG_GRAD if grad else 0) | (_lib.FLAG_FOUT if fout else 0)
                  | (_lib.FLAG_SHAPE if shape else 0) | (_lib.FLAG_F32 if f32 else 0)
-                 | (_lib.FLAG_F32H if half else 0))
+                 | (_lib.FLAG_F32H if half else 0) | (_lib.FLAG_F64H if half64 else 0))
         _lib.check(_lib.lib().tpa_eval(self._h, _lib.dptr(c), mode, flags, _lib.dptr(out),
                                        _lib.dptr(f)))
         G = None

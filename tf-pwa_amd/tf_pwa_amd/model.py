@@ -65,7 +65,8 @@ class Model:
     """Model.nll / nll_grad / nll_grad_batch / nll_grad_hessian (model.py:576-790)."""
 
     def __init__(self, amp, w_bkg=1.0, resolution_size=1, extended=False, precision="fp64",
-                 gram_precision="fp64", data_storage="fp64", evaluation="stream", **kwargs):
+                 gram_precision="fp64", data_storage="fp64", evaluation="stream",
+                 mc_storage="fp64", **kwargs):
         """precision: "fp64" evaluates every MC event per call; "mixed" replaces the MC pass
         by the partial-wave Gram matrix built once on MFMA (valid while the partial waves,
         i.e. lineshape parameters and MC weights, are unchanged; rebuilt otherwise).
@@ -78,7 +79,11 @@ class Model:
         in FP64: half the bytes per data event) or "fp32h" (the helicity-halved FP32 copy:
         one helicity slot of each mirror pair plus one lineshape phase per chain and event,
         from which the mirrored slots follow; 408 instead of 712 B per event at six
-        resonances); the FP32 options with fixed lineshapes, else FP64.
+        resonances); the FP32 options with fixed lineshapes, else FP64. "fp64h": the same
+        halved layout in FP64 (the FP64 cache's values to rounding, 808 instead of 1416 B per
+        event), for the data passes of both precisions.
+        mc_storage: the MC cache the FP64 path's MC passes read, "fp64" or "fp64h" (the halved
+        FP64 copy); fixed lineshapes, else FP64.
         evaluation: how a pass gets its partial waves, "stream" (read the cache built once,
         k_eval_pipe / k_eval_pipe32) or "fused" (recompute every wave from the angle cache per
         event in one generated kernel, tpa_eval_fused: no partial-wave cache; three-body,
@@ -87,8 +92,11 @@ class Model:
             raise ValueError("precision must be 'fp64' or 'mixed'")
         if gram_precision not in ("fp32", "fp64"):
             raise ValueError("gram_precision must be 'fp32' or 'fp64'")
-        if data_storage not in ("fp32", "fp32h", "fp64"):
-            raise ValueError("data_storage must be 'fp32', 'fp32h' or 'fp64'")
+        if data_storage not in ("fp32", "fp32h", "fp64", "fp64h"):
+            raise ValueError("data_storage must be 'fp32', 'fp32h', 'fp64' or 'fp64h'")
+        if mc_storage not in ("fp64", "fp64h"):
+            raise ValueError("mc_storage must be 'fp64' or 'fp64h'")
+        self.mc_storage = mc_storage
         self.Amp = amp
         self.evaluation = evaluation
         self.precision = precision
@@ -172,7 +180,7 @@ class Model:
                 g = (g,)
             else:
                 v, g, _ = s.eval(coef, mode, grad=grad, f32=self._data_f32(mode),
-                                 half=self._data_half(mode))
+                                 half=self._data_half(mode), half64=self._half64(mode))
                 if grad and not shape:
                     g = (g,)
             value += v
@@ -190,6 +198,15 @@ class Model:
         """Whether a pass of this mode reads the helicity-halved FP32 data caches."""
         return (mode == _lib.MODE_DATA and self.precision == "mixed"
                 and self.data_storage == "fp32h" and not self.Amp.shape_mode)
+
+    def _half64(self, mode):
+        """Whether a pass of this mode reads the helicity-halved FP64 caches (data passes with
+        data_storage "fp64h", FP64-path MC passes with mc_storage "fp64h")."""
+        if self.Amp.shape_mode:
+            return False
+        if mode == _lib.MODE_DATA:
+            return self.data_storage == "fp64h"
+        return self.precision == "fp64" and self.mc_storage == "fp64h"
 
     @staticmethod
     def _split(sets, w):
@@ -405,9 +422,18 @@ class _NativeSession:
             for s in data_sets:
                 s._keep32 = True
         elif model._data_half(_lib.MODE_DATA):
+            # (the session keeps exactly the halved copy it reads: the other one is freed)
             _lib.check(_lib.lib().tpa_fcn_set_data_half(h, 1))
             for s in data_sets:
-                s._keep32h = True
+                s._keep32h, s._keep64h = True, False
+        elif model._half64(_lib.MODE_DATA):
+            _lib.check(_lib.lib().tpa_fcn_set_data_half(h, 2))
+            for s in data_sets:
+                s._keep32h, s._keep64h = False, True
+        if not self.shape and model._half64(_lib.MODE_MC) and gram is None:
+            _lib.check(_lib.lib().tpa_fcn_set_mc_half(h, 1))
+            for s in mc_sets:
+                s._keep32h, s._keep64h = False, True
         self.n = len(self._keep["bound_kind"])
         # argument buffers with their pointers taken once (a ctypes pointer per call costs
         # ~3 us of the host's time per evaluation)
@@ -519,6 +545,7 @@ class FCN:
         for st, w in pairs:
             st.set_weight(w)
         key = (self.model.precision, self.model.gram_precision, self.model.data_storage,
+               getattr(self.model, "mc_storage", "fp64"),
                self.vm.struct_version, tuple([st._weight_version for st, _ in pairs]))
         if self._native_key != key:
             if self._native is not None:
