@@ -710,6 +710,39 @@ def main():
     del f64h
     for s_ in (data, mc):
         s_.keep_half(False)  # release the FP64 halved copies before the next paths
+    # the FP64 path with the partial waves rebuilt from the angle cache on every call, in
+    # chunks (only the angles stay resident); a few steps suffice at tens of ms per call
+    mrec = conf.get_model()
+    mrec.evaluation = "recompute"
+    frec = FCN(mrec, data, mc)
+    frec.nll_grad(x0)
+    sync()
+    L.tpa_timing_reset()
+    L.tpa_timing_enable(1)
+    t = time.perf_counter()
+    rec_steps = 3
+    for _ in range(rec_steps):
+        nrec, grec = frec.nll_grad(x0)
+    sync()
+    rec_s = (time.perf_counter() - t) / rec_steps
+    L.tpa_timing_enable(0)
+    ms_rec = ctypes.c_double(0)
+    nl_rec = ctypes.c_int64(0)
+    L.tpa_timing_read(ctypes.byref(ms_rec), ctypes.byref(nl_rec))
+    dnll_rec = abs(nrec - r64["nll"])
+    grel_rec = float(np.linalg.norm(grec - r64["g"]) / np.linalg.norm(r64["g"]))
+    n_fields = _lib.NF_TOPO * len(amp.topo) + 4
+    rec = {"evals_per_s": 1.0 / rec_s, "ms_per_step": 1e3 * rec_s, "steps": rec_steps,
+           "abs_dnll_vs_fp64": dnll_rec, "grad_rel_err_vs_fp64": grel_rec,
+           "ns_per_event": 1e9 * rec_s / (args.n_data + args.n_mc),
+           "resident_bytes_per_event": 8 * n_fields + 8,
+           "streaming_resident_bytes_per_event": 8 * n_fields + 8 + bytes_per_event - 8,
+           "angle_cache_frac": ANGLE_BYTES_PER_EVENT * (args.n_data + args.n_mc) / rec_s
+           / 1e9 / HBM_PEAK_GBS,
+           "chunk_events": 262144}
+    log("fp64, waves rebuilt per call: {:.2f} evals/s ({:.2f} ns/event), |dNLL| {:.2e}".format(
+        rec["evals_per_s"], rec["ns_per_event"], dnll_rec))
+    del frec
     # the FP32-MFMA Gram variant: its accuracy at the start point (the per-call cost is the
     # same for both Gram precisions, the Gram being built once)
     m32 = conf.get_model()
@@ -803,6 +836,7 @@ def main():
                             "data_storage": "fp64h", "mc_storage": "fp64h",
                             "roofline": roofline(r64h, args.n_data + args.n_mc,
                                                  n_data_per_step=args.n_data, half64=True)},
+            "fp64_recompute": rec,
             "mixed": {"evals_per_s": rmix["evals"], "ms_per_step": rmix["ms_per_step"],
                       "nll": rmix["nll"], "abs_dnll_vs_fp64": dnll, "grad_rel_err_vs_fp64": grel,
                       "speedup_vs_fp64": rmix["evals"] / r64["evals"], "gram": "fp64",

@@ -187,6 +187,19 @@ int tpa_events_get_pw(tpa_events_t ev, double* out, int64_t out_len);
 int tpa_eval(tpa_events_t ev, const double* coef, int mode, int flags, double* out,
              double* f_out);
 
+/* tpa_eval's sums (modes DATA and MC; flags: TPA_FLAG_GRAD) with the partial waves rebuilt
+ * from the angle cache on every call, `chunk` events at a time (chunk <= 0: 262144): each
+ * chunk's waves are built into one chunk buffer (lineshapes folded at the model's current
+ * parameters, as tpa_events_build(ev, 1)), streamed by the evaluation pass and reduced, and the
+ * chunks' sums added in chunk order. Only the angle cache and the weights need to stay
+ * resident (the event set's own partial-wave cache is neither built nor read), so samples too
+ * large for a resident cache can be evaluated; each call pays the build. At most 32 partial
+ * waves. Replaces sum_gradient (tf_pwa/model/model.py:41-94) over the reference's default
+ * path, which recomputes the amplitude from the cached angles on every evaluation
+ * (tf_pwa/amp/core.py:1392-1445, tf_pwa/dfun.py:221-238). */
+int tpa_eval_recompute(tpa_events_t ev, const double* coef, int mode, int flags, int64_t chunk,
+                       double* out);
+
 /* tpa_eval without allocation, for callers that cannot allocate device memory (code under HIP
  * graph capture, tight serving loops, harnesses that forbid hipMalloc). No counterpart in the
  * reference. The pass of tpa_eval over the event set's resident cache (flags: TPA_FLAG_GRAD,

@@ -254,6 +254,11 @@ struct tpa_events_s {
   double2* d_g64h = nullptr;  // the same layout in FP64 (TPA_FLAG_F64H)
   double2* d_rho64 = nullptr;
   unsigned long long* d_resid = nullptr;
+  // per-call rebuild (tpa_eval_recompute): one chunk of waves and the chunks' sums
+  double2* d_rchunk = nullptr;
+  int64_t rchunk_cap = 0;
+  double* d_rout = nullptr;
+  int64_t rout_cap = 0;
   bool keep32h = false;
   bool h32h_valid = false;
   bool keep64h = false;
@@ -670,15 +675,23 @@ static void launch_cal_angle(const tpa_model_s* m, const double* d_p4, int64_t n
     hipLaunchKernelGGL(k_cal_angle3, grid, dim3(256), 0, s, m->d, d_p4, n, off, S, ang);
 }
 
+// Build columns off .. off+n-1 of the angle cache (stride S) into h: the event set's or pool's
+// cache (hS = S, hoff = off: hS < 0 selects these) or a chunk buffer (row stride hS, columns
+// from hoff).
 static void launch_build_pw(const tpa_model_s* m, const double* ang, int64_t n, int64_t off,
-                            int64_t S, int fold, double2* h, hipStream_t s) {
+                            int64_t S, int fold, double2* h, hipStream_t s, int64_t hS = -1,
+                            int64_t hoff = 0) {
+  if (hS < 0) {
+    hS = S;
+    hoff = off;
+  }
   const dim3 grid((unsigned)((n + 127) / 128));
   if (m->kind == 4)
     hipLaunchKernelGGL(k_build_pw4, grid, dim3(128), 0, s, m->d4, ang, n, off, S, m->d_cg,
-                       m->d_dw, fold, h);
+                       m->d_dw, fold, h, hS, hoff);
   else
     hipLaunchKernelGGL(k_build_pw, grid, dim3(128), 0, s, m->d, ang, n, off, S, m->d_cg,
-                       m->d_dw, fold, h);
+                       m->d_dw, fold, h, hS, hoff);
 }
 
 template <int LG, int KG>
@@ -926,6 +939,8 @@ int tpa_events_destroy(tpa_events_t ev) {
   if (ev->d_rho) (void)hipFree(ev->d_rho);
   if (ev->d_g64h) (void)hipFree(ev->d_g64h);
   if (ev->d_rho64) (void)hipFree(ev->d_rho64);
+  if (ev->d_rchunk) (void)hipFree(ev->d_rchunk);
+  if (ev->d_rout) (void)hipFree(ev->d_rout);
   if (ev->d_resid) (void)hipFree(ev->d_resid);
   if (ev->d_cnt) (void)hipFree(ev->d_cnt);
   if (ev->h_flag) (void)hipHostFree(ev->h_flag);
@@ -1629,6 +1644,112 @@ int tpa_eval(tpa_events_t ev, const double* coef, int mode, int flags, double* o
              double* f_out) {
   if (!ev) return fail(TPA_ERR_ARG, "null pointer");
   return eval_impl(ev, coef, mode, flags, out, f_out, ev->m->stream, false);
+}
+
+// Per-call rebuild: the partial waves of `chunk` events at a time built from the angle cache
+// into one chunk buffer (k_build_pw / k_build_pw4, lineshapes folded at the model's current
+// parameters), each chunk streamed by the evaluation pass and reduced into its own slot, the
+// chunks' sums added on the host in chunk order (compensated for the value). Needs only the
+// angle cache and the weights resident.
+#define TPA_RECOMPUTE_CHUNK 262144
+int tpa_eval_recompute(tpa_events_t ev, const double* coef, int mode, int flags, int64_t chunk,
+                       double* out) {
+  if (!ev || !coef || !out) return fail(TPA_ERR_ARG, "null pointer");
+  if (mode != TPA_MODE_DATA && mode != TPA_MODE_MC) return fail(TPA_ERR_ARG, "bad mode");
+  if (flags & ~TPA_FLAG_GRAD)
+    return fail(TPA_ERR_UNSUPPORTED, "tpa_eval_recompute takes TPA_FLAG_GRAD only");
+  tpa_model_s* m = ev->m;
+  const TpaModel& M = m->h;
+  if (M.P > TPA_EVAL_CIN)
+    return fail(TPA_ERR_UNSUPPORTED, "tpa_eval_recompute serves at most 32 partial waves");
+  if (ev->n < 1) return fail(TPA_ERR_ARG, "empty event set");
+  const int kg = eval_kgroups(M.lam_pad, M.P);
+  if (kg < 0) return fail(TPA_ERR_UNSUPPORTED, "too many partial waves for this helicity count");
+  if (chunk <= 0) chunk = TPA_RECOMPUTE_CHUNK;
+  chunk = std::min<int64_t>(chunk, ev->n);
+  const int64_t nch = (ev->n + chunk - 1) / chunk;
+  const int nacc = 2 + 2 * M.P;
+  const int grad = (flags & TPA_FLAG_GRAD) ? 1 : 0;
+  const int nout = grad ? nacc : 2;
+  hipStream_t s = m->stream;
+  if (ev->rchunk_cap < chunk) {
+    HIPCHK(hipStreamSynchronize(s));
+    if (ev->d_rchunk) HIPCHK(hipFree(ev->d_rchunk));
+    ev->d_rchunk = nullptr;
+    ev->rchunk_cap = 0;
+    HIPCHK(hipMalloc(&ev->d_rchunk, sizeof(double2) * (size_t)M.P * chunk * M.lam_pad));
+    ev->rchunk_cap = chunk;
+  }
+  if (ev->rout_cap < nch * nacc) {
+    HIPCHK(hipStreamSynchronize(s));
+    if (ev->d_rout) HIPCHK(hipFree(ev->d_rout));
+    ev->d_rout = nullptr;
+    ev->rout_cap = 0;
+    HIPCHK(hipMalloc(&ev->d_rout, sizeof(double) * (size_t)nch * nacc));
+    ev->rout_cap = nch * nacc;
+  }
+  bool timing;
+  {
+    std::lock_guard<std::mutex> lk(g_timing_mu);
+    timing = g_timing;
+  }
+  if (timing) HIPCHK(hipEventRecord(ev->t0, s));
+  for (int64_t c = 0; c < nch; ++c) {
+    const int64_t off = c * chunk, cn = std::min<int64_t>(chunk, ev->n - off);
+    // the chunk's rows have stride cn, the event count of the pass that reads them
+    launch_build_pw(m, ev->d_ang, cn, off, ev->n, 1, ev->d_rchunk, s, cn, 0);
+    HIPCHK(hipGetLastError());
+    EvalArgs a{};
+    a.h = ev->d_rchunk;
+    a.h32 = nullptr;
+    a.w = ev->d_w + off;
+    a.coef = nullptr;
+    std::memcpy(a.cin, coef, sizeof(double2) * M.P);
+    a.partials = ev->d_part;
+    a.f_out = nullptr;
+    a.N = cn;
+    a.P = M.P;
+    a.mode = mode;
+    a.grad = grad;
+    a.nacc = nacc;
+    a.rho = nullptr;
+    a.rho64 = nullptr;
+    const int grid = eval_grid(cn, M.lam_pad * kg);
+    const int rc = dispatch_eval(M.lam_pad, kg, a, grid, s);
+    if (rc != TPA_OK) return fail(rc, "no evaluation kernel for this helicity/partial-wave count");
+    HIPCHK(hipGetLastError());
+    hipLaunchKernelGGL(k_reduce, dim3(nout), dim3(256), 0, s, ev->d_part, grid, nacc, 1,
+                       ev->d_rout + c * nacc, nullptr, nullptr, nullptr, (uint64_t)0);
+    HIPCHK(hipGetLastError());
+  }
+  if (timing) HIPCHK(hipEventRecord(ev->t1, s));
+  std::vector<double> r((size_t)nch * nacc);
+  HIPCHK(hipMemcpyAsync(r.data(), ev->d_rout, sizeof(double) * r.size(), hipMemcpyDeviceToHost,
+                        s));
+  HIPCHK(hipStreamSynchronize(s));
+  double vs = 0.0, vc = 0.0;
+  for (int64_t c = 0; c < nch; ++c) {
+    const double y = r[(size_t)c * nacc];
+    const double t = vs + y;
+    vc += std::fabs(vs) >= std::fabs(y) ? (vs - t) + y : (y - t) + vs;
+    vs = t;
+  }
+  out[0] = vs + vc;
+  out[1] = 0.0;
+  for (int j = 2; j < nout; ++j) {
+    double g = 0.0;
+    for (int64_t c = 0; c < nch; ++c) g += r[(size_t)c * nacc + j];
+    out[j] = g;
+  }
+  if (timing) {
+    HIPCHK(hipEventSynchronize(ev->t1));
+    float ms = 0.0f;
+    HIPCHK(hipEventElapsedTime(&ms, ev->t0, ev->t1));
+    std::lock_guard<std::mutex> lk(g_timing_mu);
+    g_timing_ms += ms;
+    g_timing_n += 1;
+  }
+  return TPA_OK;
 }
 
 int tpa_eval_enqueue(tpa_events_t ev, const double* coef, int mode, int flags, void* stream) {

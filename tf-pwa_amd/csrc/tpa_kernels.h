@@ -131,8 +131,10 @@ __global__ __launch_bounds__(128) void k_build_pw(const TpaModel* __restrict__ M
                                                   int64_t off, int64_t S,
                                                   const double* __restrict__ cg,
                                                   const double* __restrict__ dw, int fold,
-                                                  double2* __restrict__ h) {
-  // columns off .. off+n-1 of the angle cache and of h (both with stride S)
+                                                  double2* __restrict__ h, int64_t hS,
+                                                  int64_t hoff) {
+  // columns off .. off+n-1 of the angle cache (stride S) into columns hoff .. hoff+n-1 of h
+  // (row stride hS: the cache itself with hS = S, hoff = off, or a chunk buffer)
   const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= n) return;
   const int64_t col = off + e;
@@ -236,7 +238,7 @@ __global__ __launch_bounds__(128) void k_build_pw(const TpaModel* __restrict__ M
     for (int s1 = 0; s1 < C.n_ls1; ++s1) {
       for (int s2 = 0; s2 < C.n_ls2; ++s2) {
         const int64_t k = C.k0 + s1 * C.n_ls2 + s2;
-        double2* hk = h + ((int64_t)k * S + col) * lam_pad;
+        double2* hk = h + ((int64_t)k * hS + hoff + e) * lam_pad;
         int lf = 0;
         for (int iA = 0; iA < nA; ++iA)
           for (int o0 = 0; o0 < nfh[0]; ++o0)

@@ -83,7 +83,8 @@ __global__ __launch_bounds__(128) void k_build_pw4(const TpaModel4* __restrict__
                                                    int64_t off, int64_t S,
                                                    const double* __restrict__ cg,
                                                    const double* __restrict__ dw, int fold,
-                                                   double2* __restrict__ h) {
+                                                   double2* __restrict__ h, int64_t hS,
+                                                   int64_t hoff) {
   const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= n) return;
   const int64_t col = off + e;
@@ -154,7 +155,7 @@ __global__ __launch_bounds__(128) void k_build_pw4(const TpaModel4* __restrict__
         for (int s2 = 0; s2 < C.n_ls[2]; ++s2) {
           const int64_t k = C.k0 + (s0 * C.n_ls[1] + s1) * C.n_ls[2] + s2;
           const int sv[3] = {s0, s1, s2};
-          double2* hk = h + ((int64_t)k * S + col) * lam_pad;
+          double2* hk = h + ((int64_t)k * hS + hoff + e) * lam_pad;
           for (int iA = 0; iA < nA; ++iA) {
             cd sum = c_make(0.0, 0.0);
             for (int i1 = 0; i1 < C.r_n[0]; ++i1)

@@ -223,6 +223,7 @@ def lib():
         "tpa_hess_shape": ([vp, dp, ctypes.c_int, dp], ctypes.c_int),
         "tpa_pool_add_events": ([vp, vp, dp, ctypes.POINTER(i64)], ctypes.c_int),
         "tpa_eval_fused": ([vp, dp, ctypes.c_int, ctypes.c_int, dp], ctypes.c_int),
+        "tpa_eval_recompute": ([vp, dp, ctypes.c_int, ctypes.c_int, i64, dp], ctypes.c_int),
         "tpa_fused_source": ([vp, i64, dp, ctypes.c_int, ctypes.c_char_p, i64,
                               ctypes.POINTER(i64)], ctypes.c_int),
         "tpa_fused_compile_seconds": ([vp, dp], ctypes.c_int),
@@ -308,5 +309,5 @@ def exported_symbols():
         "tpa_pool_keep_f32", "tpa_eval_noalloc", "tpa_hess_shape", "tpa_eval_fused",
         "tpa_fused_source", "tpa_fused_compile_seconds", "tpa_pool_add_events",
         "tpa_events_keep_half", "tpa_events_half_residual", "tpa_fcn_set_data_half",
-        "tpa_events_get_half", "tpa_fcn_set_mc_half",
+        "tpa_events_get_half", "tpa_fcn_set_mc_half", "tpa_eval_recompute",
     ]

@@ -257,6 +257,19 @@ class EventSet:
             G = (z[:P].copy(), z[P:2 * P].copy(), z[2 * P:].copy()) if shape else z.copy()
         return out[0], G, f
 
+    def eval_recompute(self, coef, mode, grad=False, chunk=0):
+        """One pass with the partial waves rebuilt from the angle cache on this call, ``chunk``
+        events at a time (tpa_eval_recompute; 0: the library's default): the event set's own
+        cache is neither built nor read. (value, G or None)."""
+        self.amp.device()
+        P = self.amp.P
+        c = np.ascontiguousarray(np.asarray(coef, dtype=np.complex128)).view(np.float64)
+        out = np.zeros(2 + 2 * P, dtype=np.float64)
+        _lib.check(_lib.lib().tpa_eval_recompute(self._h, _lib.dptr(c), mode,
+                                                 _lib.FLAG_GRAD if grad else 0, int(chunk),
+                                                 _lib.dptr(out)))
+        return out[0], (out[2:].view(np.complex128).copy() if grad else None)
+
     def eval_fused(self, coef, mode, grad=False):
         """One pass of the fused kernel (tpa_eval_fused): the waves recomputed from the angle
         cache on every call, no partial-wave cache. (value, G or None)."""

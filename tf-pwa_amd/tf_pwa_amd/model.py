@@ -85,9 +85,11 @@ class Model:
         mc_storage: the MC cache the FP64 path's MC passes read, "fp64" or "fp64h" (the halved
         FP64 copy); fixed lineshapes, else FP64.
         evaluation: how a pass gets its partial waves, "stream" (read the cache built once,
-        k_eval_pipe / k_eval_pipe32) or "fused" (recompute every wave from the angle cache per
+        k_eval_pipe / k_eval_pipe32), "fused" (recompute every wave from the angle cache per
         event in one generated kernel, tpa_eval_fused: no partial-wave cache; three-body,
-        fixed lineshapes)."""
+        fixed lineshapes, at most 8 waves) or "recompute" (rebuild the waves from the angle
+        cache on every call in chunks of ``recompute_chunk`` events, tpa_eval_recompute: only
+        the angles stay resident; three- and four-body, fixed lineshapes)."""
         if precision not in ("fp64", "mixed"):
             raise ValueError("precision must be 'fp64' or 'mixed'")
         if gram_precision not in ("fp32", "fp64"):
@@ -97,6 +99,7 @@ class Model:
         if mc_storage not in ("fp64", "fp64h"):
             raise ValueError("mc_storage must be 'fp64' or 'fp64h'")
         self.mc_storage = mc_storage
+        self.recompute_chunk = int(kwargs.pop("recompute_chunk", 0))
         self.Amp = amp
         self.evaluation = evaluation
         self.precision = precision
@@ -150,8 +153,8 @@ class Model:
 
     @evaluation.setter
     def evaluation(self, value):
-        if value not in ("stream", "fused"):
-            raise ValueError("evaluation must be 'stream' or 'fused'")
+        if value not in ("stream", "fused", "recompute"):
+            raise ValueError("evaluation must be 'stream', 'fused' or 'recompute'")
         if value == "fused" and self.Amp.P > _lib.FUSED_MAX_P:
             raise NotImplementedError(
                 "the fused pass is disabled above {} partial waves (this model has {}): the "
@@ -177,6 +180,11 @@ class Model:
                 if shape:
                     raise NotImplementedError("the fused pass needs fixed lineshapes")
                 v, g = s.eval_fused(coef, mode, grad=grad)
+                g = (g,)
+            elif self.evaluation == "recompute":
+                if shape:
+                    raise NotImplementedError("the per-call rebuild needs fixed lineshapes")
+                v, g = s.eval_recompute(coef, mode, grad=grad, chunk=self.recompute_chunk)
                 g = (g,)
             else:
                 v, g, _ = s.eval(coef, mode, grad=grad, f32=self._data_f32(mode),
@@ -535,8 +543,8 @@ class FCN:
         amp = self.model.Amp
         if not self.native or (amp.shape_mode and amp.n_final != 3):
             return None
-        if getattr(self.model, "evaluation", "stream") == "fused":
-            return None  # the fused passes run through the host assembly (Model._pass)
+        if getattr(self.model, "evaluation", "stream") in ("fused", "recompute"):
+            return None  # these passes run through the host assembly (Model._pass)
         dw, mw, sw = self._split
         pairs = self._set_weights
         if pairs is None:
