@@ -3,8 +3,8 @@ built from the angle cache on every call, chunk by chunk, and streamed.
 
 Bars: against the streaming pass over the resident FP64 cache, value 1e-12 relative and G 1e-11
 of its largest entry (the same arithmetic, summed per chunk); through FCN, the oracle's NLL to
-1e-6 and gradient to 1e-7 relative (the FP64 bar), three-body and four-body (the latter against
-the streaming FCN, 1e-8).
+1e-6 and gradient to 1e-7 relative (the FP64 bar) and the streaming FCN to 1e-8, three-body and
+four-body.
 """
 
 import numpy as np
@@ -74,11 +74,9 @@ def test_recompute_fcn_matches_oracle(which, request, gpu_available):
     fr = FCN(m, data, mc)
     nll, g = fr.nll_grad()
     assert fr._native_session() is None
-    if which == "case4":
-        n0, g0 = FCN(conf.get_model(), data, mc).nll_grad()
-        assert abs(nll - n0) < 1e-8
-        assert np.linalg.norm(g - g0) <= 1e-10 * np.linalg.norm(g0)
-    else:
-        o_nll, o_g = oracle_nll_grad(case)
-        assert abs(nll - o_nll) < 1e-6, (nll, o_nll)
-        assert np.linalg.norm(g - o_g) <= 1e-7 * np.linalg.norm(o_g)
+    n0, g0 = FCN(conf.get_model(), data, mc).nll_grad()
+    assert abs(nll - n0) < 1e-8
+    assert np.linalg.norm(g - g0) <= 1e-10 * np.linalg.norm(g0)
+    o_nll, o_g = oracle_nll_grad(case)
+    assert abs(nll - o_nll) < 1e-6, (nll, o_nll)
+    assert np.linalg.norm(g - o_g) <= 1e-7 * np.linalg.norm(o_g)

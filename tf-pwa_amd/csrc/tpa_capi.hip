@@ -502,6 +502,31 @@ static int eval_grid(int64_t n, int lanes_per_event) {
   return (int)std::min<int64_t>(g, gmax);
 }
 
+// Blocks of the helicity-halved passes, which run two waves per SIMD: one resident block of
+// four waves per two SIMDs of every CU (2 x the CU count: 512 on the MI355X), so each wave
+// slot holds one wave for the whole pass and no second, partial round of blocks trails it
+// (1M-event data pass at config 2: 85.6 us at 512 blocks, 88.0 at 768, 88.9-90.0 at 1024,
+// one process each on one box). TFPWA_EVAL_GRID overrides it as for the other passes.
+static int half_grid(int64_t n, int lanes_per_event) {
+  static const int gmax = [] {
+    const char* e = std::getenv("TFPWA_EVAL_GRID");
+    if (e) {
+      const int v = std::atoi(e);
+      if (v >= 64 && v <= kEvalGridMax) return v;
+    }
+    int dev = 0, cus = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+        cus < 1)
+      return kEvalGridDefault;
+    return std::min(kEvalGridMax, std::max(64, 2 * cus));
+  }();
+  const int64_t epb = 4 * (64 / lanes_per_event);
+  int64_t g = (n + epb - 1) / epb;
+  if (g < 1) g = 1;
+  return (int)std::min<int64_t>(g, gmax);
+}
+
 #define TPA_POOL_ALIGN 64  // events: the granule of pool ranges
 #define TPA_POOL_SLOTS 2   // batches that can be in flight on a pool at once
 
@@ -1541,7 +1566,7 @@ static int eval_impl(tpa_events_s* ev, const double* coef, int mode, int flags, 
   const int kg = eval_kgroups(M.lam_pad, M.P);
   if (kg < 0) return fail(TPA_ERR_UNSUPPORTED, "too many partial waves for this helicity count");
   const int kgh = half64 ? half64_kgroups(M.P) : half_kgroups(M.P);
-  int grid = half ? eval_grid(ev->n, kgh)
+  int grid = half ? half_grid(ev->n, kgh)
                   : eval_grid(ev->n, f32 ? f32_lanes_per_event(M.lam_pad, kg) : M.lam_pad * kg);
   bool timing = false;
   if (!async) {
