@@ -268,7 +268,8 @@ void complete(tpa_server_s* S, Batch& B, std::unique_lock<std::mutex>& lk) {
 // fits are gathered and the next batch is queued behind it, so the GPU does not idle between
 // batches and a request does not wait for the batch ahead to return before it is launched.
 // A batch launches when every fit outside the running batch is waiting, when max_batch
-// requests are pending, or window_us after the first of them.
+// requests are pending, when the running batch returns (the requests gathered meanwhile go
+// at once), or, with none running, window_us after the first request.
 // While a batch runs the batcher checks it between yields rather than sleeping on a timed
 // wait: a timed wait oversleeps by the kernel's timer slack (~50 us by default), which would
 // land on every request's latency. The spinning costs one host core while the GPU works.
@@ -306,17 +307,16 @@ void batcher_main(tpa_server_s* S) {
       if (S->stop || (int)S->pending.size() >= S->clients - busy ||
           (int)S->pending.size() >= S->max_batch)
         break;
-      if (run >= 0 && tpa_pool_eval_ready(S->pool, bt[run].slot) != 0) {
-        complete(S, bt[run], lk);
-        run = -1;
+      if (run >= 0) {
+        // a batch is on the GPU: gather until it returns, then launch at once (the running
+        // batch is delivered right after the launch), so the GPU only waits for the launch
+        if (tpa_pool_eval_ready(S->pool, bt[run].slot) != 0) break;
+        nap();
         continue;
       }
       const Clock::time_point now = Clock::now();
       if (now >= deadline) break;
-      if (run >= 0)
-        nap();
-      else
-        S->cv.wait_until(lk, WaitClock::now() + (deadline - now));
+      S->cv.wait_until(lk, WaitClock::now() + (deadline - now));
     }
     if (S->stop) break;
     const int nb = run == 0 ? 1 : 0;
