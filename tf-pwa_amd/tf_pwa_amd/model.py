@@ -606,10 +606,12 @@ class FCN:
     def nll_grad(self, x=None):
         t = time.perf_counter()
         nll, g = self.get_nll_grad(x)
-        cv, cg = self._constr()
-        self.cached_nll = nll + cv
+        if self.gauss_constr:
+            cv, cg = self._constr()
+            nll, g = nll + cv, g + cg
+        self.cached_nll = nll
         self.last_time = time.perf_counter() - t
-        return float(self.cached_nll), g + cg
+        return float(nll), g
 
     def grad(self, x=None):
         return self.nll_grad(x)[1]
