@@ -213,6 +213,7 @@ def serve_once(amp, mc, toys, y0s, arrivals, storage, args, diagnose=True):
     m = len(toys)
     t = time.perf_counter()
     srv = FitServer(amp, mc, hbm_budget=args.serve_budget_gb * 1e9, max_batch=m,
+                    window_us=args.serve_window_us,
                     data_storage=storage)
     setup_s = time.perf_counter() - t
     jobs = [dict(events=toys[i], y0=y0s[i]) for i in range(m)]
@@ -568,6 +569,8 @@ def main():
     ap.add_argument("--serve-rate", type=float, default=512.0,
                     help="Poisson arrival rate of the serving leg's fits per second")
     ap.add_argument("--serve-budget-gb", type=float, default=200.0)
+    ap.add_argument("--serve-window-us", type=float, default=200.0,
+                    help="the batcher's gathering window while no batch is on the GPU")
     ap.add_argument("--serve-mc", type=int, default=10_000_000,
                     help="flat MC events generated on the device for the serving leg's Gram")
     ap.add_argument("--four-body-mc", type=int, default=50_000_000,

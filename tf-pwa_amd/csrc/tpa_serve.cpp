@@ -14,6 +14,7 @@
 // A fit's result does not depend on which other fits share its batches: tpa_pool_eval's
 // per-request sums have a fixed order, and each fit's BFGS is deterministic.
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cmath>
 #include <complex>
@@ -59,13 +60,23 @@ struct Fit {
   double nll = NAN;
   int status = -1;
   int64_t nfev = 0, niter = 0;
-  // the request slot shared with the batcher (guarded by the server mutex)
+  // the request slot shared with the batcher: coef and t_submit guarded by the server mutex;
+  // the result (out, failed, ready) by the fit's own mutex, so a fit woken with its result does
+  // not queue on the server mutex (with many fits per batch that convoy idled the GPU)
   std::vector<double> coef, out;
   bool ready = false, failed = false;
   Clock::time_point t_submit;
+  std::mutex rmu;
   std::condition_variable cv;
   std::thread th;
 };
+
+// Wake a fit waiting for its result: the fit's mutex is taken so a waiter between its
+// predicate check and its wait cannot miss the notification.
+void wake(Fit* F) {
+  { std::lock_guard<std::mutex> fl(F->rmu); }
+  F->cv.notify_all();
+}
 
 }  // namespace
 
@@ -82,8 +93,11 @@ struct tpa_server_s : tpa_cpl::CouplingMap {
   std::vector<Fit*> fits;      // by id
   std::deque<Fit*> pending;
   int clients = 0;             // fit threads alive + direct evaluations in progress
+  // copies of pending.size() and clients, written under mu, read without it by the batcher
+  // while it waits for a running batch
+  std::atomic<int> n_pending{0}, n_clients{0};
   std::vector<int> finished;   // ids not yet polled
-  bool stop = false;
+  std::atomic<bool> stop{false};  // written under mu; read also by fits waiting on their own mutex
   std::string batch_error;
   std::thread batcher;
   std::vector<double> lat_us;
@@ -95,19 +109,26 @@ namespace {
 
 // One request through the batcher: coefficients in, data sum and adjoint out.
 void request(tpa_server_s* S, Fit* F, const std::vector<cplx>& c) {
-  std::unique_lock<std::mutex> lk(S->mu);
-  if (S->stop) throw Cancelled();
-  F->coef.resize(2 * S->P);
-  for (int k = 0; k < S->P; ++k) {
-    F->coef[2 * k] = c[k].real();
-    F->coef[2 * k + 1] = c[k].imag();
+  {
+    std::lock_guard<std::mutex> lk(S->mu);
+    if (S->stop) throw Cancelled();
+    F->coef.resize(2 * S->P);
+    for (int k = 0; k < S->P; ++k) {
+      F->coef[2 * k] = c[k].real();
+      F->coef[2 * k + 1] = c[k].imag();
+    }
+    {
+      std::lock_guard<std::mutex> fl(F->rmu);
+      F->ready = false;
+      F->failed = false;
+    }
+    F->t_submit = Clock::now();
+    S->pending.push_back(F);
+    S->n_pending.store((int)S->pending.size());
   }
-  F->ready = false;
-  F->failed = false;
-  F->t_submit = Clock::now();
-  S->pending.push_back(F);
   S->cv.notify_all();
-  F->cv.wait(lk, [&] { return F->ready || S->stop; });
+  std::unique_lock<std::mutex> fl(F->rmu);
+  F->cv.wait(fl, [&] { return F->ready || S->stop; });
   if (!F->ready) throw Cancelled();
   if (F->failed) throw EngineFailure();
 }
@@ -213,6 +234,7 @@ void fit_main(tpa_server_s* S, Fit* F) {
   }
   std::lock_guard<std::mutex> lk(S->mu);
   S->clients -= 1;
+  S->n_clients.store(S->clients);
   S->finished.push_back(F->id);
   S->cv.notify_all();
 }
@@ -228,8 +250,14 @@ struct Batch {
   bool failed = false;
 };
 
-// Hand a finished batch's sums to its fits (server mutex held).
-void deliver(tpa_server_s* S, Batch& B, double ms, const std::string& err) {
+// Hand a finished batch's sums to its fits. The counters and latencies are recorded under the
+// server mutex (held on entry and on return); each fit then gets its result under its own mutex
+// with the server mutex released, so the fits woken by the batch and the fits submitting new
+// requests do not queue behind the batcher while it wakes them one by one. (The fit's mutex is
+// held across the notification: a fit of a direct evaluation lives on its caller's stack and
+// may end as soon as it sees its result.)
+void deliver(tpa_server_s* S, Batch& B, double ms, const std::string& err,
+             std::unique_lock<std::mutex>& lk) {
   const int nacc = 2 + 2 * S->P;
   const int m = (int)B.fits.size();
   const auto now = Clock::now();
@@ -238,15 +266,19 @@ void deliver(tpa_server_s* S, Batch& B, double ms, const std::string& err) {
   for (int i = 0; i < m; ++i) S->n_events += B.ranges[2 * i + 1];
   S->gpu_ms += ms;
   if (B.failed) S->batch_error = err;
+  for (int i = 0; i < m; ++i)
+    S->lat_us.push_back(std::chrono::duration<double, std::micro>(now - B.fits[i]->t_submit).count());
+  lk.unlock();
   for (int i = 0; i < m; ++i) {
     Fit* F = B.fits[i];
+    std::lock_guard<std::mutex> fl(F->rmu);
     if (!B.failed)
       F->out.assign(B.out.begin() + (size_t)nacc * i, B.out.begin() + (size_t)nacc * (i + 1));
     F->failed = B.failed;
     F->ready = true;
-    S->lat_us.push_back(std::chrono::duration<double, std::micro>(now - F->t_submit).count());
     F->cv.notify_one();
   }
+  lk.lock();
   B.fits.clear();
   B.slot = -1;
 }
@@ -261,7 +293,7 @@ void complete(tpa_server_s* S, Batch& B, std::unique_lock<std::mutex>& lk) {
   const std::string err = rc == TPA_OK ? std::string() : std::string(tpa_last_error());
   lk.lock();
   B.failed = rc != TPA_OK;
-  deliver(S, B, ms, err);
+  deliver(S, B, ms, err, lk);
 }
 
 // The batcher keeps up to two batches on the GPU: while one runs, the requests of the other
@@ -272,27 +304,27 @@ void complete(tpa_server_s* S, Batch& B, std::unique_lock<std::mutex>& lk) {
 // at once), or, with none running, window_us after the first request.
 // While a batch runs the batcher checks it between yields rather than sleeping on a timed
 // wait: a timed wait oversleeps by the kernel's timer slack (~50 us by default), which would
-// land on every request's latency. The spinning costs one host core while the GPU works.
+// land on every request's latency. The spinning costs one host core while the GPU works; it
+// runs without the server mutex, so the fits queue requests meanwhile.
 void batcher_main(tpa_server_s* S) {
   const int P = S->P;
   Batch bt[2];
   int run = -1;  // index in bt of the batch on the GPU (-1: none)
   std::unique_lock<std::mutex> lk(S->mu);
-  auto nap = [&lk]() {
-    lk.unlock();
-    std::this_thread::yield();
-    lk.lock();
-  };
   for (;;) {
-    // wait for requests; meanwhile deliver the running batch as soon as it is done
+    // wait for requests; meanwhile deliver the running batch as soon as it is done (that wait
+    // runs without the server mutex, watching the pending count)
     while (!S->stop && S->pending.empty()) {
       if (run >= 0) {
-        if (tpa_pool_eval_ready(S->pool, bt[run].slot) != 0) {
+        const int slot = bt[run].slot;
+        lk.unlock();
+        while (!S->stop && S->n_pending.load() == 0 && tpa_pool_eval_ready(S->pool, slot) == 0)
+          std::this_thread::yield();
+        lk.lock();
+        if (S->pending.empty() && tpa_pool_eval_ready(S->pool, slot) != 0) {
           complete(S, bt[run], lk);
           run = -1;
-          continue;
         }
-        nap();
       } else {
         S->cv.wait(lk);
       }
@@ -309,10 +341,17 @@ void batcher_main(tpa_server_s* S) {
         break;
       if (run >= 0) {
         // a batch is on the GPU: gather until it returns, then launch at once (the running
-        // batch is delivered right after the launch), so the GPU only waits for the launch
-        if (tpa_pool_eval_ready(S->pool, bt[run].slot) != 0) break;
-        nap();
-        continue;
+        // batch is delivered right after the launch), so the GPU only waits for the launch.
+        // The wait runs without the server mutex, watching the counters the fits update, so
+        // fits queue their requests meanwhile without waiting for the batcher.
+        const int slot = bt[run].slot;
+        lk.unlock();
+        while (!S->stop && tpa_pool_eval_ready(S->pool, slot) == 0 &&
+               S->n_pending.load() < S->n_clients.load() - busy &&
+               S->n_pending.load() < S->max_batch)
+          std::this_thread::yield();
+        lk.lock();
+        break;
       }
       const Clock::time_point now = Clock::now();
       if (now >= deadline) break;
@@ -326,6 +365,7 @@ void batcher_main(tpa_server_s* S) {
       B.fits.push_back(S->pending.front());
       S->pending.pop_front();
     }
+    S->n_pending.store((int)S->pending.size());
     const int m = (int)B.fits.size();
     B.ranges.resize(2 * m);
     B.coefs.resize((size_t)2 * P * m);
@@ -342,7 +382,7 @@ void batcher_main(tpa_server_s* S) {
     lk.lock();
     if (rc != TPA_OK) {
       B.failed = true;
-      deliver(S, B, 0.0, err);
+      deliver(S, B, 0.0, err, lk);
       continue;
     }
     B.slot = slot;
@@ -353,8 +393,8 @@ void batcher_main(tpa_server_s* S) {
   }
   if (run >= 0) complete(S, bt[run], lk);
   // wake every waiting client so it sees the stop
-  for (Fit* F : S->fits) F->cv.notify_all();
-  for (Fit* F : S->pending) F->cv.notify_all();
+  for (Fit* F : S->fits) wake(F);
+  for (Fit* F : S->pending) wake(F);
 }
 
 }  // namespace
@@ -390,7 +430,7 @@ int tpa_server_destroy(tpa_server_t S) {
     std::lock_guard<std::mutex> lk(S->mu);
     S->stop = true;
     S->cv.notify_all();
-    for (Fit* F : S->fits) F->cv.notify_all();
+    for (Fit* F : S->fits) wake(F);
   }
   if (S->batcher.joinable()) S->batcher.join();
   for (Fit* F : S->fits)
@@ -411,6 +451,7 @@ int tpa_server_nll_grad(tpa_server_t S, int64_t offset, int64_t n, double sw, co
     std::lock_guard<std::mutex> lk(S->mu);
     if (S->stop) return tpa_set_error(TPA_ERR_ARG, "server stopped");
     S->clients += 1;
+    S->n_clients.store(S->clients);
   }
   int rc = TPA_OK;
   try {
@@ -423,6 +464,7 @@ int tpa_server_nll_grad(tpa_server_t S, int64_t offset, int64_t n, double sw, co
   }
   std::lock_guard<std::mutex> lk(S->mu);
   S->clients -= 1;
+  S->n_clients.store(S->clients);
   S->cv.notify_all();
   return rc;
 }
@@ -457,6 +499,7 @@ int tpa_server_submit(tpa_server_t S, int64_t offset, int64_t n, double sw, cons
   F->x0.assign(y0, y0 + S->nv);
   S->fits.push_back(F);
   S->clients += 1;
+  S->n_clients.store(S->clients);
   F->th = std::thread(fit_main, S, F);
   *fit_id = F->id;
   return TPA_OK;
