@@ -425,6 +425,37 @@ static int dispatch_half64(int nh, const EvalArgs& a, int grid, hipStream_t s) {
   return TPA_ERR_UNSUPPORTED;
 }
 
+// The buffer-load variant of the FP32 halved pass (k_eval_half_buf): two stored slots, four
+// k-groups, every buffer offset below 2^31 bytes (the waves' rows up to KG * PML and the
+// chain phases, each up to two grid strides past the last event). Elsewhere k_eval_half.
+static bool half_buf_fits(const EvalArgs& a, int nh, int kg, int n_chain) {
+  if (nh != 2 || kg != 4 || a.P > 32) return false;
+  // the loads of the group after the last one reach up to two grid strides past N
+  const uint64_t N = (uint64_t)a.N, lim = 1ull << 31, past = 2ull * 64 * kEvalGridMax;
+  return ((uint64_t)(4 * 8 + 1) * N + past) * 16 < lim && ((uint64_t)n_chain * N + past) * 8 < lim;
+}
+
+template <bool ADJ32>
+static void launch_half_buf_t(const EvalArgs& a, int n_chain, int grid, hipStream_t s) {
+  const uint32_t hb = (uint32_t)((uint64_t)a.P * a.N * 16);
+  const uint32_t rb = (uint32_t)((uint64_t)n_chain * a.N * 8);
+  const int need = (a.P + 3) / 4;
+  if (need <= 4)
+    hipLaunchKernelGGL((k_eval_half_buf<4, 4, ADJ32>), dim3(grid), dim3(256), 0, s, a, hb, rb);
+  else if (need <= 6)
+    hipLaunchKernelGGL((k_eval_half_buf<4, 6, ADJ32>), dim3(grid), dim3(256), 0, s, a, hb, rb);
+  else
+    hipLaunchKernelGGL((k_eval_half_buf<4, 8, ADJ32>), dim3(grid), dim3(256), 0, s, a, hb, rb);
+}
+
+static void launch_half_buf(const EvalArgs& a, int n_chain, bool adj32, int grid,
+                            hipStream_t s) {
+  if (adj32)
+    launch_half_buf_t<true>(a, n_chain, grid, s);
+  else
+    launch_half_buf_t<false>(a, n_chain, grid, s);
+}
+
 static int dispatch_half(int nh, int kg, const EvalArgs& a, int grid, hipStream_t s) {
   if (a.P > 4 * 8) return TPA_ERR_UNSUPPORTED;
   if (nh == 1) {
@@ -1531,6 +1562,8 @@ static int eval_impl(tpa_events_s* ev, const double* coef, int mode, int flags, 
   if (half && (shape || f32 || (half32 && half64)))
     return fail(TPA_ERR_UNSUPPORTED,
                 "TPA_FLAG_F32H / TPA_FLAG_F64H exclude TPA_FLAG_SHAPE, TPA_FLAG_F32 and each other");
+  if ((flags & TPA_FLAG_ADJ32) && !half32)
+    return fail(TPA_ERR_UNSUPPORTED, "TPA_FLAG_ADJ32 goes with TPA_FLAG_F32H");
   if ((half32 && !ev->h32h_valid) || (half64 && !ev->h64h_valid))
     return fail(TPA_ERR_ARG, "no helicity-halved copy of the cache (tpa_events_keep_half)");
   if (half && !args_coef)
@@ -1566,6 +1599,8 @@ static int eval_impl(tpa_events_s* ev, const double* coef, int mode, int flags, 
   const int kg = eval_kgroups(M.lam_pad, M.P);
   if (kg < 0) return fail(TPA_ERR_UNSUPPORTED, "too many partial waves for this helicity count");
   const int kgh = half64 ? half64_kgroups(M.P) : half_kgroups(M.P);
+  const int half_chains = ev->m->kind == 4 ? ev->m->h4.n_chain : M.n_chain;  // rows of rho
+  const bool half_buf = half32 && half_buf_fits(a, ev->half_nh, kgh, half_chains);
   int grid = half ? half_grid(ev->n, kgh)
                   : eval_grid(ev->n, f32 ? f32_lanes_per_event(M.lam_pad, kg) : M.lam_pad * kg);
   bool timing = false;
@@ -1597,9 +1632,14 @@ static int eval_impl(tpa_events_s* ev, const double* coef, int mode, int flags, 
     grid = eval_grid(ev->n, M.lam_pad * kgs);
     rc = dispatch_shape(M.lam_pad, kgs, sa, grid, s);
   } else {
-    rc = half64 ? dispatch_half64(ev->half_nh, a, grid, s)
-         : half ? dispatch_half(ev->half_nh, kgh, a, grid, s)
-                : dispatch_eval(M.lam_pad, kg, a, grid, s);
+    if (half_buf) {
+      launch_half_buf(a, half_chains, (flags & TPA_FLAG_ADJ32) != 0, grid, s);
+      rc = TPA_OK;
+    } else {
+      rc = half64 ? dispatch_half64(ev->half_nh, a, grid, s)
+           : half ? dispatch_half(ev->half_nh, kgh, a, grid, s)
+                  : dispatch_eval(M.lam_pad, kg, a, grid, s);
+    }
   }
   if (rc != TPA_OK) return fail(rc, "no evaluation kernel for this helicity/partial-wave count");
   HIPCHK(hipGetLastError());

@@ -18,6 +18,17 @@
 
 #ifndef TPA_LOG_FN
 #define TPA_LOG_FN __device__ __forceinline__
+// A series coefficient as an opaque scalar-register value at its point of use: each FMA of
+// the series then reads it from a scalar register instead of the compiler keeping it in a
+// vector register for the whole evaluation loop of a pass.
+__device__ __forceinline__ double tpa_log_coef(double v) {
+  asm volatile("" : "+s"(v));
+  return v;
+}
+#define TPA_LOG_COEF(x) tpa_log_coef(x)
+#endif
+#ifndef TPA_LOG_COEF
+#define TPA_LOG_COEF(x) (x)
 #endif
 
 template <class Approx>
@@ -44,16 +55,18 @@ TPA_LOG_FN double tpa_log_pos(double x, Approx approx) {
   double s = num * r;
   s = fma(fma(-s, den, num), r, s);  // s = num / den to an ulp
   const double z = s * s;
-  // Estrin's scheme in w = z^2 over the pairs (1/(4i+3) + z/(4i+5)): fewer registers live at
-  // once in the evaluation passes than Horner's, the same ten coefficients
-  const double w = z * z;
-  const double q0 = fma(z, 1.0 / 5.0, 1.0 / 3.0), q1 = fma(z, 1.0 / 9.0, 1.0 / 7.0);
-  const double q2 = fma(z, 1.0 / 13.0, 1.0 / 11.0), q3 = fma(z, 1.0 / 17.0, 1.0 / 15.0);
-  const double q4 = fma(z, 1.0 / 21.0, 1.0 / 19.0);
-  double p = fma(w, q4, q3);
-  p = fma(w, p, q2);
-  p = fma(w, p, q1);
-  p = fma(w, p, q0);
+  // Horner's scheme over the ten coefficients 1/3 .. 1/21: each step takes one constant, so
+  // on the device every coefficient can sit in a scalar register (an FMA with two non-inline
+  // constants keeps one of them in a vector register for the whole pass)
+  double p = fma(z, TPA_LOG_COEF(1.0 / 21.0), TPA_LOG_COEF(1.0 / 19.0));
+  p = fma(z, p, TPA_LOG_COEF(1.0 / 17.0));
+  p = fma(z, p, TPA_LOG_COEF(1.0 / 15.0));
+  p = fma(z, p, TPA_LOG_COEF(1.0 / 13.0));
+  p = fma(z, p, TPA_LOG_COEF(1.0 / 11.0));
+  p = fma(z, p, TPA_LOG_COEF(1.0 / 9.0));
+  p = fma(z, p, TPA_LOG_COEF(1.0 / 7.0));
+  p = fma(z, p, TPA_LOG_COEF(1.0 / 5.0));
+  p = fma(z, p, TPA_LOG_COEF(1.0 / 3.0));
   const double s2 = s + s;
   const double fe = (double)e;
   // e ln2_hi + 2s carries the leading bits; 2s z p + e ln2_lo is the small tail
