@@ -883,13 +883,14 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(PML <= 6 ? 
 
 // Fixed-order reduction over the per-block partial sums: one block per output slot. Slot 0/1
 // are the (sum, compensation) pair of the value and fold into out[0]; every other slot is a
-// compensated sum. Each thread reduces a fixed strided subset, then a fixed tree in LDS
-// combines the threads, so the result does not depend on scheduling.
+// compensated sum. Each thread reduces a fixed strided subset, a fixed butterfly merges the
+// threads of a wave and thread 0 merges the four waves in order, so the result does not
+// depend on scheduling.
 // host_out: pinned host memory for the results. With cnt null every block writes its slot
-// there (and fences at system scope); with cnt non-null the slots go to out (device memory)
-// and the block that finishes last (counted on cnt, which it resets to zero) copies them to
-// host_out in one coalesced write and then writes seq to *flag, so the host can poll for the
-// result instead of waiting on the stream.
+// there (and fences at system scope), or to out (device memory) when host_out is null; with
+// cnt non-null every block writes its slot to host_out and fences at system scope before it
+// is counted on cnt, and the block counted last (which resets cnt to zero) writes seq to
+// *flag, so the host can poll for the result instead of waiting on the stream.
 __global__ __launch_bounds__(256) void k_reduce(const double* __restrict__ partials, int nblocks,
                                                 int nacc, int slot_major,
                                                 double* __restrict__ out,
@@ -899,8 +900,7 @@ __global__ __launch_bounds__(256) void k_reduce(const double* __restrict__ parti
                                                 uint64_t seq) {
   const int j = blockIdx.x;
   const int t = threadIdx.x;
-  __shared__ double ss[256], cc[256];
-  __shared__ int last;
+  __shared__ double ss[4], cc[4];
   double s = 0.0, c = 0.0;
   // partial of block b, slot q: partials[q * nblocks + b] (slot-major, contiguous per slot) or
   // partials[b * nacc + q]
@@ -910,47 +910,40 @@ __global__ __launch_bounds__(256) void k_reduce(const double* __restrict__ parti
     if (j == 0)
       for (int b = t; b < nblocks; b += 256) neumaier_add(s, c, partials[b * bs + qs]);
   }
-  ss[t] = s;
-  cc[t] = c;
-  __syncthreads();
-  for (int w = 128; w > 0; w >>= 1) {
-    if (t < w) {
-      double s1 = ss[t], c1 = cc[t];
-      neumaier_merge(s1, c1, ss[t + w], cc[t + w]);
-      ss[t] = s1;
-      cc[t] = c1;
-    }
-    __syncthreads();
+  // compensated pairs merged over the wave (a fixed butterfly), then the four waves in order
+#pragma unroll
+  for (int m = 32; m >= 1; m >>= 1) {
+    const double s2 = __shfl_xor(s, m, 64);
+    const double c2 = __shfl_xor(c, m, 64);
+    neumaier_merge(s, c, s2, c2);
   }
-  const double v = (j == 1) ? 0.0 : ss[0] + cc[0];
+  if ((t & 63) == 0) {
+    ss[t >> 6] = s;
+    cc[t >> 6] = c;
+  }
+  __syncthreads();
+  if (t != 0) return;
+  s = ss[0];
+  c = cc[0];
+  for (int w = 1; w < 4; ++w) neumaier_merge(s, c, ss[w], cc[w]);
+  const double v = (j == 1) ? 0.0 : s + c;
   if (cnt == nullptr) {
-    if (t == 0) {
-      if (host_out != nullptr) {
-        host_out[j] = v;
-        __threadfence_system();
-      } else {
-        out[j] = v;
-      }
+    if (host_out != nullptr) {
+      host_out[j] = v;
+      __threadfence_system();
+    } else {
+      out[j] = v;
     }
     return;
   }
-  if (t == 0) {
-    out[j] = v;
-    __threadfence();
-    last = atomicAdd(cnt, 1u) == gridDim.x - 1;
-  }
-  __syncthreads();
-  if (!last) return;  // uniform per block
-  __threadfence();
-  const volatile double* vout = out;  // the other blocks' slots, past this CU's caches
-  for (int i = t; i < gridDim.x; i += 256) host_out[i] = vout[i];
-  __syncthreads();
-  if (t == 0) {
-    atomicExch(cnt, 0u);
-    __threadfence_system();
-    *flag = seq;
-    __threadfence_system();
-  }
+  // each block's sum straight into the pinned output, made visible to the host before the
+  // block is counted; the last block counted raises the flag the host polls
+  host_out[j] = v;
+  __threadfence_system();
+  if (atomicAdd(cnt, 1u) != gridDim.x - 1) return;
+  atomicExch(cnt, 0u);
+  *flag = seq;
+  __threadfence_system();
 }
 
 // Plain streaming copy (16-byte loads and stores), the calibration kernel for HBM byte
