@@ -269,6 +269,10 @@ struct tpa_events_s {
   double half_resid = -1.0;
   std::vector<unsigned char> half_rot;
   std::vector<signed char> half_kch;
+  // lane plan of the halved FP32 pass (k_eval_half_chain; half_plan_pml 0: none fits)
+  int half_plan_pml = 0;
+  signed char half_plan[TPA_EVAL_CIN];
+  signed char half_pch[8];
   unsigned* d_cnt = nullptr;  // k_reduce's finishing-block counter (zero between calls)
   uint64_t* h_flag = nullptr; // pinned: the sequence number of the last completed reduction
   uint64_t seq = 0;
@@ -446,6 +450,105 @@ static void launch_half_buf_t(const EvalArgs& a, int n_chain, int grid, hipStrea
     hipLaunchKernelGGL((k_eval_half_buf<4, 6, ADJ32>), dim3(grid), dim3(256), 0, s, a, hb, rb);
   else
     hipLaunchKernelGGL((k_eval_half_buf<4, 8, ADJ32>), dim3(grid), dim3(256), 0, s, a, hb, rb);
+}
+
+// Lane plan of k_eval_half_chain<4, PML, 4>: each of the four lanes of an event holds two
+// runs of slots, [0, 4) and [4, PML), each run the waves of one chain (or pad). A chain may
+// span runs of several lanes. Greedy over the chains from the largest: a free run of exactly
+// the chain's remaining size, else the largest free run it fills, else the smallest free run
+// that takes the rest. False when the chains do not fit.
+static bool plan_half_lanes(int P, const signed char* kch, int pml, signed char* plan,
+                            signed char* pch) {
+  constexpr int KG = 4, J0 = 4;
+  if (pml <= J0 || KG * pml > TPA_EVAL_CIN) return false;
+  std::vector<std::vector<int>> waves;
+  for (int k = 0; k < P; ++k) {
+    const int c = kch[k];
+    if (c < 0) return false;
+    if ((int)waves.size() <= c) waves.resize(c + 1);
+    waves[c].push_back(k);
+  }
+  std::vector<int> order;
+  for (int c = 0; c < (int)waves.size(); ++c)
+    if (!waves[c].empty()) order.push_back(c);
+  std::stable_sort(order.begin(), order.end(),
+                   [&](int x, int y) { return waves[x].size() > waves[y].size(); });
+  // runs r = 2 kg + q: start q ? J0 : 0, size q ? pml - J0 : J0
+  bool used[2 * KG] = {false};
+  for (int i = 0; i < KG * pml; ++i) plan[i] = -1;
+  for (int i = 0; i < 2 * KG; ++i) pch[i] = -1;
+  for (int c : order) {
+    size_t done = 0;
+    while (done < waves[c].size()) {
+      const int rem = (int)(waves[c].size() - done);
+      int pick = -1, best = -1;
+      for (int r = 0; r < 2 * KG; ++r)  // exact fit
+        if (!used[r] && (r & 1 ? pml - J0 : J0) == rem) { pick = r; break; }
+      if (pick < 0)
+        for (int r = 0; r < 2 * KG; ++r) {  // largest run the chain fills
+          const int sz = r & 1 ? pml - J0 : J0;
+          if (!used[r] && sz <= rem && sz > best) { pick = r; best = sz; }
+        }
+      if (pick < 0) {
+        best = 1 << 30;
+        for (int r = 0; r < 2 * KG; ++r) {  // smallest run that takes the rest
+          const int sz = r & 1 ? pml - J0 : J0;
+          if (!used[r] && sz >= rem && sz < best) { pick = r; best = sz; }
+        }
+      }
+      if (pick < 0) return false;
+      used[pick] = true;
+      const int kg = pick >> 1, q = pick & 1;
+      const int start = q ? J0 : 0, sz = q ? pml - J0 : J0;
+      pch[kg * 2 + q] = (signed char)c;
+      for (int j = 0; j < sz && done < waves[c].size(); ++j, ++done)
+        plan[kg * pml + start + j] = (signed char)waves[c][done];
+    }
+  }
+  return true;
+}
+
+// Blocks of the chain-grouped pass: as many as are resident at once on the device (the
+// kernel's occupancy, from the runtime) up to one per 64 events; TFPWA_EVAL_GRID overrides.
+template <int PML, bool ADJ32>
+static int half_chain_grid_t(int64_t n) {
+  static const int gmax = [] {
+    const char* e = std::getenv("TFPWA_EVAL_GRID");
+    if (e) {
+      const int v = std::atoi(e);
+      if (v >= 64 && v <= kEvalGridMax) return v;
+    }
+    int dev = 0, cus = 0, per_cu = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(
+            &per_cu, reinterpret_cast<const void*>(&k_eval_half_chain<4, PML, 4, ADJ32>), 256,
+            0) != hipSuccess ||
+        cus < 1 || per_cu < 1)
+      return kEvalGridDefault;
+    return std::min(kEvalGridMax, std::max(64, per_cu * cus));
+  }();
+  int64_t g = (n + 63) / 64;
+  if (g < 1) g = 1;
+  return (int)std::min<int64_t>(g, gmax);
+}
+
+static int half_chain_grid(int64_t n, int pml, bool adj32) {
+  if (pml == 6) return adj32 ? half_chain_grid_t<6, true>(n) : half_chain_grid_t<6, false>(n);
+  return adj32 ? half_chain_grid_t<8, true>(n) : half_chain_grid_t<8, false>(n);
+}
+
+template <bool ADJ32>
+static void launch_half_chain_t(const EvalArgs& a, int pml, int n_chain, int grid,
+                                hipStream_t s) {
+  const uint32_t hb = (uint32_t)((uint64_t)a.P * a.N * 16);
+  const uint32_t rb = (uint32_t)((uint64_t)n_chain * a.N * 8);
+  if (pml == 6)
+    hipLaunchKernelGGL((k_eval_half_chain<4, 6, 4, ADJ32>), dim3(grid), dim3(256), 0, s, a, hb,
+                       rb);
+  else
+    hipLaunchKernelGGL((k_eval_half_chain<4, 8, 4, ADJ32>), dim3(grid), dim3(256), 0, s, a, hb,
+                       rb);
 }
 
 static void launch_half_buf(const EvalArgs& a, int n_chain, bool adj32, int grid,
@@ -1336,6 +1439,13 @@ static int convert_half(tpa_events_s* ev, bool fp64) {
       ev->half_kch[k] = (signed char)c;
     }
   }
+  // lane plan of the chain-grouped FP32 pass (two stored slots, 17 to 32 waves)
+  ev->half_plan_pml = 0;
+  if (nh == 2 && M.P > 16 && M.P <= 32) {
+    const int pml = M.P <= 24 ? 6 : 8;
+    if (plan_half_lanes(M.P, ev->half_kch.data(), pml, ev->half_plan, ev->half_pch))
+      ev->half_plan_pml = pml;
+  }
   const size_t ng = (size_t)M.P * ev->n * nh, nr = (size_t)n_chain * ev->n;
   if (ev->half_nh != nh) {
     if (ev->d_g32h) HIPCHK(hipFree(ev->d_g32h));
@@ -1601,8 +1711,10 @@ static int eval_impl(tpa_events_s* ev, const double* coef, int mode, int flags, 
   const int kgh = half64 ? half64_kgroups(M.P) : half_kgroups(M.P);
   const int half_chains = ev->m->kind == 4 ? ev->m->h4.n_chain : M.n_chain;  // rows of rho
   const bool half_buf = half32 && half_buf_fits(a, ev->half_nh, kgh, half_chains);
-  int grid = half ? half_grid(ev->n, kgh)
-                  : eval_grid(ev->n, f32 ? f32_lanes_per_event(M.lam_pad, kg) : M.lam_pad * kg);
+  const bool half_chain = half_buf && ev->half_plan_pml > 0;
+  int grid = half_chain ? half_chain_grid(ev->n, ev->half_plan_pml, (flags & TPA_FLAG_ADJ32) != 0)
+             : half     ? half_grid(ev->n, kgh)
+                        : eval_grid(ev->n, f32 ? f32_lanes_per_event(M.lam_pad, kg) : M.lam_pad * kg);
   bool timing = false;
   if (!async) {
     std::lock_guard<std::mutex> lk(g_timing_mu);
@@ -1632,7 +1744,15 @@ static int eval_impl(tpa_events_s* ev, const double* coef, int mode, int flags, 
     grid = eval_grid(ev->n, M.lam_pad * kgs);
     rc = dispatch_shape(M.lam_pad, kgs, sa, grid, s);
   } else {
-    if (half_buf) {
+    if (half_chain) {
+      std::memcpy(a.plan, ev->half_plan, sizeof(a.plan));
+      std::memcpy(a.pch, ev->half_pch, sizeof(a.pch));
+      if (flags & TPA_FLAG_ADJ32)
+        launch_half_chain_t<true>(a, ev->half_plan_pml, half_chains, grid, s);
+      else
+        launch_half_chain_t<false>(a, ev->half_plan_pml, half_chains, grid, s);
+      rc = TPA_OK;
+    } else if (half_buf) {
       launch_half_buf(a, half_chains, (flags & TPA_FLAG_ADJ32) != 0, grid, s);
       rc = TPA_OK;
     } else {

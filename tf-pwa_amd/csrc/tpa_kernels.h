@@ -309,6 +309,10 @@ struct EvalArgs {
   const float2* __restrict__ rho;
   const double2* __restrict__ rho64;
   signed char kch[TPA_EVAL_CIN];
+  // k_eval_half_chain: the wave of lane kg's slot j at plan[kg * PML + j] (-1: pad) and the
+  // chain of its two runs at pch[kg * 2 + run] (-1: none)
+  signed char plan[TPA_EVAL_CIN];
+  signed char pch[8];
 };
 
 // Coefficient k of an evaluation pass (from the kernel arguments when a.coef is null).
