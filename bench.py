@@ -645,12 +645,34 @@ def main():
         return (amp.P * (6 + 32 * nh + 8) + 4 * nh * (kgh - 1) + (8 * nh - 1) + 6 + 4 * nh,
                 {"log": 1, "reciprocal": 1})
 
+    def flops_per_event_chain():
+        """Algorithmic FP64 operations of one data event of the chain-grouped halved pass
+        (k_eval_half_chain), FMA = 2, NH = Lam/2 stored slots: per wave A_o += c g and
+        S += c~ conj(g) (16 NH), the adjoint conj(u A_o) g and conj(W g) (16 NH); per run of
+        one chain (two per lane, KG lanes per event) B_o += rho' S (8 NH) and W = conj(rho') u B
+        (6 NH); the k-group reductions of A and B (4 NH (KG - 1)), f (8 NH - 1), the value and
+        its compensated sum (6), the prefactors u A (2 NH). clip_log's log and reciprocal
+        apart."""
+        nh = amp.lam // 2
+        return (amp.P * 32 * nh + kgh * 2 * 14 * nh + 4 * nh * (kgh - 1) + (8 * nh - 1) + 6
+                + 2 * nh, {"log": 1, "reciprocal": 1})
+
+    # the FP32 halved pass at 17 to 32 partial waves and four helicity slots is the
+    # chain-grouped kernel (the six-resonance model's chains fit its lane plan)
+    chain_pass = amp.lam == 4 and 16 < amp.P <= 32
+
     def flops_split_adj32():
         """The same operations with the adjoint's products in FP32 (TPA_FLAG_ADJ32): the FP64
         part (t, A and B, the reductions, f, the value, the prefactors u A, u B) and the FP32
         part (per wave 16 NH + 8, plus the FP64 adds of the FP32 sums, 2 per wave every four
         events, counted as FP64)."""
         nh = amp.lam // 2
+        if chain_pass:
+            # the chain-grouped pass: per wave A and S (16 NH) in FP64 and the adjoint's two
+            # terms (16 NH) in FP32; per run B and W (14 NH) in FP64
+            f64 = (amp.P * 16 * nh + kgh * 2 * 14 * nh + 4 * nh * (kgh - 1) + (8 * nh - 1) + 6
+                   + 2 * nh + amp.P * 2 // 4)
+            return f64, amp.P * 16 * nh
         f64 = (amp.P * (6 + 16 * nh) + 4 * nh * (kgh - 1) + (8 * nh - 1) + 6 + 4 * nh
                + amp.P * 2 // 4)
         return f64, amp.P * (16 * nh + 8)
@@ -691,7 +713,10 @@ def main():
                 tr = traffic["data_1M"] if n_events_per_step == args.n_data else 0.5 * (
                     traffic["data_1M"] + traffic["mc_10M"])
         nd = n_events_per_step if n_data_per_step is None else n_data_per_step
-        fd, special = flops_per_event_half() if (half or half64) else flops_per_event(True)
+        if half and chain_pass:
+            fd, special = flops_per_event_chain()
+        else:
+            fd, special = flops_per_event_half() if (half or half64) else flops_per_event(True)
         fd32 = 0
         if adj32:
             fd, fd32 = flops_split_adj32()
@@ -699,9 +724,9 @@ def main():
         flops_step = nd * fd + (n_events_per_step - nd) * fm
         tflops = flops_step * args.steps / kernel_s / 1e12 if kernel_s > 0 else 0.0
         tflops32 = nd * fd32 * args.steps / kernel_s / 1e12 if kernel_s > 0 else 0.0
-        half_kernel = ("k_eval_half_buf<4, {}, {}>".format((amp.P + 3) // 4,
-                                                           "true" if adj32 else "false")
-                       if amp.P > 16 and amp.lam == 4 else "k_eval_half<float>")
+        half_kernel = ("k_eval_half_chain<4, {}, 4, {}>".format(6 if amp.P <= 24 else 8,
+                                                                 "true" if adj32 else "false")
+                       if chain_pass else "k_eval_half<float>")
         return {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": achieved / HBM_PEAK_GBS, "traffic": tr,
                 "kernel": ("k_eval_half<double>" if half64 else half_kernel if half
