@@ -661,22 +661,6 @@ def main():
     # chain-grouped kernel (the six-resonance model's chains fit its lane plan)
     chain_pass = amp.lam == 4 and 16 < amp.P <= 32
 
-    def flops_split_adj32():
-        """The same operations with the adjoint's products in FP32 (TPA_FLAG_ADJ32): the FP64
-        part (t, A and B, the reductions, f, the value, the prefactors u A, u B) and the FP32
-        part (per wave 16 NH + 8, plus the FP64 adds of the FP32 sums, 2 per wave every four
-        events, counted as FP64)."""
-        nh = amp.lam // 2
-        if chain_pass:
-            # the chain-grouped pass: per wave A and S (16 NH) in FP64 and the adjoint's two
-            # terms (16 NH) in FP32; per run B and W (14 NH) in FP64
-            f64 = (amp.P * 16 * nh + kgh * 2 * 14 * nh + 4 * nh * (kgh - 1) + (8 * nh - 1) + 6
-                   + 2 * nh + amp.P * 2 // 4)
-            return f64, amp.P * 16 * nh
-        f64 = (amp.P * (6 + 16 * nh) + 4 * nh * (kgh - 1) + (8 * nh - 1) + 6 + 4 * nh
-               + amp.P * 2 // 4)
-        return f64, amp.P * (16 * nh + 8)
-
     def flops_per_event(mode_data):
         """Algorithmic FP64 operations of one event of the evaluation pass, FMA = 2: A_lambda
         = sum_k c_k h_k (8 P Lam), the k-group reductions of A (2 Lam (KG - 1)), f = sum |A|^2
@@ -688,7 +672,7 @@ def main():
                 {"log": 1, "reciprocal": 1} if mode_data else {})
 
     def roofline(r, n_events_per_step, f32=False, n_data_per_step=None, half=False,
-                 half64=False, adj32=False):
+                 half64=False):
         """Roofline of a timed path's pass kernel. achieved: the 176 B/event angle cache (the
         path's input per event: 3 topologies x 5 f64, 2 x 3 aligned angles, the weight) of
         the events one launch covers, over the kernel's average launch time; the bytes the
@@ -717,15 +701,10 @@ def main():
             fd, special = flops_per_event_chain()
         else:
             fd, special = flops_per_event_half() if (half or half64) else flops_per_event(True)
-        fd32 = 0
-        if adj32:
-            fd, fd32 = flops_split_adj32()
         fm, _ = flops_per_event_half() if half64 else flops_per_event(False)
         flops_step = nd * fd + (n_events_per_step - nd) * fm
         tflops = flops_step * args.steps / kernel_s / 1e12 if kernel_s > 0 else 0.0
-        tflops32 = nd * fd32 * args.steps / kernel_s / 1e12 if kernel_s > 0 else 0.0
-        half_kernel = ("k_eval_half_chain<4, {}, 4, {}>".format(6 if amp.P <= 24 else 8,
-                                                                 "true" if adj32 else "false")
+        half_kernel = ("k_eval_half_chain<4, {}, 4>".format(6 if amp.P <= 24 else 8)
                        if chain_pass else "k_eval_half<float>")
         return {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": achieved / HBM_PEAK_GBS, "traffic": tr,
@@ -741,10 +720,7 @@ def main():
                               "special_per_data_event": special,
                               "rule": "FMA = 2 flops; log and reciprocal apart",
                               "achieved_TFLOPs": tflops, "peak_TFLOPs": FP64_PEAK_TFLOPS,
-                              "frac": tflops / FP64_PEAK_TFLOPS},
-                **({"fp32_valu": {"flops_per_event_data": fd32, "achieved_TFLOPs": tflops32,
-                                  "peak_TFLOPs": 157.3, "frac": tflops32 / 157.3}}
-                   if adj32 else {})}
+                              "frac": tflops / FP64_PEAK_TFLOPS}}
 
     log("fp64 path ...")
     r64 = time_path(FCN(conf.get_model(), data, mc))
@@ -836,23 +812,7 @@ def main():
     half_resid = max(s.half_residual() for s in fmixh.wdata.sets)
     log("mixed, helicity-halved FP32 data waves: {:.1f} evals/s, |dNLL| {:.2e}, grad rel {:.2e}, "
         "relation residual {:.1e}".format(rmixh["evals"], dnll_h, grel_h, half_resid))
-    # the same with the data pass's adjoint products in FP32 (adjoint_precision "fp32"; the
-    # value is the same, the gradient moves by the FP32 rounding of the per-event products)
-    msa = conf.get_model()
-    msa.data_storage = "fp32h"
-    msa.adjoint_precision = "fp32"
-    rmixa = time_path(FCN(msa, data, mc, precision="mixed"))
-    dnll_a = abs(rmixa["nll"] - r64["nll"])
-    grel_a = float(np.linalg.norm(rmixa["g"] - r64["g"]) / np.linalg.norm(r64["g"]))
-    grel_a_h = float(np.linalg.norm(rmixa["g"] - rmixh["g"]) / np.linalg.norm(rmixh["g"]))
-    log("mixed, helicity-halved FP32 data waves, FP32 adjoint: {:.1f} evals/s, |dNLL| {:.2e}, "
-        "grad rel {:.2e} (against the FP64 adjoint {:.2e})".format(
-            rmixa["evals"], dnll_a, grel_a, grel_a_h))
-    # headline: the fastest mixed path within the metric's |dNLL| < 1e-3 bar whose gradient
-    # is the derivative of its own value (every operation after the load FP64; FP64 path if
-    # none). The FP32-adjoint path is reported beside it: its gradient carries the FP32
-    # rounding of the per-event products, ~5e-7 of its size, a noise floor above a BFGS
-    # gradient tolerance at a million events.
+    # headline: the fastest mixed path within the metric's |dNLL| < 1e-3 bar (FP64 if none)
     cands = [("mixed_f32h_data", rmixh, dnll_h, False, True,
               "f64 arithmetic (data partial waves stored f32, one helicity of each mirror pair "
               "plus one lineshape phase per chain, widened on load; data log-sum in f64; MC "
@@ -866,11 +826,9 @@ def main():
     if ok_c:
         path, head, _, head_f32, head_half, dtype = max(ok_c, key=lambda c: c[1]["evals"])
         head_events = args.n_data
-        head_adj32 = False
     else:
         path, head, head_events, head_f32, head_half = "fp64", r64, args.n_data + args.n_mc, \
             False, False
-        head_adj32 = False
         dtype = "f64"
     ok = path != "fp64"
     # MFMA work of the Gram kernels: FP64, (2P/16)(2P/16+1)/2 16x16 tiles of 2*16*16 flops per
@@ -896,7 +854,7 @@ def main():
         "config": {"workload": "Lc->pKpi 6 resonances, 22 partial waves, 1M data + 10M MC",
                    "n_data": args.n_data, "n_mc": args.n_mc, "n_params": len(x0),
                    "path": path},
-        "roofline": roofline(head, head_events, head_f32, half=head_half, adj32=head_adj32),
+        "roofline": roofline(head, head_events, head_f32, half=head_half),
         "paths": {
             "fp64": {"evals_per_s": r64["evals"], "ms_per_step": r64["ms_per_step"],
                      "nll": r64["nll"],
@@ -925,14 +883,6 @@ def main():
                                 "gram": "fp64", "data_storage": "fp32h",
                                 "relation_residual": half_resid,
                                 "roofline": roofline(rmixh, args.n_data, half=True)},
-            "mixed_f32h_adj32": {"evals_per_s": rmixa["evals"], "ms_per_step": rmixa["ms_per_step"],
-                                 "nll": rmixa["nll"], "abs_dnll_vs_fp64": dnll_a,
-                                 "grad_rel_err_vs_fp64": grel_a,
-                                 "grad_rel_err_vs_fp64_adjoint": grel_a_h,
-                                 "speedup_vs_fp64": rmixa["evals"] / r64["evals"],
-                                 "gram": "fp64", "data_storage": "fp32h",
-                                 "adjoint_precision": "fp32",
-                                 "roofline": roofline(rmixa, args.n_data, half=True, adj32=True)},
             "mixed_fp32_gram": {"abs_dnll_vs_fp64": dnll32, "grad_rel_err_vs_fp64": grel32,
                                 "evals_per_s": "same per-call work as mixed"},
             "gram_build": {

@@ -56,15 +56,6 @@ typedef struct tpa_fcn_s* tpa_fcn_t;
 /* The same from the helicity-halved FP64 copy (tpa_events_keep_half with bit 1): the FP64
  * cache's values to rounding, 808 instead of 1416 B per event for the six-resonance model. */
 #define TPA_FLAG_F64H 32
-/* With TPA_FLAG_F32H: the adjoint's per-event products (conj(u A) h, u B h, the chain phase)
- * in FP32, summed per lane in FP32 over at most four events and then in FP64; the amplitude,
- * f, the value, its compensated sum and the per-block sums stay FP64. G moves by ~5e-7 of its
- * size (the per-event products' FP32 rounding, independent between events); the value is the
- * same bit for bit. Used where the pass over the halved copy supports it (two stored
- * helicity slots, 17 to 32 partial waves, buffer offsets below 2^31 bytes: about 3.9M events
- * at 32 waves); elsewhere the adjoint stays FP64. No reference counterpart (tf-pwa's gradient
- * is FP64 throughout, tf_pwa/model/model.py:41-94). */
-#define TPA_FLAG_ADJ32 64
 /* Largest model (partial waves) tpa_eval_fused accepts. */
 #define TPA_FUSED_MAX_P 8
 
@@ -489,8 +480,7 @@ int tpa_fcn_set_data_storage(tpa_fcn_t fcn, int data_f32);
 
 /* The session's data passes read the helicity-halved copies of the data caches
  * (tpa_events_keep_half): FP32 (TPA_FLAG_F32H) when kind = 1, FP64 (TPA_FLAG_F64H) when
- * kind = 2, FP32 with the adjoint's products in FP32 (TPA_FLAG_F32H | TPA_FLAG_ADJ32) when
- * kind = 3, the FP64 caches when kind = 0. Fixed lineshapes only. No reference counterpart: a
+ * kind = 2, the FP64 caches when kind = 0. Fixed lineshapes only. No reference counterpart: a
  * storage option of the data pass behind FCN.nll_grad (tf_pwa/model/model.py:1207-1229). */
 int tpa_fcn_set_data_half(tpa_fcn_t fcn, int kind);
 

@@ -9,9 +9,6 @@ Three bars:
   FP32 inputs, FP64 arithmetic): value 1e-12 relative, G 1e-11 of its largest entry, f 1e-12;
 - the mixed path with data_storage="fp32h" against the FP64 path and the oracle: |dNLL| < 1e-5
   and the gradient within 1e-6 relative at these sizes (the metric's bar is 1e-3).
-With TPA_FLAG_ADJ32 (adjoint_precision="fp32") the value and f stay those of the FP64-adjoint
-pass bit for bit and G moves by the FP32 rounding of its per-event products: 1e-6 of its
-largest entry.
 """
 
 import ctypes
@@ -225,67 +222,3 @@ def test_half64_storage_fp64_and_mixed_paths(which, request, gpu_available):
     assert abs(n2 - o_nll) < 1e-6
     assert np.linalg.norm(g2 - o_g) <= 1e-7 * np.linalg.norm(o_g)
 
-
-@pytest.mark.parametrize("n", [1, 257, 20000, 200003])
-def test_half_pass_fp32_adjoint(n, case2, gpu_available):
-    """TPA_FLAG_ADJ32 on the six-resonance model (the pass that supports it): value and f equal
-    to the FP64-adjoint pass bit for bit, with and without the gradient; G within 1e-6 of its
-    largest entry of the FP64-adjoint pass and of the oracle's C restatement on the cache
-    rebuilt from the same copy, in both modes (ragged event counts, several grid strides)."""
-    from oracle import build_cpu
-    from tf_pwa_amd import _lib
-
-    L = build_cpu.load()
-    conf = case2["conf"]
-    full = conf.data_from_p4(case2["mc"])
-    ev = full.select(np.arange(n) % full.n)
-    w = np.random.default_rng(n + 7).uniform(-0.5, 1.5, n)
-    ev.set_weight(w)
-    ev.keep_half(True)
-    c = conf.get_amplitude().coefficients()
-    h = rebuild_from_half(ev)
-    for mode in (_lib.MODE_DATA, _lib.MODE_MC):
-        v0, G0, f0 = ev.eval(c, mode, grad=True, fout=True, half=True)
-        v1, G1, f1 = ev.eval(c, mode, grad=True, fout=True, half=True, adj32=True)
-        v2, _, _ = ev.eval(c, mode, grad=False, half=True, adj32=True)
-        assert v1 == v0 and v2 == v0
-        np.testing.assert_array_equal(f1, f0)
-        tol = 1e-6 * np.abs(G0).max()
-        np.testing.assert_allclose(G1, G0, rtol=0, atol=tol)
-        out = _cpu_pass(L, h, w, c, 0 if mode == _lib.MODE_DATA else 1)
-        np.testing.assert_allclose(G1, out[2:].view(np.complex128), rtol=0, atol=tol)
-
-
-@pytest.mark.parametrize("which", ["case1", "case2", "case4"])
-def test_fp32_adjoint_mixed_path_close_to_fp64_and_oracle(which, request, gpu_available):
-    """Model(adjoint_precision="fp32") through FCN: the native session and the Python host
-    assembly agree, the NLL is the FP32-halved path's own (the value does not depend on the
-    adjoint's precision), and both stay within the FP32-storage bars of the FP64 path and the
-    oracle (|dNLL| < 1e-5, gradient 1e-6 relative)."""
-    from tf_pwa_amd import FCN
-
-    case = request.getfixturevalue(which)
-    conf = case["conf"]
-    data = conf.data_from_p4(case["data"], weight=case["wd"])
-    mc = conf.data_from_p4(case["mc"])
-    n1, g1 = FCN(conf.get_model(), data, mc).nll_grad()
-    mh = conf.get_model()
-    mh.data_storage = "fp32h"
-    n2, g2 = FCN(mh, data, mc, precision="mixed").nll_grad()
-    ma = conf.get_model()
-    ma.data_storage = "fp32h"
-    ma.adjoint_precision = "fp32"
-    fa = FCN(ma, data, mc, precision="mixed")
-    n3, g3 = fa.nll_grad()
-    assert fa._native is not None
-    assert n3 == n2
-    assert np.linalg.norm(g3 - g2) <= 1e-6 * np.linalg.norm(g2)
-    assert abs(n3 - n1) < 1e-5
-    assert np.linalg.norm(g3 - g1) <= 1e-6 * np.linalg.norm(g1)
-    fa.native = False
-    n4, g4 = fa.nll_grad()
-    assert abs(n4 - n3) <= 1e-12 * abs(n3)
-    assert np.abs(g4 - g3).max() <= 1e-10 * np.abs(g3).max()
-    o_nll, o_g = oracle_nll_grad(case)
-    assert abs(n3 - o_nll) < 1e-5
-    assert np.linalg.norm(g3 - o_g) <= 1e-6 * np.linalg.norm(o_g)

@@ -134,25 +134,9 @@ def test_header_flags_match_the_binding():
 
     header = open(os.path.join(ROOT, "include", "tfpwa_amd.h")).read()
     defs = dict(re.findall(r"^#define TPA_((?:FLAG|MODE)_\w+) (\d+)", header, re.M))
-    assert "FLAG_ADJ32" in defs
+    assert "FLAG_F32H" in defs
     for name, val in defs.items():
         assert getattr(_lib, name) == int(val), name
-
-
-def test_adjoint_precision_option():
-    from tf_pwa_amd import ConfigLoader, _lib
-
-    conf = ConfigLoader(config_path("lc_pkpi_6res.yml"), seed=1)
-    with pytest.raises(ValueError):
-        conf.get_model().__class__(conf.get_amplitude(), adjoint_precision="bf16")
-    m = conf.get_model()
-    m.precision = "mixed"
-    m.data_storage = "fp32h"
-    assert not m._data_adj32(_lib.MODE_DATA)
-    m.adjoint_precision = "fp32"
-    assert m._data_adj32(_lib.MODE_DATA) and not m._data_adj32(_lib.MODE_MC)
-    m.data_storage = "fp32"
-    assert not m._data_adj32(_lib.MODE_DATA)
 
 
 def test_phase_space_generator_invariants():

@@ -429,27 +429,15 @@ static int dispatch_half64(int nh, const EvalArgs& a, int grid, hipStream_t s) {
   return TPA_ERR_UNSUPPORTED;
 }
 
-// The buffer-load variant of the FP32 halved pass (k_eval_half_buf): two stored slots, four
-// k-groups, every buffer offset below 2^31 bytes (the waves' rows up to KG * PML and the
-// chain phases, each up to two grid strides past the last event). Elsewhere k_eval_half.
-static bool half_buf_fits(const EvalArgs& a, int nh, int kg, int n_chain) {
+// The chain-grouped FP32 halved pass (k_eval_half_chain): two stored slots, four k-groups,
+// every buffer offset below 2^31 bytes (the waves' rows up to KG * PML and the chain phases,
+// each up to two grid strides past the last event), and a lane plan (made with the copy).
+// Elsewhere k_eval_half.
+static bool half_chain_fits(const EvalArgs& a, int nh, int kg, int n_chain) {
   if (nh != 2 || kg != 4 || a.P > 32) return false;
   // the loads of the group after the last one reach up to two grid strides past N
   const uint64_t N = (uint64_t)a.N, lim = 1ull << 31, past = 2ull * 64 * kEvalGridMax;
   return ((uint64_t)(4 * 8 + 1) * N + past) * 16 < lim && ((uint64_t)n_chain * N + past) * 8 < lim;
-}
-
-template <bool ADJ32>
-static void launch_half_buf_t(const EvalArgs& a, int n_chain, int grid, hipStream_t s) {
-  const uint32_t hb = (uint32_t)((uint64_t)a.P * a.N * 16);
-  const uint32_t rb = (uint32_t)((uint64_t)n_chain * a.N * 8);
-  const int need = (a.P + 3) / 4;
-  if (need <= 4)
-    hipLaunchKernelGGL((k_eval_half_buf<4, 4, ADJ32>), dim3(grid), dim3(256), 0, s, a, hb, rb);
-  else if (need <= 6)
-    hipLaunchKernelGGL((k_eval_half_buf<4, 6, ADJ32>), dim3(grid), dim3(256), 0, s, a, hb, rb);
-  else
-    hipLaunchKernelGGL((k_eval_half_buf<4, 8, ADJ32>), dim3(grid), dim3(256), 0, s, a, hb, rb);
 }
 
 // Lane plan of k_eval_half_chain<4, PML, 4>: each of the four lanes of an event holds two
@@ -510,7 +498,7 @@ static bool plan_half_lanes(int P, const signed char* kch, int pml, signed char*
 
 // Blocks of the chain-grouped pass: as many as are resident at once on the device (the
 // kernel's occupancy, from the runtime) up to one per 64 events; TFPWA_EVAL_GRID overrides.
-template <int PML, bool ADJ32>
+template <int PML>
 static int half_chain_grid_t(int64_t n) {
   static const int gmax = [] {
     const char* e = std::getenv("TFPWA_EVAL_GRID");
@@ -522,8 +510,8 @@ static int half_chain_grid_t(int64_t n) {
     if (hipGetDevice(&dev) != hipSuccess ||
         hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
         hipOccupancyMaxActiveBlocksPerMultiprocessor(
-            &per_cu, reinterpret_cast<const void*>(&k_eval_half_chain<4, PML, 4, ADJ32>), 256,
-            0) != hipSuccess ||
+            &per_cu, reinterpret_cast<const void*>(&k_eval_half_chain<4, PML, 4>), 256, 0) !=
+            hipSuccess ||
         cus < 1 || per_cu < 1)
       return kEvalGridDefault;
     return std::min(kEvalGridMax, std::max(64, per_cu * cus));
@@ -533,30 +521,18 @@ static int half_chain_grid_t(int64_t n) {
   return (int)std::min<int64_t>(g, gmax);
 }
 
-static int half_chain_grid(int64_t n, int pml, bool adj32) {
-  if (pml == 6) return adj32 ? half_chain_grid_t<6, true>(n) : half_chain_grid_t<6, false>(n);
-  return adj32 ? half_chain_grid_t<8, true>(n) : half_chain_grid_t<8, false>(n);
+static int half_chain_grid(int64_t n, int pml) {
+  return pml == 6 ? half_chain_grid_t<6>(n) : half_chain_grid_t<8>(n);
 }
 
-template <bool ADJ32>
-static void launch_half_chain_t(const EvalArgs& a, int pml, int n_chain, int grid,
-                                hipStream_t s) {
+static void launch_half_chain(const EvalArgs& a, int pml, int n_chain, int grid,
+                              hipStream_t s) {
   const uint32_t hb = (uint32_t)((uint64_t)a.P * a.N * 16);
   const uint32_t rb = (uint32_t)((uint64_t)n_chain * a.N * 8);
   if (pml == 6)
-    hipLaunchKernelGGL((k_eval_half_chain<4, 6, 4, ADJ32>), dim3(grid), dim3(256), 0, s, a, hb,
-                       rb);
+    hipLaunchKernelGGL((k_eval_half_chain<4, 6, 4>), dim3(grid), dim3(256), 0, s, a, hb, rb);
   else
-    hipLaunchKernelGGL((k_eval_half_chain<4, 8, 4, ADJ32>), dim3(grid), dim3(256), 0, s, a, hb,
-                       rb);
-}
-
-static void launch_half_buf(const EvalArgs& a, int n_chain, bool adj32, int grid,
-                            hipStream_t s) {
-  if (adj32)
-    launch_half_buf_t<true>(a, n_chain, grid, s);
-  else
-    launch_half_buf_t<false>(a, n_chain, grid, s);
+    hipLaunchKernelGGL((k_eval_half_chain<4, 8, 4>), dim3(grid), dim3(256), 0, s, a, hb, rb);
 }
 
 static int dispatch_half(int nh, int kg, const EvalArgs& a, int grid, hipStream_t s) {
@@ -1672,8 +1648,6 @@ static int eval_impl(tpa_events_s* ev, const double* coef, int mode, int flags, 
   if (half && (shape || f32 || (half32 && half64)))
     return fail(TPA_ERR_UNSUPPORTED,
                 "TPA_FLAG_F32H / TPA_FLAG_F64H exclude TPA_FLAG_SHAPE, TPA_FLAG_F32 and each other");
-  if ((flags & TPA_FLAG_ADJ32) && !half32)
-    return fail(TPA_ERR_UNSUPPORTED, "TPA_FLAG_ADJ32 goes with TPA_FLAG_F32H");
   if ((half32 && !ev->h32h_valid) || (half64 && !ev->h64h_valid))
     return fail(TPA_ERR_ARG, "no helicity-halved copy of the cache (tpa_events_keep_half)");
   if (half && !args_coef)
@@ -1710,9 +1684,9 @@ static int eval_impl(tpa_events_s* ev, const double* coef, int mode, int flags, 
   if (kg < 0) return fail(TPA_ERR_UNSUPPORTED, "too many partial waves for this helicity count");
   const int kgh = half64 ? half64_kgroups(M.P) : half_kgroups(M.P);
   const int half_chains = ev->m->kind == 4 ? ev->m->h4.n_chain : M.n_chain;  // rows of rho
-  const bool half_buf = half32 && half_buf_fits(a, ev->half_nh, kgh, half_chains);
-  const bool half_chain = half_buf && ev->half_plan_pml > 0;
-  int grid = half_chain ? half_chain_grid(ev->n, ev->half_plan_pml, (flags & TPA_FLAG_ADJ32) != 0)
+  const bool half_chain =
+      half32 && ev->half_plan_pml > 0 && half_chain_fits(a, ev->half_nh, kgh, half_chains);
+  int grid = half_chain ? half_chain_grid(ev->n, ev->half_plan_pml)
              : half     ? half_grid(ev->n, kgh)
                         : eval_grid(ev->n, f32 ? f32_lanes_per_event(M.lam_pad, kg) : M.lam_pad * kg);
   bool timing = false;
@@ -1747,13 +1721,7 @@ static int eval_impl(tpa_events_s* ev, const double* coef, int mode, int flags, 
     if (half_chain) {
       std::memcpy(a.plan, ev->half_plan, sizeof(a.plan));
       std::memcpy(a.pch, ev->half_pch, sizeof(a.pch));
-      if (flags & TPA_FLAG_ADJ32)
-        launch_half_chain_t<true>(a, ev->half_plan_pml, half_chains, grid, s);
-      else
-        launch_half_chain_t<false>(a, ev->half_plan_pml, half_chains, grid, s);
-      rc = TPA_OK;
-    } else if (half_buf) {
-      launch_half_buf(a, half_chains, (flags & TPA_FLAG_ADJ32) != 0, grid, s);
+      launch_half_chain(a, ev->half_plan_pml, half_chains, grid, s);
       rc = TPA_OK;
     } else {
       rc = half64 ? dispatch_half64(ev->half_nh, a, grid, s)

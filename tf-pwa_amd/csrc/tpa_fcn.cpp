@@ -291,18 +291,14 @@ int tpa_fcn_set_data_storage(tpa_fcn_t F, int data_f32) {
 
 int tpa_fcn_set_data_half(tpa_fcn_t F, int kind) {
   if (!F) return tpa_set_error(TPA_ERR_ARG, "null pointer");
-  if (kind < 0 || kind > 3)
-    return tpa_set_error(TPA_ERR_ARG, "kind: 0, 1 (FP32), 2 (FP64) or 3 (FP32, FP32 adjoint)");
+  if (kind < 0 || kind > 2) return tpa_set_error(TPA_ERR_ARG, "kind: 0, 1 (FP32) or 2 (FP64)");
   if (kind && F->shape)
     return tpa_set_error(TPA_ERR_UNSUPPORTED, "halved data caches need fixed lineshapes");
   for (tpa_events_t ev : F->data) {
-    const int rc = tpa_events_keep_half(ev, kind == 3 ? 1 : kind);
+    const int rc = tpa_events_keep_half(ev, kind);
     if (rc != TPA_OK) return rc;
   }
-  F->data_flags = kind == 1   ? TPA_FLAG_F32H
-                  : kind == 2 ? TPA_FLAG_F64H
-                  : kind == 3 ? TPA_FLAG_F32H | TPA_FLAG_ADJ32
-                              : 0;
+  F->data_flags = kind == 1 ? TPA_FLAG_F32H : kind == 2 ? TPA_FLAG_F64H : 0;
   return TPA_OK;
 }
 

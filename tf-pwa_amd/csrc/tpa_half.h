@@ -414,355 +414,22 @@ TPA_DEV __amdgpu_buffer_rsrc_t half_rsrc(const void* p, uint32_t bytes) {
                                            0x00020000);
 }
 
-// One event group of k_eval_half_buf (NH = 2 stored FP32 slots): A_o and B_o and f in FP64
-// exactly as eval_half_group, the derivative u of the mode's function, and the adjoint: in
-// FP64 into gr64, gi64 as eval_half_group (ADJ32 false), or with FP32 products into the
-// lane's FP32 sums gr, gi (ADJ32: per event the products conj(u A_o) g, u B_o g and
-// rho' conj(Q) are rounded once each, independently per event; no coefficient enters them).
-// Returns f: the mode's value is summed by the caller, four groups at a time.
-template <int KG, int PML, bool ADJ32>
-TPA_DEV double eval_half_buf_group(int mode, bool grad, double* __restrict__ f_out,
-                                   const float4 (&hs)[PML], const float2 (&rs)[PML], double w,
-                                   int64_t e, bool valid, bool leader, const double (&cr)[PML],
-                                   const double (&ci)[PML], float (&gr)[PML], float (&gi)[PML],
-                                   double (&gr64)[PML], double (&gi64)[PML]) {
-  typedef HalfVec<2, float> V;
-  const double eps = 1.0e-6;
-  double ar[2], ai[2], br[2], bi[2];
-#pragma unroll
-  for (int o = 0; o < 2; ++o) ar[o] = ai[o] = br[o] = bi[o] = 0.0;
-#pragma unroll
-  for (int j = 0; j < PML; ++j) {
-    const double rr = rs[j].x, ri = rs[j].y;
-    const double tr = fma(cr[j], rr, -(ci[j] * ri));
-    const double ti = fma(cr[j], ri, ci[j] * rr);
-#pragma unroll
-    for (int o = 0; o < 2; ++o) {
-      double x, y;
-      V::get(hs[j], o, x, y);
-      ar[o] = fma(cr[j], x, ar[o]);
-      ar[o] = fma(-ci[j], y, ar[o]);
-      ai[o] = fma(cr[j], y, ai[o]);
-      ai[o] = fma(ci[j], x, ai[o]);
-      br[o] = fma(tr, x, br[o]);
-      br[o] = fma(ti, y, br[o]);
-      bi[o] = fma(ti, x, bi[o]);
-      bi[o] = fma(-tr, y, bi[o]);
-    }
-  }
-  double f = 0.0;
-#pragma unroll
-  for (int o = 0; o < 2; ++o) {
-    ar[o] = butterfly<1, KG>(ar[o]);
-    ai[o] = butterfly<1, KG>(ai[o]);
-    br[o] = butterfly<1, KG>(br[o]);
-    bi[o] = butterfly<1, KG>(bi[o]);
-    f += (ar[o] * ar[o] + ai[o] * ai[o]) + (br[o] * br[o] + bi[o] * bi[o]);
-  }
-  if (f_out != nullptr && valid && leader) f_out[e] = f;
-  if (grad) {
-    const double wv = valid ? w : 0.0;
-    double u;
-    if (mode == 0) {
-      // clip_log'(f) (tf_pwa/model/model.py:283-292)
-      if (f > eps) {
-        u = wv * tpa_recip_pos(f, hw_rcp);
-      } else {
-        const double t = (f - eps) / eps;
-        u = wv * (1.0 / eps - t / eps);
-      }
-    } else {
-      u = wv;
-    }
-    if constexpr (!ADJ32) {
-      double uar[2], uai[2], ubr[2], ubi[2];
-#pragma unroll
-      for (int o = 0; o < 2; ++o) {
-        uar[o] = u * ar[o];
-        uai[o] = u * ai[o];
-        ubr[o] = u * br[o];
-        ubi[o] = u * bi[o];
-      }
-#pragma unroll
-      for (int j = 0; j < PML; ++j) {
-        float4 q = hs[j];
-        half_opaque(q);
-        float2 rq = rs[j];
-        half_opaque(rq);
-        const double rr = rq.x, ri = rq.y;
-        double qr = 0.0, qi = 0.0;
-#pragma unroll
-        for (int o = 0; o < 2; ++o) {
-          double x, yy;
-          V::get(q, o, x, yy);
-          gr64[j] = fma(uar[o], x, gr64[j]);
-          gr64[j] = fma(uai[o], yy, gr64[j]);
-          gi64[j] = fma(uar[o], yy, gi64[j]);
-          gi64[j] = fma(-uai[o], x, gi64[j]);
-          qr = fma(ubr[o], x, qr);
-          qr = fma(-ubi[o], yy, qr);
-          qi = fma(ubr[o], yy, qi);
-          qi = fma(ubi[o], x, qi);
-        }
-        gr64[j] = fma(rr, qr, gr64[j]);
-        gr64[j] = fma(ri, qi, gr64[j]);
-        gi64[j] = fma(ri, qr, gi64[j]);
-        gi64[j] = fma(-rr, qi, gi64[j]);
-      }
-      return f;
-    }
-    float uar[2], uai[2], ubr[2], ubi[2];
-#pragma unroll
-    for (int o = 0; o < 2; ++o) {
-      uar[o] = (float)(u * ar[o]);
-      uai[o] = (float)(u * ai[o]);
-      ubr[o] = (float)(u * br[o]);
-      ubi[o] = (float)(u * bi[o]);
-    }
-#pragma unroll
-    for (int j = 0; j < PML; ++j) {
-      const float4 q = hs[j];
-      const float rr = rs[j].x, ri = rs[j].y;
-      const float xs[2] = {q.x, q.z}, ys[2] = {q.y, q.w};
-      float qr = 0.0f, qi = 0.0f;
-#pragma unroll
-      for (int o = 0; o < 2; ++o) {
-        const float x = xs[o], yy = ys[o];
-        gr[j] = fmaf(uar[o], x, gr[j]);
-        gr[j] = fmaf(uai[o], yy, gr[j]);
-        gi[j] = fmaf(uar[o], yy, gi[j]);
-        gi[j] = fmaf(-uai[o], x, gi[j]);
-        qr = fmaf(ubr[o], x, qr);
-        qr = fmaf(-ubi[o], yy, qr);
-        qi = fmaf(ubr[o], yy, qi);
-        qi = fmaf(ubi[o], x, qi);
-      }
-      gr[j] = fmaf(rr, qr, gr[j]);
-      gr[j] = fmaf(ri, qi, gr[j]);
-      gi[j] = fmaf(ri, qr, gi[j]);
-      gi[j] = fmaf(-rr, qi, gi[j]);
-    }
-  }
-  return f;
-}
-
-
-// The helicity-halved FP32 pass over buffer loads (NH = 2 stored slots, KG lanes per event,
-// PML waves per lane; lane layout and the FP64 amplitude of k_eval_half<2, KG, PML, float>).
-// Per event group it issues fewer vector instructions than k_eval_half:
-// - the mode's value (clip_log with its logarithm, or w f) and its compensated sum run once
-//   per four event groups: the four lanes of an event each take the stored f and w of one of
-//   the four groups, instead of all of them evaluating the same logarithm every group (the
-//   derivative, a reciprocal, stays per group);
-// - with ADJ32 (TPA_FLAG_ADJ32) the adjoint multiplies the FP32 waves in FP32 (no second
-//   widening) into per-lane FP32 sums that are added into the FP64 sums after every four
-//   groups, so an FP32 sum holds at most four events' products;
-// - every load is a buffer load whose lane offset is fixed for the pass and whose event-group
-//   and wave offsets sit in scalar registers: no address arithmetic per load; a load at or
-//   past the end of the copy, the chain phases or the weights returns zero, so the pad waves
-//   k >= P need no clamp and events past the last one read weight zero (their waves come from
-//   the next row, finite, and add exactly zero);
-// - the coefficients sit in LDS and are read at each event group, so they hold no registers
-//   between groups.
-// The host checks that every buffer offset stays below 2^31 bytes. Partial sums per block,
-// slot-major, as the other passes (k_reduce).
-template <int KG, int PML, bool ADJ32>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_eval_half_buf(
-    EvalArgs a, uint32_t h_bytes, uint32_t rho_bytes) {
-  typedef float4 HV;
-  typedef float2 RV;
-  static_assert(KG == 4, "the value is shared out over four lanes per event");
-  constexpr int EPW = 64 / KG;
-  const int lane = threadIdx.x & 63;
-  const int wave = threadIdx.x >> 6;
-  const int kg = lane & (KG - 1);
-  const int el = lane / KG;
-  const bool leader = kg == 0;
-  __shared__ double2 csh[KG * PML];
-  __shared__ double sh[4][TPA_NACC_MAX];
-  if (threadIdx.x < KG * PML)
-    csh[threadIdx.x] = (int)threadIdx.x < a.P ? eval_coef(a, threadIdx.x) : make_double2(0.0, 0.0);
-  __syncthreads();
-  const __amdgpu_buffer_rsrc_t hr = half_rsrc(a.h32, h_bytes);
-  const __amdgpu_buffer_rsrc_t rr = half_rsrc(a.rho, rho_bytes);
-  const __amdgpu_buffer_rsrc_t wr = half_rsrc(a.w, (uint32_t)(a.N * 8));
-  const uint32_t N = (uint32_t)a.N;
-  const uint32_t vh = ((uint32_t)kg * N + (uint32_t)el) * 16u;
-  uint32_t vr[PML];
-#pragma unroll
-  for (int j = 0; j < PML; ++j) {
-    const int k = kg + KG * j;
-    vr[j] = ((uint32_t)a.kch[k < a.P ? k : a.P - 1] * N + (uint32_t)el) * 8u;
-  }
-  const uint32_t vw = (uint32_t)el * 8u;
-  // the argument fields the loop needs, as locals (a lambda that reaches into the argument
-  // struct makes the compiler copy all of it into scratch memory)
-  const int mode = a.mode;
-  const bool grad = a.grad != 0;
-  double* const f_out = a.f_out;
-  const int64_t n_ev = a.N;
-  double vs = 0.0, vc = 0.0;
-  double gr[PML], gi[PML];
-  float gr32[PML], gi32[PML];
-#pragma unroll
-  for (int j = 0; j < PML; ++j) {
-    gr[j] = gi[j] = 0.0;
-    gr32[j] = gi32[j] = 0.0f;
-  }
-  typedef unsigned int u4v __attribute__((ext_vector_type(4)));
-  typedef unsigned int u2v __attribute__((ext_vector_type(2)));
-  auto load = [&](HV (&hv)[PML], RV (&rv)[PML], double& wv, uint32_t gb)
-                  __attribute__((always_inline)) {
-#pragma unroll
-    for (int j = 0; j < PML; ++j) {
-      const u4v x = __builtin_amdgcn_raw_buffer_load_b128(
-          hr, vh, (int)(((uint32_t)(KG * j) * N + gb) * 16u), 2);
-      hv[j] = make_float4(__uint_as_float(x.x), __uint_as_float(x.y), __uint_as_float(x.z),
-                          __uint_as_float(x.w));
-      const u2v r = __builtin_amdgcn_raw_buffer_load_b64(rr, vr[j], (int)(gb * 8u), 0);
-      rv[j] = make_float2(__uint_as_float(r.x), __uint_as_float(r.y));
-    }
-    const u2v x = __builtin_amdgcn_raw_buffer_load_b64(wr, vw, (int)(gb * 8u), 0);
-    wv = __hiloint2double((int)x.y, (int)x.x);
-  };
-  auto group = [&](const HV (&hv)[PML], const RV (&rv)[PML], double wv, uint32_t gb)
-                   __attribute__((always_inline)) {
-    // the coefficients from LDS at every group (an opaque index keeps them out of registers
-    // between groups)
-    int cb = kg;
-    asm volatile("" : "+v"(cb));
-    double cr[PML], ci[PML];
-#pragma unroll
-    for (int j = 0; j < PML; ++j) {
-      const double2 c = csh[cb + KG * j];
-      cr[j] = c.x;
-      ci[j] = c.y;
-    }
-    const int64_t e = (int64_t)gb + el;
-    return eval_half_buf_group<KG, PML, ADJ32>(mode, grad, f_out, hv, rv, wv, e, e < n_ev,
-                                               leader, cr, ci, gr32, gi32, gr, gi);
-  };
-  // the mode's value of up to four stored groups (f_q, w_q), q < nq: lane kg takes group kg
-  // (its weight is zero past the last event); then the FP32 adjoint sums go into the FP64 ones
-  auto values = [&](double f0, double w0_, double f1, double w1_, double f2, double w2_, double f3,
-                    double w3_, int nq) __attribute__((always_inline)) {
-    const bool k1 = kg & 1, k2 = kg & 2;
-    const double f = k2 ? (k1 ? f3 : f2) : (k1 ? f1 : f0);
-    const double w = k2 ? (k1 ? w3_ : w2_) : (k1 ? w1_ : w0_);
-    double y;
-    if (mode == 0) {
-      // clip_log(f) (tf_pwa/model/model.py:283-292) weighted by w
-      const double eps = 1.0e-6;
-      double cl;
-      if (f > eps) {
-        cl = tpa_log_pos(f, hw_rcp);
-      } else {
-        const double t = (f - eps) / eps;
-        cl = (log(eps) + t) - (t * t) / 2.0;
-      }
-      y = w * cl;
-    } else {
-      y = w * f;
-    }
-    neumaier_add(vs, vc, kg < nq ? y : 0.0);
-    if constexpr (ADJ32)
-#pragma unroll
-    for (int j = 0; j < PML; ++j) {
-      gr[j] += (double)gr32[j];
-      gi[j] += (double)gi32[j];
-      gr32[j] = gi32[j] = 0.0f;
-    }
-  };
-  const uint32_t stride = gridDim.x * 4u * EPW;
-  HV h0[PML], h1[PML];
-  RV r0[PML], r1[PML];
-  double w0, w1;
-  const uint32_t wave_u = __builtin_amdgcn_readfirstlane(wave);
-  const uint32_t first = (blockIdx.x * 4u + wave_u) * EPW;
-  // two event groups per iteration; the first pair of a four-group cycle waits in (fp0, wp0),
-  // (fp1, wp1) until the second pair is done
-  double fp0 = 0.0, wp0 = 0.0, fp1 = 0.0, wp1 = 0.0;
-  int cyc = 0;  // uniform: 1 while a first pair waits
-  load(h0, r0, w0, first);
-  uint32_t b = first;
-  for (; b < N; b += 2 * stride) {
-    load(h1, r1, w1, b + stride);
-    __builtin_amdgcn_sched_barrier(0);
-    const double fa = group(h0, r0, w0, b);
-    const double wa = w0;
-    if (b + stride >= N) {
-      if (cyc)
-        values(fp0, wp0, fp1, wp1, fa, wa, 0.0, 0.0, 3);
-      else
-        values(fa, wa, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 1);
-      cyc = 0;
-      break;
-    }
-    load(h0, r0, w0, b + 2 * stride);
-    __builtin_amdgcn_sched_barrier(0);
-    const double fb = group(h1, r1, w1, b + stride);
-    const double wb = w1;
-    if (cyc) {
-      values(fp0, wp0, fp1, wp1, fa, wa, fb, wb, 4);
-    } else {
-      fp0 = fa;
-      wp0 = wa;
-      fp1 = fb;
-      wp1 = wb;
-    }
-    cyc ^= 1;
-  }
-  if (cyc) values(fp0, wp0, fp1, wp1, 0.0, 0.0, 0.0, 0.0, 2);
-#pragma unroll
-  for (int m = 32; m >= 1; m >>= 1) {
-    const double s2 = __shfl_xor(vs, m, 64);
-    const double c2 = __shfl_xor(vc, m, 64);
-    neumaier_merge(vs, vc, s2, c2);
-  }
-  if (a.grad) {
-#pragma unroll
-    for (int j = 0; j < PML; ++j) {
-      const double xr = butterfly<KG, 64>(gr[j]), xi = butterfly<KG, 64>(gi[j]);
-      const int k = kg + KG * j;
-      if (el == 0 && k < a.P) {
-        sh[wave][2 + 2 * k] = xr;
-        sh[wave][3 + 2 * k] = xi;
-      }
-    }
-  }
-  if (lane == 0) {
-    sh[wave][0] = vs;
-    sh[wave][1] = vc;
-  }
-  __syncthreads();
-  double* out = a.partials + blockIdx.x;
-  if (threadIdx.x == 0) {
-    double s = sh[0][0], c = sh[0][1];
-    for (int wv = 1; wv < 4; ++wv) neumaier_merge(s, c, sh[wv][0], sh[wv][1]);
-    out[0] = s;
-    out[gridDim.x] = c;
-  }
-  if (a.grad) {
-    for (int j = 2 + threadIdx.x; j < a.nacc; j += blockDim.x)
-      out[(int64_t)j * gridDim.x] = ((sh[0][j] + sh[1][j]) + sh[2][j]) + sh[3][j];
-  }
-}
-
 // One event group of k_eval_half_chain: the lane's PML waves come in two runs of one chain
 // each (slots j < J0 of chain-slot 0, the rest of chain-slot 1), so the chain phase rho' of
 // a wave is that of its run: B_o = rho'_0 S_0,o + rho'_1 S_1,o with S_s,o = sum over the run
 // of c~_k conj(g_k,o), and the adjoint's phase term folds into W_s,o = conj(rho'_s) u B_o:
-// G~_k += sum_o conj(u A_o) g_k,o + conj(W_s,o g_k,o). The same sums as eval_half_buf_group,
-// with the phase multiplied per run instead of per wave. The FP64 adjoint reuses the waves
-// widened for the forward sums (206 VGPRs, two waves per SIMD: 2-3% faster than widening
-// them again at 166 VGPRs and three waves per SIMD). ADJ32 as eval_half_buf_group.
-template <int KG, int PML, int J0, bool ADJ32>
+// G~_k += sum_o conj(u A_o) g_k,o + conj(W_s,o g_k,o). The same sums as eval_half_group
+// (A, B and f in FP64 from the widened FP32 waves, the adjoint in FP64), with the phase
+// multiplied per run instead of per wave; the adjoint reuses the waves widened for the
+// forward sums (206 VGPRs, two waves per SIMD: 2-3% faster than widening them again at 166
+// VGPRs and three waves per SIMD). Returns f: the mode's value is summed by the caller, four
+// groups at a time.
+template <int KG, int PML, int J0>
 TPA_DEV double eval_half_chain_group(int mode, bool grad, double* __restrict__ f_out,
                                      const float4 (&hs)[PML], const float2 (&rs)[2], double w,
                                      int64_t e, bool valid, bool leader, const double (&cr)[PML],
-                                     const double (&ci)[PML], float (&gr)[PML],
-                                     float (&gi)[PML], double (&gr64)[PML],
-                                     double (&gi64)[PML]) {
+                                     const double (&ci)[PML], double (&gr)[PML],
+                                     double (&gi)[PML]) {
   typedef HalfVec<2, float> V;
   const double eps = 1.0e-6;
   double ar[2], ai[2], sr[2][2], si[2][2];
@@ -830,68 +497,50 @@ TPA_DEV double eval_half_chain_group(int mode, bool grad, double* __restrict__ f
       wr[1][o] = fma(p1r, ubr, p1i * ubi);
       wi[1][o] = fma(p1r, ubi, -(p1i * ubr));
     }
-    if constexpr (!ADJ32) {
-#pragma unroll
-      for (int j = 0; j < PML; ++j) {
-        const int s = j < J0 ? 0 : 1;
-        float4 q = hs[j];
-#pragma unroll
-        for (int o = 0; o < 2; ++o) {
-          double x, yy;
-          V::get(q, o, x, yy);
-          // conj(u A_o) g
-          gr64[j] = fma(uar[o], x, gr64[j]);
-          gr64[j] = fma(uai[o], yy, gr64[j]);
-          gi64[j] = fma(uar[o], yy, gi64[j]);
-          gi64[j] = fma(-uai[o], x, gi64[j]);
-          // conj(W g)
-          gr64[j] = fma(wr[s][o], x, gr64[j]);
-          gr64[j] = fma(-wi[s][o], yy, gr64[j]);
-          gi64[j] = fma(-wr[s][o], yy, gi64[j]);
-          gi64[j] = fma(-wi[s][o], x, gi64[j]);
-        }
-      }
-      return f;
-    }
-    float fuar[2], fuai[2], fwr[2][2], fwi[2][2];
-#pragma unroll
-    for (int o = 0; o < 2; ++o) {
-      fuar[o] = (float)uar[o];
-      fuai[o] = (float)uai[o];
-      fwr[0][o] = (float)wr[0][o];
-      fwi[0][o] = (float)wi[0][o];
-      fwr[1][o] = (float)wr[1][o];
-      fwi[1][o] = (float)wi[1][o];
-    }
 #pragma unroll
     for (int j = 0; j < PML; ++j) {
       const int s = j < J0 ? 0 : 1;
-      const float4 q = hs[j];
-      const float xs[2] = {q.x, q.z}, ys[2] = {q.y, q.w};
+      float4 q = hs[j];
 #pragma unroll
       for (int o = 0; o < 2; ++o) {
-        const float x = xs[o], yy = ys[o];
-        gr[j] = fmaf(fuar[o], x, gr[j]);
-        gr[j] = fmaf(fuai[o], yy, gr[j]);
-        gi[j] = fmaf(fuar[o], yy, gi[j]);
-        gi[j] = fmaf(-fuai[o], x, gi[j]);
-        gr[j] = fmaf(fwr[s][o], x, gr[j]);
-        gr[j] = fmaf(-fwi[s][o], yy, gr[j]);
-        gi[j] = fmaf(-fwr[s][o], yy, gi[j]);
-        gi[j] = fmaf(-fwi[s][o], x, gi[j]);
+        double x, yy;
+        V::get(q, o, x, yy);
+        // conj(u A_o) g
+        gr[j] = fma(uar[o], x, gr[j]);
+        gr[j] = fma(uai[o], yy, gr[j]);
+        gi[j] = fma(uar[o], yy, gi[j]);
+        gi[j] = fma(-uai[o], x, gi[j]);
+        // conj(W g)
+        gr[j] = fma(wr[s][o], x, gr[j]);
+        gr[j] = fma(-wi[s][o], yy, gr[j]);
+        gi[j] = fma(-wr[s][o], yy, gi[j]);
+        gi[j] = fma(-wi[s][o], x, gi[j]);
       }
     }
   }
   return f;
 }
 
-// The helicity-halved FP32 pass with every lane's waves in two runs of one chain each
-// (a lane plan made by the host when the copy is made, tpa_events_keep_half): the chain phase
-// rho' is loaded and multiplied once per run instead of once per wave (two phase loads per
-// lane and event group instead of PML; per wave the t = c~ rho' product and the adjoint's
-// rho' conj(Q) are gone). Otherwise k_eval_half_buf: buffer loads with per-slot lane offsets,
-// the mode's value once per four event groups, coefficients from LDS, ADJ32 likewise.
-template <int KG, int PML, int J0, bool ADJ32>
+// The helicity-halved FP32 pass (NH = 2 stored slots, KG = 4 lanes per event, PML waves per
+// lane) with every lane's waves in two runs of one chain each (a lane plan the host makes with
+// the copy, tpa_events_keep_half): the chain phase rho' is loaded and multiplied once per run
+// instead of once per wave (two phase loads per lane and event group instead of PML; per wave
+// the t = c~ rho' product and the adjoint's rho' conj(Q) are gone). The FP64 sums of
+// k_eval_half<2, 4, PML, float>, with fewer vector instructions per event group:
+// - the mode's value (clip_log with its logarithm, or w f) and its compensated sum run once
+//   per four event groups: the four lanes of an event each take the stored f and w of one of
+//   the four groups, instead of all of them evaluating the same logarithm every group (the
+//   derivative, a reciprocal, stays per group);
+// - every load is a buffer load whose lane offset is fixed for the pass (the lane's row of the
+//   copy per slot, of the chain phases per run) and whose event-group offset sits in a scalar
+//   register: no address arithmetic per load; a load at or past the end of the copy, the chain
+//   phases or the weights returns zero, so pad slots need no clamp and events past the last
+//   one read weight zero (their waves come from the next row, finite, and add exactly zero);
+// - the coefficients sit in LDS and are read at each event group, so they hold no registers
+//   between groups.
+// The host checks that every buffer offset stays below 2^31 bytes. Partial sums per block,
+// slot-major, as the other passes (k_reduce).
+template <int KG, int PML, int J0>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_eval_half_chain(
     EvalArgs a, uint32_t h_bytes, uint32_t rho_bytes) {
   typedef float4 HV;
@@ -939,12 +588,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
   const int64_t n_ev = a.N;
   double vs = 0.0, vc = 0.0;
   double gr[PML], gi[PML];
-  float gr32[PML], gi32[PML];
 #pragma unroll
-  for (int j = 0; j < PML; ++j) {
-    gr[j] = gi[j] = 0.0;
-    gr32[j] = gi32[j] = 0.0f;
-  }
+  for (int j = 0; j < PML; ++j) gr[j] = gi[j] = 0.0;
   typedef unsigned int u4v __attribute__((ext_vector_type(4)));
   typedef unsigned int u2v __attribute__((ext_vector_type(2)));
   auto load = [&](HV (&hv)[PML], RV (&rv)[2], double& wv, uint32_t gb)
@@ -977,8 +622,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
       ci[j] = c.y;
     }
     const int64_t e = (int64_t)gb + el;
-    return eval_half_chain_group<KG, PML, J0, ADJ32>(mode, grad, f_out, hv, rv, wv, e, e < n_ev,
-                                                     leader, cr, ci, gr32, gi32, gr, gi);
+    return eval_half_chain_group<KG, PML, J0>(mode, grad, f_out, hv, rv, wv, e, e < n_ev,
+                                              leader, cr, ci, gr, gi);
   };
   // the mode's value of up to four stored groups (f_q, w_q), q < nq: lane kg takes group kg
   // (its weight is zero past the last event); then the FP32 adjoint sums go into the FP64 ones
@@ -1003,13 +648,6 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
       y = w * f;
     }
     neumaier_add(vs, vc, kg < nq ? y : 0.0);
-    if constexpr (ADJ32)
-#pragma unroll
-    for (int j = 0; j < PML; ++j) {
-      gr[j] += (double)gr32[j];
-      gi[j] += (double)gi32[j];
-      gr32[j] = gi32[j] = 0.0f;
-    }
   };
   const uint32_t stride = gridDim.x * 4u * EPW;
   HV h0[PML], h1[PML];

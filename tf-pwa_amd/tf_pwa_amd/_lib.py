@@ -30,7 +30,6 @@ FLAG_SHAPE = 4
 FLAG_F32 = 8
 FLAG_F32H = 16
 FLAG_F64H = 32
-FLAG_ADJ32 = 64
 FUSED_MAX_P = 8  # largest model (partial waves) tpa_eval_fused accepts
 CPL_SLOTS = 4  # factor slots per partial wave in TpaCouplingMap (total + up to 3 vertices)
 

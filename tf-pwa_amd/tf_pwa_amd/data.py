@@ -229,13 +229,11 @@ class EventSet:
         _lib.check(_lib.lib().tpa_events_half_residual(self._h, _lib.dptr(r)))
         return float(r[0])
 
-    def eval(self, coef, mode, grad=False, fout=False, f32=False, half=False, half64=False,
-             adj32=False):
+    def eval(self, coef, mode, grad=False, fout=False, f32=False, half=False, half64=False):
         """One pass: (value, G or None, f or None). When the model floats lineshape
         parameters, G is the tuple (G, Dm, Dg) of coupling and lineshape adjoints. ``f32``:
         read the FP32 copy of the cache (keep_f32; values widened to FP64 on load); ``half``:
-        read the helicity-halved FP32 copy (keep_half); ``half64``: the halved FP64 copy;
-        ``adj32`` (with ``half``): the adjoint's per-event products in FP32 (TPA_FLAG_ADJ32)."""
+        read the helicity-halved FP32 copy (keep_half); ``half64``: the halved FP64 copy."""
         self.ensure_built()
         P = self.amp.P
         shape = not self._state[0]
@@ -250,8 +248,7 @@ class EventSet:
         f = np.empty(self.n, dtype=np.float64) if fout else None
         flags = ((_lib.FLAG_GRAD if grad else 0) | (_lib.FLAG_FOUT if fout else 0)
                  | (_lib.FLAG_SHAPE if shape else 0) | (_lib.FLAG_F32 if f32 else 0)
-                 | (_lib.FLAG_F32H if half else 0) | (_lib.FLAG_F64H if half64 else 0)
-                 | (_lib.FLAG_ADJ32 if adj32 else 0))
+                 | (_lib.FLAG_F32H if half else 0) | (_lib.FLAG_F64H if half64 else 0))
         _lib.check(_lib.lib().tpa_eval(self._h, _lib.dptr(c), mode, flags, _lib.dptr(out),
                                        _lib.dptr(f)))
         G = None

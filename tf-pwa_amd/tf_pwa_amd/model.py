@@ -66,7 +66,7 @@ class Model:
 
     def __init__(self, amp, w_bkg=1.0, resolution_size=1, extended=False, precision="fp64",
                  gram_precision="fp64", data_storage="fp64", evaluation="stream",
-                 mc_storage="fp64", adjoint_precision="fp64", **kwargs):
+                 mc_storage="fp64", **kwargs):
         """precision: "fp64" evaluates every MC event per call; "mixed" replaces the MC pass
         by the partial-wave Gram matrix built once on MFMA (valid while the partial waves,
         i.e. lineshape parameters and MC weights, are unchanged; rebuilt otherwise).
@@ -84,9 +84,6 @@ class Model:
         event), for the data passes of both precisions.
         mc_storage: the MC cache the FP64 path's MC passes read, "fp64" or "fp64h" (the halved
         FP64 copy); fixed lineshapes, else FP64.
-        adjoint_precision: with data_storage "fp32h", "fp32" multiplies the data pass's
-        adjoint per event in FP32 (TPA_FLAG_ADJ32: the gradient moves by ~5e-7 of its size,
-        the value not at all); "fp64" (the default) keeps every operation after the load FP64.
         evaluation: how a pass gets its partial waves, "stream" (read the cache built once,
         k_eval_pipe / k_eval_pipe32), "fused" (recompute every wave from the angle cache per
         event in one generated kernel, tpa_eval_fused: no partial-wave cache; three-body,
@@ -101,9 +98,6 @@ class Model:
             raise ValueError("data_storage must be 'fp32', 'fp32h', 'fp64' or 'fp64h'")
         if mc_storage not in ("fp64", "fp64h"):
             raise ValueError("mc_storage must be 'fp64' or 'fp64h'")
-        if adjoint_precision not in ("fp64", "fp32"):
-            raise ValueError("adjoint_precision must be 'fp64' or 'fp32'")
-        self.adjoint_precision = adjoint_precision
         self.mc_storage = mc_storage
         self.recompute_chunk = int(kwargs.pop("recompute_chunk", 0))
         self.Amp = amp
@@ -194,8 +188,7 @@ class Model:
                 g = (g,)
             else:
                 v, g, _ = s.eval(coef, mode, grad=grad, f32=self._data_f32(mode),
-                                 half=self._data_half(mode), half64=self._half64(mode),
-                                 adj32=self._data_adj32(mode))
+                                 half=self._data_half(mode), half64=self._half64(mode))
                 if grad and not shape:
                     g = (g,)
             value += v
@@ -213,11 +206,6 @@ class Model:
         """Whether a pass of this mode reads the helicity-halved FP32 data caches."""
         return (mode == _lib.MODE_DATA and self.precision == "mixed"
                 and self.data_storage == "fp32h" and not self.Amp.shape_mode)
-
-    def _data_adj32(self, mode):
-        """Whether a pass of this mode multiplies its adjoint in FP32 (halved FP32 data
-        caches with adjoint_precision "fp32")."""
-        return self._data_half(mode) and getattr(self, "adjoint_precision", "fp64") == "fp32"
 
     def _half64(self, mode):
         """Whether a pass of this mode reads the helicity-halved FP64 caches (data passes with
@@ -443,8 +431,7 @@ class _NativeSession:
                 s._keep32 = True
         elif model._data_half(_lib.MODE_DATA):
             # (the session keeps exactly the halved copy it reads: the other one is freed)
-            _lib.check(_lib.lib().tpa_fcn_set_data_half(
-                h, 3 if model._data_adj32(_lib.MODE_DATA) else 1))
+            _lib.check(_lib.lib().tpa_fcn_set_data_half(h, 1))
             for s in data_sets:
                 s._keep32h, s._keep64h = True, False
         elif model._half64(_lib.MODE_DATA):
@@ -567,7 +554,6 @@ class FCN:
             st.set_weight(w)
         key = (self.model.precision, self.model.gram_precision, self.model.data_storage,
                getattr(self.model, "mc_storage", "fp64"),
-               getattr(self.model, "adjoint_precision", "fp64"),
                self.vm.struct_version, tuple([st._weight_version for st, _ in pairs]))
         if self._native_key != key:
             if self._native is not None:
