@@ -563,8 +563,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
   __syncthreads();
   const __amdgpu_buffer_rsrc_t hr = half_rsrc(a.h32, h_bytes);
   const __amdgpu_buffer_rsrc_t rr = half_rsrc(a.rho, rho_bytes);
-  const __amdgpu_buffer_rsrc_t wr = half_rsrc(a.w, (uint32_t)(a.N * 8));
   const uint32_t N = (uint32_t)a.N;
+  const __amdgpu_buffer_rsrc_t wr = half_rsrc(a.w, N * 8u);
   // per slot the lane's row of the copy (a pad slot reads past its end: zeros), per run the
   // lane's row of chain phases
   uint32_t vh[PML];
@@ -592,20 +592,27 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
   for (int j = 0; j < PML; ++j) gr[j] = gi[j] = 0.0;
   typedef unsigned int u4v __attribute__((ext_vector_type(4)));
   typedef unsigned int u2v __attribute__((ext_vector_type(2)));
+  // a group that starts past the last event (the loads issued ahead of the loop's end) reads
+  // at offsets past every buffer: zeros, and no memory traffic
+  const uint32_t w_bytes = N * 8u;
   auto load = [&](HV (&hv)[PML], RV (&rv)[2], double& wv, uint32_t gb)
                   __attribute__((always_inline)) {
+    const bool past = gb >= N;
+    const int so_h = past ? (int)h_bytes : (int)(gb * 16u);
+    const int so_r = past ? (int)rho_bytes : (int)(gb * 8u);
+    const int so_w = past ? (int)w_bytes : (int)(gb * 8u);
 #pragma unroll
     for (int j = 0; j < PML; ++j) {
-      const u4v x = __builtin_amdgcn_raw_buffer_load_b128(hr, vh[j], (int)(gb * 16u), 2);
+      const u4v x = __builtin_amdgcn_raw_buffer_load_b128(hr, vh[j], so_h, 2);
       hv[j] = make_float4(__uint_as_float(x.x), __uint_as_float(x.y), __uint_as_float(x.z),
                           __uint_as_float(x.w));
     }
 #pragma unroll
     for (int q = 0; q < 2; ++q) {
-      const u2v r = __builtin_amdgcn_raw_buffer_load_b64(rr, vr[q], (int)(gb * 8u), 0);
+      const u2v r = __builtin_amdgcn_raw_buffer_load_b64(rr, vr[q], so_r, 0);
       rv[q] = make_float2(__uint_as_float(r.x), __uint_as_float(r.y));
     }
-    const u2v x = __builtin_amdgcn_raw_buffer_load_b64(wr, vw, (int)(gb * 8u), 0);
+    const u2v x = __builtin_amdgcn_raw_buffer_load_b64(wr, vw, so_w, 0);
     wv = __hiloint2double((int)x.y, (int)x.x);
   };
   auto group = [&](const HV (&hv)[PML], const RV (&rv)[2], double wv, uint32_t gb)
