@@ -153,7 +153,10 @@ int tpa_events_half_residual(tpa_events_t ev, double* out);
  * FP32 complex, and info: info[0] = nh, info[1+q] the stored slot q, info[5+q] its mirror,
  * info[9+q] its sign s_o (q < nh), info[13+k] = 1 where wave k is stored times i (phi_k = -i),
  * info[13+P+k] the chain of wave k (info_len >= 13 + 2P). So h_k,o = phi_k g_k,o and
- * h_k,-o = phi_k s_o rho_c conj(g_k,o). No reference counterpart (a diagnostic). */
+ * h_k,-o = phi_k s_o rho_c conj(g_k,o). With info_len >= 46 + 2P also the lane plan of the
+ * chain-grouped FP32 pass: info[13+2P] its waves per lane (0: no plan fits, the pass is
+ * k_eval_half), info[14+2P+i] the wave of lane slot i (-1: pad). No reference counterpart (a
+ * diagnostic). */
 int tpa_events_get_half(tpa_events_t ev, float* g, int64_t g_len, float* rho, int64_t rho_len,
                         int* info, int info_len);
 

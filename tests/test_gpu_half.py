@@ -222,3 +222,48 @@ def test_half64_storage_fp64_and_mixed_paths(which, request, gpu_available):
     assert abs(n2 - o_nll) < 1e-6
     assert np.linalg.norm(g2 - o_g) <= 1e-7 * np.linalg.norm(o_g)
 
+
+
+@pytest.mark.parametrize("n", [257, 70001])
+@pytest.mark.parametrize("cfg,planned", [("lc_pkpi_7res.yml", 8), ("lc_pkpi_9res.yml", 0)])
+def test_half_pass_larger_models(cfg, planned, n, gpu_available):
+    """The halved FP32 pass at 26 partial waves in seven chains (the chain-grouped pass with
+    eight waves per lane) and at 32 waves in nine chains (more chains than the eight lane runs:
+    no plan fits, the per-wave pass), against the oracle's C restatement on the cache rebuilt
+    from the same copy (value 1e-12 relative, G 1e-11 of its largest entry, f 1e-12), both
+    modes. The lane plan covers every wave once."""
+    import os
+
+    from oracle import build_cpu
+    from tf_pwa_amd import ConfigLoader, _lib
+    from tf_pwa_amd.phasespace import generate_phsp
+
+    L = build_cpu.load()
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "configs", cfg)
+    conf = ConfigLoader(path, seed=3)
+    order = conf.dat_order
+    masses = [conf.particles[p].mass for p in order]
+    top = conf.particles[conf.top_name].mass
+    p4 = dict(zip(order, generate_phsp(top, masses, n, np.random.default_rng(n))))
+    ev = conf.data_from_p4(p4)
+    w = np.random.default_rng(n + 5).uniform(-0.5, 1.5, n)
+    ev.set_weight(w)
+    ev.keep_half(True)
+    assert 0.0 <= ev.half_residual() <= 1e-12
+    amp = conf.get_amplitude()
+    pml, plan = ev.half_plan()
+    assert pml == planned
+    if planned:
+        waves = [k for k in plan[:4 * pml] if k >= 0]
+        assert sorted(waves) == list(range(amp.P))
+    c = amp.coefficients()
+    h = rebuild_from_half(ev)
+    for mode in (_lib.MODE_DATA, _lib.MODE_MC):
+        v, G, f = ev.eval(c, mode, grad=True, fout=True, half=True)
+        out = _cpu_pass(L, h, w, c, 0 if mode == _lib.MODE_DATA else 1)
+        scale = max(1.0, np.abs(w).sum())
+        assert abs(out[0] - v) <= 1e-12 * scale * max(1.0, abs(v) / scale)
+        np.testing.assert_allclose(out[2:].view(np.complex128), G, rtol=0,
+                                   atol=1e-11 * max(1e-300, np.abs(G).max()))
+        A = np.einsum("k,kel->el", c, h)
+        np.testing.assert_allclose(f, (np.abs(A) ** 2).sum(1), rtol=1e-12)

@@ -1585,6 +1585,12 @@ int tpa_events_get_half(tpa_events_t ev, float* g, int64_t g_len, float* rho, in
     info[13 + k] = ev->half_rot[k];
     info[13 + P + k] = ev->half_kch[k];
   }
+  // the chain-grouped pass's lane plan, when the caller's buffer takes it
+  if (info_len >= 13 + 2 * P + 1 + TPA_EVAL_CIN) {
+    info[13 + 2 * P] = ev->half_plan_pml;
+    for (int i = 0; i < TPA_EVAL_CIN; ++i)
+      info[14 + 2 * P + i] = ev->half_plan_pml > 0 ? ev->half_plan[i] : -1;
+  }
   return TPA_OK;
 }
 

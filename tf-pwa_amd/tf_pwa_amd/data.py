@@ -206,7 +206,7 @@ class EventSet:
         import ctypes
         P = self.amp.P
         nc = self.amp.n_chain if hasattr(self.amp, "n_chain") else None
-        info = np.zeros(13 + 2 * P, dtype=np.int32)
+        info = np.zeros(13 + 2 * P + 1 + 32, dtype=np.int32)
         n_chain_max = 64
         g = np.zeros(2 * P * self.n * 4, dtype=np.float32)
         rho = np.zeros(2 * n_chain_max * self.n, dtype=np.float32)
@@ -219,8 +219,16 @@ class EventSet:
         n_chain = int(kch.max()) + 1 if nc is None else nc
         g = g[:2 * P * self.n * nh].view(np.complex64).reshape(P, self.n, nh)
         rho = rho[:2 * n_chain * self.n].view(np.complex64).reshape(n_chain, self.n)
+        self._half_plan = (int(info[13 + 2 * P]), info[14 + 2 * P:14 + 2 * P + 32].copy())
         return (g, rho, info[1:1 + nh].copy(), info[5:5 + nh].copy(), info[9:9 + nh].copy(),
                 info[13:13 + P].copy(), kch)
+
+    def half_plan(self):
+        """The lane plan of the chain-grouped FP32 pass over the halved copy: (waves per lane,
+        the wave of each lane slot, -1 for pad), or (0, ...) when no plan fits and the pass is
+        the per-wave one. Read with the copy (half_copy)."""
+        self.half_copy()
+        return self._half_plan
 
     def half_residual(self):
         """Largest relative residual of the helicity-flip relation measured by the last
