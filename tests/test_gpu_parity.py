@@ -738,3 +738,44 @@ def test_native_fit_matches_scipy_fit(case1, gpu_available):
     assert rn.success
     assert abs(rn.min_nll - rs.min_nll) < 1e-6 * max(1.0, abs(rs.min_nll)), (rn.min_nll, rs.min_nll)
     assert rn.nfev > 0
+
+
+@pytest.mark.parametrize("n", [1, 257, 70001])
+def test_fp32_cache_pass_four_body_matches_oracle_on_rounded_waves(n, gpu_available):
+    """The four-body model's pass over the full FP32 copy (two helicity slots, 25 waves: the
+    deferred-value pass k_eval_pipe32d) in FP64 from the FP32-rounded waves: the oracle's C
+    restatement on the same rounded waves gives the same sums to FP64 rounding (value 1e-12
+    relative, G 1e-11 of its largest entry, f 1e-12), both modes, ragged counts."""
+    import ctypes
+
+    from conftest import config_path
+    from oracle import build_cpu
+    from tf_pwa_amd import ConfigLoader, EventSet, _lib
+
+    L = build_cpu.load()
+    conf = ConfigLoader(config_path("jpsi_kkpipi_12ch.yml"), seed=9)
+    amp = conf.get_amplitude()
+    ev = EventSet.generate(amp, n, seed=77)
+    w = np.random.default_rng(n + 3).uniform(-0.5, 1.5, n)
+    ev.set_weight(w)
+    ev.keep_f32(True)
+    dp = ctypes.POINTER(ctypes.c_double)
+    c = amp.coefficients()
+    h = np.ascontiguousarray(ev.partial_waves().astype(np.complex64).astype(np.complex128))
+    P, N, LG = h.shape
+    assert LG == 2 and 16 < P <= 32
+    cc = np.ascontiguousarray(c).view(np.float64)
+    for mode in (_lib.MODE_DATA, _lib.MODE_MC):
+        v, G, f = ev.eval(c, mode, grad=True, fout=True, f32=True)
+        v2, _, _ = ev.eval(c, mode, grad=False, f32=True)
+        assert v2 == v
+        out = np.zeros(2 + 2 * P)
+        assert L.cpu_eval(h.view(np.float64).ctypes.data_as(dp), w.ctypes.data_as(dp), N, P, LG,
+                          cc.ctypes.data_as(dp), 0 if mode == _lib.MODE_DATA else 1,
+                          out.ctypes.data_as(dp)) == 0
+        scale = max(1.0, np.abs(w).sum())
+        assert abs(out[0] - v) <= 1e-12 * scale * max(1.0, abs(v) / scale)
+        np.testing.assert_allclose(out[2:].view(np.complex128), G, rtol=0,
+                                   atol=1e-11 * max(1e-300, np.abs(G).max()))
+        A = np.einsum("k,kel->el", c, h)
+        np.testing.assert_allclose(f, (np.abs(A) ** 2).sum(1), rtol=1e-12)

@@ -332,6 +332,56 @@ static void launch_pipe32(const EvalArgs& a, int grid, hipStream_t s) {
   }
 }
 
+// The deferred-value pass over the full FP32 copy (k_eval_pipe32d): two helicity slots, four
+// k-groups (17 to 32 waves: the four-body models), every buffer offset below 2^31 bytes; its
+// grid from the kernel's occupancy. Elsewhere k_eval_pipe32.
+static bool pipe32d_fits(const EvalArgs& a, int lg, int kg) {
+  if (lg != 2 || kg != 4 || a.P <= 16 || a.P > 32) return false;
+  const uint64_t past = 2ull * 64 * kEvalGridMax;
+  return ((uint64_t)a.P * (uint64_t)a.N + past) * 16 < (1ull << 31);
+}
+
+template <int PML>
+static int pipe32d_grid_t(int64_t n) {
+  static const int gmax = [] {
+    const char* e = std::getenv("TFPWA_EVAL_GRID");
+    if (e) {
+      const int v = std::atoi(e);
+      if (v >= 64 && v <= kEvalGridMax) return v;
+    }
+    int dev = 0, cus = 0, per_cu = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(
+            &per_cu, reinterpret_cast<const void*>(&k_eval_pipe32d<4, PML>), 256, 0) !=
+            hipSuccess ||
+        cus < 1 || per_cu < 1)
+      return kEvalGridDefault;
+    return std::min(kEvalGridMax, std::max(64, per_cu * cus));
+  }();
+  int64_t g = (n + 63) / 64;
+  if (g < 1) g = 1;
+  return (int)std::min<int64_t>(g, gmax);
+}
+
+static int pipe32d_pml(int P) { return P <= 24 ? 6 : P <= 28 ? 7 : 8; }
+
+static int pipe32d_grid(int64_t n, int P) {
+  const int pml = pipe32d_pml(P);
+  return pml == 6 ? pipe32d_grid_t<6>(n) : pml == 7 ? pipe32d_grid_t<7>(n) : pipe32d_grid_t<8>(n);
+}
+
+static void launch_pipe32d(const EvalArgs& a, int grid, hipStream_t s) {
+  const uint32_t hb = (uint32_t)((uint64_t)a.P * a.N * 16);
+  const int pml = pipe32d_pml(a.P);
+  if (pml == 6)
+    hipLaunchKernelGGL((k_eval_pipe32d<4, 6>), dim3(grid), dim3(256), 0, s, a, hb);
+  else if (pml == 7)
+    hipLaunchKernelGGL((k_eval_pipe32d<4, 7>), dim3(grid), dim3(256), 0, s, a, hb);
+  else
+    hipLaunchKernelGGL((k_eval_pipe32d<4, 8>), dim3(grid), dim3(256), 0, s, a, hb);
+}
+
 // Lanes per event of the FP32-cache pass: two helicity components per lane when LG >= 2.
 static int f32_lanes_per_event(int lg, int kg) { return (lg >= 2 ? lg / 2 : 1) * kg; }
 
@@ -1692,7 +1742,9 @@ static int eval_impl(tpa_events_s* ev, const double* coef, int mode, int flags, 
   const int half_chains = ev->m->kind == 4 ? ev->m->h4.n_chain : M.n_chain;  // rows of rho
   const bool half_chain =
       half32 && ev->half_plan_pml > 0 && half_chain_fits(a, ev->half_nh, kgh, half_chains);
+  const bool pipe32d = f32 && !shape && pipe32d_fits(a, M.lam_pad, kg);
   int grid = half_chain ? half_chain_grid(ev->n, ev->half_plan_pml)
+             : pipe32d  ? pipe32d_grid(ev->n, M.P)
              : half     ? half_grid(ev->n, kgh)
                         : eval_grid(ev->n, f32 ? f32_lanes_per_event(M.lam_pad, kg) : M.lam_pad * kg);
   bool timing = false;
@@ -1728,6 +1780,9 @@ static int eval_impl(tpa_events_s* ev, const double* coef, int mode, int flags, 
       std::memcpy(a.plan, ev->half_plan, sizeof(a.plan));
       std::memcpy(a.pch, ev->half_pch, sizeof(a.pch));
       launch_half_chain(a, ev->half_plan_pml, half_chains, grid, s);
+      rc = TPA_OK;
+    } else if (pipe32d) {
+      launch_pipe32d(a, grid, s);
       rc = TPA_OK;
     } else {
       rc = half64 ? dispatch_half64(ev->half_nh, a, grid, s)

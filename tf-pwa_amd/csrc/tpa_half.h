@@ -730,3 +730,223 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
       out[(int64_t)j * gridDim.x] = ((sh[0][j] + sh[1][j]) + sh[2][j]) + sh[3][j];
   }
 }
+
+// One event group of k_eval_pipe32d (the full FP32 copy, two helicity components per lane,
+// KG lanes per event): A_lambda of both components and f in FP64 as eval_group2, the mode's
+// derivative u, and the adjoint on the FP64 waves widened for the forward sums. Returns f:
+// the mode's value is summed by the caller, once per KG groups.
+template <int KG, int PML>
+TPA_DEV double eval_pipe32d_group(int mode, bool grad, double* __restrict__ f_out,
+                                  const float4 (&hs)[PML], double w, int64_t e, bool valid,
+                                  bool leader, const double (&cr)[PML], const double (&ci)[PML],
+                                  double (&gr)[PML], double (&gi)[PML]) {
+  const double eps = 1.0e-6;
+  double ar0 = 0.0, ai0 = 0.0, ar1 = 0.0, ai1 = 0.0;
+#pragma unroll
+  for (int j = 0; j < PML; ++j) {
+    const double x0 = hs[j].x, y0 = hs[j].y, x1 = hs[j].z, y1 = hs[j].w;
+    ar0 = fma(cr[j], x0, ar0);
+    ar0 = fma(-ci[j], y0, ar0);
+    ai0 = fma(cr[j], y0, ai0);
+    ai0 = fma(ci[j], x0, ai0);
+    ar1 = fma(cr[j], x1, ar1);
+    ar1 = fma(-ci[j], y1, ar1);
+    ai1 = fma(cr[j], y1, ai1);
+    ai1 = fma(ci[j], x1, ai1);
+  }
+  ar0 = butterfly<1, KG>(ar0);
+  ai0 = butterfly<1, KG>(ai0);
+  ar1 = butterfly<1, KG>(ar1);
+  ai1 = butterfly<1, KG>(ai1);
+  const double f = (ar0 * ar0 + ai0 * ai0) + (ar1 * ar1 + ai1 * ai1);
+  if (f_out != nullptr && valid && leader) f_out[e] = f;
+  if (grad) {
+    const double wv = valid ? w : 0.0;
+    double u;
+    if (mode == 0) {
+      // clip_log'(f) (tf_pwa/model/model.py:283-292)
+      if (f > eps) {
+        u = wv * tpa_recip_pos(f, hw_rcp);
+      } else {
+        const double t = (f - eps) / eps;
+        u = wv * (1.0 / eps - t / eps);
+      }
+    } else {
+      u = wv;
+    }
+    const double tr0 = u * ar0, ti0 = -(u * ai0), tr1 = u * ar1, ti1 = -(u * ai1);
+#pragma unroll
+    for (int j = 0; j < PML; ++j) {
+      const double x0 = hs[j].x, y0 = hs[j].y, x1 = hs[j].z, y1 = hs[j].w;
+      gr[j] = fma(tr0, x0, gr[j]);
+      gr[j] = fma(-ti0, y0, gr[j]);
+      gr[j] = fma(tr1, x1, gr[j]);
+      gr[j] = fma(-ti1, y1, gr[j]);
+      gi[j] = fma(tr0, y0, gi[j]);
+      gi[j] = fma(ti0, x0, gi[j]);
+      gi[j] = fma(tr1, y1, gi[j]);
+      gi[j] = fma(ti1, x1, gi[j]);
+    }
+  }
+  return f;
+}
+
+// The evaluation pass over the full FP32 copy (a.h32, two helicity slots: the four-body
+// models' fastest data pass) with the scheme of k_eval_half_chain: KG = 4 lanes per event,
+// wave k = kg + KG j in slot j < PML; the mode's value and its compensated sum once per four
+// event groups (lane kg takes group kg); buffer loads with a fixed lane offset and scalar
+// group and wave offsets, zeros at and past the end of the copy and the weights (pad waves,
+// groups past the last event); coefficients from LDS at every group; the adjoint on the
+// forward's widened waves. The same FP64 sums as k_eval_pipe32<2, KG, PML>. The host checks
+// that every buffer offset stays below 2^31 bytes.
+template <int KG, int PML>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_eval_pipe32d(
+    EvalArgs a, uint32_t h_bytes) {
+  typedef float4 HV;
+  static_assert(KG == 4, "the value is shared out over four lanes per event");
+  constexpr int EPW = 64 / KG;
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  const int kg = lane & (KG - 1);
+  const int el = lane / KG;
+  const bool leader = kg == 0;
+  __shared__ double2 csh[KG * PML];
+  __shared__ double sh[4][TPA_NACC_MAX];
+  if (threadIdx.x < KG * PML)
+    csh[threadIdx.x] = (int)threadIdx.x < a.P ? eval_coef(a, threadIdx.x) : make_double2(0.0, 0.0);
+  __syncthreads();
+  const uint32_t N = (uint32_t)a.N;
+  const __amdgpu_buffer_rsrc_t hr = half_rsrc(a.h32, h_bytes);
+  const __amdgpu_buffer_rsrc_t wr = half_rsrc(a.w, N * 8u);
+  // row k of the copy: [k][event][2 slots] float2, 16 bytes per event
+  const uint32_t vh = ((uint32_t)kg * N + (uint32_t)el) * 16u;
+  const uint32_t vw = (uint32_t)el * 8u;
+  const int mode = a.mode;
+  const bool grad = a.grad != 0;
+  double* const f_out = a.f_out;
+  const int64_t n_ev = a.N;
+  const uint32_t w_bytes = N * 8u;
+  double vs = 0.0, vc = 0.0;
+  double gr[PML], gi[PML];
+#pragma unroll
+  for (int j = 0; j < PML; ++j) gr[j] = gi[j] = 0.0;
+  typedef unsigned int u4v __attribute__((ext_vector_type(4)));
+  typedef unsigned int u2v __attribute__((ext_vector_type(2)));
+  auto load = [&](HV (&hv)[PML], double& wv, uint32_t gb) __attribute__((always_inline)) {
+    const bool past = gb >= N;
+#pragma unroll
+    for (int j = 0; j < PML; ++j) {
+      const int so = past ? (int)h_bytes : (int)(((uint32_t)(KG * j) * N + gb) * 16u);
+      const u4v x = __builtin_amdgcn_raw_buffer_load_b128(hr, vh, so, 2);
+      hv[j] = make_float4(__uint_as_float(x.x), __uint_as_float(x.y), __uint_as_float(x.z),
+                          __uint_as_float(x.w));
+    }
+    const u2v x = __builtin_amdgcn_raw_buffer_load_b64(wr, vw, past ? (int)w_bytes : (int)(gb * 8u), 0);
+    wv = __hiloint2double((int)x.y, (int)x.x);
+  };
+  auto group = [&](const HV (&hv)[PML], double wv, uint32_t gb) __attribute__((always_inline)) {
+    int cb = kg;
+    asm volatile("" : "+v"(cb));
+    double cr[PML], ci[PML];
+#pragma unroll
+    for (int j = 0; j < PML; ++j) {
+      const double2 c = csh[cb + KG * j];
+      cr[j] = c.x;
+      ci[j] = c.y;
+    }
+    const int64_t e = (int64_t)gb + el;
+    return eval_pipe32d_group<KG, PML>(mode, grad, f_out, hv, wv, e, e < n_ev, leader, cr, ci,
+                                       gr, gi);
+  };
+  auto values = [&](double f0, double w0_, double f1, double w1_, double f2, double w2_, double f3,
+                    double w3_, int nq) __attribute__((always_inline)) {
+    const bool k1 = kg & 1, k2 = kg & 2;
+    const double f = k2 ? (k1 ? f3 : f2) : (k1 ? f1 : f0);
+    const double w = k2 ? (k1 ? w3_ : w2_) : (k1 ? w1_ : w0_);
+    double y;
+    if (mode == 0) {
+      // clip_log(f) (tf_pwa/model/model.py:283-292) weighted by w
+      const double eps = 1.0e-6;
+      double cl;
+      if (f > eps) {
+        cl = tpa_log_pos(f, hw_rcp);
+      } else {
+        const double t = (f - eps) / eps;
+        cl = (log(eps) + t) - (t * t) / 2.0;
+      }
+      y = w * cl;
+    } else {
+      y = w * f;
+    }
+    neumaier_add(vs, vc, kg < nq ? y : 0.0);
+  };
+  const uint32_t stride = gridDim.x * 4u * EPW;
+  HV h0[PML], h1[PML];
+  double w0, w1;
+  const uint32_t wave_u = __builtin_amdgcn_readfirstlane(wave);
+  const uint32_t first = (blockIdx.x * 4u + wave_u) * EPW;
+  double fp0 = 0.0, wp0 = 0.0, fp1 = 0.0, wp1 = 0.0;
+  int cyc = 0;  // uniform: 1 while a first pair of groups waits for its value
+  load(h0, w0, first);
+  for (uint32_t b = first; b < N; b += 2 * stride) {
+    load(h1, w1, b + stride);
+    __builtin_amdgcn_sched_barrier(0);
+    const double fa = group(h0, w0, b);
+    const double wa = w0;
+    if (b + stride >= N) {
+      if (cyc)
+        values(fp0, wp0, fp1, wp1, fa, wa, 0.0, 0.0, 3);
+      else
+        values(fa, wa, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 1);
+      cyc = 0;
+      break;
+    }
+    load(h0, w0, b + 2 * stride);
+    __builtin_amdgcn_sched_barrier(0);
+    const double fb = group(h1, w1, b + stride);
+    const double wb = w1;
+    if (cyc) {
+      values(fp0, wp0, fp1, wp1, fa, wa, fb, wb, 4);
+    } else {
+      fp0 = fa;
+      wp0 = wa;
+      fp1 = fb;
+      wp1 = wb;
+    }
+    cyc ^= 1;
+  }
+  if (cyc) values(fp0, wp0, fp1, wp1, 0.0, 0.0, 0.0, 0.0, 2);
+#pragma unroll
+  for (int m = 32; m >= 1; m >>= 1) {
+    const double s2 = __shfl_xor(vs, m, 64);
+    const double c2 = __shfl_xor(vc, m, 64);
+    neumaier_merge(vs, vc, s2, c2);
+  }
+  if (grad) {
+#pragma unroll
+    for (int j = 0; j < PML; ++j) {
+      const double xr = butterfly<KG, 64>(gr[j]), xi = butterfly<KG, 64>(gi[j]);
+      const int k = kg + KG * j;
+      if (el == 0 && k < a.P) {
+        sh[wave][2 + 2 * k] = xr;
+        sh[wave][3 + 2 * k] = xi;
+      }
+    }
+  }
+  if (lane == 0) {
+    sh[wave][0] = vs;
+    sh[wave][1] = vc;
+  }
+  __syncthreads();
+  double* out = a.partials + blockIdx.x;
+  if (threadIdx.x == 0) {
+    double s = sh[0][0], c = sh[0][1];
+    for (int wv = 1; wv < 4; ++wv) neumaier_merge(s, c, sh[wv][0], sh[wv][1]);
+    out[0] = s;
+    out[gridDim.x] = c;
+  }
+  if (grad) {
+    for (int j = 2 + threadIdx.x; j < a.nacc; j += blockDim.x)
+      out[(int64_t)j * gridDim.x] = ((sh[0][j] + sh[1][j]) + sh[2][j]) + sh[3][j];
+  }
+}
