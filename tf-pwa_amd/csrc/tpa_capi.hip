@@ -1755,7 +1755,7 @@ static int eval_impl(tpa_events_s* ev, const double* coef, int mode, int flags, 
   if (shape) {
     if (!ev->d_bw)
       HIPCHK(hipMalloc(&ev->d_bw, sizeof(double2) * 3 * (size_t)M.n_chain * ev->n));
-    const int64_t lgrid = (ev->n + 255) / 256;
+    const int64_t lgrid = (ev->n * M.n_chain + 255) / 256;  // one thread per chain and event
     hipLaunchKernelGGL(k_lineshape, dim3((unsigned)lgrid), dim3(256), 0, s, ev->m->d, ev->d_ang,
                        ev->n, ev->d_bw);
     HIPCHK(hipGetLastError());
