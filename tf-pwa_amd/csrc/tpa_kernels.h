@@ -1165,6 +1165,21 @@ __global__ __launch_bounds__(256) void k_eval_shape(EvalShapeArgs s) {
   const int64_t last = a.N - 1;
   const int64_t stride = (int64_t)gridDim.x * 4 * EPW;
   const int wave_u = __builtin_amdgcn_readfirstlane(wave);
+  // the waves, lineshapes and weight of the next event group are loaded before the current
+  // one is evaluated (one group ahead; the lineshape derivatives at the start of their group)
+  double2 hn[PML], bn[PML];
+  double wn;
+  auto load_next = [&](int64_t bnext) {
+    const int64_t en = bnext + el;
+    const int64_t ecn = en < a.N ? en : last;
+#pragma unroll
+    for (int j = 0; j < PML; ++j) {
+      hn[j] = load_pw<true>(hb[j] + ecn * LG);
+      bn[j] = bb[j][ecn];
+    }
+    wn = a.w[ecn];
+  };
+  load_next(((int64_t)blockIdx.x * 4 + wave_u) * EPW);
   for (int64_t b = ((int64_t)blockIdx.x * 4 + wave_u) * EPW; b < a.N; b += stride) {
     const int64_t e = b + el;
     const bool valid = e < a.N;
@@ -1172,10 +1187,12 @@ __global__ __launch_bounds__(256) void k_eval_shape(EvalShapeArgs s) {
     double2 hv[PML], bv[PML], dm[PML], dg[PML];
 #pragma unroll
     for (int j = 0; j < PML; ++j) {
-      hv[j] = load_pw<true>(hb[j] + ec * LG);
-      bv[j] = bb[j][ec];
+      hv[j] = hn[j];
+      bv[j] = bn[j];
     }
-    const double w = valid ? a.w[ec] : 0.0;
+    const double w = valid ? wn : 0.0;
+    // the derivatives of this group before the next group's loads: the waits for them then
+    // leave the next group's loads in flight
     if (a.grad) {
 #pragma unroll
       for (int j = 0; j < PML; ++j) {
@@ -1183,6 +1200,7 @@ __global__ __launch_bounds__(256) void k_eval_shape(EvalShapeArgs s) {
         dg[j] = bb[j][2 * nc + ec];
       }
     }
+    load_next(b + stride);
     double ar = 0.0, ai = 0.0;
 #pragma unroll
     for (int j = 0; j < PML; ++j) {
@@ -1196,11 +1214,12 @@ __global__ __launch_bounds__(256) void k_eval_shape(EvalShapeArgs s) {
     const double f = butterfly<1, LG>(ar * ar + ai * ai);
     double y, u;
     if (a.mode == 0) {
-      // clip_log(f) (tf_pwa/model/model.py:283-292) weighted by w
+      // clip_log(f) (tf_pwa/model/model.py:283-292) weighted by w; the log and reciprocal of
+      // tpa_log.h (about half the library's FP64 instructions)
       double cl, dcl;
       if (f > eps) {
-        cl = log(f);
-        dcl = 1.0 / f;
+        dcl = tpa_recip_pos(f, hw_rcp);
+        cl = tpa_log_pos(f, hw_rcp);
       } else {
         const double t = (f - eps) / eps;
         cl = (log(eps) + t) - (t * t) / 2.0;
