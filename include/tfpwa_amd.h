@@ -241,9 +241,11 @@ int tpa_fused_source(const void* desc, int64_t desc_bytes, const double* cg, int
  * reference counterpart. */
 int tpa_fused_compile_seconds(tpa_model_t model, double* s);
 
-/* Kernel timing with HIP events on the engine's stream: when enabled, every likelihood
- * kernel launch is bracketed by events; read returns the summed milliseconds and the
- * number of launches since the last reset. */
+/* Kernel timing with HIP events on the engine's stream. enable(k), k >= 1: one likelihood
+ * pass in k is timed, counting from the call (k = 1: every pass; k <= 0: off). A timed pass
+ * of tpa_eval records its events from its own dispatch (the kernel's execution, as a kernel
+ * trace reports it); the rebuild and fused entries bracket their whole sequence. read
+ * returns the summed milliseconds and the number of timed launches since the last reset. */
 int tpa_timing_enable(int on);
 int tpa_timing_read(double* ms, int64_t* launches);
 int tpa_timing_reset(void);

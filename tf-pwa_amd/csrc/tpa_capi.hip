@@ -2,6 +2,7 @@
 // kernel dispatcher. No C++ exception crosses the boundary; errors are status codes plus a
 // per-thread message.
 #include <hip/hip_runtime.h>
+#include <hip/hip_ext.h>
 
 #include <algorithm>
 #include <atomic>
@@ -24,6 +25,24 @@
 #include "tpa_half.h"
 #include "tpa_rtc_headers.inc"  // kRtcHeaders / kRtcHeaderNames (written by build.py)
 
+// Pass timing: armed by a timed evaluation, the next pass launched through launch_pass records
+// its start and end events from its own dispatch (hipExtLaunchKernelGGL), so the measured span
+// is the kernel's execution, as a kernel trace reports it, without the queue's dispatch latency.
+struct PassTimer {
+  hipEvent_t t0 = nullptr, t1 = nullptr;
+};
+static thread_local PassTimer* tl_pass_timer = nullptr;
+
+template <typename F, typename... Args>
+static void launch_pass(F kernel, dim3 grid, dim3 block, hipStream_t s, Args... args) {
+  if (PassTimer* t = tl_pass_timer) {
+    tl_pass_timer = nullptr;
+    hipExtLaunchKernelGGL(kernel, grid, block, 0, s, t->t0, t->t1, 0, args...);
+  } else {
+    hipLaunchKernelGGL(kernel, grid, block, 0, s, args...);
+  }
+}
+
 namespace {
 
 thread_local std::string g_err;
@@ -44,6 +63,15 @@ std::mutex g_timing_mu;
 bool g_timing = false;
 double g_timing_ms = 0.0;
 int64_t g_timing_n = 0;
+int64_t g_timing_every = 1;  // time one launch in this many (tpa_timing_enable's argument)
+int64_t g_timing_tick = 0;
+
+// Whether the launch about to be made is timed: timing on, and this launch the first of its
+// sampling period. The caller holds g_timing_mu.
+bool timing_take_locked() {
+  if (!g_timing) return false;
+  return (g_timing_tick++ % g_timing_every) == 0;
+}
 
 constexpr int kEvalGridMax = 8192;     // partial-sum slots per event set
 constexpr int kEvalGridDefault = 1024;
@@ -306,13 +334,13 @@ template <int LG, int KG, typename HT>
 static void launch_pipe(const EvalArgs& a, int grid, hipStream_t s) {
   const int need = (a.P + KG - 1) / KG;
   if (need <= 4)
-    hipLaunchKernelGGL((k_eval_pipe<LG, KG, 4, HT>), dim3(grid), dim3(256), 0, s, a);
+    launch_pass((k_eval_pipe<LG, KG, 4, HT>), dim3(grid), dim3(256), s, a);
   else if (need <= 6)
-    hipLaunchKernelGGL((k_eval_pipe<LG, KG, 6, HT>), dim3(grid), dim3(256), 0, s, a);
+    launch_pass((k_eval_pipe<LG, KG, 6, HT>), dim3(grid), dim3(256), s, a);
   else if (need <= 7)
-    hipLaunchKernelGGL((k_eval_pipe<LG, KG, 7, HT>), dim3(grid), dim3(256), 0, s, a);
+    launch_pass((k_eval_pipe<LG, KG, 7, HT>), dim3(grid), dim3(256), s, a);
   else
-    hipLaunchKernelGGL((k_eval_pipe<LG, KG, 8, HT>), dim3(grid), dim3(256), 0, s, a);
+    launch_pass((k_eval_pipe<LG, KG, 8, HT>), dim3(grid), dim3(256), s, a);
 }
 
 template <int LG, int KG>
@@ -320,13 +348,13 @@ static void launch_pipe32(const EvalArgs& a, int grid, hipStream_t s) {
   if constexpr (LG >= 2) {
     const int need = (a.P + KG - 1) / KG;
     if (need <= 4)
-      hipLaunchKernelGGL((k_eval_pipe32<LG, KG, 4>), dim3(grid), dim3(256), 0, s, a);
+      launch_pass((k_eval_pipe32<LG, KG, 4>), dim3(grid), dim3(256), s, a);
     else if (need <= 6)
-      hipLaunchKernelGGL((k_eval_pipe32<LG, KG, 6>), dim3(grid), dim3(256), 0, s, a);
+      launch_pass((k_eval_pipe32<LG, KG, 6>), dim3(grid), dim3(256), s, a);
     else if (need <= 7)
-      hipLaunchKernelGGL((k_eval_pipe32<LG, KG, 7>), dim3(grid), dim3(256), 0, s, a);
+      launch_pass((k_eval_pipe32<LG, KG, 7>), dim3(grid), dim3(256), s, a);
     else
-      hipLaunchKernelGGL((k_eval_pipe32<LG, KG, 8>), dim3(grid), dim3(256), 0, s, a);
+      launch_pass((k_eval_pipe32<LG, KG, 8>), dim3(grid), dim3(256), s, a);
   } else {
     launch_pipe<LG, KG, float2>(a, grid, s);
   }
@@ -375,11 +403,11 @@ static void launch_pipe32d(const EvalArgs& a, int grid, hipStream_t s) {
   const uint32_t hb = (uint32_t)((uint64_t)a.P * a.N * 16);
   const int pml = pipe32d_pml(a.P);
   if (pml == 6)
-    hipLaunchKernelGGL((k_eval_pipe32d<4, 6>), dim3(grid), dim3(256), 0, s, a, hb);
+    launch_pass((k_eval_pipe32d<4, 6>), dim3(grid), dim3(256), s, a, hb);
   else if (pml == 7)
-    hipLaunchKernelGGL((k_eval_pipe32d<4, 7>), dim3(grid), dim3(256), 0, s, a, hb);
+    launch_pass((k_eval_pipe32d<4, 7>), dim3(grid), dim3(256), s, a, hb);
   else
-    hipLaunchKernelGGL((k_eval_pipe32d<4, 8>), dim3(grid), dim3(256), 0, s, a, hb);
+    launch_pass((k_eval_pipe32d<4, 8>), dim3(grid), dim3(256), s, a, hb);
 }
 
 // Lanes per event of the FP32-cache pass: two helicity components per lane when LG >= 2.
@@ -397,13 +425,13 @@ static void launch_eval(const EvalArgs& a, int grid, hipStream_t s) {
       launch_pipe<LG, KG, double2>(a, grid, s);
       break;
     case 1:
-      hipLaunchKernelGGL((k_eval_fold<LG, KG, 8, 1, true>), dim3(grid), dim3(256), 0, s, a);
+      launch_pass((k_eval_fold<LG, KG, 8, 1, true>), dim3(grid), dim3(256), s, a);
       break;
     case 3:  // U = 2 with ordinary (cached) loads, for comparison
-      hipLaunchKernelGGL((k_eval_fold<LG, KG, 8, 2, false>), dim3(grid), dim3(256), 0, s, a);
+      launch_pass((k_eval_fold<LG, KG, 8, 2, false>), dim3(grid), dim3(256), s, a);
       break;
     default:
-      hipLaunchKernelGGL((k_eval_fold<LG, KG, 8, 2, true>), dim3(grid), dim3(256), 0, s, a);
+      launch_pass((k_eval_fold<LG, KG, 8, 2, true>), dim3(grid), dim3(256), s, a);
       break;
   }
 }
@@ -442,11 +470,11 @@ template <int NH, int KG>
 static void launch_half(const EvalArgs& a, int grid, hipStream_t s) {
   const int need = (a.P + KG - 1) / KG;
   if (need <= 4)
-    hipLaunchKernelGGL((k_eval_half<NH, KG, 4, float>), dim3(grid), dim3(256), 0, s, a);
+    launch_pass((k_eval_half<NH, KG, 4, float>), dim3(grid), dim3(256), s, a);
   else if (need <= 6)
-    hipLaunchKernelGGL((k_eval_half<NH, KG, 6, float>), dim3(grid), dim3(256), 0, s, a);
+    launch_pass((k_eval_half<NH, KG, 6, float>), dim3(grid), dim3(256), s, a);
   else
-    hipLaunchKernelGGL((k_eval_half<NH, KG, 8, float>), dim3(grid), dim3(256), 0, s, a);
+    launch_pass((k_eval_half<NH, KG, 8, float>), dim3(grid), dim3(256), s, a);
 }
 
 // The helicity-halved FP64 pass (k_eval_half<.., double>): a lane's register stage holds twice
@@ -457,13 +485,13 @@ static int half64_kgroups(int P) { return P <= 8 ? 2 : P <= 16 ? 4 : 8; }
 template <int NH>
 static void launch_half64(const EvalArgs& a, int grid, hipStream_t s) {
   if (a.P <= 8)
-    hipLaunchKernelGGL((k_eval_half<NH, 2, 4, double>), dim3(grid), dim3(256), 0, s, a);
+    launch_pass((k_eval_half<NH, 2, 4, double>), dim3(grid), dim3(256), s, a);
   else if (a.P <= 16)
-    hipLaunchKernelGGL((k_eval_half<NH, 4, 4, double>), dim3(grid), dim3(256), 0, s, a);
+    launch_pass((k_eval_half<NH, 4, 4, double>), dim3(grid), dim3(256), s, a);
   else if (a.P <= 24)
-    hipLaunchKernelGGL((k_eval_half<NH, 8, 3, double>), dim3(grid), dim3(256), 0, s, a);
+    launch_pass((k_eval_half<NH, 8, 3, double>), dim3(grid), dim3(256), s, a);
   else
-    hipLaunchKernelGGL((k_eval_half<NH, 8, 4, double>), dim3(grid), dim3(256), 0, s, a);
+    launch_pass((k_eval_half<NH, 8, 4, double>), dim3(grid), dim3(256), s, a);
 }
 
 static int dispatch_half64(int nh, const EvalArgs& a, int grid, hipStream_t s) {
@@ -580,9 +608,9 @@ static void launch_half_chain(const EvalArgs& a, int pml, int n_chain, int grid,
   const uint32_t hb = (uint32_t)((uint64_t)a.P * a.N * 16);
   const uint32_t rb = (uint32_t)((uint64_t)n_chain * a.N * 8);
   if (pml == 6)
-    hipLaunchKernelGGL((k_eval_half_chain<4, 6, 4>), dim3(grid), dim3(256), 0, s, a, hb, rb);
+    launch_pass((k_eval_half_chain<4, 6, 4>), dim3(grid), dim3(256), s, a, hb, rb);
   else
-    hipLaunchKernelGGL((k_eval_half_chain<4, 8, 4>), dim3(grid), dim3(256), 0, s, a, hb, rb);
+    launch_pass((k_eval_half_chain<4, 8, 4>), dim3(grid), dim3(256), s, a, hb, rb);
 }
 
 static int dispatch_half(int nh, int kg, const EvalArgs& a, int grid, hipStream_t s) {
@@ -617,11 +645,11 @@ template <int LG, int KG>
 static void launch_shape(const EvalShapeArgs& a, int grid, hipStream_t s) {
   const int need = (a.a.P + KG - 1) / KG;  // PML fitted to P as for the folded pass
   if (need <= 4)
-    hipLaunchKernelGGL((k_eval_shape<LG, KG, 4>), dim3(grid), dim3(256), 0, s, a);
+    launch_pass((k_eval_shape<LG, KG, 4>), dim3(grid), dim3(256), s, a);
   else if (need <= 6)
-    hipLaunchKernelGGL((k_eval_shape<LG, KG, 6>), dim3(grid), dim3(256), 0, s, a);
+    launch_pass((k_eval_shape<LG, KG, 6>), dim3(grid), dim3(256), s, a);
   else
-    hipLaunchKernelGGL((k_eval_shape<LG, KG, 8>), dim3(grid), dim3(256), 0, s, a);
+    launch_pass((k_eval_shape<LG, KG, 8>), dim3(grid), dim3(256), s, a);
 }
 
 template <int LG>
@@ -885,24 +913,24 @@ static void launch_multi(const MultiArgs& a, int grid, hipStream_t s) {
   if constexpr (LG >= 2) {
     if (a.h32 != nullptr) {  // the pool's FP32 copy
       if (need <= 4)
-        hipLaunchKernelGGL((k_eval_multi32<LG, KG, 4>), dim3(grid), dim3(256), 0, s, a);
+        launch_pass((k_eval_multi32<LG, KG, 4>), dim3(grid), dim3(256), s, a);
       else if (need <= 6)
-        hipLaunchKernelGGL((k_eval_multi32<LG, KG, 6>), dim3(grid), dim3(256), 0, s, a);
+        launch_pass((k_eval_multi32<LG, KG, 6>), dim3(grid), dim3(256), s, a);
       else if (need <= 7)
-        hipLaunchKernelGGL((k_eval_multi32<LG, KG, 7>), dim3(grid), dim3(256), 0, s, a);
+        launch_pass((k_eval_multi32<LG, KG, 7>), dim3(grid), dim3(256), s, a);
       else
-        hipLaunchKernelGGL((k_eval_multi32<LG, KG, 8>), dim3(grid), dim3(256), 0, s, a);
+        launch_pass((k_eval_multi32<LG, KG, 8>), dim3(grid), dim3(256), s, a);
       return;
     }
   }
   if (need <= 4)
-    hipLaunchKernelGGL((k_eval_multi<LG, KG, 4>), dim3(grid), dim3(256), 0, s, a);
+    launch_pass((k_eval_multi<LG, KG, 4>), dim3(grid), dim3(256), s, a);
   else if (need <= 6)
-    hipLaunchKernelGGL((k_eval_multi<LG, KG, 6>), dim3(grid), dim3(256), 0, s, a);
+    launch_pass((k_eval_multi<LG, KG, 6>), dim3(grid), dim3(256), s, a);
   else if (need <= 7)
-    hipLaunchKernelGGL((k_eval_multi<LG, KG, 7>), dim3(grid), dim3(256), 0, s, a);
+    launch_pass((k_eval_multi<LG, KG, 7>), dim3(grid), dim3(256), s, a);
   else
-    hipLaunchKernelGGL((k_eval_multi<LG, KG, 8>), dim3(grid), dim3(256), 0, s, a);
+    launch_pass((k_eval_multi<LG, KG, 8>), dim3(grid), dim3(256), s, a);
 }
 
 template <int LG>
@@ -1750,7 +1778,7 @@ static int eval_impl(tpa_events_s* ev, const double* coef, int mode, int flags, 
   bool timing = false;
   if (!async) {
     std::lock_guard<std::mutex> lk(g_timing_mu);
-    timing = g_timing;
+    timing = timing_take_locked();
   }
   if (shape) {
     if (!ev->d_bw)
@@ -1760,7 +1788,8 @@ static int eval_impl(tpa_events_s* ev, const double* coef, int mode, int flags, 
                        ev->n, ev->d_bw);
     HIPCHK(hipGetLastError());
   }
-  if (timing) HIPCHK(hipEventRecord(ev->t0, s));
+  PassTimer pass_timer{ev->t0, ev->t1};
+  if (timing) tl_pass_timer = &pass_timer;
   int rc;
   if (shape) {
     EvalShapeArgs sa;
@@ -1790,9 +1819,11 @@ static int eval_impl(tpa_events_s* ev, const double* coef, int mode, int flags, 
                   : dispatch_eval(M.lam_pad, kg, a, grid, s);
     }
   }
+  // a pass that did not take the timer (none launched) is not counted
+  if (timing && tl_pass_timer != nullptr) timing = false;
+  tl_pass_timer = nullptr;
   if (rc != TPA_OK) return fail(rc, "no evaluation kernel for this helicity/partial-wave count");
   HIPCHK(hipGetLastError());
-  if (timing) HIPCHK(hipEventRecord(ev->t1, s));
   const int nout = a.grad ? a.nacc : 2;
   // the sums land in pinned memory; without f_out the host polls the reduction's flag rather
   // than waiting on the stream (a bounded spin: after 2 s it waits on the stream, which also
@@ -1905,7 +1936,7 @@ int tpa_eval_recompute(tpa_events_t ev, const double* coef, int mode, int flags,
   bool timing;
   {
     std::lock_guard<std::mutex> lk(g_timing_mu);
-    timing = g_timing;
+    timing = timing_take_locked();
   }
   if (timing) HIPCHK(hipEventRecord(ev->t0, s));
   for (int64_t c = 0; c < nch; ++c) {
@@ -2093,7 +2124,7 @@ static int eval_fused_impl(tpa_events_s* ev, const double* coef, int mode, int f
   bool timing;
   {
     std::lock_guard<std::mutex> lk(g_timing_mu);
-    timing = g_timing;
+    timing = timing_take_locked();
   }
   if (timing) HIPCHK(hipEventRecord(ev->t0, s));
   void* params[] = {buf.data()};
@@ -2945,7 +2976,9 @@ int tpa_calib_copy(int64_t bytes, int reps, double* ms) {
 
 int tpa_timing_enable(int on) {
   std::lock_guard<std::mutex> lk(g_timing_mu);
-  g_timing = on != 0;
+  g_timing = on > 0;
+  g_timing_every = on > 0 ? on : 1;
+  g_timing_tick = 0;
   return TPA_OK;
 }
 
