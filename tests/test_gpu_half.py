@@ -267,3 +267,36 @@ def test_half_pass_larger_models(cfg, planned, n, gpu_available):
                                    atol=1e-11 * max(1e-300, np.abs(G).max()))
         A = np.einsum("k,kel->el", c, h)
         np.testing.assert_allclose(f, (np.abs(A) ** 2).sum(1), rtol=1e-12)
+
+
+def test_sampled_pass_timing(case2, gpu_available):
+    """tpa_timing_enable(k) times one tpa_eval pass in k, with events from the chain-grouped
+    pass's own dispatch; timed and untimed passes return the same sums."""
+    from tf_pwa_amd import _lib
+
+    L = _lib.lib()
+    conf = case2["conf"]
+    full = conf.data_from_p4(case2["mc"])
+    n = 20000
+    ev = full.select(np.arange(n) % full.n)
+    ev.set_weight(np.random.default_rng(5).uniform(0.5, 1.5, n))
+    ev.keep_half(True)
+    assert ev.half_plan()[0] > 0  # the chain-grouped pass serves this model
+    c = conf.get_amplitude().coefficients()
+    v0, G0, _ = ev.eval(c, _lib.MODE_DATA, grad=True, half=True)
+    ms, nl = ctypes.c_double(0), ctypes.c_int64(0)
+    try:
+        for every, calls, timed in ((1, 4, 4), (3, 7, 3), (8, 5, 1)):
+            _lib.check(L.tpa_timing_reset())
+            _lib.check(L.tpa_timing_enable(every))
+            for _ in range(calls):
+                v, G, _ = ev.eval(c, _lib.MODE_DATA, grad=True, half=True)
+                assert abs(v - v0) <= 1e-14 * abs(v0)
+                np.testing.assert_allclose(G, G0, rtol=0, atol=1e-14 * np.abs(G0).max())
+            _lib.check(L.tpa_timing_enable(0))
+            _lib.check(L.tpa_timing_read(ctypes.byref(ms), ctypes.byref(nl)))
+            assert nl.value == timed
+            assert 0.0 < ms.value < 1e3 * timed
+    finally:
+        L.tpa_timing_enable(0)
+        L.tpa_timing_reset()
